@@ -1,0 +1,213 @@
+/*
+ * rdk.h -- C ABI of librdk, the MI355X (gfx950) rod-dynamics contact/step
+ * library.  Plain pointers and sizes only; nothing throws across this
+ * boundary.  Every function returns an rdk_status (0 = RDK_OK).
+ *
+ * Citations name the reference interface each entry point replaces, in
+ * yeonsu-jung/rod-dynamics-3d.
+ *
+ * Host arrays are interleaved per body: x[3N], q[4N] as (w,x,y,z), v[3N],
+ * w[3N], f[3N], tau[3N]; scalars r[N] (radius), h[N] (half length of the
+ * cylinder part, the reference's Capsule::h), mass[N] (0 = static body),
+ * restitution[N].  On the device the library keeps its own planar FP64
+ * structure-of-arrays copy; host arrays are read or written only by the
+ * upload/download entry points.
+ */
+#ifndef RDK_H
+#define RDK_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rdk_ctx rdk_ctx;
+
+enum rdk_status {
+  RDK_OK = 0,
+  RDK_ERR_HIP = 1,      /* a HIP runtime call failed; see rdk_last_error */
+  RDK_ERR_ARG = 2,      /* invalid argument */
+  RDK_ERR_CAPACITY = 3, /* a fixed capacity was exceeded; *required holds the size */
+  RDK_ERR_STATE = 4,    /* call out of order (e.g. solve before build) */
+  RDK_ERR_NOCONV = 5    /* PSOR hit max_sweeps before reaching tol (results kept) */
+};
+
+/* One independent box.  Axis k is periodic when periodic[k] != 0 (the
+ * reference's PBC, main.cpp:4508-4517 / integrator.cpp:23-35).  floor_wall /
+ * ceil_wall put plane walls at lo[2] / hi[2] (axis 2 must then be
+ * non-periodic).  mu is the Coulomb coefficient used by the NSC solve for
+ * this box. */
+typedef struct rdk_system_desc {
+  double lo[3], hi[3];
+  int periodic[3];
+  int floor_wall, ceil_wall;
+  double mu;
+} rdk_system_desc;
+
+typedef struct rdk_physics {
+  double gravity[3];
+  double dt;
+  double lin_damp, ang_damp, w_max; /* integrator.cpp:14-16 globals */
+} rdk_physics;
+
+/* Broad-phase tuning.  chunk_len: rods are cut into ceil(2h/chunk_len)
+ * sub-segments for binning (<= 0: one chunk per rod).  cell_scale >= 1
+ * multiplies the minimal cell size. */
+typedef struct rdk_detect_params {
+  double delta;     /* activation margin: contact if dist < r_a + r_b + delta */
+  double chunk_len;
+  double cell_scale;
+} rdk_detect_params;
+
+typedef struct rdk_nsc_params {
+  double beta, cfm, omega; /* NscContactCfg, config.hpp:66-88 */
+  double tol;              /* fixed-point residual tolerance (velocity units) */
+  int max_sweeps;
+  int check_every;         /* residual evaluated every k sweeps */
+  int friction_proj;       /* 0 reference sequential disc clamp, 1 Euclidean disc */
+  int position_stabilization; /* scales beta by 0.01 (nsc_contact.cpp:339-341) */
+  int warm_mode;           /* 0 cold, 1 previous step (reference), 2 learned (MLP) */
+  int stop_rule;           /* 0 reference (max sum|dlambda| < 1e-6), 1 residual <= tol */
+} rdk_nsc_params;
+
+typedef struct rdk_soft_params { /* SoftContactCfg, config.hpp:6-27 */
+  double delta, k_scaler, damping, mu, mu_static, nu, kt, vel_deadband;
+  int enable_friction;
+  int friction_model; /* 0 smooth tanh/Stribeck, 1 Cundall-Strack, 2 Karnopp */
+} rdk_soft_params;
+
+typedef struct rdk_hm_params { /* HertzMindlinCfg, config.hpp:29-64 */
+  double E, nu_poisson, restitution, mu_k, mu_s, v_c, mu_roll;
+  int enable_tangential, enable_rolling;
+  int law; /* 0 reference (elastic force ~ delta^2), 1 Hertz delta^1.5 */
+} rdk_hm_params;
+
+typedef struct rdk_solve_stats {
+  int manifolds, rod_contacts, wall_contacts, colors, sweeps;
+  double residual;       /* final fixed-point residual */
+  double residual_ref;   /* reference rule: max sum|dlambda| of the last sweep */
+} rdk_solve_stats;
+
+typedef struct rdk_step_stats {
+  int64_t steps, contacts_total, manifolds_total, sweeps_total, colors_total;
+  int64_t nonconverged_steps;
+  double residual_max;
+  double ms_detect, ms_build, ms_color, ms_solve, ms_integrate; /* event timers (profiling on) */
+} rdk_step_stats;
+
+/* ---- context --------------------------------------------------------- */
+/* replaces the solver objects' value members (nsc_contact.hpp:159-188,
+ * contact_detector.hpp:77-93) and the file-static device state of
+ * cuda_broadphase.cu:293,658 */
+int rdk_create(int device, rdk_ctx** out);
+int rdk_destroy(rdk_ctx* ctx);
+const char* rdk_last_error(rdk_ctx* ctx);
+int rdk_sync(rdk_ctx* ctx);
+int rdk_version(void);
+
+/* ---- bodies and systems ---------------------------------------------- */
+/* replaces RigidBody::makeCapsule (rigid_body.cpp:11-47): mass, solid-
+ * cylinder inertia about local +Y, static when mass == 0 */
+int rdk_set_bodies(rdk_ctx* ctx, int n, const double* x, const double* q,
+                   const double* v, const double* w, const double* r,
+                   const double* h, const double* mass,
+                   const double* restitution);
+/* body_offsets[nsys+1]: system s owns bodies [off[s], off[s+1]) */
+int rdk_set_systems(rdk_ctx* ctx, int nsys, const int* body_offsets,
+                    const rdk_system_desc* desc);
+int rdk_set_physics(rdk_ctx* ctx, const rdk_physics* p);
+int rdk_upload_state(rdk_ctx* ctx, const double* x, const double* q,
+                     const double* v, const double* w);
+int rdk_upload_forces(rdk_ctx* ctx, const double* f, const double* tau);
+int rdk_download_state(rdk_ctx* ctx, double* x, double* q, double* v,
+                       double* w, double* f, double* tau);
+
+/* ---- detection (ContactDetector::detectContacts,
+ *      contact_detector.cpp:71-211,487-1035; cuda_broadphase.hpp:95-101) --- */
+int rdk_detect(rdk_ctx* ctx, const rdk_detect_params* p, int* n_contacts);
+/* sorted by (a,b), a<b; any pointer may be NULL.  type: 0 P2P, 1 E2P, 2 E2E */
+int rdk_get_contacts(rdk_ctx* ctx, int cap, int* a, int* b, double* pa,
+                     double* pb, double* nrm, double* dist, double* shift,
+                     double* s, double* t, int* type);
+/* test hook: force the pair-buffer capacity (0 = automatic) */
+int rdk_set_pair_capacity(rdk_ctx* ctx, int cap);
+int rdk_detect_info(rdk_ctx* ctx, int* n_chunks, int* n_cells,
+                    int64_t* candidates, int* regrows);
+
+/* ---- forces (SoftContactSolver::computeForces soft_contact.cpp:80-213;
+ *      HertzMindlinSolver::computeForces hertz_mindlin.cpp:183-247) ------
+ * accumulate into the device f, tau of the bodies in the last detection */
+int rdk_forces_soft(rdk_ctx* ctx, const rdk_soft_params* p, double* pe);
+int rdk_forces_hm(rdk_ctx* ctx, const rdk_hm_params* p, double* pe);
+int rdk_history_size(rdk_ctx* ctx, int* n);
+
+/* ---- NSC (NscContactSolver, nsc_contact.hpp:71-157) -------------------- */
+/* detectAndBuildManifolds + plane-wall addWallContact (nsc_contact.cpp:190-321) */
+int rdk_nsc_build(rdk_ctx* ctx, const rdk_nsc_params* p, int* n_manifolds);
+/* rec: 41 doubles per manifold (n,t1,t2,r_a,r_b, I_A^-1(r_a x n|t1|t2),
+ * I_B^-1(r_b x n|t1|t2), invMA, invMB, g_n, g_t1, g_t2, phi, v_n_pre, e);
+ * ab: 2 ints (b = -1 for a wall); lambda: 3 doubles; natural order */
+int rdk_nsc_get_manifolds(rdk_ctx* ctx, int cap, double* rec, int* ab,
+                          double* lambda);
+int rdk_nsc_set_lambda(rdk_ctx* ctx, int m, const double* lambda);
+/* solveVelocities (nsc_contact.cpp:325-655): colours, warm impulses,
+ * coloured PSOR to tol, warm-cache update */
+int rdk_nsc_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* st);
+/* fixed sweeps from the current lambda (no warm application); test hook */
+int rdk_nsc_sweeps(rdk_ctx* ctx, const rdk_nsc_params* p, int nsweeps);
+/* colour of every manifold and the colour-ordered permutation */
+int rdk_nsc_get_coloring(rdk_ctx* ctx, int cap, int* color, int* order,
+                         int* n_colors);
+int rdk_nsc_residual(rdk_ctx* ctx, const rdk_nsc_params* p, double* res);
+
+/* ---- integrators (integrator.cpp:56-159, main.cpp:4397-4483) ----------- */
+enum rdk_stage {
+  RDK_NSC_FREE = 0,  /* v += (f/m + g)dt; w += I^-1(tau - w x Iw)dt */
+  RDK_NSC_POS = 1,   /* x += v dt; q += dt/2 (0,w) q; damping; clamp; wrap; clear f,tau */
+  RDK_VV_HALF1 = 2,  /* integrateHalfPos */
+  RDK_VV_HALF2 = 3,  /* integrateSecondHalf */
+  RDK_CLEAR = 4      /* f = tau = 0 */
+};
+int rdk_integrate(rdk_ctx* ctx, int stage);
+
+/* ---- fused steps (App::physicsStep, main.cpp:4333-4858) ---------------- */
+int rdk_step_nsc(rdk_ctx* ctx, const rdk_detect_params* dp,
+                 const rdk_nsc_params* np, int nsteps, rdk_step_stats* st);
+/* model 0 soft (params = rdk_soft_params*), 1 Hertz-Mindlin (rdk_hm_params*) */
+int rdk_step_vv(rdk_ctx* ctx, const rdk_detect_params* dp, int model,
+                const void* params, int nsteps, rdk_step_stats* st);
+
+/* ---- observables (App::totalKE main.cpp:3357-3368) --------------------- */
+int rdk_observables(rdk_ctx* ctx, double* kinetic_energy,
+                    double* packing_fraction);
+
+/* ---- profiling: per-kernel HIP-event timing on the library stream ------- */
+int rdk_set_profiling(rdk_ctx* ctx, int on);
+/* time only the kernels whose id bit is set (bit k = kernel id k) */
+int rdk_set_profile_mask(rdk_ctx* ctx, uint64_t mask);
+/* kernel ids: see rdk_kernel_name */
+int rdk_kernel_times(rdk_ctx* ctx, int kernel, int64_t* launches,
+                     double* total_ms);
+const char* rdk_kernel_name(int kernel);
+int rdk_num_kernels(void);
+/* algorithmic-byte counters accumulated by the fused steps */
+int rdk_work_counters(rdk_ctx* ctx, int64_t* counters, int n);
+
+/* ---- scene generation (resetScene populate/randomInit,
+ *      main.cpp:1857-2140,2243-2263) ------------------------------------- */
+/* Random sequential addition of non-overlapping capsules in the box, using
+ * all 27 periodic images on periodic axes and a clearance `gap` between
+ * surfaces (and from walls).  Orientations uniform on S^2.  Returns the
+ * number placed in *placed (may be < n if max_attempts is hit). */
+int rdk_gen_rsa(int n, const double* r, const double* h,
+                const rdk_system_desc* box, double gap, uint64_t seed,
+                int max_attempts, double* x, double* q, int* placed);
+/* Thermal velocities: v ~ N(0, kT/m), w ~ N(0, kT/I_perp) on the two axes
+ * perpendicular to the rod, zero spin about the axis. */
+int rdk_gen_thermal(int n, const double* q, const double* r, const double* h,
+                    const double* mass, double kT, uint64_t seed, double* v,
+                    double* w);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,657 @@
+// rdk_api.hip -- the C ABI of librdk (declared in include/rdk.h).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "rdk_internal.hpp"
+
+using namespace rdk;
+
+namespace rdk {
+
+BodyPtrs body_ptrs(rdk_ctx* ctx) {
+  BodyPtrs B;
+  for (int k = 0; k < 3; ++k) {
+    B.x[k] = ctx->x[k].p;
+    B.v[k] = ctx->v[k].p;
+    B.w[k] = ctx->w[k].p;
+    B.f[k] = ctx->f[k].p;
+    B.tau[k] = ctx->tau[k].p;
+  }
+  for (int k = 0; k < 4; ++k) B.q[k] = ctx->q[k].p;
+  B.r = ctx->r.p;
+  B.h = ctx->h.p;
+  B.mass = ctx->mass.p;
+  B.inv_mass = ctx->inv_mass.p;
+  B.i_perp = ctx->i_perp.p;
+  B.i_axial = ctx->i_axial.p;
+  B.i_perp_inv = ctx->i_perp_inv.p;
+  B.i_axial_inv = ctx->i_axial_inv.p;
+  B.rest = ctx->rest.p;
+  B.sys = ctx->sys_of.p;
+  return B;
+}
+
+void prof_begin(rdk_ctx* ctx, int kernel, EventRec& rec) {
+  rec.kernel = kernel;
+  for (int k = 0; k < 2; ++k) {
+    hipEvent_t e;
+    if (!ctx->ev_pool.empty()) {
+      e = ctx->ev_pool.back();
+      ctx->ev_pool.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+      e = nullptr;
+    }
+    (k == 0 ? rec.a : rec.b) = e;
+  }
+  if (rec.a) (void)hipEventRecord(rec.a, ctx->stream);
+}
+
+void prof_end(rdk_ctx* ctx, EventRec& rec) {
+  if (rec.b) (void)hipEventRecord(rec.b, ctx->stream);
+  ctx->ev_pending.push_back(rec);
+}
+
+int prof_flush(rdk_ctx* ctx) {
+  if (ctx->ev_pending.empty()) return RDK_OK;
+  RDK_HIP(hipStreamSynchronize(ctx->stream));
+  for (auto& r : ctx->ev_pending) {
+    float ms = 0.0f;
+    if (r.a && r.b && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) ctx->k_ms[r.kernel] += ms;
+    if (r.a) ctx->ev_pool.push_back(r.a);
+    if (r.b) ctx->ev_pool.push_back(r.b);
+  }
+  ctx->ev_pending.clear();
+  return RDK_OK;
+}
+
+static const char* kKernelNames[K_NUM] = {
+    "chunk_keys", "cell_ranges", "chunk_gather", "broad_narrow", "contact_record", "sort_scan",
+    "manifold", "wall", "csr", "color", "permute", "warm_apply", "psor_sweep", "residual",
+    "nsc_free", "nsc_pos", "vv_half1", "vv_half2", "force", "body_reduce", "history", "observe",
+    "mlp", "misc"};
+
+}  // namespace rdk
+
+#define CTX_CHECK()                  \
+  do {                               \
+    if (!ctx) return RDK_ERR_ARG;    \
+    if (hipSetDevice(ctx->device) != hipSuccess) { \
+      ctx->err = "hipSetDevice failed"; \
+      return RDK_ERR_HIP;            \
+    }                                \
+  } while (0)
+
+extern "C" {
+
+int rdk_version(void) { return 1; }
+
+int rdk_create(int device, rdk_ctx** out) {
+  if (!out) return RDK_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return RDK_ERR_HIP;
+  if (device < 0 || device >= ndev) return RDK_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return RDK_ERR_HIP;
+  rdk_ctx* ctx = new (std::nothrow) rdk_ctx();
+  if (!ctx) return RDK_ERR_HIP;
+  ctx->device = device;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return RDK_ERR_HIP;
+  }
+  *out = ctx;
+  return RDK_OK;
+}
+
+int rdk_destroy(rdk_ctx* ctx) {
+  if (!ctx) return RDK_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  mlp_free(ctx);
+  for (int k = 0; k < 3; ++k) {
+    release(ctx->x[k]); release(ctx->v[k]); release(ctx->w[k]); release(ctx->f[k]); release(ctx->tau[k]);
+    release(ctx->sc[k]); release(ctx->sx[k]);
+    release(ctx->cpa[k]); release(ctx->cpb[k]); release(ctx->cn[k]); release(ctx->csh[k]);
+    release(ctx->lam[k]); release(ctx->lamc[k]); release(ctx->prev_lam[k]);
+    release(ctx->hxi[k]); release(ctx->hxi_new[k]);
+  }
+  for (int k = 0; k < 4; ++k) release(ctx->q[k]);
+  for (int k = 0; k < 12; ++k) release(ctx->cf[k]);
+  release(ctx->r); release(ctx->h); release(ctx->mass); release(ctx->inv_mass); release(ctx->i_perp);
+  release(ctx->i_axial); release(ctx->i_perp_inv); release(ctx->i_axial_inv); release(ctx->rest);
+  release(ctx->sys_of); release(ctx->sys_dev);
+  release(ctx->chunk_rod); release(ctx->chunk_k); release(ctx->rod_nchunk);
+  release(ctx->ckey); release(ctx->ckey_alt); release(ctx->cid); release(ctx->cid_alt);
+  release(ctx->cell_start); release(ctx->cell_end); release(ctx->srad); release(ctx->srod); release(ctx->sk);
+  release(ctx->pkeys); release(ctx->pkeys_alt); release(ctx->counters);
+  release(ctx->ca); release(ctx->cb); release(ctx->ctype); release(ctx->cdist); release(ctx->chs);
+  release(ctx->cs); release(ctx->ct);
+  release(ctx->mf); release(ctx->mfc); release(ctx->ma); release(ctx->mb); release(ctx->mac); release(ctx->mbc);
+  release(ctx->wall_flag); release(ctx->wall_off); release(ctx->color); release(ctx->color_tmp);
+  release(ctx->perm); release(ctx->perm_alt); release(ctx->uncolored); release(ctx->color_key);
+  release(ctx->color_key_alt); release(ctx->csr_key); release(ctx->csr_key_alt); release(ctx->csr_val);
+  release(ctx->csr_val_alt); release(ctx->body_start); release(ctx->prev_keys); release(ctx->resid);
+  release(ctx->hkeys); release(ctx->hkeys_new); release(ctx->hlast); release(ctx->hlast_new);
+  release(ctx->hflag); release(ctx->hscan); release(ctx->pe); release(ctx->temp); release(ctx->red);
+  if (ctx->h_counters) (void)hipHostFree(ctx->h_counters);
+  for (auto& r : ctx->ev_pending) {
+    if (r.a) (void)hipEventDestroy(r.a);
+    if (r.b) (void)hipEventDestroy(r.b);
+  }
+  for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return RDK_OK;
+}
+
+const char* rdk_last_error(rdk_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int rdk_sync(rdk_ctx* ctx) {
+  CTX_CHECK();
+  RDK_HIP(hipStreamSynchronize(ctx->stream));
+  return RDK_OK;
+}
+
+static int upload_planar(rdk_ctx* ctx, DevBuf<double>* dst, int comps, const double* src, int n) {
+  std::vector<double> tmp(n);
+  for (int k = 0; k < comps; ++k) {
+    for (int i = 0; i < n; ++i) tmp[i] = src[(size_t)comps * i + k];
+    RDK_HIP(hipMemcpyAsync(dst[k].p, tmp.data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    RDK_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  return RDK_OK;
+}
+
+static int download_planar(rdk_ctx* ctx, DevBuf<double>* src, int comps, double* dst, int n) {
+  std::vector<double> tmp(n);
+  for (int k = 0; k < comps; ++k) {
+    RDK_HIP(hipMemcpyAsync(tmp.data(), src[k].p, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    RDK_HIP(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < n; ++i) dst[(size_t)comps * i + k] = tmp[i];
+  }
+  return RDK_OK;
+}
+
+static int upload_scalar(rdk_ctx* ctx, DevBuf<double>& dst, const std::vector<double>& src) {
+  RDK_HIP(hipMemcpyAsync(dst.p, src.data(), src.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  RDK_HIP(hipStreamSynchronize(ctx->stream));
+  return RDK_OK;
+}
+
+int rdk_set_bodies(rdk_ctx* ctx, int n, const double* x, const double* q, const double* v, const double* w,
+                   const double* r, const double* h, const double* mass, const double* rest) {
+  CTX_CHECK();
+  if (n <= 0 || !x || !q || !r || !h || !mass) {
+    ctx->err = "rdk_set_bodies: n > 0 and x, q, r, h, mass are required";
+    return RDK_ERR_ARG;
+  }
+  ctx->n = n;
+  for (int k = 0; k < 3; ++k) {
+    RDK_TRY(ensure(ctx, ctx->x[k], n)); RDK_TRY(ensure(ctx, ctx->v[k], n)); RDK_TRY(ensure(ctx, ctx->w[k], n));
+    RDK_TRY(ensure(ctx, ctx->f[k], n)); RDK_TRY(ensure(ctx, ctx->tau[k], n));
+    RDK_HIP(hipMemsetAsync(ctx->f[k].p, 0, n * sizeof(double), ctx->stream));
+    RDK_HIP(hipMemsetAsync(ctx->tau[k].p, 0, n * sizeof(double), ctx->stream));
+    RDK_HIP(hipMemsetAsync(ctx->v[k].p, 0, n * sizeof(double), ctx->stream));
+    RDK_HIP(hipMemsetAsync(ctx->w[k].p, 0, n * sizeof(double), ctx->stream));
+  }
+  for (int k = 0; k < 4; ++k) RDK_TRY(ensure(ctx, ctx->q[k], n));
+  RDK_TRY(ensure(ctx, ctx->r, n)); RDK_TRY(ensure(ctx, ctx->h, n)); RDK_TRY(ensure(ctx, ctx->mass, n));
+  RDK_TRY(ensure(ctx, ctx->inv_mass, n)); RDK_TRY(ensure(ctx, ctx->i_perp, n)); RDK_TRY(ensure(ctx, ctx->i_axial, n));
+  RDK_TRY(ensure(ctx, ctx->i_perp_inv, n)); RDK_TRY(ensure(ctx, ctx->i_axial_inv, n));
+  RDK_TRY(ensure(ctx, ctx->rest, n)); RDK_TRY(ensure(ctx, ctx->sys_of, n));
+  ctx->h_r.assign(r, r + n);
+  ctx->h_h.assign(h, h + n);
+  ctx->h_mass.assign(mass, mass + n);
+  std::vector<double> im(n), ip(n), ia(n), ipi(n), iai(n), rs(n);
+  for (int i = 0; i < n; ++i) {
+    // rigid_body.cpp:17-38: solid cylinder of length 2h about local +Y
+    const double m = mass[i];
+    if (m > 0.0) {
+      const double L = 2.0 * h[i];
+      im[i] = 1.0 / m;
+      ip[i] = m * (3.0 * r[i] * r[i] + L * L) / 12.0;
+      ia[i] = m * (r[i] * r[i]) / 2.0;
+      ipi[i] = ip[i] > 0 ? 1.0 / ip[i] : 0.0;
+      iai[i] = ia[i] > 0 ? 1.0 / ia[i] : 0.0;
+    } else {
+      im[i] = ip[i] = ia[i] = ipi[i] = iai[i] = 0.0;
+    }
+    rs[i] = rest ? rest[i] : 0.0;
+  }
+  RDK_TRY(upload_scalar(ctx, ctx->r, ctx->h_r));
+  RDK_TRY(upload_scalar(ctx, ctx->h, ctx->h_h));
+  RDK_TRY(upload_scalar(ctx, ctx->mass, ctx->h_mass));
+  RDK_TRY(upload_scalar(ctx, ctx->inv_mass, im));
+  RDK_TRY(upload_scalar(ctx, ctx->i_perp, ip));
+  RDK_TRY(upload_scalar(ctx, ctx->i_axial, ia));
+  RDK_TRY(upload_scalar(ctx, ctx->i_perp_inv, ipi));
+  RDK_TRY(upload_scalar(ctx, ctx->i_axial_inv, iai));
+  RDK_TRY(upload_scalar(ctx, ctx->rest, rs));
+  RDK_TRY(rdk_upload_state(ctx, x, q, v, w));
+  // default: one system holding all bodies, non-periodic unit box
+  ctx->h_sys_of.assign(n, 0);
+  RDK_HIP(hipMemsetAsync(ctx->sys_of.p, 0, n * sizeof(int), ctx->stream));
+  ctx->nsys = 0;
+  ctx->built_chunk_len = -2.0;
+  ctx->n_chunks = 0;
+  ctx->nc = ctx->nm = ctx->nprev = ctx->nhist = 0;
+  ctx->have_manifolds = ctx->have_coloring = false;
+  RDK_HIP(hipStreamSynchronize(ctx->stream));
+  return RDK_OK;
+}
+
+int rdk_set_systems(rdk_ctx* ctx, int nsys, const int* off, const rdk_system_desc* desc) {
+  CTX_CHECK();
+  if (ctx->n <= 0) {
+    ctx->err = "rdk_set_systems: call rdk_set_bodies first";
+    return RDK_ERR_STATE;
+  }
+  if (nsys <= 0 || !off || !desc || off[0] != 0 || off[nsys] != ctx->n) {
+    ctx->err = "rdk_set_systems: body_offsets must start at 0 and end at n";
+    return RDK_ERR_ARG;
+  }
+  ctx->nsys = nsys;
+  ctx->sys_desc.assign(desc, desc + nsys);
+  ctx->sys_off.assign(off, off + nsys + 1);
+  ctx->sys_host.assign(nsys, SysDev{});
+  ctx->h_sys_of.assign(ctx->n, 0);
+  for (int s = 0; s < nsys; ++s) {
+    if (off[s + 1] < off[s]) {
+      ctx->err = "rdk_set_systems: body_offsets must be non-decreasing";
+      return RDK_ERR_ARG;
+    }
+    for (int i = off[s]; i < off[s + 1]; ++i) ctx->h_sys_of[i] = s;
+    SysDev& S = ctx->sys_host[s];
+    for (int k = 0; k < 3; ++k) {
+      S.lo[k] = desc[s].lo[k];
+      S.hi[k] = desc[s].hi[k];
+      S.span[k] = desc[s].hi[k] - desc[s].lo[k];
+      S.periodic[k] = desc[s].periodic[k] ? 1 : 0;
+      if (!(S.span[k] > 0.0)) {
+        ctx->err = "rdk_set_systems: box extent must be positive";
+        return RDK_ERR_ARG;
+      }
+      S.dims[k] = 1;
+      S.cell[k] = S.span[k];
+      S.inv_cell[k] = 1.0 / S.span[k];
+    }
+    if ((desc[s].floor_wall || desc[s].ceil_wall) && desc[s].periodic[2]) {
+      ctx->err = "rdk_set_systems: plane walls need a non-periodic z axis";
+      return RDK_ERR_ARG;
+    }
+    S.floor_wall = desc[s].floor_wall ? 1 : 0;
+    S.ceil_wall = desc[s].ceil_wall ? 1 : 0;
+    S.cell_off = 0;
+    S.mu = desc[s].mu;
+  }
+  RDK_HIP(hipMemcpyAsync(ctx->sys_of.p, ctx->h_sys_of.data(), ctx->n * sizeof(int), hipMemcpyHostToDevice,
+                         ctx->stream));
+  RDK_TRY(ensure(ctx, ctx->sys_dev, nsys));
+  RDK_HIP(hipMemcpyAsync(ctx->sys_dev.p, ctx->sys_host.data(), nsys * sizeof(SysDev), hipMemcpyHostToDevice,
+                         ctx->stream));
+  RDK_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->built_chunk_len = -2.0;
+  ctx->n_chunks = 0;
+  return RDK_OK;
+}
+
+int rdk_set_physics(rdk_ctx* ctx, const rdk_physics* p) {
+  CTX_CHECK();
+  if (!p || !(p->dt > 0.0)) {
+    ctx->err = "rdk_set_physics: dt must be positive";
+    return RDK_ERR_ARG;
+  }
+  ctx->phys = *p;
+  ctx->have_phys = true;
+  return RDK_OK;
+}
+
+int rdk_upload_state(rdk_ctx* ctx, const double* x, const double* q, const double* v, const double* w) {
+  CTX_CHECK();
+  if (ctx->n <= 0) return RDK_ERR_STATE;
+  if (x) RDK_TRY(upload_planar(ctx, ctx->x, 3, x, ctx->n));
+  if (q) RDK_TRY(upload_planar(ctx, ctx->q, 4, q, ctx->n));
+  if (v) RDK_TRY(upload_planar(ctx, ctx->v, 3, v, ctx->n));
+  if (w) RDK_TRY(upload_planar(ctx, ctx->w, 3, w, ctx->n));
+  return RDK_OK;
+}
+
+int rdk_upload_forces(rdk_ctx* ctx, const double* f, const double* tau) {
+  CTX_CHECK();
+  if (ctx->n <= 0) return RDK_ERR_STATE;
+  if (f) RDK_TRY(upload_planar(ctx, ctx->f, 3, f, ctx->n));
+  if (tau) RDK_TRY(upload_planar(ctx, ctx->tau, 3, tau, ctx->n));
+  return RDK_OK;
+}
+
+int rdk_download_state(rdk_ctx* ctx, double* x, double* q, double* v, double* w, double* f, double* tau) {
+  CTX_CHECK();
+  if (ctx->n <= 0) return RDK_ERR_STATE;
+  if (x) RDK_TRY(download_planar(ctx, ctx->x, 3, x, ctx->n));
+  if (q) RDK_TRY(download_planar(ctx, ctx->q, 4, q, ctx->n));
+  if (v) RDK_TRY(download_planar(ctx, ctx->v, 3, v, ctx->n));
+  if (w) RDK_TRY(download_planar(ctx, ctx->w, 3, w, ctx->n));
+  if (f) RDK_TRY(download_planar(ctx, ctx->f, 3, f, ctx->n));
+  if (tau) RDK_TRY(download_planar(ctx, ctx->tau, 3, tau, ctx->n));
+  return RDK_OK;
+}
+
+static int check_ready(rdk_ctx* ctx) {
+  if (ctx->n <= 0 || ctx->nsys <= 0) {
+    ctx->err = "bodies and systems must be set first";
+    return RDK_ERR_STATE;
+  }
+  return RDK_OK;
+}
+
+int rdk_detect(rdk_ctx* ctx, const rdk_detect_params* p, int* n_contacts) {
+  CTX_CHECK();
+  if (!p || p->delta < 0.0) return RDK_ERR_ARG;
+  RDK_TRY(check_ready(ctx));
+  RDK_TRY(detect_run(ctx, p));
+  ctx->have_manifolds = false;
+  if (n_contacts) *n_contacts = ctx->nc;
+  return prof_flush(ctx);
+}
+
+int rdk_set_pair_capacity(rdk_ctx* ctx, int cap) {
+  if (!ctx || cap < 0) return RDK_ERR_ARG;
+  ctx->pair_cap_forced = cap;
+  if (cap > 0) {
+    release(ctx->pkeys);
+    release(ctx->pkeys_alt);
+  }
+  return RDK_OK;
+}
+
+int rdk_detect_info(rdk_ctx* ctx, int* n_chunks, int* n_cells, int64_t* cand, int* regrows) {
+  if (!ctx) return RDK_ERR_ARG;
+  if (n_chunks) *n_chunks = ctx->n_chunks;
+  if (n_cells) *n_cells = ctx->n_cells;
+  if (cand) *cand = ctx->last_candidates;
+  if (regrows) *regrows = ctx->regrows;
+  return RDK_OK;
+}
+
+}  // extern "C"
+template <class T>
+static int dl(rdk_ctx* ctx, T* dst, const T* src, int n) {
+  if (!dst || n <= 0) return RDK_OK;
+  RDK_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+  return RDK_OK;
+}
+extern "C" {
+
+static int dl3(rdk_ctx* ctx, double* dst, DevBuf<double>* src, int n) {
+  if (!dst || n <= 0) return RDK_OK;
+  std::vector<double> tmp(n);
+  for (int k = 0; k < 3; ++k) {
+    RDK_HIP(hipMemcpyAsync(tmp.data(), src[k].p, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    RDK_HIP(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < n; ++i) dst[3 * i + k] = tmp[i];
+  }
+  return RDK_OK;
+}
+
+int rdk_get_contacts(rdk_ctx* ctx, int cap, int* a, int* b, double* pa, double* pb, double* nrm, double* dist,
+                     double* shift, double* s, double* t, int* type) {
+  CTX_CHECK();
+  const int n = std::min(cap, ctx->nc);
+  RDK_TRY(dl(ctx, a, ctx->ca.p, n));
+  RDK_TRY(dl(ctx, b, ctx->cb.p, n));
+  RDK_TRY(dl(ctx, type, ctx->ctype.p, n));
+  RDK_TRY(dl(ctx, dist, ctx->cdist.p, n));
+  RDK_TRY(dl(ctx, s, ctx->cs.p, n));
+  RDK_TRY(dl(ctx, t, ctx->ct.p, n));
+  RDK_TRY(dl3(ctx, pa, ctx->cpa, n));
+  RDK_TRY(dl3(ctx, pb, ctx->cpb, n));
+  RDK_TRY(dl3(ctx, nrm, ctx->cn, n));
+  RDK_TRY(dl3(ctx, shift, ctx->csh, n));
+  RDK_HIP(hipStreamSynchronize(ctx->stream));
+  return RDK_OK;
+}
+
+int rdk_forces_soft(rdk_ctx* ctx, const rdk_soft_params* p, double* pe) {
+  CTX_CHECK();
+  if (!p || !(p->delta > 0.0) || !(p->nu > 0.0)) return RDK_ERR_ARG;
+  RDK_TRY(check_ready(ctx));
+  if (!ctx->have_phys) return RDK_ERR_STATE;
+  RDK_TRY(forces_run(ctx, 0, p, pe));
+  return prof_flush(ctx);
+}
+
+int rdk_forces_hm(rdk_ctx* ctx, const rdk_hm_params* p, double* pe) {
+  CTX_CHECK();
+  if (!p || !(p->E > 0.0) || !(p->restitution > 0.0)) return RDK_ERR_ARG;
+  RDK_TRY(check_ready(ctx));
+  if (!ctx->have_phys) return RDK_ERR_STATE;
+  RDK_TRY(forces_run(ctx, 1, p, pe));
+  return prof_flush(ctx);
+}
+
+int rdk_history_size(rdk_ctx* ctx, int* n) {
+  if (!ctx || !n) return RDK_ERR_ARG;
+  *n = ctx->nhist;
+  return RDK_OK;
+}
+
+static int check_nsc(rdk_ctx* ctx, const rdk_nsc_params* p) {
+  if (!p || !(p->omega > 0.0) || p->max_sweeps < 0 || p->friction_proj < 0 || p->friction_proj > 1 ||
+      p->warm_mode < 0 || p->warm_mode > 2) {
+    ctx->err = "invalid rdk_nsc_params";
+    return RDK_ERR_ARG;
+  }
+  if (!ctx->have_phys) {
+    ctx->err = "physics parameters not set";
+    return RDK_ERR_STATE;
+  }
+  return RDK_OK;
+}
+
+int rdk_nsc_build(rdk_ctx* ctx, const rdk_nsc_params* p, int* nm) {
+  CTX_CHECK();
+  RDK_TRY(check_ready(ctx));
+  RDK_TRY(check_nsc(ctx, p));
+  RDK_TRY(nsc_build_run(ctx, p));
+  if (nm) *nm = ctx->nm;
+  return prof_flush(ctx);
+}
+
+int rdk_nsc_get_manifolds(rdk_ctx* ctx, int cap, double* rec, int* ab, double* lambda) {
+  CTX_CHECK();
+  if (!ctx->have_manifolds) return RDK_ERR_STATE;
+  const int n = std::min(cap, ctx->nm);
+  if (n <= 0) return RDK_OK;
+  const size_t stride = ctx->mf.cap / 41;
+  if (rec) {
+    std::vector<double> tmp(n);
+    for (int f = 0; f < 41; ++f) {
+      RDK_HIP(hipMemcpyAsync(tmp.data(), ctx->mf.p + (size_t)f * stride, n * sizeof(double), hipMemcpyDeviceToHost,
+                             ctx->stream));
+      RDK_HIP(hipStreamSynchronize(ctx->stream));
+      for (int i = 0; i < n; ++i) rec[(size_t)41 * i + f] = tmp[i];
+    }
+  }
+  if (ab) {
+    std::vector<int> a(n), b(n);
+    RDK_TRY(dl(ctx, a.data(), ctx->ma.p, n));
+    RDK_TRY(dl(ctx, b.data(), ctx->mb.p, n));
+    RDK_HIP(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < n; ++i) {
+      ab[2 * i] = a[i];
+      ab[2 * i + 1] = b[i];
+    }
+  }
+  if (lambda) RDK_TRY(dl3(ctx, lambda, ctx->lam, n));
+  return RDK_OK;
+}
+
+int rdk_nsc_set_lambda(rdk_ctx* ctx, int m, const double* lambda) {
+  CTX_CHECK();
+  if (!ctx->have_manifolds || m != ctx->nm || !lambda) return RDK_ERR_ARG;
+  if (m == 0) return RDK_OK;
+  RDK_TRY(upload_planar(ctx, ctx->lam, 3, lambda, m));
+  ctx->have_coloring = false;
+  return RDK_OK;
+}
+
+int rdk_nsc_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* st) {
+  CTX_CHECK();
+  RDK_TRY(check_nsc(ctx, p));
+  int s = nsc_solve_run(ctx, p, st);
+  int f = prof_flush(ctx);
+  return s != RDK_OK ? s : f;
+}
+
+int rdk_nsc_sweeps(rdk_ctx* ctx, const rdk_nsc_params* p, int nsweeps) {
+  CTX_CHECK();
+  RDK_TRY(check_nsc(ctx, p));
+  if (nsweeps < 0) return RDK_ERR_ARG;
+  RDK_TRY(nsc_sweeps_run(ctx, p, nsweeps));
+  return prof_flush(ctx);
+}
+
+int rdk_nsc_get_coloring(rdk_ctx* ctx, int cap, int* color, int* order, int* ncol) {
+  CTX_CHECK();
+  if (!ctx->have_manifolds) return RDK_ERR_STATE;
+  if (!ctx->have_coloring) RDK_TRY(nsc_color_run(ctx));
+  const int n = std::min(cap, ctx->nm);
+  RDK_TRY(dl(ctx, color, ctx->color.p, n));
+  RDK_TRY(dl(ctx, order, ctx->perm.p, n));
+  RDK_HIP(hipStreamSynchronize(ctx->stream));
+  if (ncol) *ncol = ctx->ncolor;
+  return RDK_OK;
+}
+
+int rdk_nsc_residual(rdk_ctx* ctx, const rdk_nsc_params* p, double* res) {
+  CTX_CHECK();
+  RDK_TRY(check_nsc(ctx, p));
+  if (!ctx->have_manifolds) return RDK_ERR_STATE;
+  if (!ctx->have_coloring) RDK_TRY(nsc_color_run(ctx));
+  // residual runs on the colour-ordered copy: refresh its lambda
+  RDK_TRY(nsc_residual_run(ctx, p, res, nullptr));
+  return prof_flush(ctx);
+}
+
+int rdk_integrate(rdk_ctx* ctx, int stage) {
+  CTX_CHECK();
+  RDK_TRY(check_ready(ctx));
+  RDK_TRY(integrate_run(ctx, stage));
+  return prof_flush(ctx);
+}
+
+int rdk_step_nsc(rdk_ctx* ctx, const rdk_detect_params* dp, const rdk_nsc_params* np, int nsteps,
+                 rdk_step_stats* st) {
+  CTX_CHECK();
+  RDK_TRY(check_ready(ctx));
+  RDK_TRY(check_nsc(ctx, np));
+  if (!dp || nsteps < 0) return RDK_ERR_ARG;
+  rdk_detect_params d = *dp;
+  d.delta = 0.0;  // NSC admits contacts at strict overlap (nsc_contact.cpp:24)
+  rdk_step_stats S;
+  std::memset(&S, 0, sizeof(S));
+  for (int s = 0; s < nsteps; ++s) {
+    RDK_TRY(integrate_run(ctx, RDK_NSC_FREE));
+    RDK_TRY(detect_run(ctx, &d));
+    RDK_TRY(nsc_build_run(ctx, np));
+    rdk_solve_stats ss;
+    std::memset(&ss, 0, sizeof(ss));
+    int rc = nsc_solve_run(ctx, np, &ss);
+    if (rc == RDK_ERR_NOCONV) S.nonconverged_steps++;
+    else if (rc != RDK_OK) return rc;
+    RDK_TRY(integrate_run(ctx, RDK_NSC_POS));
+    S.steps++;
+    S.contacts_total += ss.rod_contacts;
+    S.manifolds_total += ss.manifolds;
+    S.sweeps_total += ss.sweeps;
+    S.colors_total += ss.colors;
+    S.residual_max = std::max(S.residual_max, ss.residual);
+    ctx->work[W_STEPS] += 1;
+    if (ctx->profiling) RDK_TRY(prof_flush(ctx));
+  }
+  RDK_TRY(prof_flush(ctx));
+  if (st) *st = S;
+  return RDK_OK;
+}
+
+int rdk_step_vv(rdk_ctx* ctx, const rdk_detect_params* dp, int model, const void* params, int nsteps,
+                rdk_step_stats* st) {
+  CTX_CHECK();
+  RDK_TRY(check_ready(ctx));
+  if (!dp || !params || nsteps < 0 || model < 0 || model > 1) return RDK_ERR_ARG;
+  if (!ctx->have_phys) return RDK_ERR_STATE;
+  rdk_detect_params d = *dp;
+  d.delta = model == 0 ? ((const rdk_soft_params*)params)->delta : 0.0;
+  rdk_step_stats S;
+  std::memset(&S, 0, sizeof(S));
+  for (int s = 0; s < nsteps; ++s) {
+    for (int pass = 0; pass < 2; ++pass) {
+      RDK_TRY(detect_run(ctx, &d));
+      RDK_TRY(forces_run(ctx, model, params, nullptr));
+      S.contacts_total += ctx->nc;
+      if (pass == 0) {
+        RDK_TRY(integrate_run(ctx, RDK_VV_HALF1));
+        RDK_TRY(integrate_run(ctx, RDK_CLEAR));
+      } else {
+        RDK_TRY(integrate_run(ctx, RDK_VV_HALF2));
+      }
+    }
+    S.steps++;
+    ctx->work[W_STEPS] += 1;
+    if (ctx->profiling) RDK_TRY(prof_flush(ctx));
+  }
+  RDK_TRY(prof_flush(ctx));
+  if (st) *st = S;
+  return RDK_OK;
+}
+
+int rdk_observables(rdk_ctx* ctx, double* ke, double* phi) {
+  CTX_CHECK();
+  RDK_TRY(check_ready(ctx));
+  RDK_TRY(observables_run(ctx, ke, phi));
+  return prof_flush(ctx);
+}
+
+int rdk_set_profiling(rdk_ctx* ctx, int on) {
+  if (!ctx) return RDK_ERR_ARG;
+  RDK_TRY(prof_flush(ctx));
+  ctx->profiling = on != 0;
+  for (int k = 0; k < K_NUM; ++k) {
+    ctx->k_launches[k] = 0;
+    ctx->k_ms[k] = 0.0;
+  }
+  for (int k = 0; k < W_NUM; ++k) ctx->work[k] = 0;
+  return RDK_OK;
+}
+
+int rdk_set_profile_mask(rdk_ctx* ctx, uint64_t mask) {
+  if (!ctx) return RDK_ERR_ARG;
+  RDK_TRY(prof_flush(ctx));
+  ctx->prof_mask = mask;
+  return RDK_OK;
+}
+
+int rdk_kernel_times(rdk_ctx* ctx, int kernel, int64_t* launches, double* total_ms) {
+  if (!ctx || kernel < 0 || kernel >= K_NUM) return RDK_ERR_ARG;
+  RDK_TRY(prof_flush(ctx));
+  if (launches) *launches = ctx->k_launches[kernel];
+  if (total_ms) *total_ms = ctx->k_ms[kernel];
+  return RDK_OK;
+}
+
+const char* rdk_kernel_name(int kernel) {
+  if (kernel < 0 || kernel >= K_NUM) return "";
+  return kKernelNames[kernel];
+}
+
+int rdk_num_kernels(void) { return K_NUM; }
+
+int rdk_work_counters(rdk_ctx* ctx, int64_t* counters, int n) {
+  if (!ctx || !counters) return RDK_ERR_ARG;
+  for (int k = 0; k < n && k < W_NUM; ++k) counters[k] = ctx->work[k];
+  return RDK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,478 @@
+// rdk_detect.hip -- cell-list broad phase with periodic images and the FP64
+// capsule narrow phase (replaces ContactDetector::detectContactsSpatialHash,
+// contact_detector.cpp:487-809, and the O(N^2) kernels of
+// cuda_broadphase.cu:133-581).
+//
+// Pipeline per call:
+//   1. chunk_keys      every rod is cut into K = ceil(2h/chunk_len) sub-segments
+//                      ("chunks"); each chunk's bounding-sphere centre is binned
+//                      into a uniform per-system grid whose cell is at least
+//                      2*max(chunk radius) + delta, so touching chunks always sit
+//                      in the same or adjacent cells (27-cell stencil, periodic
+//                      wrap on periodic axes).
+//   2. radix sort of (cell key, chunk id)
+//   3. cell_ranges     first/last sorted entry of every occupied cell
+//   4. chunk_gather    chunk centre, rod centre, radius, rod id, chunk index in
+//                      sorted (cell) order so the stencil walk reads contiguous
+//                      memory
+//   5. broad_narrow    one thread per chunk of rod a walks the 27 cells and,
+//                      for every chunk of a rod b > a whose bounding sphere
+//                      (under the rod-centre minimum image) touches its own, runs
+//                      the full-rod Lumelsky narrow phase.  A pair is emitted
+//                      only by the chunk pair that contains the closest points
+//                      (owner rule), so each contact appears exactly once
+//                      whatever the chunking.  Emission is wavefront-aggregated
+//                      (ballot + popcount, one atomic per wavefront).
+//   6. radix sort of the 64-bit (a,b) keys -> canonical (a,b) order
+//   7. contact_record  recomputes the narrow phase per sorted pair and writes
+//                      the structure-of-arrays contact record.
+#include <algorithm>
+#include <cmath>
+
+#include "rdk_internal.hpp"
+
+namespace rdk {
+
+__device__ __forceinline__ int cell_coord(double c, double lo, double inv_cell, int dims, int periodic) {
+  int i = (int)floor((c - lo) * inv_cell);
+  if (periodic) {
+    i %= dims;
+    if (i < 0) i += dims;
+  } else {
+    i = i < 0 ? 0 : (i >= dims ? dims - 1 : i);
+  }
+  return i;
+}
+
+__device__ __forceinline__ q4 load_q(const BodyPtrs& B, int i) {
+  return q4{B.q[0][i], B.q[1][i], B.q[2][i], B.q[3][i]};
+}
+__device__ __forceinline__ d3 load3(double* const* a, int i) { return mk3(a[0][i], a[1][i], a[2][i]); }
+
+__device__ __forceinline__ void chunk_geom(const BodyPtrs& B, int i, int k, int K, d3& c, double& rad) {
+  const d3 x = load3(B.x, i);
+  const d3 u = axis_of(load_q(B, i));
+  const double h = B.h[i];
+  const d3 a1 = x - u * h;
+  const d3 a2 = x + u * h;
+  const double sc = ((double)k + 0.5) / (double)K;
+  c = a1 + (a2 - a1) * sc;
+  rad = h / (double)K + B.r[i];
+}
+
+__global__ void k_chunk_keys(BodyPtrs B, const SysDev* __restrict__ sys, const int* __restrict__ chunk_rod,
+                             const int* __restrict__ chunk_k, const int* __restrict__ rod_nchunk, int M,
+                             uint32_t* __restrict__ key, int* __restrict__ id) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M) return;
+  const int i = chunk_rod[m];
+  const SysDev S = sys[B.sys[i]];
+  d3 c;
+  double rad;
+  chunk_geom(B, i, chunk_k[m], rod_nchunk[i], c, rad);
+  int cx = cell_coord(c.x, S.lo[0], S.inv_cell[0], S.dims[0], S.periodic[0]);
+  int cy = cell_coord(c.y, S.lo[1], S.inv_cell[1], S.dims[1], S.periodic[1]);
+  int cz = cell_coord(c.z, S.lo[2], S.inv_cell[2], S.dims[2], S.periodic[2]);
+  key[m] = (uint32_t)(S.cell_off + cx + S.dims[0] * (cy + S.dims[1] * cz));
+  id[m] = m;
+}
+
+__global__ void k_fill_int(int* __restrict__ a, int n, int v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+__global__ void k_cell_ranges(const uint32_t* __restrict__ key, int M, int* __restrict__ cstart,
+                              int* __restrict__ cend) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= M) return;
+  uint32_t k = key[p];
+  if (p == 0 || key[p - 1] != k) cstart[k] = p;
+  if (p == M - 1 || key[p + 1] != k) cend[k] = p + 1;
+}
+
+struct SortedChunks {
+  double *c[3], *x[3], *rad;
+  int *rod, *k;
+};
+
+__global__ void k_chunk_gather(BodyPtrs B, const int* __restrict__ sorted_id, const int* __restrict__ chunk_rod,
+                               const int* __restrict__ chunk_k, const int* __restrict__ rod_nchunk, int M,
+                               SortedChunks S) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= M) return;
+  const int m = sorted_id[p];
+  const int i = chunk_rod[m];
+  const int k = chunk_k[m];
+  d3 c;
+  double rad;
+  chunk_geom(B, i, k, rod_nchunk[i], c, rad);
+  S.c[0][p] = c.x;
+  S.c[1][p] = c.y;
+  S.c[2][p] = c.z;
+  S.x[0][p] = B.x[0][i];
+  S.x[1][p] = B.x[1][i];
+  S.x[2][p] = B.x[2][i];
+  S.rad[p] = rad;
+  S.rod[p] = i;
+  S.k[p] = k;
+}
+
+// Wavefront-aggregated append: one atomic per wavefront of active lanes.
+__device__ __forceinline__ unsigned wave_append(unsigned* counter) {
+  const unsigned long long act = __ballot(1);
+  const int lane = __lane_id();
+  const int leader = __ffsll((long long)act) - 1;
+  const unsigned rank = (unsigned)__popcll(act & ((1ull << lane) - 1ull));
+  unsigned base = 0;
+  if (lane == leader) base = atomicAdd(counter, (unsigned)__popcll(act));
+  base = __shfl(base, leader);
+  return base + rank;
+}
+
+__device__ __forceinline__ int owner_chunk(double s, int K) {
+  int k = (int)floor(s * (double)K);
+  return k < 0 ? 0 : (k >= K ? K - 1 : k);
+}
+
+__global__ void __launch_bounds__(256) k_broad_narrow(BodyPtrs B, const SysDev* __restrict__ sys, SortedChunks SC,
+                                                      const int* __restrict__ cstart, const int* __restrict__ cend,
+                                                      const int* __restrict__ rod_nchunk, int M, double delta,
+                                                      uint64_t* __restrict__ out, unsigned cap,
+                                                      unsigned* __restrict__ counter,
+                                                      unsigned long long* __restrict__ cand_counter) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long ncand = 0;
+  if (p < M) {
+    const int a = SC.rod[p];
+    const int ka = SC.k[p];
+    const d3 ca = mk3(SC.c[0][p], SC.c[1][p], SC.c[2][p]);
+    const d3 xa = mk3(SC.x[0][p], SC.x[1][p], SC.x[2][p]);
+    const double rada = SC.rad[p];
+    const SysDev S = sys[B.sys[a]];
+    const int Ka = rod_nchunk[a];
+    const bool a_static = !(B.inv_mass[a] > 0.0);
+    const int cc[3] = {cell_coord(ca.x, S.lo[0], S.inv_cell[0], S.dims[0], S.periodic[0]),
+                       cell_coord(ca.y, S.lo[1], S.inv_cell[1], S.dims[1], S.periodic[1]),
+                       cell_coord(ca.z, S.lo[2], S.inv_cell[2], S.dims[2], S.periodic[2])};
+    d3 ua = mk3(0, 0, 0);
+    double ha = 0.0, ra = 0.0;
+    bool a_loaded = false;
+    for (int dz = -1; dz <= 1; ++dz) {
+      int z = cc[2] + dz;
+      if (S.periodic[2]) {
+        if (S.dims[2] == 1 && dz != 0) continue;
+        z = (z + S.dims[2]) % S.dims[2];
+      } else if (z < 0 || z >= S.dims[2]) continue;
+      for (int dy = -1; dy <= 1; ++dy) {
+        int y = cc[1] + dy;
+        if (S.periodic[1]) {
+          if (S.dims[1] == 1 && dy != 0) continue;
+          y = (y + S.dims[1]) % S.dims[1];
+        } else if (y < 0 || y >= S.dims[1]) continue;
+        for (int dx = -1; dx <= 1; ++dx) {
+          int x = cc[0] + dx;
+          if (S.periodic[0]) {
+            if (S.dims[0] == 1 && dx != 0) continue;
+            x = (x + S.dims[0]) % S.dims[0];
+          } else if (x < 0 || x >= S.dims[0]) continue;
+          const int cell = S.cell_off + x + S.dims[0] * (y + S.dims[1] * z);
+          const int q0 = cstart[cell], q1 = cend[cell];
+          for (int q = q0; q < q1; ++q) {
+            const int b = SC.rod[q];
+            if (b <= a) continue;
+            const d3 xb = mk3(SC.x[0][q], SC.x[1][q], SC.x[2][q]);
+            const d3 sh = mk3(image_shift(xb.x - xa.x, S.span[0], S.periodic[0]),
+                              image_shift(xb.y - xa.y, S.span[1], S.periodic[1]),
+                              image_shift(xb.z - xa.z, S.span[2], S.periodic[2]));
+            const d3 d = mk3(SC.c[0][q] + sh.x - ca.x, SC.c[1][q] + sh.y - ca.y, SC.c[2][q] + sh.z - ca.z);
+            const double thr = (rada + SC.rad[q] + delta) * (1.0 + 1e-9) + 1e-9;
+            if (!(dot(d, d) < thr * thr)) continue;
+            if (a_static && !(B.inv_mass[b] > 0.0)) continue;
+            ++ncand;
+            if (!a_loaded) {
+              ua = axis_of(load_q(B, a));
+              ha = B.h[a];
+              ra = B.r[a];
+              a_loaded = true;
+            }
+            const d3 ub = axis_of(load_q(B, b));
+            NarrowOut o;
+            if (capsule_narrow(xa, ua, ha, ra, xb, ub, B.h[b], B.r[b], S.span, S.periodic, delta, false, o)) {
+              if (owner_chunk(o.s, Ka) == ka && owner_chunk(o.t, rod_nchunk[b]) == SC.k[q]) {
+                const unsigned pos = wave_append(counter);
+                if (pos < cap) out[pos] = ((uint64_t)(uint32_t)a << 32) | (uint64_t)(uint32_t)b;
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+  // wave-level reduction of the candidate count
+  for (int off = 32; off > 0; off >>= 1) ncand += __shfl_down(ncand, off);
+  if (__lane_id() == 0 && ncand) atomicAdd(cand_counter, ncand);
+}
+
+struct ContactPtrs {
+  int *a, *b, *type;
+  double *pa[3], *pb[3], *n[3], *sh[3], *dist, *hs, *s, *t;
+};
+
+__global__ void k_contact_record(BodyPtrs B, const SysDev* __restrict__ sys, const uint64_t* __restrict__ keys, int nc,
+                                 double delta, ContactPtrs C) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nc) return;
+  const uint64_t key = keys[i];
+  const int a = (int)(key >> 32), b = (int)(key & 0xffffffffu);
+  const SysDev S = sys[B.sys[a]];
+  NarrowOut o;
+  o.type = 0;
+  o.dist = 0.0;
+  o.pa = o.pb = o.n = mk3(0, 0, 0);
+  capsule_narrow(load3(B.x, a), axis_of(load_q(B, a)), B.h[a], B.r[a], load3(B.x, b), axis_of(load_q(B, b)), B.h[b],
+                 B.r[b], S.span, S.periodic, delta, true, o);
+  C.a[i] = a;
+  C.b[i] = b;
+  C.type[i] = o.type;
+  C.pa[0][i] = o.pa.x;
+  C.pa[1][i] = o.pa.y;
+  C.pa[2][i] = o.pa.z;
+  C.pb[0][i] = o.pb.x;
+  C.pb[1][i] = o.pb.y;
+  C.pb[2][i] = o.pb.z;
+  C.n[0][i] = o.n.x;
+  C.n[1][i] = o.n.y;
+  C.n[2][i] = o.n.z;
+  C.sh[0][i] = o.shift.x;
+  C.sh[1][i] = o.shift.y;
+  C.sh[2][i] = o.shift.z;
+  C.dist[i] = o.dist;
+  C.hs[i] = B.r[a] + B.r[b];
+  C.s[i] = o.s;
+  C.t[i] = o.t;
+}
+
+// ---------------------------------------------------------------------------
+// host orchestration
+
+static int build_grid(rdk_ctx* ctx, const rdk_detect_params* p) {
+  const double cl = p->chunk_len;
+  const double scale = p->cell_scale >= 1.0 ? p->cell_scale : 1.0;
+  if (cl == ctx->built_chunk_len && scale == ctx->built_cell_scale && p->delta == ctx->built_delta &&
+      ctx->n_chunks > 0)
+    return RDK_OK;
+  const int n = ctx->n;
+  std::vector<int> nchunk(n), crod, ck;
+  crod.reserve(n);
+  ck.reserve(n);
+  std::vector<double> radmax(ctx->nsys, 0.0);
+  for (int i = 0; i < n; ++i) {
+    int K = 1;
+    if (cl > 0.0) K = std::max(1, (int)std::ceil(2.0 * ctx->h_h[i] / cl));
+    nchunk[i] = K;
+    for (int k = 0; k < K; ++k) {
+      crod.push_back(i);
+      ck.push_back(k);
+    }
+    double rad = ctx->h_h[i] / K + ctx->h_r[i];
+    int s = ctx->h_sys_of[i];
+    radmax[s] = std::max(radmax[s], rad);
+  }
+  const int M = (int)crod.size();
+  int cell_off = 0;
+  for (int s = 0; s < ctx->nsys; ++s) {
+    SysDev& S = ctx->sys_host[s];
+    const double cmin = (2.0 * radmax[s] + p->delta) * scale * (1.0 + 1e-9) + 1e-9;
+    int64_t total = 1;
+    for (int k = 0; k < 3; ++k) {
+      int d = (int)std::floor(S.span[k] / cmin);
+      if (d < 1) d = 1;
+      if (S.periodic[k] && d < 3) d = 1;
+      if (d > 1024) d = 1024;
+      S.dims[k] = d;
+      total *= d;
+    }
+    // keep the grid modest for very dense or tiny-cell configurations
+    while (total > (int64_t)(1 << 22)) {
+      total = 1;
+      for (int k = 0; k < 3; ++k) {
+        int d = std::max(1, S.dims[k] / 2);
+        if (S.periodic[k] && d < 3) d = 1;
+        S.dims[k] = d;
+        total *= d;
+      }
+    }
+    for (int k = 0; k < 3; ++k) {
+      S.cell[k] = S.span[k] / S.dims[k];
+      S.inv_cell[k] = 1.0 / S.cell[k];
+    }
+    S.cell_off = cell_off;
+    cell_off += (int)total;
+  }
+  ctx->n_cells = cell_off;
+  ctx->n_chunks = M;
+  RDK_TRY(ensure(ctx, ctx->chunk_rod, M));
+  RDK_TRY(ensure(ctx, ctx->chunk_k, M));
+  RDK_TRY(ensure(ctx, ctx->rod_nchunk, n));
+  RDK_HIP(hipMemcpyAsync(ctx->chunk_rod.p, crod.data(), M * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  RDK_HIP(hipMemcpyAsync(ctx->chunk_k.p, ck.data(), M * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  RDK_HIP(hipMemcpyAsync(ctx->rod_nchunk.p, nchunk.data(), n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  RDK_TRY(ensure(ctx, ctx->sys_dev, ctx->nsys));
+  RDK_HIP(hipMemcpyAsync(ctx->sys_dev.p, ctx->sys_host.data(), ctx->nsys * sizeof(SysDev), hipMemcpyHostToDevice,
+                         ctx->stream));
+  RDK_HIP(hipStreamSynchronize(ctx->stream));
+  RDK_TRY(ensure(ctx, ctx->ckey, M));
+  RDK_TRY(ensure(ctx, ctx->ckey_alt, M));
+  RDK_TRY(ensure(ctx, ctx->cid, M));
+  RDK_TRY(ensure(ctx, ctx->cid_alt, M));
+  RDK_TRY(ensure(ctx, ctx->cell_start, ctx->n_cells));
+  RDK_TRY(ensure(ctx, ctx->cell_end, ctx->n_cells));
+  for (int k = 0; k < 3; ++k) {
+    RDK_TRY(ensure(ctx, ctx->sc[k], M));
+    RDK_TRY(ensure(ctx, ctx->sx[k], M));
+  }
+  RDK_TRY(ensure(ctx, ctx->srad, M));
+  RDK_TRY(ensure(ctx, ctx->srod, M));
+  RDK_TRY(ensure(ctx, ctx->sk, M));
+  ctx->built_chunk_len = cl;
+  ctx->built_cell_scale = scale;
+  ctx->built_delta = p->delta;
+  return RDK_OK;
+}
+
+static int ensure_contacts(rdk_ctx* ctx, int nc) {
+  RDK_TRY(ensure(ctx, ctx->ca, nc));
+  RDK_TRY(ensure(ctx, ctx->cb, nc));
+  RDK_TRY(ensure(ctx, ctx->ctype, nc));
+  for (int k = 0; k < 3; ++k) {
+    RDK_TRY(ensure(ctx, ctx->cpa[k], nc));
+    RDK_TRY(ensure(ctx, ctx->cpb[k], nc));
+    RDK_TRY(ensure(ctx, ctx->cn[k], nc));
+    RDK_TRY(ensure(ctx, ctx->csh[k], nc));
+  }
+  RDK_TRY(ensure(ctx, ctx->cdist, nc));
+  RDK_TRY(ensure(ctx, ctx->chs, nc));
+  RDK_TRY(ensure(ctx, ctx->cs, nc));
+  RDK_TRY(ensure(ctx, ctx->ct, nc));
+  return RDK_OK;
+}
+
+int detect_run(rdk_ctx* ctx, const rdk_detect_params* p) {
+  if (ctx->n <= 0 || ctx->nsys <= 0) {
+    ctx->err = "rdk_detect: bodies and systems must be set first";
+    return RDK_ERR_STATE;
+  }
+  RDK_TRY(build_grid(ctx, p));
+  const int M = ctx->n_chunks;
+  const int nb = 256;
+  BodyPtrs B = body_ptrs(ctx);
+  {
+    ProfScope ps(ctx, K_CHUNK_KEYS);
+    hipLaunchKernelGGL(k_chunk_keys, dim3(grid_for(M, nb)), dim3(nb), 0, ctx->stream, B, ctx->sys_dev.p,
+                       ctx->chunk_rod.p, ctx->chunk_k.p, ctx->rod_nchunk.p, M, ctx->ckey.p, ctx->cid.p);
+  }
+  bool alt = false;
+  RDK_TRY(sort_pairs_u32(ctx, ctx->ckey.p, ctx->ckey_alt.p, ctx->cid.p, ctx->cid_alt.p, M,
+                         bits_for((uint64_t)std::max(1, ctx->n_cells - 1)), alt));
+  const uint32_t* skey = alt ? ctx->ckey_alt.p : ctx->ckey.p;
+  const int* sid = alt ? ctx->cid_alt.p : ctx->cid.p;
+  {
+    ProfScope ps(ctx, K_CELL_RANGES);
+    hipLaunchKernelGGL(k_fill_int, dim3(grid_for(ctx->n_cells, nb)), dim3(nb), 0, ctx->stream, ctx->cell_start.p,
+                       ctx->n_cells, 0);
+    hipLaunchKernelGGL(k_fill_int, dim3(grid_for(ctx->n_cells, nb)), dim3(nb), 0, ctx->stream, ctx->cell_end.p,
+                       ctx->n_cells, 0);
+    hipLaunchKernelGGL(k_cell_ranges, dim3(grid_for(M, nb)), dim3(nb), 0, ctx->stream, skey, M, ctx->cell_start.p,
+                       ctx->cell_end.p);
+  }
+  SortedChunks SC;
+  for (int k = 0; k < 3; ++k) {
+    SC.c[k] = ctx->sc[k].p;
+    SC.x[k] = ctx->sx[k].p;
+  }
+  SC.rad = ctx->srad.p;
+  SC.rod = ctx->srod.p;
+  SC.k = ctx->sk.p;
+  {
+    ProfScope ps(ctx, K_CHUNK_GATHER);
+    hipLaunchKernelGGL(k_chunk_gather, dim3(grid_for(M, nb)), dim3(nb), 0, ctx->stream, B, sid, ctx->chunk_rod.p,
+                       ctx->chunk_k.p, ctx->rod_nchunk.p, M, SC);
+  }
+  RDK_TRY(ensure(ctx, ctx->counters, 8));
+  if (!ctx->h_counters) RDK_HIP(hipHostMalloc((void**)&ctx->h_counters, 64, hipHostMallocDefault));
+  unsigned cap;
+  if (ctx->pair_cap_forced > 0) cap = (unsigned)ctx->pair_cap_forced;
+  else cap = (unsigned)std::max<size_t>(ctx->pkeys.cap, (size_t)std::max(1024, 4 * ctx->n));
+  unsigned total = 0;
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    RDK_TRY(ensure(ctx, ctx->pkeys, cap));
+    RDK_TRY(ensure(ctx, ctx->pkeys_alt, cap));
+    RDK_HIP(hipMemsetAsync(ctx->counters.p, 0, 8 * sizeof(unsigned), ctx->stream));
+    {
+      ProfScope ps(ctx, K_BROAD_NARROW);
+      hipLaunchKernelGGL(k_broad_narrow, dim3(grid_for(M, nb)), dim3(nb), 0, ctx->stream, B, ctx->sys_dev.p, SC,
+                         ctx->cell_start.p, ctx->cell_end.p, ctx->rod_nchunk.p, M, p->delta, ctx->pkeys.p, cap,
+                         ctx->counters.p, (unsigned long long*)(ctx->counters.p + 2));
+    }
+    RDK_HIP(hipMemcpyAsync(ctx->h_counters, ctx->counters.p, 4 * sizeof(unsigned), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    RDK_HIP(hipStreamSynchronize(ctx->stream));
+    total = ctx->h_counters[0];
+    ctx->last_candidates = (int64_t)(((uint64_t)ctx->h_counters[3] << 32) | ctx->h_counters[2]);
+    if (total <= cap) break;
+    // capacity exceeded: the kernel counted every contact; regrow and rerun
+    ctx->regrows++;
+    if (ctx->pair_cap_forced > 0) {
+      ctx->pair_cap_forced = 0;
+    }
+    cap = total + total / 4 + 64;
+    if (attempt == 2) {
+      ctx->err = "rdk_detect: pair capacity still exceeded after regrow";
+      return RDK_ERR_CAPACITY;
+    }
+  }
+  const int nc = (int)total;
+  ctx->nc = nc;
+  ctx->last_delta = p->delta;
+  ctx->work[W_BODY_PASSES] += 1;
+  ctx->work[W_CHUNKS] += M;
+  ctx->work[W_CANDIDATES] += ctx->last_candidates;
+  ctx->work[W_CONTACTS] += nc;
+  RDK_TRY(ensure_contacts(ctx, std::max(nc, 1)));
+  if (nc == 0) return RDK_OK;
+  bool kalt = false;
+  RDK_TRY(sort_keys_u64(ctx, ctx->pkeys.p, ctx->pkeys_alt.p, nc, 32 + bits_for((uint64_t)ctx->n), kalt));
+  const uint64_t* keys = kalt ? ctx->pkeys_alt.p : ctx->pkeys.p;
+  ContactPtrs C;
+  C.a = ctx->ca.p;
+  C.b = ctx->cb.p;
+  C.type = ctx->ctype.p;
+  for (int k = 0; k < 3; ++k) {
+    C.pa[k] = ctx->cpa[k].p;
+    C.pb[k] = ctx->cpb[k].p;
+    C.n[k] = ctx->cn[k].p;
+    C.sh[k] = ctx->csh[k].p;
+  }
+  C.dist = ctx->cdist.p;
+  C.hs = ctx->chs.p;
+  C.s = ctx->cs.p;
+  C.t = ctx->ct.p;
+  {
+    ProfScope ps(ctx, K_CONTACT_RECORD);
+    hipLaunchKernelGGL(k_contact_record, dim3(grid_for(nc, nb)), dim3(nb), 0, ctx->stream, B, ctx->sys_dev.p, keys,
+                       nc, p->delta, C);
+  }
+  // keep the canonical keys for history matching
+  if (kalt) {
+    uint64_t* tmp = ctx->pkeys.p;
+    size_t tc = ctx->pkeys.cap;
+    ctx->pkeys.p = ctx->pkeys_alt.p;
+    ctx->pkeys.cap = ctx->pkeys_alt.cap;
+    ctx->pkeys_alt.p = tmp;
+    ctx->pkeys_alt.cap = tc;
+  }
+  return RDK_OK;
+}
+
+}  // namespace rdk

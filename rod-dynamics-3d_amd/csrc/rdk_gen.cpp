@@ -1,0 +1,220 @@
+// rdk_gen.cpp -- host-side synthetic packings: random sequential addition of
+// non-overlapping capsules and thermal velocities (the reference's
+// resetScene "nonoverlap" populate with its 27-image check,
+// main.cpp:1857-2140, and the thermal randomInit, main.cpp:2243-2263).
+// Deterministic for a given seed (xoshiro256** stream), independent of the
+// standard library's distributions.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "../../include/rdk.h"
+#include "rdk_math.hpp"
+
+namespace {
+
+using rdk::d3;
+using rdk::mk3;
+
+struct Rng {
+  uint64_t s[4];
+  static uint64_t splitmix(uint64_t& x) {
+    uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+  explicit Rng(uint64_t seed) {
+    for (int i = 0; i < 4; ++i) s[i] = splitmix(seed);
+  }
+  static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+  uint64_t next() {
+    const uint64_t r = rotl(s[1] * 5, 7) * 9;
+    const uint64_t t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl(s[3], 45);
+    return r;
+  }
+  double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+  double normal() {
+    double u1 = uniform();
+    while (u1 <= 0.0) u1 = uniform();
+    const double u2 = uniform();
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
+  }
+};
+
+// uniformly random unit quaternion (Shoemake); its rotated +Y axis is uniform on S^2
+static rdk::q4 random_quat(Rng& g) {
+  const double u1 = g.uniform(), u2 = g.uniform(), u3 = g.uniform();
+  const double a = std::sqrt(1.0 - u1), b = std::sqrt(u1);
+  return rdk::q4{b * std::cos(2.0 * M_PI * u3), a * std::sin(2.0 * M_PI * u2), a * std::cos(2.0 * M_PI * u2),
+                 b * std::sin(2.0 * M_PI * u3)};
+}
+
+struct Grid {
+  int dims[3];
+  double lo[3], cell[3];
+  int periodic[3];
+  std::vector<std::vector<int>> cells;
+  int coord(int k, double c) const {
+    int i = (int)std::floor((c - lo[k]) / cell[k]);
+    if (periodic[k]) {
+      i %= dims[k];
+      if (i < 0) i += dims[k];
+    } else {
+      i = std::min(std::max(i, 0), dims[k] - 1);
+    }
+    return i;
+  }
+  int lin(int x, int y, int z) const { return x + dims[0] * (y + dims[1] * z); }
+};
+
+}  // namespace
+
+extern "C" int rdk_gen_rsa(int n, const double* r, const double* h, const rdk_system_desc* box, double gap,
+                           uint64_t seed, int max_attempts, double* x, double* q, int* placed) {
+  if (n < 0 || !r || !h || !box || !x || !q || !placed || gap < 0.0) return RDK_ERR_ARG;
+  *placed = 0;
+  if (n == 0) return RDK_OK;
+  Rng g(seed);
+  double span[3];
+  double rmax = 0.0, hmax = 0.0;
+  for (int i = 0; i < n; ++i) {
+    rmax = std::max(rmax, r[i]);
+    hmax = std::max(hmax, h[i]);
+  }
+  const double range = 2.0 * (hmax + rmax) + gap;
+  Grid G;
+  for (int k = 0; k < 3; ++k) {
+    span[k] = box->hi[k] - box->lo[k];
+    if (!(span[k] > 0.0)) return RDK_ERR_ARG;
+    int d = std::max(1, (int)std::floor(span[k] / range));
+    if (box->periodic[k] && d < 3) d = 1;
+    d = std::min(d, 512);
+    G.dims[k] = d;
+    G.lo[k] = box->lo[k];
+    G.cell[k] = span[k] / d;
+    G.periodic[k] = box->periodic[k];
+  }
+  G.cells.assign((size_t)G.dims[0] * G.dims[1] * G.dims[2], {});
+  std::vector<d3> cen(n), ax(n);
+  const int attempts = max_attempts > 0 ? max_attempts : 1000;
+  for (int i = 0; i < n; ++i) {
+    bool ok = false;
+    for (int att = 0; att < attempts && !ok; ++att) {
+      d3 c;
+      double* cp[3] = {&c.x, &c.y, &c.z};
+      for (int k = 0; k < 3; ++k) *cp[k] = box->lo[k] + span[k] * g.uniform();
+      const rdk::q4 qq = random_quat(g);
+      const d3 u = rdk::axis_of(qq);
+      // walls on a non-periodic z: keep both caps inside with clearance gap
+      const bool wall = box->floor_wall || box->ceil_wall;
+      if (wall) {
+        const double zlo = std::min(c.z - u.z * h[i], c.z + u.z * h[i]) - r[i];
+        const double zhi = std::max(c.z - u.z * h[i], c.z + u.z * h[i]) + r[i];
+        if (box->floor_wall && zlo < box->lo[2] + gap) continue;
+        if (box->ceil_wall && zhi > box->hi[2] - gap) continue;
+      }
+      const d3 a1 = c - u * h[i], a2 = c + u * h[i];
+      int cc[3] = {G.coord(0, c.x), G.coord(1, c.y), G.coord(2, c.z)};
+      bool clash = false;
+      for (int dz = -1; dz <= 1 && !clash; ++dz) {
+        if (G.dims[2] == 1 && dz != 0) continue;
+        int zz = cc[2] + dz;
+        if (G.periodic[2]) zz = (zz + G.dims[2]) % G.dims[2];
+        else if (zz < 0 || zz >= G.dims[2]) continue;
+        for (int dy = -1; dy <= 1 && !clash; ++dy) {
+          if (G.dims[1] == 1 && dy != 0) continue;
+          int yy = cc[1] + dy;
+          if (G.periodic[1]) yy = (yy + G.dims[1]) % G.dims[1];
+          else if (yy < 0 || yy >= G.dims[1]) continue;
+          for (int dx = -1; dx <= 1 && !clash; ++dx) {
+            if (G.dims[0] == 1 && dx != 0) continue;
+            int xx = cc[0] + dx;
+            if (G.periodic[0]) xx = (xx + G.dims[0]) % G.dims[0];
+            else if (xx < 0 || xx >= G.dims[0]) continue;
+            for (int j : G.cells[G.lin(xx, yy, zz)]) {
+              const double lim = r[i] + r[j] + gap;
+              // every periodic image within reach (27-image check)
+              int lo_i[3], hi_i[3];
+              double base[3];
+              const double dd[3] = {cen[j].x - c.x, cen[j].y - c.y, cen[j].z - c.z};
+              for (int k = 0; k < 3; ++k) {
+                base[k] = 0.0;
+                lo_i[k] = hi_i[k] = 0;
+                if (G.periodic[k]) {
+                  base[k] = -std::floor(dd[k] / span[k] + 0.5) * span[k];
+                  if (span[k] < 2.0 * range + 1e-9) {
+                    lo_i[k] = -1;
+                    hi_i[k] = 1;
+                  }
+                }
+              }
+              for (int iz = lo_i[2]; iz <= hi_i[2] && !clash; ++iz)
+                for (int iy = lo_i[1]; iy <= hi_i[1] && !clash; ++iy)
+                  for (int ix = lo_i[0]; ix <= hi_i[0] && !clash; ++ix) {
+                    const d3 sh = mk3(base[0] + ix * span[0], base[1] + iy * span[1], base[2] + iz * span[2]);
+                    const d3 cj = cen[j] + sh;
+                    const d3 dc = cj - c;
+                    const double reach = h[i] + h[j] + lim;
+                    if (rdk::dot(dc, dc) >= reach * reach) continue;
+                    const d3 b1 = cj - ax[j] * h[j], b2 = cj + ax[j] * h[j];
+                    double s, t;
+                    rdk::segseg(a1, a2, b1, b2, s, t);
+                    const d3 d = (b1 + (b2 - b1) * t) - (a1 + (a2 - a1) * s);
+                    if (rdk::dot(d, d) < lim * lim) clash = true;
+                  }
+              if (clash) break;
+            }
+          }
+        }
+      }
+      if (clash) continue;
+      cen[i] = c;
+      ax[i] = u;
+      x[3 * i] = c.x;
+      x[3 * i + 1] = c.y;
+      x[3 * i + 2] = c.z;
+      q[4 * i] = qq.w;
+      q[4 * i + 1] = qq.x;
+      q[4 * i + 2] = qq.y;
+      q[4 * i + 3] = qq.z;
+      G.cells[G.lin(cc[0], cc[1], cc[2])].push_back(i);
+      ok = true;
+    }
+    if (!ok) return RDK_OK;  // *placed holds the count placed so far
+    *placed = i + 1;
+  }
+  return RDK_OK;
+}
+
+extern "C" int rdk_gen_thermal(int n, const double* q, const double* r, const double* h, const double* mass, double kT,
+                               uint64_t seed, double* v, double* w) {
+  if (n < 0 || !q || !r || !h || !mass || !v || !w || kT < 0.0) return RDK_ERR_ARG;
+  Rng g(seed);
+  for (int i = 0; i < n; ++i) {
+    const double m = mass[i];
+    const double L = 2.0 * h[i];
+    const double Ip = m * (3.0 * r[i] * r[i] + L * L) / 12.0;
+    const double sv = m > 0.0 ? std::sqrt(kT / m) : 0.0;
+    const double sw = Ip > 0.0 ? std::sqrt(kT / Ip) : 0.0;
+    for (int k = 0; k < 3; ++k) v[3 * i + k] = sv * g.normal();
+    const d3 u = rdk::axis_of(rdk::q4{q[4 * i], q[4 * i + 1], q[4 * i + 2], q[4 * i + 3]});
+    // a unit vector perpendicular to u, then e2 = u x e1
+    const d3 ref = std::fabs(u.x) < 0.9 ? mk3(1, 0, 0) : mk3(0, 1, 0);
+    const d3 e1 = rdk::normalize(rdk::cross(u, ref));
+    const d3 e2 = rdk::cross(u, e1);
+    const double w1 = sw * g.normal(), w2 = sw * g.normal();
+    const d3 om = e1 * w1 + e2 * w2;
+    w[3 * i] = om.x;
+    w[3 * i + 1] = om.y;
+    w[3 * i + 2] = om.z;
+  }
+  return RDK_OK;
+}

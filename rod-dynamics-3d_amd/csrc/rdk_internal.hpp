@@ -1,0 +1,272 @@
+// rdk_internal.hpp -- device context of librdk (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/rdk.h"
+#include "rdk_math.hpp"
+
+namespace rdk {
+
+// Kernel ids for the per-kernel event timers.
+enum KernelId {
+  K_CHUNK_KEYS = 0,
+  K_CELL_RANGES,
+  K_CHUNK_GATHER,
+  K_BROAD_NARROW,
+  K_CONTACT_RECORD,
+  K_SORT,
+  K_MANIFOLD,
+  K_WALL,
+  K_CSR,
+  K_COLOR,
+  K_PERMUTE,
+  K_WARM_APPLY,
+  K_PSOR_SWEEP,
+  K_RESIDUAL,
+  K_NSC_FREE,
+  K_NSC_POS,
+  K_VV_HALF1,
+  K_VV_HALF2,
+  K_FORCE,
+  K_BODY_REDUCE,
+  K_HISTORY,
+  K_OBSERVE,
+  K_MLP,
+  K_MISC,
+  K_NUM
+};
+
+// Work counters (algorithmic units) accumulated by the fused steps.
+enum WorkId {
+  W_STEPS = 0,
+  W_BODY_PASSES,      // detection passes over all bodies
+  W_CHUNKS,           // chunk entries binned
+  W_CANDIDATES,       // chunk pairs tested in the broad phase
+  W_CONTACTS,         // contact records emitted
+  W_MANIFOLDS,        // manifolds built
+  W_SWEEP_CONTACTS,   // manifold updates in PSOR sweeps
+  W_RESID_CONTACTS,   // manifold evaluations in residual passes
+  W_INTEGRATE_BODIES, // body updates in integrator passes
+  W_FORCE_CONTACTS,   // soft/HM force evaluations
+  W_NUM
+};
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t cap = 0;
+};
+
+// Per-system constants on the device.
+struct SysDev {
+  double lo[3], hi[3], span[3], cell[3], inv_cell[3];
+  int periodic[3], dims[3];
+  int cell_off, floor_wall, ceil_wall, pad;
+  double mu;
+};
+
+struct BodyPtrs {
+  double *x[3], *q[4], *v[3], *w[3], *f[3], *tau[3];
+  const double *r, *h, *mass, *inv_mass, *i_perp, *i_axial, *i_perp_inv, *i_axial_inv, *rest;
+  const int* sys;
+};
+
+struct EventRec {
+  int kernel;
+  hipEvent_t a, b;
+};
+
+}  // namespace rdk
+
+struct rdk_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // ---- bodies (planar FP64) ----
+  int n = 0;
+  rdk::DevBuf<double> x[3], q[4], v[3], w[3], f[3], tau[3];
+  rdk::DevBuf<double> r, h, mass, inv_mass, i_perp, i_axial, i_perp_inv, i_axial_inv, rest;
+  rdk::DevBuf<int> sys_of;
+  std::vector<double> h_r, h_h, h_mass;
+  std::vector<int> h_sys_of;
+
+  // ---- systems ----
+  int nsys = 0;
+  std::vector<rdk_system_desc> sys_desc;
+  std::vector<int> sys_off;
+  std::vector<rdk::SysDev> sys_host;
+  rdk::DevBuf<rdk::SysDev> sys_dev;
+  rdk_physics phys{};
+  bool have_phys = false;
+
+  // ---- broad phase ----
+  double built_chunk_len = -2.0, built_cell_scale = -1.0, built_delta = -1.0;
+  int n_chunks = 0, n_cells = 0;
+  rdk::DevBuf<int> chunk_rod, chunk_k, rod_nchunk;
+  rdk::DevBuf<uint32_t> ckey, ckey_alt;
+  rdk::DevBuf<int> cid, cid_alt;
+  rdk::DevBuf<int> cell_start, cell_end;
+  rdk::DevBuf<double> sc[3], sx[3], srad;
+  rdk::DevBuf<int> srod, sk;
+  rdk::DevBuf<uint64_t> pkeys, pkeys_alt;
+  rdk::DevBuf<unsigned int> counters;  // [0] pair count, [1..] misc
+  unsigned int* h_counters = nullptr;  // pinned mirror
+  int pair_cap_forced = 0;
+  int regrows = 0;
+  int64_t last_candidates = 0;
+
+  // ---- contacts (sorted by (a,b)) ----
+  int nc = 0;
+  rdk::DevBuf<int> ca, cb, ctype;
+  rdk::DevBuf<double> cpa[3], cpb[3], cn[3], csh[3], cdist, chs, cs, ct;
+  double last_delta = 0.0;
+
+  // ---- NSC manifolds ----
+  int nm = 0, nm_wall = 0;
+  rdk::DevBuf<double> mf, mfc;  // 41 planar fields (natural / colour order)
+  rdk::DevBuf<int> ma, mb, mac, mbc;
+  rdk::DevBuf<double> lam[3], lamc[3];
+  rdk::DevBuf<int> wall_flag, wall_off;
+  rdk::DevBuf<int> color, color_tmp, perm, uncolored;
+  rdk::DevBuf<uint32_t> color_key, color_key_alt;
+  rdk::DevBuf<int> perm_alt;
+  std::vector<int> color_start;  // host, ncolor+1
+  int ncolor = 0;
+  bool have_manifolds = false, have_coloring = false;
+  // body -> manifold CSR
+  rdk::DevBuf<uint32_t> csr_key, csr_key_alt;
+  rdk::DevBuf<int> csr_val, csr_val_alt, body_start;
+  // warm cache
+  int nprev = 0;
+  rdk::DevBuf<uint64_t> prev_keys;
+  rdk::DevBuf<double> prev_lam[3];
+  rdk::DevBuf<double> resid;  // [0] residual bits, [1] ref residual bits
+  // MLP warm start (learned)
+  void* mlp = nullptr;
+
+  // ---- force history (HM spring / Cundall) ----
+  int nhist = 0;
+  uint64_t hist_frame = 0;
+  rdk::DevBuf<uint64_t> hkeys, hkeys_new;
+  rdk::DevBuf<double> hxi[3], hxi_new[3];
+  rdk::DevBuf<uint64_t> hlast, hlast_new;
+  rdk::DevBuf<int> hflag, hscan;
+  rdk::DevBuf<double> cf[12];  // per-contact force terms: Fb(3), Ta(3), Tb(3), xi(3)
+  rdk::DevBuf<double> pe;
+
+  // ---- scratch ----
+  rdk::DevBuf<char> temp;
+  rdk::DevBuf<double> red;  // reduction scratch
+
+  // ---- profiling ----
+  bool profiling = false;
+  uint64_t prof_mask = ~(uint64_t)0;
+  std::vector<rdk::EventRec> ev_pending;
+  std::vector<hipEvent_t> ev_pool;
+  int64_t k_launches[rdk::K_NUM] = {};
+  double k_ms[rdk::K_NUM] = {};
+  int64_t work[rdk::W_NUM] = {};
+};
+
+namespace rdk {
+
+// error handling
+#define RDK_HIP(call)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess) {                                                            \
+      ctx->err = std::string(#call) + ": " + hipGetErrorString(e_);                    \
+      return RDK_ERR_HIP;                                                              \
+    }                                                                                  \
+  } while (0)
+
+#define RDK_TRY(expr)        \
+  do {                       \
+    int st_ = (expr);        \
+    if (st_ != RDK_OK) return st_; \
+  } while (0)
+
+template <class T>
+int ensure(rdk_ctx* ctx, DevBuf<T>& b, size_t n) {
+  if (n <= b.cap && b.p) return RDK_OK;
+  size_t nc = n < 16 ? 16 : n + n / 4;
+  if (b.p) {
+    hipError_t e = hipFree(b.p);
+    if (e != hipSuccess) {
+      ctx->err = std::string("hipFree: ") + hipGetErrorString(e);
+      return RDK_ERR_HIP;
+    }
+  }
+  b.p = nullptr;
+  b.cap = 0;
+  hipError_t e = hipMalloc((void**)&b.p, nc * sizeof(T));
+  if (e != hipSuccess) {
+    ctx->err = std::string("hipMalloc: ") + hipGetErrorString(e);
+    return RDK_ERR_HIP;
+  }
+  b.cap = nc;
+  return RDK_OK;
+}
+
+template <class T>
+void release(DevBuf<T>& b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.cap = 0;
+}
+
+BodyPtrs body_ptrs(rdk_ctx* ctx);
+
+// event timing around one launch
+void prof_begin(rdk_ctx* ctx, int kernel, EventRec& rec);
+void prof_end(rdk_ctx* ctx, EventRec& rec);
+int prof_flush(rdk_ctx* ctx);
+
+struct ProfScope {
+  rdk_ctx* ctx;
+  EventRec rec;
+  bool on;
+  ProfScope(rdk_ctx* c, int k) : ctx(c), on(c->profiling && ((c->prof_mask >> k) & 1)) {
+    c->k_launches[k] += 1;
+    if (on) prof_begin(c, k, rec);
+  }
+  ~ProfScope() {
+    if (on) prof_end(ctx, rec);
+  }
+};
+
+static inline int grid_for(int64_t n, int block) { return (int)((n + block - 1) / block); }
+
+// stage entry points implemented in the .hip files
+int detect_run(rdk_ctx* ctx, const rdk_detect_params* p);
+int nsc_build_run(rdk_ctx* ctx, const rdk_nsc_params* p);
+int nsc_solve_run(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* st);
+int nsc_sweeps_run(rdk_ctx* ctx, const rdk_nsc_params* p, int nsweeps);
+int nsc_color_run(rdk_ctx* ctx);
+int nsc_residual_run(rdk_ctx* ctx, const rdk_nsc_params* p, double* res, double* res_ref);
+int integrate_run(rdk_ctx* ctx, int stage);
+int forces_run(rdk_ctx* ctx, int model, const void* params, double* pe);
+int observables_run(rdk_ctx* ctx, double* ke, double* phi);
+int mlp_warm_start(rdk_ctx* ctx);
+void mlp_free(rdk_ctx* ctx);
+
+// rocPRIM-backed sorting / scanning helpers (rdk_sort.hip)
+int sort_pairs_u32(rdk_ctx* ctx, uint32_t* keys, uint32_t* keys_alt, int* vals, int* vals_alt, int n,
+                   int end_bit, bool& result_in_alt);
+int sort_keys_u64(rdk_ctx* ctx, uint64_t* keys, uint64_t* keys_alt, int n, int end_bit,
+                  bool& result_in_alt);
+int exclusive_scan_int(rdk_ctx* ctx, const int* in, int* out, int n);
+
+static inline int bits_for(uint64_t maxval) {
+  int b = 1;
+  while (b < 64 && (uint64_t(1) << b) <= maxval) ++b;
+  return b;
+}
+
+}  // namespace rdk

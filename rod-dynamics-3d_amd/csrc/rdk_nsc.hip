@@ -1,0 +1,930 @@
+// rdk_nsc.hip -- NSC hard contact: manifold build, plane-wall manifolds,
+// contact-graph colouring and the coloured projected Gauss-Seidel (PSOR)
+// solve with the reference's friction clamp (NscContactSolver,
+// nsc_contact.cpp:78-117,190-321,325-655).
+//
+// The reference sweeps manifolds serially in sorted (a,b) order.  That
+// order cannot run in parallel, so manifolds are coloured such that no two
+// manifolds of one colour share a body; the manifolds of a colour are then
+// updated concurrently with exactly the serial update rule, which makes a
+// coloured sweep identical to a serial sweep in colour order.  Colouring is
+// a deterministic speculative greedy (smallest colour free among coloured
+// neighbours; ties among same-round picks go to the lower manifold index).
+#include <algorithm>
+#include <cstring>
+
+#include "rdk_internal.hpp"
+
+namespace rdk {
+
+// manifold field offsets (planar, stride = capacity)
+enum MF {
+  F_N = 0, F_T1 = 3, F_T2 = 6, F_RA = 9, F_RB = 12,
+  F_IAN = 15, F_IAT1 = 18, F_IAT2 = 21, F_IBN = 24, F_IBT1 = 27, F_IBT2 = 30,
+  F_IMA = 33, F_IMB = 34, F_GN = 35, F_GT1 = 36, F_GT2 = 37, F_PHI = 38, F_VNPRE = 39, F_E = 40,
+  F_COUNT = 41
+};
+
+struct MfPtrs {
+  double* f;
+  size_t stride;
+  int *a, *b;
+  double* lam[3];
+  __device__ __forceinline__ double& at(int field, int i) const { return f[(size_t)field * stride + i]; }
+  __device__ __forceinline__ d3 get3(int field, int i) const {
+    return mk3(at(field, i), at(field + 1, i), at(field + 2, i));
+  }
+  __device__ __forceinline__ void put3(int field, int i, d3 v) const {
+    at(field, i) = v.x;
+    at(field + 1, i) = v.y;
+    at(field + 2, i) = v.z;
+  }
+};
+
+__device__ __forceinline__ d3 ld3(double* const* a, int i) { return mk3(a[0][i], a[1][i], a[2][i]); }
+__device__ __forceinline__ void st3(double* const* a, int i, d3 v) {
+  a[0][i] = v.x;
+  a[1][i] = v.y;
+  a[2][i] = v.z;
+}
+__device__ __forceinline__ q4 ldq(const BodyPtrs& B, int i) { return q4{B.q[0][i], B.q[1][i], B.q[2][i], B.q[3][i]}; }
+
+// I_world^-1 v of body i (zero for static bodies)
+__device__ __forceinline__ d3 iw_inv(const BodyPtrs& B, int i, d3 u, d3 v) {
+  if (!(B.inv_mass[i] > 0.0)) return mk3(0, 0, 0);
+  return inertia_apply(u, B.i_perp_inv[i], B.i_axial_inv[i], v);
+}
+
+__device__ __forceinline__ double diag_term(double ima, double imb, d3 rAxd, d3 IArAxd, d3 rBxd, d3 IBrBxd,
+                                            double cfm, bool wall) {
+  double g;
+  if (wall) g = ima + dot(rAxd, IArAxd) + cfm;
+  else g = ima + imb + dot(rAxd, IArAxd) + dot(rBxd, IBrBxd) + cfm;
+  return dmax(g, 1e-12);
+}
+
+__device__ __forceinline__ int lower_bound_u64(const uint64_t* a, int n, uint64_t key) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (a[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+struct ContactIn {
+  const int *a, *b;
+  const double *pa[3], *pb[3], *n[3], *sh[3], *dist, *hs;
+};
+
+__global__ void k_manifold(BodyPtrs B, ContactIn C, int nc, MfPtrs M, double cfm, int warm, const uint64_t* prev_keys,
+                           int nprev, const double* pl0, const double* pl1, const double* pl2) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nc) return;
+  const int a = C.a[i], b = C.b[i];
+  const d3 n = mk3(C.n[0][i], C.n[1][i], C.n[2][i]);
+  const d3 pa = mk3(C.pa[0][i], C.pa[1][i], C.pa[2][i]);
+  const d3 pb = mk3(C.pb[0][i], C.pb[1][i], C.pb[2][i]);
+  const d3 sh = mk3(C.sh[0][i], C.sh[1][i], C.sh[2][i]);
+  d3 t1, t2;
+  tangent_basis(n, t1, t2);
+  const d3 xa = ld3(B.x, a), xb = ld3(B.x, b);
+  const d3 ra = pa - xa;
+  const d3 rb = pb - (xb + sh);
+  const d3 ua = axis_of(ldq(B, a)), ub = axis_of(ldq(B, b));
+  const double ima = B.inv_mass[a], imb = B.inv_mass[b];
+  const d3 dirs[3] = {n, t1, t2};
+  double g[3];
+  d3 IA[3], IB[3];
+  for (int d = 0; d < 3; ++d) {
+    const d3 rAxd = cross(ra, dirs[d]);
+    const d3 rBxd = cross(rb, dirs[d]);
+    IA[d] = iw_inv(B, a, ua, rAxd);
+    IB[d] = iw_inv(B, b, ub, rBxd);
+    g[d] = diag_term(ima, imb, rAxd, IA[d], rBxd, IB[d], cfm, false);
+  }
+  M.put3(F_N, i, n);
+  M.put3(F_T1, i, t1);
+  M.put3(F_T2, i, t2);
+  M.put3(F_RA, i, ra);
+  M.put3(F_RB, i, rb);
+  M.put3(F_IAN, i, IA[0]);
+  M.put3(F_IAT1, i, IA[1]);
+  M.put3(F_IAT2, i, IA[2]);
+  M.put3(F_IBN, i, IB[0]);
+  M.put3(F_IBT1, i, IB[1]);
+  M.put3(F_IBT2, i, IB[2]);
+  M.at(F_IMA, i) = ima;
+  M.at(F_IMB, i) = imb;
+  M.at(F_GN, i) = g[0];
+  M.at(F_GT1, i) = g[1];
+  M.at(F_GT2, i) = g[2];
+  M.at(F_PHI, i) = C.dist[i] - C.hs[i];
+  const d3 vrel = (ld3(B.v, b) + cross(ld3(B.w, b), rb)) - (ld3(B.v, a) + cross(ld3(B.w, a), ra));
+  M.at(F_VNPRE, i) = dot(n, vrel);
+  M.at(F_E, i) = dmin(B.rest[a], B.rest[b]);
+  M.a[i] = a;
+  M.b[i] = b;
+  double l0 = 0.0, l1 = 0.0, l2 = 0.0;
+  if (warm == 1 && nprev > 0) {
+    const uint64_t key = ((uint64_t)(uint32_t)a << 32) | (uint64_t)(uint32_t)b;
+    const int j = lower_bound_u64(prev_keys, nprev, key);
+    if (j < nprev && prev_keys[j] == key) {
+      l0 = dmax(0.0, pl0[j]);
+      l1 = pl1[j];
+      l2 = pl2[j];
+    }
+  }
+  M.lam[0][i] = l0;
+  M.lam[1][i] = l1;
+  M.lam[2][i] = l2;
+}
+
+// Plane-wall candidates: 4 slots per body (floor a1, floor a2, ceiling a1,
+// ceiling a2).  A slot is active when that endpoint's capsule cap penetrates
+// the plane (collision.cpp:155-195 geometry, endpoint form).
+__device__ __forceinline__ bool wall_slot(const BodyPtrs& B, const SysDev& S, int i, int slot, d3& nrm, d3& pt,
+                                          double& sd) {
+  const int plane = slot >> 1, e = slot & 1;
+  if (plane == 0 && !S.floor_wall) return false;
+  if (plane == 1 && !S.ceil_wall) return false;
+  if (!(B.inv_mass[i] > 0.0)) return false;
+  const d3 x = ld3(B.x, i);
+  const d3 u = axis_of(ldq(B, i));
+  const double h = B.h[i], r = B.r[i];
+  const d3 p = e == 0 ? x - u * h : x + u * h;
+  if (plane == 0) {
+    sd = (p.z - S.lo[2]) - r;
+    nrm = mk3(0, 0, -1);
+    pt = p - mk3(0, 0, r);
+  } else {
+    sd = (S.hi[2] - p.z) - r;
+    nrm = mk3(0, 0, 1);
+    pt = p + mk3(0, 0, r);
+  }
+  return sd < 0.0;
+}
+
+__global__ void k_wall_flags(BodyPtrs B, const SysDev* __restrict__ sys, int n, int* __restrict__ flag) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 4 * n) return;
+  const int i = t >> 2, slot = t & 3;
+  const SysDev S = sys[B.sys[i]];
+  d3 nrm, pt;
+  double sd;
+  flag[t] = wall_slot(B, S, i, slot, nrm, pt, sd) ? 1 : 0;
+}
+
+__global__ void k_wall_manifold(BodyPtrs B, const SysDev* __restrict__ sys, int nbody, const int* __restrict__ flag,
+                                const int* __restrict__ off, int base, MfPtrs M, double cfm) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 4 * nbody || !flag[t]) return;
+  const int i = t >> 2, slot = t & 3;
+  const SysDev S = sys[B.sys[i]];
+  d3 n, pt;
+  double sd;
+  wall_slot(B, S, i, slot, n, pt, sd);
+  const int m = base + off[t];
+  d3 t1, t2;
+  tangent_basis(n, t1, t2);
+  const d3 ra = pt - ld3(B.x, i);
+  const d3 u = axis_of(ldq(B, i));
+  const double ima = B.inv_mass[i];
+  const d3 dirs[3] = {n, t1, t2};
+  d3 IA[3];
+  double g[3];
+  const d3 z = mk3(0, 0, 0);
+  for (int d = 0; d < 3; ++d) {
+    const d3 rAxd = cross(ra, dirs[d]);
+    IA[d] = iw_inv(B, i, u, rAxd);
+    g[d] = diag_term(ima, 0.0, rAxd, IA[d], z, z, cfm, true);
+  }
+  M.put3(F_N, m, n);
+  M.put3(F_T1, m, t1);
+  M.put3(F_T2, m, t2);
+  M.put3(F_RA, m, ra);
+  M.put3(F_RB, m, z);
+  M.put3(F_IAN, m, IA[0]);
+  M.put3(F_IAT1, m, IA[1]);
+  M.put3(F_IAT2, m, IA[2]);
+  M.put3(F_IBN, m, z);
+  M.put3(F_IBT1, m, z);
+  M.put3(F_IBT2, m, z);
+  M.at(F_IMA, m) = ima;
+  M.at(F_IMB, m) = 0.0;
+  M.at(F_GN, m) = g[0];
+  M.at(F_GT1, m) = g[1];
+  M.at(F_GT2, m) = g[2];
+  M.at(F_PHI, m) = sd;
+  const d3 vrel = -(ld3(B.v, i) + cross(ld3(B.w, i), ra));
+  M.at(F_VNPRE, m) = dot(n, vrel);
+  M.at(F_E, m) = B.rest[i];
+  M.a[m] = i;
+  M.b[m] = -1;
+  M.lam[0][m] = 0.0;
+  M.lam[1][m] = 0.0;
+  M.lam[2][m] = 0.0;
+}
+
+__global__ void k_fill_color(int* __restrict__ a, int n, int v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+// ---- body -> manifold CSR -------------------------------------------------
+__global__ void k_csr_entries(const int* __restrict__ ma, const int* __restrict__ mb, int nm, int nbody,
+                              uint32_t* __restrict__ key, int* __restrict__ val) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nm) return;
+  key[2 * j] = (uint32_t)ma[j];
+  val[2 * j] = j;
+  const int b = mb[j];
+  key[2 * j + 1] = (uint32_t)(b >= 0 ? b : nbody);
+  val[2 * j + 1] = j;
+}
+
+__global__ void k_body_start(const uint32_t* __restrict__ key, int ne, int nbody, int* __restrict__ start) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > nbody) return;
+  int lo = 0, hi = ne;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (key[mid] < (uint32_t)i) lo = mid + 1;
+    else hi = mid;
+  }
+  start[i] = lo;
+}
+
+// ---- colouring -------------------------------------------------------------
+__global__ void k_color_assign(const int* __restrict__ ma, const int* __restrict__ mb, int nm,
+                               const int* __restrict__ bstart, const int* __restrict__ csr,
+                               const int* __restrict__ color, int* __restrict__ tent, unsigned* __restrict__ overflow) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nm) return;
+  if (color[j] >= 0) {
+    tent[j] = -1;
+    return;
+  }
+  unsigned long long m0 = 0, m1 = 0;
+  const int bodies[2] = {ma[j], mb[j]};
+  for (int s = 0; s < 2; ++s) {
+    const int bd = bodies[s];
+    if (bd < 0) continue;
+    for (int k = bstart[bd]; k < bstart[bd + 1]; ++k) {
+      const int o = csr[k];
+      if (o == j) continue;
+      const int c = color[o];
+      if (c >= 0) {
+        if (c < 64) m0 |= 1ull << c;
+        else m1 |= 1ull << (c - 64);
+      }
+    }
+  }
+  int pick;
+  if (~m0) pick = __ffsll((long long)~m0) - 1;
+  else if (~m1) pick = 64 + __ffsll((long long)~m1) - 1;
+  else {
+    pick = 127;
+    atomicAdd(overflow, 1u);
+  }
+  tent[j] = pick;
+}
+
+__global__ void k_color_resolve(const int* __restrict__ ma, const int* __restrict__ mb, int nm,
+                                const int* __restrict__ bstart, const int* __restrict__ csr,
+                                const int* __restrict__ tent, int* __restrict__ color, unsigned* __restrict__ left,
+                                unsigned* __restrict__ maxcol) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nm) return;
+  const int tj = tent[j];
+  if (tj < 0) return;
+  bool ok = true;
+  const int bodies[2] = {ma[j], mb[j]};
+  for (int s = 0; s < 2 && ok; ++s) {
+    const int bd = bodies[s];
+    if (bd < 0) continue;
+    for (int k = bstart[bd]; k < bstart[bd + 1]; ++k) {
+      const int o = csr[k];
+      if (o < j && tent[o] == tj) {
+        ok = false;
+        break;
+      }
+    }
+  }
+  if (ok) {
+    color[j] = tj;
+    atomicMax(maxcol, (unsigned)tj);
+  } else {
+    atomicAdd(left, 1u);
+  }
+}
+
+__global__ void k_color_keys(const int* __restrict__ color, int nm, uint32_t* __restrict__ key, int* __restrict__ val,
+                             int* __restrict__ hist) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nm) return;
+  key[j] = (uint32_t)color[j];
+  val[j] = j;
+  atomicAdd(&hist[color[j]], 1);
+}
+
+__global__ void k_permute(MfPtrs src, MfPtrs dst, const int* __restrict__ perm, int nm) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nm) return;
+  const int j = perm[i];
+  for (int f = 0; f < F_COUNT; ++f) dst.at(f, i) = src.at(f, j);
+  dst.a[i] = src.a[j];
+  dst.b[i] = src.b[j];
+  dst.lam[0][i] = src.lam[0][j];
+  dst.lam[1][i] = src.lam[1][j];
+  dst.lam[2][i] = src.lam[2][j];
+}
+
+__global__ void k_unpermute_lambda(MfPtrs col, MfPtrs nat, const int* __restrict__ perm, int nm) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nm) return;
+  const int j = perm[i];
+  nat.lam[0][j] = col.lam[0][i];
+  nat.lam[1][j] = col.lam[1][i];
+  nat.lam[2][j] = col.lam[2][i];
+}
+
+// ---- impulses ----------------------------------------------------------------
+// Warm-start application (nsc_contact.cpp:344-380) for one colour.
+__global__ void k_warm_apply(BodyPtrs B, MfPtrs M, int i0, int i1) {
+  int i = i0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= i1) return;
+  const int a = M.a[i], b = M.b[i];
+  const bool wall = b < 0;
+  const double ima = M.at(F_IMA, i), imb = M.at(F_IMB, i);
+  d3 va = ld3(B.v, a), wa = ld3(B.w, a);
+  d3 vb = mk3(0, 0, 0), wb = mk3(0, 0, 0);
+  if (!wall) {
+    vb = ld3(B.v, b);
+    wb = ld3(B.w, b);
+  }
+  const int dir_f[3] = {F_N, F_T1, F_T2};
+  const int ia_f[3] = {F_IAN, F_IAT1, F_IAT2};
+  const int ib_f[3] = {F_IBN, F_IBT1, F_IBT2};
+  bool touched = false;
+  for (int d = 0; d < 3; ++d) {
+    const double l = M.lam[d][i];
+    if (l != 0.0) {
+      touched = true;
+      const d3 dir = M.get3(dir_f[d], i);
+      va = va - dir * (l * ima);
+      wa = wa - M.get3(ia_f[d], i) * l;
+      if (!wall) {
+        vb = vb + dir * (l * imb);
+        wb = wb + M.get3(ib_f[d], i) * l;
+      }
+    }
+  }
+  if (!touched) return;
+  st3(B.v, a, va);
+  st3(B.w, a, wa);
+  if (!wall) {
+    st3(B.v, b, vb);
+    st3(B.w, b, wb);
+  }
+}
+
+struct SolveConst {
+  double omega, cfm, beta, dt;
+  int proj;
+};
+
+// One projected Gauss-Seidel update of manifold i (nsc_contact.cpp:385-552).
+// Body velocities are read once into registers and written back once; the
+// arithmetic is the serial reference update in the same order.
+__device__ __forceinline__ double psor_update(const BodyPtrs& B, const MfPtrs& M, int i, const SolveConst& K,
+                                              double mu) {
+  const int a = M.a[i], b = M.b[i];
+  const bool wall = b < 0;
+  d3 va = ld3(B.v, a), wa = ld3(B.w, a);
+  d3 vb = mk3(0, 0, 0), wb = mk3(0, 0, 0);
+  if (!wall) {
+    vb = ld3(B.v, b);
+    wb = ld3(B.w, b);
+  }
+  const d3 ra = M.get3(F_RA, i), rb = M.get3(F_RB, i);
+  const d3 n = M.get3(F_N, i);
+  const double ima = M.at(F_IMA, i), imb = M.at(F_IMB, i);
+  d3 v_rel = wall ? -(va + cross(wa, ra)) : (vb + cross(wb, rb)) - (va + cross(wa, ra));
+  const double phi = M.at(F_PHI, i), vnpre = M.at(F_VNPRE, i);
+  const double b_n = (phi < 0.0) ? K.beta * phi / K.dt : 0.0;
+  const double b_rest = (vnpre < 0.0) ? M.at(F_E, i) * vnpre : 0.0;
+  double ln = M.lam[0][i], lt1 = M.lam[1][i], lt2 = M.lam[2][i];
+  const double w_n = dot(n, v_rel) + b_n + b_rest + K.cfm * ln;
+  const double delta_n = -(K.omega / M.at(F_GN, i)) * w_n;
+  const double old_n = ln;
+  ln = dmax(0.0, old_n + delta_n);
+  const double dn = ln - old_n;
+  if (dn != 0.0) {
+    const d3 IAn = M.get3(F_IAN, i);
+    const d3 dvA = n * (dn * ima);
+    va = va - dvA;
+    wa = wa - IAn * dn;
+    v_rel = v_rel + (dvA + cross(IAn * dn, ra));
+    if (!wall) {
+      const d3 IBn = M.get3(F_IBN, i);
+      const d3 dvB = n * (dn * imb);
+      vb = vb + dvB;
+      wb = wb + IBn * dn;
+      v_rel = v_rel + (dvB + cross(IBn * dn, rb));
+    }
+  }
+  const double max_fric = mu * ln;
+  const d3 t1 = M.get3(F_T1, i), t2 = M.get3(F_T2, i);
+  double dt1, dt2;
+  if (K.proj == 0) {
+    const double limit_t1 = sqrt(dmax(0.0, max_fric * max_fric - lt2 * lt2));
+    const double w_t1 = dot(t1, v_rel) + K.cfm * lt1;
+    const double delta_t1 = -(K.omega / M.at(F_GT1, i)) * w_t1;
+    const double old_t1 = lt1;
+    lt1 = dclamp(old_t1 + delta_t1, -limit_t1, limit_t1);
+    dt1 = lt1 - old_t1;
+    if (dt1 != 0.0) {
+      const d3 IAt1 = M.get3(F_IAT1, i);
+      const d3 dvA = t1 * (dt1 * ima);
+      va = va - dvA;
+      wa = wa - IAt1 * dt1;
+      v_rel = v_rel + (dvA + cross(IAt1 * dt1, ra));
+      if (!wall) {
+        const d3 IBt1 = M.get3(F_IBT1, i);
+        const d3 dvB = t1 * (dt1 * imb);
+        vb = vb + dvB;
+        wb = wb + IBt1 * dt1;
+        v_rel = v_rel + (dvB + cross(IBt1 * dt1, rb));
+      }
+    }
+    const double limit_t2 = sqrt(dmax(0.0, max_fric * max_fric - lt1 * lt1));
+    const double w_t2 = dot(t2, v_rel) + K.cfm * lt2;
+    const double delta_t2 = -(K.omega / M.at(F_GT2, i)) * w_t2;
+    const double old_t2 = lt2;
+    lt2 = dclamp(old_t2 + delta_t2, -limit_t2, limit_t2);
+    dt2 = lt2 - old_t2;
+  } else {
+    const double w_t1 = dot(t1, v_rel) + K.cfm * lt1;
+    const double w_t2 = dot(t2, v_rel) + K.cfm * lt2;
+    double y1 = lt1 - (K.omega / M.at(F_GT1, i)) * w_t1;
+    double y2 = lt2 - (K.omega / M.at(F_GT2, i)) * w_t2;
+    const double ny = sqrt(y1 * y1 + y2 * y2);
+    if (ny > max_fric) {
+      const double sc = ny > 0.0 ? max_fric / ny : 0.0;
+      y1 = y1 * sc;
+      y2 = y2 * sc;
+    }
+    dt1 = y1 - lt1;
+    dt2 = y2 - lt2;
+    lt1 = y1;
+    lt2 = y2;
+    if (dt1 != 0.0) {
+      va = va - t1 * (dt1 * ima);
+      wa = wa - M.get3(F_IAT1, i) * dt1;
+      if (!wall) {
+        vb = vb + t1 * (dt1 * imb);
+        wb = wb + M.get3(F_IBT1, i) * dt1;
+      }
+    }
+  }
+  if (dt2 != 0.0) {
+    va = va - t2 * (dt2 * ima);
+    wa = wa - M.get3(F_IAT2, i) * dt2;
+    if (!wall) {
+      vb = vb + t2 * (dt2 * imb);
+      wb = wb + M.get3(F_IBT2, i) * dt2;
+    }
+  }
+  M.lam[0][i] = ln;
+  M.lam[1][i] = lt1;
+  M.lam[2][i] = lt2;
+  if (dn != 0.0 || dt1 != 0.0 || dt2 != 0.0) {
+    st3(B.v, a, va);
+    st3(B.w, a, wa);
+    if (!wall) {
+      st3(B.v, b, vb);
+      st3(B.w, b, wb);
+    }
+  }
+  return fabs(dn) + fabs(dt1) + fabs(dt2);
+}
+
+__device__ __forceinline__ double body_mu(const BodyPtrs& B, const SysDev* sys, int a) { return sys[B.sys[a]].mu; }
+
+__device__ __forceinline__ void atomic_max_pos(double* addr, double v) {
+  // non-negative doubles order like their bit patterns
+  atomicMax((unsigned long long*)addr, (unsigned long long)__double_as_longlong(v));
+}
+
+__global__ void __launch_bounds__(256) k_psor_sweep(BodyPtrs B, MfPtrs M, const SysDev* __restrict__ sys, int i0,
+                                                    int i1, SolveConst K, double* __restrict__ ref_res) {
+  int i = i0 + blockIdx.x * blockDim.x + threadIdx.x;
+  double d = 0.0;
+  if (i < i1) d = psor_update(B, M, i, K, body_mu(B, sys, M.a[i]));
+  for (int off = 32; off > 0; off >>= 1) d = dmax(d, __shfl_down(d, off));
+  if (__lane_id() == 0 && d > 0.0) atomic_max_pos(ref_res, d);
+}
+
+// Fixed-point residual (read-only), velocity units.
+__device__ __forceinline__ double residual_one(const BodyPtrs& B, const MfPtrs& M, int i, const SolveConst& K,
+                                               double mu) {
+  const int a = M.a[i], b = M.b[i];
+  const bool wall = b < 0;
+  const d3 ra = M.get3(F_RA, i), rb = M.get3(F_RB, i);
+  const d3 va = ld3(B.v, a), wa = ld3(B.w, a);
+  d3 v_rel;
+  if (wall) v_rel = -(va + cross(wa, ra));
+  else v_rel = (ld3(B.v, b) + cross(ld3(B.w, b), rb)) - (va + cross(wa, ra));
+  const double phi = M.at(F_PHI, i), vnpre = M.at(F_VNPRE, i);
+  const double b_n = (phi < 0.0) ? K.beta * phi / K.dt : 0.0;
+  const double b_rest = (vnpre < 0.0) ? M.at(F_E, i) * vnpre : 0.0;
+  const double ln = M.lam[0][i], lt1 = M.lam[1][i], lt2 = M.lam[2][i];
+  const double gn = M.at(F_GN, i), g1 = M.at(F_GT1, i), g2 = M.at(F_GT2, i);
+  const double w_n = dot(M.get3(F_N, i), v_rel) + b_n + b_rest + K.cfm * ln;
+  const double pn = dmax(0.0, ln - w_n / gn);
+  double r = gn * fabs(ln - pn);
+  const double mf = mu * ln;
+  const double w_t1 = dot(M.get3(F_T1, i), v_rel) + K.cfm * lt1;
+  const double w_t2 = dot(M.get3(F_T2, i), v_rel) + K.cfm * lt2;
+  double p1, p2;
+  if (K.proj == 0) {
+    const double L1 = sqrt(dmax(0.0, mf * mf - lt2 * lt2));
+    const double L2 = sqrt(dmax(0.0, mf * mf - lt1 * lt1));
+    p1 = dclamp(lt1 - w_t1 / g1, -L1, L1);
+    p2 = dclamp(lt2 - w_t2 / g2, -L2, L2);
+  } else {
+    p1 = lt1 - w_t1 / g1;
+    p2 = lt2 - w_t2 / g2;
+    const double ny = sqrt(p1 * p1 + p2 * p2);
+    if (ny > mf) {
+      const double sc = ny > 0.0 ? mf / ny : 0.0;
+      p1 = p1 * sc;
+      p2 = p2 * sc;
+    }
+  }
+  r = dmax(r, g1 * fabs(lt1 - p1));
+  r = dmax(r, g2 * fabs(lt2 - p2));
+  return r;
+}
+
+__global__ void __launch_bounds__(256) k_residual(BodyPtrs B, MfPtrs M, const SysDev* __restrict__ sys, int nm,
+                                                  SolveConst K, double* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double r = 0.0;
+  if (i < nm) {
+    r = residual_one(B, M, i, K, body_mu(B, sys, M.a[i]));
+    if (r != r) r = __longlong_as_double(0x7ff8000000000000ll);  // keep NaN visible
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    double o = __shfl_down(r, off);
+    r = (o != o || o > r) ? o : r;
+  }
+  if (__lane_id() == 0 && (r > 0.0 || r != r)) atomic_max_pos(out, r);
+}
+
+__global__ void k_copy_keys_lambda(const uint64_t* __restrict__ keys, MfPtrs nat, int nc, uint64_t* __restrict__ pk,
+                                   double* __restrict__ p0, double* __restrict__ p1, double* __restrict__ p2) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nc) return;
+  pk[i] = keys[i];
+  p0[i] = nat.lam[0][i];
+  p1[i] = nat.lam[1][i];
+  p2[i] = nat.lam[2][i];
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+static MfPtrs mf_ptrs(rdk_ctx* ctx, bool colour_order) {
+  MfPtrs M;
+  if (colour_order) {
+    M.f = ctx->mfc.p;
+    M.stride = ctx->mfc.cap / F_COUNT;
+    M.a = ctx->mac.p;
+    M.b = ctx->mbc.p;
+    for (int k = 0; k < 3; ++k) M.lam[k] = ctx->lamc[k].p;
+  } else {
+    M.f = ctx->mf.p;
+    M.stride = ctx->mf.cap / F_COUNT;
+    M.a = ctx->ma.p;
+    M.b = ctx->mb.p;
+    for (int k = 0; k < 3; ++k) M.lam[k] = ctx->lam[k].p;
+  }
+  return M;
+}
+
+static int ensure_manifolds(rdk_ctx* ctx, int m) {
+  m = std::max(m, 1);
+  const size_t need = (size_t)F_COUNT * m;
+  if (ctx->mf.cap < need || ctx->mfc.cap < need) {
+    // both buffers must share one stride
+    size_t cap = (size_t)(m + m / 4 + 16);
+    release(ctx->mf);
+    release(ctx->mfc);
+    RDK_TRY(ensure(ctx, ctx->mf, F_COUNT * cap));
+    RDK_TRY(ensure(ctx, ctx->mfc, F_COUNT * cap));
+    // ensure() pads; force an exact multiple of F_COUNT for the stride
+    ctx->mf.cap = (ctx->mf.cap / F_COUNT) * F_COUNT;
+    ctx->mfc.cap = (ctx->mfc.cap / F_COUNT) * F_COUNT;
+    if (ctx->mfc.cap != ctx->mf.cap) ctx->mfc.cap = ctx->mf.cap = std::min(ctx->mf.cap, ctx->mfc.cap);
+  }
+  RDK_TRY(ensure(ctx, ctx->ma, m));
+  RDK_TRY(ensure(ctx, ctx->mb, m));
+  RDK_TRY(ensure(ctx, ctx->mac, m));
+  RDK_TRY(ensure(ctx, ctx->mbc, m));
+  for (int k = 0; k < 3; ++k) {
+    RDK_TRY(ensure(ctx, ctx->lam[k], m));
+    RDK_TRY(ensure(ctx, ctx->lamc[k], m));
+  }
+  RDK_TRY(ensure(ctx, ctx->color, m));
+  RDK_TRY(ensure(ctx, ctx->color_tmp, m));
+  RDK_TRY(ensure(ctx, ctx->perm, m));
+  RDK_TRY(ensure(ctx, ctx->perm_alt, m));
+  RDK_TRY(ensure(ctx, ctx->color_key, m));
+  RDK_TRY(ensure(ctx, ctx->color_key_alt, m));
+  RDK_TRY(ensure(ctx, ctx->csr_key, 2 * m));
+  RDK_TRY(ensure(ctx, ctx->csr_key_alt, 2 * m));
+  RDK_TRY(ensure(ctx, ctx->csr_val, 2 * m));
+  RDK_TRY(ensure(ctx, ctx->csr_val_alt, 2 * m));
+  return RDK_OK;
+}
+
+int nsc_build_run(rdk_ctx* ctx, const rdk_nsc_params* p) {
+  const int nc = ctx->nc, n = ctx->n;
+  const int nb = 256;
+  BodyPtrs B = body_ptrs(ctx);
+  // wall candidates
+  bool any_wall = false;
+  for (auto& s : ctx->sys_host) any_wall |= (s.floor_wall || s.ceil_wall);
+  int nwall = 0;
+  if (any_wall) {
+    RDK_TRY(ensure(ctx, ctx->wall_flag, 4 * n + 1));
+    RDK_TRY(ensure(ctx, ctx->wall_off, 4 * n + 1));
+    {
+      ProfScope ps(ctx, K_WALL);
+      hipLaunchKernelGGL(k_wall_flags, dim3(grid_for(4 * n, nb)), dim3(nb), 0, ctx->stream, B, ctx->sys_dev.p, n,
+                         ctx->wall_flag.p);
+    }
+    RDK_TRY(exclusive_scan_int(ctx, ctx->wall_flag.p, ctx->wall_off.p, 4 * n));
+    int last[2];
+    RDK_HIP(hipMemcpyAsync(&last[0], ctx->wall_off.p + 4 * n - 1, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    RDK_HIP(hipMemcpyAsync(&last[1], ctx->wall_flag.p + 4 * n - 1, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    RDK_HIP(hipStreamSynchronize(ctx->stream));
+    nwall = last[0] + last[1];
+  }
+  const int nm = nc + nwall;
+  RDK_TRY(ensure_manifolds(ctx, nm));
+  MfPtrs M = mf_ptrs(ctx, false);
+  if (nc > 0) {
+    ContactIn C;
+    C.a = ctx->ca.p;
+    C.b = ctx->cb.p;
+    for (int k = 0; k < 3; ++k) {
+      C.pa[k] = ctx->cpa[k].p;
+      C.pb[k] = ctx->cpb[k].p;
+      C.n[k] = ctx->cn[k].p;
+      C.sh[k] = ctx->csh[k].p;
+    }
+    C.dist = ctx->cdist.p;
+    C.hs = ctx->chs.p;
+    const int warm = p->warm_mode == 1 ? 1 : 0;
+    ProfScope ps(ctx, K_MANIFOLD);
+    hipLaunchKernelGGL(k_manifold, dim3(grid_for(nc, nb)), dim3(nb), 0, ctx->stream, B, C, nc, M, p->cfm, warm,
+                       ctx->prev_keys.p, ctx->nprev, ctx->prev_lam[0].p, ctx->prev_lam[1].p, ctx->prev_lam[2].p);
+  }
+  if (nwall > 0) {
+    ProfScope ps(ctx, K_WALL);
+    hipLaunchKernelGGL(k_wall_manifold, dim3(grid_for(4 * n, nb)), dim3(nb), 0, ctx->stream, B, ctx->sys_dev.p, n,
+                       ctx->wall_flag.p, ctx->wall_off.p, nc, M, p->cfm);
+  }
+  ctx->nm = nm;
+  ctx->nm_wall = nwall;
+  ctx->have_manifolds = true;
+  ctx->have_coloring = false;
+  ctx->work[W_MANIFOLDS] += nm;
+  if (p->warm_mode == 2 && nc > 0) RDK_TRY(mlp_warm_start(ctx));
+  return RDK_OK;
+}
+
+int nsc_color_run(rdk_ctx* ctx) {
+  const int nm = ctx->nm, n = ctx->n;
+  const int nb = 256;
+  ctx->ncolor = 0;
+  ctx->color_start.assign(1, 0);
+  if (nm == 0) {
+    ctx->have_coloring = true;
+    return RDK_OK;
+  }
+  RDK_TRY(ensure(ctx, ctx->body_start, n + 1));
+  RDK_TRY(ensure(ctx, ctx->uncolored, 256));
+  {
+    ProfScope ps(ctx, K_CSR);
+    hipLaunchKernelGGL(k_csr_entries, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, ctx->ma.p, ctx->mb.p, nm, n,
+                       ctx->csr_key.p, ctx->csr_val.p);
+  }
+  bool alt = false;
+  RDK_TRY(sort_pairs_u32(ctx, ctx->csr_key.p, ctx->csr_key_alt.p, ctx->csr_val.p, ctx->csr_val_alt.p, 2 * nm,
+                         bits_for((uint64_t)n), alt));
+  const uint32_t* skey = alt ? ctx->csr_key_alt.p : ctx->csr_key.p;
+  const int* csr = alt ? ctx->csr_val_alt.p : ctx->csr_val.p;
+  {
+    ProfScope ps(ctx, K_CSR);
+    hipLaunchKernelGGL(k_body_start, dim3(grid_for(n + 1, nb)), dim3(nb), 0, ctx->stream, skey, 2 * nm, n,
+                       ctx->body_start.p);
+  }
+  {
+    ProfScope ps(ctx, K_COLOR);
+    hipLaunchKernelGGL(k_fill_color, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, ctx->color.p, nm, -1);
+  }
+  unsigned* dctr = (unsigned*)ctx->uncolored.p;  // [0] left, [1] maxcol, [2] overflow
+  RDK_HIP(hipMemsetAsync(dctr, 0, 4 * sizeof(unsigned), ctx->stream));
+  int rounds = 0;
+  unsigned hc[4];
+  for (;;) {
+    RDK_HIP(hipMemsetAsync(dctr, 0, sizeof(unsigned), ctx->stream));
+    {
+      ProfScope ps(ctx, K_COLOR);
+      hipLaunchKernelGGL(k_color_assign, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, ctx->ma.p, ctx->mb.p, nm,
+                         ctx->body_start.p, csr, ctx->color.p, ctx->color_tmp.p, dctr + 2);
+      hipLaunchKernelGGL(k_color_resolve, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, ctx->ma.p, ctx->mb.p, nm,
+                         ctx->body_start.p, csr, ctx->color_tmp.p, ctx->color.p, dctr, dctr + 1);
+    }
+    ++rounds;
+    RDK_HIP(hipMemcpyAsync(hc, dctr, 4 * sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+    RDK_HIP(hipStreamSynchronize(ctx->stream));
+    if (hc[2] != 0) {
+      ctx->err = "nsc colouring: more than 127 colours needed";
+      return RDK_ERR_CAPACITY;
+    }
+    if (hc[0] == 0) break;
+    if (rounds > 4096) {
+      ctx->err = "nsc colouring did not terminate";
+      return RDK_ERR_STATE;
+    }
+  }
+  const int ncolor = (int)hc[1] + 1;
+  // group by colour (stable: natural order inside a colour)
+  RDK_TRY(ensure(ctx, ctx->wall_off, std::max<size_t>(ctx->wall_off.cap, 128)));
+  int* hist = ctx->wall_off.p;  // reuse as 128-bin histogram
+  RDK_HIP(hipMemsetAsync(hist, 0, 128 * sizeof(int), ctx->stream));
+  {
+    ProfScope ps(ctx, K_COLOR);
+    hipLaunchKernelGGL(k_color_keys, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, ctx->color.p, nm,
+                       ctx->color_key.p, ctx->perm_alt.p, hist);
+  }
+  RDK_TRY(sort_pairs_u32(ctx, ctx->color_key.p, ctx->color_key_alt.p, ctx->perm_alt.p, ctx->perm.p, nm, 7, alt));
+  if (!alt) RDK_HIP(hipMemcpyAsync(ctx->perm.p, ctx->perm_alt.p, sizeof(int) * nm, hipMemcpyDeviceToDevice, ctx->stream));
+  std::vector<int> h(128);
+  RDK_HIP(hipMemcpyAsync(h.data(), hist, 128 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  RDK_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->color_start.assign(ncolor + 1, 0);
+  for (int c = 0; c < ncolor; ++c) ctx->color_start[c + 1] = ctx->color_start[c] + h[c];
+  ctx->ncolor = ncolor;
+  // colour-ordered manifold copy
+  {
+    ProfScope ps(ctx, K_PERMUTE);
+    hipLaunchKernelGGL(k_permute, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, mf_ptrs(ctx, false),
+                       mf_ptrs(ctx, true), ctx->perm.p, nm);
+  }
+  ctx->have_coloring = true;
+  return RDK_OK;
+}
+
+static SolveConst solve_const(rdk_ctx* ctx, const rdk_nsc_params* p) {
+  SolveConst K;
+  K.omega = p->omega;
+  K.cfm = p->cfm;
+  K.beta = p->position_stabilization ? p->beta * 0.01 : p->beta;
+  K.dt = ctx->phys.dt;
+  K.proj = p->friction_proj;
+  return K;
+}
+
+static int run_sweeps(rdk_ctx* ctx, const rdk_nsc_params* p, int nsweeps) {
+  BodyPtrs B = body_ptrs(ctx);
+  MfPtrs M = mf_ptrs(ctx, true);
+  SolveConst K = solve_const(ctx, p);
+  const int nb = 256;
+  double* ref = ctx->resid.p + 1;
+  for (int s = 0; s < nsweeps; ++s) {
+    RDK_HIP(hipMemsetAsync(ref, 0, sizeof(double), ctx->stream));
+    for (int c = 0; c < ctx->ncolor; ++c) {
+      const int i0 = ctx->color_start[c], i1 = ctx->color_start[c + 1];
+      if (i1 <= i0) continue;
+      ProfScope ps(ctx, K_PSOR_SWEEP);
+      hipLaunchKernelGGL(k_psor_sweep, dim3(grid_for(i1 - i0, nb)), dim3(nb), 0, ctx->stream, B, M, ctx->sys_dev.p,
+                         i0, i1, K, ref);
+    }
+  }
+  ctx->work[W_SWEEP_CONTACTS] += (int64_t)nsweeps * ctx->nm;
+  return RDK_OK;
+}
+
+int nsc_residual_run(rdk_ctx* ctx, const rdk_nsc_params* p, double* res, double* res_ref) {
+  RDK_TRY(ensure(ctx, ctx->resid, 4));
+  const int nm = ctx->nm;
+  double h[2] = {0.0, 0.0};
+  if (nm > 0) {
+    RDK_HIP(hipMemsetAsync(ctx->resid.p, 0, sizeof(double), ctx->stream));
+    ProfScope ps(ctx, K_RESIDUAL);
+    hipLaunchKernelGGL(k_residual, dim3(grid_for(nm, 256)), dim3(256), 0, ctx->stream, body_ptrs(ctx),
+                       mf_ptrs(ctx, true), ctx->sys_dev.p, nm, solve_const(ctx, p), ctx->resid.p);
+  }
+  if (nm > 0) {
+    RDK_HIP(hipMemcpyAsync(h, ctx->resid.p, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    RDK_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  ctx->work[W_RESID_CONTACTS] += nm;
+  if (res) *res = h[0];
+  if (res_ref) *res_ref = h[1];
+  return RDK_OK;
+}
+
+int nsc_sweeps_run(rdk_ctx* ctx, const rdk_nsc_params* p, int nsweeps) {
+  if (!ctx->have_manifolds) {
+    ctx->err = "rdk_nsc_sweeps: no manifolds (call rdk_nsc_build)";
+    return RDK_ERR_STATE;
+  }
+  RDK_TRY(ensure(ctx, ctx->resid, 4));
+  if (!ctx->have_coloring) RDK_TRY(nsc_color_run(ctx));
+  RDK_TRY(run_sweeps(ctx, p, nsweeps));
+  const int nb = 256;
+  if (ctx->nm > 0) {
+    ProfScope ps(ctx, K_PERMUTE);
+    hipLaunchKernelGGL(k_unpermute_lambda, dim3(grid_for(ctx->nm, nb)), dim3(nb), 0, ctx->stream,
+                       mf_ptrs(ctx, true), mf_ptrs(ctx, false), ctx->perm.p, ctx->nm);
+  }
+  return RDK_OK;
+}
+
+int nsc_solve_run(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* st) {
+  if (!ctx->have_manifolds) {
+    ctx->err = "rdk_nsc_solve: no manifolds (call rdk_nsc_build)";
+    return RDK_ERR_STATE;
+  }
+  RDK_TRY(ensure(ctx, ctx->resid, 4));
+  const int nm = ctx->nm, nb = 256;
+  RDK_TRY(nsc_color_run(ctx));
+  BodyPtrs B = body_ptrs(ctx);
+  MfPtrs Mc = mf_ptrs(ctx, true);
+  if (p->warm_mode != 0) {
+    for (int c = 0; c < ctx->ncolor; ++c) {
+      const int i0 = ctx->color_start[c], i1 = ctx->color_start[c + 1];
+      if (i1 <= i0) continue;
+      ProfScope ps(ctx, K_WARM_APPLY);
+      hipLaunchKernelGGL(k_warm_apply, dim3(grid_for(i1 - i0, nb)), dim3(nb), 0, ctx->stream, B, Mc, i0, i1);
+    }
+  }
+  int sweeps = 0;
+  double res = 0.0, res_ref = 0.0;
+  int status = RDK_OK;
+  if (nm > 0) {
+    if (p->stop_rule == 0) {
+      for (int it = 0; it < p->max_sweeps; ++it) {
+        RDK_TRY(run_sweeps(ctx, p, 1));
+        ++sweeps;
+        RDK_TRY(nsc_residual_run(ctx, p, &res, &res_ref));
+        if (res_ref < 1e-6) break;
+      }
+    } else {
+      const int k = std::max(1, p->check_every);
+      RDK_TRY(nsc_residual_run(ctx, p, &res, &res_ref));
+      while (!(res <= p->tol) && sweeps < p->max_sweeps) {
+        RDK_TRY(run_sweeps(ctx, p, k));
+        sweeps += k;
+        RDK_TRY(nsc_residual_run(ctx, p, &res, &res_ref));
+      }
+      if (!(res <= p->tol)) status = RDK_ERR_NOCONV;
+    }
+    ProfScope ps(ctx, K_PERMUTE);
+    hipLaunchKernelGGL(k_unpermute_lambda, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, Mc, mf_ptrs(ctx, false),
+                       ctx->perm.p, nm);
+  }
+  // warm cache for the next step: rod-rod manifolds are the first nc, sorted by key
+  const int nc = ctx->nc;
+  if (p->warm_mode != 0 && nc > 0) {
+    RDK_TRY(ensure(ctx, ctx->prev_keys, nc));
+    for (int k = 0; k < 3; ++k) RDK_TRY(ensure(ctx, ctx->prev_lam[k], nc));
+    ProfScope ps(ctx, K_MISC);
+    hipLaunchKernelGGL(k_copy_keys_lambda, dim3(grid_for(nc, nb)), dim3(nb), 0, ctx->stream, ctx->pkeys.p,
+                       mf_ptrs(ctx, false), nc, ctx->prev_keys.p, ctx->prev_lam[0].p, ctx->prev_lam[1].p,
+                       ctx->prev_lam[2].p);
+    ctx->nprev = nc;
+  } else {
+    ctx->nprev = 0;
+  }
+  if (st) {
+    st->manifolds = nm;
+    st->rod_contacts = ctx->nc;
+    st->wall_contacts = ctx->nm_wall;
+    st->colors = ctx->ncolor;
+    st->sweeps = sweeps;
+    st->residual = res;
+    st->residual_ref = res_ref;
+  }
+  return status;
+}
+
+}  // namespace rdk

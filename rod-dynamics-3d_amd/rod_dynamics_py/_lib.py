@@ -1,0 +1,402 @@
+"""ctypes binding of librdk (include/rdk.h).
+
+The library is loaded from the package directory; there is no fallback: if
+librdk.so is missing or fails to load, every entry point raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "librdk.so")
+
+RDK_OK, RDK_ERR_HIP, RDK_ERR_ARG, RDK_ERR_CAPACITY, RDK_ERR_STATE, RDK_ERR_NOCONV = range(6)
+RDK_NSC_FREE, RDK_NSC_POS, RDK_VV_HALF1, RDK_VV_HALF2, RDK_CLEAR = range(5)
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class SystemDesc(C.Structure):
+    _fields_ = [("lo", C.c_double * 3), ("hi", C.c_double * 3), ("periodic", C.c_int * 3),
+                ("floor_wall", C.c_int), ("ceil_wall", C.c_int), ("mu", C.c_double)]
+
+
+class Physics(C.Structure):
+    _fields_ = [("gravity", C.c_double * 3), ("dt", C.c_double), ("lin_damp", C.c_double),
+                ("ang_damp", C.c_double), ("w_max", C.c_double)]
+
+
+class DetectParams(C.Structure):
+    _fields_ = [("delta", C.c_double), ("chunk_len", C.c_double), ("cell_scale", C.c_double)]
+
+
+class NscParams(C.Structure):
+    _fields_ = [("beta", C.c_double), ("cfm", C.c_double), ("omega", C.c_double), ("tol", C.c_double),
+                ("max_sweeps", C.c_int), ("check_every", C.c_int), ("friction_proj", C.c_int),
+                ("position_stabilization", C.c_int), ("warm_mode", C.c_int), ("stop_rule", C.c_int)]
+
+
+class SoftParams(C.Structure):
+    _fields_ = [("delta", C.c_double), ("k_scaler", C.c_double), ("damping", C.c_double), ("mu", C.c_double),
+                ("mu_static", C.c_double), ("nu", C.c_double), ("kt", C.c_double), ("vel_deadband", C.c_double),
+                ("enable_friction", C.c_int), ("friction_model", C.c_int)]
+
+
+class HmParams(C.Structure):
+    _fields_ = [("E", C.c_double), ("nu_poisson", C.c_double), ("restitution", C.c_double), ("mu_k", C.c_double),
+                ("mu_s", C.c_double), ("v_c", C.c_double), ("mu_roll", C.c_double),
+                ("enable_tangential", C.c_int), ("enable_rolling", C.c_int), ("law", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("manifolds", C.c_int), ("rod_contacts", C.c_int), ("wall_contacts", C.c_int), ("colors", C.c_int),
+                ("sweeps", C.c_int), ("residual", C.c_double), ("residual_ref", C.c_double)]
+
+
+class StepStats(C.Structure):
+    _fields_ = [("steps", C.c_int64), ("contacts_total", C.c_int64), ("manifolds_total", C.c_int64),
+                ("sweeps_total", C.c_int64), ("colors_total", C.c_int64), ("nonconverged_steps", C.c_int64),
+                ("residual_max", C.c_double), ("ms_detect", C.c_double), ("ms_build", C.c_double),
+                ("ms_color", C.c_double), ("ms_solve", C.c_double), ("ms_integrate", C.c_double)]
+
+
+# name -> (restype, argtypes); must match include/rdk.h
+_SIGS = {
+    "rdk_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "rdk_destroy": (C.c_int, [C.c_void_p]),
+    "rdk_last_error": (C.c_char_p, [C.c_void_p]),
+    "rdk_sync": (C.c_int, [C.c_void_p]),
+    "rdk_version": (C.c_int, []),
+    "rdk_set_bodies": (C.c_int, [C.c_void_p, C.c_int] + [_dp] * 8),
+    "rdk_set_systems": (C.c_int, [C.c_void_p, C.c_int, _ip, C.POINTER(SystemDesc)]),
+    "rdk_set_physics": (C.c_int, [C.c_void_p, C.POINTER(Physics)]),
+    "rdk_upload_state": (C.c_int, [C.c_void_p] + [_dp] * 4),
+    "rdk_upload_forces": (C.c_int, [C.c_void_p] + [_dp] * 2),
+    "rdk_download_state": (C.c_int, [C.c_void_p] + [_dp] * 6),
+    "rdk_detect": (C.c_int, [C.c_void_p, C.POINTER(DetectParams), _ip]),
+    "rdk_get_contacts": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip] + [_dp] * 7 + [_ip]),
+    "rdk_set_pair_capacity": (C.c_int, [C.c_void_p, C.c_int]),
+    "rdk_detect_info": (C.c_int, [C.c_void_p, _ip, _ip, C.POINTER(C.c_int64), _ip]),
+    "rdk_forces_soft": (C.c_int, [C.c_void_p, C.POINTER(SoftParams), _dp]),
+    "rdk_forces_hm": (C.c_int, [C.c_void_p, C.POINTER(HmParams), _dp]),
+    "rdk_history_size": (C.c_int, [C.c_void_p, _ip]),
+    "rdk_nsc_build": (C.c_int, [C.c_void_p, C.POINTER(NscParams), _ip]),
+    "rdk_nsc_get_manifolds": (C.c_int, [C.c_void_p, C.c_int, _dp, _ip, _dp]),
+    "rdk_nsc_set_lambda": (C.c_int, [C.c_void_p, C.c_int, _dp]),
+    "rdk_nsc_solve": (C.c_int, [C.c_void_p, C.POINTER(NscParams), C.POINTER(SolveStats)]),
+    "rdk_nsc_sweeps": (C.c_int, [C.c_void_p, C.POINTER(NscParams), C.c_int]),
+    "rdk_nsc_get_coloring": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _ip]),
+    "rdk_nsc_residual": (C.c_int, [C.c_void_p, C.POINTER(NscParams), _dp]),
+    "rdk_integrate": (C.c_int, [C.c_void_p, C.c_int]),
+    "rdk_step_nsc": (C.c_int, [C.c_void_p, C.POINTER(DetectParams), C.POINTER(NscParams), C.c_int,
+                               C.POINTER(StepStats)]),
+    "rdk_step_vv": (C.c_int, [C.c_void_p, C.POINTER(DetectParams), C.c_int, C.c_void_p, C.c_int,
+                              C.POINTER(StepStats)]),
+    "rdk_observables": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "rdk_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "rdk_set_profile_mask": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "rdk_kernel_times": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int64), _dp]),
+    "rdk_kernel_name": (C.c_char_p, [C.c_int]),
+    "rdk_num_kernels": (C.c_int, []),
+    "rdk_work_counters": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_int]),
+    "rdk_gen_rsa": (C.c_int, [C.c_int, _dp, _dp, C.POINTER(SystemDesc), C.c_double, C.c_uint64, C.c_int, _dp, _dp,
+                              _ip]),
+    "rdk_gen_thermal": (C.c_int, [C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_uint64, _dp, _dp]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load librdk.so once; raises OSError if it is absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"librdk.so not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+class RdkError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"librdk status {status}: {msg}")
+        self.status = status
+
+
+def dptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], "expected C-contiguous float64"
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"], "expected C-contiguous int32"
+    return a.ctypes.data_as(_ip)
+
+
+def system_desc(lo, hi, periodic, floor_wall=False, ceil_wall=False, mu=0.0):
+    d = SystemDesc()
+    for k in range(3):
+        d.lo[k] = float(lo[k])
+        d.hi[k] = float(hi[k])
+        d.periodic[k] = int(bool(periodic[k]))
+    d.floor_wall = int(bool(floor_wall))
+    d.ceil_wall = int(bool(ceil_wall))
+    d.mu = float(mu)
+    return d
+
+
+def nsc_params(beta=0.0, cfm=0.0, omega=1.0, tol=1e-8, max_sweeps=10000, check_every=1, friction_proj=0,
+               position_stabilization=0, warm_mode=1, stop_rule=1):
+    p = NscParams()
+    p.beta, p.cfm, p.omega, p.tol = beta, cfm, omega, tol
+    p.max_sweeps, p.check_every, p.friction_proj = max_sweeps, check_every, friction_proj
+    p.position_stabilization, p.warm_mode, p.stop_rule = position_stabilization, warm_mode, stop_rule
+    return p
+
+
+def soft_params(delta=0.005, k_scaler=1000.0, damping=0.0, mu=0.5, mu_static=0.0, nu=1e-5, kt=1000.0,
+                vel_deadband=1e-3, enable_friction=1, friction_model=0):
+    p = SoftParams()
+    p.delta, p.k_scaler, p.damping, p.mu, p.mu_static, p.nu = delta, k_scaler, damping, mu, mu_static, nu
+    p.kt, p.vel_deadband, p.enable_friction, p.friction_model = kt, vel_deadband, enable_friction, friction_model
+    return p
+
+
+def hm_params(E=1e7, nu_poisson=0.25, restitution=0.5, mu_k=0.5, mu_s=0.5, v_c=1e-3, mu_roll=0.0,
+              enable_tangential=1, enable_rolling=0, law=0):
+    p = HmParams()
+    p.E, p.nu_poisson, p.restitution, p.mu_k, p.mu_s, p.v_c, p.mu_roll = E, nu_poisson, restitution, mu_k, mu_s, v_c, mu_roll
+    p.enable_tangential, p.enable_rolling, p.law = enable_tangential, enable_rolling, law
+    return p
+
+
+def physics(gravity=(0.0, 0.0, 0.0), dt=1e-3, lin_damp=0.0, ang_damp=0.0, w_max=1e30):
+    p = Physics()
+    for k in range(3):
+        p.gravity[k] = float(gravity[k])
+    p.dt, p.lin_damp, p.ang_damp, p.w_max = dt, lin_damp, ang_damp, w_max
+    return p
+
+
+def detect_params(delta=0.0, chunk_len=0.0, cell_scale=1.0):
+    p = DetectParams()
+    p.delta, p.chunk_len, p.cell_scale = delta, chunk_len, cell_scale
+    return p
+
+
+class Context:
+    """One device context (one HIP stream) owning the device copy of a packing."""
+
+    def __init__(self, device=0):
+        L = lib()
+        h = C.c_void_p()
+        st = L.rdk_create(int(device), C.byref(h))
+        if st != RDK_OK:
+            raise RdkError(st, "rdk_create failed (no HIP device?)")
+        self._h = h
+        self._L = L
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.rdk_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st, allow=()):
+        if st != RDK_OK and st not in allow:
+            raise RdkError(st, self._L.rdk_last_error(self._h).decode())
+        return st
+
+    def set_bodies(self, x, q, v, w, r, h, mass, rest=None):
+        n = len(r)
+        arr = [np.ascontiguousarray(a, dtype=np.float64) if a is not None else None
+               for a in (x, q, v, w, r, h, mass, rest)]
+        self._check(self._L.rdk_set_bodies(self._h, n, *[dptr(a) for a in arr]))
+        self.n = n
+
+    def set_systems(self, offsets, descs):
+        off = np.ascontiguousarray(offsets, dtype=np.int32)
+        arr = (SystemDesc * len(descs))(*descs)
+        self._check(self._L.rdk_set_systems(self._h, len(descs), iptr(off), arr))
+
+    def set_physics(self, p):
+        self._check(self._L.rdk_set_physics(self._h, C.byref(p)))
+
+    def upload_state(self, x=None, q=None, v=None, w=None):
+        arr = [np.ascontiguousarray(a, dtype=np.float64) if a is not None else None for a in (x, q, v, w)]
+        self._check(self._L.rdk_upload_state(self._h, *[dptr(a) for a in arr]))
+
+    def upload_forces(self, f=None, tau=None):
+        arr = [np.ascontiguousarray(a, dtype=np.float64) if a is not None else None for a in (f, tau)]
+        self._check(self._L.rdk_upload_forces(self._h, *[dptr(a) for a in arr]))
+
+    def download_state(self):
+        n = self.n
+        x, v, w, f, t = (np.zeros((n, 3)) for _ in range(5))
+        q = np.zeros((n, 4))
+        self._check(self._L.rdk_download_state(self._h, dptr(x), dptr(q), dptr(v), dptr(w), dptr(f), dptr(t)))
+        return dict(x=x, q=q, v=v, w=w, f=f, tau=t)
+
+    def detect(self, p):
+        nc = C.c_int()
+        self._check(self._L.rdk_detect(self._h, C.byref(p), C.byref(nc)))
+        return nc.value
+
+    def contacts(self, nc):
+        a = np.zeros(nc, np.int32)
+        b = np.zeros(nc, np.int32)
+        ty = np.zeros(nc, np.int32)
+        pa, pb, nn, sh = (np.zeros((nc, 3)) for _ in range(4))
+        dist, s, t = (np.zeros(nc) for _ in range(3))
+        self._check(self._L.rdk_get_contacts(self._h, nc, iptr(a), iptr(b), dptr(pa), dptr(pb), dptr(nn),
+                                             dptr(dist), dptr(sh), dptr(s), dptr(t), iptr(ty)))
+        return dict(a=a, b=b, pa=pa, pb=pb, n=nn, dist=dist, shift=sh, s=s, t=t, type=ty)
+
+    def set_pair_capacity(self, cap):
+        self._check(self._L.rdk_set_pair_capacity(self._h, int(cap)))
+
+    def detect_info(self):
+        a, b, r = C.c_int(), C.c_int(), C.c_int()
+        c = C.c_int64()
+        self._check(self._L.rdk_detect_info(self._h, C.byref(a), C.byref(b), C.byref(c), C.byref(r)))
+        return dict(chunks=a.value, cells=b.value, candidates=c.value, regrows=r.value)
+
+    def forces_soft(self, p):
+        pe = C.c_double()
+        self._check(self._L.rdk_forces_soft(self._h, C.byref(p), C.byref(pe)))
+        return pe.value
+
+    def forces_hm(self, p):
+        pe = C.c_double()
+        self._check(self._L.rdk_forces_hm(self._h, C.byref(p), C.byref(pe)))
+        return pe.value
+
+    def history_size(self):
+        n = C.c_int()
+        self._check(self._L.rdk_history_size(self._h, C.byref(n)))
+        return n.value
+
+    def nsc_build(self, p):
+        nm = C.c_int()
+        self._check(self._L.rdk_nsc_build(self._h, C.byref(p), C.byref(nm)))
+        return nm.value
+
+    def nsc_manifolds(self, nm):
+        rec = np.zeros((nm, 41))
+        ab = np.zeros((nm, 2), np.int32)
+        lam = np.zeros((nm, 3))
+        self._check(self._L.rdk_nsc_get_manifolds(self._h, nm, dptr(rec), iptr(ab), dptr(lam)))
+        return rec, ab, lam
+
+    def nsc_set_lambda(self, lam):
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        self._check(self._L.rdk_nsc_set_lambda(self._h, lam.shape[0], dptr(lam)))
+
+    def nsc_solve(self, p, allow_noconv=False):
+        st = SolveStats()
+        self._check(self._L.rdk_nsc_solve(self._h, C.byref(p), C.byref(st)),
+                    allow=(RDK_ERR_NOCONV,) if allow_noconv else ())
+        return {k: getattr(st, k) for k, _ in SolveStats._fields_}
+
+    def nsc_sweeps(self, p, n):
+        self._check(self._L.rdk_nsc_sweeps(self._h, C.byref(p), int(n)))
+
+    def nsc_coloring(self, nm):
+        col = np.zeros(nm, np.int32)
+        order = np.zeros(nm, np.int32)
+        nc = C.c_int()
+        self._check(self._L.rdk_nsc_get_coloring(self._h, nm, iptr(col), iptr(order), C.byref(nc)))
+        return col, order, nc.value
+
+    def nsc_residual(self, p):
+        r = C.c_double()
+        self._check(self._L.rdk_nsc_residual(self._h, C.byref(p), C.byref(r)))
+        return r.value
+
+    def integrate(self, stage):
+        self._check(self._L.rdk_integrate(self._h, int(stage)))
+
+    def step_nsc(self, dp, np_, nsteps):
+        st = StepStats()
+        self._check(self._L.rdk_step_nsc(self._h, C.byref(dp), C.byref(np_), int(nsteps), C.byref(st)))
+        return {k: getattr(st, k) for k, _ in StepStats._fields_}
+
+    def step_vv(self, dp, model, params, nsteps):
+        st = StepStats()
+        self._check(self._L.rdk_step_vv(self._h, C.byref(dp), int(model), C.cast(C.byref(params), C.c_void_p),
+                                        int(nsteps), C.byref(st)))
+        return {k: getattr(st, k) for k, _ in StepStats._fields_}
+
+    def observables(self):
+        ke, phi = C.c_double(), C.c_double()
+        self._check(self._L.rdk_observables(self._h, C.byref(ke), C.byref(phi)))
+        return ke.value, phi.value
+
+    def sync(self):
+        self._check(self._L.rdk_sync(self._h))
+
+    def set_profiling(self, on):
+        self._check(self._L.rdk_set_profiling(self._h, int(bool(on))))
+
+    def set_profile_mask(self, mask):
+        self._check(self._L.rdk_set_profile_mask(self._h, C.c_uint64(mask)))
+
+    def kernel_times(self):
+        out = {}
+        for k in range(self._L.rdk_num_kernels()):
+            n = C.c_int64()
+            ms = C.c_double()
+            self._check(self._L.rdk_kernel_times(self._h, k, C.byref(n), C.byref(ms)))
+            out[self._L.rdk_kernel_name(k).decode()] = (n.value, ms.value)
+        return out
+
+    def work_counters(self):
+        arr = (C.c_int64 * 16)()
+        self._check(self._L.rdk_work_counters(self._h, arr, 16))
+        names = ["steps", "body_passes", "chunks", "candidates", "contacts", "manifolds", "sweep_contacts",
+                 "resid_contacts", "integrate_bodies", "force_contacts"]
+        return {nm: arr[i] for i, nm in enumerate(names)}
+
+
+def gen_rsa(r, h, desc, gap=0.0, seed=0, max_attempts=2000):
+    r = np.ascontiguousarray(r, np.float64)
+    h = np.ascontiguousarray(h, np.float64)
+    n = len(r)
+    x = np.zeros((n, 3))
+    q = np.zeros((n, 4))
+    placed = C.c_int()
+    st = lib().rdk_gen_rsa(n, dptr(r), dptr(h), C.byref(desc), float(gap), C.c_uint64(seed), int(max_attempts),
+                           dptr(x), dptr(q), C.byref(placed))
+    if st != RDK_OK:
+        raise RdkError(st, "rdk_gen_rsa failed")
+    return x, q, placed.value
+
+
+def gen_thermal(q, r, h, mass, kT, seed):
+    q = np.ascontiguousarray(q, np.float64)
+    n = len(r)
+    v = np.zeros((n, 3))
+    w = np.zeros((n, 3))
+    st = lib().rdk_gen_thermal(n, dptr(q), dptr(np.ascontiguousarray(r, np.float64)),
+                               dptr(np.ascontiguousarray(h, np.float64)), dptr(np.ascontiguousarray(mass, np.float64)),
+                               float(kT), C.c_uint64(seed), dptr(v), dptr(w))
+    if st != RDK_OK:
+        raise RdkError(st, "rdk_gen_thermal failed")
+    return v, w

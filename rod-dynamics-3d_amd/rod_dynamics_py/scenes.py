@@ -1,0 +1,108 @@
+"""Synthetic packings for the benchmark configurations.
+
+Each builder returns a ``Scene``: body arrays (interleaved per body, the
+layout of include/rdk.h), the boxes, the physics constants and the contact
+model parameters.  Seeds default to ``0x5EED0000 + config id``.
+
+  C1  soft/HM thermal gas: N=1000, d=1, L=10, periodic cube at phi=0.10
+  C2  hard-contact PSOR settling: N=1e5, L/d=20, periodic x,y, floor and
+      ceiling walls, phi=0.05, gravity -z, mu=0.5, e=0, omega=1.3, dt=1e-3
+  C5  learned-warm-start bed: as C2 with N=2e5 (settled by running C2)
+
+Masses follow the reference's capsule factory (density * pi r^2 * 2h, the
+cylinder part only; rigid_body.cpp:19-24) unless a config asks for unit mass.
+"""
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+
+SEED0 = 0x5EED0000
+
+
+def capsule_volume(r, h):
+    return math.pi * r * r * (2.0 * h) + 4.0 / 3.0 * math.pi * r ** 3
+
+
+@dataclass
+class Scene:
+    name: str
+    x: np.ndarray
+    q: np.ndarray
+    v: np.ndarray
+    w: np.ndarray
+    r: np.ndarray
+    h: np.ndarray
+    mass: np.ndarray
+    rest: np.ndarray
+    offsets: np.ndarray
+    descs: list
+    phys: object
+    model: str  # "nsc" | "soft" | "hm"
+    params: object = None
+    detect: object = None
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def n(self):
+        return len(self.r)
+
+    def load(self, ctx):
+        ctx.set_bodies(self.x, self.q, self.v, self.w, self.r, self.h, self.mass, self.rest)
+        ctx.set_systems(self.offsets, self.descs)
+        ctx.set_physics(self.phys)
+
+    def box(self, s=0):
+        d = self.descs[s]
+        return (np.array(d.lo[:]), np.array(d.hi[:]), [int(p) for p in d.periodic],
+                int(d.floor_wall), int(d.ceil_wall))
+
+
+def box_side(n, r, h, phi):
+    return (n * capsule_volume(r, h) / phi) ** (1.0 / 3.0)
+
+
+def c2_settling(n=100_000, phi=0.05, seed=SEED0 + 2, mu=0.5, aspect=20.0, gap=0.0):
+    """Hard-contact PSOR settling, scaled to n rods."""
+    r0, h0 = 0.5, 0.5 * aspect
+    side = box_side(n, r0, h0, phi)
+    desc = L.system_desc((0, 0, 0), (side, side, side), (1, 1, 0), floor_wall=True, ceil_wall=True, mu=mu)
+    r = np.full(n, r0)
+    h = np.full(n, h0)
+    x, q, placed = L.gen_rsa(r, h, desc, gap=gap, seed=seed, max_attempts=20000)
+    if placed != n:
+        raise RuntimeError(f"RSA placed {placed} of {n} rods")
+    mass = 1.0 * math.pi * r * r * (2.0 * h)
+    nscp = L.nsc_params(beta=0.0, cfm=0.0, omega=1.3, tol=1e-8, max_sweeps=10000, check_every=4,
+                        friction_proj=0, position_stabilization=0, warm_mode=1, stop_rule=1)
+    return Scene(name=f"C2_settling_N{n}", x=x, q=q, v=np.zeros((n, 3)), w=np.zeros((n, 3)), r=r, h=h, mass=mass,
+                 rest=np.zeros(n), offsets=np.array([0, n], np.int32), descs=[desc],
+                 phys=L.physics((0.0, 0.0, -1.0), 1e-3, 0.08, 0.12, 80.0), model="nsc", params=nscp,
+                 detect=L.detect_params(0.0, 0.0, 1.0), extra=dict(phi=phi, side=side, mu=mu, seed=seed))
+
+
+def c1_thermal(n=1000, phi=0.10, seed=SEED0 + 1, model="hm", aspect=10.0, kT=1.0):
+    """Soft / Hertz-Mindlin thermal gas, unit mass."""
+    r0, h0 = 0.5, 0.5 * aspect
+    side = box_side(n, r0, h0, phi)
+    desc = L.system_desc((0, 0, 0), (side, side, side), (1, 1, 1), mu=0.5)
+    r = np.full(n, r0)
+    h = np.full(n, h0)
+    x, q, placed = L.gen_rsa(r, h, desc, gap=0.0, seed=seed, max_attempts=20000)
+    if placed != n:
+        raise RuntimeError(f"RSA placed {placed} of {n} rods")
+    mass = np.ones(n)
+    v, w = L.gen_thermal(q, r, h, mass, kT, seed + 17)
+    if model == "hm":
+        params = L.hm_params(E=1e7, nu_poisson=0.25, restitution=0.5, mu_k=0.5, mu_s=0.5, v_c=1e-3, mu_roll=0.0,
+                             enable_tangential=1, enable_rolling=0, law=0)
+        det = L.detect_params(0.0, 0.0, 1.0)
+    else:
+        params = L.soft_params(delta=0.005, k_scaler=1e4, damping=0.0, mu=0.5, mu_static=0.0, nu=1e-3,
+                               enable_friction=1, friction_model=0)
+        det = L.detect_params(0.005, 0.0, 1.0)
+    return Scene(name=f"C1_{model}_N{n}", x=x, q=q, v=v, w=w, r=r, h=h, mass=mass, rest=np.full(n, 0.5),
+                 offsets=np.array([0, n], np.int32), descs=[desc], phys=L.physics((0, 0, 0), 1e-4, 0.0, 0.0, 1e30),
+                 model=model, params=params, detect=det, extra=dict(phi=phi, side=side, seed=seed))

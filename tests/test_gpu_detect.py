@@ -1,0 +1,103 @@
+"""GPU detection parity: the librdk contact list must equal the oracle's
+bit for bit (pairs, order, and every FP64 field) on the same FP64 state."""
+import numpy as np
+import pytest
+
+import oracle_py as O
+from rod_dynamics_py import _lib as L
+from rod_dynamics_py import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def load(ctx, x, q, r, h, mass, lo, hi, periodic, fw=0, cw=0):
+    n = len(r)
+    ctx.set_bodies(x, q, np.zeros((n, 3)), np.zeros((n, 3)), r, h, mass, np.zeros(n))
+    ctx.set_systems([0, n], [L.system_desc(lo, hi, periodic, fw, cw, 0.5)])
+    ctx.set_physics(L.physics())
+
+
+def compare(gc, oc, nc):
+    assert np.array_equal(gc["a"], oc["a"]) and np.array_equal(gc["b"], oc["b"])
+    for k in ("pa", "pb", "n", "dist", "shift", "s", "t", "type"):
+        assert np.array_equal(gc[k], oc[k]), k
+
+
+def rods(n, side, seed, rlo=0.2, rhi=0.5, hlo=0.5, hhi=3.0):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0, side, size=(n, 3))
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    return x, q, rng.uniform(rlo, rhi, n), rng.uniform(hlo, hhi, n)
+
+
+@pytest.mark.parametrize("periodic", [(1, 1, 1), (0, 0, 0), (1, 1, 0)])
+@pytest.mark.parametrize("chunk_len", [0.0, 1.0, 2.5])
+def test_detect_bit_identical_polydisperse(gpu_ctx_factory, periodic, chunk_len):
+    n, side = 3000, 30.0
+    x, q, r, h = rods(n, side, 5)
+    print(f"detect polydisperse periodic={periodic} chunk_len={chunk_len}", flush=True)
+    ctx = gpu_ctx_factory()
+    load(ctx, x, q, r, h, np.ones(n), [0] * 3, [side] * 3, periodic)
+    W = O.World(x, q, None, None, r, h, np.ones(n))
+    W.set_box([0] * 3, [side] * 3, periodic)
+    for delta in (0.0, 0.05):
+        nc = ctx.detect(L.detect_params(delta, chunk_len, 1.0))
+        no = W.detect(0, delta)
+        assert nc == no and nc > 100
+        compare(ctx.contacts(nc), W.contacts(no), nc)
+
+
+def test_detect_positions_outside_box_and_static_pairs(gpu_ctx_factory):
+    n, side = 1500, 20.0
+    x, q, r, h = rods(n, side, 9)
+    x[:50] += side * 1.3  # unwrapped positions on periodic axes / outside a walled axis
+    mass = np.ones(n)
+    mass[::7] = 0.0  # static bodies: static-static pairs are skipped
+    for periodic in [(1, 1, 1), (1, 1, 0)]:
+        ctx = gpu_ctx_factory()
+        load(ctx, x, q, r, h, mass, [0] * 3, [side] * 3, periodic)
+        W = O.World(x, q, None, None, r, h, mass)
+        W.set_box([0] * 3, [side] * 3, periodic)
+        nc = ctx.detect(L.detect_params(0.02, 1.5, 1.0))
+        no = W.detect(0, 0.02)
+        assert nc == no
+        compare(ctx.contacts(nc), W.contacts(no), nc)
+
+
+def test_detect_c2_like_and_c1_like(gpu_ctx_factory):
+    sc = scenes.c2_settling(n=4000)
+    ctx = gpu_ctx_factory()
+    sc.load(ctx)
+    W = O.world_from_scene(sc)
+    # the RSA start has no overlaps; a contact margin of 1.0 gives many near pairs
+    for delta, chunk in [(1.0, 0.0), (1.0, 4.0), (0.0, 0.0)]:
+        nc = ctx.detect(L.detect_params(delta, chunk, 1.0))
+        no = W.detect(0, delta)
+        assert nc == no
+        if nc:
+            compare(ctx.contacts(nc), W.contacts(no), nc)
+    sc1 = scenes.c1_thermal(n=1000)
+    ctx1 = gpu_ctx_factory()
+    sc1.load(ctx1)
+    W1 = O.world_from_scene(sc1)
+    nc = ctx1.detect(L.detect_params(0.5, 2.0, 1.0))
+    no = W1.detect(1, 0.5)
+    assert nc == no and nc > 500
+    compare(ctx1.contacts(nc), W1.contacts(no), nc)
+
+
+def test_detect_capacity_regrow(gpu_ctx_factory):
+    n, side = 2000, 18.0
+    x, q, r, h = rods(n, side, 13)
+    ctx = gpu_ctx_factory()
+    load(ctx, x, q, r, h, np.ones(n), [0] * 3, [side] * 3, (1, 1, 1))
+    ctx.set_pair_capacity(16)  # force the overflow path from the host
+    nc = ctx.detect(L.detect_params(0.05, 0.0, 1.0))
+    info = ctx.detect_info()
+    assert info["regrows"] >= 1
+    W = O.World(x, q, None, None, r, h, np.ones(n))
+    W.set_box([0] * 3, [side] * 3, (1, 1, 1))
+    no = W.detect(0, 0.05)
+    assert nc == no and nc > 16
+    compare(ctx.contacts(nc), W.contacts(no), nc)

@@ -1,0 +1,173 @@
+"""GPU NSC parity: manifolds against the oracle, colouring validity, a
+coloured sweep equal bit for bit to the serial reference sweep taken in the
+same (colour) order, and the converged solve checked by the oracle's own
+residual."""
+import numpy as np
+import pytest
+
+import oracle_py as O
+from rod_dynamics_py import _lib as L
+
+pytestmark = pytest.mark.gpu
+
+
+def overlapping_bed(n=2000, side=22.0, seed=21, walls=True, mu=0.5, rest=0.3):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0, side, size=(n, 3))
+    if walls:
+        x[: n // 10, 2] = rng.uniform(0.0, 1.0, n // 10)  # some rods through the floor
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    r = rng.uniform(0.3, 0.5, n)
+    h = rng.uniform(0.5, 2.5, n)
+    mass = rng.uniform(0.5, 2.0, n)
+    v = rng.normal(size=(n, 3))
+    w = rng.normal(size=(n, 3))
+    periodic = (1, 1, 0) if walls else (1, 1, 1)
+    return dict(x=x, q=q, v=v, w=w, r=r, h=h, mass=mass, rest=np.full(n, rest), side=side, periodic=periodic,
+                walls=walls, mu=mu)
+
+
+def gpu_load(ctx, B, dt=1e-3):
+    ctx.set_bodies(B["x"], B["q"], B["v"], B["w"], B["r"], B["h"], B["mass"], B["rest"])
+    ctx.set_systems([0, len(B["r"])], [L.system_desc([0] * 3, [B["side"]] * 3, B["periodic"], B["walls"], B["walls"],
+                                                     B["mu"])])
+    ctx.set_physics(L.physics((0, 0, -1), dt, 0, 0, 1e30))
+
+
+def oracle_load(B, dt=1e-3):
+    W = O.World(B["x"], B["q"], B["v"], B["w"], B["r"], B["h"], B["mass"], B["rest"])
+    W.set_box([0] * 3, [B["side"]] * 3, B["periodic"], B["walls"], B["walls"])
+    W.set_physics((0, 0, -1), dt)
+    return W
+
+
+def test_manifolds_match_oracle(gpu_ctx_factory):
+    B = overlapping_bed()
+    print("manifolds n=2000 walls", flush=True)
+    ctx = gpu_ctx_factory()
+    gpu_load(ctx, B)
+    p = L.nsc_params(beta=0.2, cfm=1e-3, omega=1.0, warm_mode=0)
+    nc = ctx.detect(L.detect_params(0.0, 0.0, 1.0))
+    nm = ctx.nsc_build(p)
+    W = oracle_load(B)
+    assert W.detect(0, 0.0) == nc
+    op = O.nsc_params(mu=B["mu"], beta=0.2, cfm=1e-3, warm_start=0)
+    assert W.nsc_build(op) == nm
+    rec_g, ab_g, lam_g = ctx.nsc_manifolds(nm)
+    rec_o, ab_o, lam_o = W.manifolds()
+    assert nm > nc > 100  # walls present
+    assert np.array_equal(ab_g, ab_o)
+    # geometry fields are computed with identical operation order: bitwise
+    assert np.array_equal(rec_g[:, 0:15], rec_o[:, 0:15])
+    assert np.array_equal(rec_g[:, 38], rec_o[:, 38])
+    # inertia-weighted fields: closed-form vs matrix inertia, FP64 rounding only
+    scale = np.abs(rec_o).max(axis=0) + 1e-300
+    err = np.abs(rec_g - rec_o) / scale
+    assert err.max() < 1e-13, err.max(axis=0)
+
+
+def test_coloring_is_proper(gpu_ctx_factory):
+    B = overlapping_bed(seed=5)
+    ctx = gpu_ctx_factory()
+    gpu_load(ctx, B)
+    p = L.nsc_params(warm_mode=0)
+    ctx.detect(L.detect_params())
+    nm = ctx.nsc_build(p)
+    col, order, ncol = ctx.nsc_coloring(nm)
+    _, ab, _ = ctx.nsc_manifolds(nm)
+    assert sorted(order.tolist()) == list(range(nm))
+    assert np.all(np.diff(col[order]) >= 0)
+    for c in range(ncol):
+        idx = np.nonzero(col == c)[0]
+        bodies = np.concatenate([ab[idx, 0], ab[idx, 1][ab[idx, 1] >= 0]])
+        assert len(np.unique(bodies)) == len(bodies), f"colour {c} shares a body"
+    # within a colour the natural order is kept
+    for c in range(ncol):
+        seg = order[col[order] == c]
+        assert np.all(np.diff(seg) > 0)
+    # degree bound: colours <= 2 * max body degree
+    deg = np.bincount(np.concatenate([ab[:, 0], ab[:, 1][ab[:, 1] >= 0]]))
+    assert ncol <= 2 * deg.max()
+
+
+@pytest.mark.parametrize("proj", [0, 1])
+def test_colored_sweeps_bitwise_equal_serial_sweeps_in_color_order(gpu_ctx_factory, proj):
+    B = overlapping_bed(seed=8)
+    ctx = gpu_ctx_factory()
+    gpu_load(ctx, B)
+    p = L.nsc_params(beta=0.2, cfm=1e-4, omega=1.3, friction_proj=proj, warm_mode=0)
+    ctx.detect(L.detect_params())
+    nm = ctx.nsc_build(p)
+    rec, ab, lam0 = ctx.nsc_manifolds(nm)
+    col, order, ncol = ctx.nsc_coloring(nm)
+    s0 = ctx.download_state()
+    ctx.nsc_sweeps(p, 7)
+    _, _, lam_g = ctx.nsc_manifolds(nm)
+    s1 = ctx.download_state()
+    W = oracle_load(B)
+    W.set_state(v=s0["v"], w=s0["w"])
+    W.set_manifolds(rec, ab, lam0)
+    op = O.nsc_params(mu=B["mu"], beta=0.2, cfm=1e-4, omega=1.3, friction_proj=proj)
+    W.sweeps(op, 7, order)
+    _, _, lam_o = W.manifolds()
+    so = W.state()
+    assert np.array_equal(lam_g, lam_o)
+    assert np.array_equal(s1["v"], so["v"]) and np.array_equal(s1["w"], so["w"])
+    assert np.abs(lam_g).max() > 0
+
+
+@pytest.mark.parametrize("proj", [0, 1])
+def test_solve_reaches_residual_checked_by_oracle(gpu_ctx_factory, proj):
+    B = overlapping_bed(seed=31, mu=0.4)
+    ctx = gpu_ctx_factory()
+    gpu_load(ctx, B)
+    p = L.nsc_params(beta=0.1, cfm=0.0, omega=1.3, tol=1e-8, max_sweeps=20000, check_every=4, friction_proj=proj,
+                     warm_mode=0)
+    ctx.detect(L.detect_params())
+    nm = ctx.nsc_build(p)
+    rec, ab, _ = ctx.nsc_manifolds(nm)
+    s0 = ctx.download_state()
+    st = ctx.nsc_solve(p)
+    assert st["residual"] <= 1e-8 and st["sweeps"] > 0
+    _, _, lam = ctx.nsc_manifolds(nm)
+    s1 = ctx.download_state()
+    op = O.nsc_params(mu=0.4, beta=0.1, cfm=0.0, omega=1.3, friction_proj=proj)
+    W = oracle_load(B)
+    # velocities after the solve are the pre-solve velocities plus M^-1 J^T lambda
+    W.set_state(v=s0["v"], w=s0["w"])
+    W.set_manifolds(rec, ab, lam)
+    W.apply_warm()
+    so = W.state()
+    assert np.allclose(so["v"], s1["v"], rtol=0, atol=1e-11)
+    assert np.allclose(so["w"], s1["w"], rtol=0, atol=1e-11)
+    # the oracle's residual of the GPU solution
+    W.set_state(v=s1["v"], w=s1["w"])
+    assert W.residual(op) <= 1e-8 * (1 + 1e-6)
+    # admissibility: lambda_n >= 0 and the friction bound
+    assert lam[:, 0].min() >= 0.0
+    tang = np.hypot(lam[:, 1], lam[:, 2]) if proj == 1 else np.maximum(np.abs(lam[:, 1]), np.abs(lam[:, 2]))
+    assert np.all(tang <= 0.4 * lam[:, 0] * (1 + 1e-12) + 1e-15)
+
+
+def test_frictionless_unique_solution_matches_serial_oracle(gpu_ctx_factory):
+    # mu = 0 with positive-definite Delassus: the LCP solution is unique, so the
+    # coloured and the serial sorted-order Gauss-Seidel must agree
+    B = overlapping_bed(n=600, side=16.0, seed=44, walls=False, mu=0.0)
+    ctx = gpu_ctx_factory()
+    gpu_load(ctx, B)
+    p = L.nsc_params(beta=0.0, cfm=1e-3, omega=1.0, tol=1e-12, max_sweeps=100000, check_every=8, warm_mode=0)
+    ctx.detect(L.detect_params())
+    nm = ctx.nsc_build(p)
+    assert nm > 50
+    st = ctx.nsc_solve(p)
+    assert st["residual"] <= 1e-12
+    _, _, lam_g = ctx.nsc_manifolds(nm)
+    W = oracle_load(B)
+    W.detect(0, 0.0)
+    op = O.nsc_params(mu=0.0, beta=0.0, cfm=1e-3, omega=1.0, tol=1e-12, max_sweeps=100000, warm_start=0)
+    W.nsc_build(op)
+    r, sw = W.solve(op)
+    assert r <= 1e-12
+    _, _, lam_o = W.manifolds()
+    assert np.allclose(lam_g, lam_o, rtol=1e-7, atol=1e-10)

@@ -1,0 +1,167 @@
+"""Pins of the oracle's geometry and detection (CPU only).
+
+Reference tests followed: smoke_test.py:105-126 (quaternion convention),
+smoke_test.py:129-162,207-233 (contacts through a periodic boundary),
+tests/dev_utils/debug_segseg.cpp:82-98 (perpendicular crossing).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+
+
+def quat_axis_angle(axis, ang):
+    axis = np.asarray(axis, float) / np.linalg.norm(axis)
+    return np.array([math.cos(ang / 2), *(axis * math.sin(ang / 2))])
+
+
+def rot_matrix(q):
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def test_axis_convention_matches_reference_smoke():
+    c = math.sqrt(0.5)
+    # 90 degrees about x maps the rod's local +Y onto +Z (smoke_test.py:121-126)
+    u = O.axis([c, c, 0, 0])
+    assert np.allclose(u, [0, 0, 1], atol=1e-15)
+    assert np.allclose(O.axis([1, 0, 0, 0]), [0, 1, 0])
+    rng = np.random.default_rng(1)
+    for _ in range(50):
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        # axis = R e_y with the textbook rotation matrix of a unit quaternion
+        assert np.allclose(O.axis(q), rot_matrix(q) @ [0, 1, 0], atol=1e-14)
+
+
+def seg_dist(a1, a2, b1, b2):
+    s, t = O.segseg(a1, a2, b1, b2)
+    pa = a1 + s * (a2 - a1)
+    pb = b1 + t * (b2 - b1)
+    return np.linalg.norm(pb - pa), s, t
+
+
+def test_segseg_perpendicular_crossing():
+    # debug_segseg.cpp:82-98: x-segment and a y-segment through it
+    a1, a2 = np.array([-0.25, 0, 0]), np.array([0.25, 0, 0])
+    b1, b2 = np.array([0, -0.1, 0]), np.array([0, 0.4, 0])
+    d, s, t = seg_dist(a1, a2, b1, b2)
+    assert d == pytest.approx(0.0, abs=1e-15)
+    assert s == pytest.approx(0.5)
+    assert t == pytest.approx(0.2)
+
+
+def test_segseg_parallel_and_endpoint_cases():
+    # exactly parallel: the den == 0 branch; distance is the line separation
+    a1, a2 = np.array([0.0, 0, 0]), np.array([1.0, 0, 0])
+    b1, b2 = np.array([0.5, 0.3, 0]), np.array([2.0, 0.3, 0])
+    d, s, t = seg_dist(a1, a2, b1, b2)
+    assert d == pytest.approx(0.3, abs=1e-15)
+    # collinear disjoint: endpoint to endpoint
+    b1, b2 = np.array([1.5, 0, 0]), np.array([3.0, 0, 0])
+    d, s, t = seg_dist(a1, a2, b1, b2)
+    assert d == pytest.approx(0.5, abs=1e-15)
+    assert (s, t) == (1.0, 0.0)
+    # skew, closest points at both ends
+    b1, b2 = np.array([2.0, 1.0, 0.5]), np.array([3.0, 2.0, 0.5])
+    d, s, t = seg_dist(a1, a2, b1, b2)
+    assert d == pytest.approx(math.sqrt(1 + 1 + 0.25), abs=1e-14)
+
+
+def test_segseg_against_brute_force_minimum():
+    rng = np.random.default_rng(7)
+    grid = np.linspace(0, 1, 201)
+    for _ in range(150):
+        a1, a2, b1, b2 = rng.normal(size=(4, 3))
+        if rng.random() < 0.2:  # near-parallel cases
+            b2 = b1 + (a2 - a1) * rng.uniform(0.5, 2) + rng.normal(scale=1e-3, size=3)
+        d, s, t = seg_dist(a1, a2, b1, b2)
+        P = a1[None, :] + grid[:, None] * (a2 - a1)[None, :]
+        Q = b1[None, :] + grid[:, None] * (b2 - b1)[None, :]
+        D = np.linalg.norm(P[:, None, :] - Q[None, :, :], axis=2)
+        dmin = D.min()
+        # the analytic minimum is never above the sampled one, and within grid resolution of it
+        assert d <= dmin + 1e-12
+        step = max(np.linalg.norm(a2 - a1), np.linalg.norm(b2 - b1)) / 200
+        assert d >= dmin - step
+
+
+def random_rods(n, side, r, h, seed, periodic=(1, 1, 1)):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0, side, size=(n, 3))
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    rr = np.full(n, r) if np.isscalar(r) else r
+    hh = np.full(n, h) if np.isscalar(h) else h
+    W = O.World(x, q, None, None, rr, hh, np.ones(n))
+    W.set_box([0, 0, 0], [side] * 3, periodic)
+    return W, x, q, rr, hh
+
+
+def brute_pairs(x, q, r, h, side, periodic, delta):
+    """Independent numpy restatement: all pairs, centre minimum image, segment distance."""
+    n = len(r)
+    out = set()
+    u = np.array([O.axis(qq) for qq in q])
+    for i in range(n):
+        for j in range(i + 1, n):
+            d = x[j] - x[i]
+            sh = np.where(np.array(periodic) > 0, -np.floor(d / side + 0.5) * side, 0.0)
+            a1, a2 = x[i] - u[i] * h[i], x[i] + u[i] * h[i]
+            b1, b2 = x[j] - u[j] * h[j] + sh, x[j] + u[j] * h[j] + sh
+            dist, _, _ = seg_dist(a1, a2, b1, b2)
+            if dist < r[i] + r[j] + delta:
+                out.add((i, j))
+    return out
+
+
+@pytest.mark.parametrize("periodic", [(1, 1, 1), (0, 0, 0), (1, 1, 0)])
+def test_naive_and_hash_detection_match_brute_force(periodic):
+    n, side = 120, 12.0
+    rng = np.random.default_rng(3)
+    r = rng.uniform(0.2, 0.5, n)
+    h = rng.uniform(0.5, 2.0, n)
+    W, x, q, r, h = random_rods(n, side, r, h, 11, periodic)
+    ref = brute_pairs(x, q, r, h, side, periodic, 0.05)
+    for method, cs in [(0, -1.0), (1, -1.0), (1, 1.3), (1, 0.77)]:
+        nc = W.detect(method, 0.05, cs)
+        c = W.contacts(nc)
+        got = set(zip(c["a"].tolist(), c["b"].tolist()))
+        assert got == ref, (method, cs)
+        # sorted by (a,b), unique
+        keys = c["a"].astype(np.int64) * n + c["b"]
+        assert np.all(np.diff(keys) > 0)
+    assert len(ref) > 20
+
+
+def test_pbc_contact_through_boundary_spatial_hash():
+    # smoke_test.py:207-233: two capsules overlapping only across x = +-0.5
+    x = np.array([[0.46, 0, 0], [-0.46, 0, 0]])
+    q = np.array([[1.0, 0, 0, 0]] * 2)
+    W = O.World(x, q, None, None, np.array([0.05, 0.05]), np.array([0.2, 0.2]), np.array([1.0, 1.0]))
+    W.set_box([-0.5] * 3, [0.5] * 3, [1, 1, 1])
+    for method, cs in [(0, -1), (1, 0.17)]:
+        assert W.detect(method, 0.005, cs) == 1
+        c = W.contacts(1)
+        assert c["shift"][0][0] == pytest.approx(1.0)
+        assert c["dist"][0] == pytest.approx(0.08, abs=1e-12)
+    W.set_box([-0.5] * 3, [0.5] * 3, [0, 0, 0])
+    assert W.detect(0, 0.005) == 0
+
+
+def test_contact_record_fields():
+    # two parallel rods side by side along y, overlapping by 0.1
+    x = np.array([[0.0, 0, 0], [0.9, 0, 0]])
+    q = np.array([[1.0, 0, 0, 0]] * 2)
+    W = O.World(x, q, None, None, np.array([0.5, 0.5]), np.array([1.0, 1.0]), np.array([1.0, 1.0]))
+    W.set_box([-10] * 3, [10] * 3, [0, 0, 0])
+    assert W.detect(0, 0.0) == 1
+    c = W.contacts(1)
+    assert c["dist"][0] == pytest.approx(0.9)
+    assert np.allclose(c["n"][0], [1, 0, 0])
+    # parallel branch: s = 0, t from the projection -> both at the start: point-to-point
+    assert c["type"][0] == 0
