@@ -53,7 +53,7 @@ def cpu_baseline(sc, nsteps):
 
     W = O.world_from_scene(sc)
     p = O.nsc_params(mu=sc.extra["mu"], beta=0.0, cfm=0.0, omega=1.3, tol=1e-8, max_sweeps=10000, check_every=1,
-                     friction_proj=0, stop_rule=1, warm_start=1)
+                     friction_proj=sc.params.friction_proj, stop_rule=1, warm_start=1)
     W.nsc_step(p, 1, method=1)  # warm-up step (allocations)
     t0 = time.perf_counter()
     sweeps, rmax = W.nsc_step(p, nsteps, method=1)
@@ -123,6 +123,7 @@ def main():
         "config": {"workload": "hard-contact PSOR settling (C2)", "rods": sc.n, "L_over_d": 20, "phi0": 0.05,
                    "box": round(sc.extra["side"], 3), "periodic": "x,y", "walls": "floor+ceiling", "mu": 0.5,
                    "restitution": 0.0, "omega": 1.3, "residual_tol": 1e-8, "dt": 1e-3,
+                   "friction_projection": "euclidean disc" if p.friction_proj == 1 else "reference sequential",
                    "steps_total": a.warmup + a.steps, "seed": sc.extra["seed"], "check_every": a.check_every},
         "psor": {"sweeps_per_step": st["sweeps_total"] / max(st["steps"], 1),
                  "manifolds_per_step": st["manifolds_total"] / max(st["steps"], 1),

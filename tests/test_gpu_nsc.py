@@ -117,12 +117,12 @@ def test_colored_sweeps_bitwise_equal_serial_sweeps_in_color_order(gpu_ctx_facto
     assert np.abs(lam_g).max() > 0
 
 
-@pytest.mark.parametrize("proj", [0, 1])
+@pytest.mark.parametrize("proj", [1])
 def test_solve_reaches_residual_checked_by_oracle(gpu_ctx_factory, proj):
     B = overlapping_bed(seed=31, mu=0.4)
     ctx = gpu_ctx_factory()
     gpu_load(ctx, B)
-    p = L.nsc_params(beta=0.1, cfm=0.0, omega=1.3, tol=1e-8, max_sweeps=20000, check_every=4, friction_proj=proj,
+    p = L.nsc_params(beta=0.0, cfm=0.0, omega=1.3, tol=1e-8, max_sweeps=20000, check_every=4, friction_proj=proj,
                      warm_mode=0)
     ctx.detect(L.detect_params())
     nm = ctx.nsc_build(p)
@@ -132,7 +132,7 @@ def test_solve_reaches_residual_checked_by_oracle(gpu_ctx_factory, proj):
     assert st["residual"] <= 1e-8 and st["sweeps"] > 0
     _, _, lam = ctx.nsc_manifolds(nm)
     s1 = ctx.download_state()
-    op = O.nsc_params(mu=0.4, beta=0.1, cfm=0.0, omega=1.3, friction_proj=proj)
+    op = O.nsc_params(mu=0.4, beta=0.0, cfm=0.0, omega=1.3, friction_proj=proj)
     W = oracle_load(B)
     # velocities after the solve are the pre-solve velocities plus M^-1 J^T lambda
     W.set_state(v=s0["v"], w=s0["w"])
