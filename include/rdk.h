@@ -67,6 +67,9 @@ typedef struct rdk_nsc_params {
   int position_stabilization; /* scales beta by 0.01 (nsc_contact.cpp:339-341) */
   int warm_mode;           /* 0 cold, 1 previous step (reference), 2 learned (MLP) */
   int stop_rule;           /* 0 reference (max sum|dlambda| < 1e-6), 1 residual <= tol */
+  int warm_walls;          /* 1: plane-wall manifolds also start from the previous
+                              step's impulses, keyed by (rod, plane, endpoint); the
+                              reference starts wall contacts cold */
 } rdk_nsc_params;
 
 typedef struct rdk_soft_params { /* SoftContactCfg, config.hpp:6-27 */
@@ -143,9 +146,12 @@ int rdk_history_size(rdk_ctx* ctx, int* n);
 /* ---- NSC (NscContactSolver, nsc_contact.hpp:71-157) -------------------- */
 /* detectAndBuildManifolds + plane-wall addWallContact (nsc_contact.cpp:190-321) */
 int rdk_nsc_build(rdk_ctx* ctx, const rdk_nsc_params* p, int* n_manifolds);
-/* rec: 41 doubles per manifold (n,t1,t2,r_a,r_b, I_A^-1(r_a x n|t1|t2),
- * I_B^-1(r_b x n|t1|t2), invMA, invMB, g_n, g_t1, g_t2, phi, v_n_pre, e);
- * ab: 2 ints (b = -1 for a wall); lambda: 3 doubles; natural order */
+/* rec: RDK_MF_FIELDS doubles per manifold (n,t1,t2,r_a,r_b,
+ * I_A^-1(r_a x n|t1|t2), I_B^-1(r_b x n|t1|t2), invMA, invMB, g_n, g_t1,
+ * g_t2, phi, v_n_pre, e, g_tb = largest eigenvalue of the (t1,t2) block);
+ * ab: 2 ints (b < 0 for a wall: b = -1 - slot, slot = 2*plane + endpoint);
+ * lambda: 3 doubles; natural order */
+#define RDK_MF_FIELDS 42
 int rdk_nsc_get_manifolds(rdk_ctx* ctx, int cap, double* rec, int* ab,
                           double* lambda);
 int rdk_nsc_set_lambda(rdk_ctx* ctx, int m, const double* lambda);

@@ -11,7 +11,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
-MF_FIELDS = 41
+MF_FIELDS = 42
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -20,7 +20,8 @@ _ip = C.POINTER(C.c_int)
 class NscParams(C.Structure):
     _fields_ = [("mu", C.c_double), ("beta", C.c_double), ("cfm", C.c_double), ("omega", C.c_double),
                 ("tol", C.c_double), ("max_sweeps", C.c_int), ("check_every", C.c_int), ("friction_proj", C.c_int),
-                ("stop_rule", C.c_int), ("position_stabilization", C.c_int), ("warm_start", C.c_int)]
+                ("stop_rule", C.c_int), ("position_stabilization", C.c_int), ("warm_start", C.c_int),
+                ("warm_walls", C.c_int)]
 
 
 class SoftParams(C.Structure):
@@ -100,8 +101,9 @@ def _i(a):
 
 
 def nsc_params(mu=0.5, beta=0.0, cfm=0.0, omega=1.0, tol=1e-8, max_sweeps=10000, check_every=1, friction_proj=0,
-               stop_rule=1, position_stabilization=0, warm_start=1):
+               stop_rule=1, position_stabilization=0, warm_start=1, warm_walls=0):
     p = NscParams()
+    p.warm_walls = warm_walls
     p.mu, p.beta, p.cfm, p.omega, p.tol = mu, beta, cfm, omega, tol
     p.max_sweeps, p.check_every, p.friction_proj, p.stop_rule = max_sweeps, check_every, friction_proj, stop_rule
     p.position_stabilization, p.warm_start = position_stabilization, warm_start

@@ -157,6 +157,7 @@ struct Manifold {
   V3 n, t1, t2, ra, rb;
   V3 IAn, IAt1, IAt2, IBn, IBt1, IBt2;
   double invMA, invMB, gn, gt1, gt2, phi, vnpre, e;
+  double gtb;  // largest eigenvalue of the (t1,t2) Delassus block (incl. cfm)
   double ln, lt1, lt2;
 };
 
@@ -177,6 +178,7 @@ struct orc_world {
   std::vector<Contact> contacts;
   std::vector<Manifold> mf;
   std::unordered_map<uint64_t, std::array<double, 3>> warm;
+  std::unordered_map<uint64_t, std::array<double, 3>> warm_wall;  // key (rod << 2) | slot
   std::unordered_map<uint64_t, HistEntry> hm_hist;
   uint64_t hm_frame = 0;
   std::unordered_map<uint64_t, V3> cs_hist;  // Cundall-Strack, never pruned
@@ -476,6 +478,15 @@ static double diag_term(double ima, double imb, const M3& IA, const M3& IB, V3 r
   return std::max(g, 1e-12);
 }
 
+// Largest eigenvalue of the 2x2 tangential block [[g_t1, W12], [W12, g_t2]],
+// W12 = (r_a x t1).I_a^-1(r_a x t2) + (r_b x t1).I_b^-1(r_b x t2).
+static double tangent_block_max(const Manifold& m, bool wall) {
+  double w12 = dot(cross(m.ra, m.t1), m.IAt2);
+  if (!wall) w12 = w12 + dot(cross(m.rb, m.t1), m.IBt2);
+  const double hm = 0.5 * (m.gt1 + m.gt2), hd = 0.5 * (m.gt1 - m.gt2);
+  return hm + std::sqrt(hd * hd + w12 * w12);
+}
+
 // nsc_contact.cpp:239-321 (rod-rod) followed by plane-wall manifolds
 // (nsc_contact.cpp:190-235 with capsule-plane geometry after collision.cpp:155-195,
 // one contact per penetrating endpoint).
@@ -508,6 +519,7 @@ static void nsc_build(orc_world& W, const orc_nsc_params& P) {
     V3 vrel = sub(add(Bb.v, cross(Bb.w, m.rb)), add(A.v, cross(A.w, m.ra)));
     m.vnpre = dot(m.n, vrel);
     m.e = std::min(A.rest, Bb.rest);
+    m.gtb = tangent_block_max(m, false);
     m.ln = m.lt1 = m.lt2 = 0.0;
     if (P.warm_start) {
       auto it = W.warm.find(pair_key(cp.a, cp.b));
@@ -545,7 +557,7 @@ static void nsc_build(orc_world& W, const orc_nsc_params& P) {
           if (!(sd < 0.0)) continue;
           Manifold m;
           m.a = i;
-          m.b = -1;
+          m.b = -1 - (2 * plane + e);
           m.n = nrm;
           m.phi = sd;  // = -penetration
           tangent_basis(m.n, m.t1, m.t2);
@@ -564,7 +576,16 @@ static void nsc_build(orc_world& W, const orc_nsc_params& P) {
           V3 vrel = neg(add(A.v, cross(A.w, m.ra)));
           m.vnpre = dot(m.n, vrel);
           m.e = A.rest;
+          m.gtb = tangent_block_max(m, true);
           m.ln = m.lt1 = m.lt2 = 0.0;
+          if (P.warm_start && P.warm_walls) {
+            auto it = W.warm_wall.find(((uint64_t)(uint32_t)i << 2) | (uint64_t)(2 * plane + e));
+            if (it != W.warm_wall.end()) {
+              m.ln = std::max(0.0, it->second[0]);
+              m.lt1 = it->second[1];
+              m.lt2 = it->second[2];
+            }
+          }
           W.mf.push_back(m);
         }
       }
@@ -665,11 +686,14 @@ static double psor_one(orc_world& W, Manifold& m, const orc_nsc_params& P) {
     m.lt2 = clampd(old_t2 + delta_t2, -limit_t2, limit_t2);
     dt2 = m.lt2 - old_t2;
   } else {
-    // Euclidean disc projection of the joint tangential update.
+    // Euclidean disc: projected step of the joint tangential impulse with one
+    // scalar step omega/g_tb (g_tb bounds the 2x2 tangential block), whose
+    // fixed point is the maximal-dissipation condition w_t = -alpha lambda_t.
     double w_t1 = dot(m.t1, v_rel) + cfm * m.lt1;
     double w_t2 = dot(m.t2, v_rel) + cfm * m.lt2;
-    double y1 = m.lt1 - (omega / m.gt1) * w_t1;
-    double y2 = m.lt2 - (omega / m.gt2) * w_t2;
+    const double st = omega / m.gtb;
+    double y1 = m.lt1 - st * w_t1;
+    double y2 = m.lt2 - st * w_t2;
     double ny = std::sqrt(y1 * y1 + y2 * y2);
     if (ny > max_fric) {
       double sc = ny > 0.0 ? max_fric / ny : 0.0;
@@ -722,18 +746,20 @@ static double residual_one(const orc_world& W, const Manifold& m, const orc_nsc_
     double L2 = std::sqrt(std::max(0.0, mf * mf - m.lt1 * m.lt1));
     p1 = clampd(m.lt1 - w_t1 / m.gt1, -L1, L1);
     p2 = clampd(m.lt2 - w_t2 / m.gt2, -L2, L2);
+    r = std::max(r, m.gt1 * std::fabs(m.lt1 - p1));
+    r = std::max(r, m.gt2 * std::fabs(m.lt2 - p2));
   } else {
-    p1 = m.lt1 - w_t1 / m.gt1;
-    p2 = m.lt2 - w_t2 / m.gt2;
+    p1 = m.lt1 - w_t1 / m.gtb;
+    p2 = m.lt2 - w_t2 / m.gtb;
     double ny = std::sqrt(p1 * p1 + p2 * p2);
     if (ny > mf) {
       double sc = ny > 0.0 ? mf / ny : 0.0;
       p1 = p1 * sc;
       p2 = p2 * sc;
     }
+    const double d1 = m.lt1 - p1, d2 = m.lt2 - p2;
+    r = std::max(r, m.gtb * std::sqrt(d1 * d1 + d2 * d2));
   }
-  r = std::max(r, m.gt1 * std::fabs(m.lt1 - p1));
-  r = std::max(r, m.gt2 * std::fabs(m.lt2 - p2));
   return r;
 }
 
@@ -781,10 +807,17 @@ static int solve(orc_world& W, const orc_nsc_params& P, const int* order, double
 }
 
 // nsc_contact.cpp:646-654
+// Wall entries: the reference keys them (i,i) and never matches them; the
+// build keeps them separately, keyed by (rod, slot), for the optional
+// warm_walls mode.
 static void store_warm(orc_world& W) {
   W.warm.clear();
+  W.warm_wall.clear();
   for (const Manifold& m : W.mf) {
-    if (m.b < 0) continue;  // wall entries are keyed (i,i) and never matched
+    if (m.b < 0) {
+      W.warm_wall[((uint64_t)(uint32_t)m.a << 2) | (uint64_t)(-1 - m.b)] = {m.ln, m.lt1, m.lt2};
+      continue;
+    }
     W.warm[pair_key(m.a, m.b)] = {m.ln, m.lt1, m.lt2};
   }
 }
@@ -1212,6 +1245,7 @@ static void mf_to_rec(const Manifold& m, double* r) {
   r[38] = m.phi;
   r[39] = m.vnpre;
   r[40] = m.e;
+  r[41] = m.gtb;
 }
 static void rec_to_mf(const double* r, Manifold& m) {
   V3* vs[11] = {&m.n, &m.t1, &m.t2, &m.ra, &m.rb, &m.IAn, &m.IAt1, &m.IAt2, &m.IBn, &m.IBt1, &m.IBt2};
@@ -1224,6 +1258,7 @@ static void rec_to_mf(const double* r, Manifold& m) {
   m.phi = r[38];
   m.vnpre = r[39];
   m.e = r[40];
+  m.gtb = r[41];
 }
 
 int orc_nsc_get_manifolds(orc_world* W, int cap, double* rec, int* ab, double* lambda) {

@@ -43,6 +43,7 @@ typedef struct orc_nsc_params {
   int stop_rule;         /* 0 = reference rule (max sum|dlambda| < 1e-6), 1 = residual <= tol */
   int position_stabilization; /* only scales beta (x0.01) as the reference does */
   int warm_start;        /* 0 = cold, 1 = previous-step key match (reference) */
+  int warm_walls;        /* 1 = plane-wall manifolds also warm-started, keyed (rod, slot) */
 } orc_nsc_params;
 
 typedef struct orc_soft_params {
@@ -85,7 +86,7 @@ int orc_get_contacts(orc_world* W, int cap, int* a, int* b, double* pa,
 /* NSC */
 int orc_nsc_build(orc_world* W, const orc_nsc_params* P);
 int orc_nsc_num_manifolds(orc_world* W);
-/* rec: ORC_MF_FIELDS doubles per manifold; ab: 2 ints (b=-1 for a wall) */
+/* rec: ORC_MF_FIELDS doubles per manifold; ab: 2 ints (b = -1 - slot for a wall) */
 int orc_nsc_get_manifolds(orc_world* W, int cap, double* rec, int* ab,
                           double* lambda);
 int orc_nsc_set_manifolds(orc_world* W, int m, const double* rec,
@@ -118,7 +119,7 @@ int orc_vv_step(orc_world* W, int model, const void* params, int nsteps,
 double orc_kinetic_energy(orc_world* W);
 double orc_packing_fraction(orc_world* W);
 
-#define ORC_MF_FIELDS 41
+#define ORC_MF_FIELDS 42
 
 #ifdef __cplusplus
 }

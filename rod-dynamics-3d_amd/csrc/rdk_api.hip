@@ -114,7 +114,7 @@ int rdk_destroy(rdk_ctx* ctx) {
     release(ctx->x[k]); release(ctx->v[k]); release(ctx->w[k]); release(ctx->f[k]); release(ctx->tau[k]);
     release(ctx->sc[k]); release(ctx->sx[k]);
     release(ctx->cpa[k]); release(ctx->cpb[k]); release(ctx->cn[k]); release(ctx->csh[k]);
-    release(ctx->lam[k]); release(ctx->lamc[k]); release(ctx->prev_lam[k]);
+    release(ctx->lam[k]); release(ctx->lamc[k]); release(ctx->prev_lam[k]); release(ctx->prev_wlam[k]);
     release(ctx->hxi[k]); release(ctx->hxi_new[k]);
   }
   for (int k = 0; k < 4; ++k) release(ctx->q[k]);
@@ -124,7 +124,7 @@ int rdk_destroy(rdk_ctx* ctx) {
   release(ctx->sys_of); release(ctx->sys_dev);
   release(ctx->chunk_rod); release(ctx->chunk_k); release(ctx->rod_nchunk);
   release(ctx->ckey); release(ctx->ckey_alt); release(ctx->cid); release(ctx->cid_alt);
-  release(ctx->cell_start); release(ctx->cell_end); release(ctx->srad); release(ctx->srod); release(ctx->sk);
+  release(ctx->cell_start); release(ctx->cell_end); release(ctx->srad); release(ctx->scf); release(ctx->srod); release(ctx->sk);
   release(ctx->pkeys); release(ctx->pkeys_alt); release(ctx->counters);
   release(ctx->ca); release(ctx->cb); release(ctx->ctype); release(ctx->cdist); release(ctx->chs);
   release(ctx->cs); release(ctx->ct);
@@ -132,7 +132,7 @@ int rdk_destroy(rdk_ctx* ctx) {
   release(ctx->wall_flag); release(ctx->wall_off); release(ctx->color); release(ctx->color_tmp);
   release(ctx->perm); release(ctx->perm_alt); release(ctx->uncolored); release(ctx->color_key);
   release(ctx->color_key_alt); release(ctx->csr_key); release(ctx->csr_key_alt); release(ctx->csr_val);
-  release(ctx->csr_val_alt); release(ctx->body_start); release(ctx->prev_keys); release(ctx->resid);
+  release(ctx->csr_val_alt); release(ctx->body_start); release(ctx->d_cstart); release(ctx->small_out); release(ctx->prev_keys); release(ctx->prev_wkeys); release(ctx->resid);
   release(ctx->hkeys); release(ctx->hkeys_new); release(ctx->hlast); release(ctx->hlast_new);
   release(ctx->hflag); release(ctx->hscan); release(ctx->pe); release(ctx->temp); release(ctx->red);
   if (ctx->h_counters) (void)hipHostFree(ctx->h_counters);
@@ -236,7 +236,7 @@ int rdk_set_bodies(rdk_ctx* ctx, int n, const double* x, const double* q, const 
   ctx->nsys = 0;
   ctx->built_chunk_len = -2.0;
   ctx->n_chunks = 0;
-  ctx->nc = ctx->nm = ctx->nprev = ctx->nhist = 0;
+  ctx->nc = ctx->nm = ctx->nprev = ctx->nprev_w = ctx->nhist = 0;
   ctx->have_manifolds = ctx->have_coloring = false;
   RDK_HIP(hipStreamSynchronize(ctx->stream));
   return RDK_OK;
@@ -464,14 +464,14 @@ int rdk_nsc_get_manifolds(rdk_ctx* ctx, int cap, double* rec, int* ab, double* l
   if (!ctx->have_manifolds) return RDK_ERR_STATE;
   const int n = std::min(cap, ctx->nm);
   if (n <= 0) return RDK_OK;
-  const size_t stride = ctx->mf.cap / 41;
+  const size_t stride = ctx->mf.cap / RDK_MF_FIELDS;
   if (rec) {
     std::vector<double> tmp(n);
-    for (int f = 0; f < 41; ++f) {
+    for (int f = 0; f < RDK_MF_FIELDS; ++f) {
       RDK_HIP(hipMemcpyAsync(tmp.data(), ctx->mf.p + (size_t)f * stride, n * sizeof(double), hipMemcpyDeviceToHost,
                              ctx->stream));
       RDK_HIP(hipStreamSynchronize(ctx->stream));
-      for (int i = 0; i < n; ++i) rec[(size_t)41 * i + f] = tmp[i];
+      for (int i = 0; i < n; ++i) rec[(size_t)RDK_MF_FIELDS * i + f] = tmp[i];
     }
   }
   if (ab) {
@@ -516,7 +516,8 @@ int rdk_nsc_sweeps(rdk_ctx* ctx, const rdk_nsc_params* p, int nsweeps) {
 int rdk_nsc_get_coloring(rdk_ctx* ctx, int cap, int* color, int* order, int* ncol) {
   CTX_CHECK();
   if (!ctx->have_manifolds) return RDK_ERR_STATE;
-  if (!ctx->have_coloring) RDK_TRY(nsc_color_run(ctx));
+  if (!ctx->have_coloring) RDK_TRY(nsc_color_run(ctx, true));
+  RDK_TRY(color_to_host(ctx));
   const int n = std::min(cap, ctx->nm);
   RDK_TRY(dl(ctx, color, ctx->color.p, n));
   RDK_TRY(dl(ctx, order, ctx->perm.p, n));
@@ -529,7 +530,7 @@ int rdk_nsc_residual(rdk_ctx* ctx, const rdk_nsc_params* p, double* res) {
   CTX_CHECK();
   RDK_TRY(check_nsc(ctx, p));
   if (!ctx->have_manifolds) return RDK_ERR_STATE;
-  if (!ctx->have_coloring) RDK_TRY(nsc_color_run(ctx));
+  if (!ctx->have_coloring) RDK_TRY(nsc_color_run(ctx, false));
   // residual runs on the colour-ordered copy: refresh its lambda
   RDK_TRY(nsc_residual_run(ctx, p, res, nullptr));
   return prof_flush(ctx);

@@ -15,14 +15,16 @@
 //   4. chunk_gather    chunk centre, rod centre, radius, rod id, chunk index in
 //                      sorted (cell) order so the stencil walk reads contiguous
 //                      memory
-//   5. broad_narrow    one thread per chunk of rod a walks the 27 cells and,
-//                      for every chunk of a rod b > a whose bounding sphere
-//                      (under the rod-centre minimum image) touches its own, runs
-//                      the full-rod Lumelsky narrow phase.  A pair is emitted
-//                      only by the chunk pair that contains the closest points
-//                      (owner rule), so each contact appears exactly once
-//                      whatever the chunking.  Emission is wavefront-aggregated
-//                      (ballot + popcount, one atomic per wavefront).
+//   5. broad_narrow    one thread per sorted chunk walks the 27 cells; each
+//                      unordered chunk pair is examined once (by the lower
+//                      sorted position).  An FP32 bounding-sphere test with the
+//                      chunk-centre minimum image and a rounding margin rejects
+//                      far pairs conservatively; survivors run the full-rod FP64
+//                      Lumelsky narrow phase.  A pair is emitted only by the
+//                      chunk pair that contains the closest points (owner rule),
+//                      so each contact appears exactly once whatever the
+//                      chunking.  Emission is wavefront-aggregated (ballot +
+//                      popcount, one atomic per wavefront).
 //   6. radix sort of the 64-bit (a,b) keys -> canonical (a,b) order
 //   7. contact_record  recomputes the narrow phase per sorted pair and writes
 //                      the structure-of-arrays contact record.
@@ -94,6 +96,7 @@ __global__ void k_cell_ranges(const uint32_t* __restrict__ key, int M, int* __re
 struct SortedChunks {
   double *c[3], *x[3], *rad;
   int *rod, *k;
+  float4* cf;  // FP32 (centre, radius) for the conservative prefilter
 };
 
 __global__ void k_chunk_gather(BodyPtrs B, const int* __restrict__ sorted_id, const int* __restrict__ chunk_rod,
@@ -116,6 +119,7 @@ __global__ void k_chunk_gather(BodyPtrs B, const int* __restrict__ sorted_id, co
   S.rad[p] = rad;
   S.rod[p] = i;
   S.k[p] = k;
+  S.cf[p] = make_float4((float)c.x, (float)c.y, (float)c.z, (float)rad);
 }
 
 // Wavefront-aggregated append: one atomic per wavefront of active lanes.
@@ -135,6 +139,11 @@ __device__ __forceinline__ int owner_chunk(double s, int K) {
   return k < 0 ? 0 : (k >= K ? K - 1 : k);
 }
 
+// One thread per sorted chunk p.  Every unordered chunk pair (p, q) is
+// examined once, by the thread with the lower sorted position, so the 27-cell
+// walk starts at q = p + 1.  The FP32 prefilter uses the chunk-centre minimum
+// image, which is never farther than the rod-centre image the narrow phase
+// uses, plus a rounding margin, so it never rejects a touching pair.
 __global__ void __launch_bounds__(256) k_broad_narrow(BodyPtrs B, const SysDev* __restrict__ sys, SortedChunks SC,
                                                       const int* __restrict__ cstart, const int* __restrict__ cend,
                                                       const int* __restrict__ rod_nchunk, int M, double delta,
@@ -145,19 +154,15 @@ __global__ void __launch_bounds__(256) k_broad_narrow(BodyPtrs B, const SysDev* 
   unsigned long long ncand = 0;
   if (p < M) {
     const int a = SC.rod[p];
-    const int ka = SC.k[p];
-    const d3 ca = mk3(SC.c[0][p], SC.c[1][p], SC.c[2][p]);
-    const d3 xa = mk3(SC.x[0][p], SC.x[1][p], SC.x[2][p]);
-    const double rada = SC.rad[p];
+    const float4 me = SC.cf[p];
     const SysDev S = sys[B.sys[a]];
-    const int Ka = rod_nchunk[a];
     const bool a_static = !(B.inv_mass[a] > 0.0);
-    const int cc[3] = {cell_coord(ca.x, S.lo[0], S.inv_cell[0], S.dims[0], S.periodic[0]),
-                       cell_coord(ca.y, S.lo[1], S.inv_cell[1], S.dims[1], S.periodic[1]),
-                       cell_coord(ca.z, S.lo[2], S.inv_cell[2], S.dims[2], S.periodic[2])};
-    d3 ua = mk3(0, 0, 0);
-    double ha = 0.0, ra = 0.0;
-    bool a_loaded = false;
+    const float spanf[3] = {(float)S.span[0], (float)S.span[1], (float)S.span[2]};
+    const float ispanf[3] = {(float)(1.0 / S.span[0]), (float)(1.0 / S.span[1]), (float)(1.0 / S.span[2])};
+    const float extra = (float)delta + S.fmargin;
+    const int cc[3] = {cell_coord(SC.c[0][p], S.lo[0], S.inv_cell[0], S.dims[0], S.periodic[0]),
+                       cell_coord(SC.c[1][p], S.lo[1], S.inv_cell[1], S.dims[1], S.periodic[1]),
+                       cell_coord(SC.c[2][p], S.lo[2], S.inv_cell[2], S.dims[2], S.periodic[2])};
     for (int dz = -1; dz <= 1; ++dz) {
       int z = cc[2] + dz;
       if (S.periodic[2]) {
@@ -177,31 +182,30 @@ __global__ void __launch_bounds__(256) k_broad_narrow(BodyPtrs B, const SysDev* 
             x = (x + S.dims[0]) % S.dims[0];
           } else if (x < 0 || x >= S.dims[0]) continue;
           const int cell = S.cell_off + x + S.dims[0] * (y + S.dims[1] * z);
-          const int q0 = cstart[cell], q1 = cend[cell];
-          for (int q = q0; q < q1; ++q) {
+          const int q1 = cend[cell];
+          int q = cstart[cell];
+          if (q <= p) q = p + 1;
+          for (; q < q1; ++q) {
+            const float4 o = SC.cf[q];
+            float dxf = o.x - me.x, dyf = o.y - me.y, dzf = o.z - me.z;
+            if (S.periodic[0]) dxf -= spanf[0] * rintf(dxf * ispanf[0]);
+            if (S.periodic[1]) dyf -= spanf[1] * rintf(dyf * ispanf[1]);
+            if (S.periodic[2]) dzf -= spanf[2] * rintf(dzf * ispanf[2]);
+            const float thr = me.w + o.w + extra;
+            if (!(dxf * dxf + dyf * dyf + dzf * dzf < thr * thr)) continue;
             const int b = SC.rod[q];
-            if (b <= a) continue;
-            const d3 xb = mk3(SC.x[0][q], SC.x[1][q], SC.x[2][q]);
-            const d3 sh = mk3(image_shift(xb.x - xa.x, S.span[0], S.periodic[0]),
-                              image_shift(xb.y - xa.y, S.span[1], S.periodic[1]),
-                              image_shift(xb.z - xa.z, S.span[2], S.periodic[2]));
-            const d3 d = mk3(SC.c[0][q] + sh.x - ca.x, SC.c[1][q] + sh.y - ca.y, SC.c[2][q] + sh.z - ca.z);
-            const double thr = (rada + SC.rad[q] + delta) * (1.0 + 1e-9) + 1e-9;
-            if (!(dot(d, d) < thr * thr)) continue;
+            if (b == a) continue;
             if (a_static && !(B.inv_mass[b] > 0.0)) continue;
             ++ncand;
-            if (!a_loaded) {
-              ua = axis_of(load_q(B, a));
-              ha = B.h[a];
-              ra = B.r[a];
-              a_loaded = true;
-            }
-            const d3 ub = axis_of(load_q(B, b));
-            NarrowOut o;
-            if (capsule_narrow(xa, ua, ha, ra, xb, ub, B.h[b], B.r[b], S.span, S.periodic, delta, false, o)) {
-              if (owner_chunk(o.s, Ka) == ka && owner_chunk(o.t, rod_nchunk[b]) == SC.k[q]) {
+            // exact FP64 narrow phase in canonical (lower index first) order
+            const int A = a < b ? a : b, Bi = a < b ? b : a;
+            const int kA = a < b ? SC.k[p] : SC.k[q], kB = a < b ? SC.k[q] : SC.k[p];
+            NarrowOut no;
+            if (capsule_narrow(load3(B.x, A), axis_of(load_q(B, A)), B.h[A], B.r[A], load3(B.x, Bi),
+                               axis_of(load_q(B, Bi)), B.h[Bi], B.r[Bi], S.span, S.periodic, delta, false, no)) {
+              if (owner_chunk(no.s, rod_nchunk[A]) == kA && owner_chunk(no.t, rod_nchunk[Bi]) == kB) {
                 const unsigned pos = wave_append(counter);
-                if (pos < cap) out[pos] = ((uint64_t)(uint32_t)a << 32) | (uint64_t)(uint32_t)b;
+                if (pos < cap) out[pos] = ((uint64_t)(uint32_t)A << 32) | (uint64_t)(uint32_t)Bi;
               }
             }
           }
@@ -303,10 +307,15 @@ static int build_grid(rdk_ctx* ctx, const rdk_detect_params* p) {
         total *= d;
       }
     }
+    double amax = 0.0;
     for (int k = 0; k < 3; ++k) {
       S.cell[k] = S.span[k] / S.dims[k];
       S.inv_cell[k] = 1.0 / S.cell[k];
+      amax = std::max(amax, std::max(std::fabs(S.lo[k]), std::fabs(S.hi[k])) + 2.0 * S.span[k]);
     }
+    // FP32 prefilter margin: rounding of centres, radii, the image shift and
+    // the squared distance, bounded by a few ulps of the largest coordinate
+    S.fmargin = (float)(64.0 * amax * 1.1920928955078125e-07 + 1e-6 * (2.0 * radmax[s] + p->delta) + 1e-6);
     S.cell_off = cell_off;
     cell_off += (int)total;
   }
@@ -333,6 +342,7 @@ static int build_grid(rdk_ctx* ctx, const rdk_detect_params* p) {
     RDK_TRY(ensure(ctx, ctx->sx[k], M));
   }
   RDK_TRY(ensure(ctx, ctx->srad, M));
+  RDK_TRY(ensure(ctx, ctx->scf, M));
   RDK_TRY(ensure(ctx, ctx->srod, M));
   RDK_TRY(ensure(ctx, ctx->sk, M));
   ctx->built_chunk_len = cl;
@@ -394,6 +404,7 @@ int detect_run(rdk_ctx* ctx, const rdk_detect_params* p) {
   SC.rad = ctx->srad.p;
   SC.rod = ctx->srod.p;
   SC.k = ctx->sk.p;
+  SC.cf = ctx->scf.p;
   {
     ProfScope ps(ctx, K_CHUNK_GATHER);
     hipLaunchKernelGGL(k_chunk_gather, dim3(grid_for(M, nb)), dim3(nb), 0, ctx->stream, B, sid, ctx->chunk_rod.p,

@@ -66,7 +66,8 @@ struct DevBuf {
 struct SysDev {
   double lo[3], hi[3], span[3], cell[3], inv_cell[3];
   int periodic[3], dims[3];
-  int cell_off, floor_wall, ceil_wall, pad;
+  int cell_off, floor_wall, ceil_wall;
+  float fmargin;  // FP32 broad-phase prefilter margin
   double mu;
 };
 
@@ -113,6 +114,7 @@ struct rdk_ctx {
   rdk::DevBuf<int> cid, cid_alt;
   rdk::DevBuf<int> cell_start, cell_end;
   rdk::DevBuf<double> sc[3], sx[3], srad;
+  rdk::DevBuf<float4> scf;
   rdk::DevBuf<int> srod, sk;
   rdk::DevBuf<uint64_t> pkeys, pkeys_alt;
   rdk::DevBuf<unsigned int> counters;  // [0] pair count, [1..] misc
@@ -138,12 +140,16 @@ struct rdk_ctx {
   rdk::DevBuf<int> perm_alt;
   std::vector<int> color_start;  // host, ncolor+1
   int ncolor = 0;
-  bool have_manifolds = false, have_coloring = false;
+  bool have_manifolds = false, have_coloring = false, color_host_valid = false;
+  rdk::DevBuf<int> d_cstart;
+  rdk::DevBuf<char> small_out;
   // body -> manifold CSR
   rdk::DevBuf<uint32_t> csr_key, csr_key_alt;
   rdk::DevBuf<int> csr_val, csr_val_alt, body_start;
   // warm cache
-  int nprev = 0;
+  int nprev = 0, nprev_w = 0;
+  rdk::DevBuf<uint64_t> prev_wkeys;
+  rdk::DevBuf<double> prev_wlam[3];
   rdk::DevBuf<uint64_t> prev_keys;
   rdk::DevBuf<double> prev_lam[3];
   rdk::DevBuf<double> resid;  // [0] residual bits, [1] ref residual bits
@@ -248,7 +254,8 @@ int detect_run(rdk_ctx* ctx, const rdk_detect_params* p);
 int nsc_build_run(rdk_ctx* ctx, const rdk_nsc_params* p);
 int nsc_solve_run(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* st);
 int nsc_sweeps_run(rdk_ctx* ctx, const rdk_nsc_params* p, int nsweeps);
-int nsc_color_run(rdk_ctx* ctx);
+int nsc_color_run(rdk_ctx* ctx, bool need_host);
+int color_to_host(rdk_ctx* ctx);
 int nsc_residual_run(rdk_ctx* ctx, const rdk_nsc_params* p, double* res, double* res_ref);
 int integrate_run(rdk_ctx* ctx, int stage);
 int forces_run(rdk_ctx* ctx, int model, const void* params, double* pe);

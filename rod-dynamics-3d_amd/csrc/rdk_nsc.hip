@@ -22,7 +22,7 @@ enum MF {
   F_N = 0, F_T1 = 3, F_T2 = 6, F_RA = 9, F_RB = 12,
   F_IAN = 15, F_IAT1 = 18, F_IAT2 = 21, F_IBN = 24, F_IBT1 = 27, F_IBT2 = 30,
   F_IMA = 33, F_IMB = 34, F_GN = 35, F_GT1 = 36, F_GT2 = 37, F_PHI = 38, F_VNPRE = 39, F_E = 40,
-  F_COUNT = 41
+  F_GTB = 41, F_COUNT = 42
 };
 
 struct MfPtrs {
@@ -71,6 +71,16 @@ __device__ __forceinline__ int lower_bound_u64(const uint64_t* a, int n, uint64_
     else hi = mid;
   }
   return lo;
+}
+
+// Largest eigenvalue of the 2x2 tangential Delassus block
+// [[g_t1, W12], [W12, g_t2]], W12 = (r_a x t1).I_a^-1(r_a x t2) (+ B term).
+__device__ __forceinline__ double tangent_block_max(d3 ra, d3 t1, d3 IAt2, d3 rb, d3 IBt2, double g1, double g2,
+                                                    bool wall) {
+  double w12 = dot(cross(ra, t1), IAt2);
+  if (!wall) w12 = w12 + dot(cross(rb, t1), IBt2);
+  const double hm = 0.5 * (g1 + g2), hd = 0.5 * (g1 - g2);
+  return hm + sqrt(hd * hd + w12 * w12);
 }
 
 struct ContactIn {
@@ -124,6 +134,7 @@ __global__ void k_manifold(BodyPtrs B, ContactIn C, int nc, MfPtrs M, double cfm
   const d3 vrel = (ld3(B.v, b) + cross(ld3(B.w, b), rb)) - (ld3(B.v, a) + cross(ld3(B.w, a), ra));
   M.at(F_VNPRE, i) = dot(n, vrel);
   M.at(F_E, i) = dmin(B.rest[a], B.rest[b]);
+  M.at(F_GTB, i) = tangent_block_max(ra, t1, IA[2], rb, IB[2], g[1], g[2], false);
   M.a[i] = a;
   M.b[i] = b;
   double l0 = 0.0, l1 = 0.0, l2 = 0.0;
@@ -177,7 +188,9 @@ __global__ void k_wall_flags(BodyPtrs B, const SysDev* __restrict__ sys, int n, 
 }
 
 __global__ void k_wall_manifold(BodyPtrs B, const SysDev* __restrict__ sys, int nbody, const int* __restrict__ flag,
-                                const int* __restrict__ off, int base, MfPtrs M, double cfm) {
+                                const int* __restrict__ off, int base, MfPtrs M, double cfm, int warm,
+                                const uint64_t* __restrict__ prev_keys, int nprev, const double* __restrict__ pl0,
+                                const double* __restrict__ pl1, const double* __restrict__ pl2) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 4 * nbody || !flag[t]) return;
   const int i = t >> 2, slot = t & 3;
@@ -220,11 +233,33 @@ __global__ void k_wall_manifold(BodyPtrs B, const SysDev* __restrict__ sys, int 
   const d3 vrel = -(ld3(B.v, i) + cross(ld3(B.w, i), ra));
   M.at(F_VNPRE, m) = dot(n, vrel);
   M.at(F_E, m) = B.rest[i];
+  M.at(F_GTB, m) = tangent_block_max(ra, t1, IA[2], z, z, g[1], g[2], true);
   M.a[m] = i;
-  M.b[m] = -1;
-  M.lam[0][m] = 0.0;
-  M.lam[1][m] = 0.0;
-  M.lam[2][m] = 0.0;
+  M.b[m] = -1 - slot;
+  double l0 = 0.0, l1 = 0.0, l2 = 0.0;
+  if (warm && nprev > 0) {
+    const uint64_t key = ((uint64_t)(uint32_t)i << 2) | (uint64_t)slot;
+    const int j = lower_bound_u64(prev_keys, nprev, key);
+    if (j < nprev && prev_keys[j] == key) {
+      l0 = dmax(0.0, pl0[j]);
+      l1 = pl1[j];
+      l2 = pl2[j];
+    }
+  }
+  M.lam[0][m] = l0;
+  M.lam[1][m] = l1;
+  M.lam[2][m] = l2;
+}
+
+__global__ void k_copy_wall_keys_lambda(MfPtrs nat, int base, int nw, uint64_t* __restrict__ pk,
+                                        double* __restrict__ p0, double* __restrict__ p1, double* __restrict__ p2) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nw) return;
+  const int m = base + i;
+  pk[i] = ((uint64_t)(uint32_t)nat.a[m] << 2) | (uint64_t)(-1 - nat.b[m]);
+  p0[i] = nat.lam[0][m];
+  p1[i] = nat.lam[1][m];
+  p2[i] = nat.lam[2][m];
 }
 
 __global__ void k_fill_color(int* __restrict__ a, int n, int v) {
@@ -351,10 +386,8 @@ __global__ void k_unpermute_lambda(MfPtrs col, MfPtrs nat, const int* __restrict
 }
 
 // ---- impulses ----------------------------------------------------------------
-// Warm-start application (nsc_contact.cpp:344-380) for one colour.
-__global__ void k_warm_apply(BodyPtrs B, MfPtrs M, int i0, int i1) {
-  int i = i0 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= i1) return;
+// Warm-start impulse application of manifold i (nsc_contact.cpp:344-380).
+__device__ __forceinline__ void warm_apply_one(const BodyPtrs& B, const MfPtrs& M, int i) {
   const int a = M.a[i], b = M.b[i];
   const bool wall = b < 0;
   const double ima = M.at(F_IMA, i), imb = M.at(F_IMB, i);
@@ -388,6 +421,13 @@ __global__ void k_warm_apply(BodyPtrs B, MfPtrs M, int i0, int i1) {
     st3(B.v, b, vb);
     st3(B.w, b, wb);
   }
+}
+
+// Warm-start application for one colour.
+__global__ void k_warm_apply(BodyPtrs B, MfPtrs M, int i0, int i1) {
+  int i = i0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= i1) return;
+  warm_apply_one(B, M, i);
 }
 
 struct SolveConst {
@@ -466,10 +506,12 @@ __device__ __forceinline__ double psor_update(const BodyPtrs& B, const MfPtrs& M
     lt2 = dclamp(old_t2 + delta_t2, -limit_t2, limit_t2);
     dt2 = lt2 - old_t2;
   } else {
+    // Euclidean disc: one scalar step omega/g_tb for the tangential pair
     const double w_t1 = dot(t1, v_rel) + K.cfm * lt1;
     const double w_t2 = dot(t2, v_rel) + K.cfm * lt2;
-    double y1 = lt1 - (K.omega / M.at(F_GT1, i)) * w_t1;
-    double y2 = lt2 - (K.omega / M.at(F_GT2, i)) * w_t2;
+    const double st = K.omega / M.at(F_GTB, i);
+    double y1 = lt1 - st * w_t1;
+    double y2 = lt2 - st * w_t2;
     const double ny = sqrt(y1 * y1 + y2 * y2);
     if (ny > max_fric) {
       const double sc = ny > 0.0 ? max_fric / ny : 0.0;
@@ -554,18 +596,21 @@ __device__ __forceinline__ double residual_one(const BodyPtrs& B, const MfPtrs& 
     const double L2 = sqrt(dmax(0.0, mf * mf - lt1 * lt1));
     p1 = dclamp(lt1 - w_t1 / g1, -L1, L1);
     p2 = dclamp(lt2 - w_t2 / g2, -L2, L2);
+    r = dmax(r, g1 * fabs(lt1 - p1));
+    r = dmax(r, g2 * fabs(lt2 - p2));
   } else {
-    p1 = lt1 - w_t1 / g1;
-    p2 = lt2 - w_t2 / g2;
+    const double gtb = M.at(F_GTB, i);
+    p1 = lt1 - w_t1 / gtb;
+    p2 = lt2 - w_t2 / gtb;
     const double ny = sqrt(p1 * p1 + p2 * p2);
     if (ny > mf) {
       const double sc = ny > 0.0 ? mf / ny : 0.0;
       p1 = p1 * sc;
       p2 = p2 * sc;
     }
+    const double d1 = lt1 - p1, d2 = lt2 - p2;
+    r = dmax(r, gtb * sqrt(d1 * d1 + d2 * d2));
   }
-  r = dmax(r, g1 * fabs(lt1 - p1));
-  r = dmax(r, g2 * fabs(lt2 - p2));
   return r;
 }
 
@@ -592,6 +637,186 @@ __global__ void k_copy_keys_lambda(const uint64_t* __restrict__ keys, MfPtrs nat
   p0[i] = nat.lam[0][i];
   p1[i] = nat.lam[1][i];
   p2[i] = nat.lam[2][i];
+}
+
+// ---- single-workgroup paths for small manifold counts ----------------------
+// One workgroup (SMALL_BLOCK threads) runs every colouring round, or the
+// warm-start application plus all PSOR sweeps with their residual checks, in
+// one launch; workgroup barriers separate colours and rounds.
+constexpr int SMALL_BLOCK = 1024;
+
+__device__ __forceinline__ double block_max(double v, double* red) {
+  for (int off = 32; off > 0; off >>= 1) {
+    double o = __shfl_down(v, off);
+    v = (o != o || o > v) ? o : v;
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double r = 0.0;
+  const int nw = (blockDim.x + 63) >> 6;
+  for (int k = 0; k < nw; ++k) {
+    const double o = red[k];
+    r = (o != o || o > r) ? o : r;
+  }
+  __syncthreads();
+  return r;
+}
+
+__global__ void __launch_bounds__(SMALL_BLOCK) k_color_small(const int* __restrict__ ma, const int* __restrict__ mb,
+                                                             int nm, const int* __restrict__ bstart,
+                                                             const int* __restrict__ csr, int* __restrict__ color,
+                                                             int* __restrict__ tent, unsigned* __restrict__ ctr) {
+  __shared__ unsigned s_left, s_over, s_max;
+  for (int j = threadIdx.x; j < nm; j += blockDim.x) color[j] = -1;
+  if (threadIdx.x == 0) {
+    s_over = 0;
+    s_max = 0;
+  }
+  __syncthreads();
+  for (int round = 0; round < 4096; ++round) {
+    if (threadIdx.x == 0) s_left = 0;
+    // assign: smallest colour free among coloured neighbours
+    for (int j = threadIdx.x; j < nm; j += blockDim.x) {
+      if (color[j] >= 0) {
+        tent[j] = -1;
+        continue;
+      }
+      unsigned long long m0 = 0, m1 = 0;
+      const int bodies[2] = {ma[j], mb[j]};
+      for (int s = 0; s < 2; ++s) {
+        const int bd = bodies[s];
+        if (bd < 0) continue;
+        for (int k = bstart[bd]; k < bstart[bd + 1]; ++k) {
+          const int o = csr[k];
+          if (o == j) continue;
+          const int c = color[o];
+          if (c >= 0) {
+            if (c < 64) m0 |= 1ull << c;
+            else m1 |= 1ull << (c - 64);
+          }
+        }
+      }
+      int pick;
+      if (~m0) pick = __ffsll((long long)~m0) - 1;
+      else if (~m1) pick = 64 + __ffsll((long long)~m1) - 1;
+      else {
+        pick = 127;
+        atomicAdd(&s_over, 1u);
+      }
+      tent[j] = pick;
+    }
+    __syncthreads();
+    // resolve: a pick survives unless a lower-index neighbour picked the same colour
+    for (int j = threadIdx.x; j < nm; j += blockDim.x) {
+      const int tj = tent[j];
+      if (tj < 0) continue;
+      bool ok = true;
+      const int bodies[2] = {ma[j], mb[j]};
+      for (int s = 0; s < 2 && ok; ++s) {
+        const int bd = bodies[s];
+        if (bd < 0) continue;
+        for (int k = bstart[bd]; k < bstart[bd + 1]; ++k) {
+          const int o = csr[k];
+          if (o < j && tent[o] == tj) {
+            ok = false;
+            break;
+          }
+        }
+      }
+      if (ok) {
+        color[j] = tj;
+        atomicMax(&s_max, (unsigned)tj);
+      } else {
+        atomicAdd(&s_left, 1u);
+      }
+    }
+    __syncthreads();
+    const unsigned left = s_left;
+    __syncthreads();
+    if (left == 0 || s_over) break;
+  }
+  if (threadIdx.x == 0) {
+    ctr[0] = s_left;
+    ctr[1] = s_max;
+    ctr[2] = s_over;
+  }
+}
+
+// colour histogram -> offsets (device), ncolor in ctr[3]
+__global__ void k_color_offsets(const int* __restrict__ color, int nm, const unsigned* __restrict__ maxcol,
+                                int* __restrict__ cstart, unsigned* __restrict__ ctr) {
+  __shared__ int hist[128];
+  for (int c = threadIdx.x; c < 128; c += blockDim.x) hist[c] = 0;
+  __syncthreads();
+  for (int j = threadIdx.x; j < nm; j += blockDim.x) atomicAdd(&hist[color[j]], 1);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nc = (int)(*maxcol) + 1;
+    int acc = 0;
+    for (int c = 0; c < nc; ++c) {
+      cstart[c] = acc;
+      acc += hist[c];
+    }
+    cstart[nc] = acc;
+    ctr[3] = (unsigned)nc;
+  }
+}
+
+struct SmallOut {
+  int sweeps;
+  int ncolor;
+  double res, res_ref;
+};
+
+__global__ void __launch_bounds__(SMALL_BLOCK) k_psor_small(BodyPtrs B, MfPtrs M, const SysDev* __restrict__ sys,
+                                                            const int* __restrict__ cstart,
+                                                            const unsigned* __restrict__ ctr, int nm, SolveConst K,
+                                                            int warm, double tol, int max_sweeps, int check_every,
+                                                            int stop_rule, SmallOut* __restrict__ out) {
+  __shared__ double red[SMALL_BLOCK / 64];
+  __shared__ int cs[129];
+  const int ncolor = (int)ctr[3];
+  for (int c = threadIdx.x; c <= ncolor; c += blockDim.x) cs[c] = cstart[c];
+  __syncthreads();
+  if (warm) {
+    for (int c = 0; c < ncolor; ++c) {
+      for (int i = cs[c] + threadIdx.x; i < cs[c + 1]; i += blockDim.x) warm_apply_one(B, M, i);
+      __syncthreads();
+    }
+  }
+  int sweeps = 0;
+  double res = 0.0, res_ref = 0.0;
+  const int k = check_every > 0 ? check_every : 1;
+  if (stop_rule == 1) {
+    double r = 0.0;
+    for (int i = threadIdx.x; i < nm; i += blockDim.x) r = dmax(r, residual_one(B, M, i, K, sys[B.sys[M.a[i]]].mu));
+    res = block_max(r, red);
+  }
+  while (sweeps < max_sweeps && !(stop_rule == 1 && res <= tol)) {
+    const int kk = stop_rule == 1 ? k : 1;
+    double dref = 0.0;
+    for (int s = 0; s < kk; ++s) {
+      dref = 0.0;
+      for (int c = 0; c < ncolor; ++c) {
+        for (int i = cs[c] + threadIdx.x; i < cs[c + 1]; i += blockDim.x)
+          dref = dmax(dref, psor_update(B, M, i, K, sys[B.sys[M.a[i]]].mu));
+        __syncthreads();
+      }
+    }
+    sweeps += kk;
+    res_ref = block_max(dref, red);
+    double r = 0.0;
+    for (int i = threadIdx.x; i < nm; i += blockDim.x) r = dmax(r, residual_one(B, M, i, K, sys[B.sys[M.a[i]]].mu));
+    res = block_max(r, red);
+    if (stop_rule == 0 && res_ref < 1e-6) break;
+  }
+  if (threadIdx.x == 0) {
+    out->sweeps = sweeps;
+    out->ncolor = ncolor;
+    out->res = res;
+    out->res_ref = res_ref;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -696,8 +921,10 @@ int nsc_build_run(rdk_ctx* ctx, const rdk_nsc_params* p) {
   }
   if (nwall > 0) {
     ProfScope ps(ctx, K_WALL);
+    const int wwarm = (p->warm_mode != 0 && p->warm_walls) ? 1 : 0;
     hipLaunchKernelGGL(k_wall_manifold, dim3(grid_for(4 * n, nb)), dim3(nb), 0, ctx->stream, B, ctx->sys_dev.p, n,
-                       ctx->wall_flag.p, ctx->wall_off.p, nc, M, p->cfm);
+                       ctx->wall_flag.p, ctx->wall_off.p, nc, M, p->cfm, wwarm, ctx->prev_wkeys.p, ctx->nprev_w,
+                       ctx->prev_wlam[0].p, ctx->prev_wlam[1].p, ctx->prev_wlam[2].p);
   }
   ctx->nm = nm;
   ctx->nm_wall = nwall;
@@ -708,17 +935,31 @@ int nsc_build_run(rdk_ctx* ctx, const rdk_nsc_params* p) {
   return RDK_OK;
 }
 
-int nsc_color_run(rdk_ctx* ctx) {
+static int small_m_limit() {
+  static int lim = -1;
+  if (lim < 0) {
+    const char* e = getenv("RDK_SMALL_M");
+    lim = e ? atoi(e) : 16384;
+  }
+  return lim;
+}
+
+// Colour the manifolds and build the colour-ordered copy.  need_host: also
+// bring the colour offsets to the host (the multi-launch sweep path needs
+// them); the single-workgroup path keeps them on the device.
+int nsc_color_run(rdk_ctx* ctx, bool need_host) {
   const int nm = ctx->nm, n = ctx->n;
   const int nb = 256;
   ctx->ncolor = 0;
   ctx->color_start.assign(1, 0);
+  ctx->color_host_valid = true;
   if (nm == 0) {
     ctx->have_coloring = true;
     return RDK_OK;
   }
   RDK_TRY(ensure(ctx, ctx->body_start, n + 1));
   RDK_TRY(ensure(ctx, ctx->uncolored, 256));
+  RDK_TRY(ensure(ctx, ctx->d_cstart, 132));
   {
     ProfScope ps(ctx, K_CSR);
     hipLaunchKernelGGL(k_csr_entries, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, ctx->ma.p, ctx->mb.p, nm, n,
@@ -734,40 +975,46 @@ int nsc_color_run(rdk_ctx* ctx) {
     hipLaunchKernelGGL(k_body_start, dim3(grid_for(n + 1, nb)), dim3(nb), 0, ctx->stream, skey, 2 * nm, n,
                        ctx->body_start.p);
   }
-  {
-    ProfScope ps(ctx, K_COLOR);
-    hipLaunchKernelGGL(k_fill_color, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, ctx->color.p, nm, -1);
-  }
-  unsigned* dctr = (unsigned*)ctx->uncolored.p;  // [0] left, [1] maxcol, [2] overflow
+  unsigned* dctr = (unsigned*)ctx->uncolored.p;  // [0] left, [1] maxcol, [2] overflow, [3] ncolor
   RDK_HIP(hipMemsetAsync(dctr, 0, 4 * sizeof(unsigned), ctx->stream));
-  int rounds = 0;
-  unsigned hc[4];
-  for (;;) {
-    RDK_HIP(hipMemsetAsync(dctr, 0, sizeof(unsigned), ctx->stream));
+  const bool small = nm <= small_m_limit();
+  if (small) {
+    ProfScope ps(ctx, K_COLOR);
+    hipLaunchKernelGGL(k_color_small, dim3(1), dim3(SMALL_BLOCK), 0, ctx->stream, ctx->ma.p, ctx->mb.p, nm,
+                       ctx->body_start.p, csr, ctx->color.p, ctx->color_tmp.p, dctr);
+  } else {
     {
       ProfScope ps(ctx, K_COLOR);
-      hipLaunchKernelGGL(k_color_assign, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, ctx->ma.p, ctx->mb.p, nm,
-                         ctx->body_start.p, csr, ctx->color.p, ctx->color_tmp.p, dctr + 2);
-      hipLaunchKernelGGL(k_color_resolve, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, ctx->ma.p, ctx->mb.p, nm,
-                         ctx->body_start.p, csr, ctx->color_tmp.p, ctx->color.p, dctr, dctr + 1);
+      hipLaunchKernelGGL(k_fill_color, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, ctx->color.p, nm, -1);
     }
-    ++rounds;
-    RDK_HIP(hipMemcpyAsync(hc, dctr, 4 * sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
-    RDK_HIP(hipStreamSynchronize(ctx->stream));
-    if (hc[2] != 0) {
-      ctx->err = "nsc colouring: more than 127 colours needed";
-      return RDK_ERR_CAPACITY;
-    }
-    if (hc[0] == 0) break;
-    if (rounds > 4096) {
-      ctx->err = "nsc colouring did not terminate";
-      return RDK_ERR_STATE;
+    int rounds = 0;
+    unsigned hc[4];
+    for (;;) {
+      RDK_HIP(hipMemsetAsync(dctr, 0, sizeof(unsigned), ctx->stream));
+      {
+        ProfScope ps(ctx, K_COLOR);
+        hipLaunchKernelGGL(k_color_assign, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, ctx->ma.p, ctx->mb.p,
+                           nm, ctx->body_start.p, csr, ctx->color.p, ctx->color_tmp.p, dctr + 2);
+        hipLaunchKernelGGL(k_color_resolve, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, ctx->ma.p, ctx->mb.p,
+                           nm, ctx->body_start.p, csr, ctx->color_tmp.p, ctx->color.p, dctr, dctr + 1);
+      }
+      ++rounds;
+      RDK_HIP(hipMemcpyAsync(hc, dctr, 4 * sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+      RDK_HIP(hipStreamSynchronize(ctx->stream));
+      if (hc[2] != 0) {
+        ctx->err = "nsc colouring: more than 127 colours needed";
+        return RDK_ERR_CAPACITY;
+      }
+      if (hc[0] == 0) break;
+      if (rounds > 4096) {
+        ctx->err = "nsc colouring did not terminate";
+        return RDK_ERR_STATE;
+      }
     }
   }
-  const int ncolor = (int)hc[1] + 1;
   // group by colour (stable: natural order inside a colour)
   RDK_TRY(ensure(ctx, ctx->wall_off, std::max<size_t>(ctx->wall_off.cap, 128)));
-  int* hist = ctx->wall_off.p;  // reuse as 128-bin histogram
+  int* hist = ctx->wall_off.p;  // scratch 128-bin histogram
   RDK_HIP(hipMemsetAsync(hist, 0, 128 * sizeof(int), ctx->stream));
   {
     ProfScope ps(ctx, K_COLOR);
@@ -775,20 +1022,44 @@ int nsc_color_run(rdk_ctx* ctx) {
                        ctx->color_key.p, ctx->perm_alt.p, hist);
   }
   RDK_TRY(sort_pairs_u32(ctx, ctx->color_key.p, ctx->color_key_alt.p, ctx->perm_alt.p, ctx->perm.p, nm, 7, alt));
-  if (!alt) RDK_HIP(hipMemcpyAsync(ctx->perm.p, ctx->perm_alt.p, sizeof(int) * nm, hipMemcpyDeviceToDevice, ctx->stream));
-  std::vector<int> h(128);
-  RDK_HIP(hipMemcpyAsync(h.data(), hist, 128 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-  RDK_HIP(hipStreamSynchronize(ctx->stream));
-  ctx->color_start.assign(ncolor + 1, 0);
-  for (int c = 0; c < ncolor; ++c) ctx->color_start[c + 1] = ctx->color_start[c] + h[c];
-  ctx->ncolor = ncolor;
-  // colour-ordered manifold copy
+  if (!alt)
+    RDK_HIP(hipMemcpyAsync(ctx->perm.p, ctx->perm_alt.p, sizeof(int) * nm, hipMemcpyDeviceToDevice, ctx->stream));
+  {
+    ProfScope ps(ctx, K_COLOR);
+    hipLaunchKernelGGL(k_color_offsets, dim3(1), dim3(256), 0, ctx->stream, ctx->color.p, nm, dctr + 1,
+                       ctx->d_cstart.p, dctr);
+  }
   {
     ProfScope ps(ctx, K_PERMUTE);
     hipLaunchKernelGGL(k_permute, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, mf_ptrs(ctx, false),
                        mf_ptrs(ctx, true), ctx->perm.p, nm);
   }
   ctx->have_coloring = true;
+  ctx->color_host_valid = false;
+  if (need_host || !small) RDK_TRY(color_to_host(ctx));
+  return RDK_OK;
+}
+
+int color_to_host(rdk_ctx* ctx) {
+  if (ctx->color_host_valid) return RDK_OK;
+  unsigned hc[4];
+  RDK_HIP(hipMemcpyAsync(hc, ctx->uncolored.p, 4 * sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+  RDK_HIP(hipStreamSynchronize(ctx->stream));
+  if (hc[2] != 0) {
+    ctx->err = "nsc colouring: more than 127 colours needed";
+    return RDK_ERR_CAPACITY;
+  }
+  if (hc[0] != 0) {
+    ctx->err = "nsc colouring did not terminate";
+    return RDK_ERR_STATE;
+  }
+  const int ncolor = (int)hc[3];
+  ctx->color_start.assign(ncolor + 1, 0);
+  RDK_HIP(hipMemcpyAsync(ctx->color_start.data(), ctx->d_cstart.p, (ncolor + 1) * sizeof(int), hipMemcpyDeviceToHost,
+                         ctx->stream));
+  RDK_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->ncolor = ncolor;
+  ctx->color_host_valid = true;
   return RDK_OK;
 }
 
@@ -848,7 +1119,8 @@ int nsc_sweeps_run(rdk_ctx* ctx, const rdk_nsc_params* p, int nsweeps) {
     return RDK_ERR_STATE;
   }
   RDK_TRY(ensure(ctx, ctx->resid, 4));
-  if (!ctx->have_coloring) RDK_TRY(nsc_color_run(ctx));
+  if (!ctx->have_coloring) RDK_TRY(nsc_color_run(ctx, true));
+  RDK_TRY(color_to_host(ctx));
   RDK_TRY(run_sweeps(ctx, p, nsweeps));
   const int nb = 256;
   if (ctx->nm > 0) {
@@ -866,10 +1138,49 @@ int nsc_solve_run(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* st) {
   }
   RDK_TRY(ensure(ctx, ctx->resid, 4));
   const int nm = ctx->nm, nb = 256;
-  RDK_TRY(nsc_color_run(ctx));
+  const bool small = nm > 0 && nm <= small_m_limit();
+  RDK_TRY(nsc_color_run(ctx, false));
   BodyPtrs B = body_ptrs(ctx);
   MfPtrs Mc = mf_ptrs(ctx, true);
-  if (p->warm_mode != 0) {
+  int sweeps = 0;
+  double res = 0.0, res_ref = 0.0;
+  int status = RDK_OK;
+  if (small) {
+    RDK_TRY(ensure(ctx, ctx->small_out, sizeof(SmallOut)));
+    {
+      ProfScope ps(ctx, K_PSOR_SWEEP);
+      hipLaunchKernelGGL(k_psor_small, dim3(1), dim3(SMALL_BLOCK), 0, ctx->stream, B, Mc, ctx->sys_dev.p,
+                         ctx->d_cstart.p, (const unsigned*)ctx->uncolored.p, nm, solve_const(ctx, p),
+                         p->warm_mode != 0 ? 1 : 0, p->tol, p->max_sweeps, std::max(1, p->check_every), p->stop_rule,
+                         (SmallOut*)ctx->small_out.p);
+    }
+    SmallOut so;
+    unsigned hc[4];
+    RDK_HIP(hipMemcpyAsync(&so, ctx->small_out.p, sizeof(SmallOut), hipMemcpyDeviceToHost, ctx->stream));
+    RDK_HIP(hipMemcpyAsync(hc, ctx->uncolored.p, 4 * sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+    RDK_HIP(hipStreamSynchronize(ctx->stream));
+    if (hc[2] != 0) {
+      ctx->err = "nsc colouring: more than 127 colours needed";
+      return RDK_ERR_CAPACITY;
+    }
+    if (hc[0] != 0) {
+      ctx->err = "nsc colouring did not terminate";
+      return RDK_ERR_STATE;
+    }
+    ctx->ncolor = so.ncolor;
+    sweeps = so.sweeps;
+    res = so.res;
+    res_ref = so.res_ref;
+    ctx->work[W_SWEEP_CONTACTS] += (int64_t)sweeps * nm;
+    ctx->work[W_RESID_CONTACTS] += (int64_t)nm * (1 + (p->stop_rule == 1 ? sweeps / std::max(1, p->check_every) : sweeps));
+    if (p->stop_rule == 1 && !(res <= p->tol)) status = RDK_ERR_NOCONV;
+    ProfScope ps(ctx, K_PERMUTE);
+    hipLaunchKernelGGL(k_unpermute_lambda, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, Mc, mf_ptrs(ctx, false),
+                       ctx->perm.p, nm);
+  } else if (nm > 0) {
+    RDK_TRY(color_to_host(ctx));
+  }
+  if (!small && p->warm_mode != 0) {
     for (int c = 0; c < ctx->ncolor; ++c) {
       const int i0 = ctx->color_start[c], i1 = ctx->color_start[c + 1];
       if (i1 <= i0) continue;
@@ -877,10 +1188,7 @@ int nsc_solve_run(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* st) {
       hipLaunchKernelGGL(k_warm_apply, dim3(grid_for(i1 - i0, nb)), dim3(nb), 0, ctx->stream, B, Mc, i0, i1);
     }
   }
-  int sweeps = 0;
-  double res = 0.0, res_ref = 0.0;
-  int status = RDK_OK;
-  if (nm > 0) {
+  if (!small && nm > 0) {
     if (p->stop_rule == 0) {
       for (int it = 0; it < p->max_sweeps; ++it) {
         RDK_TRY(run_sweeps(ctx, p, 1));
@@ -914,6 +1222,17 @@ int nsc_solve_run(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* st) {
     ctx->nprev = nc;
   } else {
     ctx->nprev = 0;
+  }
+  const int nw = ctx->nm_wall;
+  if (p->warm_mode != 0 && p->warm_walls && nw > 0) {
+    RDK_TRY(ensure(ctx, ctx->prev_wkeys, nw));
+    for (int k = 0; k < 3; ++k) RDK_TRY(ensure(ctx, ctx->prev_wlam[k], nw));
+    ProfScope ps(ctx, K_MISC);
+    hipLaunchKernelGGL(k_copy_wall_keys_lambda, dim3(grid_for(nw, nb)), dim3(nb), 0, ctx->stream, mf_ptrs(ctx, false),
+                       nc, nw, ctx->prev_wkeys.p, ctx->prev_wlam[0].p, ctx->prev_wlam[1].p, ctx->prev_wlam[2].p);
+    ctx->nprev_w = nw;
+  } else {
+    ctx->nprev_w = 0;
   }
   if (st) {
     st->manifolds = nm;

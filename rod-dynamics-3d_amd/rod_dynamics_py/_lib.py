@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "librdk.so")
 
 RDK_OK, RDK_ERR_HIP, RDK_ERR_ARG, RDK_ERR_CAPACITY, RDK_ERR_STATE, RDK_ERR_NOCONV = range(6)
 RDK_NSC_FREE, RDK_NSC_POS, RDK_VV_HALF1, RDK_VV_HALF2, RDK_CLEAR = range(5)
+MF_FIELDS = 42  # RDK_MF_FIELDS
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -35,7 +36,8 @@ class DetectParams(C.Structure):
 class NscParams(C.Structure):
     _fields_ = [("beta", C.c_double), ("cfm", C.c_double), ("omega", C.c_double), ("tol", C.c_double),
                 ("max_sweeps", C.c_int), ("check_every", C.c_int), ("friction_proj", C.c_int),
-                ("position_stabilization", C.c_int), ("warm_mode", C.c_int), ("stop_rule", C.c_int)]
+                ("position_stabilization", C.c_int), ("warm_mode", C.c_int), ("stop_rule", C.c_int),
+                ("warm_walls", C.c_int)]
 
 
 class SoftParams(C.Structure):
@@ -157,8 +159,9 @@ def system_desc(lo, hi, periodic, floor_wall=False, ceil_wall=False, mu=0.0):
 
 
 def nsc_params(beta=0.0, cfm=0.0, omega=1.0, tol=1e-8, max_sweeps=10000, check_every=1, friction_proj=0,
-               position_stabilization=0, warm_mode=1, stop_rule=1):
+               position_stabilization=0, warm_mode=1, stop_rule=1, warm_walls=0):
     p = NscParams()
+    p.warm_walls = warm_walls
     p.beta, p.cfm, p.omega, p.tol = beta, cfm, omega, tol
     p.max_sweeps, p.check_every, p.friction_proj = max_sweeps, check_every, friction_proj
     p.position_stabilization, p.warm_mode, p.stop_rule = position_stabilization, warm_mode, stop_rule
@@ -299,7 +302,7 @@ class Context:
         return nm.value
 
     def nsc_manifolds(self, nm):
-        rec = np.zeros((nm, 41))
+        rec = np.zeros((nm, MF_FIELDS))
         ab = np.zeros((nm, 2), np.int32)
         lam = np.zeros((nm, 3))
         self._check(self._L.rdk_nsc_get_manifolds(self._h, nm, dptr(rec), iptr(ab), dptr(lam)))
