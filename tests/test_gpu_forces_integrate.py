@@ -108,7 +108,8 @@ def test_integrator_stages_match_oracle(gpu_ctx_factory, stage):
         assert np.array_equal(sg["x"], so["x"])  # position update and wrap: same operations
 
 
-def test_nsc_steps_match_oracle_short_horizon(gpu_ctx_factory):
+@pytest.mark.parametrize("warm_walls", [0, 1])
+def test_nsc_steps_match_oracle_short_horizon(gpu_ctx_factory, warm_walls):
     sc = scenes.c2_settling(n=1500)
     v0 = np.tile([0.0, 0.0, -5.0], (sc.n, 1))
     ctx = gpu_ctx_factory()
@@ -117,11 +118,13 @@ def test_nsc_steps_match_oracle_short_horizon(gpu_ctx_factory):
     p = sc.params
     p.tol = 1e-12
     p.check_every = 1
+    p.warm_walls = warm_walls
     st = ctx.step_nsc(sc.detect, p, 30)
     assert st["nonconverged_steps"] == 0 and st["manifolds_total"] > 0
     W = O.world_from_scene(sc)
     W.set_state(v=v0)
-    op = O.nsc_params(mu=0.5, beta=0.0, cfm=0.0, omega=1.3, tol=1e-12, friction_proj=1, warm_start=1)
+    op = O.nsc_params(mu=0.5, beta=0.0, cfm=0.0, omega=1.3, tol=1e-12, friction_proj=1, warm_start=1,
+                      warm_walls=warm_walls)
     W.nsc_step(op, 30, method=1)
     sg, so = ctx.download_state(), W.state()
     # colour-order vs sorted-order Gauss-Seidel: equal at convergence up to the
