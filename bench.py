@@ -42,7 +42,7 @@ def parse():
     ap.add_argument("--n", type=int, default=100_000)
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--chunk-len", type=float, default=0.0)
+    ap.add_argument("--chunk-len", type=float, default=4.0)
     ap.add_argument("--check-every", type=int, default=4)
     return ap.parse_args()
 
@@ -125,7 +125,8 @@ def main():
                    "box": round(sc.extra["side"], 3), "periodic": "x,y", "walls": "floor+ceiling", "mu": 0.5,
                    "restitution": 0.0, "omega": 1.3, "residual_tol": 1e-8, "dt": 1e-3,
                    "friction_projection": "euclidean disc" if p.friction_proj == 1 else "reference sequential",
-                   "steps_total": a.warmup + a.steps, "seed": sc.extra["seed"], "check_every": a.check_every},
+                   "steps_total": a.warmup + a.steps, "seed": sc.extra["seed"], "check_every": a.check_every,
+                   "chunk_len": a.chunk_len},
         "psor": {"sweeps_per_step": st["sweeps_total"] / max(st["steps"], 1),
                  "manifolds_per_step": st["manifolds_total"] / max(st["steps"], 1),
                  "rod_contacts_per_step": st["contacts_total"] / max(st["steps"], 1),

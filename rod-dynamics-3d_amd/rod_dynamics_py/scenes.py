@@ -84,7 +84,7 @@ def c2_settling(n=100_000, phi=0.05, seed=SEED0 + 2, mu=0.5, aspect=20.0, gap=0.
     return Scene(name=f"C2_settling_N{n}", x=x, q=q, v=np.zeros((n, 3)), w=np.zeros((n, 3)), r=r, h=h, mass=mass,
                  rest=np.zeros(n), offsets=np.array([0, n], np.int32), descs=[desc],
                  phys=L.physics((0.0, 0.0, -1.0), 1e-3, 0.08, 0.12, 80.0), model="nsc", params=nscp,
-                 detect=L.detect_params(0.0, 0.0, 1.0), extra=dict(phi=phi, side=side, mu=mu, seed=seed))
+                 detect=L.detect_params(0.0, 4.0, 1.0), extra=dict(phi=phi, side=side, mu=mu, seed=seed))
 
 
 def c1_thermal(n=1000, phi=0.10, seed=SEED0 + 1, model="hm", aspect=10.0, kT=1.0):
