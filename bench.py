@@ -44,6 +44,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chunk-len", type=float, default=4.0)
     ap.add_argument("--check-every", type=int, default=4)
+    ap.add_argument("--block", type=int, default=250)
     return ap.parse_args()
 
 
@@ -96,7 +97,19 @@ def main():
     print(f"[bench] warmup {a.warmup} steps done; dominant kernel {dom}; timing {a.steps} steps", file=sys.stderr,
           flush=True)
     t1 = time.perf_counter()
-    st = ctx.step_nsc(det, p, a.steps)
+    st = None
+    done = 0
+    while done < a.steps:  # blocks of steps, one progress line each (stderr)
+        k = min(a.block, a.steps - done)
+        s_ = ctx.step_nsc(det, p, k)
+        done += k
+        if st is None:
+            st = dict(s_)
+        else:
+            for key in st:
+                st[key] = max(st[key], s_[key]) if key == "residual_max" else st[key] + s_[key]
+        print(f"[bench] step {a.warmup + done}: {time.perf_counter() - t1:.2f}s, manifolds/step "
+              f"{s_['manifolds_total'] / k:.0f}, sweeps/step {s_['sweeps_total'] / k:.1f}", file=sys.stderr, flush=True)
     ctx.sync()
     el = time.perf_counter() - t1
     kt = ctx.kernel_times()

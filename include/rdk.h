@@ -164,6 +164,14 @@ int rdk_nsc_sweeps(rdk_ctx* ctx, const rdk_nsc_params* p, int nsweeps);
 int rdk_nsc_get_coloring(rdk_ctx* ctx, int cap, int* color, int* order,
                          int* n_colors);
 int rdk_nsc_residual(rdk_ctx* ctx, const rdk_nsc_params* p, double* res);
+/* 0 (default): solve each island (connected component of the rod-rod contact
+ * graph) independently on the GPU whenever every island fits one workgroup;
+ * 1: always sweep all manifolds globally, colour by colour */
+int rdk_set_solver_mode(rdk_ctx* ctx, int mode);
+/* after rdk_nsc_solve: islands and their split (tiny, medium, large), and the
+ * manifold order the last solve swept in (natural indices) */
+int rdk_nsc_island_info(rdk_ctx* ctx, int* n_islands, int* counts3);
+int rdk_nsc_get_solve_order(rdk_ctx* ctx, int cap, int* order);
 
 /* ---- integrators (integrator.cpp:56-159, main.cpp:4397-4483) ----------- */
 enum rdk_stage {

@@ -70,7 +70,7 @@ static const char* kKernelNames[K_NUM] = {
     "chunk_keys", "cell_ranges", "chunk_gather", "broad_narrow", "contact_record", "sort_scan",
     "manifold", "wall", "csr", "color", "permute", "warm_apply", "psor_sweep", "residual",
     "nsc_free", "nsc_pos", "vv_half1", "vv_half2", "force", "body_reduce", "history", "observe",
-    "mlp", "misc"};
+    "mlp", "island", "misc"};
 
 }  // namespace rdk
 
@@ -132,7 +132,10 @@ int rdk_destroy(rdk_ctx* ctx) {
   release(ctx->wall_flag); release(ctx->wall_off); release(ctx->color); release(ctx->color_tmp);
   release(ctx->perm); release(ctx->perm_alt); release(ctx->uncolored); release(ctx->color_key);
   release(ctx->color_key_alt); release(ctx->csr_key); release(ctx->csr_key_alt); release(ctx->csr_val);
-  release(ctx->csr_val_alt); release(ctx->body_start); release(ctx->d_cstart); release(ctx->small_out); release(ctx->prev_keys); release(ctx->prev_wkeys); release(ctx->resid);
+  release(ctx->csr_val_alt); release(ctx->body_start); release(ctx->d_cstart); release(ctx->small_out);
+  release(ctx->uf_parent); release(ctx->uf_label); release(ctx->sbody); release(ctx->sbody_alt);
+  release(ctx->iflag); release(ctx->ipos); release(ctx->istart); release(ctx->ilists); release(ctx->bseg);
+  release(ctx->icnt); release(ctx->blab); release(ctx->blab_alt); release(ctx->istats); release(ctx->prev_keys); release(ctx->prev_wkeys); release(ctx->resid);
   release(ctx->hkeys); release(ctx->hkeys_new); release(ctx->hlast); release(ctx->hlast_new);
   release(ctx->hflag); release(ctx->hscan); release(ctx->pe); release(ctx->temp); release(ctx->red);
   if (ctx->h_counters) (void)hipHostFree(ctx->h_counters);
@@ -534,6 +537,29 @@ int rdk_nsc_residual(rdk_ctx* ctx, const rdk_nsc_params* p, double* res) {
   // residual runs on the colour-ordered copy: refresh its lambda
   RDK_TRY(nsc_residual_run(ctx, p, res, nullptr));
   return prof_flush(ctx);
+}
+
+int rdk_set_solver_mode(rdk_ctx* ctx, int mode) {
+  if (!ctx || mode < 0 || mode > 1) return RDK_ERR_ARG;
+  ctx->solver_mode = mode;
+  return RDK_OK;
+}
+
+int rdk_nsc_island_info(rdk_ctx* ctx, int* n_islands, int* counts3) {
+  if (!ctx) return RDK_ERR_ARG;
+  if (n_islands) *n_islands = ctx->nislands;
+  if (counts3)
+    for (int k = 0; k < 3; ++k) counts3[k] = ctx->island_counts[k];
+  return RDK_OK;
+}
+
+int rdk_nsc_get_solve_order(rdk_ctx* ctx, int cap, int* order) {
+  CTX_CHECK();
+  if (!order) return RDK_ERR_ARG;
+  const int n = std::min(cap, ctx->nm);
+  RDK_TRY(dl(ctx, order, ctx->perm.p, n));
+  RDK_HIP(hipStreamSynchronize(ctx->stream));
+  return RDK_OK;
 }
 
 int rdk_integrate(rdk_ctx* ctx, int stage) {

@@ -37,6 +37,7 @@ enum KernelId {
   K_HISTORY,
   K_OBSERVE,
   K_MLP,
+  K_ISLAND,
   K_MISC,
   K_NUM
 };
@@ -142,6 +143,13 @@ struct rdk_ctx {
   int ncolor = 0;
   bool have_manifolds = false, have_coloring = false, color_host_valid = false;
   rdk::DevBuf<int> d_cstart;
+  // islands
+  int solver_mode = 0;  // 0 islands when possible, 1 always global coloured
+  int nislands = 0;
+  int island_counts[3] = {0, 0, 0};
+  rdk::DevBuf<int> uf_parent, uf_label, sbody, sbody_alt, iflag, ipos, istart, ilists, bseg, icnt;
+  rdk::DevBuf<uint32_t> blab, blab_alt;
+  rdk::DevBuf<char> istats;
   rdk::DevBuf<char> small_out;
   // body -> manifold CSR
   rdk::DevBuf<uint32_t> csr_key, csr_key_alt;

@@ -819,6 +819,343 @@ __global__ void __launch_bounds__(SMALL_BLOCK) k_psor_small(BodyPtrs B, MfPtrs M
   }
 }
 
+// ---- island-parallel PSOR ---------------------------------------------------
+// Rod-rod manifolds join bodies into islands (connected components; wall
+// manifolds join nothing).  Islands share no body, so Gauss-Seidel on each
+// island is independent of the others and the per-island sweeps in colour
+// order are exactly the global serial sweep restricted to that island.  Each
+// island stops at its own residual.  Tiny islands: one lane, serial.  Medium
+// islands: one 64-lane workgroup with manifolds and body velocities staged in
+// LDS.  Large islands: one 256-lane workgroup on global memory.
+
+__device__ __forceinline__ int uf_load(int* p, int x) {
+  return __hip_atomic_load(&p[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int uf_find(int* parent, int x) {
+  for (;;) {
+    const int p = uf_load(parent, x);
+    if (p == x) return x;
+    const int gp = uf_load(parent, p);
+    if (gp != p) atomicCAS(&parent[x], p, gp);  // path halving
+    x = gp;
+  }
+}
+
+__global__ void k_uf_init(int* __restrict__ parent, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) parent[i] = i;
+}
+
+// link the higher root under the lower one: every root ends as the smallest
+// body index of its island, so labels are deterministic
+__global__ void k_uf_link(const int* __restrict__ ma, const int* __restrict__ mb, int nm, int* parent) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nm) return;
+  const int a = ma[m], b = mb[m];
+  if (b < 0) return;
+  for (int guard = 0; guard < (1 << 20); ++guard) {
+    const int ra = uf_find(parent, a), rb = uf_find(parent, b);
+    if (ra == rb) return;
+    const int hi = ra > rb ? ra : rb, lo = ra > rb ? rb : ra;
+    if (atomicCAS(&parent[hi], hi, lo) == hi) return;
+  }
+}
+
+__global__ void k_uf_flatten(int* parent, int n, int* __restrict__ label) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) label[i] = uf_find(parent, i);
+}
+
+__global__ void k_island_keys(const int* __restrict__ ma, const int* __restrict__ color, int nm,
+                              const int* __restrict__ label, uint32_t* __restrict__ key, int* __restrict__ val) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nm) return;
+  key[m] = ((uint32_t)label[ma[m]] << 7) | (uint32_t)color[m];
+  val[m] = m;
+}
+
+__global__ void k_body_label_keys(const int* __restrict__ label, int n, uint32_t* __restrict__ key,
+                                  int* __restrict__ val) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  key[i] = (uint32_t)label[i];
+  val[i] = i;
+}
+
+__global__ void k_island_flags(const uint32_t* __restrict__ skey, int nm, int* __restrict__ flag) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nm) return;
+  flag[i] = (i == 0 || (skey[i] >> 7) != (skey[i - 1] >> 7)) ? 1 : 0;
+}
+
+__global__ void k_island_starts(const int* __restrict__ flag, const int* __restrict__ pos, int nm,
+                                int* __restrict__ istart) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nm) return;
+  if (flag[i]) istart[pos[i]] = i;
+  if (i == nm - 1) istart[pos[i] + flag[i]] = nm;
+}
+
+__device__ __forceinline__ int lb_u32(const uint32_t* a, int lo, int hi, uint32_t key) {
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+struct IslandClass {
+  int tiny_max, med_max, med_bmax, large_max;
+};
+
+// island list per class; counts in cnt[0..3] (tiny, medium, large, giant)
+__global__ void k_island_classify(const int* __restrict__ istart, int nisl, const uint32_t* __restrict__ skey,
+                                  const uint32_t* __restrict__ blab, int nbody, IslandClass C,
+                                  int* __restrict__ lists, int* __restrict__ cnt, int* __restrict__ bseg) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nisl) return;
+  const int s = istart[k], e = istart[k + 1];
+  const uint32_t L = skey[s] >> 7;
+  const int bs = lb_u32(blab, 0, nbody, L), be = lb_u32(blab, bs, nbody, L + 1);
+  bseg[2 * k] = bs;
+  bseg[2 * k + 1] = be;
+  const int size = e - s, nb = be - bs;
+  int cls;
+  if (size <= C.tiny_max) cls = 0;
+  else if (size <= C.med_max && nb <= C.med_bmax) cls = 1;
+  else if (size <= C.large_max) cls = 2;
+  else cls = 3;
+  const int slot = atomicAdd(&cnt[cls], 1);
+  lists[cls * nisl + slot] = k;
+}
+
+struct IslandStats {
+  int max_sweeps, nonconv, pad0, pad1;
+  unsigned long long manifold_sweeps;
+  double res, res_ref;
+};
+
+__device__ __forceinline__ void island_report(IslandStats* S, int sweeps, int size, double res, double res_ref,
+                                              bool nonconv) {
+  atomicMax(&S->max_sweeps, sweeps);
+  atomicAdd(&S->manifold_sweeps, (unsigned long long)sweeps * (unsigned long long)size);
+  if (res > 0.0 || res != res) atomic_max_pos(&S->res, res != res ? __longlong_as_double(0x7ff8000000000000ll) : res);
+  if (res_ref > 0.0) atomic_max_pos(&S->res_ref, res_ref);
+  if (nonconv) atomicAdd(&S->nonconv, 1);
+}
+
+struct IslandSolve {
+  SolveConst K;
+  double tol;
+  int max_sweeps, check_every, stop_rule, warm;
+};
+
+// tiny islands: one lane, serial sweeps in island (colour) order
+__global__ void k_island_tiny(BodyPtrs B, MfPtrs M, const SysDev* __restrict__ sys, const int* __restrict__ istart,
+                              const int* __restrict__ list, int count, IslandSolve P, IslandStats* S) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  const int k = list[t];
+  const int s = istart[k], e = istart[k + 1];
+  const double mu = sys[B.sys[M.a[s]]].mu;
+  if (P.warm)
+    for (int i = s; i < e; ++i) warm_apply_one(B, M, i);
+  int sweeps = 0;
+  double res = 0.0, res_ref = 0.0;
+  if (P.stop_rule == 1)
+    for (int i = s; i < e; ++i) res = dmax(res, residual_one(B, M, i, P.K, mu));
+  const int kk = P.stop_rule == 1 ? P.check_every : 1;
+  while (sweeps < P.max_sweeps && !(P.stop_rule == 1 && res <= P.tol)) {
+    for (int it = 0; it < kk; ++it) {
+      res_ref = 0.0;
+      for (int i = s; i < e; ++i) res_ref = dmax(res_ref, psor_update(B, M, i, P.K, mu));
+    }
+    sweeps += kk;
+    res = 0.0;
+    for (int i = s; i < e; ++i) res = dmax(res, residual_one(B, M, i, P.K, mu));
+    if (P.stop_rule == 0 && res_ref < 1e-6) break;
+  }
+  island_report(S, sweeps, e - s, res, res_ref, P.stop_rule == 1 && !(res <= P.tol));
+}
+
+constexpr int MED_THREADS = 64;
+constexpr int MED_CAP = 128;   // manifolds staged in LDS
+constexpr int MED_BCAP = 160;  // bodies staged in LDS
+constexpr int LARGE_THREADS = 256;
+
+__device__ __forceinline__ double wave_max64(double v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_down(v, off);
+    v = (o != o || o > v) ? o : v;
+  }
+  return __shfl(v, 0);
+}
+
+// medium islands: one 64-lane workgroup, island data staged in LDS
+__global__ void __launch_bounds__(MED_THREADS) k_island_medium(BodyPtrs B, MfPtrs M, const SysDev* __restrict__ sys,
+                                                              const int* __restrict__ istart,
+                                                              const uint32_t* __restrict__ skey,
+                                                              const int* __restrict__ bseg,
+                                                              const int* __restrict__ sbody,
+                                                              const int* __restrict__ list, IslandSolve P,
+                                                              IslandStats* S) {
+  __shared__ double f_l[F_COUNT * MED_CAP];
+  __shared__ double lam_l[3 * MED_CAP];
+  __shared__ int a_l[MED_CAP], b_l[MED_CAP];
+  __shared__ double v_l[3 * MED_BCAP], w_l[3 * MED_BCAP];
+  __shared__ int body_l[MED_BCAP];
+  __shared__ int cs[129];
+  const int k = list[blockIdx.x];
+  const int s = istart[k], e = istart[k + 1], n = e - s;
+  const int bs = bseg[2 * k], nb = bseg[2 * k + 1] - bs;
+  const int tid = threadIdx.x;
+  for (int j = tid; j < nb; j += MED_THREADS) {
+    const int g = sbody[bs + j];
+    body_l[j] = g;
+    for (int c = 0; c < 3; ++c) {
+      v_l[c * MED_BCAP + j] = B.v[c][g];
+      w_l[c * MED_BCAP + j] = B.w[c][g];
+    }
+  }
+  __syncthreads();
+  for (int j = tid; j < n; j += MED_THREADS) {
+    const int i = s + j;
+    for (int f = 0; f < F_COUNT; ++f) f_l[f * MED_CAP + j] = M.at(f, i);
+    for (int c = 0; c < 3; ++c) lam_l[c * MED_CAP + j] = M.lam[c][i];
+    // local body slots (the island's bodies are sorted by global index)
+    int lo = 0, hi = nb;
+    const int ga = M.a[i];
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (body_l[mid] < ga) lo = mid + 1;
+      else hi = mid;
+    }
+    a_l[j] = lo;
+    const int gb = M.b[i];
+    if (gb >= 0) {
+      lo = 0;
+      hi = nb;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (body_l[mid] < gb) lo = mid + 1;
+        else hi = mid;
+      }
+      b_l[j] = lo;
+    } else {
+      b_l[j] = -1;
+    }
+  }
+  // colour boundaries inside the island (manifolds are sorted by colour)
+  const uint32_t base = skey[s] & ~127u;
+  const int ncol = (int)(skey[e - 1] & 127u) + 1;
+  for (int c = tid; c <= ncol; c += MED_THREADS) cs[c] = lb_u32(skey, s, e, base | (uint32_t)c) - s;
+  __syncthreads();
+  MfPtrs L;
+  L.f = f_l;
+  L.stride = MED_CAP;
+  L.a = a_l;
+  L.b = b_l;
+  for (int c = 0; c < 3; ++c) L.lam[c] = lam_l + c * MED_CAP;
+  BodyPtrs LB = B;
+  for (int c = 0; c < 3; ++c) {
+    LB.v[c] = v_l + c * MED_BCAP;
+    LB.w[c] = w_l + c * MED_BCAP;
+  }
+  const double mu = sys[B.sys[M.a[s]]].mu;
+  if (P.warm) {
+    for (int c = 0; c < ncol; ++c) {
+      for (int j = cs[c] + tid; j < cs[c + 1]; j += MED_THREADS) warm_apply_one(LB, L, j);
+      __syncthreads();
+    }
+  }
+  int sweeps = 0;
+  double res = 0.0, res_ref = 0.0;
+  if (P.stop_rule == 1) {
+    double r = 0.0;
+    for (int j = tid; j < n; j += MED_THREADS) r = dmax(r, residual_one(LB, L, j, P.K, mu));
+    res = wave_max64(r);
+  }
+  const int kk = P.stop_rule == 1 ? P.check_every : 1;
+  while (sweeps < P.max_sweeps && !(P.stop_rule == 1 && res <= P.tol)) {
+    double d = 0.0;
+    for (int it = 0; it < kk; ++it) {
+      d = 0.0;
+      for (int c = 0; c < ncol; ++c) {
+        for (int j = cs[c] + tid; j < cs[c + 1]; j += MED_THREADS) d = dmax(d, psor_update(LB, L, j, P.K, mu));
+        __syncthreads();
+      }
+    }
+    sweeps += kk;
+    res_ref = wave_max64(d);
+    double r = 0.0;
+    for (int j = tid; j < n; j += MED_THREADS) r = dmax(r, residual_one(LB, L, j, P.K, mu));
+    res = wave_max64(r);
+    if (P.stop_rule == 0 && res_ref < 1e-6) break;
+  }
+  __syncthreads();
+  for (int j = tid; j < n; j += MED_THREADS)
+    for (int c = 0; c < 3; ++c) M.lam[c][s + j] = lam_l[c * MED_CAP + j];
+  for (int j = tid; j < nb; j += MED_THREADS) {
+    const int g = body_l[j];
+    for (int c = 0; c < 3; ++c) {
+      B.v[c][g] = v_l[c * MED_BCAP + j];
+      B.w[c][g] = w_l[c * MED_BCAP + j];
+    }
+  }
+  if (tid == 0) island_report(S, sweeps, n, res, res_ref, P.stop_rule == 1 && !(res <= P.tol));
+}
+
+// large islands: one 256-lane workgroup on global memory
+__global__ void __launch_bounds__(LARGE_THREADS) k_island_large(BodyPtrs B, MfPtrs M, const SysDev* __restrict__ sys,
+                                                               const int* __restrict__ istart,
+                                                               const uint32_t* __restrict__ skey,
+                                                               const int* __restrict__ list, IslandSolve P,
+                                                               IslandStats* S) {
+  __shared__ double red[LARGE_THREADS / 64];
+  __shared__ int cs[129];
+  const int k = list[blockIdx.x];
+  const int s = istart[k], e = istart[k + 1];
+  const int tid = threadIdx.x;
+  const uint32_t base = skey[s] & ~127u;
+  const int ncol = (int)(skey[e - 1] & 127u) + 1;
+  for (int c = tid; c <= ncol; c += LARGE_THREADS) cs[c] = lb_u32(skey, s, e, base | (uint32_t)c);
+  __syncthreads();
+  const double mu = sys[B.sys[M.a[s]]].mu;
+  if (P.warm) {
+    for (int c = 0; c < ncol; ++c) {
+      for (int i = cs[c] + tid; i < cs[c + 1]; i += LARGE_THREADS) warm_apply_one(B, M, i);
+      __syncthreads();
+    }
+  }
+  int sweeps = 0;
+  double res = 0.0, res_ref = 0.0;
+  if (P.stop_rule == 1) {
+    double r = 0.0;
+    for (int i = s + tid; i < e; i += LARGE_THREADS) r = dmax(r, residual_one(B, M, i, P.K, mu));
+    res = block_max(r, red);
+  }
+  const int kk = P.stop_rule == 1 ? P.check_every : 1;
+  while (sweeps < P.max_sweeps && !(P.stop_rule == 1 && res <= P.tol)) {
+    double d = 0.0;
+    for (int it = 0; it < kk; ++it) {
+      d = 0.0;
+      for (int c = 0; c < ncol; ++c) {
+        for (int i = cs[c] + tid; i < cs[c + 1]; i += LARGE_THREADS) d = dmax(d, psor_update(B, M, i, P.K, mu));
+        __syncthreads();
+      }
+    }
+    sweeps += kk;
+    res_ref = block_max(d, red);
+    double r = 0.0;
+    for (int i = s + tid; i < e; i += LARGE_THREADS) r = dmax(r, residual_one(B, M, i, P.K, mu));
+    res = block_max(r, red);
+    if (P.stop_rule == 0 && res_ref < 1e-6) break;
+  }
+  if (tid == 0) island_report(S, sweeps, e - s, res, res_ref, P.stop_rule == 1 && !(res <= P.tol));
+}
+
 // ---------------------------------------------------------------------------
 // host side
 
@@ -947,16 +1284,11 @@ static int small_m_limit() {
 // Colour the manifolds and build the colour-ordered copy.  need_host: also
 // bring the colour offsets to the host (the multi-launch sweep path needs
 // them); the single-workgroup path keeps them on the device.
-int nsc_color_run(rdk_ctx* ctx, bool need_host) {
+// Colour every manifold (colour[] on the device).
+static int color_compute(rdk_ctx* ctx) {
   const int nm = ctx->nm, n = ctx->n;
   const int nb = 256;
-  ctx->ncolor = 0;
-  ctx->color_start.assign(1, 0);
-  ctx->color_host_valid = true;
-  if (nm == 0) {
-    ctx->have_coloring = true;
-    return RDK_OK;
-  }
+  if (nm == 0) return RDK_OK;
   RDK_TRY(ensure(ctx, ctx->body_start, n + 1));
   RDK_TRY(ensure(ctx, ctx->uncolored, 256));
   RDK_TRY(ensure(ctx, ctx->d_cstart, 132));
@@ -978,6 +1310,7 @@ int nsc_color_run(rdk_ctx* ctx, bool need_host) {
   unsigned* dctr = (unsigned*)ctx->uncolored.p;  // [0] left, [1] maxcol, [2] overflow, [3] ncolor
   RDK_HIP(hipMemsetAsync(dctr, 0, 4 * sizeof(unsigned), ctx->stream));
   const bool small = nm <= small_m_limit();
+  (void)alt;
   if (small) {
     ProfScope ps(ctx, K_COLOR);
     hipLaunchKernelGGL(k_color_small, dim3(1), dim3(SMALL_BLOCK), 0, ctx->stream, ctx->ma.p, ctx->mb.p, nm,
@@ -1012,7 +1345,16 @@ int nsc_color_run(rdk_ctx* ctx, bool need_host) {
       }
     }
   }
-  // group by colour (stable: natural order inside a colour)
+  return RDK_OK;
+}
+
+// Group manifolds by colour (stable) and build the colour-ordered copy.
+static int color_group(rdk_ctx* ctx, bool need_host) {
+  const int nm = ctx->nm;
+  const int nb = 256;
+  const bool small = nm <= small_m_limit();
+  unsigned* dctr = (unsigned*)ctx->uncolored.p;
+  bool alt = false;
   RDK_TRY(ensure(ctx, ctx->wall_off, std::max<size_t>(ctx->wall_off.cap, 128)));
   int* hist = ctx->wall_off.p;  // scratch 128-bin histogram
   RDK_HIP(hipMemsetAsync(hist, 0, 128 * sizeof(int), ctx->stream));
@@ -1038,6 +1380,18 @@ int nsc_color_run(rdk_ctx* ctx, bool need_host) {
   ctx->color_host_valid = false;
   if (need_host || !small) RDK_TRY(color_to_host(ctx));
   return RDK_OK;
+}
+
+int nsc_color_run(rdk_ctx* ctx, bool need_host) {
+  ctx->ncolor = 0;
+  ctx->color_start.assign(1, 0);
+  ctx->color_host_valid = true;
+  if (ctx->nm == 0) {
+    ctx->have_coloring = true;
+    return RDK_OK;
+  }
+  RDK_TRY(color_compute(ctx));
+  return color_group(ctx, need_host);
 }
 
 int color_to_host(rdk_ctx* ctx) {
@@ -1131,15 +1485,208 @@ int nsc_sweeps_run(rdk_ctx* ctx, const rdk_nsc_params* p, int nsweeps) {
   return RDK_OK;
 }
 
+
+static int store_warm_cache(rdk_ctx* ctx, const rdk_nsc_params* p) {
+  const int nb = 256;
+  // warm cache for the next step: rod-rod manifolds are the first nc, sorted by key
+  const int nc = ctx->nc;
+  if (p->warm_mode != 0 && nc > 0) {
+    RDK_TRY(ensure(ctx, ctx->prev_keys, nc));
+    for (int k = 0; k < 3; ++k) RDK_TRY(ensure(ctx, ctx->prev_lam[k], nc));
+    ProfScope ps(ctx, K_MISC);
+    hipLaunchKernelGGL(k_copy_keys_lambda, dim3(grid_for(nc, nb)), dim3(nb), 0, ctx->stream, ctx->pkeys.p,
+                       mf_ptrs(ctx, false), nc, ctx->prev_keys.p, ctx->prev_lam[0].p, ctx->prev_lam[1].p,
+                       ctx->prev_lam[2].p);
+    ctx->nprev = nc;
+  } else {
+    ctx->nprev = 0;
+  }
+  const int nw = ctx->nm_wall;
+  if (p->warm_mode != 0 && p->warm_walls && nw > 0) {
+    RDK_TRY(ensure(ctx, ctx->prev_wkeys, nw));
+    for (int k = 0; k < 3; ++k) RDK_TRY(ensure(ctx, ctx->prev_wlam[k], nw));
+    ProfScope ps(ctx, K_MISC);
+    hipLaunchKernelGGL(k_copy_wall_keys_lambda, dim3(grid_for(nw, nb)), dim3(nb), 0, ctx->stream, mf_ptrs(ctx, false),
+                       nc, nw, ctx->prev_wkeys.p, ctx->prev_wlam[0].p, ctx->prev_wlam[1].p, ctx->prev_wlam[2].p);
+    ctx->nprev_w = nw;
+  } else {
+    ctx->nprev_w = 0;
+  }
+  return RDK_OK;
+}
+
+// Island-parallel solve.  Sets *handled = false (and does nothing else after
+// colouring) when some island is too large for one workgroup; the caller then
+// runs the global coloured path.
+static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* st, bool* handled, int* status) {
+  *handled = false;
+  const int nm = ctx->nm, n = ctx->n, nb = 256;
+  if (n >= (1 << 25)) return RDK_OK;
+  RDK_TRY(color_compute(ctx));
+  RDK_TRY(ensure(ctx, ctx->uf_parent, n));
+  RDK_TRY(ensure(ctx, ctx->uf_label, n));
+  RDK_TRY(ensure(ctx, ctx->blab, n));
+  RDK_TRY(ensure(ctx, ctx->blab_alt, n));
+  RDK_TRY(ensure(ctx, ctx->sbody, n));
+  RDK_TRY(ensure(ctx, ctx->sbody_alt, n));
+  RDK_TRY(ensure(ctx, ctx->iflag, nm + 1));
+  RDK_TRY(ensure(ctx, ctx->ipos, nm + 1));
+  RDK_TRY(ensure(ctx, ctx->istart, nm + 2));
+  RDK_TRY(ensure(ctx, ctx->ilists, 4 * (size_t)nm + 4));
+  RDK_TRY(ensure(ctx, ctx->bseg, 2 * (size_t)nm + 2));
+  RDK_TRY(ensure(ctx, ctx->icnt, 8));
+  RDK_TRY(ensure(ctx, ctx->istats, sizeof(IslandStats)));
+  {
+    ProfScope ps(ctx, K_ISLAND);
+    hipLaunchKernelGGL(k_uf_init, dim3(grid_for(n, nb)), dim3(nb), 0, ctx->stream, ctx->uf_parent.p, n);
+    hipLaunchKernelGGL(k_uf_link, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, ctx->ma.p, ctx->mb.p, nm,
+                       ctx->uf_parent.p);
+    hipLaunchKernelGGL(k_uf_flatten, dim3(grid_for(n, nb)), dim3(nb), 0, ctx->stream, ctx->uf_parent.p, n,
+                       ctx->uf_label.p);
+    hipLaunchKernelGGL(k_body_label_keys, dim3(grid_for(n, nb)), dim3(nb), 0, ctx->stream, ctx->uf_label.p, n,
+                       ctx->blab.p, ctx->sbody.p);
+  }
+  bool balt = false, malt = false;
+  RDK_TRY(sort_pairs_u32(ctx, ctx->blab.p, ctx->blab_alt.p, ctx->sbody.p, ctx->sbody_alt.p, n,
+                         bits_for((uint64_t)n), balt));
+  const uint32_t* blab = balt ? ctx->blab_alt.p : ctx->blab.p;
+  const int* sbody = balt ? ctx->sbody_alt.p : ctx->sbody.p;
+  {
+    ProfScope ps(ctx, K_ISLAND);
+    hipLaunchKernelGGL(k_island_keys, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, ctx->ma.p, ctx->color.p, nm,
+                       ctx->uf_label.p, ctx->color_key.p, ctx->perm_alt.p);
+  }
+  RDK_TRY(sort_pairs_u32(ctx, ctx->color_key.p, ctx->color_key_alt.p, ctx->perm_alt.p, ctx->perm.p, nm,
+                         7 + bits_for((uint64_t)n), malt));
+  if (!malt)
+    RDK_HIP(hipMemcpyAsync(ctx->perm.p, ctx->perm_alt.p, sizeof(int) * nm, hipMemcpyDeviceToDevice, ctx->stream));
+  const uint32_t* skey = malt ? ctx->color_key_alt.p : ctx->color_key.p;
+  {
+    ProfScope ps(ctx, K_PERMUTE);
+    hipLaunchKernelGGL(k_permute, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, mf_ptrs(ctx, false),
+                       mf_ptrs(ctx, true), ctx->perm.p, nm);
+  }
+  {
+    ProfScope ps(ctx, K_ISLAND);
+    hipLaunchKernelGGL(k_island_flags, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, skey, nm, ctx->iflag.p);
+  }
+  RDK_TRY(exclusive_scan_int(ctx, ctx->iflag.p, ctx->ipos.p, nm));
+  {
+    ProfScope ps(ctx, K_ISLAND);
+    hipLaunchKernelGGL(k_island_starts, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, ctx->iflag.p, ctx->ipos.p,
+                       nm, ctx->istart.p);
+  }
+  int last[2];
+  RDK_HIP(hipMemcpyAsync(last, ctx->ipos.p + nm - 1, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  RDK_HIP(hipMemcpyAsync(last + 1, ctx->iflag.p + nm - 1, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  RDK_HIP(hipMemsetAsync(ctx->icnt.p, 0, 8 * sizeof(int), ctx->stream));
+  RDK_HIP(hipStreamSynchronize(ctx->stream));
+  const int nisl = last[0] + last[1];
+  IslandClass C;
+  C.tiny_max = 8;
+  C.med_max = MED_CAP;
+  C.med_bmax = MED_BCAP;
+  C.large_max = 8192;
+  {
+    ProfScope ps(ctx, K_ISLAND);
+    hipLaunchKernelGGL(k_island_classify, dim3(grid_for(nisl, nb)), dim3(nb), 0, ctx->stream, ctx->istart.p, nisl,
+                       skey, blab, n, C, ctx->ilists.p, ctx->icnt.p, ctx->bseg.p);
+  }
+  int cnt[4];
+  RDK_HIP(hipMemcpyAsync(cnt, ctx->icnt.p, 4 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  RDK_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->nislands = nisl;
+  if (cnt[3] > 0) return RDK_OK;  // a giant island: global coloured path
+  IslandSolve P;
+  P.K = solve_const(ctx, p);
+  P.tol = p->tol;
+  P.max_sweeps = p->max_sweeps;
+  P.check_every = std::max(1, p->check_every);
+  P.stop_rule = p->stop_rule;
+  P.warm = p->warm_mode != 0 ? 1 : 0;
+  IslandStats* S = (IslandStats*)ctx->istats.p;
+  RDK_HIP(hipMemsetAsync(S, 0, sizeof(IslandStats), ctx->stream));
+  BodyPtrs B = body_ptrs(ctx);
+  MfPtrs Mc = mf_ptrs(ctx, true);
+  {
+    ProfScope ps(ctx, K_PSOR_SWEEP);
+    if (cnt[0] > 0)
+      hipLaunchKernelGGL(k_island_tiny, dim3(grid_for(cnt[0], 64)), dim3(64), 0, ctx->stream, B, Mc, ctx->sys_dev.p,
+                         ctx->istart.p, ctx->ilists.p, cnt[0], P, S);
+    if (cnt[1] > 0)
+      hipLaunchKernelGGL(k_island_medium, dim3(cnt[1]), dim3(MED_THREADS), 0, ctx->stream, B, Mc, ctx->sys_dev.p,
+                         ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p + nisl, P, S);
+    if (cnt[2] > 0)
+      hipLaunchKernelGGL(k_island_large, dim3(cnt[2]), dim3(LARGE_THREADS), 0, ctx->stream, B, Mc, ctx->sys_dev.p,
+                         ctx->istart.p, skey, ctx->ilists.p + 2 * nisl, P, S);
+  }
+  {
+    ProfScope ps(ctx, K_PERMUTE);
+    hipLaunchKernelGGL(k_unpermute_lambda, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, Mc, mf_ptrs(ctx, false),
+                       ctx->perm.p, nm);
+  }
+  IslandStats hs;
+  unsigned hc[4];
+  RDK_HIP(hipMemcpyAsync(&hs, S, sizeof(IslandStats), hipMemcpyDeviceToHost, ctx->stream));
+  RDK_HIP(hipMemcpyAsync(hc, ctx->uncolored.p, 4 * sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+  RDK_HIP(hipStreamSynchronize(ctx->stream));
+  if (hc[2] != 0) {
+    ctx->err = "nsc colouring: more than 127 colours needed";
+    return RDK_ERR_CAPACITY;
+  }
+  ctx->ncolor = (int)hc[1] + 1;
+  ctx->have_coloring = false;  // the solve-ordered copy is island-ordered
+  ctx->island_counts[0] = cnt[0];
+  ctx->island_counts[1] = cnt[1];
+  ctx->island_counts[2] = cnt[2];
+  ctx->work[W_SWEEP_CONTACTS] += (int64_t)hs.manifold_sweeps;
+  *status = hs.nonconv > 0 ? RDK_ERR_NOCONV : RDK_OK;
+  if (st) {
+    st->sweeps = hs.max_sweeps;
+    st->residual = hs.res;
+    st->residual_ref = hs.res_ref;
+  }
+  *handled = true;
+  return RDK_OK;
+}
+
+static int global_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* st, bool colored);
+
 int nsc_solve_run(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* st) {
   if (!ctx->have_manifolds) {
     ctx->err = "rdk_nsc_solve: no manifolds (call rdk_nsc_build)";
     return RDK_ERR_STATE;
   }
+  if (ctx->nm > 0 && ctx->solver_mode != 1) {
+    bool handled = false;
+    int status = RDK_OK;
+    rdk_solve_stats tmp;
+    std::memset(&tmp, 0, sizeof(tmp));
+    RDK_TRY(island_solve(ctx, p, &tmp, &handled, &status));
+    if (handled) {
+      RDK_TRY(store_warm_cache(ctx, p));
+      if (st) {
+        *st = tmp;
+        st->manifolds = ctx->nm;
+        st->rod_contacts = ctx->nc;
+        st->wall_contacts = ctx->nm_wall;
+        st->colors = ctx->ncolor;
+      }
+      return status;
+    }
+    RDK_TRY(color_group(ctx, false));
+    return global_solve(ctx, p, st, true);
+  }
+  return global_solve(ctx, p, st, false);
+}
+
+// Global coloured solve (all manifolds; used when an island is too large for
+// one workgroup, or when forced).
+static int global_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* st, bool colored) {
   RDK_TRY(ensure(ctx, ctx->resid, 4));
   const int nm = ctx->nm, nb = 256;
   const bool small = nm > 0 && nm <= small_m_limit();
-  RDK_TRY(nsc_color_run(ctx, false));
+  if (!colored) RDK_TRY(nsc_color_run(ctx, false));
   BodyPtrs B = body_ptrs(ctx);
   MfPtrs Mc = mf_ptrs(ctx, true);
   int sweeps = 0;
@@ -1210,30 +1757,7 @@ int nsc_solve_run(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* st) {
     hipLaunchKernelGGL(k_unpermute_lambda, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, Mc, mf_ptrs(ctx, false),
                        ctx->perm.p, nm);
   }
-  // warm cache for the next step: rod-rod manifolds are the first nc, sorted by key
-  const int nc = ctx->nc;
-  if (p->warm_mode != 0 && nc > 0) {
-    RDK_TRY(ensure(ctx, ctx->prev_keys, nc));
-    for (int k = 0; k < 3; ++k) RDK_TRY(ensure(ctx, ctx->prev_lam[k], nc));
-    ProfScope ps(ctx, K_MISC);
-    hipLaunchKernelGGL(k_copy_keys_lambda, dim3(grid_for(nc, nb)), dim3(nb), 0, ctx->stream, ctx->pkeys.p,
-                       mf_ptrs(ctx, false), nc, ctx->prev_keys.p, ctx->prev_lam[0].p, ctx->prev_lam[1].p,
-                       ctx->prev_lam[2].p);
-    ctx->nprev = nc;
-  } else {
-    ctx->nprev = 0;
-  }
-  const int nw = ctx->nm_wall;
-  if (p->warm_mode != 0 && p->warm_walls && nw > 0) {
-    RDK_TRY(ensure(ctx, ctx->prev_wkeys, nw));
-    for (int k = 0; k < 3; ++k) RDK_TRY(ensure(ctx, ctx->prev_wlam[k], nw));
-    ProfScope ps(ctx, K_MISC);
-    hipLaunchKernelGGL(k_copy_wall_keys_lambda, dim3(grid_for(nw, nb)), dim3(nb), 0, ctx->stream, mf_ptrs(ctx, false),
-                       nc, nw, ctx->prev_wkeys.p, ctx->prev_wlam[0].p, ctx->prev_wlam[1].p, ctx->prev_wlam[2].p);
-    ctx->nprev_w = nw;
-  } else {
-    ctx->nprev_w = 0;
-  }
+  RDK_TRY(store_warm_cache(ctx, p));
   if (st) {
     st->manifolds = nm;
     st->rod_contacts = ctx->nc;

@@ -91,6 +91,9 @@ _SIGS = {
     "rdk_nsc_sweeps": (C.c_int, [C.c_void_p, C.POINTER(NscParams), C.c_int]),
     "rdk_nsc_get_coloring": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _ip]),
     "rdk_nsc_residual": (C.c_int, [C.c_void_p, C.POINTER(NscParams), _dp]),
+    "rdk_set_solver_mode": (C.c_int, [C.c_void_p, C.c_int]),
+    "rdk_nsc_island_info": (C.c_int, [C.c_void_p, _ip, _ip]),
+    "rdk_nsc_get_solve_order": (C.c_int, [C.c_void_p, C.c_int, _ip]),
     "rdk_integrate": (C.c_int, [C.c_void_p, C.c_int]),
     "rdk_step_nsc": (C.c_int, [C.c_void_p, C.POINTER(DetectParams), C.POINTER(NscParams), C.c_int,
                                C.POINTER(StepStats)]),
@@ -332,6 +335,20 @@ class Context:
         r = C.c_double()
         self._check(self._L.rdk_nsc_residual(self._h, C.byref(p), C.byref(r)))
         return r.value
+
+    def set_solver_mode(self, mode):
+        self._check(self._L.rdk_set_solver_mode(self._h, int(mode)))
+
+    def island_info(self):
+        n = C.c_int()
+        cnt = np.zeros(3, np.int32)
+        self._check(self._L.rdk_nsc_island_info(self._h, C.byref(n), iptr(cnt)))
+        return n.value, cnt.tolist()
+
+    def solve_order(self, nm):
+        order = np.zeros(nm, np.int32)
+        self._check(self._L.rdk_nsc_get_solve_order(self._h, nm, iptr(order)))
+        return order
 
     def integrate(self, stage):
         self._check(self._L.rdk_integrate(self._h, int(stage)))

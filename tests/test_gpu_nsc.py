@@ -171,3 +171,70 @@ def test_frictionless_unique_solution_matches_serial_oracle(gpu_ctx_factory):
     assert r <= 1e-12
     _, _, lam_o = W.manifolds()
     assert np.allclose(lam_g, lam_o, rtol=1e-7, atol=1e-10)
+
+
+def clusters(nclust=40, seed=77, mu=0.5):
+    """Separate clusters of overlapping rods (islands of 2..40 rods) above a floor."""
+    rng = np.random.default_rng(seed)
+    xs, qs = [], []
+    side = 12.0 * int(np.ceil(np.sqrt(nclust)))
+    for c in range(nclust):
+        k = int(rng.integers(2, 41))
+        cx, cy = 12.0 * (c % int(np.sqrt(nclust) + 1)) + 6, 12.0 * (c // int(np.sqrt(nclust) + 1)) + 6
+        x = np.stack([rng.uniform(cx - 2, cx + 2, k), rng.uniform(cy - 2, cy + 2, k), rng.uniform(0.2, 3.0, k)], 1)
+        q = rng.normal(size=(k, 4))
+        q /= np.linalg.norm(q, axis=1, keepdims=True)
+        xs.append(x)
+        qs.append(q)
+    x = np.concatenate(xs)
+    q = np.concatenate(qs)
+    n = len(x)
+    side = max(side, x[:, 0].max() + 10, x[:, 1].max() + 10)
+    return dict(x=x, q=q, v=rng.normal(size=(n, 3)), w=rng.normal(size=(n, 3)), r=np.full(n, 0.4),
+                h=rng.uniform(0.8, 2.0, n), mass=rng.uniform(0.5, 2.0, n), rest=np.full(n, 0.0), side=side,
+                periodic=(1, 1, 0), walls=True, mu=mu)
+
+
+@pytest.mark.parametrize("proj", [0, 1])
+def test_island_sweeps_bitwise_equal_serial_sweeps_in_solve_order(gpu_ctx_factory, proj):
+    B = clusters()
+    ctx = gpu_ctx_factory()
+    gpu_load(ctx, B)
+    p = L.nsc_params(beta=0.1, cfm=1e-4, omega=1.3, tol=0.0, max_sweeps=9, check_every=1, friction_proj=proj,
+                     warm_mode=0)
+    ctx.detect(L.detect_params())
+    nm = ctx.nsc_build(p)
+    rec, ab, lam0 = ctx.nsc_manifolds(nm)
+    s0 = ctx.download_state()
+    st = ctx.nsc_solve(p, allow_noconv=True)
+    nisl, counts = ctx.island_info()
+    assert counts[0] > 0 and counts[1] > 0, counts  # tiny and medium islands both present
+    order = ctx.solve_order(nm)
+    _, _, lam_g = ctx.nsc_manifolds(nm)
+    s1 = ctx.download_state()
+    W = oracle_load(B)
+    W.set_state(v=s0["v"], w=s0["w"])
+    W.set_manifolds(rec, ab, lam0)
+    op = O.nsc_params(mu=B["mu"], beta=0.1, cfm=1e-4, omega=1.3, friction_proj=proj)
+    W.sweeps(op, 9, order)
+    _, _, lam_o = W.manifolds()
+    so = W.state()
+    assert np.array_equal(lam_g, lam_o)
+    assert np.array_equal(s1["v"], so["v"]) and np.array_equal(s1["w"], so["w"])
+
+
+def test_island_and_global_solves_agree(gpu_ctx_factory):
+    B = clusters(seed=5, mu=0.0)  # frictionless: unique solution per island
+    out = []
+    for mode in (0, 1):
+        ctx = gpu_ctx_factory()
+        gpu_load(ctx, B)
+        ctx.set_solver_mode(mode)
+        p = L.nsc_params(beta=0.0, cfm=1e-3, omega=1.0, tol=1e-12, max_sweeps=200000, check_every=4, friction_proj=1,
+                         warm_mode=0)
+        ctx.detect(L.detect_params())
+        nm = ctx.nsc_build(p)
+        st = ctx.nsc_solve(p)
+        assert st["residual"] <= 1e-12
+        out.append(ctx.nsc_manifolds(nm)[2])
+    assert np.allclose(out[0], out[1], rtol=1e-7, atol=1e-10)
