@@ -70,7 +70,7 @@ static const char* kKernelNames[K_NUM] = {
     "chunk_keys", "cell_ranges", "chunk_gather", "broad_narrow", "contact_record", "sort_scan",
     "manifold", "wall", "csr", "color", "permute", "warm_apply", "psor_sweep", "residual",
     "nsc_free", "nsc_pos", "vv_half1", "vv_half2", "force", "body_reduce", "history", "observe",
-    "mlp", "island", "misc"};
+    "mlp", "island", "island_tiny", "island_medium", "island_large", "misc"};
 
 }  // namespace rdk
 

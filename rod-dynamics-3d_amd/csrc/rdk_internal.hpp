@@ -38,6 +38,9 @@ enum KernelId {
   K_OBSERVE,
   K_MLP,
   K_ISLAND,
+  K_ISL_TINY,
+  K_ISL_MEDIUM,
+  K_ISL_LARGE,
   K_MISC,
   K_NUM
 };

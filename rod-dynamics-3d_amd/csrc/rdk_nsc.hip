@@ -1608,17 +1608,20 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
   RDK_HIP(hipMemsetAsync(S, 0, sizeof(IslandStats), ctx->stream));
   BodyPtrs B = body_ptrs(ctx);
   MfPtrs Mc = mf_ptrs(ctx, true);
-  {
-    ProfScope ps(ctx, K_PSOR_SWEEP);
-    if (cnt[0] > 0)
-      hipLaunchKernelGGL(k_island_tiny, dim3(grid_for(cnt[0], 64)), dim3(64), 0, ctx->stream, B, Mc, ctx->sys_dev.p,
-                         ctx->istart.p, ctx->ilists.p, cnt[0], P, S);
-    if (cnt[1] > 0)
-      hipLaunchKernelGGL(k_island_medium, dim3(cnt[1]), dim3(MED_THREADS), 0, ctx->stream, B, Mc, ctx->sys_dev.p,
-                         ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p + nisl, P, S);
-    if (cnt[2] > 0)
-      hipLaunchKernelGGL(k_island_large, dim3(cnt[2]), dim3(LARGE_THREADS), 0, ctx->stream, B, Mc, ctx->sys_dev.p,
-                         ctx->istart.p, skey, ctx->ilists.p + 2 * nisl, P, S);
+  if (cnt[0] > 0) {
+    ProfScope ps(ctx, K_ISL_TINY);
+    hipLaunchKernelGGL(k_island_tiny, dim3(grid_for(cnt[0], 64)), dim3(64), 0, ctx->stream, B, Mc, ctx->sys_dev.p,
+                       ctx->istart.p, ctx->ilists.p, cnt[0], P, S);
+  }
+  if (cnt[1] > 0) {
+    ProfScope ps(ctx, K_ISL_MEDIUM);
+    hipLaunchKernelGGL(k_island_medium, dim3(cnt[1]), dim3(MED_THREADS), 0, ctx->stream, B, Mc, ctx->sys_dev.p,
+                       ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p + nisl, P, S);
+  }
+  if (cnt[2] > 0) {
+    ProfScope ps(ctx, K_ISL_LARGE);
+    hipLaunchKernelGGL(k_island_large, dim3(cnt[2]), dim3(LARGE_THREADS), 0, ctx->stream, B, Mc, ctx->sys_dev.p,
+                       ctx->istart.p, skey, ctx->ilists.p + 2 * nisl, P, S);
   }
   {
     ProfScope ps(ctx, K_PERMUTE);
