@@ -97,7 +97,10 @@ int rdk_create(int device, rdk_ctx** out) {
   rdk_ctx* ctx = new (std::nothrow) rdk_ctx();
   if (!ctx) return RDK_ERR_HIP;
   ctx->device = device;
-  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) {
     delete ctx;
     return RDK_ERR_HIP;
   }
@@ -144,6 +147,9 @@ int rdk_destroy(rdk_ctx* ctx) {
     if (r.b) (void)hipEventDestroy(r.b);
   }
   for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
+  if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+  if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+  if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return RDK_OK;

@@ -91,6 +91,8 @@ struct EventRec {
 struct rdk_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;  // side stream (default priority)
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::string err;
 
   // ---- bodies (planar FP64) ----

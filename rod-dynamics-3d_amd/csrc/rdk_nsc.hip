@@ -1608,10 +1608,19 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
   RDK_HIP(hipMemsetAsync(S, 0, sizeof(IslandStats), ctx->stream));
   BodyPtrs B = body_ptrs(ctx);
   MfPtrs Mc = mf_ptrs(ctx, true);
+  // tiny islands run on the side stream, concurrently with the workgroup islands
+  const bool side = cnt[0] > 0 && (cnt[1] > 0 || cnt[2] > 0) && ctx->stream2 && !ctx->profiling;
   if (cnt[0] > 0) {
+    hipStream_t s_tiny = ctx->stream;
+    if (side) {
+      RDK_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
+      RDK_HIP(hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
+      s_tiny = ctx->stream2;
+    }
     ProfScope ps(ctx, K_ISL_TINY);
-    hipLaunchKernelGGL(k_island_tiny, dim3(grid_for(cnt[0], 64)), dim3(64), 0, ctx->stream, B, Mc, ctx->sys_dev.p,
+    hipLaunchKernelGGL(k_island_tiny, dim3(grid_for(cnt[0], 64)), dim3(64), 0, s_tiny, B, Mc, ctx->sys_dev.p,
                        ctx->istart.p, ctx->ilists.p, cnt[0], P, S);
+    if (side) RDK_HIP(hipEventRecord(ctx->ev_join, ctx->stream2));
   }
   if (cnt[1] > 0) {
     ProfScope ps(ctx, K_ISL_MEDIUM);
@@ -1623,6 +1632,7 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
     hipLaunchKernelGGL(k_island_large, dim3(cnt[2]), dim3(LARGE_THREADS), 0, ctx->stream, B, Mc, ctx->sys_dev.p,
                        ctx->istart.p, skey, ctx->ilists.p + 2 * nisl, P, S);
   }
+  if (side) RDK_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
   {
     ProfScope ps(ctx, K_PERMUTE);
     hipLaunchKernelGGL(k_unpermute_lambda, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, Mc, mf_ptrs(ctx, false),
