@@ -1,0 +1,107 @@
+"""CPU-side checks of librdk: it loads, exports every symbol include/rdk.h
+declares, and its host-side scene generator produces what it promises
+(non-overlapping capsules checked by the oracle's brute force, thermal
+statistics).  No GPU call is made."""
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+from rod_dynamics_py import _lib as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "rdk.h")).read()
+    # function declarations: a return type at the start of a line, then the name
+    return sorted(set(re.findall(r"^(?:int|const char\s*\*)\s*(rdk_[a-z0-9_]+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = L.lib()
+    names = declared_symbols()
+    assert len(names) > 30
+    for name in names:
+        assert hasattr(lib, name), name
+    # and the ctypes table covers the whole header
+    assert set(names) <= set(L._SIGS), sorted(set(names) - set(L._SIGS))
+    assert lib.rdk_version() == 1
+    assert lib.rdk_num_kernels() > 10
+
+
+def test_rsa_packing_has_no_overlaps_and_respects_walls():
+    desc = L.system_desc((0, 0, 0), (30, 30, 30), (1, 1, 0), floor_wall=True, ceil_wall=True)
+    n = 600
+    r = np.full(n, 0.5)
+    h = np.full(n, 3.0)
+    x, q, placed = L.gen_rsa(r, h, desc, gap=0.0, seed=3)
+    assert placed == n
+    assert np.allclose(np.linalg.norm(q, axis=1), 1.0)
+    W = O.World(x, q, None, None, r, h, np.ones(n))
+    W.set_box([0, 0, 0], [30, 30, 30], [1, 1, 0])
+    assert W.detect(0, 0.0) == 0
+    u = np.array([O.axis(qq) for qq in q])
+    zlo = np.minimum(x[:, 2] - u[:, 2] * h, x[:, 2] + u[:, 2] * h) - r
+    zhi = np.maximum(x[:, 2] - u[:, 2] * h, x[:, 2] + u[:, 2] * h) + r
+    assert zlo.min() >= 0.0 and zhi.max() <= 30.0
+    # orientations uniform on the sphere: <u_z^2> = 1/3
+    assert abs((u[:, 2] ** 2).mean() - 1 / 3) < 0.05
+    # determinism
+    x2, q2, _ = L.gen_rsa(r, h, desc, gap=0.0, seed=3)
+    assert np.array_equal(x, x2) and np.array_equal(q, q2)
+
+
+def test_rsa_small_periodic_box_uses_all_images():
+    # box side 8 < 2 (L + d): several periodic images matter
+    desc = L.system_desc((0, 0, 0), (8, 8, 8), (1, 1, 1))
+    n = 40
+    r = np.full(n, 0.3)
+    h = np.full(n, 2.5)
+    x, q, placed = L.gen_rsa(r, h, desc, gap=0.0, seed=9)
+    assert placed == n
+    u = np.array([O.axis(qq) for qq in q])
+    for i in range(n):
+        for j in range(i + 1, n):
+            best = np.inf
+            for sx in (-8, 0, 8):
+                for sy in (-8, 0, 8):
+                    for sz in (-8, 0, 8):
+                        sh = np.array([sx, sy, sz]) + (-np.round((x[j] - x[i]) / 8) * 8)
+                        a1, a2 = x[i] - u[i] * h[i], x[i] + u[i] * h[i]
+                        b1, b2 = x[j] - u[j] * h[j] + sh, x[j] + u[j] * h[j] + sh
+                        s, t = O.segseg(a1, a2, b1, b2)
+                        best = min(best, np.linalg.norm(b1 + t * (b2 - b1) - a1 - s * (a2 - a1)))
+            assert best >= 0.6 - 1e-12
+
+
+def test_thermal_velocities_statistics():
+    n = 20000
+    rng = np.random.default_rng(1)
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    r = np.full(n, 0.5)
+    h = np.full(n, 5.0)
+    m = np.full(n, 2.0)
+    v, w = L.gen_thermal(q, r, h, m, 1.5, seed=4)
+    assert abs(v.mean()) < 0.02 and v.var() == pytest.approx(1.5 / 2.0, rel=0.03)
+    Ip = 2.0 * (3 * 0.25 + 100) / 12
+    u = np.array([O.axis(qq) for qq in q[:500]])
+    assert np.abs(np.einsum("ij,ij->i", w[:500], u)).max() < 1e-12  # no spin about the axis
+    assert (w ** 2).sum(axis=1).mean() == pytest.approx(2 * 1.5 / Ip, rel=0.03)
+
+
+def test_no_gpu_means_loud_failure_not_fallback():
+    # without a HIP device the context cannot be created; nothing silently runs on the CPU
+    import ctypes
+    h = ctypes.c_void_p()
+    st = L.lib().rdk_create(0, ctypes.byref(h))
+    if st == L.RDK_OK:
+        L.lib().rdk_destroy(h)
+        pytest.skip("a GPU is present")
+    assert st in (L.RDK_ERR_HIP, L.RDK_ERR_ARG)
+    with pytest.raises(L.RdkError):
+        L.Context(0)
