@@ -45,7 +45,23 @@ def parse():
     ap.add_argument("--chunk-len", type=float, default=4.0)
     ap.add_argument("--check-every", type=int, default=4)
     ap.add_argument("--block", type=int, default=250)
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB five times (HBM counter calibration)")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    """A plain copy kernel of known size: FETCH_SIZE / WRITE_SIZE of this run
+    give the scale factor between the counters and bytes moved."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_per_launch": 4 * n, "launches": 5}), flush=True)
 
 
 def cpu_baseline(sc, nsteps):
@@ -72,6 +88,9 @@ def main():
     if a.gpus != 1:
         print("bench.py: this build measures one GPU only (one MI355X per box); --gpus must be 1", file=sys.stderr)
         sys.exit(2)
+    if a.copy_calib > 0:
+        copy_calibration(a.copy_calib)
+        return
     from rod_dynamics_py import _lib as L
     from rod_dynamics_py import scenes
 

@@ -168,9 +168,10 @@ int rdk_nsc_residual(rdk_ctx* ctx, const rdk_nsc_params* p, double* res);
  * graph) independently on the GPU whenever every island fits one workgroup;
  * 1: always sweep all manifolds globally, colour by colour */
 int rdk_set_solver_mode(rdk_ctx* ctx, int mode);
-/* after rdk_nsc_solve: islands and their split (tiny, medium, large), and the
- * manifold order the last solve swept in (natural indices) */
-int rdk_nsc_island_info(rdk_ctx* ctx, int* n_islands, int* counts3);
+/* after rdk_nsc_solve: islands and their split by solver class (tiny <= 8
+ * manifolds, small <= 64, medium <= 256, large <= 8192), and the manifold
+ * order the last solve swept in (natural indices) */
+int rdk_nsc_island_info(rdk_ctx* ctx, int* n_islands, int* counts4);
 int rdk_nsc_get_solve_order(rdk_ctx* ctx, int cap, int* order);
 
 /* ---- integrators (integrator.cpp:56-159, main.cpp:4397-4483) ----------- */
@@ -220,6 +221,37 @@ int rdk_gen_rsa(int n, const double* r, const double* h,
 int rdk_gen_thermal(int n, const double* q, const double* r, const double* h,
                     const double* mass, double kT, uint64_t seed, double* v,
                     double* w);
+
+/* ---- learned PSOR warm start (warm_mode 2) ----------------------------
+ * No counterpart in the reference: its warm start is the previous-step
+ * impulse cache (src/physics/nsc_contact.cpp:305-317, warm_mode 1).  A
+ * per-manifold MLP 12 -> 64 -> 64 -> 3 (SiLU) predicts the converged
+ * impulses; PSOR then iterates from the prediction to the same tolerance.
+ * Parameters are FP32, row-major per layer: W1[12][64], b1[64],
+ * W2[64][64], b2[64], W3[64][3], b3[3].  Inference runs BF16 on the matrix
+ * cores; training runs FP32 with AdamW. */
+#define RDK_MLP_IN 12
+#define RDK_MLP_OUT 3
+int rdk_mlp_num_params(void);
+/* Glorot-uniform weights, zero biases, fresh AdamW state. */
+int rdk_mlp_init(rdk_ctx* ctx, uint64_t seed);
+int rdk_mlp_set_params(rdk_ctx* ctx, const float* params, int n);
+int rdk_mlp_get_params(rdk_ctx* ctx, float* params, int n);
+/* Features of the current manifolds (natural order, after rdk_nsc_build):
+ * X[cap][12]; lscale[i] is the impulse unit of manifold i (outputs and
+ * training targets are lambda / lscale). */
+int rdk_mlp_features(rdk_ctx* ctx, int cap, float* X, double* lscale);
+/* Forward pass on host features: Y[n][3] (BF16 matrix-core path). */
+int rdk_mlp_infer(rdk_ctx* ctx, const float* X, int n, float* Y);
+/* Gradient of the mean squared error over n samples (FP32 path):
+ * grad[num_params], *loss = mean over n*3 outputs. */
+int rdk_mlp_grad(rdk_ctx* ctx, const float* X, const float* T, int n,
+                 float* grad, float* loss);
+/* Mini-batch AdamW (beta 0.9/0.999, eps 1e-8, decoupled weight decay on
+ * the weight matrices) for `epochs` shuffled passes over (X, T). */
+int rdk_mlp_train(rdk_ctx* ctx, const float* X, const float* T, int n,
+                  int epochs, int batch, float lr, float weight_decay,
+                  uint64_t seed, float* final_loss);
 
 #ifdef __cplusplus
 }

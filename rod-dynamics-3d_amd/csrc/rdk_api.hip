@@ -70,7 +70,7 @@ static const char* kKernelNames[K_NUM] = {
     "chunk_keys", "cell_ranges", "chunk_gather", "broad_narrow", "contact_record", "sort_scan",
     "manifold", "wall", "csr", "color", "permute", "warm_apply", "psor_sweep", "residual",
     "nsc_free", "nsc_pos", "vv_half1", "vv_half2", "force", "body_reduce", "history", "observe",
-    "mlp", "island", "island_tiny", "island_medium", "island_large", "misc"};
+    "mlp", "island", "island_tiny", "island_small", "island_medium", "island_large", "misc"};
 
 }  // namespace rdk
 
@@ -551,11 +551,11 @@ int rdk_set_solver_mode(rdk_ctx* ctx, int mode) {
   return RDK_OK;
 }
 
-int rdk_nsc_island_info(rdk_ctx* ctx, int* n_islands, int* counts3) {
+int rdk_nsc_island_info(rdk_ctx* ctx, int* n_islands, int* counts4) {
   if (!ctx) return RDK_ERR_ARG;
   if (n_islands) *n_islands = ctx->nislands;
-  if (counts3)
-    for (int k = 0; k < 3; ++k) counts3[k] = ctx->island_counts[k];
+  if (counts4)
+    for (int k = 0; k < 4; ++k) counts4[k] = ctx->island_counts[k];
   return RDK_OK;
 }
 

@@ -39,6 +39,7 @@ enum KernelId {
   K_MLP,
   K_ISLAND,
   K_ISL_TINY,
+  K_ISL_SMALL,
   K_ISL_MEDIUM,
   K_ISL_LARGE,
   K_MISC,
@@ -151,7 +152,7 @@ struct rdk_ctx {
   // islands
   int solver_mode = 0;  // 0 islands when possible, 1 always global coloured
   int nislands = 0;
-  int island_counts[3] = {0, 0, 0};
+  int island_counts[4] = {0, 0, 0, 0};  // tiny, small, medium, large
   rdk::DevBuf<int> uf_parent, uf_label, sbody, sbody_alt, iflag, ipos, istart, ilists, bseg, icnt;
   rdk::DevBuf<uint32_t> blab, blab_alt;
   rdk::DevBuf<char> istats;

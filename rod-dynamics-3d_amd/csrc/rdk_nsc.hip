@@ -34,6 +34,9 @@ struct MfPtrs {
   __device__ __forceinline__ d3 get3(int field, int i) const {
     return mk3(at(field, i), at(field + 1, i), at(field + 2, i));
   }
+  __device__ __forceinline__ int ia(int i) const { return a[i]; }
+  __device__ __forceinline__ int ib(int i) const { return b[i]; }
+  __device__ __forceinline__ double& lr(int c, int i) const { return lam[c][i]; }
   __device__ __forceinline__ void put3(int field, int i, d3 v) const {
     at(field, i) = v.x;
     at(field + 1, i) = v.y;
@@ -387,8 +390,9 @@ __global__ void k_unpermute_lambda(MfPtrs col, MfPtrs nat, const int* __restrict
 
 // ---- impulses ----------------------------------------------------------------
 // Warm-start impulse application of manifold i (nsc_contact.cpp:344-380).
-__device__ __forceinline__ void warm_apply_one(const BodyPtrs& B, const MfPtrs& M, int i) {
-  const int a = M.a[i], b = M.b[i];
+template <class MP>
+__device__ __forceinline__ void warm_apply_one(const BodyPtrs& B, MP& M, int i) {
+  const int a = M.ia(i), b = M.ib(i);
   const bool wall = b < 0;
   const double ima = M.at(F_IMA, i), imb = M.at(F_IMB, i);
   d3 va = ld3(B.v, a), wa = ld3(B.w, a);
@@ -402,7 +406,7 @@ __device__ __forceinline__ void warm_apply_one(const BodyPtrs& B, const MfPtrs& 
   const int ib_f[3] = {F_IBN, F_IBT1, F_IBT2};
   bool touched = false;
   for (int d = 0; d < 3; ++d) {
-    const double l = M.lam[d][i];
+    const double l = M.lr(d, i);
     if (l != 0.0) {
       touched = true;
       const d3 dir = M.get3(dir_f[d], i);
@@ -438,9 +442,9 @@ struct SolveConst {
 // One projected Gauss-Seidel update of manifold i (nsc_contact.cpp:385-552).
 // Body velocities are read once into registers and written back once; the
 // arithmetic is the serial reference update in the same order.
-__device__ __forceinline__ double psor_update(const BodyPtrs& B, const MfPtrs& M, int i, const SolveConst& K,
-                                              double mu) {
-  const int a = M.a[i], b = M.b[i];
+template <class MP>
+__device__ __forceinline__ double psor_update(const BodyPtrs& B, MP& M, int i, const SolveConst& K, double mu) {
+  const int a = M.ia(i), b = M.ib(i);
   const bool wall = b < 0;
   d3 va = ld3(B.v, a), wa = ld3(B.w, a);
   d3 vb = mk3(0, 0, 0), wb = mk3(0, 0, 0);
@@ -455,7 +459,7 @@ __device__ __forceinline__ double psor_update(const BodyPtrs& B, const MfPtrs& M
   const double phi = M.at(F_PHI, i), vnpre = M.at(F_VNPRE, i);
   const double b_n = (phi < 0.0) ? K.beta * phi / K.dt : 0.0;
   const double b_rest = (vnpre < 0.0) ? M.at(F_E, i) * vnpre : 0.0;
-  double ln = M.lam[0][i], lt1 = M.lam[1][i], lt2 = M.lam[2][i];
+  double ln = M.lr(0, i), lt1 = M.lr(1, i), lt2 = M.lr(2, i);
   const double w_n = dot(n, v_rel) + b_n + b_rest + K.cfm * ln;
   const double delta_n = -(K.omega / M.at(F_GN, i)) * w_n;
   const double old_n = ln;
@@ -539,9 +543,9 @@ __device__ __forceinline__ double psor_update(const BodyPtrs& B, const MfPtrs& M
       wb = wb + M.get3(F_IBT2, i) * dt2;
     }
   }
-  M.lam[0][i] = ln;
-  M.lam[1][i] = lt1;
-  M.lam[2][i] = lt2;
+  M.lr(0, i) = ln;
+  M.lr(1, i) = lt1;
+  M.lr(2, i) = lt2;
   if (dn != 0.0 || dt1 != 0.0 || dt2 != 0.0) {
     st3(B.v, a, va);
     st3(B.w, a, wa);
@@ -570,9 +574,9 @@ __global__ void __launch_bounds__(256) k_psor_sweep(BodyPtrs B, MfPtrs M, const 
 }
 
 // Fixed-point residual (read-only), velocity units.
-__device__ __forceinline__ double residual_one(const BodyPtrs& B, const MfPtrs& M, int i, const SolveConst& K,
-                                               double mu) {
-  const int a = M.a[i], b = M.b[i];
+template <class MP>
+__device__ __forceinline__ double residual_one(const BodyPtrs& B, MP& M, int i, const SolveConst& K, double mu) {
+  const int a = M.ia(i), b = M.ib(i);
   const bool wall = b < 0;
   const d3 ra = M.get3(F_RA, i), rb = M.get3(F_RB, i);
   const d3 va = ld3(B.v, a), wa = ld3(B.w, a);
@@ -582,7 +586,7 @@ __device__ __forceinline__ double residual_one(const BodyPtrs& B, const MfPtrs& 
   const double phi = M.at(F_PHI, i), vnpre = M.at(F_VNPRE, i);
   const double b_n = (phi < 0.0) ? K.beta * phi / K.dt : 0.0;
   const double b_rest = (vnpre < 0.0) ? M.at(F_E, i) * vnpre : 0.0;
-  const double ln = M.lam[0][i], lt1 = M.lam[1][i], lt2 = M.lam[2][i];
+  const double ln = M.lr(0, i), lt1 = M.lr(1, i), lt2 = M.lr(2, i);
   const double gn = M.at(F_GN, i), g1 = M.at(F_GT1, i), g2 = M.at(F_GT2, i);
   const double w_n = dot(M.get3(F_N, i), v_rel) + b_n + b_rest + K.cfm * ln;
   const double pn = dmax(0.0, ln - w_n / gn);
@@ -907,10 +911,11 @@ __device__ __forceinline__ int lb_u32(const uint32_t* a, int lo, int hi, uint32_
 }
 
 struct IslandClass {
-  int tiny_max, med_max, med_bmax, large_max;
+  int max_m[3], max_b[3];  // register classes: manifolds and bodies per island
+  int large_max;
 };
 
-// island list per class; counts in cnt[0..3] (tiny, medium, large, giant)
+// island list per class; counts in cnt[0..4] (tiny, small, medium, large, giant)
 __global__ void k_island_classify(const int* __restrict__ istart, int nisl, const uint32_t* __restrict__ skey,
                                   const uint32_t* __restrict__ blab, int nbody, IslandClass C,
                                   int* __restrict__ lists, int* __restrict__ cnt, int* __restrict__ bseg) {
@@ -922,11 +927,11 @@ __global__ void k_island_classify(const int* __restrict__ istart, int nisl, cons
   bseg[2 * k] = bs;
   bseg[2 * k + 1] = be;
   const int size = e - s, nb = be - bs;
-  int cls;
-  if (size <= C.tiny_max) cls = 0;
-  else if (size <= C.med_max && nb <= C.med_bmax) cls = 1;
-  else if (size <= C.large_max) cls = 2;
-  else cls = 3;
+  int cls = 4;
+  if (size <= C.max_m[0] && nb <= C.max_b[0]) cls = 0;
+  else if (size <= C.max_m[1] && nb <= C.max_b[1]) cls = 1;
+  else if (size <= C.max_m[2] && nb <= C.max_b[2]) cls = 2;
+  else if (size <= C.large_max) cls = 3;
   const int slot = atomicAdd(&cnt[cls], 1);
   lists[cls * nisl + slot] = k;
 }
@@ -952,160 +957,167 @@ struct IslandSolve {
   int max_sweeps, check_every, stop_rule, warm;
 };
 
-// tiny islands: one lane, serial sweeps in island (colour) order
-__global__ void k_island_tiny(BodyPtrs B, MfPtrs M, const SysDev* __restrict__ sys, const int* __restrict__ istart,
-                              const int* __restrict__ list, int count, IslandSolve P, IslandStats* S) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= count) return;
-  const int k = list[t];
-  const int s = istart[k], e = istart[k + 1];
-  const double mu = sys[B.sys[M.a[s]]].mu;
-  if (P.warm)
-    for (int i = s; i < e; ++i) warm_apply_one(B, M, i);
-  int sweeps = 0;
-  double res = 0.0, res_ref = 0.0;
-  if (P.stop_rule == 1)
-    for (int i = s; i < e; ++i) res = dmax(res, residual_one(B, M, i, P.K, mu));
-  const int kk = P.stop_rule == 1 ? P.check_every : 1;
-  while (sweeps < P.max_sweeps && !(P.stop_rule == 1 && res <= P.tol)) {
-    for (int it = 0; it < kk; ++it) {
-      res_ref = 0.0;
-      for (int i = s; i < e; ++i) res_ref = dmax(res_ref, psor_update(B, M, i, P.K, mu));
-    }
-    sweeps += kk;
-    res = 0.0;
-    for (int i = s; i < e; ++i) res = dmax(res, residual_one(B, M, i, P.K, mu));
-    if (P.stop_rule == 0 && res_ref < 1e-6) break;
-  }
-  island_report(S, sweeps, e - s, res, res_ref, P.stop_rule == 1 && !(res <= P.tol));
-}
-
-constexpr int MED_THREADS = 64;
-constexpr int MED_CAP = 128;   // manifolds staged in LDS
-constexpr int MED_BCAP = 160;  // bodies staged in LDS
 constexpr int LARGE_THREADS = 256;
 
-__device__ __forceinline__ double wave_max64(double v) {
-  for (int off = 32; off > 0; off >>= 1) {
-    const double o = __shfl_down(v, off);
-    v = (o != o || o > v) ? o : v;
+// One manifold held in registers (every field index is a compile-time
+// constant after inlining) with its bodies as local LDS slots.
+struct MfReg {
+  double f[F_COUNT];
+  double lam[3];
+  int a, b;
+  __device__ __forceinline__ double at(int field, int) const { return f[field]; }
+  __device__ __forceinline__ d3 get3(int field, int) const { return mk3(f[field], f[field + 1], f[field + 2]); }
+  __device__ __forceinline__ int ia(int) const { return a; }
+  __device__ __forceinline__ int ib(int) const { return b; }
+  __device__ __forceinline__ double& lr(int c, int) { return lam[c]; }
+};
+
+__device__ __forceinline__ double nan_max(double v, double o) { return (o != o || o > v) ? o : v; }
+
+// max over the GROUP lanes of one island (GROUP <= 64: shuffles inside the
+// wavefront; larger groups combine the per-wave maxima through LDS)
+template <int GROUP>
+__device__ __forceinline__ double group_max(double v, double* red) {
+  constexpr int W = GROUP < 64 ? GROUP : 64;
+  for (int off = W / 2; off > 0; off >>= 1) v = nan_max(v, __shfl_xor(v, off, W));
+  if constexpr (GROUP > 64) {
+    constexpr int NW = GROUP / 64;
+    const int wv = threadIdx.x >> 6, base = (threadIdx.x / GROUP) * NW;
+    if ((threadIdx.x & 63) == 0) red[wv] = v;
+    __syncthreads();
+    v = red[base];
+    for (int q = 1; q < NW; ++q) v = nan_max(v, red[base + q]);
+    __syncthreads();
   }
-  return __shfl(v, 0);
+  return v;
 }
 
-// medium islands: one 64-lane workgroup, island data staged in LDS
-__global__ void __launch_bounds__(MED_THREADS) k_island_medium(BodyPtrs B, MfPtrs M, const SysDev* __restrict__ sys,
-                                                              const int* __restrict__ istart,
-                                                              const uint32_t* __restrict__ skey,
-                                                              const int* __restrict__ bseg,
-                                                              const int* __restrict__ sbody,
-                                                              const int* __restrict__ list, IslandSolve P,
-                                                              IslandStats* S) {
-  __shared__ double f_l[F_COUNT * MED_CAP];
-  __shared__ double lam_l[3 * MED_CAP];
-  __shared__ int a_l[MED_CAP], b_l[MED_CAP];
-  __shared__ double v_l[3 * MED_BCAP], w_l[3 * MED_BCAP];
-  __shared__ int body_l[MED_BCAP];
-  __shared__ int cs[129];
-  const int k = list[blockIdx.x];
-  const int s = istart[k], e = istart[k + 1], n = e - s;
-  const int bs = bseg[2 * k], nb = bseg[2 * k + 1] - bs;
-  const int tid = threadIdx.x;
-  for (int j = tid; j < nb; j += MED_THREADS) {
-    const int g = sbody[bs + j];
-    body_l[j] = g;
+__device__ __forceinline__ int lb_int(const int* a, int n, int key) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Island solver with the manifolds in registers: GROUP lanes per island (one
+// manifold each), WG / GROUP islands per workgroup, the island's body
+// velocities in LDS.  Colours are swept in order and the manifolds of one
+// colour in parallel: they share no body, so every sweep equals the serial
+// sweep in colour order bit for bit (nsc_contact.cpp:385-552 per update).
+template <int GROUP, int WG>
+__global__ void __launch_bounds__(WG) k_island_reg(BodyPtrs B, MfPtrs M, const SysDev* __restrict__ sys,
+                                                   const int* __restrict__ istart, const uint32_t* __restrict__ skey,
+                                                   const int* __restrict__ bseg, const int* __restrict__ sbody,
+                                                   const int* __restrict__ list, int count, IslandSolve P,
+                                                   IslandStats* S) {
+  constexpr int IPB = WG / GROUP;     // islands per workgroup
+  constexpr int BCAP = GROUP + 8;     // bodies per island (a connected island of m manifolds has <= m + 1)
+  __shared__ double v_l[3][IPB * BCAP], w_l[3][IPB * BCAP];
+  __shared__ int body_l[IPB * BCAP];
+  __shared__ double red[WG / 64 + 1];
+  __shared__ int s_ncol;
+  const int tid = threadIdx.x, slot = tid / GROUP, j = tid % GROUP;
+  const int t = blockIdx.x * IPB + slot;
+  const bool has = t < count;
+  int s = 0, n = 0, bs = 0, nb = 0;
+  if (has) {
+    const int k = list[t];
+    s = istart[k];
+    n = istart[k + 1] - s;
+    bs = bseg[2 * k];
+    nb = bseg[2 * k + 1] - bs;
+  }
+  int* bl = body_l + slot * BCAP;
+  for (int q = j; q < nb; q += GROUP) {
+    const int g = sbody[bs + q];
+    bl[q] = g;
     for (int c = 0; c < 3; ++c) {
-      v_l[c * MED_BCAP + j] = B.v[c][g];
-      w_l[c * MED_BCAP + j] = B.w[c][g];
+      v_l[c][slot * BCAP + q] = B.v[c][g];
+      w_l[c][slot * BCAP + q] = B.w[c][g];
     }
   }
+  if (tid == 0) s_ncol = 0;
   __syncthreads();
-  for (int j = tid; j < n; j += MED_THREADS) {
+  const bool live = has && j < n;
+  MfReg R;
+  int col = -1;
+  if (live) {
     const int i = s + j;
-    for (int f = 0; f < F_COUNT; ++f) f_l[f * MED_CAP + j] = M.at(f, i);
-    for (int c = 0; c < 3; ++c) lam_l[c * MED_CAP + j] = M.lam[c][i];
-    // local body slots (the island's bodies are sorted by global index)
-    int lo = 0, hi = nb;
-    const int ga = M.a[i];
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (body_l[mid] < ga) lo = mid + 1;
-      else hi = mid;
-    }
-    a_l[j] = lo;
+#pragma unroll
+    for (int f = 0; f < F_COUNT; ++f) R.f[f] = M.at(f, i);
+    for (int c = 0; c < 3; ++c) R.lam[c] = M.lam[c][i];
+    R.a = slot * BCAP + lb_int(bl, nb, M.a[i]);
     const int gb = M.b[i];
-    if (gb >= 0) {
-      lo = 0;
-      hi = nb;
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (body_l[mid] < gb) lo = mid + 1;
-        else hi = mid;
-      }
-      b_l[j] = lo;
-    } else {
-      b_l[j] = -1;
-    }
+    R.b = gb >= 0 ? slot * BCAP + lb_int(bl, nb, gb) : -1;
+    col = (int)(skey[i] & 127u);
+    atomicMax(&s_ncol, col + 1);
+  } else {
+#pragma unroll
+    for (int f = 0; f < F_COUNT; ++f) R.f[f] = 0.0;
+    R.lam[0] = R.lam[1] = R.lam[2] = 0.0;
+    R.a = R.b = 0;
   }
-  // colour boundaries inside the island (manifolds are sorted by colour)
-  const uint32_t base = skey[s] & ~127u;
-  const int ncol = (int)(skey[e - 1] & 127u) + 1;
-  for (int c = tid; c <= ncol; c += MED_THREADS) cs[c] = lb_u32(skey, s, e, base | (uint32_t)c) - s;
   __syncthreads();
-  MfPtrs L;
-  L.f = f_l;
-  L.stride = MED_CAP;
-  L.a = a_l;
-  L.b = b_l;
-  for (int c = 0; c < 3; ++c) L.lam[c] = lam_l + c * MED_CAP;
+  const int ncol = s_ncol;
   BodyPtrs LB = B;
   for (int c = 0; c < 3; ++c) {
-    LB.v[c] = v_l + c * MED_BCAP;
-    LB.w[c] = w_l + c * MED_BCAP;
+    LB.v[c] = v_l[c];
+    LB.w[c] = w_l[c];
   }
-  const double mu = sys[B.sys[M.a[s]]].mu;
+  const double mu = has ? sys[B.sys[M.a[s]]].mu : 0.0;
   if (P.warm) {
     for (int c = 0; c < ncol; ++c) {
-      for (int j = cs[c] + tid; j < cs[c + 1]; j += MED_THREADS) warm_apply_one(LB, L, j);
+      if (live && col == c) warm_apply_one(LB, R, 0);
       __syncthreads();
     }
   }
   int sweeps = 0;
   double res = 0.0, res_ref = 0.0;
+  bool done = !has || P.max_sweeps <= 0;
   if (P.stop_rule == 1) {
-    double r = 0.0;
-    for (int j = tid; j < n; j += MED_THREADS) r = dmax(r, residual_one(LB, L, j, P.K, mu));
-    res = wave_max64(r);
+    res = group_max<GROUP>(live ? residual_one(LB, R, 0, P.K, mu) : 0.0, red);
+    done = done || res <= P.tol;
   }
   const int kk = P.stop_rule == 1 ? P.check_every : 1;
-  while (sweeps < P.max_sweeps && !(P.stop_rule == 1 && res <= P.tol)) {
+  while (__syncthreads_or(!done)) {
+    const bool act = live && !done;
     double d = 0.0;
     for (int it = 0; it < kk; ++it) {
-      d = 0.0;
       for (int c = 0; c < ncol; ++c) {
-        for (int j = cs[c] + tid; j < cs[c + 1]; j += MED_THREADS) d = dmax(d, psor_update(LB, L, j, P.K, mu));
+        if (act && col == c) d = psor_update(LB, R, 0, P.K, mu);
         __syncthreads();
       }
     }
-    sweeps += kk;
-    res_ref = wave_max64(d);
-    double r = 0.0;
-    for (int j = tid; j < n; j += MED_THREADS) r = dmax(r, residual_one(LB, L, j, P.K, mu));
-    res = wave_max64(r);
-    if (P.stop_rule == 0 && res_ref < 1e-6) break;
-  }
-  __syncthreads();
-  for (int j = tid; j < n; j += MED_THREADS)
-    for (int c = 0; c < 3; ++c) M.lam[c][s + j] = lam_l[c * MED_CAP + j];
-  for (int j = tid; j < nb; j += MED_THREADS) {
-    const int g = body_l[j];
-    for (int c = 0; c < 3; ++c) {
-      B.v[c][g] = v_l[c * MED_BCAP + j];
-      B.w[c][g] = w_l[c * MED_BCAP + j];
+    const double rr = group_max<GROUP>(act ? d : 0.0, red);
+    const double rs = group_max<GROUP>(act ? residual_one(LB, R, 0, P.K, mu) : 0.0, red);
+    if (!done) {
+      sweeps += kk;
+      res_ref = rr;
+      res = rs;
+      if (P.stop_rule == 0 && res_ref < 1e-6) done = true;
+      if (P.stop_rule == 1 && res <= P.tol) done = true;
+      if (sweeps >= P.max_sweeps) done = true;
     }
   }
-  if (tid == 0) island_report(S, sweeps, n, res, res_ref, P.stop_rule == 1 && !(res <= P.tol));
+  if (live)
+    for (int c = 0; c < 3; ++c) M.lam[c][s + j] = R.lam[c];
+  for (int q = j; q < nb; q += GROUP) {
+    const int g = bl[q];
+    for (int c = 0; c < 3; ++c) {
+      B.v[c][g] = v_l[c][slot * BCAP + q];
+      B.w[c][g] = w_l[c][slot * BCAP + q];
+    }
+  }
+  if (has && j == 0) island_report(S, sweeps, n, res, res_ref, P.stop_rule == 1 && !(res <= P.tol));
 }
+
+// island classes: manifolds and bodies per island for the register kernels
+constexpr int ISL_G0 = 8, ISL_WG0 = 64;     // tiny: 8 islands per wavefront
+constexpr int ISL_G1 = 64, ISL_WG1 = 64;    // small: one island per wavefront
+constexpr int ISL_G2 = 256, ISL_WG2 = 256;  // medium: one island per 4-wave workgroup
 
 // large islands: one 256-lane workgroup on global memory
 __global__ void __launch_bounds__(LARGE_THREADS) k_island_large(BodyPtrs B, MfPtrs M, const SysDev* __restrict__ sys,
@@ -1251,7 +1263,7 @@ int nsc_build_run(rdk_ctx* ctx, const rdk_nsc_params* p) {
     }
     C.dist = ctx->cdist.p;
     C.hs = ctx->chs.p;
-    const int warm = p->warm_mode == 1 ? 1 : 0;
+    const int warm = p->warm_mode != 0 ? 1 : 0;  // the learned start corrects the previous-step one
     ProfScope ps(ctx, K_MANIFOLD);
     hipLaunchKernelGGL(k_manifold, dim3(grid_for(nc, nb)), dim3(nb), 0, ctx->stream, B, C, nc, M, p->cfm, warm,
                        ctx->prev_keys.p, ctx->nprev, ctx->prev_lam[0].p, ctx->prev_lam[1].p, ctx->prev_lam[2].p);
@@ -1268,7 +1280,7 @@ int nsc_build_run(rdk_ctx* ctx, const rdk_nsc_params* p) {
   ctx->have_manifolds = true;
   ctx->have_coloring = false;
   ctx->work[W_MANIFOLDS] += nm;
-  if (p->warm_mode == 2 && nc > 0) RDK_TRY(mlp_warm_start(ctx));
+  if (p->warm_mode == 2 && nm > 0) RDK_TRY(mlp_warm_start(ctx));
   return RDK_OK;
 }
 
@@ -1532,7 +1544,7 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
   RDK_TRY(ensure(ctx, ctx->iflag, nm + 1));
   RDK_TRY(ensure(ctx, ctx->ipos, nm + 1));
   RDK_TRY(ensure(ctx, ctx->istart, nm + 2));
-  RDK_TRY(ensure(ctx, ctx->ilists, 4 * (size_t)nm + 4));
+  RDK_TRY(ensure(ctx, ctx->ilists, 5 * (size_t)nm + 5));
   RDK_TRY(ensure(ctx, ctx->bseg, 2 * (size_t)nm + 2));
   RDK_TRY(ensure(ctx, ctx->icnt, 8));
   RDK_TRY(ensure(ctx, ctx->istats, sizeof(IslandStats)));
@@ -1583,20 +1595,23 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
   RDK_HIP(hipStreamSynchronize(ctx->stream));
   const int nisl = last[0] + last[1];
   IslandClass C;
-  C.tiny_max = 8;
-  C.med_max = MED_CAP;
-  C.med_bmax = MED_BCAP;
+  C.max_m[0] = ISL_G0;
+  C.max_m[1] = ISL_G1;
+  C.max_m[2] = ISL_G2;
+  C.max_b[0] = ISL_G0 + 8;
+  C.max_b[1] = ISL_G1 + 8;
+  C.max_b[2] = ISL_G2 + 8;
   C.large_max = 8192;
   {
     ProfScope ps(ctx, K_ISLAND);
     hipLaunchKernelGGL(k_island_classify, dim3(grid_for(nisl, nb)), dim3(nb), 0, ctx->stream, ctx->istart.p, nisl,
                        skey, blab, n, C, ctx->ilists.p, ctx->icnt.p, ctx->bseg.p);
   }
-  int cnt[4];
-  RDK_HIP(hipMemcpyAsync(cnt, ctx->icnt.p, 4 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  int cnt[5];
+  RDK_HIP(hipMemcpyAsync(cnt, ctx->icnt.p, 5 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   RDK_HIP(hipStreamSynchronize(ctx->stream));
   ctx->nislands = nisl;
-  if (cnt[3] > 0) return RDK_OK;  // a giant island: global coloured path
+  if (cnt[4] > 0) return RDK_OK;  // a giant island: global coloured path
   IslandSolve P;
   P.K = solve_const(ctx, p);
   P.tol = p->tol;
@@ -1608,8 +1623,8 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
   RDK_HIP(hipMemsetAsync(S, 0, sizeof(IslandStats), ctx->stream));
   BodyPtrs B = body_ptrs(ctx);
   MfPtrs Mc = mf_ptrs(ctx, true);
-  // tiny islands run on the side stream, concurrently with the workgroup islands
-  const bool side = cnt[0] > 0 && (cnt[1] > 0 || cnt[2] > 0) && ctx->stream2 && !ctx->profiling;
+  // tiny islands run on the side stream, concurrently with the other classes
+  const bool side = cnt[0] > 0 && (cnt[1] > 0 || cnt[2] > 0 || cnt[3] > 0) && ctx->stream2 && !ctx->profiling;
   if (cnt[0] > 0) {
     hipStream_t s_tiny = ctx->stream;
     if (side) {
@@ -1618,19 +1633,26 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
       s_tiny = ctx->stream2;
     }
     ProfScope ps(ctx, K_ISL_TINY);
-    hipLaunchKernelGGL(k_island_tiny, dim3(grid_for(cnt[0], 64)), dim3(64), 0, s_tiny, B, Mc, ctx->sys_dev.p,
-                       ctx->istart.p, ctx->ilists.p, cnt[0], P, S);
+    constexpr int ipb = ISL_WG0 / ISL_G0;
+    hipLaunchKernelGGL((k_island_reg<ISL_G0, ISL_WG0>), dim3(grid_for(cnt[0], ipb)), dim3(ISL_WG0), 0, s_tiny, B, Mc,
+                       ctx->sys_dev.p, ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p, cnt[0], P, S);
     if (side) RDK_HIP(hipEventRecord(ctx->ev_join, ctx->stream2));
   }
   if (cnt[1] > 0) {
-    ProfScope ps(ctx, K_ISL_MEDIUM);
-    hipLaunchKernelGGL(k_island_medium, dim3(cnt[1]), dim3(MED_THREADS), 0, ctx->stream, B, Mc, ctx->sys_dev.p,
-                       ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p + nisl, P, S);
+    ProfScope ps(ctx, K_ISL_SMALL);
+    hipLaunchKernelGGL((k_island_reg<ISL_G1, ISL_WG1>), dim3(cnt[1]), dim3(ISL_WG1), 0, ctx->stream, B, Mc,
+                       ctx->sys_dev.p, ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p + nisl, cnt[1], P, S);
   }
   if (cnt[2] > 0) {
+    ProfScope ps(ctx, K_ISL_MEDIUM);
+    hipLaunchKernelGGL((k_island_reg<ISL_G2, ISL_WG2>), dim3(cnt[2]), dim3(ISL_WG2), 0, ctx->stream, B, Mc,
+                       ctx->sys_dev.p, ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p + 2 * nisl, cnt[2], P,
+                       S);
+  }
+  if (cnt[3] > 0) {
     ProfScope ps(ctx, K_ISL_LARGE);
-    hipLaunchKernelGGL(k_island_large, dim3(cnt[2]), dim3(LARGE_THREADS), 0, ctx->stream, B, Mc, ctx->sys_dev.p,
-                       ctx->istart.p, skey, ctx->ilists.p + 2 * nisl, P, S);
+    hipLaunchKernelGGL(k_island_large, dim3(cnt[3]), dim3(LARGE_THREADS), 0, ctx->stream, B, Mc, ctx->sys_dev.p,
+                       ctx->istart.p, skey, ctx->ilists.p + 3 * nisl, P, S);
   }
   if (side) RDK_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
   {
@@ -1649,9 +1671,7 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
   }
   ctx->ncolor = (int)hc[1] + 1;
   ctx->have_coloring = false;  // the solve-ordered copy is island-ordered
-  ctx->island_counts[0] = cnt[0];
-  ctx->island_counts[1] = cnt[1];
-  ctx->island_counts[2] = cnt[2];
+  for (int c = 0; c < 4; ++c) ctx->island_counts[c] = cnt[c];
   ctx->work[W_SWEEP_CONTACTS] += (int64_t)hs.manifold_sweeps;
   *status = hs.nonconv > 0 ? RDK_ERR_NOCONV : RDK_OK;
   if (st) {

@@ -14,9 +14,11 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "librdk.so")
 RDK_OK, RDK_ERR_HIP, RDK_ERR_ARG, RDK_ERR_CAPACITY, RDK_ERR_STATE, RDK_ERR_NOCONV = range(6)
 RDK_NSC_FREE, RDK_NSC_POS, RDK_VV_HALF1, RDK_VV_HALF2, RDK_CLEAR = range(5)
 MF_FIELDS = 42  # RDK_MF_FIELDS
+MLP_IN, MLP_OUT, MLP_H = 12, 3, 64  # RDK_MLP_IN, RDK_MLP_OUT, hidden width
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
+_fp = C.POINTER(C.c_float)
 
 
 class SystemDesc(C.Structure):
@@ -109,6 +111,15 @@ _SIGS = {
     "rdk_gen_rsa": (C.c_int, [C.c_int, _dp, _dp, C.POINTER(SystemDesc), C.c_double, C.c_uint64, C.c_int, _dp, _dp,
                               _ip]),
     "rdk_gen_thermal": (C.c_int, [C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_uint64, _dp, _dp]),
+    "rdk_mlp_num_params": (C.c_int, []),
+    "rdk_mlp_init": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "rdk_mlp_set_params": (C.c_int, [C.c_void_p, _fp, C.c_int]),
+    "rdk_mlp_get_params": (C.c_int, [C.c_void_p, _fp, C.c_int]),
+    "rdk_mlp_features": (C.c_int, [C.c_void_p, C.c_int, _fp, _dp]),
+    "rdk_mlp_infer": (C.c_int, [C.c_void_p, _fp, C.c_int, _fp]),
+    "rdk_mlp_grad": (C.c_int, [C.c_void_p, _fp, _fp, C.c_int, _fp, _fp]),
+    "rdk_mlp_train": (C.c_int, [C.c_void_p, _fp, _fp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_uint64,
+                                _fp]),
 }
 
 _lib = None
@@ -140,6 +151,13 @@ def dptr(a):
         return None
     assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], "expected C-contiguous float64"
     return a.ctypes.data_as(_dp)
+
+
+def fptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"], "expected C-contiguous float32"
+    return a.ctypes.data_as(_fp)
 
 
 def iptr(a):
@@ -341,7 +359,7 @@ class Context:
 
     def island_info(self):
         n = C.c_int()
-        cnt = np.zeros(3, np.int32)
+        cnt = np.zeros(4, np.int32)
         self._check(self._L.rdk_nsc_island_info(self._h, C.byref(n), iptr(cnt)))
         return n.value, cnt.tolist()
 
@@ -393,6 +411,48 @@ class Context:
         names = ["steps", "body_passes", "chunks", "candidates", "contacts", "manifolds", "sweep_contacts",
                  "resid_contacts", "integrate_bodies", "force_contacts"]
         return {nm: arr[i] for i, nm in enumerate(names)}
+
+    # ---- learned warm start ----
+    def mlp_init(self, seed=0):
+        self._check(self._L.rdk_mlp_init(self._h, int(seed)))
+
+    def mlp_set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        self._check(self._L.rdk_mlp_set_params(self._h, fptr(p), int(p.size)))
+
+    def mlp_get_params(self):
+        p = np.zeros(self._L.rdk_mlp_num_params(), np.float32)
+        self._check(self._L.rdk_mlp_get_params(self._h, fptr(p), int(p.size)))
+        return p
+
+    def mlp_features(self, nm):
+        X = np.zeros((max(nm, 1), MLP_IN), np.float32)
+        ls = np.zeros(max(nm, 1), np.float64)
+        self._check(self._L.rdk_mlp_features(self._h, int(nm), fptr(X), dptr(ls)))
+        return X[:nm], ls[:nm]
+
+    def mlp_infer(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        n = X.shape[0]
+        Y = np.zeros((max(n, 1), MLP_OUT), np.float32)
+        self._check(self._L.rdk_mlp_infer(self._h, fptr(X), int(n), fptr(Y)))
+        return Y[:n]
+
+    def mlp_grad(self, X, T):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        g = np.zeros(self._L.rdk_mlp_num_params(), np.float32)
+        loss = np.zeros(1, np.float32)
+        self._check(self._L.rdk_mlp_grad(self._h, fptr(X), fptr(T), int(X.shape[0]), fptr(g), fptr(loss)))
+        return g, float(loss[0])
+
+    def mlp_train(self, X, T, epochs=10, batch=256, lr=1e-3, weight_decay=0.0, seed=0):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        loss = np.zeros(1, np.float32)
+        self._check(self._L.rdk_mlp_train(self._h, fptr(X), fptr(T), int(X.shape[0]), int(epochs), int(batch),
+                                          float(lr), float(weight_decay), int(seed), fptr(loss)))
+        return float(loss[0])
 
 
 def gen_rsa(r, h, desc, gap=0.0, seed=0, max_attempts=2000):

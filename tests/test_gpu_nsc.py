@@ -208,7 +208,7 @@ def test_island_sweeps_bitwise_equal_serial_sweeps_in_solve_order(gpu_ctx_factor
     s0 = ctx.download_state()
     st = ctx.nsc_solve(p, allow_noconv=True)
     nisl, counts = ctx.island_info()
-    assert counts[0] > 0 and counts[1] > 0, counts  # tiny and medium islands both present
+    assert counts[0] > 0 and counts[1] > 0 and counts[2] > 0, counts  # tiny, small and medium islands present
     order = ctx.solve_order(nm)
     _, _, lam_g = ctx.nsc_manifolds(nm)
     s1 = ctx.download_state()
