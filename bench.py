@@ -45,6 +45,9 @@ def parse():
     ap.add_argument("--chunk-len", type=float, default=4.0)
     ap.add_argument("--check-every", type=int, default=4)
     ap.add_argument("--block", type=int, default=250)
+    ap.add_argument("--ab-steps", type=int, default=20,
+                    help="steps after the timed protocol solved from cold, previous-step and learned starts")
+    ap.add_argument("--weights", default=os.path.join(ROOT, "rod-dynamics-3d_amd", "weights", "warmstart_c2.npy"))
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (HBM counter calibration)")
     return ap.parse_args()
@@ -81,6 +84,28 @@ def cpu_baseline(sc, nsteps):
             "sample": f"steps 2..{nsteps + 1} of the same N={sc.n} settling run (same seed), FP64 oracle: reference "
                       f"spatial hash + serial sorted-order PSOR to residual 1e-8, OpenMP {cores} threads",
             "seconds": dt, "sweeps_per_step": sweeps / max(nsteps, 1)}
+
+
+def learned_ab(ctx, sc, a):
+    """Sweeps/step from the cold, previous-step and learned starts on the same
+    manifolds and pre-solve velocities, over the steps that follow the timed
+    protocol (the settled bed)."""
+    import numpy as np
+    from rod_dynamics_py import warmstart as WS
+
+    if not os.path.exists(a.weights):
+        return {"status": f"no trained weights at {os.path.relpath(a.weights, ROOT)}"}
+    ctx.set_profiling(False)
+    ctx.mlp_set_params(np.load(a.weights))
+    rows = []
+    for k in range(a.ab_steps):
+        rows.append(WS.ab_step(ctx, sc.detect, sc.params))
+        if (k + 1) % 5 == 0:
+            print(f"[bench] learned-start A/B step {k + 1}/{a.ab_steps}", file=sys.stderr, flush=True)
+    res = WS.ab_summary(rows)
+    res["weights"] = os.path.relpath(a.weights, ROOT)
+    res["bed"] = f"steps {a.warmup + a.steps + 1}..{a.warmup + a.steps + a.ab_steps} of the same run"
+    return res
 
 
 def main():
@@ -170,6 +195,8 @@ def main():
                      "launches": launches, "avg_launch_ms": kms / max(launches, 1),
                      "bytes_per_unit": bpu, "unit_name": unit, "units": units},
     }
+    if a.ab_steps > 0:
+        out["learned_warm_start"] = learned_ab(ctx, sc, a)
     ctx.close()
     if not a.no_cpu:
         print(f"[bench] cpu baseline: {a.cpu_steps} oracle steps", file=sys.stderr, flush=True)

@@ -70,6 +70,10 @@ typedef struct rdk_nsc_params {
   int warm_walls;          /* 1: plane-wall manifolds also start from the previous
                               step's impulses, keyed by (rod, plane, endpoint); the
                               reference starts wall contacts cold */
+  int stall_sweeps;        /* stop_rule 1 only; 0 = off.  > 0: at sweeps = k, 2k,
+                              4k, ... an island (or the global solve) whose residual
+                              has not at least halved since the previous milestone
+                              stops there and counts as not converged */
 } rdk_nsc_params;
 
 typedef struct rdk_soft_params { /* SoftContactCfg, config.hpp:6-27 */

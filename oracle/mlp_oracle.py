@@ -183,16 +183,19 @@ def features(mf, a, b, state, r, mu_of_body, vs, prev=None, prev_wall=None):
 
 
 def apply_prediction(Y, ls, mu, lam_prev=None):
-    """Start impulses: previous-step impulses + prediction (units of lscale),
+    """Start impulses: manifolds with a previous normal impulse keep the
+    previous-step impulses; the others take the prediction (units of lscale)
     projected onto lambda_n >= 0 and the friction disc |lambda_t| <= mu lambda_n."""
     Y = np.asarray(Y, np.float64)
     base = np.zeros((Y.shape[0], 3)) if lam_prev is None else np.asarray(lam_prev, np.float64)
-    ln = np.maximum(0.0, base[:, 0] + Y[:, 0] * ls)
-    t = base[:, 1:3] + Y[:, 1:3] * ls[:, None]
+    ln = np.maximum(0.0, Y[:, 0] * ls)
+    t = Y[:, 1:3] * ls[:, None]
     lim = mu * ln
     nt = np.linalg.norm(t, axis=1)
     sc = np.where(nt > lim, np.where(nt > 0, lim / np.where(nt > 0, nt, 1.0), 0.0), 1.0)
-    return np.column_stack([ln, t * sc[:, None]])
+    pred = np.column_stack([ln, t * sc[:, None]])
+    keep = base[:, 0] > 0.0
+    return np.where(keep[:, None], base, pred)
 
 
 def adamw_step(p, g, m, v, t, lr, wd, b1=0.9, b2=0.999, eps=1e-8):

@@ -115,7 +115,6 @@ int rdk_destroy(rdk_ctx* ctx) {
   mlp_free(ctx);
   for (int k = 0; k < 3; ++k) {
     release(ctx->x[k]); release(ctx->v[k]); release(ctx->w[k]); release(ctx->f[k]); release(ctx->tau[k]);
-    release(ctx->sc[k]); release(ctx->sx[k]);
     release(ctx->cpa[k]); release(ctx->cpb[k]); release(ctx->cn[k]); release(ctx->csh[k]);
     release(ctx->lam[k]); release(ctx->lamc[k]); release(ctx->prev_lam[k]); release(ctx->prev_wlam[k]);
     release(ctx->hxi[k]); release(ctx->hxi_new[k]);
@@ -127,7 +126,7 @@ int rdk_destroy(rdk_ctx* ctx) {
   release(ctx->sys_of); release(ctx->sys_dev);
   release(ctx->chunk_rod); release(ctx->chunk_k); release(ctx->rod_nchunk);
   release(ctx->ckey); release(ctx->ckey_alt); release(ctx->cid); release(ctx->cid_alt);
-  release(ctx->cell_start); release(ctx->cell_end); release(ctx->srad); release(ctx->scf); release(ctx->srod); release(ctx->sk);
+  release(ctx->cell_start); release(ctx->scf); release(ctx->srod); release(ctx->sk);
   release(ctx->pkeys); release(ctx->pkeys_alt); release(ctx->counters);
   release(ctx->ca); release(ctx->cb); release(ctx->ctype); release(ctx->cdist); release(ctx->chs);
   release(ctx->cs); release(ctx->ct);

@@ -11,13 +11,17 @@
 //                      in the same or adjacent cells (27-cell stencil, periodic
 //                      wrap on periodic axes).
 //   2. radix sort of (cell key, chunk id)
-//   3. cell_ranges     first/last sorted entry of every occupied cell
-//   4. chunk_gather    chunk centre, rod centre, radius, rod id, chunk index in
-//                      sorted (cell) order so the stencil walk reads contiguous
-//                      memory
-//   5. broad_narrow    one thread per sorted chunk walks the 27 cells; each
-//                      unordered chunk pair is examined once (by the lower
-//                      sorted position).  An FP32 bounding-sphere test with the
+//   3. cell_start      dense start array: start[c] = first sorted entry with
+//                      cell >= c, for every cell (and one past the last), so
+//                      the three x-adjacent cells of a stencil row are one
+//                      contiguous range [start[x-1], start[x+2])
+//   4. chunk_gather    FP32 (centre, radius), rod id and chunk index in sorted
+//                      (cell) order so the stencil walk reads contiguous memory
+//   5. broad_narrow    one thread per sorted chunk walks the 9 stencil rows
+//                      (27 cells); each unordered chunk pair is examined once
+//                      (by the lower sorted position).  Workgroups are mapped
+//                      so that each XCD (own L2) walks one contiguous run of
+//                      the sorted chunks, i.e. one slab of the grid.  An FP32 bounding-sphere test with the
 //                      chunk-centre minimum image and a rounding margin rejects
 //                      far pairs conservatively; survivors run the full-rod FP64
 //                      Lumelsky narrow phase.  A pair is emitted only by the
@@ -84,17 +88,21 @@ __global__ void k_fill_int(int* __restrict__ a, int n, int v) {
   if (i < n) a[i] = v;
 }
 
-__global__ void k_cell_ranges(const uint32_t* __restrict__ key, int M, int* __restrict__ cstart,
-                              int* __restrict__ cend) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= M) return;
-  uint32_t k = key[p];
-  if (p == 0 || key[p - 1] != k) cstart[k] = p;
-  if (p == M - 1 || key[p + 1] != k) cend[k] = p + 1;
+// start[c] = lower bound of cell c in the sorted keys, c = 0 .. ncell
+__global__ void k_cell_start(const uint32_t* __restrict__ key, int M, int ncell, int* __restrict__ start) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > ncell) return;
+  int lo = 0, hi = M;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (key[mid] < (uint32_t)c) lo = mid + 1;
+    else hi = mid;
+  }
+  start[c] = lo;
 }
 
 struct SortedChunks {
-  double *c[3], *x[3], *rad;
+  const uint32_t* key;  // sorted cell keys
   int *rod, *k;
   float4* cf;  // FP32 (centre, radius) for the conservative prefilter
 };
@@ -110,16 +118,18 @@ __global__ void k_chunk_gather(BodyPtrs B, const int* __restrict__ sorted_id, co
   d3 c;
   double rad;
   chunk_geom(B, i, k, rod_nchunk[i], c, rad);
-  S.c[0][p] = c.x;
-  S.c[1][p] = c.y;
-  S.c[2][p] = c.z;
-  S.x[0][p] = B.x[0][i];
-  S.x[1][p] = B.x[1][i];
-  S.x[2][p] = B.x[2][i];
-  S.rad[p] = rad;
   S.rod[p] = i;
   S.k[p] = k;
   S.cf[p] = make_float4((float)c.x, (float)c.y, (float)c.z, (float)rad);
+}
+
+// Workgroups are dispatched round-robin over the XCDs (8 on MI355X, each with
+// its own L2).  Renumber them so that XCD x runs one contiguous block range.
+constexpr int NUM_XCD = 8;
+__device__ __forceinline__ int xcd_block(int blk, int nblk) {
+  const int q = nblk / NUM_XCD, r = nblk % NUM_XCD;
+  const int x = blk % NUM_XCD, j = blk / NUM_XCD;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + j;
 }
 
 // Wavefront-aggregated append: one atomic per wavefront of active lanes.
@@ -140,17 +150,46 @@ __device__ __forceinline__ int owner_chunk(double s, int K) {
 }
 
 // One thread per sorted chunk p.  Every unordered chunk pair (p, q) is
-// examined once, by the thread with the lower sorted position, so the 27-cell
-// walk starts at q = p + 1.  The FP32 prefilter uses the chunk-centre minimum
+// examined once, by the thread with the lower sorted position, so each row
+// range is clipped to q > p.  The FP32 prefilter uses the chunk-centre minimum
 // image, which is never farther than the rod-centre image the narrow phase
 // uses, plus a rounding margin, so it never rejects a touching pair.
+__device__ __forceinline__ void chunk_pair(const BodyPtrs& B, const SysDev& S, const SortedChunks& SC,
+                                           const int* __restrict__ rod_nchunk, int p, int q, int a, bool a_static,
+                                           float4 me, const float* spanf, const float* ispanf, float extra,
+                                           double delta, uint64_t* __restrict__ out, unsigned cap,
+                                           unsigned* __restrict__ counter, unsigned long long& ncand) {
+  const float4 o = SC.cf[q];
+  float dxf = o.x - me.x, dyf = o.y - me.y, dzf = o.z - me.z;
+  if (S.periodic[0]) dxf -= spanf[0] * rintf(dxf * ispanf[0]);
+  if (S.periodic[1]) dyf -= spanf[1] * rintf(dyf * ispanf[1]);
+  if (S.periodic[2]) dzf -= spanf[2] * rintf(dzf * ispanf[2]);
+  const float thr = me.w + o.w + extra;
+  if (!(dxf * dxf + dyf * dyf + dzf * dzf < thr * thr)) return;
+  const int b = SC.rod[q];
+  if (b == a) return;
+  if (a_static && !(B.inv_mass[b] > 0.0)) return;
+  ++ncand;
+  // exact FP64 narrow phase in canonical (lower index first) order
+  const int A = a < b ? a : b, Bi = a < b ? b : a;
+  const int kA = a < b ? SC.k[p] : SC.k[q], kB = a < b ? SC.k[q] : SC.k[p];
+  NarrowOut no;
+  if (capsule_narrow(load3(B.x, A), axis_of(load_q(B, A)), B.h[A], B.r[A], load3(B.x, Bi), axis_of(load_q(B, Bi)),
+                     B.h[Bi], B.r[Bi], S.span, S.periodic, delta, false, no)) {
+    if (owner_chunk(no.s, rod_nchunk[A]) == kA && owner_chunk(no.t, rod_nchunk[Bi]) == kB) {
+      const unsigned pos = wave_append(counter);
+      if (pos < cap) out[pos] = ((uint64_t)(uint32_t)A << 32) | (uint64_t)(uint32_t)Bi;
+    }
+  }
+}
+
 __global__ void __launch_bounds__(256) k_broad_narrow(BodyPtrs B, const SysDev* __restrict__ sys, SortedChunks SC,
-                                                      const int* __restrict__ cstart, const int* __restrict__ cend,
+                                                      const int* __restrict__ cstart,
                                                       const int* __restrict__ rod_nchunk, int M, double delta,
                                                       uint64_t* __restrict__ out, unsigned cap,
                                                       unsigned* __restrict__ counter,
                                                       unsigned long long* __restrict__ cand_counter) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   unsigned long long ncand = 0;
   if (p < M) {
     const int a = SC.rod[p];
@@ -160,55 +199,51 @@ __global__ void __launch_bounds__(256) k_broad_narrow(BodyPtrs B, const SysDev* 
     const float spanf[3] = {(float)S.span[0], (float)S.span[1], (float)S.span[2]};
     const float ispanf[3] = {(float)(1.0 / S.span[0]), (float)(1.0 / S.span[1]), (float)(1.0 / S.span[2])};
     const float extra = (float)delta + S.fmargin;
-    const int cc[3] = {cell_coord(SC.c[0][p], S.lo[0], S.inv_cell[0], S.dims[0], S.periodic[0]),
-                       cell_coord(SC.c[1][p], S.lo[1], S.inv_cell[1], S.dims[1], S.periodic[1]),
-                       cell_coord(SC.c[2][p], S.lo[2], S.inv_cell[2], S.dims[2], S.periodic[2])};
+    const int d0 = S.dims[0], d1 = S.dims[1], d2 = S.dims[2];
+    const int cell = (int)SC.key[p] - S.cell_off;
+    const int cx = cell % d0, cy = (cell / d0) % d1, cz = cell / (d0 * d1);
+    // x extent of a stencil row as up to two contiguous cell ranges
+    int xa0, xa1, xb0 = 0, xb1 = 0;  // [xa0, xa1] and, after a periodic wrap, [xb0, xb1]
+    bool two = false;
+    if (d0 == 1) {
+      xa0 = xa1 = 0;
+    } else if (!S.periodic[0]) {
+      xa0 = cx > 0 ? cx - 1 : 0;
+      xa1 = cx < d0 - 1 ? cx + 1 : d0 - 1;
+    } else if (cx == 0) {
+      xa0 = 0;
+      xa1 = 1;
+      xb0 = xb1 = d0 - 1;
+      two = true;
+    } else if (cx == d0 - 1) {
+      xa0 = d0 - 2;
+      xa1 = d0 - 1;
+      xb0 = xb1 = 0;
+      two = true;
+    } else {
+      xa0 = cx - 1;
+      xa1 = cx + 1;
+    }
     for (int dz = -1; dz <= 1; ++dz) {
-      int z = cc[2] + dz;
+      int z = cz + dz;
       if (S.periodic[2]) {
-        if (S.dims[2] == 1 && dz != 0) continue;
-        z = (z + S.dims[2]) % S.dims[2];
-      } else if (z < 0 || z >= S.dims[2]) continue;
+        if (d2 == 1 && dz != 0) continue;
+        z = (z + d2) % d2;
+      } else if (z < 0 || z >= d2) continue;
       for (int dy = -1; dy <= 1; ++dy) {
-        int y = cc[1] + dy;
+        int y = cy + dy;
         if (S.periodic[1]) {
-          if (S.dims[1] == 1 && dy != 0) continue;
-          y = (y + S.dims[1]) % S.dims[1];
-        } else if (y < 0 || y >= S.dims[1]) continue;
-        for (int dx = -1; dx <= 1; ++dx) {
-          int x = cc[0] + dx;
-          if (S.periodic[0]) {
-            if (S.dims[0] == 1 && dx != 0) continue;
-            x = (x + S.dims[0]) % S.dims[0];
-          } else if (x < 0 || x >= S.dims[0]) continue;
-          const int cell = S.cell_off + x + S.dims[0] * (y + S.dims[1] * z);
-          const int q1 = cend[cell];
-          int q = cstart[cell];
+          if (d1 == 1 && dy != 0) continue;
+          y = (y + d1) % d1;
+        } else if (y < 0 || y >= d1) continue;
+        const int row = S.cell_off + d0 * (y + d1 * z);
+        for (int part = 0; part < (two ? 2 : 1); ++part) {
+          int q = cstart[row + (part ? xb0 : xa0)];
+          const int q1 = cstart[row + (part ? xb1 : xa1) + 1];
           if (q <= p) q = p + 1;
-          for (; q < q1; ++q) {
-            const float4 o = SC.cf[q];
-            float dxf = o.x - me.x, dyf = o.y - me.y, dzf = o.z - me.z;
-            if (S.periodic[0]) dxf -= spanf[0] * rintf(dxf * ispanf[0]);
-            if (S.periodic[1]) dyf -= spanf[1] * rintf(dyf * ispanf[1]);
-            if (S.periodic[2]) dzf -= spanf[2] * rintf(dzf * ispanf[2]);
-            const float thr = me.w + o.w + extra;
-            if (!(dxf * dxf + dyf * dyf + dzf * dzf < thr * thr)) continue;
-            const int b = SC.rod[q];
-            if (b == a) continue;
-            if (a_static && !(B.inv_mass[b] > 0.0)) continue;
-            ++ncand;
-            // exact FP64 narrow phase in canonical (lower index first) order
-            const int A = a < b ? a : b, Bi = a < b ? b : a;
-            const int kA = a < b ? SC.k[p] : SC.k[q], kB = a < b ? SC.k[q] : SC.k[p];
-            NarrowOut no;
-            if (capsule_narrow(load3(B.x, A), axis_of(load_q(B, A)), B.h[A], B.r[A], load3(B.x, Bi),
-                               axis_of(load_q(B, Bi)), B.h[Bi], B.r[Bi], S.span, S.periodic, delta, false, no)) {
-              if (owner_chunk(no.s, rod_nchunk[A]) == kA && owner_chunk(no.t, rod_nchunk[Bi]) == kB) {
-                const unsigned pos = wave_append(counter);
-                if (pos < cap) out[pos] = ((uint64_t)(uint32_t)A << 32) | (uint64_t)(uint32_t)Bi;
-              }
-            }
-          }
+          for (; q < q1; ++q)
+            chunk_pair(B, S, SC, rod_nchunk, p, q, a, a_static, me, spanf, ispanf, extra, delta, out, cap, counter,
+                       ncand);
         }
       }
     }
@@ -335,13 +370,7 @@ static int build_grid(rdk_ctx* ctx, const rdk_detect_params* p) {
   RDK_TRY(ensure(ctx, ctx->ckey_alt, M));
   RDK_TRY(ensure(ctx, ctx->cid, M));
   RDK_TRY(ensure(ctx, ctx->cid_alt, M));
-  RDK_TRY(ensure(ctx, ctx->cell_start, ctx->n_cells));
-  RDK_TRY(ensure(ctx, ctx->cell_end, ctx->n_cells));
-  for (int k = 0; k < 3; ++k) {
-    RDK_TRY(ensure(ctx, ctx->sc[k], M));
-    RDK_TRY(ensure(ctx, ctx->sx[k], M));
-  }
-  RDK_TRY(ensure(ctx, ctx->srad, M));
+  RDK_TRY(ensure(ctx, ctx->cell_start, (size_t)ctx->n_cells + 1));
   RDK_TRY(ensure(ctx, ctx->scf, M));
   RDK_TRY(ensure(ctx, ctx->srod, M));
   RDK_TRY(ensure(ctx, ctx->sk, M));
@@ -389,19 +418,11 @@ int detect_run(rdk_ctx* ctx, const rdk_detect_params* p) {
   const int* sid = alt ? ctx->cid_alt.p : ctx->cid.p;
   {
     ProfScope ps(ctx, K_CELL_RANGES);
-    hipLaunchKernelGGL(k_fill_int, dim3(grid_for(ctx->n_cells, nb)), dim3(nb), 0, ctx->stream, ctx->cell_start.p,
-                       ctx->n_cells, 0);
-    hipLaunchKernelGGL(k_fill_int, dim3(grid_for(ctx->n_cells, nb)), dim3(nb), 0, ctx->stream, ctx->cell_end.p,
-                       ctx->n_cells, 0);
-    hipLaunchKernelGGL(k_cell_ranges, dim3(grid_for(M, nb)), dim3(nb), 0, ctx->stream, skey, M, ctx->cell_start.p,
-                       ctx->cell_end.p);
+    hipLaunchKernelGGL(k_cell_start, dim3(grid_for((int64_t)ctx->n_cells + 1, nb)), dim3(nb), 0, ctx->stream, skey,
+                       M, ctx->n_cells, ctx->cell_start.p);
   }
   SortedChunks SC;
-  for (int k = 0; k < 3; ++k) {
-    SC.c[k] = ctx->sc[k].p;
-    SC.x[k] = ctx->sx[k].p;
-  }
-  SC.rad = ctx->srad.p;
+  SC.key = skey;
   SC.rod = ctx->srod.p;
   SC.k = ctx->sk.p;
   SC.cf = ctx->scf.p;
@@ -423,7 +444,7 @@ int detect_run(rdk_ctx* ctx, const rdk_detect_params* p) {
     {
       ProfScope ps(ctx, K_BROAD_NARROW);
       hipLaunchKernelGGL(k_broad_narrow, dim3(grid_for(M, nb)), dim3(nb), 0, ctx->stream, B, ctx->sys_dev.p, SC,
-                         ctx->cell_start.p, ctx->cell_end.p, ctx->rod_nchunk.p, M, p->delta, ctx->pkeys.p, cap,
+                         ctx->cell_start.p, ctx->rod_nchunk.p, M, p->delta, ctx->pkeys.p, cap,
                          ctx->counters.p, (unsigned long long*)(ctx->counters.p + 2));
     }
     RDK_HIP(hipMemcpyAsync(ctx->h_counters, ctx->counters.p, 4 * sizeof(unsigned), hipMemcpyDeviceToHost,

@@ -119,8 +119,7 @@ struct rdk_ctx {
   rdk::DevBuf<int> chunk_rod, chunk_k, rod_nchunk;
   rdk::DevBuf<uint32_t> ckey, ckey_alt;
   rdk::DevBuf<int> cid, cid_alt;
-  rdk::DevBuf<int> cell_start, cell_end;
-  rdk::DevBuf<double> sc[3], sx[3], srad;
+  rdk::DevBuf<int> cell_start;  // dense: n_cells + 1 entries
   rdk::DevBuf<float4> scf;
   rdk::DevBuf<int> srod, sk;
   rdk::DevBuf<uint64_t> pkeys, pkeys_alt;

@@ -439,6 +439,29 @@ struct SolveConst {
   int proj;
 };
 
+// Stall rule (stop_rule 1, stall_sweeps > 0): at sweep counts stall, 2 stall,
+// 4 stall, ... the residual must have at least halved since the previous such
+// milestone, otherwise the solve stops there (reported as not converged).
+// Wedged, statically indeterminate groups of rods let the impulses drift along
+// a null direction of the Delassus matrix with the residual on a floor; this
+// stops them without touching solves that converge (linear convergence halves
+// the residual many times over between milestones).
+struct StallTrack {
+  int next;
+  double ref;
+  __host__ __device__ void init(int stall) {
+    next = stall;
+    ref = -1.0;
+  }
+  __host__ __device__ bool stalled(int sweeps, double res) {
+    if (next <= 0 || sweeps < next) return false;
+    const bool st = ref >= 0.0 && !(res <= 0.5 * ref);
+    ref = res;
+    while (next <= sweeps) next *= 2;
+    return st;
+  }
+};
+
 // One projected Gauss-Seidel update of manifold i (nsc_contact.cpp:385-552).
 // Body velocities are read once into registers and written back once; the
 // arithmetic is the serial reference update in the same order.
@@ -777,7 +800,7 @@ __global__ void __launch_bounds__(SMALL_BLOCK) k_psor_small(BodyPtrs B, MfPtrs M
                                                             const int* __restrict__ cstart,
                                                             const unsigned* __restrict__ ctr, int nm, SolveConst K,
                                                             int warm, double tol, int max_sweeps, int check_every,
-                                                            int stop_rule, SmallOut* __restrict__ out) {
+                                                            int stop_rule, int stall, SmallOut* __restrict__ out) {
   __shared__ double red[SMALL_BLOCK / 64];
   __shared__ int cs[129];
   const int ncolor = (int)ctr[3];
@@ -792,6 +815,8 @@ __global__ void __launch_bounds__(SMALL_BLOCK) k_psor_small(BodyPtrs B, MfPtrs M
   int sweeps = 0;
   double res = 0.0, res_ref = 0.0;
   const int k = check_every > 0 ? check_every : 1;
+  StallTrack trk;
+  trk.init(stall);
   if (stop_rule == 1) {
     double r = 0.0;
     for (int i = threadIdx.x; i < nm; i += blockDim.x) r = dmax(r, residual_one(B, M, i, K, sys[B.sys[M.a[i]]].mu));
@@ -814,6 +839,7 @@ __global__ void __launch_bounds__(SMALL_BLOCK) k_psor_small(BodyPtrs B, MfPtrs M
     for (int i = threadIdx.x; i < nm; i += blockDim.x) r = dmax(r, residual_one(B, M, i, K, sys[B.sys[M.a[i]]].mu));
     res = block_max(r, red);
     if (stop_rule == 0 && res_ref < 1e-6) break;
+    if (stop_rule == 1 && !(res <= tol) && trk.stalled(sweeps, res)) break;
   }
   if (threadIdx.x == 0) {
     out->sweeps = sweeps;
@@ -954,7 +980,7 @@ __device__ __forceinline__ void island_report(IslandStats* S, int sweeps, int si
 struct IslandSolve {
   SolveConst K;
   double tol;
-  int max_sweeps, check_every, stop_rule, warm;
+  int max_sweeps, check_every, stop_rule, warm, stall;
 };
 
 constexpr int LARGE_THREADS = 256;
@@ -1077,6 +1103,8 @@ __global__ void __launch_bounds__(WG) k_island_reg(BodyPtrs B, MfPtrs M, const S
   int sweeps = 0;
   double res = 0.0, res_ref = 0.0;
   bool done = !has || P.max_sweeps <= 0;
+  StallTrack trk;
+  trk.init(P.stall);
   if (P.stop_rule == 1) {
     res = group_max<GROUP>(live ? residual_one(LB, R, 0, P.K, mu) : 0.0, red);
     done = done || res <= P.tol;
@@ -1100,6 +1128,7 @@ __global__ void __launch_bounds__(WG) k_island_reg(BodyPtrs B, MfPtrs M, const S
       if (P.stop_rule == 0 && res_ref < 1e-6) done = true;
       if (P.stop_rule == 1 && res <= P.tol) done = true;
       if (sweeps >= P.max_sweeps) done = true;
+      if (!done && P.stop_rule == 1 && trk.stalled(sweeps, res)) done = true;
     }
   }
   if (live)
@@ -1143,6 +1172,8 @@ __global__ void __launch_bounds__(LARGE_THREADS) k_island_large(BodyPtrs B, MfPt
   }
   int sweeps = 0;
   double res = 0.0, res_ref = 0.0;
+  StallTrack trk;
+  trk.init(P.stall);
   if (P.stop_rule == 1) {
     double r = 0.0;
     for (int i = s + tid; i < e; i += LARGE_THREADS) r = dmax(r, residual_one(B, M, i, P.K, mu));
@@ -1164,6 +1195,7 @@ __global__ void __launch_bounds__(LARGE_THREADS) k_island_large(BodyPtrs B, MfPt
     for (int i = s + tid; i < e; i += LARGE_THREADS) r = dmax(r, residual_one(B, M, i, P.K, mu));
     res = block_max(r, red);
     if (P.stop_rule == 0 && res_ref < 1e-6) break;
+    if (P.stop_rule == 1 && !(res <= P.tol) && trk.stalled(sweeps, res)) break;
   }
   if (tid == 0) island_report(S, sweeps, e - s, res, res_ref, P.stop_rule == 1 && !(res <= P.tol));
 }
@@ -1619,6 +1651,7 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
   P.check_every = std::max(1, p->check_every);
   P.stop_rule = p->stop_rule;
   P.warm = p->warm_mode != 0 ? 1 : 0;
+  P.stall = p->stall_sweeps;
   IslandStats* S = (IslandStats*)ctx->istats.p;
   RDK_HIP(hipMemsetAsync(S, 0, sizeof(IslandStats), ctx->stream));
   BodyPtrs B = body_ptrs(ctx);
@@ -1732,7 +1765,7 @@ static int global_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
       hipLaunchKernelGGL(k_psor_small, dim3(1), dim3(SMALL_BLOCK), 0, ctx->stream, B, Mc, ctx->sys_dev.p,
                          ctx->d_cstart.p, (const unsigned*)ctx->uncolored.p, nm, solve_const(ctx, p),
                          p->warm_mode != 0 ? 1 : 0, p->tol, p->max_sweeps, std::max(1, p->check_every), p->stop_rule,
-                         (SmallOut*)ctx->small_out.p);
+                         p->stall_sweeps, (SmallOut*)ctx->small_out.p);
     }
     SmallOut so;
     unsigned hc[4];
@@ -1778,11 +1811,14 @@ static int global_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
       }
     } else {
       const int k = std::max(1, p->check_every);
+      StallTrack trk;
+      trk.init(p->stall_sweeps);
       RDK_TRY(nsc_residual_run(ctx, p, &res, &res_ref));
       while (!(res <= p->tol) && sweeps < p->max_sweeps) {
         RDK_TRY(run_sweeps(ctx, p, k));
         sweeps += k;
         RDK_TRY(nsc_residual_run(ctx, p, &res, &res_ref));
+        if (!(res <= p->tol) && trk.stalled(sweeps, res)) break;
       }
       if (!(res <= p->tol)) status = RDK_ERR_NOCONV;
     }

@@ -39,7 +39,7 @@ class NscParams(C.Structure):
     _fields_ = [("beta", C.c_double), ("cfm", C.c_double), ("omega", C.c_double), ("tol", C.c_double),
                 ("max_sweeps", C.c_int), ("check_every", C.c_int), ("friction_proj", C.c_int),
                 ("position_stabilization", C.c_int), ("warm_mode", C.c_int), ("stop_rule", C.c_int),
-                ("warm_walls", C.c_int)]
+                ("warm_walls", C.c_int), ("stall_sweeps", C.c_int)]
 
 
 class SoftParams(C.Structure):
@@ -180,9 +180,10 @@ def system_desc(lo, hi, periodic, floor_wall=False, ceil_wall=False, mu=0.0):
 
 
 def nsc_params(beta=0.0, cfm=0.0, omega=1.0, tol=1e-8, max_sweeps=10000, check_every=1, friction_proj=0,
-               position_stabilization=0, warm_mode=1, stop_rule=1, warm_walls=0):
+               position_stabilization=0, warm_mode=1, stop_rule=1, warm_walls=0, stall_sweeps=0):
     p = NscParams()
     p.warm_walls = warm_walls
+    p.stall_sweeps = stall_sweeps
     p.beta, p.cfm, p.omega, p.tol = beta, cfm, omega, tol
     p.max_sweeps, p.check_every, p.friction_proj = max_sweeps, check_every, friction_proj
     p.position_stabilization, p.warm_mode, p.stop_rule = position_stabilization, warm_mode, stop_rule

@@ -123,7 +123,7 @@ def test_features_match_oracle(gpu_ctx_factory):
     assert np.allclose(Xo[:, 11] * lso, np.maximum(lam_start[:, 0], 0.0), rtol=1e-12, atol=1e-300)
 
 
-def test_learned_start_is_previous_plus_prediction(gpu_ctx_factory):
+def test_learned_start_keeps_cached_impulses_and_predicts_the_rest(gpu_ctx_factory):
     B = overlapping_bed(n=2000, seed=62, mu=0.4)
     ctx = gpu_ctx_factory()
     ctx.mlp_set_params(_rand_params(5, 0.1))
@@ -140,7 +140,10 @@ def test_learned_start_is_previous_plus_prediction(gpu_ctx_factory):
     _, _, lam_learned = ctx.nsc_manifolds(nm)
     expect = M.apply_prediction(Y, ls, 0.4, lam_prev)
     assert np.allclose(lam_learned, expect, rtol=1e-12, atol=1e-15 * np.abs(expect).max())
-    assert not np.allclose(lam_learned, lam_prev)
+    kept = lam_prev[:, 0] > 0
+    assert kept.sum() > 100 and (~kept).sum() > 10
+    assert np.array_equal(lam_learned[kept], lam_prev[kept])
+    assert not np.allclose(lam_learned[~kept], lam_prev[~kept])
 
 
 def test_solve_from_learned_start_reaches_residual(gpu_ctx_factory):

@@ -238,3 +238,29 @@ def test_island_and_global_solves_agree(gpu_ctx_factory):
         assert st["residual"] <= 1e-12
         out.append(ctx.nsc_manifolds(nm)[2])
     assert np.allclose(out[0], out[1], rtol=1e-7, atol=1e-10)
+
+
+def test_stall_rule(gpu_ctx_factory):
+    """tol = 0 never converges: the residual reaches the rounding floor, stops
+    halving between milestones and the solve stops long before max_sweeps.  A
+    solve that converges is not affected by the rule (bitwise the same)."""
+    B = clusters(seed=5, mu=0.0)
+    ctx = gpu_ctx_factory()
+    gpu_load(ctx, B)
+    p = L.nsc_params(beta=0.0, cfm=1e-3, omega=1.0, tol=0.0, max_sweeps=400000, check_every=4, friction_proj=1,
+                     warm_mode=0, stall_sweeps=64)
+    ctx.detect(L.detect_params())
+    nm = ctx.nsc_build(p)
+    st = ctx.nsc_solve(p, allow_noconv=True)
+    assert 64 <= st["sweeps"] < 400000 and st["residual"] < 1e-13, st
+    lams = []
+    for stall in (0, 64):
+        gpu_load(ctx, B)
+        q = L.nsc_params(beta=0.0, cfm=1e-3, omega=1.0, tol=1e-11, max_sweeps=400000, check_every=4,
+                         friction_proj=1, warm_mode=0, stall_sweeps=stall)
+        ctx.detect(L.detect_params())
+        ctx.nsc_build(q)
+        st = ctx.nsc_solve(q)
+        assert st["residual"] <= 1e-11
+        lams.append(ctx.nsc_manifolds(nm)[2])
+    assert np.array_equal(lams[0], lams[1])

@@ -145,12 +145,13 @@ def test_apply_prediction_cone():
     assert np.allclose(L[2], [2.0, 0.2, 0.0])   # inside the disc, kept
 
 
-def test_apply_prediction_adds_previous_impulses():
-    Y = np.array([[0.5, 0.0, 0.0], [-3.0, 0.0, 0.0]])
-    prev = np.array([[1.0, 0.2, -0.1], [1.0, 0.2, 0.0]])
-    L = M.apply_prediction(Y, np.array([2.0, 1.0]), 0.5, prev)
-    assert np.allclose(L[0], [2.0, 0.2, -0.1])
-    assert np.allclose(L[1], [0.0, 0.0, 0.0])   # normal pushed below zero -> no friction either
+def test_apply_prediction_keeps_previous_impulses():
+    Y = np.array([[0.5, 0.0, 0.0], [3.0, 1.0, 0.0], [-3.0, 0.0, 0.0]])
+    prev = np.array([[1.0, 0.2, -0.1], [0.0, 0.0, 0.0], [0.0, 0.0, 0.0]])
+    L = M.apply_prediction(Y, np.array([2.0, 1.0, 1.0]), 0.5, prev)
+    assert np.array_equal(L[0], prev[0])           # a previous impulse is kept as is
+    assert np.allclose(L[1], [3.0, 1.0, 0.0])      # new contact: prediction, inside the disc (|t| = 1 <= 1.5)
+    assert np.allclose(L[2], [0.0, 0.0, 0.0])      # predicted separation
 
 
 def test_adamw_first_step_closed_form():
