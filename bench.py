@@ -1,16 +1,23 @@
 #!/usr/bin/env python3
-"""Benchmark: hard-contact PSOR settling (N=1e5 capsules, L/d=20) on one GPU.
+"""Benchmark of the rod-dynamics step on one GPU.  Prints one JSON line.
 
-A "step" is one full hard-contact time step of the packing: free flight,
-cell-list broad phase + FP64 narrow phase, manifold build (rod-rod and
-floor/ceiling walls), contact-graph colouring, coloured PSOR to the
-fixed-point residual 1e-8, position/quaternion update.  The workload is the
-settling protocol of the reference's hard-contact path: 1e5 rods, d=1, L=20,
+Default workload (--workload c2): hard-contact PSOR settling, the protocol the
+headline metric is quoted on.  A "step" is one full hard-contact time step:
+free flight, cell-list broad phase + FP64 narrow phase, manifold build (rod-rod
+and floor/ceiling walls), contact islands and colouring, PSOR to the
+fixed-point residual 1e-8, position/quaternion update.  1e5 rods, d=1, L=20,
 random non-overlapping start at phi=0.05 in a 319^3 box periodic in x,y with
-floor and ceiling walls, gravity -z, mu=0.5, e=0, omega=1.3, dt=1e-3,
-5000 steps (warmup + timed).  State lives in HBM for the whole run.
+floor and ceiling walls, gravity -z, mu=0.5, e=0, omega=1.3, dt=1e-3, 5000
+steps (20 warm-up + 4980 timed).  State lives in HBM for the whole run.  After
+the timed protocol the settled bed is solved from the cold, previous-step and
+learned starts (same manifolds and velocities) to report sweeps/step.
 
-Prints one JSON line (rank 0).
+--workload c1: Hertz-Mindlin thermal gas, 1000 rods L/d=10, phi=0.10,
+periodic, dt=1e-4, 10 000 steps (velocity Verlet, two force evaluations per
+step as the reference integrator does).
+--workload c3: dense polydisperse Hertz-Mindlin, 1e6 rods (L lognormal,
+median 16), generated at phi=0.05, compressed to phi=0.30 (untimed
+preparation), then 2000 timed steps at dt=5e-5 with an affine strain rate 1e-3.
 """
 import argparse
 import json
@@ -31,16 +38,25 @@ BYTES = {
     "nsc_free": ("bodies", 280),
     "nsc_pos": ("bodies", 280),
     "contact_record": ("contacts", 112 + 144),
+    "force": ("force_contacts", 320),    # record 112 + x,q,v,w of two bodies 208
+    "body_reduce": ("force_contacts", 192),
+    "history": ("force_contacts", 64),
+    "vv_half1": ("bodies", 280),
+    "vv_half2": ("bodies", 280),
 }
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# (scaled by a calibrated copy), for the C2 workload's dominant kernel
+TRAFFIC = {}
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4980)
+    ap.add_argument("--workload", choices=["c2", "c1", "c3"], default="c2")
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--n", type=int, default=100_000)
-    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--n", type=int, default=None)
+    ap.add_argument("--cpu-steps", type=int, default=None, help="CPU baseline sample size (steps)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chunk-len", type=float, default=4.0)
     ap.add_argument("--check-every", type=int, default=4)
@@ -50,7 +66,19 @@ def parse():
     ap.add_argument("--weights", default=os.path.join(ROOT, "rod-dynamics-3d_amd", "weights", "warmstart_c2.npy"))
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (HBM counter calibration)")
-    return ap.parse_args()
+    a = ap.parse_args()
+    defaults = {"c2": (4980, 100_000, 5), "c1": (9980, 1000, 2000), "c3": (2000, 1_000_000, 2)}[a.workload]
+    if a.steps is None:
+        a.steps = defaults[0]
+    if a.n is None:
+        a.n = defaults[1]
+    if a.cpu_steps is None:
+        a.cpu_steps = defaults[2]
+    return a
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def copy_calibration(mib):
@@ -67,23 +95,138 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes_per_launch": 4 * n, "launches": 5}), flush=True)
 
 
-def cpu_baseline(sc, nsteps):
+def oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py as O
 
-    W = O.world_from_scene(sc)
+    return O
+
+
+def cores():
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+def rank_kernels(ctx, run_warmup):
+    """Warm-up with every kernel timed; returns the dominant kernel among those
+    with a byte model and leaves only that kernel's timer on."""
+    ctx.set_profiling(True)
+    run_warmup()
+    kt = ctx.kernel_times()
+    ranked = sorted(((ms, name) for name, (n, ms) in kt.items() if name in BYTES), reverse=True)
+    dom = ranked[0][1] if ranked else "broad_narrow"
+    names = [ctx._L.rdk_kernel_name(k).decode() for k in range(ctx._L.rdk_num_kernels())]
+    ctx.set_profiling(True)  # resets counters
+    ctx.set_profile_mask(1 << names.index(dom))
+    ctx.sync()
+    return dom
+
+
+def roofline(ctx, dom, n):
+    kt = ctx.kernel_times()
+    wc = ctx.work_counters()
+    launches, kms = kt[dom]
+    unit, bpu = BYTES[dom]
+    units = {"bodies": wc["body_passes"] * n if dom == "broad_narrow" else launches * n,
+             "sweep_contacts": wc["sweep_contacts"], "resid_contacts": wc["resid_contacts"],
+             "manifolds": wc["manifolds"], "contacts": wc["contacts"], "force_contacts": wc["force_contacts"]}[unit]
+    achieved = (bpu * units / 1e9) / (kms / 1e3) if kms > 0 else 0.0
+    return {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC.get(dom), "launches": launches,
+            "avg_launch_ms": kms / max(launches, 1), "bytes_per_unit": bpu, "unit_name": unit, "units": units}
+
+
+def base_line(metric, value, a, el, dtype, config):
+    return {"metric": metric, "value": value, "unit": "rod-steps/s", "n_gpus": 1, "steps": a.steps,
+            "warmup": a.warmup, "ms_per_step": 1e3 * el / a.steps, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config}
+
+
+# ---------------------------------------------------------------------------- C2
+def cpu_baseline_c2(sc, snaps, total_steps, O):
+    """The FP64 oracle (reference spatial hash + serial sorted-order PSOR to
+    1e-8, OpenMP elsewhere) on GPU-state snapshots spread over the protocol:
+    at each snapshot one unmeasured step (fills the warm cache), then one timed
+    step; the per-phase step times are extrapolated to the whole protocol."""
     p = O.nsc_params(mu=sc.extra["mu"], beta=0.0, cfm=0.0, omega=1.3, tol=1e-8, max_sweeps=10000, check_every=1,
                      friction_proj=sc.params.friction_proj, stop_rule=1, warm_start=1,
                      warm_walls=sc.params.warm_walls)
-    W.nsc_step(p, 1, method=1)  # warm-up step (allocations)
+    W = O.world_from_scene(sc)
+    per = []
+    t_all = 0.0
+    for step, st in snaps:
+        W.set_state(x=st["x"], q=st["q"], v=st["v"], w=st["w"])
+        W.nsc_step(p, 1, method=1)
+        t0 = time.perf_counter()
+        sweeps, _ = W.nsc_step(p, 1, method=1)
+        dt = time.perf_counter() - t0
+        t_all += dt
+        per.append({"step": step + 2, "seconds": dt, "sweeps": sweeps})
+        log(f"cpu baseline: step {step + 2}: {dt:.2f}s, {sweeps} sweeps")
+    est = sum(r["seconds"] for r in per) * total_steps / len(per)
+    return {"value": sc.n * total_steps / est, "unit": "rod-steps/s", "cores": cores(), "kind": "port",
+            "sample": f"FP64 oracle (reference spatial hash + serial sorted-order PSOR to residual 1e-8, OpenMP "
+                      f"{cores()} threads) timed for one step at each of {len(per)} GPU-state snapshots spread over "
+                      f"the same run (one unmeasured step before each); per-step times extrapolated to the "
+                      f"{total_steps}-step protocol",
+            "seconds_measured": t_all, "per_snapshot": per}
+
+
+def run_c2(a, L, scenes):
     t0 = time.perf_counter()
-    sweeps, rmax = W.nsc_step(p, nsteps, method=1)
-    dt = time.perf_counter() - t0
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": sc.n * nsteps / dt, "unit": "rod-steps/s", "cores": cores, "kind": "port",
-            "sample": f"steps 2..{nsteps + 1} of the same N={sc.n} settling run (same seed), FP64 oracle: reference "
-                      f"spatial hash + serial sorted-order PSOR to residual 1e-8, OpenMP {cores} threads",
-            "seconds": dt, "sweeps_per_step": sweeps / max(nsteps, 1)}
+    sc = scenes.c2_settling(n=a.n)
+    log(f"scene {sc.name} built in {time.perf_counter() - t0:.1f}s (box {sc.extra['side']:.2f})")
+    p = sc.params
+    p.check_every = a.check_every
+    det = L.detect_params(0.0, a.chunk_len, 1.0)
+    ctx = L.Context(0)
+    sc.load(ctx)
+    dom = rank_kernels(ctx, lambda: ctx.step_nsc(det, p, a.warmup))
+    log(f"warmup {a.warmup} steps done; dominant kernel {dom}; timing {a.steps} steps")
+    total = a.warmup + a.steps
+    snap_at = [int(total * (k + 0.5) / 5) for k in range(5)] if not a.no_cpu else []
+    snaps = []
+    t_snap = 0.0
+    st = None
+    done = 0
+    t1 = time.perf_counter()
+    while done < a.steps:  # blocks of steps, one progress line each (stderr)
+        nxt = [s - a.warmup for s in snap_at if s - a.warmup > done]
+        k = min(a.block, a.steps - done, (nxt[0] - done) if nxt else a.steps)
+        s_ = ctx.step_nsc(det, p, k)
+        done += k
+        if st is None:
+            st = dict(s_)
+        else:
+            for key in st:
+                st[key] = max(st[key], s_[key]) if key == "residual_max" else st[key] + s_[key]
+        if nxt and done == nxt[0]:
+            ts = time.perf_counter()
+            snaps.append((a.warmup + done, ctx.download_state()))  # excluded from the timed region below
+            t_snap += time.perf_counter() - ts
+        log(f"step {a.warmup + done}: {time.perf_counter() - t1 - t_snap:.2f}s, manifolds/step "
+            f"{s_['manifolds_total'] / k:.0f}, sweeps/step {s_['sweeps_total'] / k:.1f}")
+    ctx.sync()
+    el = time.perf_counter() - t1 - t_snap
+    out = base_line("rod-steps/s at PSOR residual 1e-8", sc.n * a.steps / el, a, el, "f64", {
+        "workload": "hard-contact PSOR settling (C2)", "rods": sc.n, "L_over_d": 20, "phi0": 0.05,
+        "box": round(sc.extra["side"], 3), "periodic": "x,y", "walls": "floor+ceiling", "mu": 0.5,
+        "restitution": 0.0, "omega": 1.3, "residual_tol": 1e-8, "dt": 1e-3,
+        "friction_projection": "euclidean disc" if p.friction_proj == 1 else "reference sequential",
+        "steps_total": total, "seed": sc.extra["seed"], "check_every": a.check_every, "chunk_len": a.chunk_len,
+        "stall_sweeps": p.stall_sweeps})
+    out["psor"] = {"sweeps_per_step": st["sweeps_total"] / max(st["steps"], 1),
+                   "manifolds_per_step": st["manifolds_total"] / max(st["steps"], 1),
+                   "rod_contacts_per_step": st["contacts_total"] / max(st["steps"], 1),
+                   "colors_per_step": st["colors_total"] / max(st["steps"], 1),
+                   "residual_max": st["residual_max"], "nonconverged_steps": st["nonconverged_steps"],
+                   "warm_start": "previous step (rod-rod and plane-wall contacts)"}
+    out["roofline"] = roofline(ctx, dom, sc.n)
+    if a.ab_steps > 0:
+        out["learned_warm_start"] = learned_ab(ctx, sc, a)
+    ctx.close()
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_c2(sc, snaps, total, oracle())
+    return out
 
 
 def learned_ab(ctx, sc, a):
@@ -101,11 +244,96 @@ def learned_ab(ctx, sc, a):
     for k in range(a.ab_steps):
         rows.append(WS.ab_step(ctx, sc.detect, sc.params))
         if (k + 1) % 5 == 0:
-            print(f"[bench] learned-start A/B step {k + 1}/{a.ab_steps}", file=sys.stderr, flush=True)
+            log(f"learned-start A/B step {k + 1}/{a.ab_steps}")
     res = WS.ab_summary(rows)
     res["weights"] = os.path.relpath(a.weights, ROOT)
     res["bed"] = f"steps {a.warmup + a.steps + 1}..{a.warmup + a.steps + a.ab_steps} of the same run"
     return res
+
+
+# ---------------------------------------------------------------------- C1 / C3
+def cpu_baseline_vv(sc, state, nsteps, O, note):
+    W = O.world_from_scene(sc)
+    W.set_state(x=state["x"], q=state["q"], v=state["v"], w=state["w"])
+    hp = sc.params
+    p = O.hm_params(E=hp.E, nu_poisson=hp.nu_poisson, restitution=hp.restitution, mu_k=hp.mu_k, mu_s=hp.mu_s,
+                    v_c=hp.v_c, mu_roll=hp.mu_roll, enable_tangential=hp.enable_tangential,
+                    enable_rolling=hp.enable_rolling, law=hp.law)
+    W.vv_step(1, p, 1, method=1)  # history and allocations
+    t0 = time.perf_counter()
+    W.vv_step(1, p, nsteps, method=1)
+    dt = time.perf_counter() - t0
+    return {"value": sc.n * nsteps / dt, "unit": "rod-steps/s", "cores": cores(), "kind": "port",
+            "sample": f"{nsteps} velocity-Verlet Hertz-Mindlin steps of the FP64 oracle (reference spatial hash, "
+                      f"two force evaluations per step, OpenMP {cores()} threads) from {note}",
+            "seconds": dt}
+
+
+def run_vv(a, L, scenes, which):
+    from rod_dynamics_py import protocols
+
+    t0 = time.perf_counter()
+    if which == "c1":
+        sc = scenes.c1_thermal(n=a.n, model="hm")
+    else:
+        sc = scenes.c3_dense(n=a.n)
+    sc.detect = L.detect_params(0.0, a.chunk_len if which == "c3" else 0.0, 1.0)
+    log(f"scene {sc.name} built in {time.perf_counter() - t0:.1f}s (box {sc.extra['side']:.2f})")
+    ctx = L.Context(0)
+    sc.load(ctx)
+    prep = {}
+    if which == "c3":
+        t0 = time.perf_counter()
+        nsteps = protocols.compress_nsc(ctx, sc, sc.extra["side1"], log=sys.stderr)
+        st0 = ctx.download_state()
+        v, w = L.gen_thermal(st0["q"], sc.r, sc.h, sc.mass, 1.0, sc.extra["seed"] + 29)
+        ctx.upload_state(v=v, w=w)
+        prep = {"compression": f"affine increments of 0.998 with two overdamped hard-contact steps each "
+                               f"({nsteps} steps, beta 0.8, dt 1e-2) from phi 0.05 to phi "
+                               f"{protocols.packing_fraction_from_side(sc, sc.extra['side_now']):.4f}, then thermal "
+                               f"velocities T=1", "prep_seconds": time.perf_counter() - t0}
+        log(f"compressed to side {sc.extra['side_now']:.3f} in {prep['prep_seconds']:.1f}s")
+    rate = 1e-3 if which == "c3" else 0.0
+    dom = rank_kernels(ctx, lambda: protocols.strained_vv(ctx, sc, a.warmup, rate))
+    log(f"warmup {a.warmup} steps done; dominant kernel {dom}; timing {a.steps} steps")
+    done = 0
+    st = None
+    t1 = time.perf_counter()
+    while done < a.steps:
+        k = min(a.block, a.steps - done)
+        s_ = protocols.strained_vv(ctx, sc, k, rate) if rate != 0.0 else ctx.step_vv(sc.detect, 1, sc.params, k)
+        done += k
+        if st is None:
+            st = dict(s_)
+        else:
+            for key in st:
+                st[key] = max(st[key], s_[key]) if key == "residual_max" else st[key] + s_[key]
+        log(f"step {a.warmup + done}: {time.perf_counter() - t1:.2f}s, contacts/evaluation "
+            f"{s_['contacts_total'] / (2 * k):.0f}")
+    ctx.sync()
+    el = time.perf_counter() - t1
+    if which == "c1":
+        cfg = {"workload": "Hertz-Mindlin thermal gas (C1)", "rods": sc.n, "L_over_d": 10, "phi": 0.10,
+               "box": round(sc.extra["side"], 3), "periodic": "x,y,z", "mu": 0.5, "E": 1e7, "nu": 0.25,
+               "restitution": 0.5, "dt": 1e-4, "steps_total": a.warmup + a.steps, "seed": sc.extra["seed"],
+               "force_evaluations_per_step": 2}
+    else:
+        cfg = {"workload": "dense polydisperse Hertz-Mindlin (C3)", "rods": sc.n,
+               "L": "lognormal median 16 sigma_log 0.3, truncated [4, 64]", "mean_L": round(sc.extra["mean_length"], 3),
+               "phi": 0.30, "box": round(sc.extra["side_now"], 3), "periodic": "x,y,z", "mu": 0.3, "E": 1e7,
+               "nu": 0.25, "restitution": 0.5, "dt": 5e-5, "strain_rate": 1e-3, "steps_timed": a.steps,
+               "seed": sc.extra["seed"], "force_evaluations_per_step": 2, "chunk_len": a.chunk_len}
+        cfg.update(prep)
+    out = base_line("rod-steps/s", sc.n * a.steps / el, a, el, "f64", cfg)
+    out["contacts_per_evaluation"] = st["contacts_total"] / (2 * max(st["steps"], 1))
+    out["roofline"] = roofline(ctx, dom, sc.n)
+    state = ctx.download_state()
+    ctx.close()
+    if not a.no_cpu:
+        log(f"cpu baseline: {a.cpu_steps} oracle steps")
+        out["cpu_baseline"] = cpu_baseline_vv(sc, state, a.cpu_steps, oracle(),
+                                              "the GPU state after the timed steps of the same run")
+    return out
 
 
 def main():
@@ -119,88 +347,10 @@ def main():
     from rod_dynamics_py import _lib as L
     from rod_dynamics_py import scenes
 
-    t0 = time.perf_counter()
-    sc = scenes.c2_settling(n=a.n)
-    print(f"[bench] scene {sc.name} built in {time.perf_counter() - t0:.1f}s (box {sc.extra['side']:.2f})",
-          file=sys.stderr, flush=True)
-    p = sc.params
-    p.check_every = a.check_every
-    det = L.detect_params(0.0, a.chunk_len, 1.0)
-    ctx = L.Context(0)
-    sc.load(ctx)
-    # warmup with every kernel timed, to rank the kernels
-    ctx.set_profiling(True)
-    ctx.step_nsc(det, p, a.warmup)
-    kt = ctx.kernel_times()
-    ranked = sorted(((ms, name) for name, (n, ms) in kt.items() if name in BYTES), reverse=True)
-    dom = ranked[0][1] if ranked else "broad_narrow"
-    names = [ctx._L.rdk_kernel_name(k).decode() for k in range(ctx._L.rdk_num_kernels())]
-    ctx.set_profiling(True)  # resets counters
-    ctx.set_profile_mask(1 << names.index(dom))
-    ctx.sync()
-    print(f"[bench] warmup {a.warmup} steps done; dominant kernel {dom}; timing {a.steps} steps", file=sys.stderr,
-          flush=True)
-    t1 = time.perf_counter()
-    st = None
-    done = 0
-    while done < a.steps:  # blocks of steps, one progress line each (stderr)
-        k = min(a.block, a.steps - done)
-        s_ = ctx.step_nsc(det, p, k)
-        done += k
-        if st is None:
-            st = dict(s_)
-        else:
-            for key in st:
-                st[key] = max(st[key], s_[key]) if key == "residual_max" else st[key] + s_[key]
-        print(f"[bench] step {a.warmup + done}: {time.perf_counter() - t1:.2f}s, manifolds/step "
-              f"{s_['manifolds_total'] / k:.0f}, sweeps/step {s_['sweeps_total'] / k:.1f}", file=sys.stderr, flush=True)
-    ctx.sync()
-    el = time.perf_counter() - t1
-    kt = ctx.kernel_times()
-    wc = ctx.work_counters()
-    launches, kms = kt[dom]
-    unit, bpu = BYTES[dom]
-    units = {"bodies": wc["body_passes"] * sc.n if dom == "broad_narrow" else launches * sc.n,
-             "sweep_contacts": wc["sweep_contacts"], "resid_contacts": wc["resid_contacts"],
-             "manifolds": wc["manifolds"], "contacts": wc["contacts"]}[unit]
-    achieved = (bpu * units / 1e9) / (kms / 1e3) if kms > 0 else 0.0
-    value = sc.n * a.steps / el
-    out = {
-        "metric": "rod-steps/s at PSOR residual 1e-8",
-        "value": value,
-        "unit": "rod-steps/s",
-        "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": 1e3 * el / a.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": "hard-contact PSOR settling (C2)", "rods": sc.n, "L_over_d": 20, "phi0": 0.05,
-                   "box": round(sc.extra["side"], 3), "periodic": "x,y", "walls": "floor+ceiling", "mu": 0.5,
-                   "restitution": 0.0, "omega": 1.3, "residual_tol": 1e-8, "dt": 1e-3,
-                   "friction_projection": "euclidean disc" if p.friction_proj == 1 else "reference sequential",
-                   "steps_total": a.warmup + a.steps, "seed": sc.extra["seed"], "check_every": a.check_every,
-                   "chunk_len": a.chunk_len},
-        "psor": {"sweeps_per_step": st["sweeps_total"] / max(st["steps"], 1),
-                 "manifolds_per_step": st["manifolds_total"] / max(st["steps"], 1),
-                 "rod_contacts_per_step": st["contacts_total"] / max(st["steps"], 1),
-                 "colors_per_step": st["colors_total"] / max(st["steps"], 1),
-                 "residual_max": st["residual_max"], "nonconverged_steps": st["nonconverged_steps"],
-                 "warm_start": "previous step (rod-rod and plane-wall contacts)"},
-        "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "launches": launches, "avg_launch_ms": kms / max(launches, 1),
-                     "bytes_per_unit": bpu, "unit_name": unit, "units": units},
-    }
-    if a.ab_steps > 0:
-        out["learned_warm_start"] = learned_ab(ctx, sc, a)
-    ctx.close()
-    if not a.no_cpu:
-        print(f"[bench] cpu baseline: {a.cpu_steps} oracle steps", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(sc, a.cpu_steps)
+    if a.workload == "c2":
+        out = run_c2(a, L, scenes)
+    else:
+        out = run_vv(a, L, scenes, a.workload)
     print(json.dumps(out), flush=True)
 
 

@@ -72,7 +72,7 @@ typedef struct rdk_nsc_params {
                               reference starts wall contacts cold */
   int stall_sweeps;        /* stop_rule 1 only; 0 = off.  > 0: at sweeps = k, 2k,
                               4k, ... an island (or the global solve) whose residual
-                              has not at least halved since the previous milestone
+                              has not dropped by 10 % since the previous milestone
                               stops there and counts as not converged */
 } rdk_nsc_params;
 
@@ -225,6 +225,14 @@ int rdk_gen_rsa(int n, const double* r, const double* h,
 int rdk_gen_thermal(int n, const double* q, const double* r, const double* h,
                     const double* mass, double kT, uint64_t seed, double* v,
                     double* w);
+
+/* ---- affine strain ----------------------------------------------------
+ * No counterpart in the reference (it has no strain or compression); the
+ * dense Hertz-Mindlin workload applies x <- lo + s (x - lo) and scales every
+ * box by s (isotropic, about the lower corner) once per step, with
+ * s = 1 - strain_rate * dt.  Velocities are not changed.  Cell grids are
+ * resized for the new boxes. */
+int rdk_affine_scale(rdk_ctx* ctx, double s);
 
 /* ---- learned PSOR warm start (warm_mode 2) ----------------------------
  * No counterpart in the reference: its warm start is the previous-step

@@ -639,6 +639,17 @@ int rdk_step_vv(rdk_ctx* ctx, const rdk_detect_params* dp, int model, const void
   return RDK_OK;
 }
 
+int rdk_affine_scale(rdk_ctx* ctx, double s) {
+  CTX_CHECK();
+  RDK_TRY(check_ready(ctx));
+  if (!(s > 0.0) || !std::isfinite(s)) {
+    ctx->err = "rdk_affine_scale: the factor must be positive and finite";
+    return RDK_ERR_ARG;
+  }
+  RDK_TRY(affine_run(ctx, s));
+  return prof_flush(ctx);
+}
+
 int rdk_observables(rdk_ctx* ctx, double* ke, double* phi) {
   CTX_CHECK();
   RDK_TRY(check_ready(ctx));

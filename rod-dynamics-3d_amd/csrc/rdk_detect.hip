@@ -295,34 +295,14 @@ __global__ void k_contact_record(BodyPtrs B, const SysDev* __restrict__ sys, con
 // ---------------------------------------------------------------------------
 // host orchestration
 
-static int build_grid(rdk_ctx* ctx, const rdk_detect_params* p) {
-  const double cl = p->chunk_len;
-  const double scale = p->cell_scale >= 1.0 ? p->cell_scale : 1.0;
-  if (cl == ctx->built_chunk_len && scale == ctx->built_cell_scale && p->delta == ctx->built_delta &&
-      ctx->n_chunks > 0)
-    return RDK_OK;
-  const int n = ctx->n;
-  std::vector<int> nchunk(n), crod, ck;
-  crod.reserve(n);
-  ck.reserve(n);
-  std::vector<double> radmax(ctx->nsys, 0.0);
-  for (int i = 0; i < n; ++i) {
-    int K = 1;
-    if (cl > 0.0) K = std::max(1, (int)std::ceil(2.0 * ctx->h_h[i] / cl));
-    nchunk[i] = K;
-    for (int k = 0; k < K; ++k) {
-      crod.push_back(i);
-      ck.push_back(k);
-    }
-    double rad = ctx->h_h[i] / K + ctx->h_r[i];
-    int s = ctx->h_sys_of[i];
-    radmax[s] = std::max(radmax[s], rad);
-  }
-  const int M = (int)crod.size();
+// Per-system cell grids for the current boxes: cell >= 2 * max chunk radius
+// + delta (times cell_scale), periodic axes with fewer than 3 cells use one.
+static int size_grids(rdk_ctx* ctx, double delta, double scale) {
   int cell_off = 0;
   for (int s = 0; s < ctx->nsys; ++s) {
     SysDev& S = ctx->sys_host[s];
-    const double cmin = (2.0 * radmax[s] + p->delta) * scale * (1.0 + 1e-9) + 1e-9;
+    const double radmax = ctx->sys_radmax[s];
+    const double cmin = (2.0 * radmax + delta) * scale * (1.0 + 1e-9) + 1e-9;
     int64_t total = 1;
     for (int k = 0; k < 3; ++k) {
       int d = (int)std::floor(S.span[k] / cmin);
@@ -350,30 +330,66 @@ static int build_grid(rdk_ctx* ctx, const rdk_detect_params* p) {
     }
     // FP32 prefilter margin: rounding of centres, radii, the image shift and
     // the squared distance, bounded by a few ulps of the largest coordinate
-    S.fmargin = (float)(64.0 * amax * 1.1920928955078125e-07 + 1e-6 * (2.0 * radmax[s] + p->delta) + 1e-6);
+    S.fmargin = (float)(64.0 * amax * 1.1920928955078125e-07 + 1e-6 * (2.0 * radmax + delta) + 1e-6);
     S.cell_off = cell_off;
     cell_off += (int)total;
   }
   ctx->n_cells = cell_off;
-  ctx->n_chunks = M;
-  RDK_TRY(ensure(ctx, ctx->chunk_rod, M));
-  RDK_TRY(ensure(ctx, ctx->chunk_k, M));
-  RDK_TRY(ensure(ctx, ctx->rod_nchunk, n));
-  RDK_HIP(hipMemcpyAsync(ctx->chunk_rod.p, crod.data(), M * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-  RDK_HIP(hipMemcpyAsync(ctx->chunk_k.p, ck.data(), M * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-  RDK_HIP(hipMemcpyAsync(ctx->rod_nchunk.p, nchunk.data(), n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
   RDK_TRY(ensure(ctx, ctx->sys_dev, ctx->nsys));
   RDK_HIP(hipMemcpyAsync(ctx->sys_dev.p, ctx->sys_host.data(), ctx->nsys * sizeof(SysDev), hipMemcpyHostToDevice,
                          ctx->stream));
-  RDK_HIP(hipStreamSynchronize(ctx->stream));
-  RDK_TRY(ensure(ctx, ctx->ckey, M));
-  RDK_TRY(ensure(ctx, ctx->ckey_alt, M));
-  RDK_TRY(ensure(ctx, ctx->cid, M));
-  RDK_TRY(ensure(ctx, ctx->cid_alt, M));
   RDK_TRY(ensure(ctx, ctx->cell_start, (size_t)ctx->n_cells + 1));
-  RDK_TRY(ensure(ctx, ctx->scf, M));
-  RDK_TRY(ensure(ctx, ctx->srod, M));
-  RDK_TRY(ensure(ctx, ctx->sk, M));
+  ctx->grid_dirty = false;
+  return RDK_OK;
+}
+
+int regrid(rdk_ctx* ctx) {
+  if (ctx->n_chunks <= 0) return RDK_OK;  // no grid yet: the next detection sizes it
+  return size_grids(ctx, ctx->built_delta, ctx->built_cell_scale);
+}
+
+static int build_grid(rdk_ctx* ctx, const rdk_detect_params* p) {
+  const double cl = p->chunk_len;
+  const double scale = p->cell_scale >= 1.0 ? p->cell_scale : 1.0;
+  const bool chunks_ok = cl == ctx->built_chunk_len && ctx->n_chunks > 0;
+  if (chunks_ok && scale == ctx->built_cell_scale && p->delta == ctx->built_delta && !ctx->grid_dirty)
+    return RDK_OK;
+  if (!chunks_ok) {
+    const int n = ctx->n;
+    std::vector<int> nchunk(n), crod, ck;
+    crod.reserve(n);
+    ck.reserve(n);
+    ctx->sys_radmax.assign(ctx->nsys, 0.0);
+    for (int i = 0; i < n; ++i) {
+      int K = 1;
+      if (cl > 0.0) K = std::max(1, (int)std::ceil(2.0 * ctx->h_h[i] / cl));
+      nchunk[i] = K;
+      for (int k = 0; k < K; ++k) {
+        crod.push_back(i);
+        ck.push_back(k);
+      }
+      const double rad = ctx->h_h[i] / K + ctx->h_r[i];
+      const int s = ctx->h_sys_of[i];
+      ctx->sys_radmax[s] = std::max(ctx->sys_radmax[s], rad);
+    }
+    const int M = (int)crod.size();
+    ctx->n_chunks = M;
+    RDK_TRY(ensure(ctx, ctx->chunk_rod, M));
+    RDK_TRY(ensure(ctx, ctx->chunk_k, M));
+    RDK_TRY(ensure(ctx, ctx->rod_nchunk, n));
+    RDK_HIP(hipMemcpyAsync(ctx->chunk_rod.p, crod.data(), M * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    RDK_HIP(hipMemcpyAsync(ctx->chunk_k.p, ck.data(), M * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    RDK_HIP(hipMemcpyAsync(ctx->rod_nchunk.p, nchunk.data(), n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    RDK_TRY(ensure(ctx, ctx->ckey, M));
+    RDK_TRY(ensure(ctx, ctx->ckey_alt, M));
+    RDK_TRY(ensure(ctx, ctx->cid, M));
+    RDK_TRY(ensure(ctx, ctx->cid_alt, M));
+    RDK_TRY(ensure(ctx, ctx->scf, M));
+    RDK_TRY(ensure(ctx, ctx->srod, M));
+    RDK_TRY(ensure(ctx, ctx->sk, M));
+  }
+  RDK_TRY(size_grids(ctx, p->delta, scale));
+  RDK_HIP(hipStreamSynchronize(ctx->stream));  // host chunk tables are released on return
   ctx->built_chunk_len = cl;
   ctx->built_cell_scale = scale;
   ctx->built_delta = p->delta;

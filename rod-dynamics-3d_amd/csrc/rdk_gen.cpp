@@ -57,11 +57,15 @@ static rdk::q4 random_quat(Rng& g) {
                  b * std::sin(2.0 * M_PI * u3)};
 }
 
-struct Grid {
+// Uniform grid of rod chunks: every rod is cut into K = ceil(2h / chunk)
+// pieces and each piece's bounding-sphere centre is binned (cell >= 2 * max
+// piece radius + gap), so a rod that touches the candidate has a piece in a
+// cell adjacent to one of the candidate's pieces.  Cells hold linked lists.
+struct ChunkGrid {
   int dims[3];
   double lo[3], cell[3];
   int periodic[3];
-  std::vector<std::vector<int>> cells;
+  std::vector<int> head, next, rod;
   int coord(int k, double c) const {
     int i = (int)std::floor((c - lo[k]) / cell[k]);
     if (periodic[k]) {
@@ -73,7 +77,14 @@ struct Grid {
     return i;
   }
   int lin(int x, int y, int z) const { return x + dims[0] * (y + dims[1] * z); }
+  void insert(int c, int i) {
+    next.push_back(head[c]);
+    rod.push_back(i);
+    head[c] = (int)rod.size() - 1;
+  }
 };
+
+int chunks_of(double h, double chunk) { return std::max(1, (int)std::ceil(2.0 * h / chunk)); }
 
 }  // namespace
 
@@ -89,24 +100,34 @@ extern "C" int rdk_gen_rsa(int n, const double* r, const double* h, const rdk_sy
     rmax = std::max(rmax, r[i]);
     hmax = std::max(hmax, h[i]);
   }
-  const double range = 2.0 * (hmax + rmax) + gap;
-  Grid G;
+  const double range = 2.0 * (hmax + rmax) + gap;  // reach of two whole rods (image selection)
+  const double chunk = std::max(8.0 * rmax, 1e-6);
+  double crad = 0.0;
+  for (int i = 0; i < n; ++i) crad = std::max(crad, h[i] / chunks_of(h[i], chunk) + r[i]);
+  const double cmin = (2.0 * crad + gap) * (1.0 + 1e-9) + 1e-12;
+  ChunkGrid G;
+  int64_t ncell = 1;
   for (int k = 0; k < 3; ++k) {
     span[k] = box->hi[k] - box->lo[k];
     if (!(span[k] > 0.0)) return RDK_ERR_ARG;
-    int d = std::max(1, (int)std::floor(span[k] / range));
+    int d = std::max(1, (int)std::floor(span[k] / cmin));
+    d = std::min(d, 256);
     if (box->periodic[k] && d < 3) d = 1;
-    d = std::min(d, 512);
     G.dims[k] = d;
     G.lo[k] = box->lo[k];
     G.cell[k] = span[k] / d;
     G.periodic[k] = box->periodic[k];
+    ncell *= d;
   }
-  G.cells.assign((size_t)G.dims[0] * G.dims[1] * G.dims[2], {});
+  G.head.assign((size_t)ncell, -1);
   std::vector<d3> cen(n), ax(n);
+  std::vector<int> stamp(n, -1);
+  int tag = 0;
   const int attempts = max_attempts > 0 ? max_attempts : 1000;
+  const bool wall = box->floor_wall || box->ceil_wall;
   for (int i = 0; i < n; ++i) {
     bool ok = false;
+    const int K = chunks_of(h[i], chunk);
     for (int att = 0; att < attempts && !ok; ++att) {
       d3 c;
       double* cp[3] = {&c.x, &c.y, &c.z};
@@ -114,7 +135,6 @@ extern "C" int rdk_gen_rsa(int n, const double* r, const double* h, const rdk_sy
       const rdk::q4 qq = random_quat(g);
       const d3 u = rdk::axis_of(qq);
       // walls on a non-periodic z: keep both caps inside with clearance gap
-      const bool wall = box->floor_wall || box->ceil_wall;
       if (wall) {
         const double zlo = std::min(c.z - u.z * h[i], c.z + u.z * h[i]) - r[i];
         const double zhi = std::max(c.z - u.z * h[i], c.z + u.z * h[i]) + r[i];
@@ -122,55 +142,61 @@ extern "C" int rdk_gen_rsa(int n, const double* r, const double* h, const rdk_sy
         if (box->ceil_wall && zhi > box->hi[2] - gap) continue;
       }
       const d3 a1 = c - u * h[i], a2 = c + u * h[i];
-      int cc[3] = {G.coord(0, c.x), G.coord(1, c.y), G.coord(2, c.z)};
+      ++tag;
       bool clash = false;
-      for (int dz = -1; dz <= 1 && !clash; ++dz) {
-        if (G.dims[2] == 1 && dz != 0) continue;
-        int zz = cc[2] + dz;
-        if (G.periodic[2]) zz = (zz + G.dims[2]) % G.dims[2];
-        else if (zz < 0 || zz >= G.dims[2]) continue;
-        for (int dy = -1; dy <= 1 && !clash; ++dy) {
-          if (G.dims[1] == 1 && dy != 0) continue;
-          int yy = cc[1] + dy;
-          if (G.periodic[1]) yy = (yy + G.dims[1]) % G.dims[1];
-          else if (yy < 0 || yy >= G.dims[1]) continue;
-          for (int dx = -1; dx <= 1 && !clash; ++dx) {
-            if (G.dims[0] == 1 && dx != 0) continue;
-            int xx = cc[0] + dx;
-            if (G.periodic[0]) xx = (xx + G.dims[0]) % G.dims[0];
-            else if (xx < 0 || xx >= G.dims[0]) continue;
-            for (int j : G.cells[G.lin(xx, yy, zz)]) {
-              const double lim = r[i] + r[j] + gap;
-              // every periodic image within reach (27-image check)
-              int lo_i[3], hi_i[3];
-              double base[3];
-              const double dd[3] = {cen[j].x - c.x, cen[j].y - c.y, cen[j].z - c.z};
-              for (int k = 0; k < 3; ++k) {
-                base[k] = 0.0;
-                lo_i[k] = hi_i[k] = 0;
-                if (G.periodic[k]) {
-                  base[k] = -std::floor(dd[k] / span[k] + 0.5) * span[k];
-                  if (span[k] < 2.0 * range + 1e-9) {
-                    lo_i[k] = -1;
-                    hi_i[k] = 1;
+      for (int kc = 0; kc < K && !clash; ++kc) {
+        const d3 pc = a1 + (a2 - a1) * (((double)kc + 0.5) / (double)K);
+        const int cc[3] = {G.coord(0, pc.x), G.coord(1, pc.y), G.coord(2, pc.z)};
+        for (int dz = -1; dz <= 1 && !clash; ++dz) {
+          if (G.dims[2] == 1 && dz != 0) continue;
+          int zz = cc[2] + dz;
+          if (G.periodic[2]) zz = (zz + G.dims[2]) % G.dims[2];
+          else if (zz < 0 || zz >= G.dims[2]) continue;
+          for (int dy = -1; dy <= 1 && !clash; ++dy) {
+            if (G.dims[1] == 1 && dy != 0) continue;
+            int yy = cc[1] + dy;
+            if (G.periodic[1]) yy = (yy + G.dims[1]) % G.dims[1];
+            else if (yy < 0 || yy >= G.dims[1]) continue;
+            for (int dx = -1; dx <= 1 && !clash; ++dx) {
+              if (G.dims[0] == 1 && dx != 0) continue;
+              int xx = cc[0] + dx;
+              if (G.periodic[0]) xx = (xx + G.dims[0]) % G.dims[0];
+              else if (xx < 0 || xx >= G.dims[0]) continue;
+              for (int e = G.head[G.lin(xx, yy, zz)]; e >= 0 && !clash; e = G.next[e]) {
+                const int j = G.rod[e];
+                if (stamp[j] == tag) continue;  // rod already tested for this attempt
+                stamp[j] = tag;
+                const double lim = r[i] + r[j] + gap;
+                // every periodic image within reach (27-image check)
+                int lo_i[3], hi_i[3];
+                double base[3];
+                const double dd[3] = {cen[j].x - c.x, cen[j].y - c.y, cen[j].z - c.z};
+                for (int k = 0; k < 3; ++k) {
+                  base[k] = 0.0;
+                  lo_i[k] = hi_i[k] = 0;
+                  if (G.periodic[k]) {
+                    base[k] = -std::floor(dd[k] / span[k] + 0.5) * span[k];
+                    if (span[k] < 2.0 * range + 1e-9) {
+                      lo_i[k] = -1;
+                      hi_i[k] = 1;
+                    }
                   }
                 }
+                for (int iz = lo_i[2]; iz <= hi_i[2] && !clash; ++iz)
+                  for (int iy = lo_i[1]; iy <= hi_i[1] && !clash; ++iy)
+                    for (int ix = lo_i[0]; ix <= hi_i[0] && !clash; ++ix) {
+                      const d3 sh = mk3(base[0] + ix * span[0], base[1] + iy * span[1], base[2] + iz * span[2]);
+                      const d3 cj = cen[j] + sh;
+                      const d3 dc = cj - c;
+                      const double reach = h[i] + h[j] + lim;
+                      if (rdk::dot(dc, dc) >= reach * reach) continue;
+                      const d3 b1 = cj - ax[j] * h[j], b2 = cj + ax[j] * h[j];
+                      double s, t;
+                      rdk::segseg(a1, a2, b1, b2, s, t);
+                      const d3 d = (b1 + (b2 - b1) * t) - (a1 + (a2 - a1) * s);
+                      if (rdk::dot(d, d) < lim * lim) clash = true;
+                    }
               }
-              for (int iz = lo_i[2]; iz <= hi_i[2] && !clash; ++iz)
-                for (int iy = lo_i[1]; iy <= hi_i[1] && !clash; ++iy)
-                  for (int ix = lo_i[0]; ix <= hi_i[0] && !clash; ++ix) {
-                    const d3 sh = mk3(base[0] + ix * span[0], base[1] + iy * span[1], base[2] + iz * span[2]);
-                    const d3 cj = cen[j] + sh;
-                    const d3 dc = cj - c;
-                    const double reach = h[i] + h[j] + lim;
-                    if (rdk::dot(dc, dc) >= reach * reach) continue;
-                    const d3 b1 = cj - ax[j] * h[j], b2 = cj + ax[j] * h[j];
-                    double s, t;
-                    rdk::segseg(a1, a2, b1, b2, s, t);
-                    const d3 d = (b1 + (b2 - b1) * t) - (a1 + (a2 - a1) * s);
-                    if (rdk::dot(d, d) < lim * lim) clash = true;
-                  }
-              if (clash) break;
             }
           }
         }
@@ -185,7 +211,10 @@ extern "C" int rdk_gen_rsa(int n, const double* r, const double* h, const rdk_sy
       q[4 * i + 1] = qq.x;
       q[4 * i + 2] = qq.y;
       q[4 * i + 3] = qq.z;
-      G.cells[G.lin(cc[0], cc[1], cc[2])].push_back(i);
+      for (int kc = 0; kc < K; ++kc) {
+        const d3 pc = a1 + (a2 - a1) * (((double)kc + 0.5) / (double)K);
+        G.insert(G.lin(G.coord(0, pc.x), G.coord(1, pc.y), G.coord(2, pc.z)), i);
+      }
       ok = true;
     }
     if (!ok) return RDK_OK;  // *placed holds the count placed so far

@@ -208,4 +208,35 @@ int observables_run(rdk_ctx* ctx, double* ke, double* phi) {
   return RDK_OK;
 }
 
+// Affine isotropic scaling about each box's lower corner: x <- lo + s (x - lo).
+__global__ void k_affine(BodyPtrs B, const SysDev* __restrict__ sys, int n, double s) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const SysDev& S = sys[B.sys[i]];
+  for (int k = 0; k < 3; ++k) B.x[k][i] = S.lo[k] + s * (B.x[k][i] - S.lo[k]);
+}
+
+int affine_run(rdk_ctx* ctx, double s) {
+  for (int sy = 0; sy < ctx->nsys; ++sy) {
+    SysDev& S = ctx->sys_host[sy];
+    for (int k = 0; k < 3; ++k) {
+      S.span[k] *= s;
+      S.hi[k] = S.lo[k] + S.span[k];
+      ctx->sys_desc[sy].hi[k] = S.hi[k];
+    }
+  }
+  // positions scale with the old device constants (lo is unchanged)
+  {
+    ProfScope ps(ctx, K_NSC_POS);
+    hipLaunchKernelGGL(k_affine, dim3(grid_for(ctx->n, 256)), dim3(256), 0, ctx->stream, body_ptrs(ctx),
+                       ctx->sys_dev.p, ctx->n, s);
+  }
+  if (ctx->n_chunks > 0) return regrid(ctx);  // cells, margins and spans for the new boxes
+  ctx->grid_dirty = true;
+  RDK_TRY(ensure(ctx, ctx->sys_dev, ctx->nsys));
+  RDK_HIP(hipMemcpyAsync(ctx->sys_dev.p, ctx->sys_host.data(), ctx->nsys * sizeof(SysDev), hipMemcpyHostToDevice,
+                         ctx->stream));
+  return RDK_OK;
+}
+
 }  // namespace rdk

@@ -116,6 +116,8 @@ struct rdk_ctx {
   // ---- broad phase ----
   double built_chunk_len = -2.0, built_cell_scale = -1.0, built_delta = -1.0;
   int n_chunks = 0, n_cells = 0;
+  std::vector<double> sys_radmax;  // largest chunk bounding radius per system
+  bool grid_dirty = false;         // a box changed since the grid was sized
   rdk::DevBuf<int> chunk_rod, chunk_k, rod_nchunk;
   rdk::DevBuf<uint32_t> ckey, ckey_alt;
   rdk::DevBuf<int> cid, cid_alt;
@@ -264,6 +266,8 @@ static inline int grid_for(int64_t n, int block) { return (int)((n + block - 1) 
 
 // stage entry points implemented in the .hip files
 int detect_run(rdk_ctx* ctx, const rdk_detect_params* p);
+int regrid(rdk_ctx* ctx);
+int affine_run(rdk_ctx* ctx, double s);
 int nsc_build_run(rdk_ctx* ctx, const rdk_nsc_params* p);
 int nsc_solve_run(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* st);
 int nsc_sweeps_run(rdk_ctx* ctx, const rdk_nsc_params* p, int nsweeps);

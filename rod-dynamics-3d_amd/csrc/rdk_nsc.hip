@@ -440,12 +440,12 @@ struct SolveConst {
 };
 
 // Stall rule (stop_rule 1, stall_sweeps > 0): at sweep counts stall, 2 stall,
-// 4 stall, ... the residual must have at least halved since the previous such
-// milestone, otherwise the solve stops there (reported as not converged).
-// Wedged, statically indeterminate groups of rods let the impulses drift along
-// a null direction of the Delassus matrix with the residual on a floor; this
-// stops them without touching solves that converge (linear convergence halves
-// the residual many times over between milestones).
+// 4 stall, ... the residual must have dropped by at least 10 % since the
+// previous such milestone, otherwise the solve stops there (reported as not
+// converged).  Wedged, statically indeterminate groups of rods let the
+// impulses drift along a null direction of the Delassus matrix while the
+// residual sits on a floor or creeps up; this stops them without touching
+// solves that still make progress, however slowly.
 struct StallTrack {
   int next;
   double ref;
@@ -455,7 +455,7 @@ struct StallTrack {
   }
   __host__ __device__ bool stalled(int sweeps, double res) {
     if (next <= 0 || sweeps < next) return false;
-    const bool st = ref >= 0.0 && !(res <= 0.5 * ref);
+    const bool st = ref >= 0.0 && !(res <= 0.9 * ref);
     ref = res;
     while (next <= sweeps) next *= 2;
     return st;

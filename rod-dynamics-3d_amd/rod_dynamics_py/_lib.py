@@ -111,6 +111,7 @@ _SIGS = {
     "rdk_gen_rsa": (C.c_int, [C.c_int, _dp, _dp, C.POINTER(SystemDesc), C.c_double, C.c_uint64, C.c_int, _dp, _dp,
                               _ip]),
     "rdk_gen_thermal": (C.c_int, [C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_uint64, _dp, _dp]),
+    "rdk_affine_scale": (C.c_int, [C.c_void_p, C.c_double]),
     "rdk_mlp_num_params": (C.c_int, []),
     "rdk_mlp_init": (C.c_int, [C.c_void_p, C.c_uint64]),
     "rdk_mlp_set_params": (C.c_int, [C.c_void_p, _fp, C.c_int]),
@@ -412,6 +413,9 @@ class Context:
         names = ["steps", "body_passes", "chunks", "candidates", "contacts", "manifolds", "sweep_contacts",
                  "resid_contacts", "integrate_bodies", "force_contacts"]
         return {nm: arr[i] for i, nm in enumerate(names)}
+
+    def affine_scale(self, s):
+        self._check(self._L.rdk_affine_scale(self._h, float(s)))
 
     # ---- learned warm start ----
     def mlp_init(self, seed=0):

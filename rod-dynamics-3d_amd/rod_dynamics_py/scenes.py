@@ -7,6 +7,10 @@ model parameters.  Seeds default to ``0x5EED0000 + config id``.
   C1  soft/HM thermal gas: N=1000, d=1, L=10, periodic cube at phi=0.10
   C2  hard-contact PSOR settling: N=1e5, L/d=20, periodic x,y, floor and
       ceiling walls, phi=0.05, gravity -z, mu=0.5, e=0, omega=1.3, dt=1e-3
+  C3  dense polydisperse Hertz-Mindlin: N=1e6, L ~ lognormal(median 16,
+      sigma_log 0.3) truncated to [4, 64], d=1, periodic cube generated at
+      phi=0.05 and compressed affinely to phi=0.30 (build protocol), mu=0.3,
+      E=1e7, nu=0.25, e=0.5, thermal T=1, dt=5e-5
   C5  learned-warm-start bed: as C2 with N=2e5 (settled by running C2)
 
 Masses follow the reference's capsule factory (density * pi r^2 * 2h, the
@@ -79,9 +83,9 @@ def c2_settling(n=100_000, phi=0.05, seed=SEED0 + 2, mu=0.5, aspect=20.0, gap=0.
     # converge to a 1e-8 fixed-point residual on sliding beds
     # Plane-wall contacts also start from the previous step's impulses
     # (the reference has no plane walls and starts its cylinder-wall contacts cold).
-    # Islands whose residual stops halving between 1000, 2000, 4000, ... sweeps
-    # (wedged rods: impulses drift along a null direction) stop and count as
-    # not converged instead of running to max_sweeps.
+    # Islands whose residual no longer drops by 10 % between 1000, 2000, 4000,
+    # ... sweeps (wedged rods: impulses drift along a null direction) stop and
+    # count as not converged instead of running to max_sweeps.
     nscp = L.nsc_params(beta=0.0, cfm=0.0, omega=1.3, tol=1e-8, max_sweeps=10000, check_every=4,
                         friction_proj=1, position_stabilization=0, warm_mode=1, stop_rule=1, warm_walls=1,
                         stall_sweeps=1000)
@@ -114,3 +118,39 @@ def c1_thermal(n=1000, phi=0.10, seed=SEED0 + 1, model="hm", aspect=10.0, kT=1.0
     return Scene(name=f"C1_{model}_N{n}", x=x, q=q, v=v, w=w, r=r, h=h, mass=mass, rest=np.full(n, 0.5),
                  offsets=np.array([0, n], np.int32), descs=[desc], phys=L.physics((0, 0, 0), 1e-4, 0.0, 0.0, 1e30),
                  model=model, params=params, detect=det, extra=dict(phi=phi, side=side, seed=seed))
+
+
+def lognormal_lengths(n, rng, median=16.0, sigma=0.3, lo=4.0, hi=64.0):
+    """Lengths L ~ lognormal(median, sigma_log), truncated to [lo, hi] by redrawing."""
+    L = np.exp(np.log(median) + sigma * rng.standard_normal(n))
+    bad = (L < lo) | (L > hi)
+    while bad.any():
+        L[bad] = np.exp(np.log(median) + sigma * rng.standard_normal(int(bad.sum())))
+        bad = (L < lo) | (L > hi)
+    return L
+
+
+def c3_dense(n=1_000_000, phi0=0.05, phi1=0.30, seed=SEED0 + 3, mu=0.3, kT=1.0):
+    """Dense polydisperse Hertz-Mindlin packing, generated
+    by RSA at phi0; ``extra['side1']`` is the side at the target phi1 that the
+    compression protocol (rod_dynamics_py.protocols.compress) reaches."""
+    rng = np.random.default_rng(seed)
+    Ls = lognormal_lengths(n, rng)
+    r = np.full(n, 0.5)
+    h = 0.5 * Ls
+    vol = float(np.sum(math.pi * r * r * Ls + 4.0 / 3.0 * math.pi * r ** 3))
+    side0 = (vol / phi0) ** (1.0 / 3.0)
+    side1 = (vol / phi1) ** (1.0 / 3.0)
+    desc = L.system_desc((0, 0, 0), (side0, side0, side0), (1, 1, 1), mu=mu)
+    x, q, placed = L.gen_rsa(r, h, desc, gap=0.0, seed=seed, max_attempts=20000)
+    if placed != n:
+        raise RuntimeError(f"RSA placed {placed} of {n} rods")
+    mass = math.pi * r * r * Ls  # density 1, cylinder part (rigid_body.cpp:19-24)
+    v, w = L.gen_thermal(q, r, h, mass, kT, seed + 17)
+    params = L.hm_params(E=1e7, nu_poisson=0.25, restitution=0.5, mu_k=mu, mu_s=mu, v_c=1e-3, mu_roll=0.0,
+                         enable_tangential=1, enable_rolling=0, law=0)
+    return Scene(name=f"C3_hm_N{n}", x=x, q=q, v=v, w=w, r=r, h=h, mass=mass, rest=np.full(n, 0.5),
+                 offsets=np.array([0, n], np.int32), descs=[desc], phys=L.physics((0, 0, 0), 5e-5, 0.0, 0.0, 1e30),
+                 model="hm", params=params, detect=L.detect_params(0.0, 4.0, 1.0),
+                 extra=dict(phi0=phi0, phi1=phi1, side=side0, side1=side1, seed=seed, mean_length=float(Ls.mean()),
+                            truncation=[4.0, 64.0]))

@@ -242,7 +242,7 @@ def test_island_and_global_solves_agree(gpu_ctx_factory):
 
 def test_stall_rule(gpu_ctx_factory):
     """tol = 0 never converges: the residual reaches the rounding floor, stops
-    halving between milestones and the solve stops long before max_sweeps.  A
+    dropping between milestones and the solve stops long before max_sweeps.  A
     solve that converges is not affected by the rule (bitwise the same)."""
     B = clusters(seed=5, mu=0.0)
     ctx = gpu_ctx_factory()
