@@ -18,6 +18,9 @@ step as the reference integrator does).
 --workload c3: dense polydisperse Hertz-Mindlin, 1e6 rods (L lognormal,
 median 16), generated at phi=0.05, compressed to phi=0.30 (untimed
 preparation), then 2000 timed steps at dt=5e-5 with an affine strain rate 1e-3.
+--workload c4: 256 independent periodic hard-contact packings of uneven size
+(N 500..20000, L/d 4..64, mu 0..1) stepped together, one launch per stage for
+all systems, chunk length autotuned in the warm-up, 1000 steps.
 """
 import argparse
 import json
@@ -52,13 +55,13 @@ TRAFFIC = {}
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["c2", "c1", "c3"], default="c2")
+    ap.add_argument("--workload", choices=["c2", "c1", "c3", "c4"], default="c2")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--cpu-steps", type=int, default=None, help="CPU baseline sample size (steps)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--chunk-len", type=float, default=4.0)
+    ap.add_argument("--chunk-len", type=float, default=0.0, help="0: autotune over a few steps of the warm-up")
     ap.add_argument("--check-every", type=int, default=4)
     ap.add_argument("--block", type=int, default=250)
     ap.add_argument("--ab-steps", type=int, default=20,
@@ -67,7 +70,8 @@ def parse():
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (HBM counter calibration)")
     a = ap.parse_args()
-    defaults = {"c2": (4980, 100_000, 5), "c1": (9980, 1000, 2000), "c3": (2000, 1_000_000, 2)}[a.workload]
+    defaults = {"c2": (4980, 100_000, 5), "c1": (9980, 1000, 2000), "c3": (2000, 1_000_000, 2),
+                "c4": (980, 256, 8)}[a.workload]
     if a.steps is None:
         a.steps = defaults[0]
     if a.n is None:
@@ -104,6 +108,24 @@ def oracle():
 
 def cores():
     return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+def autotune(ctx, run_steps, cands, reps=3):
+    """Time `reps` steps (after one untimed step) for each chunk length in
+    `cands`; detection results do not depend on the chunking, so these are
+    ordinary steps of the run.  Returns (best chunk length, ms per step by
+    candidate, steps taken)."""
+    tune = {}
+    for cl in cands:
+        run_steps(cl, 1)
+        ctx.sync()
+        t = time.perf_counter()
+        run_steps(cl, reps)
+        ctx.sync()
+        tune[cl] = 1e3 * (time.perf_counter() - t) / reps
+    best = min(tune, key=tune.get)
+    log(f"autotune chunk_len {best} ({', '.join(f'{k}: {v:.2f} ms' for k, v in tune.items())})")
+    return best, {str(k): v for k, v in tune.items()}, len(cands) * (reps + 1)
 
 
 def rank_kernels(ctx, run_warmup):
@@ -177,10 +199,15 @@ def run_c2(a, L, scenes):
     log(f"scene {sc.name} built in {time.perf_counter() - t0:.1f}s (box {sc.extra['side']:.2f})")
     p = sc.params
     p.check_every = a.check_every
-    det = L.detect_params(0.0, a.chunk_len, 1.0)
     ctx = L.Context(0)
     sc.load(ctx)
-    dom = rank_kernels(ctx, lambda: ctx.step_nsc(det, p, a.warmup))
+    if a.chunk_len > 0:
+        best, tune, used = a.chunk_len, {}, 0
+    else:  # --chunk-len 0: autotune inside the warm-up
+        best, tune, used = autotune(ctx, lambda cl, k: ctx.step_nsc(L.detect_params(0.0, cl, 1.0), p, k),
+                                    (2.0, 4.0, 8.0), reps=min(3, max(1, a.warmup // 3 - 1)))
+    det = L.detect_params(0.0, best, 1.0)
+    dom = rank_kernels(ctx, lambda: ctx.step_nsc(det, p, max(1, a.warmup - used)))
     log(f"warmup {a.warmup} steps done; dominant kernel {dom}; timing {a.steps} steps")
     total = a.warmup + a.steps
     snap_at = [int(total * (k + 0.5) / 5) for k in range(5)] if not a.no_cpu else []
@@ -212,7 +239,8 @@ def run_c2(a, L, scenes):
         "box": round(sc.extra["side"], 3), "periodic": "x,y", "walls": "floor+ceiling", "mu": 0.5,
         "restitution": 0.0, "omega": 1.3, "residual_tol": 1e-8, "dt": 1e-3,
         "friction_projection": "euclidean disc" if p.friction_proj == 1 else "reference sequential",
-        "steps_total": total, "seed": sc.extra["seed"], "check_every": a.check_every, "chunk_len": a.chunk_len,
+        "steps_total": total + max(0, used + max(1, a.warmup - used) - a.warmup), "seed": sc.extra["seed"],
+        "check_every": a.check_every, "chunk_len": best, "chunk_len_autotune_ms": tune,
         "stall_sweeps": p.stall_sweeps})
     out["psor"] = {"sweeps_per_step": st["sweeps_total"] / max(st["steps"], 1),
                    "manifolds_per_step": st["manifolds_total"] / max(st["steps"], 1),
@@ -277,7 +305,7 @@ def run_vv(a, L, scenes, which):
         sc = scenes.c1_thermal(n=a.n, model="hm")
     else:
         sc = scenes.c3_dense(n=a.n)
-    sc.detect = L.detect_params(0.0, a.chunk_len if which == "c3" else 0.0, 1.0)
+    sc.detect = L.detect_params(0.0, a.chunk_len, 1.0)
     log(f"scene {sc.name} built in {time.perf_counter() - t0:.1f}s (box {sc.extra['side']:.2f})")
     ctx = L.Context(0)
     sc.load(ctx)
@@ -294,7 +322,16 @@ def run_vv(a, L, scenes, which):
                                f"velocities T=1", "prep_seconds": time.perf_counter() - t0}
         log(f"compressed to side {sc.extra['side_now']:.3f} in {prep['prep_seconds']:.1f}s")
     rate = 1e-3 if which == "c3" else 0.0
-    dom = rank_kernels(ctx, lambda: protocols.strained_vv(ctx, sc, a.warmup, rate))
+    tune, used = {}, 0
+    if a.chunk_len <= 0:  # autotune inside the warm-up
+        def run(cl, k):
+            sc.detect = L.detect_params(0.0, cl, 1.0)
+            protocols.strained_vv(ctx, sc, k, rate)
+
+        best, tune, used = autotune(ctx, run, (1.0, 2.0, 4.0) if which == "c3" else (1.0, 2.0, 4.0, 10.0),
+                                    reps=min(3, max(1, a.warmup // 4 - 1)))
+        sc.detect = L.detect_params(0.0, best, 1.0)
+    dom = rank_kernels(ctx, lambda: protocols.strained_vv(ctx, sc, max(1, a.warmup - used), rate))
     log(f"warmup {a.warmup} steps done; dominant kernel {dom}; timing {a.steps} steps")
     done = 0
     st = None
@@ -315,14 +352,16 @@ def run_vv(a, L, scenes, which):
     if which == "c1":
         cfg = {"workload": "Hertz-Mindlin thermal gas (C1)", "rods": sc.n, "L_over_d": 10, "phi": 0.10,
                "box": round(sc.extra["side"], 3), "periodic": "x,y,z", "mu": 0.5, "E": 1e7, "nu": 0.25,
-               "restitution": 0.5, "dt": 1e-4, "steps_total": a.warmup + a.steps, "seed": sc.extra["seed"],
-               "force_evaluations_per_step": 2}
+               "restitution": 0.5, "dt": 1e-4, "steps_total": max(a.warmup, used + 1) + a.steps,
+               "seed": sc.extra["seed"], "force_evaluations_per_step": 2, "chunk_len": sc.detect.chunk_len,
+               "chunk_len_autotune_ms": tune}
     else:
         cfg = {"workload": "dense polydisperse Hertz-Mindlin (C3)", "rods": sc.n,
                "L": "lognormal median 16 sigma_log 0.3, truncated [4, 64]", "mean_L": round(sc.extra["mean_length"], 3),
                "phi": 0.30, "box": round(sc.extra["side_now"], 3), "periodic": "x,y,z", "mu": 0.3, "E": 1e7,
                "nu": 0.25, "restitution": 0.5, "dt": 5e-5, "strain_rate": 1e-3, "steps_timed": a.steps,
-               "seed": sc.extra["seed"], "force_evaluations_per_step": 2, "chunk_len": a.chunk_len}
+               "seed": sc.extra["seed"], "force_evaluations_per_step": 2, "chunk_len": sc.detect.chunk_len,
+               "chunk_len_autotune_ms": tune}
         cfg.update(prep)
     out = base_line("rod-steps/s", sc.n * a.steps / el, a, el, "f64", cfg)
     out["contacts_per_evaluation"] = st["contacts_total"] / (2 * max(st["steps"], 1))
@@ -333,6 +372,90 @@ def run_vv(a, L, scenes, which):
         log(f"cpu baseline: {a.cpu_steps} oracle steps")
         out["cpu_baseline"] = cpu_baseline_vv(sc, state, a.cpu_steps, oracle(),
                                               "the GPU state after the timed steps of the same run")
+    return out
+
+
+# ---------------------------------------------------------------------------- C4
+def cpu_baseline_c4(sc, state, nsample, O):
+    """Systems run one after another on the FP64 oracle with all cores, as the
+    reference would: a size-stratified sample of systems, one unmeasured and
+    one timed step each from the GPU state after the run."""
+    import numpy as np
+
+    sizes = np.diff(sc.offsets)
+    order = np.argsort(sizes)
+    pick = [int(order[int((k + 0.5) * len(order) / nsample)]) for k in range(nsample)]
+    rods, secs, per = 0, 0.0, []
+    for k in pick:
+        a0, a1 = int(sc.offsets[k]), int(sc.offsets[k + 1])
+        d = sc.descs[k]
+        W = O.World(state["x"][a0:a1], state["q"][a0:a1], state["v"][a0:a1], state["w"][a0:a1], sc.r[a0:a1],
+                    sc.h[a0:a1], sc.mass[a0:a1], sc.rest[a0:a1])
+        W.set_box(list(d.lo), list(d.hi), [int(v) for v in d.periodic], 0, 0)
+        W.set_physics(tuple(sc.phys.gravity[:]), sc.phys.dt)
+        p = O.nsc_params(mu=d.mu, beta=0.0, cfm=0.0, omega=1.3, tol=1e-8, max_sweeps=10000, check_every=1,
+                         friction_proj=1, stop_rule=1, warm_start=1)
+        W.nsc_step(p, 1, method=1)
+        t0 = time.perf_counter()
+        sw, _ = W.nsc_step(p, 1, method=1)
+        dt = time.perf_counter() - t0
+        rods += a1 - a0
+        secs += dt
+        per.append({"system": k, "rods": a1 - a0, "seconds": dt, "sweeps": sw})
+        log(f"cpu baseline: system {k} ({a1 - a0} rods): {dt:.3f}s")
+    return {"value": rods / secs, "unit": "rod-steps/s", "cores": cores(), "kind": "port",
+            "sample": f"{nsample} systems (size-stratified) run one after another on the FP64 oracle (reference "
+                      f"spatial hash + serial sorted-order PSOR to 1e-8, OpenMP {cores()} threads), one timed step "
+                      f"each after one unmeasured step, from the GPU state after the run",
+            "seconds": secs, "per_system": per}
+
+
+def run_c4(a, L, scenes):
+    t0 = time.perf_counter()
+    sc = scenes.c4_batched(nsys=a.n, threads=min(16, os.cpu_count() or 1))
+    log(f"scene {sc.name} built in {time.perf_counter() - t0:.1f}s ({sc.n} rods, {sc.extra['redraws']} redraws)")
+    p = sc.params
+    ctx = L.Context(0)
+    sc.load(ctx)
+    best, tune, used = autotune(ctx, lambda cl, k: ctx.step_nsc(L.detect_params(0.0, cl, 1.0), p, k),
+                                (2.0, 4.0, 8.0), reps=3)
+    det = L.detect_params(0.0, best, 1.0)
+    dom = rank_kernels(ctx, lambda: ctx.step_nsc(det, p, a.warmup))
+    log(f"warmup {a.warmup} steps done; dominant kernel {dom}; timing {a.steps} steps")
+    st = None
+    done = 0
+    t1 = time.perf_counter()
+    while done < a.steps:
+        k = min(a.block, a.steps - done)
+        s_ = ctx.step_nsc(det, p, k)
+        done += k
+        if st is None:
+            st = dict(s_)
+        else:
+            for key in st:
+                st[key] = max(st[key], s_[key]) if key == "residual_max" else st[key] + s_[key]
+        log(f"step {done}: {time.perf_counter() - t1:.2f}s, manifolds/step {s_['manifolds_total'] / k:.0f}, "
+            f"sweeps/step {s_['sweeps_total'] / k:.1f}")
+    ctx.sync()
+    el = time.perf_counter() - t1
+    tuned_steps = used
+    out = base_line("rod-steps/s at PSOR residual 1e-8", sc.n * a.steps / el, a, el, "f64", {
+        "workload": "batched hard-contact packings (C4)", "systems": len(sc.descs), "rods": sc.n,
+        "N": "floor(exp(U[ln500, ln20000]))", "L_over_d": "{4,8,16,32,64}", "mu": "U[0,1]",
+        "phi": "U[min(0.05,cap), cap], cap = min(0.25, 0.85 x RSA saturation)", "periodic": "x,y,z",
+        "restitution": sc.extra["restitution"], "kT": sc.extra["kT"], "gravity": 0.0, "omega": 1.3,
+        "residual_tol": 1e-8, "dt": 1e-3, "redraws": sc.extra["redraws"], "seed": sc.extra["seed"],
+        "chunk_len_autotuned": best, "autotune_ms_per_step": tune,
+        "steps_total": tuned_steps + a.warmup + a.steps, "stall_sweeps": p.stall_sweeps,
+        "hipgraph": "not used: each step reads contact and island counts back to size its launches"})
+    out["psor"] = {"sweeps_per_step": st["sweeps_total"] / max(st["steps"], 1),
+                   "manifolds_per_step": st["manifolds_total"] / max(st["steps"], 1),
+                   "residual_max": st["residual_max"], "nonconverged_steps": st["nonconverged_steps"]}
+    out["roofline"] = roofline(ctx, dom, sc.n)
+    state = ctx.download_state()
+    ctx.close()
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_c4(sc, state, a.cpu_steps, oracle())
     return out
 
 
@@ -349,6 +472,8 @@ def main():
 
     if a.workload == "c2":
         out = run_c2(a, L, scenes)
+    elif a.workload == "c4":
+        out = run_c4(a, L, scenes)
     else:
         out = run_vv(a, L, scenes, a.workload)
     print(json.dumps(out), flush=True)

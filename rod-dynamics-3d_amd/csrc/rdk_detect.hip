@@ -149,28 +149,15 @@ __device__ __forceinline__ int owner_chunk(double s, int K) {
   return k < 0 ? 0 : (k >= K ? K - 1 : k);
 }
 
-// One thread per sorted chunk p.  Every unordered chunk pair (p, q) is
-// examined once, by the thread with the lower sorted position, so each row
-// range is clipped to q > p.  The FP32 prefilter uses the chunk-centre minimum
-// image, which is never farther than the rod-centre image the narrow phase
-// uses, plus a rounding margin, so it never rejects a touching pair.
-__device__ __forceinline__ void chunk_pair(const BodyPtrs& B, const SysDev& S, const SortedChunks& SC,
-                                           const int* __restrict__ rod_nchunk, int p, int q, int a, bool a_static,
-                                           float4 me, const float* spanf, const float* ispanf, float extra,
-                                           double delta, uint64_t* __restrict__ out, unsigned cap,
-                                           unsigned* __restrict__ counter, unsigned long long& ncand) {
-  const float4 o = SC.cf[q];
-  float dxf = o.x - me.x, dyf = o.y - me.y, dzf = o.z - me.z;
-  if (S.periodic[0]) dxf -= spanf[0] * rintf(dxf * ispanf[0]);
-  if (S.periodic[1]) dyf -= spanf[1] * rintf(dyf * ispanf[1]);
-  if (S.periodic[2]) dzf -= spanf[2] * rintf(dzf * ispanf[2]);
-  const float thr = me.w + o.w + extra;
-  if (!(dxf * dxf + dyf * dyf + dzf * dzf < thr * thr)) return;
-  const int b = SC.rod[q];
-  if (b == a) return;
-  if (a_static && !(B.inv_mass[b] > 0.0)) return;
-  ++ncand;
-  // exact FP64 narrow phase in canonical (lower index first) order
+// FP64 narrow phase of one surviving chunk pair (p, q), both in sorted
+// order; emits the canonical (lower, higher) rod pair when this chunk pair
+// owns the closest points.
+__device__ __forceinline__ void narrow_pair(const BodyPtrs& B, const SysDev* __restrict__ sys,
+                                            const SortedChunks& SC, const int* __restrict__ rod_nchunk, int p, int q,
+                                            double delta, uint64_t* __restrict__ out, unsigned cap,
+                                            unsigned* __restrict__ counter) {
+  const int a = SC.rod[p], b = SC.rod[q];
+  const SysDev& S = sys[B.sys[a]];
   const int A = a < b ? a : b, Bi = a < b ? b : a;
   const int kA = a < b ? SC.k[p] : SC.k[q], kB = a < b ? SC.k[q] : SC.k[p];
   NarrowOut no;
@@ -183,74 +170,158 @@ __device__ __forceinline__ void chunk_pair(const BodyPtrs& B, const SysDev& S, c
   }
 }
 
-__global__ void __launch_bounds__(256) k_broad_narrow(BodyPtrs B, const SysDev* __restrict__ sys, SortedChunks SC,
-                                                      const int* __restrict__ cstart,
-                                                      const int* __restrict__ rod_nchunk, int M, double delta,
-                                                      uint64_t* __restrict__ out, unsigned cap,
-                                                      unsigned* __restrict__ counter,
-                                                      unsigned long long* __restrict__ cand_counter) {
-  const int p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  unsigned long long ncand = 0;
-  if (p < M) {
-    const int a = SC.rod[p];
-    const float4 me = SC.cf[p];
-    const SysDev S = sys[B.sys[a]];
-    const bool a_static = !(B.inv_mass[a] > 0.0);
-    const float spanf[3] = {(float)S.span[0], (float)S.span[1], (float)S.span[2]};
-    const float ispanf[3] = {(float)(1.0 / S.span[0]), (float)(1.0 / S.span[1]), (float)(1.0 / S.span[2])};
-    const float extra = (float)delta + S.fmargin;
+// Stencil ranges of one chunk: 9 rows (dz, dy), each as up to two contiguous
+// x ranges of sorted positions (two after a periodic wrap).  Range r = 2 *
+// row + part.  Returns false when no range is left; q is clipped to q > p.
+struct StencilWalk {
+  int p, cx, cy, cz, xa0, xa1, xb0, xb1;
+  bool two;
+  int r, q, q1;
+  __device__ __forceinline__ bool open_next(const SysDev& S, const int* __restrict__ cstart) {
     const int d0 = S.dims[0], d1 = S.dims[1], d2 = S.dims[2];
-    const int cell = (int)SC.key[p] - S.cell_off;
-    const int cx = cell % d0, cy = (cell / d0) % d1, cz = cell / (d0 * d1);
-    // x extent of a stencil row as up to two contiguous cell ranges
-    int xa0, xa1, xb0 = 0, xb1 = 0;  // [xa0, xa1] and, after a periodic wrap, [xb0, xb1]
-    bool two = false;
-    if (d0 == 1) {
-      xa0 = xa1 = 0;
-    } else if (!S.periodic[0]) {
-      xa0 = cx > 0 ? cx - 1 : 0;
-      xa1 = cx < d0 - 1 ? cx + 1 : d0 - 1;
-    } else if (cx == 0) {
-      xa0 = 0;
-      xa1 = 1;
-      xb0 = xb1 = d0 - 1;
-      two = true;
-    } else if (cx == d0 - 1) {
-      xa0 = d0 - 2;
-      xa1 = d0 - 1;
-      xb0 = xb1 = 0;
-      two = true;
-    } else {
-      xa0 = cx - 1;
-      xa1 = cx + 1;
-    }
-    for (int dz = -1; dz <= 1; ++dz) {
+    while (r < 18) {
+      const int row = r >> 1, part = r & 1;
+      ++r;
+      if (part == 1 && !two) continue;
+      const int dz = row / 3 - 1, dy = row % 3 - 1;
       int z = cz + dz;
       if (S.periodic[2]) {
         if (d2 == 1 && dz != 0) continue;
         z = (z + d2) % d2;
       } else if (z < 0 || z >= d2) continue;
-      for (int dy = -1; dy <= 1; ++dy) {
-        int y = cy + dy;
-        if (S.periodic[1]) {
-          if (d1 == 1 && dy != 0) continue;
-          y = (y + d1) % d1;
-        } else if (y < 0 || y >= d1) continue;
-        const int row = S.cell_off + d0 * (y + d1 * z);
-        for (int part = 0; part < (two ? 2 : 1); ++part) {
-          int q = cstart[row + (part ? xb0 : xa0)];
-          const int q1 = cstart[row + (part ? xb1 : xa1) + 1];
-          if (q <= p) q = p + 1;
-          for (; q < q1; ++q)
-            chunk_pair(B, S, SC, rod_nchunk, p, q, a, a_static, me, spanf, ispanf, extra, delta, out, cap, counter,
-                       ncand);
-        }
-      }
+      int y = cy + dy;
+      if (S.periodic[1]) {
+        if (d1 == 1 && dy != 0) continue;
+        y = (y + d1) % d1;
+      } else if (y < 0 || y >= d1) continue;
+      const int base = S.cell_off + d0 * (y + d1 * z);
+      q = cstart[base + (part ? xb0 : xa0)];
+      q1 = cstart[base + (part ? xb1 : xa1) + 1];
+      if (q <= p) q = p + 1;
+      if (q < q1) return true;
     }
+    return false;
+  }
+};
+
+constexpr int BN_THREADS = 256;
+constexpr int BN_QUEUE = 128;  // per wavefront: < 64 pending + one append round of <= 64
+
+// One thread per sorted chunk p.  Every unordered chunk pair (p, q) is
+// examined once, by the thread with the lower sorted position.  Lanes run the
+// cheap FP32 prefilter (chunk-centre minimum image + rounding margin, never
+// farther than the rod-centre image the narrow phase uses, so it never rejects
+// a touching pair) and append survivors to a per-wavefront LDS queue; each
+// time 64 are queued, all 64 lanes run the FP64 narrow phase on one entry
+// each, so the expensive part runs without divergence.
+__global__ void __launch_bounds__(BN_THREADS) k_broad_narrow(BodyPtrs B, const SysDev* __restrict__ sys,
+                                                             SortedChunks SC, const int* __restrict__ cstart,
+                                                             const int* __restrict__ rod_nchunk, int M, double delta,
+                                                             uint64_t* __restrict__ out, unsigned cap,
+                                                             unsigned* __restrict__ counter,
+                                                             unsigned long long* __restrict__ cand_counter) {
+  __shared__ int2 queue[BN_THREADS / 64][BN_QUEUE];
+  int2* Q = queue[threadIdx.x >> 6];
+  const int lane = __lane_id();
+  const int p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  unsigned long long ncand = 0;
+  StencilWalk W;
+  W.p = p;
+  W.r = 0;
+  bool active = p < M;
+  int a = 0;
+  float4 me = make_float4(0.f, 0.f, 0.f, 0.f);
+  bool a_static = false;
+  float spanf[3] = {0.f, 0.f, 0.f}, ispanf[3] = {0.f, 0.f, 0.f}, extra = 0.f;
+  int per[3] = {0, 0, 0};
+  if (active) {
+    a = SC.rod[p];
+    me = SC.cf[p];
+    const SysDev& S = sys[B.sys[a]];
+    a_static = !(B.inv_mass[a] > 0.0);
+    for (int k = 0; k < 3; ++k) {
+      spanf[k] = (float)S.span[k];
+      ispanf[k] = (float)(1.0 / S.span[k]);
+      per[k] = S.periodic[k];
+    }
+    extra = (float)delta + S.fmargin;
+    const int d0 = S.dims[0], d1 = S.dims[1];
+    const int cell = (int)SC.key[p] - S.cell_off;
+    W.cx = cell % d0;
+    W.cy = (cell / d0) % d1;
+    W.cz = cell / (d0 * d1);
+    W.two = false;
+    W.xb0 = W.xb1 = 0;
+    if (d0 == 1) {
+      W.xa0 = W.xa1 = 0;
+    } else if (!S.periodic[0]) {
+      W.xa0 = W.cx > 0 ? W.cx - 1 : 0;
+      W.xa1 = W.cx < d0 - 1 ? W.cx + 1 : d0 - 1;
+    } else if (W.cx == 0) {
+      W.xa0 = 0;
+      W.xa1 = 1;
+      W.xb0 = W.xb1 = d0 - 1;
+      W.two = true;
+    } else if (W.cx == d0 - 1) {
+      W.xa0 = d0 - 2;
+      W.xa1 = d0 - 1;
+      W.xb0 = W.xb1 = 0;
+      W.two = true;
+    } else {
+      W.xa0 = W.cx - 1;
+      W.xa1 = W.cx + 1;
+    }
+    active = W.open_next(S, cstart);
+  }
+  int pending = 0;  // wave-uniform queue length
+  while (__any(active)) {
+    bool hit = false;
+    int qh = 0;
+    if (active) {
+      const int q = W.q;
+      const float4 o = SC.cf[q];
+      float dxf = o.x - me.x, dyf = o.y - me.y, dzf = o.z - me.z;
+      if (per[0]) dxf -= spanf[0] * rintf(dxf * ispanf[0]);
+      if (per[1]) dyf -= spanf[1] * rintf(dyf * ispanf[1]);
+      if (per[2]) dzf -= spanf[2] * rintf(dzf * ispanf[2]);
+      const float thr = me.w + o.w + extra;
+      if (dxf * dxf + dyf * dyf + dzf * dzf < thr * thr) {
+        const int b = SC.rod[q];
+        hit = b != a && !(a_static && !(B.inv_mass[b] > 0.0));
+        qh = q;
+      }
+      W.q = q + 1;
+      if (W.q >= W.q1) active = W.open_next(sys[B.sys[a]], cstart);
+    }
+    const unsigned long long m = __ballot(hit);
+    const int nhit = __popcll(m);
+    if (hit) {
+      ++ncand;
+      Q[pending + __popcll(m & ((1ull << lane) - 1ull))] = make_int2(p, qh);
+    }
+    pending += nhit;
+    if (pending >= 64) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const int2 e = Q[pending - 64 + lane];
+      narrow_pair(B, sys, SC, rod_nchunk, e.x, e.y, delta, out, cap, counter);
+      pending -= 64;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (lane < pending) {
+    const int2 e = Q[lane];
+    narrow_pair(B, sys, SC, rod_nchunk, e.x, e.y, delta, out, cap, counter);
   }
   // wave-level reduction of the candidate count
   for (int off = 32; off > 0; off >>= 1) ncand += __shfl_down(ncand, off);
-  if (__lane_id() == 0 && ncand) atomicAdd(cand_counter, ncand);
+  if (lane == 0 && ncand) atomicAdd(cand_counter, ncand);
 }
 
 struct ContactPtrs {

@@ -22,7 +22,7 @@ def compress_nsc(ctx, sc, side_target, factor=0.998, steps_per=2, sweeps=30, log
     ctx.set_physics(L.physics((0.0, 0.0, 0.0), dt, 1.0 / dt, 1.0 / dt, 1e30))  # v, w -> 0 after each step
     p = L.nsc_params(beta=0.8, cfm=1e-4, omega=1.0, tol=0.0, max_sweeps=sweeps, check_every=sweeps, friction_proj=1,
                      warm_mode=1, stop_rule=1)
-    det = L.detect_params(0.0, sc.detect.chunk_len, sc.detect.cell_scale)
+    det = L.detect_params(0.0, sc.detect.chunk_len if sc.detect.chunk_len > 0 else 2.0, sc.detect.cell_scale)
     steps = 0
     k = 0
     while side > side_target * (1.0 + 1e-12):

@@ -101,3 +101,65 @@ def test_detect_capacity_regrow(gpu_ctx_factory):
     no = W.detect(0, 0.05)
     assert nc == no and nc > 16
     compare(ctx.contacts(nc), W.contacts(no), nc)
+
+
+def test_multi_system_detection_equals_per_system_oracle(gpu_ctx_factory):
+    """Several independent boxes in one context (the batched workload's
+    layout): the contact list is the concatenation of each system's own list,
+    with body indices offset by the system's first body."""
+    rng = np.random.default_rng(12)
+    parts, descs, offs = [], [], [0]
+    for k, (n, side, per, hl) in enumerate([(700, 18.0, (1, 1, 1), 2.0), (1500, 25.0, (1, 1, 1), 4.0),
+                                            (400, 12.0, (1, 1, 0), 1.0), (900, 30.0, (1, 1, 1), 8.0)]):
+        x, q, r, h = rods(n, side, 100 + k, hlo=0.5, hhi=hl)
+        x += 7.0 * k  # boxes at different origins
+        parts.append((x, q, r, h, side, per, 7.0 * k))
+        descs.append(L.system_desc([7.0 * k] * 3, [7.0 * k + side] * 3, per, 0, 0, 0.5))
+        offs.append(offs[-1] + n)
+    X = np.concatenate([p[0] for p in parts])
+    Q = np.concatenate([p[1] for p in parts])
+    R = np.concatenate([p[2] for p in parts])
+    H = np.concatenate([p[3] for p in parts])
+    n = len(R)
+    ctx = gpu_ctx_factory()
+    ctx.set_bodies(X, Q, np.zeros((n, 3)), np.zeros((n, 3)), R, H, np.ones(n), np.zeros(n))
+    ctx.set_systems(offs, descs)
+    ctx.set_physics(L.physics())
+    for chunk_len in (0.0, 2.0):
+        nc = ctx.detect(L.detect_params(0.0, chunk_len, 1.0))
+        gc = ctx.contacts(nc)
+        expect = {k: [] for k in ("a", "b", "pa", "pb", "n", "dist", "shift", "s", "t", "type")}
+        for k, (x, q, r, h, side, per, o) in enumerate(parts):
+            W = O.World(x, q, None, None, r, h, np.ones(len(r)))
+            W.set_box([o] * 3, [o + side] * 3, per)
+            no = W.detect(0, 0.0)
+            oc = W.contacts(no)
+            for key in expect:
+                v = oc[key] + offs[k] if key in ("a", "b") else oc[key]
+                expect[key].append(v)
+        expect = {k: np.concatenate(v) for k, v in expect.items()}
+        assert nc == len(expect["a"]) and nc > 200
+        compare(gc, expect, nc)
+
+
+def test_affine_scale_then_detect(gpu_ctx_factory):
+    """rdk_affine_scale maps x -> lo + s (x - lo) and the box likewise; the
+    next detection (grid resized) equals the oracle's on the scaled state."""
+    n, side = 2500, 30.0
+    x, q, r, h = rods(n, side, 31)
+    lo = np.array([-3.0, 1.0, 2.0])
+    x = x + lo
+    ctx = gpu_ctx_factory()
+    load(ctx, x, q, r, h, np.ones(n), lo, lo + side, (1, 1, 1))
+    dp = L.detect_params(0.0, 2.0, 1.0)
+    ctx.detect(dp)  # grid sized for the first box
+    s = 0.75  # exact in binary: the box spans stay exact on both sides
+    ctx.affine_scale(s)
+    st = ctx.download_state()
+    assert np.array_equal(st["x"], lo + s * (x - lo))
+    nc = ctx.detect(dp)
+    W = O.World(lo + s * (x - lo), q, None, None, r, h, np.ones(n))
+    W.set_box(lo, lo + s * side, (1, 1, 1))
+    no = W.detect(0, 0.0)
+    assert nc == no and nc > 200
+    compare(ctx.contacts(nc), W.contacts(no), nc)

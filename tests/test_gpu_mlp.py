@@ -127,7 +127,8 @@ def test_learned_start_keeps_cached_impulses_and_predicts_the_rest(gpu_ctx_facto
     B = overlapping_bed(n=2000, seed=62, mu=0.4)
     ctx = gpu_ctx_factory()
     prm = _rand_params(5, 0.1)
-    prm[M.OFF_B3:] = [2.0, 0.5, -0.5]  # predict a load on every uncached manifold
+    prm[M.OFF_W3:M.OFF_B3] = 0.0
+    prm[M.OFF_B3:] = [2.0, 0.5, -0.5]  # output = b3: a load of 2 lambda_s on every uncached manifold
     ctx.mlp_set_params(prm)
     p1 = L.nsc_params(beta=0.0, cfm=0.0, omega=1.3, tol=1e-8, max_sweeps=20000, check_every=4, friction_proj=1,
                       warm_mode=1, warm_walls=1)
@@ -145,7 +146,8 @@ def test_learned_start_keeps_cached_impulses_and_predicts_the_rest(gpu_ctx_facto
     kept = lam_prev[:, 0] > 0
     assert kept.sum() > 100 and (~kept).sum() > 10
     assert np.array_equal(lam_learned[kept], lam_prev[kept])
-    assert np.all(lam_learned[~kept, 0] > 0.0) and np.all(lam_prev[~kept, 0] == 0.0)
+    assert np.all(lam_prev[~kept, 0] == 0.0)
+    assert np.allclose(lam_learned[~kept, 0], 2.0 * ls[~kept], rtol=1e-6)
 
 
 def test_solve_from_learned_start_reaches_residual(gpu_ctx_factory):
