@@ -42,6 +42,7 @@ enum KernelId {
   K_ISL_SMALL,
   K_ISL_MEDIUM,
   K_ISL_LARGE,
+  K_ISL_HUGE,
   K_MISC,
   K_NUM
 };
@@ -157,6 +158,7 @@ struct rdk_ctx {
   rdk::DevBuf<int> uf_parent, uf_label, sbody, sbody_alt, iflag, ipos, istart, ilists, bseg, icnt;
   rdk::DevBuf<uint32_t> blab, blab_alt;
   rdk::DevBuf<char> istats;
+  rdk::DevBuf<int2> islot;  // local body slots of large-island manifolds
   rdk::DevBuf<char> small_out;
   // body -> manifold CSR
   rdk::DevBuf<uint32_t> csr_key, csr_key_alt;

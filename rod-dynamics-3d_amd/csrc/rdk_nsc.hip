@@ -938,10 +938,11 @@ __device__ __forceinline__ int lb_u32(const uint32_t* a, int lo, int hi, uint32_
 
 struct IslandClass {
   int max_m[3], max_b[3];  // register classes: manifolds and bodies per island
-  int large_max;
+  int large_max, lds_bmax;  // workgroup classes: manifolds, and bodies that fit in LDS
 };
 
-// island list per class; counts in cnt[0..4] (tiny, small, medium, large, giant)
+// island list per class; counts in cnt[0..5] (tiny, small, medium, large with
+// bodies in LDS, large on global memory, giant)
 __global__ void k_island_classify(const int* __restrict__ istart, int nisl, const uint32_t* __restrict__ skey,
                                   const uint32_t* __restrict__ blab, int nbody, IslandClass C,
                                   int* __restrict__ lists, int* __restrict__ cnt, int* __restrict__ bseg) {
@@ -953,11 +954,11 @@ __global__ void k_island_classify(const int* __restrict__ istart, int nisl, cons
   bseg[2 * k] = bs;
   bseg[2 * k + 1] = be;
   const int size = e - s, nb = be - bs;
-  int cls = 4;
+  int cls = 5;
   if (size <= C.max_m[0] && nb <= C.max_b[0]) cls = 0;
   else if (size <= C.max_m[1] && nb <= C.max_b[1]) cls = 1;
   else if (size <= C.max_m[2] && nb <= C.max_b[2]) cls = 2;
-  else if (size <= C.large_max) cls = 3;
+  else if (size <= C.large_max) cls = nb <= C.lds_bmax ? 3 : 4;
   const int slot = atomicAdd(&cnt[cls], 1);
   lists[cls * nisl + slot] = k;
 }
@@ -1147,6 +1148,103 @@ __global__ void __launch_bounds__(WG) k_island_reg(BodyPtrs B, MfPtrs M, const S
 constexpr int ISL_G0 = 8, ISL_WG0 = 64;     // tiny: 8 islands per wavefront
 constexpr int ISL_G1 = 64, ISL_WG1 = 64;    // small: one island per wavefront
 constexpr int ISL_G2 = 256, ISL_WG2 = 256;  // medium: one island per 4-wave workgroup
+
+// Large islands whose bodies fit in LDS: 512 lanes, body velocities in LDS,
+// manifold fields and impulses in global memory (L2-resident), each
+// manifold's local body slots looked up once per solve.
+constexpr int LDS_THREADS = 512;
+constexpr int LDS_BCAP = 2560;  // 6 doubles + 1 id per body: 133 KB of the 160 KB LDS
+
+struct MfSlots {
+  MfPtrs M;
+  const int2* slot;  // local (LDS) body slots per manifold, b = -1 for a wall
+  __device__ __forceinline__ double& at(int field, int i) const { return M.at(field, i); }
+  __device__ __forceinline__ d3 get3(int field, int i) const { return M.get3(field, i); }
+  __device__ __forceinline__ int ia(int i) const { return slot[i].x; }
+  __device__ __forceinline__ int ib(int i) const { return slot[i].y; }
+  __device__ __forceinline__ double& lr(int c, int i) const { return M.lam[c][i]; }
+};
+
+__global__ void __launch_bounds__(LDS_THREADS) k_island_lds(BodyPtrs B, MfPtrs M, const SysDev* __restrict__ sys,
+                                                           const int* __restrict__ istart,
+                                                           const uint32_t* __restrict__ skey,
+                                                           const int* __restrict__ bseg, const int* __restrict__ sbody,
+                                                           const int* __restrict__ list, IslandSolve P,
+                                                           IslandStats* S, int2* __restrict__ slot) {
+  __shared__ double v_l[3][LDS_BCAP], w_l[3][LDS_BCAP];
+  __shared__ int body_l[LDS_BCAP];
+  __shared__ double red[LDS_THREADS / 64];
+  __shared__ int cs[129];
+  const int k = list[blockIdx.x];
+  const int s = istart[k], e = istart[k + 1];
+  const int bs = bseg[2 * k], nb = bseg[2 * k + 1] - bs;
+  const int tid = threadIdx.x;
+  for (int q = tid; q < nb; q += LDS_THREADS) {
+    const int g = sbody[bs + q];
+    body_l[q] = g;
+    for (int c = 0; c < 3; ++c) {
+      v_l[c][q] = B.v[c][g];
+      w_l[c][q] = B.w[c][g];
+    }
+  }
+  const uint32_t base = skey[s] & ~127u;
+  const int ncol = (int)(skey[e - 1] & 127u) + 1;
+  for (int c = tid; c <= ncol; c += LDS_THREADS) cs[c] = lb_u32(skey, s, e, base | (uint32_t)c);
+  __syncthreads();
+  for (int i = s + tid; i < e; i += LDS_THREADS) {
+    const int gb = M.b[i];
+    slot[i] = make_int2(lb_int(body_l, nb, M.a[i]), gb >= 0 ? lb_int(body_l, nb, gb) : -1);
+  }
+  __syncthreads();
+  BodyPtrs LB = B;
+  for (int c = 0; c < 3; ++c) {
+    LB.v[c] = v_l[c];
+    LB.w[c] = w_l[c];
+  }
+  MfSlots A{M, slot};
+  const double mu = sys[B.sys[M.a[s]]].mu;
+  if (P.warm) {
+    for (int c = 0; c < ncol; ++c) {
+      for (int i = cs[c] + tid; i < cs[c + 1]; i += LDS_THREADS) warm_apply_one(LB, A, i);
+      __syncthreads();
+    }
+  }
+  int sweeps = 0;
+  double res = 0.0, res_ref = 0.0;
+  StallTrack trk;
+  trk.init(P.stall);
+  if (P.stop_rule == 1) {
+    double r = 0.0;
+    for (int i = s + tid; i < e; i += LDS_THREADS) r = dmax(r, residual_one(LB, A, i, P.K, mu));
+    res = block_max(r, red);
+  }
+  const int kk = P.stop_rule == 1 ? P.check_every : 1;
+  while (sweeps < P.max_sweeps && !(P.stop_rule == 1 && res <= P.tol)) {
+    double d = 0.0;
+    for (int it = 0; it < kk; ++it) {
+      d = 0.0;
+      for (int c = 0; c < ncol; ++c) {
+        for (int i = cs[c] + tid; i < cs[c + 1]; i += LDS_THREADS) d = dmax(d, psor_update(LB, A, i, P.K, mu));
+        __syncthreads();
+      }
+    }
+    sweeps += kk;
+    res_ref = block_max(d, red);
+    double r = 0.0;
+    for (int i = s + tid; i < e; i += LDS_THREADS) r = dmax(r, residual_one(LB, A, i, P.K, mu));
+    res = block_max(r, red);
+    if (P.stop_rule == 0 && res_ref < 1e-6) break;
+    if (P.stop_rule == 1 && !(res <= P.tol) && trk.stalled(sweeps, res)) break;
+  }
+  for (int q = tid; q < nb; q += LDS_THREADS) {
+    const int g = body_l[q];
+    for (int c = 0; c < 3; ++c) {
+      B.v[c][g] = v_l[c][q];
+      B.w[c][g] = w_l[c][q];
+    }
+  }
+  if (tid == 0) island_report(S, sweeps, e - s, res, res_ref, P.stop_rule == 1 && !(res <= P.tol));
+}
 
 // large islands: one 256-lane workgroup on global memory
 __global__ void __launch_bounds__(LARGE_THREADS) k_island_large(BodyPtrs B, MfPtrs M, const SysDev* __restrict__ sys,
@@ -1576,7 +1674,7 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
   RDK_TRY(ensure(ctx, ctx->iflag, nm + 1));
   RDK_TRY(ensure(ctx, ctx->ipos, nm + 1));
   RDK_TRY(ensure(ctx, ctx->istart, nm + 2));
-  RDK_TRY(ensure(ctx, ctx->ilists, 5 * (size_t)nm + 5));
+  RDK_TRY(ensure(ctx, ctx->ilists, 6 * (size_t)nm + 6));
   RDK_TRY(ensure(ctx, ctx->bseg, 2 * (size_t)nm + 2));
   RDK_TRY(ensure(ctx, ctx->icnt, 8));
   RDK_TRY(ensure(ctx, ctx->istats, sizeof(IslandStats)));
@@ -1634,16 +1732,17 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
   C.max_b[1] = ISL_G1 + 8;
   C.max_b[2] = ISL_G2 + 8;
   C.large_max = 8192;
+  C.lds_bmax = LDS_BCAP;
   {
     ProfScope ps(ctx, K_ISLAND);
     hipLaunchKernelGGL(k_island_classify, dim3(grid_for(nisl, nb)), dim3(nb), 0, ctx->stream, ctx->istart.p, nisl,
                        skey, blab, n, C, ctx->ilists.p, ctx->icnt.p, ctx->bseg.p);
   }
-  int cnt[5];
-  RDK_HIP(hipMemcpyAsync(cnt, ctx->icnt.p, 5 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  int cnt[6];
+  RDK_HIP(hipMemcpyAsync(cnt, ctx->icnt.p, 6 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   RDK_HIP(hipStreamSynchronize(ctx->stream));
   ctx->nislands = nisl;
-  if (cnt[4] > 0) return RDK_OK;  // a giant island: global coloured path
+  if (cnt[5] > 0) return RDK_OK;  // a giant island: global coloured path
   IslandSolve P;
   P.K = solve_const(ctx, p);
   P.tol = p->tol;
@@ -1657,7 +1756,8 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
   BodyPtrs B = body_ptrs(ctx);
   MfPtrs Mc = mf_ptrs(ctx, true);
   // tiny islands run on the side stream, concurrently with the other classes
-  const bool side = cnt[0] > 0 && (cnt[1] > 0 || cnt[2] > 0 || cnt[3] > 0) && ctx->stream2 && !ctx->profiling;
+  const bool side = cnt[0] > 0 && (cnt[1] > 0 || cnt[2] > 0 || cnt[3] > 0 || cnt[4] > 0) && ctx->stream2 &&
+                    !ctx->profiling;
   if (cnt[0] > 0) {
     hipStream_t s_tiny = ctx->stream;
     if (side) {
@@ -1683,9 +1783,15 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
                        S);
   }
   if (cnt[3] > 0) {
+    RDK_TRY(ensure(ctx, ctx->islot, nm));
     ProfScope ps(ctx, K_ISL_LARGE);
-    hipLaunchKernelGGL(k_island_large, dim3(cnt[3]), dim3(LARGE_THREADS), 0, ctx->stream, B, Mc, ctx->sys_dev.p,
-                       ctx->istart.p, skey, ctx->ilists.p + 3 * nisl, P, S);
+    hipLaunchKernelGGL(k_island_lds, dim3(cnt[3]), dim3(LDS_THREADS), 0, ctx->stream, B, Mc, ctx->sys_dev.p,
+                       ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p + 3 * nisl, P, S, ctx->islot.p);
+  }
+  if (cnt[4] > 0) {
+    ProfScope ps(ctx, K_ISL_HUGE);
+    hipLaunchKernelGGL(k_island_large, dim3(cnt[4]), dim3(LARGE_THREADS), 0, ctx->stream, B, Mc, ctx->sys_dev.p,
+                       ctx->istart.p, skey, ctx->ilists.p + 4 * nisl, P, S);
   }
   if (side) RDK_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
   {
@@ -1704,7 +1810,8 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
   }
   ctx->ncolor = (int)hc[1] + 1;
   ctx->have_coloring = false;  // the solve-ordered copy is island-ordered
-  for (int c = 0; c < 4; ++c) ctx->island_counts[c] = cnt[c];
+  for (int c = 0; c < 3; ++c) ctx->island_counts[c] = cnt[c];
+  ctx->island_counts[3] = cnt[3] + cnt[4];
   ctx->work[W_SWEEP_CONTACTS] += (int64_t)hs.manifold_sweeps;
   *status = hs.nonconv > 0 ? RDK_ERR_NOCONV : RDK_OK;
   if (st) {

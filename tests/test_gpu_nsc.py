@@ -195,9 +195,27 @@ def clusters(nclust=40, seed=77, mu=0.5):
                 periodic=(1, 1, 0), walls=True, mu=mu)
 
 
+def big_clusters(nclust=3, k=160, seed=91, mu=0.5):
+    """A few dense clusters of many rods: islands of several hundred manifolds."""
+    rng = np.random.default_rng(seed)
+    xs, qs = [], []
+    for c in range(nclust):
+        cx = 30.0 * c + 15.0
+        xs.append(np.stack([rng.uniform(cx - 4, cx + 4, k), rng.uniform(11, 19, k), rng.uniform(0.2, 4.0, k)], 1))
+        q = rng.normal(size=(k, 4))
+        qs.append(q / np.linalg.norm(q, axis=1, keepdims=True))
+    x, q = np.concatenate(xs), np.concatenate(qs)
+    n = len(x)
+    return dict(x=x, q=q, v=rng.normal(size=(n, 3)), w=rng.normal(size=(n, 3)), r=np.full(n, 0.4),
+                h=rng.uniform(1.0, 3.0, n), mass=rng.uniform(0.5, 2.0, n), rest=np.full(n, 0.0),
+                side=30.0 * nclust, periodic=(1, 1, 0), walls=True, mu=mu)
+
+
 @pytest.mark.parametrize("proj", [0, 1])
-def test_island_sweeps_bitwise_equal_serial_sweeps_in_solve_order(gpu_ctx_factory, proj):
-    B = clusters()
+@pytest.mark.parametrize("scene", ["clusters", "big_clusters"])
+def test_island_sweeps_bitwise_equal_serial_sweeps_in_solve_order(gpu_ctx_factory, proj, scene):
+    B = clusters() if scene == "clusters" else big_clusters()
+    print(f"island sweeps {scene} proj={proj}", flush=True)
     ctx = gpu_ctx_factory()
     gpu_load(ctx, B)
     p = L.nsc_params(beta=0.1, cfm=1e-4, omega=1.3, tol=0.0, max_sweeps=9, check_every=1, friction_proj=proj,
@@ -208,7 +226,10 @@ def test_island_sweeps_bitwise_equal_serial_sweeps_in_solve_order(gpu_ctx_factor
     s0 = ctx.download_state()
     st = ctx.nsc_solve(p, allow_noconv=True)
     nisl, counts = ctx.island_info()
-    assert counts[0] > 0 and counts[1] > 0 and counts[2] > 0, counts  # tiny, small and medium islands present
+    if scene == "clusters":
+        assert counts[0] > 0 and counts[1] > 0 and counts[2] > 0, counts  # tiny, small and medium islands present
+    else:
+        assert counts[3] > 0, counts  # large islands (bodies in LDS)
     order = ctx.solve_order(nm)
     _, _, lam_g = ctx.nsc_manifolds(nm)
     s1 = ctx.download_state()
