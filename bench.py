@@ -365,6 +365,8 @@ def run_vv(a, L, scenes, which):
         cfg.update(prep)
     out = base_line("rod-steps/s", sc.n * a.steps / el, a, el, "f64", cfg)
     out["contacts_per_evaluation"] = st["contacts_total"] / (2 * max(st["steps"], 1))
+    wc = ctx.work_counters()
+    out["narrow_phase_candidates_per_evaluation"] = wc["candidates"] / max(wc["body_passes"], 1)
     out["roofline"] = roofline(ctx, dom, sc.n)
     state = ctx.download_state()
     ctx.close()
