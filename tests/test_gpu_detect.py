@@ -163,3 +163,22 @@ def test_affine_scale_then_detect(gpu_ctx_factory):
     no = W.detect(0, 0.0)
     assert nc == no and nc > 200
     compare(ctx.contacts(nc), W.contacts(no), nc)
+
+
+def test_reference_broadphase_benchmark_geometry(gpu_ctx_factory):
+    """The reference's broad-phase benchmark geometry (benchmark_broadphase.cpp)
+    at N = 512 and 4000: the GPU list equals the oracle's bit for bit, and the
+    counts are the ones the reference publishes (1730, 13190)."""
+    from bench_geometry import bench_rods
+
+    for n, ref in ((512, 1730), (4000, 13190)):
+        x, q, r, h, lo, hi = bench_rods(n)
+        ctx = gpu_ctx_factory()
+        load(ctx, x, q, r, h, np.ones(n), [lo] * 3, [hi] * 3, (1, 1, 1))
+        for chunk_len in (0.0, 0.2):
+            nc = ctx.detect(L.detect_params(0.005, chunk_len, 1.0))
+            assert nc == ref
+            W = O.World(x, q, None, None, r, h, np.ones(n))
+            W.set_box([lo] * 3, [hi] * 3, (1, 1, 1))
+            no = W.detect(0, 0.005)
+            compare(ctx.contacts(nc), W.contacts(no), nc)
