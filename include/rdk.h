@@ -73,7 +73,9 @@ typedef struct rdk_nsc_params {
   int stall_sweeps;        /* stop_rule 1 only; 0 = off.  > 0: at sweeps = k, 2k,
                               4k, ... an island (or the global solve) whose residual
                               has not dropped by 10 % since the previous milestone
-                              stops there and counts as not converged */
+                              continues with omega = 1 if omega > 1 (over-relaxed
+                              PSOR can cycle), else stops there and counts as not
+                              converged */
 } rdk_nsc_params;
 
 typedef struct rdk_soft_params { /* SoftContactCfg, config.hpp:6-27 */

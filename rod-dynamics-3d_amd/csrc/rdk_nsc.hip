@@ -441,11 +441,13 @@ struct SolveConst {
 
 // Stall rule (stop_rule 1, stall_sweeps > 0): at sweep counts stall, 2 stall,
 // 4 stall, ... the residual must have dropped by at least 10 % since the
-// previous such milestone, otherwise the solve stops there (reported as not
-// converged).  Wedged, statically indeterminate groups of rods let the
-// impulses drift along a null direction of the Delassus matrix while the
-// residual sits on a floor or creeps up; this stops them without touching
-// solves that still make progress, however slowly.
+// previous such milestone.  The first time it has not, an over-relaxed solve
+// (omega > 1) continues with omega = 1: with high friction and restitution,
+// over-relaxed PSOR can settle on a limit cycle that plain Gauss-Seidel does
+// not have.  The next stall stops the solve (reported as not converged):
+// wedged, statically indeterminate groups of rods let the impulses drift along
+// a null direction of the Delassus matrix while the residual sits on a floor,
+// at any omega.  Solves that keep making progress are untouched.
 struct StallTrack {
   int next;
   double ref;
@@ -839,7 +841,10 @@ __global__ void __launch_bounds__(SMALL_BLOCK) k_psor_small(BodyPtrs B, MfPtrs M
     for (int i = threadIdx.x; i < nm; i += blockDim.x) r = dmax(r, residual_one(B, M, i, K, sys[B.sys[M.a[i]]].mu));
     res = block_max(r, red);
     if (stop_rule == 0 && res_ref < 1e-6) break;
-    if (stop_rule == 1 && !(res <= tol) && trk.stalled(sweeps, res)) break;
+    if (stop_rule == 1 && !(res <= tol) && trk.stalled(sweeps, res)) {
+      if (K.omega > 1.0) K.omega = 1.0;
+      else break;
+    }
   }
   if (threadIdx.x == 0) {
     out->sweeps = sweeps;
@@ -1106,8 +1111,9 @@ __global__ void __launch_bounds__(WG) k_island_reg(BodyPtrs B, MfPtrs M, const S
   bool done = !has || P.max_sweeps <= 0;
   StallTrack trk;
   trk.init(P.stall);
+  SolveConst K = P.K;  // omega may drop to 1 on a stall (per island)
   if (P.stop_rule == 1) {
-    res = group_max<GROUP>(live ? residual_one(LB, R, 0, P.K, mu) : 0.0, red);
+    res = group_max<GROUP>(live ? residual_one(LB, R, 0, K, mu) : 0.0, red);
     done = done || res <= P.tol;
   }
   const int kk = P.stop_rule == 1 ? P.check_every : 1;
@@ -1116,12 +1122,12 @@ __global__ void __launch_bounds__(WG) k_island_reg(BodyPtrs B, MfPtrs M, const S
     double d = 0.0;
     for (int it = 0; it < kk; ++it) {
       for (int c = 0; c < ncol; ++c) {
-        if (act && col == c) d = psor_update(LB, R, 0, P.K, mu);
+        if (act && col == c) d = psor_update(LB, R, 0, K, mu);
         __syncthreads();
       }
     }
     const double rr = group_max<GROUP>(act ? d : 0.0, red);
-    const double rs = group_max<GROUP>(act ? residual_one(LB, R, 0, P.K, mu) : 0.0, red);
+    const double rs = group_max<GROUP>(act ? residual_one(LB, R, 0, K, mu) : 0.0, red);
     if (!done) {
       sweeps += kk;
       res_ref = rr;
@@ -1129,7 +1135,10 @@ __global__ void __launch_bounds__(WG) k_island_reg(BodyPtrs B, MfPtrs M, const S
       if (P.stop_rule == 0 && res_ref < 1e-6) done = true;
       if (P.stop_rule == 1 && res <= P.tol) done = true;
       if (sweeps >= P.max_sweeps) done = true;
-      if (!done && P.stop_rule == 1 && trk.stalled(sweeps, res)) done = true;
+      if (!done && P.stop_rule == 1 && trk.stalled(sweeps, res)) {
+        if (K.omega > 1.0) K.omega = 1.0;
+        else done = true;
+      }
     }
   }
   if (live)
@@ -1213,9 +1222,10 @@ __global__ void __launch_bounds__(LDS_THREADS) k_island_lds(BodyPtrs B, MfPtrs M
   double res = 0.0, res_ref = 0.0;
   StallTrack trk;
   trk.init(P.stall);
+  SolveConst K = P.K;  // omega may drop to 1 on a stall
   if (P.stop_rule == 1) {
     double r = 0.0;
-    for (int i = s + tid; i < e; i += LDS_THREADS) r = dmax(r, residual_one(LB, A, i, P.K, mu));
+    for (int i = s + tid; i < e; i += LDS_THREADS) r = dmax(r, residual_one(LB, A, i, K, mu));
     res = block_max(r, red);
   }
   const int kk = P.stop_rule == 1 ? P.check_every : 1;
@@ -1224,17 +1234,20 @@ __global__ void __launch_bounds__(LDS_THREADS) k_island_lds(BodyPtrs B, MfPtrs M
     for (int it = 0; it < kk; ++it) {
       d = 0.0;
       for (int c = 0; c < ncol; ++c) {
-        for (int i = cs[c] + tid; i < cs[c + 1]; i += LDS_THREADS) d = dmax(d, psor_update(LB, A, i, P.K, mu));
+        for (int i = cs[c] + tid; i < cs[c + 1]; i += LDS_THREADS) d = dmax(d, psor_update(LB, A, i, K, mu));
         __syncthreads();
       }
     }
     sweeps += kk;
     res_ref = block_max(d, red);
     double r = 0.0;
-    for (int i = s + tid; i < e; i += LDS_THREADS) r = dmax(r, residual_one(LB, A, i, P.K, mu));
+    for (int i = s + tid; i < e; i += LDS_THREADS) r = dmax(r, residual_one(LB, A, i, K, mu));
     res = block_max(r, red);
     if (P.stop_rule == 0 && res_ref < 1e-6) break;
-    if (P.stop_rule == 1 && !(res <= P.tol) && trk.stalled(sweeps, res)) break;
+    if (P.stop_rule == 1 && !(res <= P.tol) && trk.stalled(sweeps, res)) {
+      if (K.omega > 1.0) K.omega = 1.0;
+      else break;
+    }
   }
   for (int q = tid; q < nb; q += LDS_THREADS) {
     const int g = body_l[q];
@@ -1272,9 +1285,10 @@ __global__ void __launch_bounds__(LARGE_THREADS) k_island_large(BodyPtrs B, MfPt
   double res = 0.0, res_ref = 0.0;
   StallTrack trk;
   trk.init(P.stall);
+  SolveConst K = P.K;  // omega may drop to 1 on a stall
   if (P.stop_rule == 1) {
     double r = 0.0;
-    for (int i = s + tid; i < e; i += LARGE_THREADS) r = dmax(r, residual_one(B, M, i, P.K, mu));
+    for (int i = s + tid; i < e; i += LARGE_THREADS) r = dmax(r, residual_one(B, M, i, K, mu));
     res = block_max(r, red);
   }
   const int kk = P.stop_rule == 1 ? P.check_every : 1;
@@ -1283,17 +1297,20 @@ __global__ void __launch_bounds__(LARGE_THREADS) k_island_large(BodyPtrs B, MfPt
     for (int it = 0; it < kk; ++it) {
       d = 0.0;
       for (int c = 0; c < ncol; ++c) {
-        for (int i = cs[c] + tid; i < cs[c + 1]; i += LARGE_THREADS) d = dmax(d, psor_update(B, M, i, P.K, mu));
+        for (int i = cs[c] + tid; i < cs[c + 1]; i += LARGE_THREADS) d = dmax(d, psor_update(B, M, i, K, mu));
         __syncthreads();
       }
     }
     sweeps += kk;
     res_ref = block_max(d, red);
     double r = 0.0;
-    for (int i = s + tid; i < e; i += LARGE_THREADS) r = dmax(r, residual_one(B, M, i, P.K, mu));
+    for (int i = s + tid; i < e; i += LARGE_THREADS) r = dmax(r, residual_one(B, M, i, K, mu));
     res = block_max(r, red);
     if (P.stop_rule == 0 && res_ref < 1e-6) break;
-    if (P.stop_rule == 1 && !(res <= P.tol) && trk.stalled(sweeps, res)) break;
+    if (P.stop_rule == 1 && !(res <= P.tol) && trk.stalled(sweeps, res)) {
+      if (K.omega > 1.0) K.omega = 1.0;
+      else break;
+    }
   }
   if (tid == 0) island_report(S, sweeps, e - s, res, res_ref, P.stop_rule == 1 && !(res <= P.tol));
 }
@@ -1920,12 +1937,16 @@ static int global_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
       const int k = std::max(1, p->check_every);
       StallTrack trk;
       trk.init(p->stall_sweeps);
+      rdk_nsc_params q = *p;  // omega may drop to 1 on a stall
       RDK_TRY(nsc_residual_run(ctx, p, &res, &res_ref));
       while (!(res <= p->tol) && sweeps < p->max_sweeps) {
-        RDK_TRY(run_sweeps(ctx, p, k));
+        RDK_TRY(run_sweeps(ctx, &q, k));
         sweeps += k;
         RDK_TRY(nsc_residual_run(ctx, p, &res, &res_ref));
-        if (!(res <= p->tol) && trk.stalled(sweeps, res)) break;
+        if (!(res <= p->tol) && trk.stalled(sweeps, res)) {
+          if (q.omega > 1.0) q.omega = 1.0;
+          else break;
+        }
       }
       if (!(res <= p->tol)) status = RDK_ERR_NOCONV;
     }

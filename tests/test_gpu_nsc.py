@@ -285,3 +285,38 @@ def test_stall_rule(gpu_ctx_factory):
         assert st["residual"] <= 1e-11
         lams.append(ctx.nsc_manifolds(nm)[2])
     assert np.array_equal(lams[0], lams[1])
+
+
+def test_stall_rule_omega_fallback_breaks_limit_cycle(gpu_ctx_factory):
+    """The captured limit cycle: without the stall rule the over-relaxed solve
+    ends at max_sweeps on the cycle; with it, omega drops to 1 at the first
+    stall and the solve converges (the residual is the oracle's)."""
+    import os
+
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "c4_limit_cycle.npz"))
+    out = {}
+    for stall in (0, 64):
+        ctx = gpu_ctx_factory()
+        ctx.set_bodies(d["x"], d["q"], d["v"], d["w"], d["r"], d["h"], d["mass"], d["rest"])
+        ctx.set_systems([0, len(d["r"])], [L.system_desc(d["lo"], d["hi"], (1, 1, 1), 0, 0, float(d["mu"]))])
+        ctx.set_physics(L.physics((0, 0, 0), 1e-3, 0, 0, 1e30))
+        p = L.nsc_params(beta=0.0, cfm=0.0, omega=1.3, tol=1e-8, max_sweeps=3000, check_every=4, friction_proj=1,
+                         warm_mode=1, stall_sweeps=stall)
+        assert ctx.detect(L.detect_params()) == 11
+        q0 = L.nsc_params(warm_mode=0)
+        ctx.nsc_build(q0)
+        rec, ab, _ = ctx.nsc_manifolds(11)
+        assert np.array_equal(ab, d["ab"])
+        ctx.nsc_set_lambda(d["lam0"])
+        st = ctx.nsc_solve(p, allow_noconv=True)
+        out[stall] = st
+        if stall:
+            _, _, lam = ctx.nsc_manifolds(11)
+            s1 = ctx.download_state()
+            W = O.World(d["x"], d["q"], s1["v"], s1["w"], d["r"], d["h"], d["mass"], d["rest"])
+            W.set_box(list(d["lo"]), list(d["hi"]), [1, 1, 1], 0, 0)
+            W.set_physics((0, 0, 0), 1e-3)
+            W.set_manifolds(rec, ab, lam)
+            assert W.residual(O.nsc_params(mu=float(d["mu"]), beta=0.0, cfm=0.0, omega=1.3, friction_proj=1)) <= 1e-8
+    assert out[0]["residual"] > 0.1 and out[0]["sweeps"] >= 3000, out[0]
+    assert out[64]["residual"] <= 1e-8 and out[64]["sweeps"] < 3000, out[64]

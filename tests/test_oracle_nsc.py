@@ -248,3 +248,28 @@ def test_warm_start_reduces_sweeps_on_a_resting_bed():
     W2.set_state(v=np.zeros((200, 3)), w=np.zeros((200, 3)))
     s_warm, _ = W2.nsc_step(warm, 3, method=0)
     assert s_warm < s_cold
+
+
+def test_overrelaxed_psor_limit_cycle_fixture():
+    """Captured case: with mu = 0.977 and restitution 0.5, PSOR at omega = 1.3
+    settles on a limit cycle (residual 0.242) while omega = 1 converges; the
+    stall rule's omega fallback exists for this."""
+    import os
+
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "c4_limit_cycle.npz"))
+    W = O.World(d["x"], d["q"], d["v"], d["w"], d["r"], d["h"], d["mass"], d["rest"])
+    W.set_box(list(d["lo"]), list(d["hi"]), [1, 1, 1], 0, 0)
+    W.set_physics((0, 0, 0), 1e-3)
+    assert W.detect(0, 0.0) == 11
+    assert W.nsc_build(O.nsc_params(mu=float(d["mu"]), warm_start=0)) == 11
+    rec, ab, _ = W.manifolds()
+    assert np.array_equal(ab, d["ab"])
+    res = {}
+    for om in (1.3, 1.0):
+        op = O.nsc_params(mu=float(d["mu"]), beta=0.0, cfm=0.0, omega=om, friction_proj=1)
+        W.set_state(v=d["v"], w=d["w"])
+        W.set_manifolds(rec, ab, d["lam0"])
+        W.apply_warm()
+        W.sweeps(op, 2000, None)
+        res[om] = W.residual(op)
+    assert res[1.3] > 0.1 and res[1.0] < 1e-12, res
