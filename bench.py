@@ -48,8 +48,9 @@ BYTES = {
     "vv_half2": ("bodies", 280),
 }
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
-# (scaled by a calibrated copy), for the C2 workload's dominant kernel
-TRAFFIC = {}
+# (FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0, from a calibrated 1 GiB copy)
+# for (workload, kernel); recorded with the command in profiles/
+TRAFFIC = {("c2", "broad_narrow"): 773.6e6 + 0.26e6}
 
 
 def parse():
@@ -143,7 +144,7 @@ def rank_kernels(ctx, run_warmup):
     return dom
 
 
-def roofline(ctx, dom, n):
+def roofline(ctx, dom, n, workload):
     kt = ctx.kernel_times()
     wc = ctx.work_counters()
     launches, kms = kt[dom]
@@ -153,7 +154,7 @@ def roofline(ctx, dom, n):
              "manifolds": wc["manifolds"], "contacts": wc["contacts"], "force_contacts": wc["force_contacts"]}[unit]
     achieved = (bpu * units / 1e9) / (kms / 1e3) if kms > 0 else 0.0
     return {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC.get(dom), "launches": launches,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC.get((workload, dom)), "launches": launches,
             "avg_launch_ms": kms / max(launches, 1), "bytes_per_unit": bpu, "unit_name": unit, "units": units}
 
 
@@ -248,7 +249,7 @@ def run_c2(a, L, scenes):
                    "colors_per_step": st["colors_total"] / max(st["steps"], 1),
                    "residual_max": st["residual_max"], "nonconverged_steps": st["nonconverged_steps"],
                    "warm_start": "previous step (rod-rod and plane-wall contacts)"}
-    out["roofline"] = roofline(ctx, dom, sc.n)
+    out["roofline"] = roofline(ctx, dom, sc.n, "c2")
     if a.ab_steps > 0:
         out["learned_warm_start"] = learned_ab(ctx, sc, a)
     ctx.close()
@@ -367,7 +368,7 @@ def run_vv(a, L, scenes, which):
     out["contacts_per_evaluation"] = st["contacts_total"] / (2 * max(st["steps"], 1))
     wc = ctx.work_counters()
     out["narrow_phase_candidates_per_evaluation"] = wc["candidates"] / max(wc["body_passes"], 1)
-    out["roofline"] = roofline(ctx, dom, sc.n)
+    out["roofline"] = roofline(ctx, dom, sc.n, which)
     state = ctx.download_state()
     ctx.close()
     if not a.no_cpu:
@@ -453,7 +454,7 @@ def run_c4(a, L, scenes):
     out["psor"] = {"sweeps_per_step": st["sweeps_total"] / max(st["steps"], 1),
                    "manifolds_per_step": st["manifolds_total"] / max(st["steps"], 1),
                    "residual_max": st["residual_max"], "nonconverged_steps": st["nonconverged_steps"]}
-    out["roofline"] = roofline(ctx, dom, sc.n)
+    out["roofline"] = roofline(ctx, dom, sc.n, "c4")
     state = ctx.download_state()
     ctx.close()
     if not a.no_cpu:
