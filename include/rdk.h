@@ -76,6 +76,11 @@ typedef struct rdk_nsc_params {
                               continues with omega = 1 if omega > 1 (over-relaxed
                               PSOR can cycle), else stops there and counts as not
                               converged */
+  int position_iters;      /* position projection (config.hpp:77-80), run by the step
+                              after the position update when position_stabilization
+                              is set: outer passes (reference default 5) */
+  int position_psor_iters; /* inner normal-only sweeps per pass (default 50) */
+  double slop;             /* allowed penetration before correction (default 1e-4) */
 } rdk_nsc_params;
 
 typedef struct rdk_soft_params { /* SoftContactCfg, config.hpp:6-27 */
@@ -178,6 +183,15 @@ int rdk_set_solver_mode(rdk_ctx* ctx, int mode);
  * manifolds, small <= 64, medium <= 256, large <= 8192), and the manifold
  * order the last solve swept in (natural indices) */
 int rdk_nsc_island_info(rdk_ctx* ctx, int* n_islands, int* counts4);
+/* projectPositions, multi-body branch (nsc_contact.cpp:718-880): detect at the
+ * current positions, then up to position_iters passes of normal-only position
+ * PSOR (omega, cfm, slop from p) over those pairs, each applying
+ * x += dx, q <- normalize(angleAxis(|dth|, dth/|dth|) q) and the periodic wrap.
+ * *passes: passes that found pairs deeper than slop; *n_first: their count in
+ * the first pass.  Leaves the manifold buffers holding position manifolds
+ * (rdk_nsc_get_solve_order gives the last pass's sweep order). */
+int rdk_nsc_project(rdk_ctx* ctx, const rdk_detect_params* dp, const rdk_nsc_params* p, int* passes,
+                    int* n_first);
 int rdk_nsc_get_solve_order(rdk_ctx* ctx, int cap, int* order);
 
 /* ---- integrators (integrator.cpp:56-159, main.cpp:4397-4483) ----------- */

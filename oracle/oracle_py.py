@@ -21,7 +21,8 @@ class NscParams(C.Structure):
     _fields_ = [("mu", C.c_double), ("beta", C.c_double), ("cfm", C.c_double), ("omega", C.c_double),
                 ("tol", C.c_double), ("max_sweeps", C.c_int), ("check_every", C.c_int), ("friction_proj", C.c_int),
                 ("stop_rule", C.c_int), ("position_stabilization", C.c_int), ("warm_start", C.c_int),
-                ("warm_walls", C.c_int)]
+                ("warm_walls", C.c_int), ("position_iters", C.c_int), ("position_psor_iters", C.c_int),
+                ("slop", C.c_double)]
 
 
 class SoftParams(C.Structure):
@@ -72,6 +73,8 @@ def lib():
         L.orc_nsc_residual.restype = C.c_double
         L.orc_nsc_residual.argtypes = [C.c_void_p, C.POINTER(NscParams)]
         L.orc_nsc_store_warm.argtypes = [C.c_void_p]
+        L.orc_nsc_project.argtypes = [C.c_void_p, C.POINTER(NscParams), _ip, _ip]
+        L.orc_nsc_project.restype = C.c_int
         L.orc_nsc_step.argtypes = [C.c_void_p, C.POINTER(NscParams), C.c_int, C.c_int, C.c_double, _dp, _dp]
         L.orc_integrate.argtypes = [C.c_void_p, C.c_int]
         L.orc_soft_forces.argtypes = [C.c_void_p, C.POINTER(SoftParams), _dp]
@@ -101,8 +104,10 @@ def _i(a):
 
 
 def nsc_params(mu=0.5, beta=0.0, cfm=0.0, omega=1.0, tol=1e-8, max_sweeps=10000, check_every=1, friction_proj=0,
-               stop_rule=1, position_stabilization=0, warm_start=1, warm_walls=0):
+               stop_rule=1, position_stabilization=0, warm_start=1, warm_walls=0, position_iters=5,
+               position_psor_iters=50, slop=1e-4):
     p = NscParams()
+    p.position_iters, p.position_psor_iters, p.slop = position_iters, position_psor_iters, slop
     p.warm_walls = warm_walls
     p.mu, p.beta, p.cfm, p.omega, p.tol = mu, beta, cfm, omega, tol
     p.max_sweeps, p.check_every, p.friction_proj, p.stop_rule = max_sweeps, check_every, friction_proj, stop_rule
@@ -216,6 +221,14 @@ class World:
 
     def residual(self, p):
         return self._L.orc_nsc_residual(self._h, C.byref(p))
+
+    def project(self, p, order=None):
+        """Position projection over the current contact list; returns (passes, first-pass count)."""
+        npos = C.c_int()
+        o = None if order is None else np.ascontiguousarray(order, np.int32)
+        passes = self._L.orc_nsc_project(self._h, C.byref(p), None if o is None else o.ctypes.data_as(_ip),
+                                         C.byref(npos))
+        return passes, npos.value
 
     def store_warm(self):
         self._L.orc_nsc_store_warm(self._h)

@@ -7,6 +7,7 @@
 #include "rd_oracle.h"
 
 #include <algorithm>
+#include <limits>
 #include <array>
 #include <cmath>
 #include <cstdint>
@@ -1326,10 +1327,109 @@ int orc_nsc_step(orc_world* W, const orc_nsc_params* P, int nsteps, int method, 
     if (P->warm_start) store_warm(*W);
     else W->warm.clear();
     integrate(*W, 1);
+    if (P->position_stabilization && P->position_iters > 0) {
+      detect(*W, method, 0.0, cs);
+      orc_nsc_project(W, P, nullptr, nullptr);
+    }
   }
   if (sweeps_total) *sweeps_total = st;
   if (residual_max) *residual_max = rmax;
   return 0;
+}
+
+// ---- position stabilisation (nsc_contact.cpp:659-880, multi-body branch) ----
+namespace {
+struct PosManifold {
+  int a, b;
+  V3 n, ra, rb, IAn, IBn;
+  double phi, gn, lam;
+};
+
+// narrow phase of a cached pair at the current positions (nsc_contact.cpp:772-815)
+static bool pos_manifold(const orc_world& W, int ia, int ib, double hs, double slop, double cfm, PosManifold& p) {
+  Contact c;
+  if (!capsule_capsule(W, ia, ib, std::numeric_limits<double>::infinity(), c)) return false;
+  const double phi = c.dist - hs;
+  if (!(phi < -slop)) return false;
+  const Body& A = W.B[ia];
+  const Body& Bd = W.B[ib];
+  p.a = ia;
+  p.b = ib;
+  p.n = c.n;
+  p.phi = phi + slop;
+  p.ra = sub(c.pa, A.x);
+  p.rb = sub(c.pb, add(Bd.x, c.shift));
+  const M3 IA = iworld_inv(A), IB = iworld_inv(Bd);
+  p.gn = diag_term(A.invMass, Bd.invMass, IA, IB, p.ra, p.rb, p.n, cfm, false);
+  p.IAn = mv(IA, cross(p.ra, p.n));
+  p.IBn = mv(IB, cross(p.rb, p.n));
+  p.lam = 0.0;
+  return true;
+}
+}  // namespace
+
+int orc_nsc_project(orc_world* Wp, const orc_nsc_params* P, const int* order, int* npos) {
+  orc_world& W = *Wp;
+  const int n = (int)W.B.size();
+  // cached pairs: every contact of the detection, sorted (a,b)
+  std::vector<std::array<int, 2>> pairs;
+  std::vector<double> hsum;
+  for (const Contact& c : W.contacts) {
+    pairs.push_back({c.a, c.b});
+    hsum.push_back(c.hsum);
+  }
+  std::vector<V3> dx(n), dth(n);
+  std::vector<PosManifold> pm;
+  int passes = 0;
+  if (npos) *npos = 0;
+  for (int outer = 0; outer < P->position_iters; ++outer) {
+    pm.clear();
+    for (size_t k = 0; k < pairs.size(); ++k) {
+      PosManifold p;
+      if (pos_manifold(W, pairs[k][0], pairs[k][1], hsum[k], P->slop, P->cfm, p)) pm.push_back(p);
+    }
+    if (outer == 0 && npos) *npos = (int)pm.size();
+    if (pm.empty()) break;
+    ++passes;
+    std::fill(dx.begin(), dx.end(), mk(0, 0, 0));
+    std::fill(dth.begin(), dth.end(), mk(0, 0, 0));
+    const int m = (int)pm.size();
+    for (int inner = 0; inner < P->position_psor_iters; ++inner) {
+      double pres = 0.0;
+      for (int j = 0; j < m; ++j) {
+        PosManifold& p = pm[(outer == 0 && order) ? order[j] : j];
+        const Body& A = W.B[p.a];
+        const Body& Bd = W.B[p.b];
+        const double corr =
+            dot(p.n, sub(sub(add(dx[p.b], cross(dth[p.b], p.rb)), dx[p.a]), cross(dth[p.a], p.ra)));
+        const double res = corr + p.phi + P->cfm * p.lam;
+        const double delta = -(P->omega / p.gn) * res;
+        const double old = p.lam;
+        p.lam = std::max(0.0, old + delta);
+        const double dl = p.lam - old;
+        if (dl != 0.0) {
+          dx[p.a] = sub(dx[p.a], mul(p.n, dl * A.invMass));
+          dth[p.a] = sub(dth[p.a], mul(p.IAn, dl));
+          dx[p.b] = add(dx[p.b], mul(p.n, dl * Bd.invMass));
+          dth[p.b] = add(dth[p.b], mul(p.IBn, dl));
+        }
+        pres = std::max(pres, std::fabs(dl));
+      }
+      if (pres < 1e-8) break;
+    }
+    for (int i = 0; i < n; ++i) {
+      Body& b = W.B[i];
+      if (b.invMass <= 0.0) continue;
+      b.x = add(b.x, dx[i]);
+      const double angle = length(dth[i]);
+      if (angle > 1e-8) {
+        const V3 ax = mk(dth[i].x / angle, dth[i].y / angle, dth[i].z / angle);
+        b.q = qnormalize(qmul(angle_axis(angle, ax), b.q));
+      }
+      wrap(W, b);
+    }
+  }
+  return passes;
 }
 
 void orc_integrate(orc_world* W, int stage) { integrate(*W, stage); }

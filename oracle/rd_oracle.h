@@ -44,6 +44,9 @@ typedef struct orc_nsc_params {
   int position_stabilization; /* only scales beta (x0.01) as the reference does */
   int warm_start;        /* 0 = cold, 1 = previous-step key match (reference) */
   int warm_walls;        /* 1 = plane-wall manifolds also warm-started, keyed (rod, slot) */
+  int position_iters;    /* position projection: outer passes (config.hpp:78) */
+  int position_psor_iters; /* inner normal-only sweeps per pass (config.hpp:79) */
+  double slop;           /* allowed penetration before correction (config.hpp:80) */
 } orc_nsc_params;
 
 typedef struct orc_soft_params {
@@ -107,6 +110,12 @@ int orc_nsc_step(orc_world* W, const orc_nsc_params* P, int nsteps,
  * 2 VV half kick + drift, 3 VV second half kick (+damping, clamp, clear),
  * 4 clear forces */
 void orc_integrate(orc_world* W, int stage);
+/* NscContactSolver::projectPositions, multi-body branch (nsc_contact.cpp:
+ * 718-880): the pairs are the current contact list (detect first, delta 0).
+ * order: sweep order of the first pass's position manifolds (NULL = sorted
+ * pair order, the reference's).  Returns the number of passes that found
+ * penetrating pairs; npos (may be NULL) receives the first pass's count. */
+int orc_nsc_project(orc_world* W, const orc_nsc_params* P, const int* order, int* npos);
 
 /* soft penalty / Hertz-Mindlin force accumulation into f, tau */
 int orc_soft_forces(orc_world* W, const orc_soft_params* P, double* pe);

@@ -70,7 +70,7 @@ static const char* kKernelNames[K_NUM] = {
     "chunk_keys", "cell_ranges", "chunk_gather", "broad_narrow", "contact_record", "sort_scan",
     "manifold", "wall", "csr", "color", "permute", "warm_apply", "psor_sweep", "residual",
     "nsc_free", "nsc_pos", "vv_half1", "vv_half2", "force", "body_reduce", "history", "observe",
-    "mlp", "island", "island_tiny", "island_small", "island_medium", "island_large", "island_huge", "misc"};
+    "mlp", "island", "island_tiny", "island_small", "island_medium", "island_large", "island_huge", "pos_project", "misc"};
 
 }  // namespace rdk
 
@@ -550,6 +550,14 @@ int rdk_set_solver_mode(rdk_ctx* ctx, int mode) {
   return RDK_OK;
 }
 
+int rdk_nsc_project(rdk_ctx* ctx, const rdk_detect_params* dp, const rdk_nsc_params* p, int* passes, int* n_first) {
+  CTX_CHECK();
+  RDK_TRY(check_ready(ctx));
+  if (!dp || !p || p->position_iters < 0 || p->position_psor_iters < 0 || !(p->slop >= 0.0)) return RDK_ERR_ARG;
+  RDK_TRY(nsc_project_run(ctx, dp, p, passes, n_first));
+  return prof_flush(ctx);
+}
+
 int rdk_nsc_island_info(rdk_ctx* ctx, int* n_islands, int* counts4) {
   if (!ctx) return RDK_ERR_ARG;
   if (n_islands) *n_islands = ctx->nislands;
@@ -594,6 +602,8 @@ int rdk_step_nsc(rdk_ctx* ctx, const rdk_detect_params* dp, const rdk_nsc_params
     if (rc == RDK_ERR_NOCONV) S.nonconverged_steps++;
     else if (rc != RDK_OK) return rc;
     RDK_TRY(integrate_run(ctx, RDK_NSC_POS));
+    if (np->position_stabilization && np->position_iters > 0)
+      RDK_TRY(nsc_project_run(ctx, &d, np, nullptr, nullptr));  // main.cpp:4497-4503
     S.steps++;
     S.contacts_total += ss.rod_contacts;
     S.manifolds_total += ss.manifolds;

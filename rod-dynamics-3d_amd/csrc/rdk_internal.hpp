@@ -43,6 +43,7 @@ enum KernelId {
   K_ISL_MEDIUM,
   K_ISL_LARGE,
   K_ISL_HUGE,
+  K_POSPROJ,
   K_MISC,
   K_NUM
 };
@@ -159,6 +160,9 @@ struct rdk_ctx {
   rdk::DevBuf<uint32_t> blab, blab_alt;
   rdk::DevBuf<char> istats;
   rdk::DevBuf<int2> islot;  // local body slots of large-island manifolds
+  // position projection
+  rdk::DevBuf<int> ppa, ppb, pflag, ppos;
+  rdk::DevBuf<double> pphs, ptmp, pdx[3], pdth[3];
   rdk::DevBuf<char> small_out;
   // body -> manifold CSR
   rdk::DevBuf<uint32_t> csr_key, csr_key_alt;
@@ -280,6 +284,7 @@ int integrate_run(rdk_ctx* ctx, int stage);
 int forces_run(rdk_ctx* ctx, int model, const void* params, double* pe);
 int observables_run(rdk_ctx* ctx, double* ke, double* phi);
 int mlp_warm_start(rdk_ctx* ctx);
+int nsc_project_run(rdk_ctx* ctx, const rdk_detect_params* dp, const rdk_nsc_params* p, int* passes, int* npos);
 void mlp_free(rdk_ctx* ctx);
 
 // rocPRIM-backed sorting / scanning helpers (rdk_sort.hip)

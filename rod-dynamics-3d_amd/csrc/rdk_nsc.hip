@@ -1967,4 +1967,253 @@ static int global_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
   return status;
 }
 
+// ---- position stabilisation (nsc_contact.cpp:659-880, multi-body branch) ----
+// Normal-only PSOR on position corrections.  The pairs are the contact list of
+// a detection at the current positions (cached for all passes); every pass
+// recomputes the narrow phase of each cached pair, keeps those penetrating
+// deeper than slop, sweeps up to position_psor_iters times (colour order; a
+// coloured sweep equals the serial sweep in colour order) on per-body dx, dtheta
+// accumulators, then applies x += dx, q = normalize(angleAxis(|dth|, dth/|dth|) q)
+// and wraps.  The manifold buffers are reused for the position manifolds.
+
+struct PosPairs {
+  const int *a, *b;
+  const double* hs;
+};
+
+constexpr int PF_COUNT = 17;  // n 3, ra 3, rb 3, IAn 3, IBn 3, phi + slop, g_n
+
+__global__ void k_pos_pairs(BodyPtrs B, const SysDev* __restrict__ sys, PosPairs P, int np, double slop, double cfm,
+                            int* __restrict__ flag, double* __restrict__ tmp, size_t stride) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const int a = P.a[i], b = P.b[i];
+  const SysDev& S = sys[B.sys[a]];
+  const d3 xa = ld3(B.x, a), xb = ld3(B.x, b);
+  const d3 ua = axis_of(ldq(B, a)), ub = axis_of(ldq(B, b));
+  NarrowOut no;
+  capsule_narrow(xa, ua, B.h[a], B.r[a], xb, ub, B.h[b], B.r[b], S.span, S.periodic, HUGE_VAL, true, no);
+  const double phi = no.dist - P.hs[i];
+  if (!(phi < -slop)) {
+    flag[i] = 0;
+    return;
+  }
+  flag[i] = 1;
+  const d3 n = no.n;
+  const d3 ra = no.pa - xa;
+  const d3 rb = no.pb - (xb + no.shift);
+  const d3 rAxn = cross(ra, n), rBxn = cross(rb, n);
+  const d3 IA = iw_inv(B, a, ua, rAxn), IB = iw_inv(B, b, ub, rBxn);
+  const double vals[PF_COUNT] = {n.x,  n.y,  n.z,  ra.x, ra.y, ra.z, rb.x, rb.y,       rb.z,
+                                 IA.x, IA.y, IA.z, IB.x, IB.y, IB.z, phi + slop,
+                                 diag_term(B.inv_mass[a], B.inv_mass[b], rAxn, IA, rBxn, IB, cfm, false)};
+  for (int f = 0; f < PF_COUNT; ++f) tmp[(size_t)f * stride + i] = vals[f];
+}
+
+__global__ void k_pos_compact(BodyPtrs B, PosPairs P, const int* __restrict__ flag, const int* __restrict__ pos, int np,
+                              const double* __restrict__ tmp, size_t stride, MfPtrs M) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np || !flag[i]) return;
+  const int j = pos[i];
+  auto t = [&](int f) { return tmp[(size_t)f * stride + i]; };
+  M.put3(F_N, j, mk3(t(0), t(1), t(2)));
+  M.put3(F_RA, j, mk3(t(3), t(4), t(5)));
+  M.put3(F_RB, j, mk3(t(6), t(7), t(8)));
+  M.put3(F_IAN, j, mk3(t(9), t(10), t(11)));
+  M.put3(F_IBN, j, mk3(t(12), t(13), t(14)));
+  M.at(F_PHI, j) = t(15);
+  M.at(F_GN, j) = t(16);
+  M.at(F_IMA, j) = B.inv_mass[P.a[i]];
+  M.at(F_IMB, j) = B.inv_mass[P.b[i]];
+  M.a[j] = P.a[i];
+  M.b[j] = P.b[i];
+  M.lam[0][j] = 0.0;
+}
+
+struct PosAcc {
+  double *dx[3], *dth[3];
+};
+
+// one normal-only position update (nsc_contact.cpp:828-851)
+__device__ __forceinline__ double pos_update(const PosAcc& D, const MfPtrs& M, int i, double omega, double cfm) {
+  const int a = M.a[i], b = M.b[i];
+  const d3 n = M.get3(F_N, i), ra = M.get3(F_RA, i), rb = M.get3(F_RB, i);
+  const d3 dxa = ld3(D.dx, a), dxb = ld3(D.dx, b), tha = ld3(D.dth, a), thb = ld3(D.dth, b);
+  const double corr = dot(n, ((dxb + cross(thb, rb)) - dxa) - cross(tha, ra));
+  const double lam = M.lam[0][i];
+  const double res = corr + M.at(F_PHI, i) + cfm * lam;
+  const double delta = -(omega / M.at(F_GN, i)) * res;
+  const double nl = dmax(0.0, lam + delta);
+  const double dl = nl - lam;
+  M.lam[0][i] = nl;
+  if (dl != 0.0) {
+    st3(D.dx, a, dxa - n * (dl * M.at(F_IMA, i)));
+    st3(D.dth, a, tha - M.get3(F_IAN, i) * dl);
+    st3(D.dx, b, dxb + n * (dl * M.at(F_IMB, i)));
+    st3(D.dth, b, thb + M.get3(F_IBN, i) * dl);
+  }
+  return fabs(dl);
+}
+
+// all inner sweeps in one workgroup (colour starts on the device)
+__global__ void __launch_bounds__(SMALL_BLOCK) k_pos_small(PosAcc D, MfPtrs M, const int* __restrict__ cstart,
+                                                           const unsigned* __restrict__ ctr, double omega, double cfm,
+                                                           int iters, int* __restrict__ sweeps_out) {
+  __shared__ double red[SMALL_BLOCK / 64];
+  __shared__ int cs[129];
+  const int ncolor = (int)ctr[3];
+  for (int c = threadIdx.x; c <= ncolor; c += blockDim.x) cs[c] = cstart[c];
+  __syncthreads();
+  int it = 0;
+  for (; it < iters; ++it) {
+    double d = 0.0;
+    for (int c = 0; c < ncolor; ++c) {
+      for (int i = cs[c] + threadIdx.x; i < cs[c + 1]; i += blockDim.x) d = dmax(d, pos_update(D, M, i, omega, cfm));
+      __syncthreads();
+    }
+    if (block_max(d, red) < 1e-8) {
+      ++it;
+      break;
+    }
+  }
+  if (threadIdx.x == 0) *sweeps_out = it;
+}
+
+__global__ void k_pos_sweep(PosAcc D, MfPtrs M, int i0, int i1, double omega, double cfm, double* __restrict__ dmax_out) {
+  const int i = i0 + blockIdx.x * blockDim.x + threadIdx.x;
+  double d = 0.0;
+  if (i < i1) d = pos_update(D, M, i, omega, cfm);
+  for (int off = 32; off > 0; off >>= 1) d = dmax(d, __shfl_down(d, off));
+  if (__lane_id() == 0 && d > 0.0) atomic_max_pos(dmax_out, d);
+}
+
+// x += dx; q = normalize(angleAxis(|dth|, dth / |dth|) q) (nsc_contact.cpp:866-877); wrap
+__global__ void k_pos_apply(BodyPtrs B, const SysDev* __restrict__ sys, int n, PosAcc D) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !(B.inv_mass[i] > 0.0)) return;
+  const SysDev& S = sys[B.sys[i]];
+  d3 x = ld3(B.x, i) + ld3(D.dx, i);
+  const d3 th = ld3(D.dth, i);
+  const double angle = len(th);
+  if (angle > 1e-8) {
+    const d3 ax = mk3(th.x / angle, th.y / angle, th.z / angle);
+    const double sh = sin(angle * 0.5);
+    const q4 dq = q4{cos(angle * 0.5), ax.x * sh, ax.y * sh, ax.z * sh};
+    const q4 q = qnormalize(qmul(dq, ldq(B, i)));
+    B.q[0][i] = q.w;
+    B.q[1][i] = q.x;
+    B.q[2][i] = q.y;
+    B.q[3][i] = q.z;
+  }
+  double* xp[3] = {&x.x, &x.y, &x.z};
+  for (int k = 0; k < 3; ++k) {
+    if (!S.periodic[k] || !(S.span[k] > 0.0)) continue;
+    int g = 0;
+    while (*xp[k] < S.lo[k] && g++ < 8) *xp[k] += S.span[k];
+    g = 0;
+    while (*xp[k] >= S.hi[k] && g++ < 8) *xp[k] -= S.span[k];
+  }
+  st3(B.x, i, x);
+}
+
+int nsc_project_run(rdk_ctx* ctx, const rdk_detect_params* dp, const rdk_nsc_params* p, int* passes_out,
+                    int* npos_out) {
+  const int n = ctx->n, nb = 256;
+  if (passes_out) *passes_out = 0;
+  if (npos_out) *npos_out = 0;
+  if (p->position_iters <= 0) return RDK_OK;
+  rdk_detect_params d = *dp;
+  d.delta = 0.0;  // the NSC detector admits contacts at strict overlap (nsc_contact.cpp:24)
+  RDK_TRY(detect_run(ctx, &d));
+  const int np = ctx->nc;
+  if (np == 0) return RDK_OK;
+  // cached pairs (sorted (a,b), every contact of the detection)
+  RDK_TRY(ensure(ctx, ctx->ppa, np));
+  RDK_TRY(ensure(ctx, ctx->ppb, np));
+  RDK_TRY(ensure(ctx, ctx->pphs, np));
+  RDK_TRY(ensure(ctx, ctx->pflag, np + 1));
+  RDK_TRY(ensure(ctx, ctx->ppos, np + 1));
+  RDK_TRY(ensure(ctx, ctx->ptmp, (size_t)PF_COUNT * np));
+  for (int k = 0; k < 3; ++k) {
+    RDK_TRY(ensure(ctx, ctx->pdx[k], n));
+    RDK_TRY(ensure(ctx, ctx->pdth[k], n));
+  }
+  RDK_HIP(hipMemcpyAsync(ctx->ppa.p, ctx->ca.p, np * sizeof(int), hipMemcpyDeviceToDevice, ctx->stream));
+  RDK_HIP(hipMemcpyAsync(ctx->ppb.p, ctx->cb.p, np * sizeof(int), hipMemcpyDeviceToDevice, ctx->stream));
+  RDK_HIP(hipMemcpyAsync(ctx->pphs.p, ctx->chs.p, np * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+  PosPairs P{ctx->ppa.p, ctx->ppb.p, ctx->pphs.p};
+  PosAcc D;
+  for (int k = 0; k < 3; ++k) {
+    D.dx[k] = ctx->pdx[k].p;
+    D.dth[k] = ctx->pdth[k].p;
+  }
+  BodyPtrs B = body_ptrs(ctx);
+  const size_t stride = (size_t)np;
+  int passes = 0;
+  for (int outer = 0; outer < p->position_iters; ++outer) {
+    {
+      ProfScope ps(ctx, K_POSPROJ);
+      hipLaunchKernelGGL(k_pos_pairs, dim3(grid_for(np, nb)), dim3(nb), 0, ctx->stream, B, ctx->sys_dev.p, P, np,
+                         p->slop, p->cfm, ctx->pflag.p, ctx->ptmp.p, stride);
+    }
+    RDK_TRY(exclusive_scan_int(ctx, ctx->pflag.p, ctx->ppos.p, np));
+    int last[2];
+    RDK_HIP(hipMemcpyAsync(&last[0], ctx->ppos.p + np - 1, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    RDK_HIP(hipMemcpyAsync(&last[1], ctx->pflag.p + np - 1, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    RDK_HIP(hipStreamSynchronize(ctx->stream));
+    const int m = last[0] + last[1];
+    if (outer == 0 && npos_out) *npos_out = m;
+    if (m == 0) break;
+    ++passes;
+    RDK_TRY(ensure_manifolds(ctx, m));
+    {
+      ProfScope ps(ctx, K_POSPROJ);
+      hipLaunchKernelGGL(k_pos_compact, dim3(grid_for(np, nb)), dim3(nb), 0, ctx->stream, B, P, ctx->pflag.p,
+                         ctx->ppos.p, np, ctx->ptmp.p, stride, mf_ptrs(ctx, false));
+    }
+    ctx->nm = m;
+    ctx->nm_wall = 0;
+    RDK_TRY(nsc_color_run(ctx, false));
+    for (int k = 0; k < 3; ++k) {
+      RDK_HIP(hipMemsetAsync(D.dx[k], 0, n * sizeof(double), ctx->stream));
+      RDK_HIP(hipMemsetAsync(D.dth[k], 0, n * sizeof(double), ctx->stream));
+    }
+    MfPtrs Mc = mf_ptrs(ctx, true);
+    if (m <= small_m_limit()) {
+      RDK_TRY(ensure(ctx, ctx->small_out, sizeof(int)));
+      ProfScope ps(ctx, K_POSPROJ);
+      hipLaunchKernelGGL(k_pos_small, dim3(1), dim3(SMALL_BLOCK), 0, ctx->stream, D, Mc, ctx->d_cstart.p,
+                         (const unsigned*)ctx->uncolored.p, p->omega, p->cfm, p->position_psor_iters,
+                         (int*)ctx->small_out.p);
+    } else {
+      RDK_TRY(color_to_host(ctx));
+      RDK_TRY(ensure(ctx, ctx->resid, 4));
+      for (int it = 0; it < p->position_psor_iters; ++it) {
+        RDK_HIP(hipMemsetAsync(ctx->resid.p + 2, 0, sizeof(double), ctx->stream));
+        for (int c = 0; c < ctx->ncolor; ++c) {
+          const int i0 = ctx->color_start[c], i1 = ctx->color_start[c + 1];
+          if (i1 <= i0) continue;
+          ProfScope ps(ctx, K_POSPROJ);
+          hipLaunchKernelGGL(k_pos_sweep, dim3(grid_for(i1 - i0, nb)), dim3(nb), 0, ctx->stream, D, Mc, i0, i1,
+                             p->omega, p->cfm, ctx->resid.p + 2);
+        }
+        double dm = 0.0;
+        RDK_HIP(hipMemcpyAsync(&dm, ctx->resid.p + 2, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        RDK_HIP(hipStreamSynchronize(ctx->stream));
+        if (dm < 1e-8) break;
+      }
+    }
+    {
+      ProfScope ps(ctx, K_POSPROJ);
+      hipLaunchKernelGGL(k_pos_apply, dim3(grid_for(n, nb)), dim3(nb), 0, ctx->stream, B, ctx->sys_dev.p, n, D);
+    }
+  }
+  // the manifold buffers now hold position manifolds; the contact list is the
+  // pre-projection detection
+  ctx->have_manifolds = false;
+  ctx->have_coloring = false;
+  if (passes_out) *passes_out = passes;
+  return RDK_OK;
+}
+
 }  // namespace rdk

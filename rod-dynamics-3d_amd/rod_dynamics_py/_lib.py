@@ -39,7 +39,8 @@ class NscParams(C.Structure):
     _fields_ = [("beta", C.c_double), ("cfm", C.c_double), ("omega", C.c_double), ("tol", C.c_double),
                 ("max_sweeps", C.c_int), ("check_every", C.c_int), ("friction_proj", C.c_int),
                 ("position_stabilization", C.c_int), ("warm_mode", C.c_int), ("stop_rule", C.c_int),
-                ("warm_walls", C.c_int), ("stall_sweeps", C.c_int)]
+                ("warm_walls", C.c_int), ("stall_sweeps", C.c_int), ("position_iters", C.c_int),
+                ("position_psor_iters", C.c_int), ("slop", C.c_double)]
 
 
 class SoftParams(C.Structure):
@@ -95,6 +96,7 @@ _SIGS = {
     "rdk_nsc_residual": (C.c_int, [C.c_void_p, C.POINTER(NscParams), _dp]),
     "rdk_set_solver_mode": (C.c_int, [C.c_void_p, C.c_int]),
     "rdk_nsc_island_info": (C.c_int, [C.c_void_p, _ip, _ip]),
+    "rdk_nsc_project": (C.c_int, [C.c_void_p, C.POINTER(DetectParams), C.POINTER(NscParams), _ip, _ip]),
     "rdk_nsc_get_solve_order": (C.c_int, [C.c_void_p, C.c_int, _ip]),
     "rdk_integrate": (C.c_int, [C.c_void_p, C.c_int]),
     "rdk_step_nsc": (C.c_int, [C.c_void_p, C.POINTER(DetectParams), C.POINTER(NscParams), C.c_int,
@@ -181,8 +183,10 @@ def system_desc(lo, hi, periodic, floor_wall=False, ceil_wall=False, mu=0.0):
 
 
 def nsc_params(beta=0.0, cfm=0.0, omega=1.0, tol=1e-8, max_sweeps=10000, check_every=1, friction_proj=0,
-               position_stabilization=0, warm_mode=1, stop_rule=1, warm_walls=0, stall_sweeps=0):
+               position_stabilization=0, warm_mode=1, stop_rule=1, warm_walls=0, stall_sweeps=0, position_iters=5,
+               position_psor_iters=50, slop=1e-4):
     p = NscParams()
+    p.position_iters, p.position_psor_iters, p.slop = position_iters, position_psor_iters, slop
     p.warm_walls = warm_walls
     p.stall_sweeps = stall_sweeps
     p.beta, p.cfm, p.omega, p.tol = beta, cfm, omega, tol
@@ -358,6 +362,11 @@ class Context:
 
     def set_solver_mode(self, mode):
         self._check(self._L.rdk_set_solver_mode(self._h, int(mode)))
+
+    def nsc_project(self, dp, p):
+        passes, nfirst = C.c_int(), C.c_int()
+        self._check(self._L.rdk_nsc_project(self._h, C.byref(dp), C.byref(p), C.byref(passes), C.byref(nfirst)))
+        return passes.value, nfirst.value
 
     def island_info(self):
         n = C.c_int()

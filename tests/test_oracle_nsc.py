@@ -273,3 +273,68 @@ def test_overrelaxed_psor_limit_cycle_fixture():
         W.sweeps(op, 2000, None)
         res[om] = W.residual(op)
     assert res[1.3] > 0.1 and res[1.0] < 1e-12, res
+
+
+def _crossing_pair(gap):
+    """Rod A along x at the origin, rod B along y above it; the contact is at
+    both rod centres, so the correction is a pure translation."""
+    import math
+
+    c = math.cos(math.pi / 4)
+    x = np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 1.0 + gap]])
+    q = np.array([[c, 0.0, 0.0, c], [1.0, 0.0, 0.0, 0.0]])  # A's axis (local +Y) rotated onto x
+    W = O.World(x, q, np.zeros((2, 3)), np.zeros((2, 3)), np.full(2, 0.5), np.full(2, 2.0), np.array([1.0, 3.0]))
+    W.set_box([-50] * 3, [50] * 3, [0, 0, 0])
+    return W
+
+
+def test_position_projection_single_contact_closed_form():
+    # phi = -0.05: one pass, omega = 1, cfm = 0 moves the pair apart to phi = -slop
+    W = _crossing_pair(-0.05)
+    assert W.detect(0, 0.0) == 1
+    p = O.nsc_params(omega=1.0, cfm=0.0, position_stabilization=1, position_iters=1, position_psor_iters=50, slop=1e-4)
+    passes, npos = W.project(p)
+    assert passes == 1 and npos == 1
+    st = W.state()
+    gap = st["x"][1, 2] - st["x"][0, 2] - 1.0
+    assert abs(gap - (-1e-4)) < 1e-14
+    # the correction splits by inverse mass: dx_A = -lambda/m_A, dx_B = +lambda/m_B, lambda = 0.0499/(1 + 1/3)
+    lam = (0.05 - 1e-4) / (1.0 + 1.0 / 3.0)
+    assert np.allclose(st["x"][0], [0, 0, -lam], atol=1e-15) and np.allclose(st["x"][1], [0, 0, 0.95 + lam / 3], atol=1e-15)
+    # within slop: nothing to do
+    W2 = _crossing_pair(-5e-5)
+    W2.detect(0, 0.0)
+    assert W2.project(p) == (0, 0)
+
+
+def test_position_projection_conserves_mass_weighted_position_and_reduces_penetration():
+    rng = np.random.default_rng(17)
+    n, side = 400, 14.0
+    x = rng.uniform(0, side, size=(n, 3))
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    r = rng.uniform(0.3, 0.5, n)
+    h = rng.uniform(0.5, 2.0, n)
+    m = rng.uniform(0.5, 2.0, n)
+    W = O.World(x, q, np.zeros((n, 3)), np.zeros((n, 3)), r, h, m)
+    W.set_box([-100] * 3, [100] * 3, [0, 0, 0])
+
+    def pen():
+        nc = W.detect(0, 0.0)
+        c = W.contacts(nc)
+        d = (r[c["a"]] + r[c["b"]]) - c["dist"]
+        return nc, float(d.sum()), float(np.median(d))
+
+    nc0, pen0, med0 = pen()
+    assert nc0 > 100 and pen0 > 100.0 and med0 > 0.1
+    x0 = W.state()["x"]
+    p = O.nsc_params(omega=1.0, cfm=0.0, position_stabilization=1, position_iters=5, position_psor_iters=50, slop=1e-4)
+    passes, npos = W.project(p)
+    assert passes >= 1 and npos > 50
+    x1 = W.state()["x"]
+    # corrections are equal and opposite along each normal, weighted by inverse mass
+    assert np.allclose((m[:, None] * (x1 - x0)).sum(0), 0.0, atol=1e-11)
+    nc1, pen1, med1 = pen()
+    # total depth drops by > 5x; the typical pair ends exactly at the allowed slop
+    assert pen1 < 0.2 * pen0
+    assert abs(med1 - 1e-4) < 1e-6
