@@ -182,3 +182,27 @@ def test_reference_broadphase_benchmark_geometry(gpu_ctx_factory):
             W.set_box([lo] * 3, [hi] * 3, (1, 1, 1))
             no = W.detect(0, 0.005)
             compare(ctx.contacts(nc), W.contacts(no), nc)
+
+
+def test_reference_packings(gpu_ctx_factory):
+    """Frozen relaxed packings of the reference (aspect ratios 10 to 1000, in
+    free space) at the contact criterion of its metadata (segment distance
+    < 1.01 d): the GPU list equals the oracle's bit for bit and its count
+    equals assets/packings_metadata.csv's n_contacts."""
+    from packings import contact_delta, free_box, golden_packings
+
+    for pid, x, q, r, h, ref in golden_packings():
+        print(f"packing {pid}", flush=True)
+        n = len(r)
+        lo, hi = free_box(x)
+        ctx = gpu_ctx_factory()
+        load(ctx, x, q, r, h, np.ones(n), lo, hi, (0, 0, 0))
+        delta = contact_delta(r)
+        W = O.World(x, q, None, None, r, h, np.ones(n))
+        W.set_box(lo, hi, (0, 0, 0))
+        no = W.detect(0, delta)
+        oc = W.contacts(no)
+        for chunk_len in (0.0, 0.1):
+            nc = ctx.detect(L.detect_params(delta, chunk_len, 1.0))
+            assert nc == ref, (pid, chunk_len, nc, ref)
+            compare(ctx.contacts(nc), oc, nc)

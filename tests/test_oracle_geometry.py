@@ -5,6 +5,7 @@ smoke_test.py:129-162,207-233 (contacts through a periodic boundary),
 tests/dev_utils/debug_segseg.cpp:82-98 (perpendicular crossing).
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -189,3 +190,75 @@ def test_reference_broadphase_benchmark_contact_counts():
             assert abs(got - ref) <= 0.01 * ref, (n, got, ref)
         else:
             assert got == ref, (n, got, ref)
+
+
+def _packing_count(x, q, r, h, hashed):
+    from packings import contact_delta, free_box
+
+    n = len(r)
+    W = O.World(x, q, None, None, r, h, np.ones(n))
+    lo, hi = free_box(x)
+    W.set_box(lo, hi, [0, 0, 0])
+    return W.detect(1 if hashed else 0, contact_delta(r))
+
+
+def test_reference_packings_contact_counts():
+    """Frozen relaxed packings of the reference (one per aspect ratio, 10 to
+    1000) and the contact counts its metadata holds for them
+    (assets/packings_metadata.csv, computed by repro/build_packing_metadata.py
+    as segment distance < 1.01 d): the oracle's detection with a gap
+    tolerance of 0.01 d reproduces every count exactly."""
+    from packings import golden_packings
+
+    seen = 0
+    for pid, x, q, r, h, ref in golden_packings():
+        assert np.allclose(np.linalg.norm(q, axis=1), 1.0, atol=1e-14)
+        got = _packing_count(x, q, r, h, hashed=False)
+        assert got == ref, (pid, got, ref)
+        assert _packing_count(x, q, r, h, hashed=True) == ref, pid
+        seen += 1
+    assert seen == 10
+
+
+def test_packing_endpoints_round_trip():
+    """The endpoint -> (centre, quaternion, half length) conversion returns
+    the endpoints (up to the symmetric flip of the axis)."""
+    from packings import rods_from_endpoints
+
+    rng = np.random.default_rng(5)
+    p0 = rng.normal(size=(200, 3))
+    d = rng.normal(size=(200, 3))
+    d[:3] = [[0, -1, 0], [0, 1, 0], [1e-9, -1, 0]]  # axis along -Y must not degenerate
+    ends = np.hstack([p0, p0 + d])
+    x, q, r, h = rods_from_endpoints(ends, 0.1)
+    for i in range(len(x)):
+        u = O.axis(q[i])
+        a, b = x[i] - h[i] * u, x[i] + h[i] * u
+        ok = (np.allclose(a, ends[i, :3], atol=1e-12) and np.allclose(b, ends[i, 3:], atol=1e-12)) or \
+             (np.allclose(b, ends[i, :3], atol=1e-12) and np.allclose(a, ends[i, 3:], atol=1e-12))
+        assert ok, i
+
+
+@pytest.mark.skipif(not os.path.exists("/root/reference/assets/packings_metadata.csv"),
+                    reason="the reference tree is not present")
+def test_all_reference_packings_contact_counts():
+    """Every packing listed in assets/packings_metadata.csv (60 of them)."""
+    import csv
+
+    from packings import rods_from_endpoints
+
+    ref_root = "/root/reference"
+    rows = list(csv.DictReader(open(os.path.join(ref_root, "assets", "packings_metadata.csv"))))
+    assert len(rows) >= 60
+    for row in rows:
+        radius, ends = None, []
+        for line in open(os.path.join(ref_root, row["path"])):
+            line = line.strip()
+            if line.startswith("#"):
+                if "rod_radius" in line:
+                    radius = float(line.split("=")[1].split()[0])
+            elif line:
+                ends.append([float(t) for t in line.split()[:6]])
+        x, q, r, h = rods_from_endpoints(np.asarray(ends), radius)
+        got = _packing_count(x, q, r, h, hashed=False)
+        assert got == int(row["n_contacts"]), (row["packing_id"], got, row["n_contacts"])
