@@ -50,7 +50,7 @@ BYTES = {
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 # (FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0, from a calibrated 1 GiB copy)
 # for (workload, kernel); recorded with the command in profiles/
-TRAFFIC = {("c2", "broad_narrow"): 773.6e6 + 0.26e6}
+TRAFFIC = {}
 
 
 def parse():
@@ -64,6 +64,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chunk-len", type=float, default=0.0, help="0: autotune over a few steps of the warm-up")
     ap.add_argument("--check-every", type=int, default=4)
+    ap.add_argument("--stall-sweeps", type=int, default=None,
+                    help="override the workload's stall-rule window (hard-contact workloads)")
     ap.add_argument("--block", type=int, default=250)
     ap.add_argument("--ab-steps", type=int, default=20,
                     help="steps after the timed protocol solved from cold, previous-step and learned starts")
@@ -200,13 +202,15 @@ def run_c2(a, L, scenes):
     log(f"scene {sc.name} built in {time.perf_counter() - t0:.1f}s (box {sc.extra['side']:.2f})")
     p = sc.params
     p.check_every = a.check_every
+    if a.stall_sweeps is not None:
+        p.stall_sweeps = a.stall_sweeps
     ctx = L.Context(0)
     sc.load(ctx)
     if a.chunk_len > 0:
         best, tune, used = a.chunk_len, {}, 0
     else:  # --chunk-len 0: autotune inside the warm-up
         best, tune, used = autotune(ctx, lambda cl, k: ctx.step_nsc(L.detect_params(0.0, cl, 1.0), p, k),
-                                    (2.0, 4.0, 8.0), reps=min(3, max(1, a.warmup // 3 - 1)))
+                                    (2.0, 4.0, 8.0, 16.0), reps=min(3, max(1, a.warmup // 3 - 1)))
     det = L.detect_params(0.0, best, 1.0)
     dom = rank_kernels(ctx, lambda: ctx.step_nsc(det, p, max(1, a.warmup - used)))
     log(f"warmup {a.warmup} steps done; dominant kernel {dom}; timing {a.steps} steps")
@@ -242,7 +246,7 @@ def run_c2(a, L, scenes):
         "friction_projection": "euclidean disc" if p.friction_proj == 1 else "reference sequential",
         "steps_total": total + max(0, used + max(1, a.warmup - used) - a.warmup), "seed": sc.extra["seed"],
         "check_every": a.check_every, "chunk_len": best, "chunk_len_autotune_ms": tune,
-        "stall_sweeps": p.stall_sweeps})
+        "stall_sweeps": p.stall_sweeps, "stall_omega": p.stall_omega})
     out["psor"] = {"sweeps_per_step": st["sweeps_total"] / max(st["steps"], 1),
                    "manifolds_per_step": st["manifolds_total"] / max(st["steps"], 1),
                    "rod_contacts_per_step": st["contacts_total"] / max(st["steps"], 1),
@@ -418,10 +422,12 @@ def run_c4(a, L, scenes):
     sc = scenes.c4_batched(nsys=a.n, threads=min(16, os.cpu_count() or 1))
     log(f"scene {sc.name} built in {time.perf_counter() - t0:.1f}s ({sc.n} rods, {sc.extra['redraws']} redraws)")
     p = sc.params
+    if a.stall_sweeps is not None:
+        p.stall_sweeps = a.stall_sweeps
     ctx = L.Context(0)
     sc.load(ctx)
     best, tune, used = autotune(ctx, lambda cl, k: ctx.step_nsc(L.detect_params(0.0, cl, 1.0), p, k),
-                                (2.0, 4.0, 8.0), reps=3)
+                                (2.0, 4.0, 8.0, 16.0), reps=3)
     det = L.detect_params(0.0, best, 1.0)
     dom = rank_kernels(ctx, lambda: ctx.step_nsc(det, p, a.warmup))
     log(f"warmup {a.warmup} steps done; dominant kernel {dom}; timing {a.steps} steps")
@@ -450,6 +456,7 @@ def run_c4(a, L, scenes):
         "residual_tol": 1e-8, "dt": 1e-3, "redraws": sc.extra["redraws"], "seed": sc.extra["seed"],
         "chunk_len_autotuned": best, "autotune_ms_per_step": tune,
         "steps_total": tuned_steps + a.warmup + a.steps, "stall_sweeps": p.stall_sweeps,
+        "stall_omega": p.stall_omega,
         "hipgraph": "not used: each step reads contact and island counts back to size its launches"})
     out["psor"] = {"sweeps_per_step": st["sweeps_total"] / max(st["steps"], 1),
                    "manifolds_per_step": st["manifolds_total"] / max(st["steps"], 1),

@@ -73,14 +73,16 @@ typedef struct rdk_nsc_params {
   int stall_sweeps;        /* stop_rule 1 only; 0 = off.  > 0: at sweeps = k, 2k,
                               4k, ... an island (or the global solve) whose residual
                               has not dropped by 10 % since the previous milestone
-                              continues with omega = 1 if omega > 1 (over-relaxed
-                              PSOR can cycle), else stops there and counts as not
-                              converged */
+                              continues with omega = stall_omega if its omega is
+                              larger (over-relaxed PSOR can cycle), else stops there
+                              and counts as not converged */
   int position_iters;      /* position projection (config.hpp:77-80), run by the step
                               after the position update when position_stabilization
                               is set: outer passes (reference default 5) */
   int position_psor_iters; /* inner normal-only sweeps per pass (default 50) */
   double slop;             /* allowed penetration before correction (default 1e-4) */
+  double stall_omega;      /* omega after the first stall (1 = plain Gauss-Seidel);
+                              0 = stop at the first stall */
 } rdk_nsc_params;
 
 typedef struct rdk_soft_params { /* SoftContactCfg, config.hpp:6-27 */
@@ -224,7 +226,10 @@ int rdk_kernel_times(rdk_ctx* ctx, int kernel, int64_t* launches,
                      double* total_ms);
 const char* rdk_kernel_name(int kernel);
 int rdk_num_kernels(void);
-/* algorithmic-byte counters accumulated by the fused steps */
+/* algorithmic-byte counters accumulated by the fused steps, in this order:
+ * steps, detection passes, chunks binned, chunk pairs passing the sphere
+ * test, contacts, manifolds, manifold updates in sweeps, manifold residual
+ * evaluations, body integrations, force evaluations, FP64 narrow-phase calls */
 int rdk_work_counters(rdk_ctx* ctx, int64_t* counters, int n);
 
 /* ---- scene generation (resetScene populate/randomInit,

@@ -15,18 +15,24 @@
 //                      cell >= c, for every cell (and one past the last), so
 //                      the three x-adjacent cells of a stencil row are one
 //                      contiguous range [start[x-1], start[x+2])
-//   4. chunk_gather    FP32 (centre, radius), rod id and chunk index in sorted
-//                      (cell) order so the stencil walk reads contiguous memory
+//   4. chunk_gather    FP32 (centre, radius), FP32 (axis, half length), rod id
+//                      and chunk index in sorted (cell) order so the stencil
+//                      walk reads contiguous memory
 //   5. broad_narrow    one thread per sorted chunk walks the 9 stencil rows
-//                      (27 cells); each unordered chunk pair is examined once
-//                      (by the lower sorted position).  Workgroups are mapped
-//                      so that each XCD (own L2) walks one contiguous run of
-//                      the sorted chunks, i.e. one slab of the grid.  An FP32 bounding-sphere test with the
-//                      chunk-centre minimum image and a rounding margin rejects
-//                      far pairs conservatively; survivors run the full-rod FP64
-//                      Lumelsky narrow phase.  A pair is emitted only by the
-//                      chunk pair that contains the closest points (owner rule),
-//                      so each contact appears exactly once whatever the
+//                      (27 cells), several candidates in flight per lane;
+//                      each unordered chunk pair is examined once (by the
+//                      lower sorted position).  Workgroups are mapped so that
+//                      each XCD (own L2) walks one contiguous run of the
+//                      sorted chunks, i.e. one slab of the grid.  An FP32
+//                      bounding-sphere test with the chunk-centre minimum
+//                      image and a rounding margin rejects far pairs
+//                      conservatively; survivors are queued per wavefront,
+//                      pass a separating-axis lower bound on the chunk-segment
+//                      distance (also FP32, also conservative), and what is
+//                      left runs the full-rod FP64 Lumelsky narrow phase, 64
+//                      pairs at a time.  A pair is emitted only by the chunk
+//                      pair that contains the closest points (owner rule), so
+//                      each contact appears exactly once whatever the
 //                      chunking.  Emission is wavefront-aggregated (ballot +
 //                      popcount, one atomic per wavefront).
 //   6. radix sort of the 64-bit (a,b) keys -> canonical (a,b) order
@@ -55,9 +61,9 @@ __device__ __forceinline__ q4 load_q(const BodyPtrs& B, int i) {
 }
 __device__ __forceinline__ d3 load3(double* const* a, int i) { return mk3(a[0][i], a[1][i], a[2][i]); }
 
-__device__ __forceinline__ void chunk_geom(const BodyPtrs& B, int i, int k, int K, d3& c, double& rad) {
+__device__ __forceinline__ void chunk_geom(const BodyPtrs& B, int i, int k, int K, d3& c, double& rad, d3& u) {
   const d3 x = load3(B.x, i);
-  const d3 u = axis_of(load_q(B, i));
+  u = axis_of(load_q(B, i));
   const double h = B.h[i];
   const d3 a1 = x - u * h;
   const d3 a2 = x + u * h;
@@ -73,9 +79,9 @@ __global__ void k_chunk_keys(BodyPtrs B, const SysDev* __restrict__ sys, const i
   if (m >= M) return;
   const int i = chunk_rod[m];
   const SysDev S = sys[B.sys[i]];
-  d3 c;
+  d3 c, u;
   double rad;
-  chunk_geom(B, i, chunk_k[m], rod_nchunk[i], c, rad);
+  chunk_geom(B, i, chunk_k[m], rod_nchunk[i], c, rad, u);
   int cx = cell_coord(c.x, S.lo[0], S.inv_cell[0], S.dims[0], S.periodic[0]);
   int cy = cell_coord(c.y, S.lo[1], S.inv_cell[1], S.dims[1], S.periodic[1]);
   int cz = cell_coord(c.z, S.lo[2], S.inv_cell[2], S.dims[2], S.periodic[2]);
@@ -105,6 +111,7 @@ struct SortedChunks {
   const uint32_t* key;  // sorted cell keys
   int *rod, *k;
   float4* cf;  // FP32 (centre, radius) for the conservative prefilter
+  float4* ca;  // FP32 (axis, half length of the chunk segment)
 };
 
 __global__ void k_chunk_gather(BodyPtrs B, const int* __restrict__ sorted_id, const int* __restrict__ chunk_rod,
@@ -115,12 +122,14 @@ __global__ void k_chunk_gather(BodyPtrs B, const int* __restrict__ sorted_id, co
   const int m = sorted_id[p];
   const int i = chunk_rod[m];
   const int k = chunk_k[m];
-  d3 c;
+  d3 c, u;
   double rad;
-  chunk_geom(B, i, k, rod_nchunk[i], c, rad);
+  const int K = rod_nchunk[i];
+  chunk_geom(B, i, k, K, c, rad, u);
   S.rod[p] = i;
   S.k[p] = k;
   S.cf[p] = make_float4((float)c.x, (float)c.y, (float)c.z, (float)rad);
+  S.ca[p] = make_float4((float)u.x, (float)u.y, (float)u.z, (float)(B.h[i] / (double)K));
 }
 
 // Workgroups are dispatched round-robin over the XCDs (8 on MI355X, each with
@@ -174,29 +183,29 @@ __device__ __forceinline__ void narrow_pair(const BodyPtrs& B, const SysDev* __r
 // x ranges of sorted positions (two after a periodic wrap).  Range r = 2 *
 // row + part.  Returns false when no range is left; q is clipped to q > p.
 struct StencilWalk {
-  int p, cx, cy, cz, xa0, xa1, xb0, xb1;
-  bool two;
+  int p, cy, cz, xa0, xa1, xb;  // x ranges: [xa0, xa1] and, after a periodic wrap, the single cell xb
+  int d0, d1, d2, off;  // the system's grid (kept in registers: on the path to the cstart loads)
+  bool two, py, pz;     // periodic in y, z
   int r, q, q1;
-  __device__ __forceinline__ bool open_next(const SysDev& S, const int* __restrict__ cstart) {
-    const int d0 = S.dims[0], d1 = S.dims[1], d2 = S.dims[2];
+  __device__ __forceinline__ bool open_next(const int* __restrict__ cstart) {
     while (r < 18) {
       const int row = r >> 1, part = r & 1;
       ++r;
       if (part == 1 && !two) continue;
       const int dz = row / 3 - 1, dy = row % 3 - 1;
       int z = cz + dz;
-      if (S.periodic[2]) {
+      if (pz) {
         if (d2 == 1 && dz != 0) continue;
-        z = (z + d2) % d2;
+        z += z < 0 ? d2 : (z >= d2 ? -d2 : 0);  // |dz| <= 1, d2 == 1 or >= 3
       } else if (z < 0 || z >= d2) continue;
       int y = cy + dy;
-      if (S.periodic[1]) {
+      if (py) {
         if (d1 == 1 && dy != 0) continue;
-        y = (y + d1) % d1;
+        y += y < 0 ? d1 : (y >= d1 ? -d1 : 0);
       } else if (y < 0 || y >= d1) continue;
-      const int base = S.cell_off + d0 * (y + d1 * z);
-      q = cstart[base + (part ? xb0 : xa0)];
-      q1 = cstart[base + (part ? xb1 : xa1) + 1];
+      const int base = off + d0 * (y + d1 * z);
+      q = cstart[base + (part ? xb : xa0)];
+      q1 = cstart[base + (part ? xb : xa1) + 1];
       if (q <= p) q = p + 1;
       if (q < q1) return true;
     }
@@ -207,24 +216,123 @@ struct StencilWalk {
 constexpr int BN_THREADS = 256;
 constexpr int BN_QUEUE = 128;  // per wavefront: < 64 pending + one append round of <= 64
 
-// One thread per sorted chunk p.  Every unordered chunk pair (p, q) is
-// examined once, by the thread with the lower sorted position.  Lanes run the
-// cheap FP32 prefilter (chunk-centre minimum image + rounding margin, never
-// farther than the rod-centre image the narrow phase uses, so it never rejects
-// a touching pair) and append survivors to a per-wavefront LDS queue; each
-// time 64 are queued, all 64 lanes run the FP64 narrow phase on one entry
-// each, so the expensive part runs without divergence.
-__global__ void __launch_bounds__(BN_THREADS) k_broad_narrow(BodyPtrs B, const SysDev* __restrict__ sys,
+// Separating-axis lower bound on the distance between the axis segments of
+// chunks p and q, over the axes u_p, u_q and u_p x u_q (the last only when it
+// is not short, so its normalisation stays accurate): the distance between
+// two point sets is at least the gap between their projections on any unit
+// axis.  If the owner chunk pair of a touching rod pair is (p, q), its
+// segments are closer than r_p + r_q + delta, so the bound is too; the pair is
+// dropped only when the FP32 bound exceeds that by more than the rounding
+// margin (centre rounding as in the sphere test, plus 1.6e-4 of the sphere
+// threshold for the axes, half lengths and the normalisation, >> FP32 eps).
+// Valid only when a single periodic image of q can lie within the sphere
+// threshold (every periodic span > 2 x threshold): then the chunk-centre
+// minimum image is the image the narrow phase uses; otherwise the pair is
+// kept.
+__device__ __forceinline__ bool sat_keep(const SortedChunks& SC, int p, int q, const float* spanf,
+                                         const float* ispanf, int per, float extra) {
+  const float4 cp = SC.cf[p], cq = SC.cf[q], ap = SC.ca[p], aq = SC.ca[q];
+  float dx = cq.x - cp.x, dy = cq.y - cp.y, dz = cq.z - cp.z;
+  if (per & 1) dx -= spanf[0] * rintf(dx * ispanf[0]);
+  if (per & 2) dy -= spanf[1] * rintf(dy * ispanf[1]);
+  if (per & 4) dz -= spanf[2] * rintf(dz * ispanf[2]);
+  const float bound = cp.w + cq.w + extra;
+  const float two = 2.01f * bound;
+  if (((per & 1) && !(spanf[0] > two)) || ((per & 2) && !(spanf[1] > two)) || ((per & 4) && !(spanf[2] > two)))
+    return true;
+  const float hp = ap.w, hq = aq.w;
+  const float lim = (cp.w - hp) + (cq.w - hq) + extra + 1.6e-4f * bound;
+  const float c = fabsf(ap.x * aq.x + ap.y * aq.y + ap.z * aq.z);
+  const float dp = fabsf(dx * ap.x + dy * ap.y + dz * ap.z);
+  const float dq = fabsf(dx * aq.x + dy * aq.y + dz * aq.z);
+  float lb = fmaxf(dp - hp - hq * c, dq - hq - hp * c);
+  const float nx = ap.y * aq.z - ap.z * aq.y, ny = ap.z * aq.x - ap.x * aq.z, nz = ap.x * aq.y - ap.y * aq.x;
+  const float nn = nx * nx + ny * ny + nz * nz;
+  if (nn > 1.0f / 256.0f) lb = fmaxf(lb, fabsf(dx * nx + dy * ny + dz * nz) * rsqrtf(nn));
+  return lb <= lim;
+}
+
+// The system constants of one chunk lane, shared across the wavefront: every
+// queued entry (p, q) was found by lane p - p0 of the same wavefront, so its
+// box and margin are read from that lane.
+struct LaneSys {
+  float span[3], ispan[3], extra;
+  int per;  // bit k: axis k periodic
+};
+
+// Wavefront-wide barrier around the LDS queue (a wavefront executes in
+// lockstep; the fences order its LDS writes before other lanes' reads).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Runs the separating-axis test on n <= 64 queued sphere survivors (one per
+// lane, entries src[0..n)) and appends the kept pairs to the narrow queue;
+// whenever the narrow queue holds 64, all lanes run the FP64 narrow phase on
+// one entry each.  Called by the whole wavefront (n and pend2 are uniform).
+__device__ __forceinline__ void sat_stage(const int2* __restrict__ src, int n, int p0, const LaneSys& own,
+                                          int2* __restrict__ Q2, int& pend2, const BodyPtrs& B,
+                                          const SysDev* __restrict__ sys, const SortedChunks& SC,
+                                          const int* __restrict__ rod_nchunk, double delta,
+                                          uint64_t* __restrict__ out, unsigned cap, unsigned* __restrict__ counter,
+                                          unsigned long long& nnarrow) {
+  const int lane = __lane_id();
+  int2 e = make_int2(0, 0);
+  if (lane < n) e = src[lane];
+  const int from = lane < n ? e.x - p0 : lane;
+  float span[3], ispan[3];
+  for (int k = 0; k < 3; ++k) {
+    span[k] = __shfl(own.span[k], from);
+    ispan[k] = __shfl(own.ispan[k], from);
+  }
+  const float extra = __shfl(own.extra, from);
+  const int per = __shfl(own.per, from);
+  const bool keep = lane < n && sat_keep(SC, e.x, e.y, span, ispan, per, extra);
+  const unsigned long long m = __ballot(keep);
+  if (keep) {
+    ++nnarrow;
+    Q2[pend2 + __popcll(m & ((1ull << lane) - 1ull))] = e;
+  }
+  pend2 += __popcll(m);
+  if (pend2 >= 64) {
+    wave_sync();
+    const int2 f = Q2[pend2 - 64 + lane];
+    narrow_pair(B, sys, SC, rod_nchunk, f.x, f.y, delta, out, cap, counter);
+    pend2 -= 64;
+    wave_sync();
+  }
+}
+
+constexpr int BN_UNROLL = 2;  // candidates loaded per lane per step (independent loads in flight)
+
+// Broad + narrow phase, one thread per sorted chunk p.  Every unordered chunk
+// pair (p, q) is examined once, by the thread with the lower sorted position.
+// The stencil walk is latency-bound (a lane's candidates are loaded from L2
+// one after another), so each step a lane loads the next BN_UNROLL candidates
+// of its current stencil range at once and then runs the FP32 sphere
+// prefilter on them (chunk-centre minimum image + rounding margin, never
+// farther than the rod-centre image the narrow phase uses, so it never
+// rejects a touching pair).  Survivors are queued per wavefront in LDS; each
+// time 64 are queued the 64 lanes run the separating-axis test on one entry
+// each and queue what passes for the narrow phase; each time that queue holds
+// 64, all lanes run the FP64 narrow phase on one entry each.  The expensive
+// stages thus run without divergence, and nothing is appended to global
+// memory but the contacts themselves.
+__global__ void __launch_bounds__(BN_THREADS, 4) k_broad_narrow(BodyPtrs B, const SysDev* __restrict__ sys,
                                                              SortedChunks SC, const int* __restrict__ cstart,
                                                              const int* __restrict__ rod_nchunk, int M, double delta,
                                                              uint64_t* __restrict__ out, unsigned cap,
                                                              unsigned* __restrict__ counter,
                                                              unsigned long long* __restrict__ cand_counter) {
   __shared__ int2 queue[BN_THREADS / 64][BN_QUEUE];
+  __shared__ int2 queue2[BN_THREADS / 64][BN_QUEUE];
   int2* Q = queue[threadIdx.x >> 6];
+  int2* Q2 = queue2[threadIdx.x >> 6];
   const int lane = __lane_id();
   const int p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  unsigned long long ncand = 0;
+  unsigned long long ncand = 0, nnarrow = 0;
   StencilWalk W;
   W.p = p;
   W.r = 0;
@@ -232,96 +340,117 @@ __global__ void __launch_bounds__(BN_THREADS) k_broad_narrow(BodyPtrs B, const S
   int a = 0;
   float4 me = make_float4(0.f, 0.f, 0.f, 0.f);
   bool a_static = false;
-  float spanf[3] = {0.f, 0.f, 0.f}, ispanf[3] = {0.f, 0.f, 0.f}, extra = 0.f;
-  int per[3] = {0, 0, 0};
+  LaneSys own;
+  own.per = 0;
+  own.extra = 0.f;
+  for (int k = 0; k < 3; ++k) own.span[k] = own.ispan[k] = 0.f;
   if (active) {
     a = SC.rod[p];
     me = SC.cf[p];
     const SysDev& S = sys[B.sys[a]];
     a_static = !(B.inv_mass[a] > 0.0);
     for (int k = 0; k < 3; ++k) {
-      spanf[k] = (float)S.span[k];
-      ispanf[k] = (float)(1.0 / S.span[k]);
-      per[k] = S.periodic[k];
+      own.span[k] = (float)S.span[k];
+      own.ispan[k] = (float)(1.0 / S.span[k]);
+      own.per |= S.periodic[k] ? (1 << k) : 0;
     }
-    extra = (float)delta + S.fmargin;
+    own.extra = (float)delta + S.fmargin;
     const int d0 = S.dims[0], d1 = S.dims[1];
+    W.d0 = d0;
+    W.d1 = d1;
+    W.d2 = S.dims[2];
+    W.off = S.cell_off;
+    W.py = S.periodic[1] != 0;
+    W.pz = S.periodic[2] != 0;
     const int cell = (int)SC.key[p] - S.cell_off;
-    W.cx = cell % d0;
+    const int cx = cell % d0;
     W.cy = (cell / d0) % d1;
     W.cz = cell / (d0 * d1);
     W.two = false;
-    W.xb0 = W.xb1 = 0;
+    W.xb = 0;
     if (d0 == 1) {
       W.xa0 = W.xa1 = 0;
     } else if (!S.periodic[0]) {
-      W.xa0 = W.cx > 0 ? W.cx - 1 : 0;
-      W.xa1 = W.cx < d0 - 1 ? W.cx + 1 : d0 - 1;
-    } else if (W.cx == 0) {
+      W.xa0 = cx > 0 ? cx - 1 : 0;
+      W.xa1 = cx < d0 - 1 ? cx + 1 : d0 - 1;
+    } else if (cx == 0) {
       W.xa0 = 0;
       W.xa1 = 1;
-      W.xb0 = W.xb1 = d0 - 1;
+      W.xb = d0 - 1;
       W.two = true;
-    } else if (W.cx == d0 - 1) {
+    } else if (cx == d0 - 1) {
       W.xa0 = d0 - 2;
       W.xa1 = d0 - 1;
-      W.xb0 = W.xb1 = 0;
+      W.xb = 0;
       W.two = true;
     } else {
-      W.xa0 = W.cx - 1;
-      W.xa1 = W.cx + 1;
+      W.xa0 = cx - 1;
+      W.xa1 = cx + 1;
     }
-    active = W.open_next(S, cstart);
+    active = W.open_next(cstart);
   }
-  int pending = 0;  // wave-uniform queue length
+  const int p0 = p - lane;  // the wavefront's lanes hold chunks p0 .. p0 + 63
+  int pending = 0, pend2 = 0;  // wave-uniform queue lengths
   while (__any(active)) {
-    bool hit = false;
-    int qh = 0;
-    if (active) {
-      const int q = W.q;
-      const float4 o = SC.cf[q];
-      float dxf = o.x - me.x, dyf = o.y - me.y, dzf = o.z - me.z;
-      if (per[0]) dxf -= spanf[0] * rintf(dxf * ispanf[0]);
-      if (per[1]) dyf -= spanf[1] * rintf(dyf * ispanf[1]);
-      if (per[2]) dzf -= spanf[2] * rintf(dzf * ispanf[2]);
-      const float thr = me.w + o.w + extra;
-      if (dxf * dxf + dyf * dyf + dzf * dzf < thr * thr) {
-        const int b = SC.rod[q];
-        hit = b != a && !(a_static && !(B.inv_mass[b] > 0.0));
-        qh = q;
+    float4 o[BN_UNROLL];
+    int rb[BN_UNROLL];
+    const int qs = W.q;
+    const int nh = active ? min(BN_UNROLL, W.q1 - W.q) : 0;
+#pragma unroll
+    for (int j = 0; j < BN_UNROLL; ++j) {
+      if (j < nh) {
+        o[j] = SC.cf[qs + j];
+        rb[j] = SC.rod[qs + j];
       }
-      W.q = q + 1;
-      if (W.q >= W.q1) active = W.open_next(sys[B.sys[a]], cstart);
     }
-    const unsigned long long m = __ballot(hit);
-    const int nhit = __popcll(m);
-    if (hit) {
-      ++ncand;
-      Q[pending + __popcll(m & ((1ull << lane) - 1ull))] = make_int2(p, qh);
+    if (active) {
+      W.q += nh;
+      if (W.q >= W.q1) active = W.open_next(cstart);
     }
-    pending += nhit;
-    if (pending >= 64) {
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      const int2 e = Q[pending - 64 + lane];
-      narrow_pair(B, sys, SC, rod_nchunk, e.x, e.y, delta, out, cap, counter);
-      pending -= 64;
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int j = 0; j < BN_UNROLL; ++j) {
+      bool hit = false;
+      if (j < nh) {
+        float dx = o[j].x - me.x, dy = o[j].y - me.y, dz = o[j].z - me.z;
+        if (own.per & 1) dx -= own.span[0] * rintf(dx * own.ispan[0]);
+        if (own.per & 2) dy -= own.span[1] * rintf(dy * own.ispan[1]);
+        if (own.per & 4) dz -= own.span[2] * rintf(dz * own.ispan[2]);
+        const float thr = me.w + o[j].w + own.extra;
+        if (dx * dx + dy * dy + dz * dz < thr * thr) {
+          const int b = rb[j];
+          hit = b != a && !(a_static && !(B.inv_mass[b] > 0.0));
+        }
+      }
+      const unsigned long long m = __ballot(hit);
+      if (hit) {
+        ++ncand;
+        Q[pending + __popcll(m & ((1ull << lane) - 1ull))] = make_int2(p, qs + j);
+      }
+      pending += __popcll(m);
+      if (pending >= 64) {
+        wave_sync();
+        sat_stage(Q + pending - 64, 64, p0, own, Q2, pend2, B, sys, SC, rod_nchunk, delta, out, cap, counter,
+                  nnarrow);
+        pending -= 64;
+        wave_sync();
+      }
     }
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  if (lane < pending) {
-    const int2 e = Q[lane];
+  wave_sync();
+  if (pending > 0)
+    sat_stage(Q, pending, p0, own, Q2, pend2, B, sys, SC, rod_nchunk, delta, out, cap, counter, nnarrow);
+  wave_sync();
+  if (lane < pend2) {
+    const int2 e = Q2[lane];
     narrow_pair(B, sys, SC, rod_nchunk, e.x, e.y, delta, out, cap, counter);
   }
-  // wave-level reduction of the candidate count
-  for (int off = 32; off > 0; off >>= 1) ncand += __shfl_down(ncand, off);
+  // wave-level reduction of the sphere-survivor and narrow-phase counts
+  for (int off = 32; off > 0; off >>= 1) {
+    ncand += __shfl_down(ncand, off);
+    nnarrow += __shfl_down(nnarrow, off);
+  }
   if (lane == 0 && ncand) atomicAdd(cand_counter, ncand);
+  if (lane == 0 && nnarrow) atomicAdd(cand_counter + 1, nnarrow);
 }
 
 struct ContactPtrs {
@@ -456,6 +585,7 @@ static int build_grid(rdk_ctx* ctx, const rdk_detect_params* p) {
     RDK_TRY(ensure(ctx, ctx->cid, M));
     RDK_TRY(ensure(ctx, ctx->cid_alt, M));
     RDK_TRY(ensure(ctx, ctx->scf, M));
+    RDK_TRY(ensure(ctx, ctx->sca, M));
     RDK_TRY(ensure(ctx, ctx->srod, M));
     RDK_TRY(ensure(ctx, ctx->sk, M));
   }
@@ -513,6 +643,7 @@ int detect_run(rdk_ctx* ctx, const rdk_detect_params* p) {
   SC.rod = ctx->srod.p;
   SC.k = ctx->sk.p;
   SC.cf = ctx->scf.p;
+  SC.ca = ctx->sca.p;
   {
     ProfScope ps(ctx, K_CHUNK_GATHER);
     hipLaunchKernelGGL(k_chunk_gather, dim3(grid_for(M, nb)), dim3(nb), 0, ctx->stream, B, sid, ctx->chunk_rod.p,
@@ -530,21 +661,20 @@ int detect_run(rdk_ctx* ctx, const rdk_detect_params* p) {
     RDK_HIP(hipMemsetAsync(ctx->counters.p, 0, 8 * sizeof(unsigned), ctx->stream));
     {
       ProfScope ps(ctx, K_BROAD_NARROW);
-      hipLaunchKernelGGL(k_broad_narrow, dim3(grid_for(M, nb)), dim3(nb), 0, ctx->stream, B, ctx->sys_dev.p, SC,
-                         ctx->cell_start.p, ctx->rod_nchunk.p, M, p->delta, ctx->pkeys.p, cap,
+      hipLaunchKernelGGL(k_broad_narrow, dim3(grid_for(M, BN_THREADS)), dim3(BN_THREADS), 0, ctx->stream, B,
+                         ctx->sys_dev.p, SC, ctx->cell_start.p, ctx->rod_nchunk.p, M, p->delta, ctx->pkeys.p, cap,
                          ctx->counters.p, (unsigned long long*)(ctx->counters.p + 2));
     }
-    RDK_HIP(hipMemcpyAsync(ctx->h_counters, ctx->counters.p, 4 * sizeof(unsigned), hipMemcpyDeviceToHost,
+    RDK_HIP(hipMemcpyAsync(ctx->h_counters, ctx->counters.p, 6 * sizeof(unsigned), hipMemcpyDeviceToHost,
                            ctx->stream));
     RDK_HIP(hipStreamSynchronize(ctx->stream));
     total = ctx->h_counters[0];
     ctx->last_candidates = (int64_t)(((uint64_t)ctx->h_counters[3] << 32) | ctx->h_counters[2]);
+    ctx->last_narrow = (int64_t)(((uint64_t)ctx->h_counters[5] << 32) | ctx->h_counters[4]);
     if (total <= cap) break;
     // capacity exceeded: the kernel counted every contact; regrow and rerun
     ctx->regrows++;
-    if (ctx->pair_cap_forced > 0) {
-      ctx->pair_cap_forced = 0;
-    }
+    ctx->pair_cap_forced = 0;
     cap = total + total / 4 + 64;
     if (attempt == 2) {
       ctx->err = "rdk_detect: pair capacity still exceeded after regrow";
@@ -557,6 +687,7 @@ int detect_run(rdk_ctx* ctx, const rdk_detect_params* p) {
   ctx->work[W_BODY_PASSES] += 1;
   ctx->work[W_CHUNKS] += M;
   ctx->work[W_CANDIDATES] += ctx->last_candidates;
+  ctx->work[W_NARROW] += ctx->last_narrow;
   ctx->work[W_CONTACTS] += nc;
   RDK_TRY(ensure_contacts(ctx, std::max(nc, 1)));
   if (nc == 0) return RDK_OK;

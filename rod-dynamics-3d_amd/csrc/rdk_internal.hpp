@@ -53,13 +53,14 @@ enum WorkId {
   W_STEPS = 0,
   W_BODY_PASSES,      // detection passes over all bodies
   W_CHUNKS,           // chunk entries binned
-  W_CANDIDATES,       // chunk pairs tested in the broad phase
+  W_CANDIDATES,       // chunk pairs passing the bounding-sphere test
   W_CONTACTS,         // contact records emitted
   W_MANIFOLDS,        // manifolds built
   W_SWEEP_CONTACTS,   // manifold updates in PSOR sweeps
   W_RESID_CONTACTS,   // manifold evaluations in residual passes
   W_INTEGRATE_BODIES, // body updates in integrator passes
   W_FORCE_CONTACTS,   // soft/HM force evaluations
+  W_NARROW,           // FP64 narrow-phase calls (pairs passing the separating-axis test)
   W_NUM
 };
 
@@ -124,7 +125,7 @@ struct rdk_ctx {
   rdk::DevBuf<uint32_t> ckey, ckey_alt;
   rdk::DevBuf<int> cid, cid_alt;
   rdk::DevBuf<int> cell_start;  // dense: n_cells + 1 entries
-  rdk::DevBuf<float4> scf;
+  rdk::DevBuf<float4> scf, sca;
   rdk::DevBuf<int> srod, sk;
   rdk::DevBuf<uint64_t> pkeys, pkeys_alt;
   rdk::DevBuf<unsigned int> counters;  // [0] pair count, [1..] misc
@@ -132,6 +133,7 @@ struct rdk_ctx {
   int pair_cap_forced = 0;
   int regrows = 0;
   int64_t last_candidates = 0;
+  int64_t last_narrow = 0;
 
   // ---- contacts (sorted by (a,b)) ----
   int nc = 0;

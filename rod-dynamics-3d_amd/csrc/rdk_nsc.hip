@@ -436,15 +436,17 @@ __global__ void k_warm_apply(BodyPtrs B, MfPtrs M, int i0, int i1) {
 
 struct SolveConst {
   double omega, cfm, beta, dt;
+  double stall_omega;  // omega after the first stall; 0: stop at the first stall
   int proj;
 };
 
 // Stall rule (stop_rule 1, stall_sweeps > 0): at sweep counts stall, 2 stall,
 // 4 stall, ... the residual must have dropped by at least 10 % since the
-// previous such milestone.  The first time it has not, an over-relaxed solve
-// (omega > 1) continues with omega = 1: with high friction and restitution,
-// over-relaxed PSOR can settle on a limit cycle that plain Gauss-Seidel does
-// not have.  The next stall stops the solve (reported as not converged):
+// previous such milestone.  The first time it has not, a solve whose omega is
+// above stall_omega (> 0) continues with omega = stall_omega: with high
+// friction and restitution, over-relaxed PSOR can settle on a limit cycle that
+// plain Gauss-Seidel does not have.  The next stall (or the first, when
+// stall_omega is 0) stops the solve (reported as not converged):
 // wedged, statically indeterminate groups of rods let the impulses drift along
 // a null direction of the Delassus matrix while the residual sits on a floor,
 // at any omega.  Solves that keep making progress are untouched.
@@ -842,7 +844,7 @@ __global__ void __launch_bounds__(SMALL_BLOCK) k_psor_small(BodyPtrs B, MfPtrs M
     res = block_max(r, red);
     if (stop_rule == 0 && res_ref < 1e-6) break;
     if (stop_rule == 1 && !(res <= tol) && trk.stalled(sweeps, res)) {
-      if (K.omega > 1.0) K.omega = 1.0;
+      if (K.stall_omega > 0.0 && K.omega > K.stall_omega) K.omega = K.stall_omega;
       else break;
     }
   }
@@ -1111,7 +1113,7 @@ __global__ void __launch_bounds__(WG) k_island_reg(BodyPtrs B, MfPtrs M, const S
   bool done = !has || P.max_sweeps <= 0;
   StallTrack trk;
   trk.init(P.stall);
-  SolveConst K = P.K;  // omega may drop to 1 on a stall (per island)
+  SolveConst K = P.K;  // omega may drop to stall_omega on a stall (per island)
   if (P.stop_rule == 1) {
     res = group_max<GROUP>(live ? residual_one(LB, R, 0, K, mu) : 0.0, red);
     done = done || res <= P.tol;
@@ -1136,7 +1138,7 @@ __global__ void __launch_bounds__(WG) k_island_reg(BodyPtrs B, MfPtrs M, const S
       if (P.stop_rule == 1 && res <= P.tol) done = true;
       if (sweeps >= P.max_sweeps) done = true;
       if (!done && P.stop_rule == 1 && trk.stalled(sweeps, res)) {
-        if (K.omega > 1.0) K.omega = 1.0;
+        if (K.stall_omega > 0.0 && K.omega > K.stall_omega) K.omega = K.stall_omega;
         else done = true;
       }
     }
@@ -1222,7 +1224,7 @@ __global__ void __launch_bounds__(LDS_THREADS) k_island_lds(BodyPtrs B, MfPtrs M
   double res = 0.0, res_ref = 0.0;
   StallTrack trk;
   trk.init(P.stall);
-  SolveConst K = P.K;  // omega may drop to 1 on a stall
+  SolveConst K = P.K;  // omega may drop to stall_omega on a stall
   if (P.stop_rule == 1) {
     double r = 0.0;
     for (int i = s + tid; i < e; i += LDS_THREADS) r = dmax(r, residual_one(LB, A, i, K, mu));
@@ -1245,7 +1247,7 @@ __global__ void __launch_bounds__(LDS_THREADS) k_island_lds(BodyPtrs B, MfPtrs M
     res = block_max(r, red);
     if (P.stop_rule == 0 && res_ref < 1e-6) break;
     if (P.stop_rule == 1 && !(res <= P.tol) && trk.stalled(sweeps, res)) {
-      if (K.omega > 1.0) K.omega = 1.0;
+      if (K.stall_omega > 0.0 && K.omega > K.stall_omega) K.omega = K.stall_omega;
       else break;
     }
   }
@@ -1285,7 +1287,7 @@ __global__ void __launch_bounds__(LARGE_THREADS) k_island_large(BodyPtrs B, MfPt
   double res = 0.0, res_ref = 0.0;
   StallTrack trk;
   trk.init(P.stall);
-  SolveConst K = P.K;  // omega may drop to 1 on a stall
+  SolveConst K = P.K;  // omega may drop to stall_omega on a stall
   if (P.stop_rule == 1) {
     double r = 0.0;
     for (int i = s + tid; i < e; i += LARGE_THREADS) r = dmax(r, residual_one(B, M, i, K, mu));
@@ -1308,7 +1310,7 @@ __global__ void __launch_bounds__(LARGE_THREADS) k_island_large(BodyPtrs B, MfPt
     res = block_max(r, red);
     if (P.stop_rule == 0 && res_ref < 1e-6) break;
     if (P.stop_rule == 1 && !(res <= P.tol) && trk.stalled(sweeps, res)) {
-      if (K.omega > 1.0) K.omega = 1.0;
+      if (K.stall_omega > 0.0 && K.omega > K.stall_omega) K.omega = K.stall_omega;
       else break;
     }
   }
@@ -1582,6 +1584,7 @@ static SolveConst solve_const(rdk_ctx* ctx, const rdk_nsc_params* p) {
   K.cfm = p->cfm;
   K.beta = p->position_stabilization ? p->beta * 0.01 : p->beta;
   K.dt = ctx->phys.dt;
+  K.stall_omega = p->stall_omega;
   K.proj = p->friction_proj;
   return K;
 }
@@ -1937,14 +1940,14 @@ static int global_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
       const int k = std::max(1, p->check_every);
       StallTrack trk;
       trk.init(p->stall_sweeps);
-      rdk_nsc_params q = *p;  // omega may drop to 1 on a stall
+      rdk_nsc_params q = *p;  // omega may drop to stall_omega on a stall
       RDK_TRY(nsc_residual_run(ctx, p, &res, &res_ref));
       while (!(res <= p->tol) && sweeps < p->max_sweeps) {
         RDK_TRY(run_sweeps(ctx, &q, k));
         sweeps += k;
         RDK_TRY(nsc_residual_run(ctx, p, &res, &res_ref));
         if (!(res <= p->tol) && trk.stalled(sweeps, res)) {
-          if (q.omega > 1.0) q.omega = 1.0;
+          if (p->stall_omega > 0.0 && q.omega > p->stall_omega) q.omega = p->stall_omega;
           else break;
         }
       }

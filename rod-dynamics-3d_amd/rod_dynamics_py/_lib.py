@@ -40,7 +40,7 @@ class NscParams(C.Structure):
                 ("max_sweeps", C.c_int), ("check_every", C.c_int), ("friction_proj", C.c_int),
                 ("position_stabilization", C.c_int), ("warm_mode", C.c_int), ("stop_rule", C.c_int),
                 ("warm_walls", C.c_int), ("stall_sweeps", C.c_int), ("position_iters", C.c_int),
-                ("position_psor_iters", C.c_int), ("slop", C.c_double)]
+                ("position_psor_iters", C.c_int), ("slop", C.c_double), ("stall_omega", C.c_double)]
 
 
 class SoftParams(C.Structure):
@@ -184,9 +184,10 @@ def system_desc(lo, hi, periodic, floor_wall=False, ceil_wall=False, mu=0.0):
 
 def nsc_params(beta=0.0, cfm=0.0, omega=1.0, tol=1e-8, max_sweeps=10000, check_every=1, friction_proj=0,
                position_stabilization=0, warm_mode=1, stop_rule=1, warm_walls=0, stall_sweeps=0, position_iters=5,
-               position_psor_iters=50, slop=1e-4):
+               position_psor_iters=50, slop=1e-4, stall_omega=1.0):
     p = NscParams()
     p.position_iters, p.position_psor_iters, p.slop = position_iters, position_psor_iters, slop
+    p.stall_omega = stall_omega
     p.warm_walls = warm_walls
     p.stall_sweeps = stall_sweeps
     p.beta, p.cfm, p.omega, p.tol = beta, cfm, omega, tol
@@ -420,7 +421,7 @@ class Context:
         arr = (C.c_int64 * 16)()
         self._check(self._L.rdk_work_counters(self._h, arr, 16))
         names = ["steps", "body_passes", "chunks", "candidates", "contacts", "manifolds", "sweep_contacts",
-                 "resid_contacts", "integrate_bodies", "force_contacts"]
+                 "resid_contacts", "integrate_bodies", "force_contacts", "narrow_calls"]
         return {nm: arr[i] for i, nm in enumerate(names)}
 
     def affine_scale(self, s):

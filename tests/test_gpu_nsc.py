@@ -289,19 +289,20 @@ def test_stall_rule(gpu_ctx_factory):
 
 def test_stall_rule_omega_fallback_breaks_limit_cycle(gpu_ctx_factory):
     """The captured limit cycle: without the stall rule the over-relaxed solve
-    ends at max_sweeps on the cycle; with it, omega drops to 1 at the first
-    stall and the solve converges (the residual is the oracle's)."""
+    ends at max_sweeps on the cycle; with it, omega drops to stall_omega = 1
+    at the first stall and the solve converges (the residual is the oracle's);
+    with stall_omega = 0 the solve stops at the first stall instead."""
     import os
 
     d = np.load(os.path.join(os.path.dirname(__file__), "golden", "c4_limit_cycle.npz"))
     out = {}
-    for stall in (0, 64):
+    for stall, stall_omega in ((0, 1.0), (64, 1.0), (64, 0.0)):
         ctx = gpu_ctx_factory()
         ctx.set_bodies(d["x"], d["q"], d["v"], d["w"], d["r"], d["h"], d["mass"], d["rest"])
         ctx.set_systems([0, len(d["r"])], [L.system_desc(d["lo"], d["hi"], (1, 1, 1), 0, 0, float(d["mu"]))])
         ctx.set_physics(L.physics((0, 0, 0), 1e-3, 0, 0, 1e30))
         p = L.nsc_params(beta=0.0, cfm=0.0, omega=1.3, tol=1e-8, max_sweeps=3000, check_every=4, friction_proj=1,
-                         warm_mode=1, stall_sweeps=stall)
+                         warm_mode=1, stall_sweeps=stall, stall_omega=stall_omega)
         assert ctx.detect(L.detect_params()) == 11
         q0 = L.nsc_params(warm_mode=0)
         ctx.nsc_build(q0)
@@ -309,8 +310,8 @@ def test_stall_rule_omega_fallback_breaks_limit_cycle(gpu_ctx_factory):
         assert np.array_equal(ab, d["ab"])
         ctx.nsc_set_lambda(d["lam0"])
         st = ctx.nsc_solve(p, allow_noconv=True)
-        out[stall] = st
-        if stall:
+        out[(stall, stall_omega)] = st
+        if stall and stall_omega > 0:
             _, _, lam = ctx.nsc_manifolds(11)
             s1 = ctx.download_state()
             W = O.World(d["x"], d["q"], s1["v"], s1["w"], d["r"], d["h"], d["mass"], d["rest"])
@@ -318,5 +319,10 @@ def test_stall_rule_omega_fallback_breaks_limit_cycle(gpu_ctx_factory):
             W.set_physics((0, 0, 0), 1e-3)
             W.set_manifolds(rec, ab, lam)
             assert W.residual(O.nsc_params(mu=float(d["mu"]), beta=0.0, cfm=0.0, omega=1.3, friction_proj=1)) <= 1e-8
-    assert out[0]["residual"] > 0.1 and out[0]["sweeps"] >= 3000, out[0]
-    assert out[64]["residual"] <= 1e-8 and out[64]["sweeps"] < 3000, out[64]
+    cyc, fb, stop = out[(0, 1.0)], out[(64, 1.0)], out[(64, 0.0)]
+    assert cyc["residual"] > 0.1 and cyc["sweeps"] >= 3000, cyc
+    assert fb["residual"] <= 1e-8 and fb["sweeps"] < 3000, fb
+    # the same trajectory up to the milestone (64 * 2^k) at which the
+    # fallback solve switched omega; this one stops there
+    assert stop["residual"] > 0.1 and stop["sweeps"] in (128, 256, 512, 1024, 2048), stop
+    assert stop["sweeps"] < fb["sweeps"]

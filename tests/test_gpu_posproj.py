@@ -73,6 +73,7 @@ def test_several_passes_reduce_penetration_like_the_oracle(gpu_ctx_factory):
     s0, _ = pen_gpu()
     p = L.nsc_params(omega=1.0, cfm=0.0, position_stabilization=1, position_iters=5, position_psor_iters=50, slop=1e-4)
     ctx.nsc_project(L.detect_params(0.0, 1.0, 1.0), p)
+    W.detect(0, 0.0)  # the oracle projects over the current contact list; the GPU entry point detects itself
     W.project(O.nsc_params(omega=1.0, cfm=0.0, position_stabilization=1, position_iters=5, position_psor_iters=50,
                            slop=1e-4))
     sg, mg = pen_gpu()
