@@ -50,7 +50,7 @@ BYTES = {
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 # (FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0, from a calibrated 1 GiB copy)
 # for (workload, kernel); recorded with the command in profiles/
-TRAFFIC = {}
+TRAFFIC = {("c2", "broad_narrow"): 128.2e6 + 0.21e6, ("c4", "broad_narrow"): 479.4e6 + 6.42e6}
 
 
 def parse():

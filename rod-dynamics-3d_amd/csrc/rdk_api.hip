@@ -137,7 +137,7 @@ int rdk_destroy(rdk_ctx* ctx) {
   release(ctx->csr_val_alt); release(ctx->body_start); release(ctx->d_cstart); release(ctx->small_out);
   release(ctx->uf_parent); release(ctx->uf_label); release(ctx->sbody); release(ctx->sbody_alt);
   release(ctx->iflag); release(ctx->ipos); release(ctx->istart); release(ctx->ilists); release(ctx->bseg);
-  release(ctx->icnt); release(ctx->blab); release(ctx->blab_alt); release(ctx->istats); release(ctx->islot); release(ctx->prev_keys); release(ctx->prev_wkeys); release(ctx->resid);
+  release(ctx->icnt); release(ctx->blab); release(ctx->blab_alt); release(ctx->istats); release(ctx->hcs); release(ctx->hres); release(ctx->islot); release(ctx->prev_keys); release(ctx->prev_wkeys); release(ctx->resid);
   release(ctx->hkeys); release(ctx->hkeys_new); release(ctx->hlast); release(ctx->hlast_new);
   release(ctx->hflag); release(ctx->hscan); release(ctx->pe); release(ctx->temp); release(ctx->red);
   if (ctx->h_counters) (void)hipHostFree(ctx->h_counters);

@@ -161,6 +161,8 @@ struct rdk_ctx {
   rdk::DevBuf<int> uf_parent, uf_label, sbody, sbody_alt, iflag, ipos, istart, ilists, bseg, icnt;
   rdk::DevBuf<uint32_t> blab, blab_alt;
   rdk::DevBuf<char> istats;
+  rdk::DevBuf<int> hcs;       // colour starts of the huge islands
+  rdk::DevBuf<double> hres;   // residual and last-sweep change of the huge-island solve
   rdk::DevBuf<int2> islot;  // local body slots of large-island manifolds
   // position projection
   rdk::DevBuf<int> ppa, ppb, pflag, ppos;

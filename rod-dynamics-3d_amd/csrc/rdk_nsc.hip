@@ -645,11 +645,11 @@ __device__ __forceinline__ double residual_one(const BodyPtrs& B, MP& M, int i, 
   return r;
 }
 
-__global__ void __launch_bounds__(256) k_residual(BodyPtrs B, MfPtrs M, const SysDev* __restrict__ sys, int nm,
-                                                  SolveConst K, double* __restrict__ out) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_residual(BodyPtrs B, MfPtrs M, const SysDev* __restrict__ sys, int i0,
+                                                  int i1, SolveConst K, double* __restrict__ out) {
+  int i = i0 + blockIdx.x * blockDim.x + threadIdx.x;
   double r = 0.0;
-  if (i < nm) {
+  if (i < i1) {
     r = residual_one(B, M, i, K, body_mu(B, sys, M.a[i]));
     if (r != r) r = __longlong_as_double(0x7ff8000000000000ll);  // keep NaN visible
   }
@@ -991,7 +991,6 @@ struct IslandSolve {
   int max_sweeps, check_every, stop_rule, warm, stall;
 };
 
-constexpr int LARGE_THREADS = 256;
 
 // One manifold held in registers (every field index is a compile-time
 // constant after inlining) with its bodies as local LDS slots.
@@ -1261,60 +1260,20 @@ __global__ void __launch_bounds__(LDS_THREADS) k_island_lds(BodyPtrs B, MfPtrs M
   if (tid == 0) island_report(S, sweeps, e - s, res, res_ref, P.stop_rule == 1 && !(res <= P.tol));
 }
 
-// large islands: one 256-lane workgroup on global memory
-__global__ void __launch_bounds__(LARGE_THREADS) k_island_large(BodyPtrs B, MfPtrs M, const SysDev* __restrict__ sys,
-                                                               const int* __restrict__ istart,
-                                                               const uint32_t* __restrict__ skey,
-                                                               const int* __restrict__ list, IslandSolve P,
-                                                               IslandStats* S) {
-  __shared__ double red[LARGE_THREADS / 64];
-  __shared__ int cs[129];
-  const int k = list[blockIdx.x];
-  const int s = istart[k], e = istart[k + 1];
-  const int tid = threadIdx.x;
+// Colour starts of listed islands (the huge class), for the host-driven
+// colour-by-colour solve: out[k * HUGE_CS + c] = first manifold of colour c,
+// c = 0 .. ncol <= 128 (the last is the island end), out[k * HUGE_CS + 129] = ncol.
+constexpr int HUGE_CS = 130;
+__global__ void k_island_colours(const int* __restrict__ istart, const uint32_t* __restrict__ skey,
+                                 const int* __restrict__ list, int nlist, int* __restrict__ out) {
+  const int k = blockIdx.x;
+  if (k >= nlist) return;
+  const int isl = list[k];
+  const int s = istart[isl], e = istart[isl + 1];
   const uint32_t base = skey[s] & ~127u;
   const int ncol = (int)(skey[e - 1] & 127u) + 1;
-  for (int c = tid; c <= ncol; c += LARGE_THREADS) cs[c] = lb_u32(skey, s, e, base | (uint32_t)c);
-  __syncthreads();
-  const double mu = sys[B.sys[M.a[s]]].mu;
-  if (P.warm) {
-    for (int c = 0; c < ncol; ++c) {
-      for (int i = cs[c] + tid; i < cs[c + 1]; i += LARGE_THREADS) warm_apply_one(B, M, i);
-      __syncthreads();
-    }
-  }
-  int sweeps = 0;
-  double res = 0.0, res_ref = 0.0;
-  StallTrack trk;
-  trk.init(P.stall);
-  SolveConst K = P.K;  // omega may drop to stall_omega on a stall
-  if (P.stop_rule == 1) {
-    double r = 0.0;
-    for (int i = s + tid; i < e; i += LARGE_THREADS) r = dmax(r, residual_one(B, M, i, K, mu));
-    res = block_max(r, red);
-  }
-  const int kk = P.stop_rule == 1 ? P.check_every : 1;
-  while (sweeps < P.max_sweeps && !(P.stop_rule == 1 && res <= P.tol)) {
-    double d = 0.0;
-    for (int it = 0; it < kk; ++it) {
-      d = 0.0;
-      for (int c = 0; c < ncol; ++c) {
-        for (int i = cs[c] + tid; i < cs[c + 1]; i += LARGE_THREADS) d = dmax(d, psor_update(B, M, i, K, mu));
-        __syncthreads();
-      }
-    }
-    sweeps += kk;
-    res_ref = block_max(d, red);
-    double r = 0.0;
-    for (int i = s + tid; i < e; i += LARGE_THREADS) r = dmax(r, residual_one(B, M, i, K, mu));
-    res = block_max(r, red);
-    if (P.stop_rule == 0 && res_ref < 1e-6) break;
-    if (P.stop_rule == 1 && !(res <= P.tol) && trk.stalled(sweeps, res)) {
-      if (K.stall_omega > 0.0 && K.omega > K.stall_omega) K.omega = K.stall_omega;
-      else break;
-    }
-  }
-  if (tid == 0) island_report(S, sweeps, e - s, res, res_ref, P.stop_rule == 1 && !(res <= P.tol));
+  for (int c = threadIdx.x; c <= ncol; c += blockDim.x) out[k * HUGE_CS + c] = lb_u32(skey, s, e, base | (uint32_t)c);
+  if (threadIdx.x == 0) out[k * HUGE_CS + 129] = ncol;
 }
 
 // ---------------------------------------------------------------------------
@@ -1617,7 +1576,7 @@ int nsc_residual_run(rdk_ctx* ctx, const rdk_nsc_params* p, double* res, double*
     RDK_HIP(hipMemsetAsync(ctx->resid.p, 0, sizeof(double), ctx->stream));
     ProfScope ps(ctx, K_RESIDUAL);
     hipLaunchKernelGGL(k_residual, dim3(grid_for(nm, 256)), dim3(256), 0, ctx->stream, body_ptrs(ctx),
-                       mf_ptrs(ctx, true), ctx->sys_dev.p, nm, solve_const(ctx, p), ctx->resid.p);
+                       mf_ptrs(ctx, true), ctx->sys_dev.p, 0, nm, solve_const(ctx, p), ctx->resid.p);
   }
   if (nm > 0) {
     RDK_HIP(hipMemcpyAsync(h, ctx->resid.p, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -1680,6 +1639,158 @@ static int store_warm_cache(rdk_ctx* ctx, const rdk_nsc_params* p) {
 // Island-parallel solve.  Sets *handled = false (and does nothing else after
 // colouring) when some island is too large for one workgroup; the caller then
 // runs the global coloured path.
+// Huge islands (more bodies than fit in LDS): solved colour by colour with
+// one launch per colour spread over the whole GPU, in the order the island
+// kernels sweep (the same serial colour-order Gauss-Seidel), with the stall
+// rule and residual checks of the island kernels evaluated on the host.  A
+// single workgroup on such an island is bound by one CU's latency and L2
+// bandwidth; one launch per colour uses as many CUs as the colour has
+// manifolds / 256.
+struct HugeOut {
+  int max_sweeps = 0, nonconv = 0;
+  int64_t manifold_sweeps = 0;
+  double res = 0.0, res_ref = 0.0;
+};
+
+__global__ void k_zero_doubles(double* __restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+static int huge_islands_solve(rdk_ctx* ctx, const IslandSolve& P, const uint32_t* skey, const int* list, int nh,
+                              hipStream_t st, HugeOut& out) {
+  RDK_TRY(ensure(ctx, ctx->hcs, (size_t)nh * HUGE_CS));
+  RDK_TRY(ensure(ctx, ctx->hres, 2));
+  {
+    ProfScope ps(ctx, K_ISL_HUGE);
+    hipLaunchKernelGGL(k_island_colours, dim3(nh), dim3(128), 0, st, ctx->istart.p, skey, list, nh, ctx->hcs.p);
+  }
+  std::vector<int> hcs((size_t)nh * HUGE_CS);
+  RDK_HIP(hipMemcpyAsync(hcs.data(), ctx->hcs.p, hcs.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+  RDK_HIP(hipStreamSynchronize(st));
+  BodyPtrs B = body_ptrs(ctx);
+  MfPtrs Mc = mf_ptrs(ctx, true);
+  const int nb = 256;
+  double* dres = ctx->hres.p;
+  double* dref = ctx->hres.p + 1;
+  const int kk = P.stop_rule == 1 ? P.check_every : 1;
+  // Graph replay of one check block (kk sweeps, then the residual) when the
+  // per-kernel timers are off (they record events on the main stream).  The
+  // counters are cleared by a kernel: no memset nodes in the graph.
+  const bool graphs = !ctx->profiling;
+  for (int h = 0; h < nh; ++h) {
+    const int* cs = hcs.data() + (size_t)h * HUGE_CS;
+    const int ncol = cs[129];
+    const int s = cs[0], e = cs[ncol];
+    int nlaunch = 0;
+    auto enqueue_block = [&](const SolveConst& Kc, bool sweep) {
+      nlaunch = 0;
+      if (sweep) {
+        for (int it = 0; it < kk; ++it) {
+          if (it == kk - 1) {
+            hipLaunchKernelGGL(k_zero_doubles, dim3(1), dim3(64), 0, st, ctx->hres.p, 2);
+            ++nlaunch;
+          }
+          for (int c = 0; c < ncol; ++c) {
+            if (cs[c + 1] <= cs[c]) continue;
+            ProfScope ps(ctx, K_ISL_HUGE);
+            hipLaunchKernelGGL(k_psor_sweep, dim3(grid_for(cs[c + 1] - cs[c], nb)), dim3(nb), 0, st, B, Mc,
+                               ctx->sys_dev.p, cs[c], cs[c + 1], Kc, dref);
+            ++nlaunch;
+          }
+        }
+      } else {
+        hipLaunchKernelGGL(k_zero_doubles, dim3(1), dim3(64), 0, st, ctx->hres.p, 2);
+        ++nlaunch;
+      }
+      ProfScope ps(ctx, K_ISL_HUGE);
+      hipLaunchKernelGGL(k_residual, dim3(grid_for(e - s, nb)), dim3(nb), 0, st, B, Mc, ctx->sys_dev.p, s, e, Kc,
+                         dres);
+      ++nlaunch;
+    };
+    auto read_back = [&](double* res, double* ref) -> int {
+      double hv[2];
+      RDK_HIP(hipMemcpyAsync(hv, dres, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+      RDK_HIP(hipStreamSynchronize(st));
+      *res = hv[0];
+      if (ref) *ref = hv[1];
+      ctx->work[W_RESID_CONTACTS] += e - s;
+      return RDK_OK;
+    };
+    if (P.warm) {
+      for (int c = 0; c < ncol; ++c) {
+        if (cs[c + 1] <= cs[c]) continue;
+        ProfScope ps(ctx, K_ISL_HUGE);
+        hipLaunchKernelGGL(k_warm_apply, dim3(grid_for(cs[c + 1] - cs[c], nb)), dim3(nb), 0, st, B, Mc, cs[c],
+                           cs[c + 1]);
+      }
+    }
+    int sweeps = 0;
+    double res = 0.0, res_ref = 0.0;
+    StallTrack trk;
+    trk.init(P.stall);
+    SolveConst K = P.K;  // omega may drop to stall_omega on a stall
+    if (P.stop_rule == 1) {
+      enqueue_block(K, false);
+      RDK_TRY(read_back(&res, nullptr));
+    }
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    auto drop_graph = [&]() {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      if (graph) (void)hipGraphDestroy(graph);
+      exec = nullptr;
+      graph = nullptr;
+    };
+    int rc = RDK_OK;
+    while (rc == RDK_OK && sweeps < P.max_sweeps && !(P.stop_rule == 1 && res <= P.tol)) {
+      if (graphs) {
+        if (!exec) {
+          if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            rc = RDK_ERR_HIP;
+            break;
+          }
+          enqueue_block(K, true);
+          if (hipStreamEndCapture(st, &graph) != hipSuccess ||
+              hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+            rc = RDK_ERR_HIP;
+            break;
+          }
+        }
+        if (hipGraphLaunch(exec, st) != hipSuccess) {
+          rc = RDK_ERR_HIP;
+          break;
+        }
+        ctx->k_launches[K_ISL_HUGE] += nlaunch;
+      } else {
+        enqueue_block(K, true);
+      }
+      sweeps += kk;
+      if ((rc = read_back(&res, &res_ref)) != RDK_OK) break;
+      if (P.stop_rule == 0 && res_ref < 1e-6) break;
+      if (P.stop_rule == 1 && !(res <= P.tol) && trk.stalled(sweeps, res)) {
+        if (K.stall_omega > 0.0 && K.omega > K.stall_omega) {
+          K.omega = K.stall_omega;
+          drop_graph();  // the sweeps are recaptured with the new omega
+        } else {
+          break;
+        }
+      }
+    }
+    drop_graph();
+    if (rc != RDK_OK) {
+      ctx->err = "huge-island solve: graph capture or launch failed";
+      return rc;
+    }
+    out.max_sweeps = std::max(out.max_sweeps, sweeps);
+    out.manifold_sweeps += (int64_t)sweeps * (e - s);
+    out.res = std::max(out.res, res);
+    out.res_ref = std::max(out.res_ref, res_ref);
+    if (P.stop_rule == 1 && !(res <= P.tol)) out.nonconv++;
+  }
+  return RDK_OK;
+}
+
 static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* st, bool* handled, int* status) {
   *handled = false;
   const int nm = ctx->nm, n = ctx->n, nb = 256;
@@ -1775,21 +1886,20 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
   RDK_HIP(hipMemsetAsync(S, 0, sizeof(IslandStats), ctx->stream));
   BodyPtrs B = body_ptrs(ctx);
   MfPtrs Mc = mf_ptrs(ctx, true);
-  // tiny islands run on the side stream, concurrently with the other classes
-  const bool side = cnt[0] > 0 && (cnt[1] > 0 || cnt[2] > 0 || cnt[3] > 0 || cnt[4] > 0) && ctx->stream2 &&
-                    !ctx->profiling;
+  // Tiny islands, and the host-driven solve of huge islands, run on the side
+  // stream, concurrently with the other classes (islands share no body).
+  const bool side = (cnt[0] > 0 || cnt[4] > 0) && (cnt[1] > 0 || cnt[2] > 0 || cnt[3] > 0 || (cnt[0] > 0 && cnt[4] > 0)) &&
+                    ctx->stream2 && !ctx->profiling;
+  hipStream_t s_side = side ? ctx->stream2 : ctx->stream;
+  if (side) {
+    RDK_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
+    RDK_HIP(hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
+  }
   if (cnt[0] > 0) {
-    hipStream_t s_tiny = ctx->stream;
-    if (side) {
-      RDK_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
-      RDK_HIP(hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
-      s_tiny = ctx->stream2;
-    }
     ProfScope ps(ctx, K_ISL_TINY);
     constexpr int ipb = ISL_WG0 / ISL_G0;
-    hipLaunchKernelGGL((k_island_reg<ISL_G0, ISL_WG0>), dim3(grid_for(cnt[0], ipb)), dim3(ISL_WG0), 0, s_tiny, B, Mc,
+    hipLaunchKernelGGL((k_island_reg<ISL_G0, ISL_WG0>), dim3(grid_for(cnt[0], ipb)), dim3(ISL_WG0), 0, s_side, B, Mc,
                        ctx->sys_dev.p, ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p, cnt[0], P, S);
-    if (side) RDK_HIP(hipEventRecord(ctx->ev_join, ctx->stream2));
   }
   if (cnt[1] > 0) {
     ProfScope ps(ctx, K_ISL_SMALL);
@@ -1808,12 +1918,12 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
     hipLaunchKernelGGL(k_island_lds, dim3(cnt[3]), dim3(LDS_THREADS), 0, ctx->stream, B, Mc, ctx->sys_dev.p,
                        ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p + 3 * nisl, P, S, ctx->islot.p);
   }
-  if (cnt[4] > 0) {
-    ProfScope ps(ctx, K_ISL_HUGE);
-    hipLaunchKernelGGL(k_island_large, dim3(cnt[4]), dim3(LARGE_THREADS), 0, ctx->stream, B, Mc, ctx->sys_dev.p,
-                       ctx->istart.p, skey, ctx->ilists.p + 4 * nisl, P, S);
+  HugeOut hu;
+  if (cnt[4] > 0) RDK_TRY(huge_islands_solve(ctx, P, skey, ctx->ilists.p + 4 * nisl, cnt[4], s_side, hu));
+  if (side) {
+    RDK_HIP(hipEventRecord(ctx->ev_join, ctx->stream2));
+    RDK_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
   }
-  if (side) RDK_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
   {
     ProfScope ps(ctx, K_PERMUTE);
     hipLaunchKernelGGL(k_unpermute_lambda, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, Mc, mf_ptrs(ctx, false),
@@ -1832,12 +1942,13 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
   ctx->have_coloring = false;  // the solve-ordered copy is island-ordered
   for (int c = 0; c < 3; ++c) ctx->island_counts[c] = cnt[c];
   ctx->island_counts[3] = cnt[3] + cnt[4];
-  ctx->work[W_SWEEP_CONTACTS] += (int64_t)hs.manifold_sweeps;
-  *status = hs.nonconv > 0 ? RDK_ERR_NOCONV : RDK_OK;
+  ctx->work[W_SWEEP_CONTACTS] += (int64_t)hs.manifold_sweeps + hu.manifold_sweeps;
+  *status = hs.nonconv + hu.nonconv > 0 ? RDK_ERR_NOCONV : RDK_OK;
   if (st) {
-    st->sweeps = hs.max_sweeps;
-    st->residual = hs.res;
-    st->residual_ref = hs.res_ref;
+    st->sweeps = std::max(hs.max_sweeps, hu.max_sweeps);
+    const double rs = hs.res, rh = hu.res;
+    st->residual = (rs != rs || rh != rh) ? (rs != rs ? rs : rh) : std::max(rs, rh);
+    st->residual_ref = std::max(hs.res_ref, hu.res_ref);
   }
   *handled = true;
   return RDK_OK;

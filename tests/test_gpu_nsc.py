@@ -211,10 +211,69 @@ def big_clusters(nclust=3, k=160, seed=91, mu=0.5):
                 side=30.0 * nclust, periodic=(1, 1, 0), walls=True, mu=mu)
 
 
+def log_pile(ny=100, nz=30, seed=97, mu=0.5):
+    """Rods along x stacked on a (y, z) lattice with spacing 0.78 < 2r = 0.8
+    and small random tilts: every rod overlaps its four lattice neighbours, so
+    the pile is one island of ny * nz bodies (more than the LDS island class
+    holds) and about 2 ny nz manifolds, plus floor contacts of the bottom layer."""
+    rng = np.random.default_rng(seed)
+    jy, jz = np.meshgrid(np.arange(ny), np.arange(nz), indexing="ij")
+    n = ny * nz
+    x = np.stack([30.0 + rng.uniform(-0.5, 0.5, n), 5.0 + 0.78 * jy.ravel(), 0.38 + 0.78 * jz.ravel()], 1)
+    # axis +Y rotated onto +X (quaternion about z by -90 degrees), then tilted slightly
+    base = np.array([np.sqrt(0.5), 0.0, 0.0, -np.sqrt(0.5)])
+    tilt = rng.normal(size=(n, 3)) * 0.02
+    ang = np.linalg.norm(tilt, axis=1)
+    dq = np.column_stack([np.cos(ang / 2), tilt * (np.sin(ang / 2) / ang)[:, None]])
+    w0, v0 = dq[:, :1], dq[:, 1:]
+    bw, bv = base[0], base[1:]
+    q = np.column_stack([w0[:, 0] * bw - v0 @ bv, w0 * bv + bw * v0 + np.cross(v0, bv)])
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    side = 0.78 * ny + 12.0
+    return dict(x=x, q=q, v=0.1 * rng.normal(size=(n, 3)), w=0.1 * rng.normal(size=(n, 3)), r=np.full(n, 0.4),
+                h=np.full(n, 2.0), mass=np.full(n, 1.0), rest=np.full(n, 0.0), side=side, periodic=(1, 1, 0),
+                walls=True, mu=mu)
+
+
+def pile_and_clusters():
+    """A huge island (log pile) next to clusters of every smaller class, so the
+    huge-island solve runs beside the island kernels on the side stream."""
+    P = log_pile(ny=70, nz=40, seed=99)
+    C = clusters(nclust=30, seed=78)
+    C["x"] = C["x"] + np.array([40.0, 0.0, 0.0])
+    out = {k: np.concatenate([P[k], C[k]]) for k in ("x", "q", "v", "w", "r", "h", "mass", "rest")}
+    out.update(side=max(P["side"], C["x"][:, 0].max() + 10.0, C["x"][:, 1].max() + 10.0), periodic=(1, 1, 0),
+               walls=True, mu=0.5)
+    return out
+
+
+def largest_island(ab, n):
+    """(manifolds, bodies) of the largest rod-rod island, by union-find on the host."""
+    par = np.arange(n)
+
+    def find(i):
+        while par[i] != i:
+            par[i] = par[par[i]]
+            i = par[i]
+        return i
+
+    rr = ab[ab[:, 1] >= 0]
+    for a, b in rr:
+        ra, rb = find(a), find(b)
+        if ra != rb:
+            par[ra] = rb
+    lab = np.array([find(a) for a in rr[:, 0]])
+    u, cnt = np.unique(lab, return_counts=True)
+    big = u[np.argmax(cnt)]
+    bodies = np.unique(rr[lab == big].ravel())
+    return int(cnt.max()), int(len(bodies))
+
+
 @pytest.mark.parametrize("proj", [0, 1])
-@pytest.mark.parametrize("scene", ["clusters", "big_clusters"])
+@pytest.mark.parametrize("scene", ["clusters", "big_clusters", "log_pile", "pile_and_clusters"])
 def test_island_sweeps_bitwise_equal_serial_sweeps_in_solve_order(gpu_ctx_factory, proj, scene):
-    B = clusters() if scene == "clusters" else big_clusters()
+    B = {"clusters": clusters, "big_clusters": big_clusters, "log_pile": log_pile,
+         "pile_and_clusters": pile_and_clusters}[scene]()
     print(f"island sweeps {scene} proj={proj}", flush=True)
     ctx = gpu_ctx_factory()
     gpu_load(ctx, B)
@@ -228,8 +287,13 @@ def test_island_sweeps_bitwise_equal_serial_sweeps_in_solve_order(gpu_ctx_factor
     nisl, counts = ctx.island_info()
     if scene == "clusters":
         assert counts[0] > 0 and counts[1] > 0 and counts[2] > 0, counts  # tiny, small and medium islands present
-    else:
+    elif scene == "big_clusters":
         assert counts[3] > 0, counts  # large islands (bodies in LDS)
+    else:
+        m, nb = largest_island(ab, len(B["r"]))
+        assert nb > 2560 and m <= 8192 and counts[3] >= 1, (m, nb, counts)  # a huge island (colour-by-colour path)
+        if scene == "pile_and_clusters":
+            assert counts[0] > 0 and counts[1] > 0, counts
     order = ctx.solve_order(nm)
     _, _, lam_g = ctx.nsc_manifolds(nm)
     s1 = ctx.download_state()
@@ -326,3 +390,63 @@ def test_stall_rule_omega_fallback_breaks_limit_cycle(gpu_ctx_factory):
     # fallback solve switched omega; this one stops there
     assert stop["residual"] > 0.1 and stop["sweeps"] in (128, 256, 512, 1024, 2048), stop
     assert stop["sweeps"] < fb["sweeps"]
+
+
+def test_huge_island_solve_reaches_residual_checked_by_oracle(gpu_ctx_factory):
+    """The host-driven colour-by-colour solve of a huge island (residual
+    checks, stall rule) stops at the tolerance; the oracle's residual of the
+    GPU impulses confirms it, and the impulses are admissible."""
+    B = log_pile(ny=70, nz=40, seed=98)
+    ctx = gpu_ctx_factory()
+    gpu_load(ctx, B)
+    p = L.nsc_params(beta=0.0, cfm=0.0, omega=1.3, tol=1e-6, max_sweeps=20000, check_every=4, friction_proj=1,
+                     warm_mode=0, stall_sweeps=1000, stall_omega=1.0)
+    ctx.detect(L.detect_params())
+    nm = ctx.nsc_build(p)
+    rec, ab, lam0 = ctx.nsc_manifolds(nm)
+    m, nb = largest_island(ab, len(B["r"]))
+    assert nb > 2560 and m <= 8192, (m, nb)
+    s0 = ctx.download_state()
+    st = ctx.nsc_solve(p)
+    assert st["residual"] <= 1e-6 and st["sweeps"] > 4, st
+    _, _, lam = ctx.nsc_manifolds(nm)
+    s1 = ctx.download_state()
+    W = oracle_load(B)
+    W.set_state(v=s0["v"], w=s0["w"])
+    W.set_manifolds(rec, ab, lam)
+    W.apply_warm()
+    so = W.state()
+    assert np.allclose(so["v"], s1["v"], rtol=0, atol=1e-10)
+    W.set_state(v=s1["v"], w=s1["w"])
+    op = O.nsc_params(mu=B["mu"], beta=0.0, cfm=0.0, omega=1.3, friction_proj=1)
+    assert W.residual(op) <= 1e-6 * (1 + 1e-6)
+    assert lam[:, 0].min() >= 0.0
+    assert np.all(np.hypot(lam[:, 1], lam[:, 2]) <= B["mu"] * lam[:, 0] * (1 + 1e-12) + 1e-15)
+
+
+def test_huge_island_graph_replays_bitwise_equal_serial_sweeps(gpu_ctx_factory):
+    """Check blocks of four sweeps replayed from a captured graph (three
+    replays, a host read-back between them) equal twelve serial sweeps of the
+    oracle in the same order, bit for bit."""
+    B = log_pile(ny=70, nz=40, seed=100)
+    ctx = gpu_ctx_factory()
+    gpu_load(ctx, B)
+    p = L.nsc_params(beta=0.1, cfm=1e-4, omega=1.3, tol=0.0, max_sweeps=12, check_every=4, friction_proj=1,
+                     warm_mode=0)
+    ctx.detect(L.detect_params())
+    nm = ctx.nsc_build(p)
+    rec, ab, lam0 = ctx.nsc_manifolds(nm)
+    s0 = ctx.download_state()
+    st = ctx.nsc_solve(p, allow_noconv=True)
+    assert st["sweeps"] == 12, st
+    order = ctx.solve_order(nm)
+    _, _, lam_g = ctx.nsc_manifolds(nm)
+    s1 = ctx.download_state()
+    W = oracle_load(B)
+    W.set_state(v=s0["v"], w=s0["w"])
+    W.set_manifolds(rec, ab, lam0)
+    W.sweeps(O.nsc_params(mu=B["mu"], beta=0.1, cfm=1e-4, omega=1.3, friction_proj=1), 12, order)
+    _, _, lam_o = W.manifolds()
+    so = W.state()
+    assert np.array_equal(lam_g, lam_o)
+    assert np.array_equal(s1["v"], so["v"]) and np.array_equal(s1["w"], so["w"])
