@@ -21,6 +21,9 @@ preparation), then 2000 timed steps at dt=5e-5 with an affine strain rate 1e-3.
 --workload c4: 256 independent periodic hard-contact packings of uneven size
 (N 500..20000, L/d 4..64, mu 0..1) stepped together, one launch per stage for
 all systems, chunk length autotuned in the warm-up, 1000 steps.
+--workload c5: the settling protocol at 2e5 rods from an evaluation seed, then
+the settled bed solved from the cold, previous-step and learned starts (the
+learned start trained on other seeds).
 """
 import argparse
 import json
@@ -35,7 +38,7 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per unit (ideal DRAM model, FP64 = 8 B, index = 4 B)
 BYTES = {
     "broad_narrow": ("bodies", 72),      # x, q, r, h of every body once per pass
-    "psor_sweep": ("sweep_contacts", 568),  # manifold 328 + lambda r/w 48 + 2x(v,w) gather 96 + scatter 96
+    "psor_sweep": ("psor_timed_updates", 568),  # manifold 328 + lambda r/w 48 + 2x(v,w) gather 96 + scatter 96
     "residual": ("resid_contacts", 472),
     "manifold": ("manifolds", 696),
     "nsc_free": ("bodies", 280),
@@ -56,7 +59,7 @@ TRAFFIC = {("c2", "broad_narrow"): 128.2e6 + 0.21e6, ("c4", "broad_narrow"): 479
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["c2", "c1", "c3", "c4"], default="c2")
+    ap.add_argument("--workload", choices=["c2", "c1", "c3", "c4", "c5"], default="c2")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=None)
@@ -69,12 +72,16 @@ def parse():
     ap.add_argument("--block", type=int, default=250)
     ap.add_argument("--ab-steps", type=int, default=20,
                     help="steps after the timed protocol solved from cold, previous-step and learned starts")
-    ap.add_argument("--weights", default=os.path.join(ROOT, "rod-dynamics-3d_amd", "weights", "warmstart_c2.npy"))
+    ap.add_argument("--weights", default=None,
+                    help="learned-start weights (default: weights/warmstart_c2.npy, warmstart_c5.npy for --workload c5)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (HBM counter calibration)")
     a = ap.parse_args()
-    defaults = {"c2": (4980, 100_000, 5), "c1": (9980, 1000, 2000), "c3": (2000, 1_000_000, 2),
+    defaults = {"c2": (4980, 100_000, 5), "c5": (4980, 200_000, 5), "c1": (9980, 1000, 2000), "c3": (2000, 1_000_000, 2),
                 "c4": (980, 256, 8)}[a.workload]
+    if a.weights is None:
+        a.weights = os.path.join(ROOT, "rod-dynamics-3d_amd", "weights",
+                                 "warmstart_c5.npy" if a.workload == "c5" else "warmstart_c2.npy")
     if a.steps is None:
         a.steps = defaults[0]
     if a.n is None:
@@ -131,14 +138,22 @@ def autotune(ctx, run_steps, cands, reps=3):
     return best, {str(k): v for k, v in tune.items()}, len(cands) * (reps + 1)
 
 
-def rank_kernels(ctx, run_warmup):
+# Dominant kernel of a workload whose warm-up is not representative, from the
+# rocprofv3 kernel statistics of the whole protocol (profiles/): on the
+# settling bed the 20 warm-up steps are the dilute phase (broad phase first),
+# while over the 5000 steps the colour launches of the huge-island PSOR solve
+# take the largest share of GPU time.
+DOMINANT = {"c2": "psor_sweep", "c5": "psor_sweep"}
+
+
+def rank_kernels(ctx, run_warmup, fixed=None):
     """Warm-up with every kernel timed; returns the dominant kernel among those
-    with a byte model and leaves only that kernel's timer on."""
+    with a byte model (or `fixed`) and leaves only that kernel's timer on."""
     ctx.set_profiling(True)
     run_warmup()
     kt = ctx.kernel_times()
     ranked = sorted(((ms, name) for name, (n, ms) in kt.items() if name in BYTES), reverse=True)
-    dom = ranked[0][1] if ranked else "broad_narrow"
+    dom = fixed or (ranked[0][1] if ranked else "broad_narrow")
     names = [ctx._L.rdk_kernel_name(k).decode() for k in range(ctx._L.rdk_num_kernels())]
     ctx.set_profiling(True)  # resets counters
     ctx.set_profile_mask(1 << names.index(dom))
@@ -153,6 +168,7 @@ def roofline(ctx, dom, n, workload):
     unit, bpu = BYTES[dom]
     units = {"bodies": wc["body_passes"] * n if dom == "broad_narrow" else launches * n,
              "sweep_contacts": wc["sweep_contacts"], "resid_contacts": wc["resid_contacts"],
+             "psor_timed_updates": wc["psor_timed_updates"],
              "manifolds": wc["manifolds"], "contacts": wc["contacts"], "force_contacts": wc["force_contacts"]}[unit]
     achieved = (bpu * units / 1e9) / (kms / 1e3) if kms > 0 else 0.0
     return {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -197,8 +213,17 @@ def cpu_baseline_c2(sc, snaps, total_steps, O):
 
 
 def run_c2(a, L, scenes):
+    """The settling protocol; --workload c5 runs it at N = 2e5 from an evaluation
+    seed (the learned start is trained on other seeds) and then compares the
+    starts on the settled bed."""
     t0 = time.perf_counter()
-    sc = scenes.c2_settling(n=a.n)
+    c5 = a.workload == "c5"
+    if c5:
+        from rod_dynamics_py import warmstart as WS
+
+        sc = scenes.c2_settling(n=a.n, seed=WS.EVAL_SEED0)
+    else:
+        sc = scenes.c2_settling(n=a.n)
     log(f"scene {sc.name} built in {time.perf_counter() - t0:.1f}s (box {sc.extra['side']:.2f})")
     p = sc.params
     p.check_every = a.check_every
@@ -212,7 +237,7 @@ def run_c2(a, L, scenes):
         best, tune, used = autotune(ctx, lambda cl, k: ctx.step_nsc(L.detect_params(0.0, cl, 1.0), p, k),
                                     (2.0, 4.0, 8.0, 16.0), reps=min(3, max(1, a.warmup // 3 - 1)))
     det = L.detect_params(0.0, best, 1.0)
-    dom = rank_kernels(ctx, lambda: ctx.step_nsc(det, p, max(1, a.warmup - used)))
+    dom = rank_kernels(ctx, lambda: ctx.step_nsc(det, p, max(1, a.warmup - used)), DOMINANT.get(a.workload))
     log(f"warmup {a.warmup} steps done; dominant kernel {dom}; timing {a.steps} steps")
     total = a.warmup + a.steps
     snap_at = [int(total * (k + 0.5) / 5) for k in range(5)] if not a.no_cpu else []
@@ -240,20 +265,23 @@ def run_c2(a, L, scenes):
     ctx.sync()
     el = time.perf_counter() - t1 - t_snap
     out = base_line("rod-steps/s at PSOR residual 1e-8", sc.n * a.steps / el, a, el, "f64", {
-        "workload": "hard-contact PSOR settling (C2)", "rods": sc.n, "L_over_d": 20, "phi0": 0.05,
+        "workload": ("learned-start bed: hard-contact settling, evaluation seed (C5)" if c5 else
+                     "hard-contact PSOR settling (C2)"), "rods": sc.n, "L_over_d": 20, "phi0": 0.05,
         "box": round(sc.extra["side"], 3), "periodic": "x,y", "walls": "floor+ceiling", "mu": 0.5,
         "restitution": 0.0, "omega": 1.3, "residual_tol": 1e-8, "dt": 1e-3,
         "friction_projection": "euclidean disc" if p.friction_proj == 1 else "reference sequential",
         "steps_total": total + max(0, used + max(1, a.warmup - used) - a.warmup), "seed": sc.extra["seed"],
         "check_every": a.check_every, "chunk_len": best, "chunk_len_autotune_ms": tune,
-        "stall_sweeps": p.stall_sweeps, "stall_omega": p.stall_omega})
+        "stall_sweeps": p.stall_sweeps, "stall_omega": p.stall_omega,
+        "hipgraph": "check blocks of the huge-island solve (sweeps x colours + residual) captured once per island "
+                    "and step, replayed; the step itself reads counts back and is not captured"})
     out["psor"] = {"sweeps_per_step": st["sweeps_total"] / max(st["steps"], 1),
                    "manifolds_per_step": st["manifolds_total"] / max(st["steps"], 1),
                    "rod_contacts_per_step": st["contacts_total"] / max(st["steps"], 1),
                    "colors_per_step": st["colors_total"] / max(st["steps"], 1),
                    "residual_max": st["residual_max"], "nonconverged_steps": st["nonconverged_steps"],
                    "warm_start": "previous step (rod-rod and plane-wall contacts)"}
-    out["roofline"] = roofline(ctx, dom, sc.n, "c2")
+    out["roofline"] = roofline(ctx, dom, sc.n, a.workload)
     if a.ab_steps > 0:
         out["learned_warm_start"] = learned_ab(ctx, sc, a)
     ctx.close()
@@ -457,7 +485,8 @@ def run_c4(a, L, scenes):
         "chunk_len_autotuned": best, "autotune_ms_per_step": tune,
         "steps_total": tuned_steps + a.warmup + a.steps, "stall_sweeps": p.stall_sweeps,
         "stall_omega": p.stall_omega,
-        "hipgraph": "not used: each step reads contact and island counts back to size its launches"})
+        "hipgraph": "the step reads contact and island counts back to size its launches and is not captured; "
+                    "huge islands (none here) replay captured check blocks"})
     out["psor"] = {"sweeps_per_step": st["sweeps_total"] / max(st["steps"], 1),
                    "manifolds_per_step": st["manifolds_total"] / max(st["steps"], 1),
                    "residual_max": st["residual_max"], "nonconverged_steps": st["nonconverged_steps"]}
@@ -480,7 +509,7 @@ def main():
     from rod_dynamics_py import _lib as L
     from rod_dynamics_py import scenes
 
-    if a.workload == "c2":
+    if a.workload in ("c2", "c5"):
         out = run_c2(a, L, scenes)
     elif a.workload == "c4":
         out = run_c4(a, L, scenes)

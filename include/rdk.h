@@ -217,11 +217,15 @@ int rdk_step_vv(rdk_ctx* ctx, const rdk_detect_params* dp, int model,
 int rdk_observables(rdk_ctx* ctx, double* kinetic_energy,
                     double* packing_fraction);
 
-/* ---- profiling: per-kernel HIP-event timing on the library stream ------- */
+/* ---- profiling: per-kernel HIP-event timing on the stream each kernel runs on */
 int rdk_set_profiling(rdk_ctx* ctx, int on);
-/* time only the kernels whose id bit is set (bit k = kernel id k) */
+/* time only the kernels whose id bit is set (bit k = kernel id k).  The
+ * colour launches of the huge-island solve are replayed from captured graphs;
+ * while "psor_sweep" is timed, one check block in 16 runs without the graph
+ * and its launches are timed (a sample over the whole timed region). */
 int rdk_set_profile_mask(rdk_ctx* ctx, uint64_t mask);
-/* kernel ids: see rdk_kernel_name */
+/* kernel ids: see rdk_kernel_name.  launches = launches whose time was
+ * recorded, total_ms = their summed event time */
 int rdk_kernel_times(rdk_ctx* ctx, int kernel, int64_t* launches,
                      double* total_ms);
 const char* rdk_kernel_name(int kernel);

@@ -1,6 +1,7 @@
 // rdk_api.hip -- the C ABI of librdk (declared in include/rdk.h).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 
@@ -33,7 +34,16 @@ BodyPtrs body_ptrs(rdk_ctx* ctx) {
   return B;
 }
 
-void prof_begin(rdk_ctx* ctx, int kernel, EventRec& rec) {
+bool graphs_enabled() {
+  static int on = -1;
+  if (on < 0) {
+    const char* e = getenv("RDK_GRAPHS");
+    on = (e && e[0] == '0') ? 0 : 1;
+  }
+  return on == 1;
+}
+
+void prof_begin(rdk_ctx* ctx, int kernel, EventRec& rec, hipStream_t s) {
   rec.kernel = kernel;
   for (int k = 0; k < 2; ++k) {
     hipEvent_t e;
@@ -45,11 +55,30 @@ void prof_begin(rdk_ctx* ctx, int kernel, EventRec& rec) {
     }
     (k == 0 ? rec.a : rec.b) = e;
   }
-  if (rec.a) (void)hipEventRecord(rec.a, ctx->stream);
+  if (rec.a) (void)hipEventRecord(rec.a, s);
 }
 
-void prof_end(rdk_ctx* ctx, EventRec& rec) {
-  if (rec.b) (void)hipEventRecord(rec.b, ctx->stream);
+bool prof_ext_events(rdk_ctx* ctx, int k, EventRec& rec) {
+  if (!timed(ctx, k)) return false;
+  rec.kernel = k;
+  for (int j = 0; j < 2; ++j) {
+    hipEvent_t e;
+    if (!ctx->ev_pool.empty()) {
+      e = ctx->ev_pool.back();
+      ctx->ev_pool.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+      e = nullptr;
+    }
+    (j == 0 ? rec.a : rec.b) = e;
+  }
+  ctx->k_launches[k] += 1;
+  return true;
+}
+
+void prof_ext_done(rdk_ctx* ctx, EventRec& rec) { ctx->ev_pending.push_back(rec); }
+
+void prof_end(rdk_ctx* ctx, EventRec& rec, hipStream_t s) {
+  if (rec.b) (void)hipEventRecord(rec.b, s);
   ctx->ev_pending.push_back(rec);
 }
 
@@ -58,6 +87,7 @@ int prof_flush(rdk_ctx* ctx) {
   RDK_HIP(hipStreamSynchronize(ctx->stream));
   for (auto& r : ctx->ev_pending) {
     float ms = 0.0f;
+    if (r.b) (void)hipEventSynchronize(r.b);  // events may sit on the side stream
     if (r.a && r.b && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) ctx->k_ms[r.kernel] += ms;
     if (r.a) ctx->ev_pool.push_back(r.a);
     if (r.b) ctx->ev_pool.push_back(r.b);
@@ -68,7 +98,7 @@ int prof_flush(rdk_ctx* ctx) {
 
 static const char* kKernelNames[K_NUM] = {
     "chunk_keys", "cell_ranges", "chunk_gather", "broad_narrow", "contact_record", "sort_scan",
-    "manifold", "wall", "csr", "color", "permute", "warm_apply", "psor_sweep", "residual",
+    "manifold", "wall", "csr", "color", "permute", "warm_apply", "psor_sweep", "psor_small", "residual",
     "nsc_free", "nsc_pos", "vv_half1", "vv_half2", "force", "body_reduce", "history", "observe",
     "mlp", "island", "island_tiny", "island_small", "island_medium", "island_large", "island_huge", "pos_project", "misc"};
 

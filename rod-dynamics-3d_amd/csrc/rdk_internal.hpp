@@ -27,6 +27,7 @@ enum KernelId {
   K_PERMUTE,
   K_WARM_APPLY,
   K_PSOR_SWEEP,
+  K_PSOR_SMALL,
   K_RESIDUAL,
   K_NSC_FREE,
   K_NSC_POS,
@@ -61,6 +62,7 @@ enum WorkId {
   W_INTEGRATE_BODIES, // body updates in integrator passes
   W_FORCE_CONTACTS,   // soft/HM force evaluations
   W_NARROW,           // FP64 narrow-phase calls (pairs passing the separating-axis test)
+  W_PSOR_TIMED,       // manifold updates in k_psor_sweep launches whose time was recorded
   W_NUM
 };
 
@@ -200,7 +202,8 @@ struct rdk_ctx {
   uint64_t prof_mask = ~(uint64_t)0;
   std::vector<rdk::EventRec> ev_pending;
   std::vector<hipEvent_t> ev_pool;
-  int64_t k_launches[rdk::K_NUM] = {};
+  int64_t k_launches[rdk::K_NUM] = {};  // timed launches
+  int64_t huge_blocks = 0;              // check blocks of the huge-island solve (timing sample)
   double k_ms[rdk::K_NUM] = {};
   int64_t work[rdk::W_NUM] = {};
 };
@@ -255,24 +258,45 @@ void release(DevBuf<T>& b) {
 BodyPtrs body_ptrs(rdk_ctx* ctx);
 
 // event timing around one launch
-void prof_begin(rdk_ctx* ctx, int kernel, EventRec& rec);
-void prof_end(rdk_ctx* ctx, EventRec& rec);
+void prof_begin(rdk_ctx* ctx, int kernel, EventRec& rec, hipStream_t s);
+void prof_end(rdk_ctx* ctx, EventRec& rec, hipStream_t s);
 int prof_flush(rdk_ctx* ctx);
 
+// Times the launches made while it lives (events on the stream they run on).
 struct ProfScope {
   rdk_ctx* ctx;
   EventRec rec;
+  hipStream_t s;
   bool on;
-  ProfScope(rdk_ctx* c, int k) : ctx(c), on(c->profiling && ((c->prof_mask >> k) & 1)) {
-    c->k_launches[k] += 1;
-    if (on) prof_begin(c, k, rec);
+  ProfScope(rdk_ctx* c, int k, hipStream_t st = nullptr)
+      : ctx(c), s(st ? st : c->stream), on(c->profiling && ((c->prof_mask >> k) & 1)) {
+    if (on) {
+      c->k_launches[k] += 1;
+      prof_begin(c, k, rec, s);
+    }
   }
   ~ProfScope() {
-    if (on) prof_end(ctx, rec);
+    if (on) prof_end(ctx, rec, s);
   }
 };
 
 static inline int grid_for(int64_t n, int block) { return (int)((n + block - 1) / block); }
+
+// Events for a launch made with hipExtLaunchKernelGGL, which stamps them at
+// the kernel's own start and end (no dispatch gap in the measurement, which
+// matters for launches of a few microseconds).  Returns false when kernel k
+// is not being timed; otherwise the caller launches with rec.a / rec.b and
+// then calls prof_ext_done.
+bool prof_ext_events(rdk_ctx* ctx, int k, EventRec& rec);
+void prof_ext_done(rdk_ctx* ctx, EventRec& rec);
+
+// True when kernel k's launches are being timed (timed launches are kept out
+// of graphs: a graph replay cannot be split into its kernels by events).
+static inline bool timed(const rdk_ctx* ctx, int k) { return ctx->profiling && ((ctx->prof_mask >> k) & 1); }
+
+// Graph replay of repeated launch sequences; RDK_GRAPHS=0 turns it off (for
+// tools that cannot follow captured graphs).
+bool graphs_enabled();
 
 // stage entry points implemented in the .hip files
 int detect_run(rdk_ctx* ctx, const rdk_detect_params* p);

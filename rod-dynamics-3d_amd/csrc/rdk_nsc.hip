@@ -13,6 +13,8 @@
 #include <algorithm>
 #include <cstring>
 
+#include <hip/hip_ext.h>
+
 #include "rdk_internal.hpp"
 
 namespace rdk {
@@ -1559,9 +1561,16 @@ static int run_sweeps(rdk_ctx* ctx, const rdk_nsc_params* p, int nsweeps) {
     for (int c = 0; c < ctx->ncolor; ++c) {
       const int i0 = ctx->color_start[c], i1 = ctx->color_start[c + 1];
       if (i1 <= i0) continue;
-      ProfScope ps(ctx, K_PSOR_SWEEP);
-      hipLaunchKernelGGL(k_psor_sweep, dim3(grid_for(i1 - i0, nb)), dim3(nb), 0, ctx->stream, B, M, ctx->sys_dev.p,
-                         i0, i1, K, ref);
+      EventRec rec;
+      if (prof_ext_events(ctx, K_PSOR_SWEEP, rec)) {
+        hipExtLaunchKernelGGL(k_psor_sweep, dim3(grid_for(i1 - i0, nb)), dim3(nb), 0, ctx->stream, rec.a, rec.b, 0,
+                              B, M, ctx->sys_dev.p, i0, i1, K, ref);
+        prof_ext_done(ctx, rec);
+        ctx->work[W_PSOR_TIMED] += i1 - i0;
+      } else {
+        hipLaunchKernelGGL(k_psor_sweep, dim3(grid_for(i1 - i0, nb)), dim3(nb), 0, ctx->stream, B, M,
+                           ctx->sys_dev.p, i0, i1, K, ref);
+      }
     }
   }
   ctx->work[W_SWEEP_CONTACTS] += (int64_t)nsweeps * ctx->nm;
@@ -1662,7 +1671,7 @@ static int huge_islands_solve(rdk_ctx* ctx, const IslandSolve& P, const uint32_t
   RDK_TRY(ensure(ctx, ctx->hcs, (size_t)nh * HUGE_CS));
   RDK_TRY(ensure(ctx, ctx->hres, 2));
   {
-    ProfScope ps(ctx, K_ISL_HUGE);
+    ProfScope ps(ctx, K_ISL_HUGE, st);
     hipLaunchKernelGGL(k_island_colours, dim3(nh), dim3(128), 0, st, ctx->istart.p, skey, list, nh, ctx->hcs.p);
   }
   std::vector<int> hcs((size_t)nh * HUGE_CS);
@@ -1674,39 +1683,44 @@ static int huge_islands_solve(rdk_ctx* ctx, const IslandSolve& P, const uint32_t
   double* dres = ctx->hres.p;
   double* dref = ctx->hres.p + 1;
   const int kk = P.stop_rule == 1 ? P.check_every : 1;
-  // Graph replay of one check block (kk sweeps, then the residual) when the
-  // per-kernel timers are off (they record events on the main stream).  The
-  // counters are cleared by a kernel: no memset nodes in the graph.
-  const bool graphs = !ctx->profiling;
+  // Graph replay of one check block (kk sweeps, then the residual).  The
+  // counters are cleared by a kernel: no memset nodes in the graph.  While
+  // the colour launches are timed, one block in 16 runs eagerly with its
+  // launches timed (a sample over the whole run); timing the whole huge path
+  // turns the graphs off.
+  const bool graphs = graphs_enabled() && !timed(ctx, K_ISL_HUGE);
+  const bool sample = timed(ctx, K_PSOR_SWEEP);
   for (int h = 0; h < nh; ++h) {
     const int* cs = hcs.data() + (size_t)h * HUGE_CS;
     const int ncol = cs[129];
     const int s = cs[0], e = cs[ncol];
-    int nlaunch = 0;
-    auto enqueue_block = [&](const SolveConst& Kc, bool sweep) {
-      nlaunch = 0;
+    auto enqueue_block = [&](const SolveConst& Kc, bool sweep, bool time_sweeps) {
       if (sweep) {
         for (int it = 0; it < kk; ++it) {
           if (it == kk - 1) {
             hipLaunchKernelGGL(k_zero_doubles, dim3(1), dim3(64), 0, st, ctx->hres.p, 2);
-            ++nlaunch;
           }
           for (int c = 0; c < ncol; ++c) {
             if (cs[c + 1] <= cs[c]) continue;
-            ProfScope ps(ctx, K_ISL_HUGE);
-            hipLaunchKernelGGL(k_psor_sweep, dim3(grid_for(cs[c + 1] - cs[c], nb)), dim3(nb), 0, st, B, Mc,
-                               ctx->sys_dev.p, cs[c], cs[c + 1], Kc, dref);
-            ++nlaunch;
+            EventRec rec;
+            if (time_sweeps && prof_ext_events(ctx, K_PSOR_SWEEP, rec)) {
+              hipExtLaunchKernelGGL(k_psor_sweep, dim3(grid_for(cs[c + 1] - cs[c], nb)), dim3(nb), 0, st, rec.a,
+                                    rec.b, 0, B, Mc, ctx->sys_dev.p, cs[c], cs[c + 1], Kc, dref);
+              prof_ext_done(ctx, rec);
+              ctx->work[W_PSOR_TIMED] += cs[c + 1] - cs[c];
+            } else {
+              ProfScope ps(ctx, K_ISL_HUGE, st);
+              hipLaunchKernelGGL(k_psor_sweep, dim3(grid_for(cs[c + 1] - cs[c], nb)), dim3(nb), 0, st, B, Mc,
+                                 ctx->sys_dev.p, cs[c], cs[c + 1], Kc, dref);
+            }
           }
         }
       } else {
         hipLaunchKernelGGL(k_zero_doubles, dim3(1), dim3(64), 0, st, ctx->hres.p, 2);
-        ++nlaunch;
       }
-      ProfScope ps(ctx, K_ISL_HUGE);
+      ProfScope ps(ctx, K_ISL_HUGE, st);
       hipLaunchKernelGGL(k_residual, dim3(grid_for(e - s, nb)), dim3(nb), 0, st, B, Mc, ctx->sys_dev.p, s, e, Kc,
                          dres);
-      ++nlaunch;
     };
     auto read_back = [&](double* res, double* ref) -> int {
       double hv[2];
@@ -1720,7 +1734,7 @@ static int huge_islands_solve(rdk_ctx* ctx, const IslandSolve& P, const uint32_t
     if (P.warm) {
       for (int c = 0; c < ncol; ++c) {
         if (cs[c + 1] <= cs[c]) continue;
-        ProfScope ps(ctx, K_ISL_HUGE);
+        ProfScope ps(ctx, K_ISL_HUGE, st);
         hipLaunchKernelGGL(k_warm_apply, dim3(grid_for(cs[c + 1] - cs[c], nb)), dim3(nb), 0, st, B, Mc, cs[c],
                            cs[c + 1]);
       }
@@ -1731,7 +1745,7 @@ static int huge_islands_solve(rdk_ctx* ctx, const IslandSolve& P, const uint32_t
     trk.init(P.stall);
     SolveConst K = P.K;  // omega may drop to stall_omega on a stall
     if (P.stop_rule == 1) {
-      enqueue_block(K, false);
+      enqueue_block(K, false, false);
       RDK_TRY(read_back(&res, nullptr));
     }
     hipGraph_t graph = nullptr;
@@ -1744,13 +1758,14 @@ static int huge_islands_solve(rdk_ctx* ctx, const IslandSolve& P, const uint32_t
     };
     int rc = RDK_OK;
     while (rc == RDK_OK && sweeps < P.max_sweeps && !(P.stop_rule == 1 && res <= P.tol)) {
-      if (graphs) {
+      const bool timed_block = sample && (ctx->huge_blocks++ % 16 == 0);
+      if (graphs && !timed_block) {
         if (!exec) {
           if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
             rc = RDK_ERR_HIP;
             break;
           }
-          enqueue_block(K, true);
+          enqueue_block(K, true, false);
           if (hipStreamEndCapture(st, &graph) != hipSuccess ||
               hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
             rc = RDK_ERR_HIP;
@@ -1761,9 +1776,8 @@ static int huge_islands_solve(rdk_ctx* ctx, const IslandSolve& P, const uint32_t
           rc = RDK_ERR_HIP;
           break;
         }
-        ctx->k_launches[K_ISL_HUGE] += nlaunch;
       } else {
-        enqueue_block(K, true);
+        enqueue_block(K, true, timed_block);
       }
       sweeps += kk;
       if ((rc = read_back(&res, &res_ref)) != RDK_OK) break;
@@ -1889,14 +1903,14 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
   // Tiny islands, and the host-driven solve of huge islands, run on the side
   // stream, concurrently with the other classes (islands share no body).
   const bool side = (cnt[0] > 0 || cnt[4] > 0) && (cnt[1] > 0 || cnt[2] > 0 || cnt[3] > 0 || (cnt[0] > 0 && cnt[4] > 0)) &&
-                    ctx->stream2 && !ctx->profiling;
+                    ctx->stream2;
   hipStream_t s_side = side ? ctx->stream2 : ctx->stream;
   if (side) {
     RDK_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
     RDK_HIP(hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
   }
   if (cnt[0] > 0) {
-    ProfScope ps(ctx, K_ISL_TINY);
+    ProfScope ps(ctx, K_ISL_TINY, s_side);
     constexpr int ipb = ISL_WG0 / ISL_G0;
     hipLaunchKernelGGL((k_island_reg<ISL_G0, ISL_WG0>), dim3(grid_for(cnt[0], ipb)), dim3(ISL_WG0), 0, s_side, B, Mc,
                        ctx->sys_dev.p, ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p, cnt[0], P, S);
@@ -1999,7 +2013,7 @@ static int global_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
   if (small) {
     RDK_TRY(ensure(ctx, ctx->small_out, sizeof(SmallOut)));
     {
-      ProfScope ps(ctx, K_PSOR_SWEEP);
+      ProfScope ps(ctx, K_PSOR_SMALL);
       hipLaunchKernelGGL(k_psor_small, dim3(1), dim3(SMALL_BLOCK), 0, ctx->stream, B, Mc, ctx->sys_dev.p,
                          ctx->d_cstart.p, (const unsigned*)ctx->uncolored.p, nm, solve_const(ctx, p),
                          p->warm_mode != 0 ? 1 : 0, p->tol, p->max_sweeps, std::max(1, p->check_every), p->stop_rule,

@@ -5,5 +5,6 @@ compute runs in librdk.so (HIP, gfx950) through its C ABI.
 """
 from . import _lib
 from ._lib import Context, RdkError, lib
+from .simulator import Simulator
 
-__all__ = ["Context", "RdkError", "lib", "_lib"]
+__all__ = ["Context", "RdkError", "Simulator", "lib", "_lib"]

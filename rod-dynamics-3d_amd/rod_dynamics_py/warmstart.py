@@ -42,9 +42,12 @@ def _det0(det):
     return L.detect_params(0.0, det.chunk_len, det.cell_scale)
 
 
-def record_step(ctx, det, p):
+def record_step(ctx, det, p, select="uncached"):
     """One hard-contact step in stages, as rdk_step_nsc runs it, returning the
-    training pairs of its manifolds (or None when the solve did not converge)."""
+    training pairs of its manifolds (or None when the solve did not converge):
+    those without a previous-step normal impulse (select="uncached", the ones
+    the learned start is applied to) or all of them (select="all"; the
+    previous-step impulse is one of the features)."""
     ctx.integrate(L.RDK_NSC_FREE)
     ctx.detect(_det0(det))
     p1 = _wo_warm(p, 1)
@@ -56,14 +59,14 @@ def record_step(ctx, det, p):
         st = ctx.nsc_solve(p1, allow_noconv=True)
         _, _, lam = ctx.nsc_manifolds(nm)
         if st["residual"] <= p.tol:
-            sel = lam0[:, 0] <= 0.0  # the manifolds the learned start is used for
+            sel = lam0[:, 0] <= 0.0 if select == "uncached" else np.ones(len(lam0), bool)
             T = lam[sel] / ls[sel, None]
             out = (X[sel], T.astype(np.float32))
     ctx.integrate(L.RDK_NSC_POS)
     return out
 
 
-def collect(ctx, sc, steps, record_from, every, max_pairs, log=None):
+def collect(ctx, sc, steps, record_from, every, max_pairs, log=None, select="uncached"):
     """Run the settling protocol of scene `sc` for `steps` steps, recording the
     pairs of every `every`-th step from `record_from` on."""
     p = _wo_warm(sc.params, 1)
@@ -76,7 +79,7 @@ def collect(ctx, sc, steps, record_from, every, max_pairs, log=None):
             ctx.step_nsc(_det0(sc.detect), p, k)
             s += k
             continue
-        r = record_step(ctx, sc.detect, p)
+        r = record_step(ctx, sc.detect, p, select)
         s += 1
         if r is not None:
             Xs.append(r[0])
@@ -211,6 +214,8 @@ def main(argv=None):
     t.add_argument("--pairs", type=int, default=1_000_000)
     t.add_argument("--epochs", type=int, default=20)
     t.add_argument("--save-data", default="")
+    t.add_argument("--select", choices=["uncached", "all"], default="uncached",
+                   help="training pairs: manifolds without a previous-step impulse, or all manifolds")
     e = sub.add_parser("eval")
     e.add_argument("--weights", required=True)
     e.add_argument("--n", type=int, default=100_000)
@@ -224,7 +229,7 @@ def main(argv=None):
         for k in range(a.seeds):
             sc = scenes.c2_settling(n=a.n, seed=TRAIN_SEED0 + k)
             sc.load(ctx)
-            X, T = collect(ctx, sc, a.steps, a.record_from, a.every, per_seed, log=sys.stderr)
+            X, T = collect(ctx, sc, a.steps, a.record_from, a.every, per_seed, log=sys.stderr, select=a.select)
             print(f"[warmstart] seed {k}: {len(X)} pairs", file=sys.stderr, flush=True)
             Xs.append(X)
             Ts.append(T)
@@ -234,6 +239,8 @@ def main(argv=None):
         p, rep = train(ctx, X, T, epochs=a.epochs)
         np.save(a.out, p)
         rep["pairs_collected"] = int(len(X))
+        rep["select"] = a.select
+        rep["rods"] = a.n
         print(json.dumps(rep), flush=True)
     else:
         p = np.load(a.weights)

@@ -1,0 +1,72 @@
+"""The reference's Python session API on the GPU: a scene JSON with explicit
+bodies (the elastic exchange of smoke_test.py:165-195 through Simulator), and
+a frozen relaxed packing of the reference loaded from its endpoint file."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+from rod_dynamics_py.simulator import Simulator
+
+pytestmark = pytest.mark.gpu
+
+
+def test_elastic_exchange_through_the_session_api(tmp_path):
+    # crossed rods (one along y, one along z) touching at their centres: the
+    # impulse passes through both centres of mass, so an elastic frictionless
+    # contact swaps the velocities of equal masses
+    scene = {
+        "physics": {"dt": 1e-3, "gravity": [0, 0, 0], "lin_damp": 0.0, "ang_damp": 0.0,
+                    "nsc": {"enabled": True, "mu": 0.0, "beta": 0.0, "cfm": 0.0, "omega": 1.0, "velocity_iters": 200,
+                            "position_stabilization": False}},
+        "scene": {"bodies": [
+            {"length": 1.0, "diameter": 0.5, "density": 1000.0, "restitution": 1.0, "pos": [-0.249, 0.0, 0.0],
+             "v_lin": [1.0, 0.0, 0.0]},
+            {"length": 1.0, "diameter": 0.5, "density": 1000.0, "restitution": 1.0, "pos": [0.249, 0.0, 0.0],
+             "rot_quat": [0.7071067811865476, 0.7071067811865476, 0.0, 0.0], "v_lin": [-1.0, 0.0, 0.0]}]}}
+    path = tmp_path / "elastic.json"
+    path.write_text(json.dumps(scene))
+    sim = Simulator(str(path))
+    d0 = sim.diagnostics()
+    assert d0["frame_index"] == 0 and d0["rod_count"] == 2
+    ke0 = d0["last_ke"]
+    sim.step()
+    d = sim.diagnostics()
+    assert d["frame_index"] == 1 and d["last_hit_count"] == 1 and d["dt"] == 1e-3
+    r = sim.rods()
+    assert np.allclose(r[0]["linear_velocity"], [-1, 0, 0], atol=1e-9)
+    assert np.allclose(r[1]["linear_velocity"], [1, 0, 0], atol=1e-9)
+    assert d["last_ke"] == pytest.approx(ke0, rel=1e-9)
+    m = 1000.0 * np.pi * 0.25 ** 2 * 1.0  # density * pi r^2 * L (cylinder part), rigid_body.cpp:19-24
+    assert r[0]["mass"] == pytest.approx(m) and r[0]["inverse_mass"] == pytest.approx(1 / m)
+    assert r[0]["shape"] == "capsule" and r[0]["radius"] == 0.25 and r[0]["half_height"] == 0.5
+    # the rod axes in the reference frame: +Y (identity) and +Z (90 degrees about x)
+    assert np.allclose(np.subtract(r[0]["endpoint_b"], r[0]["endpoint_a"]), [0, 1.0, 0], atol=1e-12)
+    assert np.allclose(np.subtract(r[1]["endpoint_b"], r[1]["endpoint_a"]), [0, 0, 1.0], atol=1e-12)
+    assert np.allclose(r[0]["angular_velocity"], 0, atol=1e-12) and np.allclose(r[1]["angular_velocity"], 0, atol=1e-12)
+    sim.close()
+
+
+def test_reference_packing_from_its_endpoint_file(tmp_path):
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "reference_packings.npz"))
+    ends, radius = z["ends_5"], float(z["radius_5"])  # a 200-rod packing, L/d = 10
+    lines = [f"# rod_radius={radius}"] + [" ".join(f"{v:.17g}" for v in e) for e in ends]
+    (tmp_path / "x_relaxed.txt").write_text("\n".join(lines) + "\n")
+    scene = {"physics": {"dt": 1e-4, "gravity": [0, 0, 0], "nsc": {"enabled": True, "position_stabilization": False}}}
+    (tmp_path / "s.json").write_text(json.dumps(scene))
+    sim = Simulator(str(tmp_path / "s.json"), str(tmp_path / "x_relaxed.txt"))
+    r = sim.rods()
+    assert len(r) == len(ends)
+    a = np.array([x["endpoint_a"] for x in r])
+    b = np.array([x["endpoint_b"] for x in r])
+    assert np.allclose(a, ends[:, :3], atol=1e-12) and np.allclose(b, ends[:, 3:], atol=1e-12)
+    assert np.allclose([x["radius"] for x in r], radius)
+    st = sim.ctx.download_state()  # the library frame: the oracle sees the same FP64 rods
+    sim.step()
+    n = len(ends)
+    W = O.World(st["x"], st["q"], None, None, sim.r, sim.h, np.ones(n))
+    W.set_box(st["x"].min(0) - 10, st["x"].max(0) + 10, [0, 0, 0])
+    assert sim.diagnostics()["last_hit_count"] == W.detect(0, 0.0)  # every overlapping pair is a manifold
+    sim.close()
