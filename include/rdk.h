@@ -65,7 +65,9 @@ typedef struct rdk_nsc_params {
   int check_every;         /* residual evaluated every k sweeps */
   int friction_proj;       /* 0 reference sequential disc clamp, 1 Euclidean disc */
   int position_stabilization; /* scales beta by 0.01 (nsc_contact.cpp:339-341) */
-  int warm_mode;           /* 0 cold, 1 previous step (reference), 2 learned (MLP) */
+  int warm_mode;           /* 0 cold, 1 previous step (reference), 2 learned (MLP) for the
+                              manifolds without a previous-step normal impulse, 3 learned for
+                              every manifold (the previous-step impulse is an input) */
   int stop_rule;           /* 0 reference (max sum|dlambda| < 1e-6), 1 residual <= tol */
   int warm_walls;          /* 1: plane-wall manifolds also start from the previous
                               step's impulses, keyed by (rod, plane, endpoint); the
@@ -216,6 +218,18 @@ int rdk_step_vv(rdk_ctx* ctx, const rdk_detect_params* dp, int model,
 /* ---- observables (App::totalKE main.cpp:3357-3368) --------------------- */
 int rdk_observables(rdk_ctx* ctx, double* kinetic_energy,
                     double* packing_fraction);
+
+/* ---- entanglement observer (App::computeEntanglement, main.cpp:4287-4331;
+ *      pairwise_abs_linking_sum_with_cutoff, entanglement_adapter.cpp:56-117)
+ * Per system s: sums[s] = sum over rod pairs of |Gauss linking number| of
+ * the two axis segments, pairs[s] = number of pairs summed; every pair for
+ * cutoff <= 0, else the pairs whose segments are at most cutoff apart.  On
+ * periodic systems each segment endpoint is wrapped into the box first, as
+ * the reference does.  The reference prunes pairs by centre distance minus
+ * 1.0 before the segment distance (exact for its unit-length rods); this
+ * test uses the half lengths, so it is exact for any length.  The mean
+ * sums[s] / pairs[s] is the reference's normalised entanglement (ebar). */
+int rdk_entanglement(rdk_ctx* ctx, double cutoff, double* sums, int64_t* pairs);
 
 /* ---- profiling: per-kernel HIP-event timing on the stream each kernel runs on */
 int rdk_set_profiling(rdk_ctx* ctx, int on);

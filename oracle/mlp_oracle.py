@@ -182,10 +182,11 @@ def features(mf, a, b, state, r, mu_of_body, vs, prev=None, prev_wall=None):
     return X, ls
 
 
-def apply_prediction(Y, ls, mu, lam_prev=None):
+def apply_prediction(Y, ls, mu, lam_prev=None, all_manifolds=False):
     """Start impulses: manifolds with a previous normal impulse keep the
-    previous-step impulses; the others take the prediction (units of lscale)
-    projected onto lambda_n >= 0 and the friction disc |lambda_t| <= mu lambda_n."""
+    previous-step impulses (unless all_manifolds); the others take the
+    prediction (units of lscale) projected onto lambda_n >= 0 and the friction
+    disc |lambda_t| <= mu lambda_n."""
     Y = np.asarray(Y, np.float64)
     base = np.zeros((Y.shape[0], 3)) if lam_prev is None else np.asarray(lam_prev, np.float64)
     ln = np.maximum(0.0, Y[:, 0] * ls)
@@ -194,7 +195,7 @@ def apply_prediction(Y, ls, mu, lam_prev=None):
     nt = np.linalg.norm(t, axis=1)
     sc = np.where(nt > lim, np.where(nt > 0, lim / np.where(nt > 0, nt, 1.0), 0.0), 1.0)
     pred = np.column_stack([ln, t * sc[:, None]])
-    keep = base[:, 0] > 0.0
+    keep = (base[:, 0] > 0.0) & (not all_manifolds)
     return np.where(keep[:, None], base, pred)
 
 

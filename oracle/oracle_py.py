@@ -85,6 +85,10 @@ def lib():
         L.orc_kinetic_energy.argtypes = [C.c_void_p]
         L.orc_packing_fraction.restype = C.c_double
         L.orc_packing_fraction.argtypes = [C.c_void_p]
+        L.orc_linking_number.restype = C.c_double
+        L.orc_linking_number.argtypes = [_dp, _dp, _dp, _dp]
+        L.orc_entanglement.restype = C.c_double
+        L.orc_entanglement.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_longlong)]
         _lib = L
     return _lib
 
@@ -265,6 +269,18 @@ class World:
 
     def packing_fraction(self):
         return self._L.orc_packing_fraction(self._h)
+
+    def entanglement(self, cutoff=-1.0):
+        """(sum of |Lk| over rod pairs, pair count); all pairs for cutoff <= 0."""
+        n = C.c_longlong()
+        s = self._L.orc_entanglement(self._h, float(cutoff), C.byref(n))
+        return s, n.value
+
+
+def linking_number(a0, a1, b0, b1):
+    """Gauss linking number of the segments a0a1 and b0b1."""
+    v = [np.ascontiguousarray(x, np.float64) for x in (a0, a1, b0, b1)]
+    return lib().orc_linking_number(*[_d(x) for x in v])
 
 
 def axis(q):

@@ -1485,6 +1485,81 @@ double orc_kinetic_energy(orc_world* W) {
   return ke;
 }
 
+// ---- entanglement observer (main.cpp:4287-4331, entanglement_adapter.cpp:56-117)
+// Gauss linking number of two straight segments a0a1, b0b1: the signed solid
+// angle of the tetrahedron they span over 4 pi (Klenin & Langowski 2000):
+// Omega = asin(n1.n2) + asin(n2.n3) + asin(n3.n4) + asin(n4.n1) with the face
+// normals n1 = r13 x r14, n2 = r14 x r24, n3 = r24 x r23, n4 = r23 x r13
+// (r_ij = p_j - p_i, p1..p4 = a0, a1, b0, b1), each normalised as
+// n / (|n| + 1e-6), and the sign of (r34 x r12) . r13.  The 1e-6 is the
+// convention of the reference's linking-number routine (its dependency
+// entanglement-cpp is not vendored): with it this formula reproduces the
+// published ebar0 of all 60 reference packings
+// (assets/packings_metadata.csv) to 1e-6.
+static double linking_number(V3 p1, V3 p2, V3 p3, V3 p4) {
+  const V3 r13 = sub(p3, p1), r14 = sub(p4, p1), r23 = sub(p3, p2), r24 = sub(p4, p2);
+  const V3 r12 = sub(p2, p1), r34 = sub(p4, p3);
+  auto unit = [](V3 v) {
+    const double n = std::sqrt(dot(v, v)) + 1e-6;
+    return mk(v.x / n, v.y / n, v.z / n);
+  };
+  const V3 n1 = unit(cross(r13, r14)), n2 = unit(cross(r14, r24));
+  const V3 n3 = unit(cross(r24, r23)), n4 = unit(cross(r23, r13));
+  auto as = [](double x) { return std::asin(x < -1.0 ? -1.0 : (x > 1.0 ? 1.0 : x)); };
+  const double om = as(dot(n1, n2)) + as(dot(n2, n3)) + as(dot(n3, n4)) + as(dot(n4, n1));
+  const double tp = dot(cross(r34, r12), r13);
+  const double sg = tp > 0.0 ? 1.0 : (tp < 0.0 ? -1.0 : 0.0);
+  return om * sg / (4.0 * M_PI);
+}
+
+// Rod axis endpoints x -/+ u h; on a periodic world each endpoint is wrapped
+// into the box on its own, as the reference does before the pair sum
+// (main.cpp:4294-4300).
+static void rod_segment(const orc_world& W, const Body& b, V3& a, V3& e) {
+  const V3 u = axis_of(b.q);
+  a = sub(b.x, mul(u, b.h));
+  e = add(b.x, mul(u, b.h));
+  const bool pbc = W.periodic[0] || W.periodic[1] || W.periodic[2];
+  if (!pbc) return;
+  double* pa[3] = {&a.x, &a.y, &a.z};
+  double* pe[3] = {&e.x, &e.y, &e.z};
+  for (int k = 0; k < 3; ++k) {
+    if (!W.periodic[k]) continue;
+    const double span = W.hi[k] - W.lo[k];
+    *pa[k] -= span * std::floor((*pa[k] - W.lo[k]) / span);
+    *pe[k] -= span * std::floor((*pe[k] - W.lo[k]) / span);
+  }
+}
+
+double orc_linking_number(const double* a0, const double* a1, const double* b0, const double* b1) {
+  return linking_number(mk(a0[0], a0[1], a0[2]), mk(a1[0], a1[1], a1[2]), mk(b0[0], b0[1], b0[2]),
+                        mk(b1[0], b1[1], b1[2]));
+}
+
+// Sum of |Lk| over the rod pairs (all of them when cutoff <= 0, else those
+// whose axis segments are at most cutoff apart), and the pair count
+// (entanglement_adapter.cpp:56-117).
+double orc_entanglement(orc_world* W, double cutoff, long long* pairs) {
+  const int n = (int)W->B.size();
+  std::vector<V3> a(n), e(n);
+  for (int i = 0; i < n; ++i) rod_segment(*W, W->B[i], a[i], e[i]);
+  double sum = 0.0;
+  long long np = 0;
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j) {
+      if (cutoff > 0.0) {
+        double s, t;
+        segseg(a[i], e[i], a[j], e[j], s, t);
+        const V3 d = sub(add(a[i], mul(sub(e[i], a[i]), s)), add(a[j], mul(sub(e[j], a[j]), t)));
+        if (std::sqrt(dot(d, d)) > cutoff) continue;
+      }
+      sum += std::fabs(linking_number(a[i], e[i], a[j], e[j]));
+      ++np;
+    }
+  if (pairs) *pairs = np;
+  return sum;
+}
+
 double orc_packing_fraction(orc_world* W) {
   double v = 0.0;
   for (const Body& b : W->B) v += M_PI * b.r * b.r * (2.0 * b.h) + (4.0 / 3.0) * M_PI * b.r * b.r * b.r;

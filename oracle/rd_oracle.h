@@ -127,6 +127,10 @@ int orc_vv_step(orc_world* W, int model, const void* params, int nsteps,
 
 double orc_kinetic_energy(orc_world* W);
 double orc_packing_fraction(orc_world* W);
+/* entanglement observer: Gauss linking number of two segments, and the sum
+ * of |Lk| over rod pairs (all pairs for cutoff <= 0) with the pair count */
+double orc_linking_number(const double* a0, const double* a1, const double* b0, const double* b1);
+double orc_entanglement(orc_world* W, double cutoff, long long* pairs);
 
 #define ORC_MF_FIELDS 42
 

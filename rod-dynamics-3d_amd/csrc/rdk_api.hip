@@ -167,7 +167,9 @@ int rdk_destroy(rdk_ctx* ctx) {
   release(ctx->csr_val_alt); release(ctx->body_start); release(ctx->d_cstart); release(ctx->small_out);
   release(ctx->uf_parent); release(ctx->uf_label); release(ctx->sbody); release(ctx->sbody_alt);
   release(ctx->iflag); release(ctx->ipos); release(ctx->istart); release(ctx->ilists); release(ctx->bseg);
-  release(ctx->icnt); release(ctx->blab); release(ctx->blab_alt); release(ctx->istats); release(ctx->hcs); release(ctx->hres); release(ctx->islot); release(ctx->prev_keys); release(ctx->prev_wkeys); release(ctx->resid);
+  release(ctx->icnt); release(ctx->blab); release(ctx->blab_alt); release(ctx->istats); release(ctx->hcs);
+  for (int k = 0; k < 3; ++k) { release(ctx->ent_a[k]); release(ctx->ent_e[k]); }
+  release(ctx->ent_out); release(ctx->ent_psum); release(ctx->ent_pcnt); release(ctx->hres); release(ctx->islot); release(ctx->prev_keys); release(ctx->prev_wkeys); release(ctx->resid);
   release(ctx->hkeys); release(ctx->hkeys_new); release(ctx->hlast); release(ctx->hlast_new);
   release(ctx->hflag); release(ctx->hscan); release(ctx->pe); release(ctx->temp); release(ctx->red);
   if (ctx->h_counters) (void)hipHostFree(ctx->h_counters);
@@ -477,7 +479,7 @@ int rdk_history_size(rdk_ctx* ctx, int* n) {
 
 static int check_nsc(rdk_ctx* ctx, const rdk_nsc_params* p) {
   if (!p || !(p->omega > 0.0) || p->max_sweeps < 0 || p->friction_proj < 0 || p->friction_proj > 1 ||
-      p->warm_mode < 0 || p->warm_mode > 2) {
+      p->warm_mode < 0 || p->warm_mode > 3) {
     ctx->err = "invalid rdk_nsc_params";
     return RDK_ERR_ARG;
   }
@@ -694,6 +696,17 @@ int rdk_observables(rdk_ctx* ctx, double* ke, double* phi) {
   CTX_CHECK();
   RDK_TRY(check_ready(ctx));
   RDK_TRY(observables_run(ctx, ke, phi));
+  return prof_flush(ctx);
+}
+
+int rdk_entanglement(rdk_ctx* ctx, double cutoff, double* sums, int64_t* pairs) {
+  CTX_CHECK();
+  RDK_TRY(check_ready(ctx));
+  if (!(cutoff == cutoff)) {
+    ctx->err = "rdk_entanglement: cutoff is NaN";
+    return RDK_ERR_ARG;
+  }
+  RDK_TRY(entanglement_run(ctx, cutoff, sums, pairs));
   return prof_flush(ctx);
 }
 

@@ -164,6 +164,8 @@ struct rdk_ctx {
   rdk::DevBuf<uint32_t> blab, blab_alt;
   rdk::DevBuf<char> istats;
   rdk::DevBuf<int> hcs;       // colour starts of the huge islands
+  rdk::DevBuf<double> ent_a[3], ent_e[3], ent_out, ent_psum;  // entanglement observer
+  rdk::DevBuf<unsigned long long> ent_pcnt;
   rdk::DevBuf<double> hres;   // residual and last-sweep change of the huge-island solve
   rdk::DevBuf<int2> islot;  // local body slots of large-island manifolds
   // position projection
@@ -311,7 +313,8 @@ int nsc_residual_run(rdk_ctx* ctx, const rdk_nsc_params* p, double* res, double*
 int integrate_run(rdk_ctx* ctx, int stage);
 int forces_run(rdk_ctx* ctx, int model, const void* params, double* pe);
 int observables_run(rdk_ctx* ctx, double* ke, double* phi);
-int mlp_warm_start(rdk_ctx* ctx);
+int entanglement_run(rdk_ctx* ctx, double cutoff, double* sums, int64_t* pairs);
+int mlp_warm_start(rdk_ctx* ctx, bool all);
 int nsc_project_run(rdk_ctx* ctx, const rdk_detect_params* dp, const rdk_nsc_params* p, int* passes, int* npos);
 void mlp_free(rdk_ctx* ctx);
 

@@ -246,11 +246,11 @@ __global__ void __launch_bounds__(64 * INF_WAVES) k_mlp_infer(const float* __res
 // l0..l2 from the cache lookup) keep it; the others take the prediction,
 // projected onto lambda_n >= 0 and the friction disc.
 __global__ void k_mlp_apply(const float* __restrict__ Y, const double* __restrict__ lscale, const int* __restrict__ ma,
-                            const SysDev* __restrict__ sys, const int* __restrict__ bsys, int nm, double* __restrict__ l0,
-                            double* __restrict__ l1, double* __restrict__ l2) {
+                            const SysDev* __restrict__ sys, const int* __restrict__ bsys, int nm, int all,
+                            double* __restrict__ l0, double* __restrict__ l1, double* __restrict__ l2) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nm) return;
-  if (l0[i] > 0.0) return;
+  if (!all && l0[i] > 0.0) return;  // keep the cached start
   const double ls = lscale[i];
   const double ln = dmax(0.0, (double)Y[3 * i] * ls);
   double a = (double)Y[3 * i + 1] * ls, b = (double)Y[3 * i + 2] * ls;
@@ -464,7 +464,7 @@ static int mlp_features(rdk_ctx* ctx, Mlp* M) {
   return RDK_OK;
 }
 
-int mlp_warm_start(rdk_ctx* ctx) {
+int mlp_warm_start(rdk_ctx* ctx, bool all) {
   Mlp* M = get_mlp(ctx);
   if (!M->ready) {
     ctx->err = "learned warm start: no network loaded (rdk_mlp_init / rdk_mlp_set_params / rdk_mlp_train)";
@@ -477,7 +477,8 @@ int mlp_warm_start(rdk_ctx* ctx) {
   RDK_TRY(mlp_infer_dev(ctx, M, M->feat.p, nm, M->out.p));
   ProfScope ps(ctx, K_MLP);
   hipLaunchKernelGGL(k_mlp_apply, dim3(grid_for(nm, 256)), dim3(256), 0, ctx->stream, M->out.p, M->lscale.p,
-                     ctx->ma.p, ctx->sys_dev.p, ctx->sys_of.p, nm, ctx->lam[0].p, ctx->lam[1].p, ctx->lam[2].p);
+                     ctx->ma.p, ctx->sys_dev.p, ctx->sys_of.p, nm, all ? 1 : 0, ctx->lam[0].p, ctx->lam[1].p,
+                     ctx->lam[2].p);
   return RDK_OK;
 }
 

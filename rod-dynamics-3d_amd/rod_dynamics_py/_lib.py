@@ -110,6 +110,7 @@ _SIGS = {
     "rdk_kernel_name": (C.c_char_p, [C.c_int]),
     "rdk_num_kernels": (C.c_int, []),
     "rdk_work_counters": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_int]),
+    "rdk_entanglement": (C.c_int, [C.c_void_p, C.c_double, _dp, C.POINTER(C.c_int64)]),
     "rdk_gen_rsa": (C.c_int, [C.c_int, _dp, _dp, C.POINTER(SystemDesc), C.c_double, C.c_uint64, C.c_int, _dp, _dp,
                               _ip]),
     "rdk_gen_thermal": (C.c_int, [C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_uint64, _dp, _dp]),
@@ -238,6 +239,7 @@ class Context:
         self._h = h
         self._L = L
         self.n = 0
+        self._nsys = 0
 
     def close(self):
         if getattr(self, "_h", None):
@@ -266,6 +268,7 @@ class Context:
         off = np.ascontiguousarray(offsets, dtype=np.int32)
         arr = (SystemDesc * len(descs))(*descs)
         self._check(self._L.rdk_set_systems(self._h, len(descs), iptr(off), arr))
+        self._nsys = len(descs)
 
     def set_physics(self, p):
         self._check(self._L.rdk_set_physics(self._h, C.byref(p)))
@@ -398,6 +401,16 @@ class Context:
         ke, phi = C.c_double(), C.c_double()
         self._check(self._L.rdk_observables(self._h, C.byref(ke), C.byref(phi)))
         return ke.value, phi.value
+
+    def entanglement(self, cutoff=-1.0, nsys=None):
+        """Per system: (sum of |linking number| over rod pairs, pair count);
+        every pair for cutoff <= 0."""
+        k = max(int(nsys or self._nsys), 1)
+        sums = np.zeros(k)
+        pairs = np.zeros(k, np.int64)
+        self._check(self._L.rdk_entanglement(self._h, float(cutoff), dptr(sums),
+                                             pairs.ctypes.data_as(C.POINTER(C.c_int64))))
+        return sums, pairs
 
     def sync(self):
         self._check(self._L.rdk_sync(self._h))

@@ -1,8 +1,9 @@
 """Writes tests/golden/reference_packings.npz from the reference's committed
 packings (assets/initial-configs/*/x_relaxed.txt: rod endpoints, header
 rod_radius) and their contact counts (assets/packings_metadata.csv, column
-n_contacts: pairs whose segment distance is below 1.01 d, free space).  One
-packing per aspect ratio of the first seed group.  Run in a container that
+n_contacts: pairs whose segment distance is below 1.01 d, free space; ebar0:
+mean |linking number| over all rod pairs, rounded to 6 digits; ent_pairs: the
+pair count).  One packing per aspect ratio of the first seed group.  Run in a container that
 has the reference tree at /root/reference."""
 import csv
 import re
@@ -47,6 +48,8 @@ def main():
         out[f"ends_{k}"] = ends
         out[f"radius_{k}"] = np.float64(radius)
         out[f"n_contacts_{k}"] = np.int64(r["n_contacts"])
+        out[f"ebar0_{k}"] = np.float64(r["ebar0"])
+        out[f"ent_pairs_{k}"] = np.int64(r["ent_pairs"])
         out[f"id_{k}"] = np.array(r["packing_id"])
     out["count"] = np.int64(len(chosen))
     np.savez_compressed(OUT, **out)

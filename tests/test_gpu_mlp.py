@@ -194,3 +194,29 @@ def test_training_fits_a_smooth_target(gpu_ctx_factory):
     # the trained BF16 inference tracks the FP64 forward of the trained weights
     Y = ctx.mlp_infer(X[:1000])
     assert np.mean(np.abs(Y - M.forward(p1, X[:1000]))) < 0.02
+
+
+def test_learned_start_for_every_manifold(gpu_ctx_factory):
+    """warm_mode 3: the prediction replaces the cached start on every manifold
+    (the cached impulse is one of the inputs)."""
+    B = overlapping_bed(n=2000, seed=64, mu=0.4)
+    ctx = gpu_ctx_factory()
+    prm = _rand_params(6, 0.1)
+    prm[M.OFF_W3:M.OFF_B3] = 0.0
+    prm[M.OFF_B3:] = [2.0, 0.5, -0.5]
+    ctx.mlp_set_params(prm)
+    p1 = L.nsc_params(beta=0.0, cfm=0.0, omega=1.3, tol=1e-8, max_sweeps=20000, check_every=4, friction_proj=1,
+                      warm_mode=1, warm_walls=1)
+    _settled_pair_of_builds(ctx, B, p1)
+    nm = ctx.nsc_build(p1)
+    _, _, lam_prev = ctx.nsc_manifolds(nm)
+    X, ls = ctx.mlp_features(nm)
+    Y = ctx.mlp_infer(X)
+    p3 = L.nsc_params(beta=0.0, cfm=0.0, omega=1.3, tol=1e-8, max_sweeps=20000, check_every=4, friction_proj=1,
+                      warm_mode=3, warm_walls=1)
+    assert ctx.nsc_build(p3) == nm
+    _, _, lam3 = ctx.nsc_manifolds(nm)
+    expect = M.apply_prediction(Y, ls, 0.4, lam_prev, all_manifolds=True)
+    assert np.allclose(lam3, expect, rtol=1e-12, atol=1e-15 * np.abs(expect).max())
+    assert (lam_prev[:, 0] > 0).sum() > 100
+    assert np.allclose(lam3[:, 0], 2.0 * ls, rtol=1e-6)

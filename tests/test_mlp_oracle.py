@@ -154,6 +154,14 @@ def test_apply_prediction_keeps_previous_impulses():
     assert np.allclose(L[2], [0.0, 0.0, 0.0])      # predicted separation
 
 
+def test_apply_prediction_to_every_manifold():
+    Y = np.array([[0.5, 0.0, 0.0], [3.0, 1.0, 0.0]])
+    prev = np.array([[1.0, 0.2, -0.1], [0.0, 0.0, 0.0]])
+    L = M.apply_prediction(Y, np.array([2.0, 1.0]), 0.5, prev, all_manifolds=True)
+    assert np.allclose(L[0], [1.0, 0.0, 0.0])      # the prediction replaces the cached impulse
+    assert np.allclose(L[1], [3.0, 1.0, 0.0])
+
+
 def test_adamw_first_step_closed_form():
     # first step: m_hat = g, v_hat = g^2, so the step is lr * g / (|g| + eps)
     rng = np.random.default_rng(9)
