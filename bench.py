@@ -24,6 +24,11 @@ all systems, chunk length autotuned in the warm-up, 1000 steps.
 --workload c5: the settling protocol at 2e5 rods from an evaluation seed, then
 the settled bed solved from the cold, previous-step and learned starts (the
 learned start trained on other seeds).
+--workload populate: the reference's non-overlapping populate (serial random
+sequential addition, FP32, std::mt19937 stream) reproduced on the GPU: 1e6
+rods, L=20, d=1, periodic box at phi=0.05; rods placed per second, with the
+serial oracle timed on a 1e5-rod sample of the same density (and the GPU
+result on that sample compared with it bit for bit).
 """
 import argparse
 import json
@@ -49,6 +54,7 @@ BYTES = {
     "history": ("force_contacts", 64),
     "vv_half1": ("bodies", 280),
     "vv_half2": ("bodies", 280),
+    "populate": ("candidates", 28),      # candidate centre + direction 24, decision 4
 }
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 # (FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0, from a calibrated 1 GiB copy)
@@ -59,7 +65,7 @@ TRAFFIC = {("c2", "broad_narrow"): 128.2e6 + 0.21e6, ("c4", "broad_narrow"): 479
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["c2", "c1", "c3", "c4", "c5"], default="c2")
+    ap.add_argument("--workload", choices=["c2", "c1", "c3", "c4", "c5", "populate"], default="c2")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=None)
@@ -78,7 +84,7 @@ def parse():
                     help="only run a device-to-device copy of MIB MiB five times (HBM counter calibration)")
     a = ap.parse_args()
     defaults = {"c2": (4980, 100_000, 5), "c5": (4980, 200_000, 5), "c1": (9980, 1000, 2000), "c3": (2000, 1_000_000, 2),
-                "c4": (980, 256, 8)}[a.workload]
+                "c4": (980, 256, 8), "populate": (1, 1_000_000, 100_000)}[a.workload]
     if a.weights is None:
         a.weights = os.path.join(ROOT, "rod-dynamics-3d_amd", "weights",
                                  "warmstart_c5.npy" if a.workload == "c5" else "warmstart_c2.npy")
@@ -498,6 +504,54 @@ def run_c4(a, L, scenes):
     return out
 
 
+# ---------------------------------------------------------------------- populate
+def run_populate(a, L):
+    import math
+
+    import numpy as np
+
+    length, dia, phi, seed, att = 20.0, 1.0, 0.05, 1592590338, 1000
+    vol = math.pi * (dia / 2) ** 2 * length + 4.0 / 3.0 * math.pi * (dia / 2) ** 3
+
+    def side(n):
+        return (n * vol / phi) ** (1.0 / 3.0)
+
+    ctx = L.Context(0)
+    ctx.populate_nonoverlap(1000, length, dia, [0, 0, 0], [side(1000)] * 3, 1, 0.0, seed, att)  # warm-up
+    ctx.set_profiling(True)
+    box = side(a.n)
+    t0 = time.perf_counter()
+    c, u, q, drawn = ctx.populate_nonoverlap(a.n, length, dia, [0, 0, 0], [box] * 3, 1, 0.0, seed, att)
+    el = time.perf_counter() - t0
+    log(f"placed {len(c)} of {a.n} rods from {drawn} candidates in {el:.2f}s")
+    kt = ctx.kernel_times()["populate"]
+    out = {"metric": "rods placed/s (non-overlapping populate)", "value": len(c) / el, "unit": "rods/s",
+           "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": 1e3 * el, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "reference populate nonoverlap", "rods": a.n, "placed": len(c),
+                      "candidates": drawn, "length": length, "diameter": dia, "phi": phi, "box": box,
+                      "periodic": "x,y,z", "cell_size": "max(L/4, 1.25 d)", "seed": seed, "max_attempts": att}}
+    achieved = (BYTES["populate"][1] * drawn / 1e9) / (kt[1] / 1e3) if kt[1] > 0 else 0.0
+    out["roofline"] = {"kernel": "populate", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None, "launches": kt[0],
+                       "avg_launch_ms": kt[1] / max(kt[0], 1), "bytes_per_unit": BYTES["populate"][1],
+                       "unit_name": "candidates", "units": drawn}
+    if not a.no_cpu:
+        O = oracle()
+        ns = a.cpu_steps
+        t1 = time.perf_counter()
+        k, oc, ou, odrawn = O.populate_nonoverlap(ns, length, dia, [0, 0, 0], [side(ns)] * 3, 1, 0.0, seed, att)
+        cel = time.perf_counter() - t1
+        gc, gu, _, gdrawn = ctx.populate_nonoverlap(ns, length, dia, [0, 0, 0], [side(ns)] * 3, 1, 0.0, seed, att)
+        same = bool(len(gc) == k and gdrawn == odrawn and np.array_equal(gc, oc) and np.array_equal(gu, ou))
+        log(f"oracle: {k} rods in {cel:.2f}s; GPU result on the sample equal: {same}")
+        out["cpu_baseline"] = {"value": k / cel, "unit": "rods/s", "cores": 1, "kind": "port",
+                               "sample": f"serial oracle of the same populate at {ns} rods, same density and seed "
+                                         f"({cel:.1f} s)", "sample_gpu_bitwise_equal": same}
+    ctx.close()
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -509,7 +563,9 @@ def main():
     from rod_dynamics_py import _lib as L
     from rod_dynamics_py import scenes
 
-    if a.workload in ("c2", "c5"):
+    if a.workload == "populate":
+        out = run_populate(a, L)
+    elif a.workload in ("c2", "c5"):
         out = run_c2(a, L, scenes)
     elif a.workload == "c4":
         out = run_c4(a, L, scenes)

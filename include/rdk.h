@@ -259,6 +259,25 @@ int rdk_work_counters(rdk_ctx* ctx, int64_t* counters, int n);
 int rdk_gen_rsa(int n, const double* r, const double* h,
                 const rdk_system_desc* box, double gap, uint64_t seed,
                 int max_attempts, double* x, double* q, int* placed);
+/* The reference's populate mode "nonoverlap" (App::resetScene,
+ * main.cpp:1857-2140), reproduced bit for bit on the GPU: equal capsules of
+ * length `length` and diameter `diameter` drawn from std::mt19937(seed)
+ * through std::uniform_real_distribution<float> (5 numbers per attempt:
+ * centre uniform in [bmin, bmax), direction uniform on S^2), accepted when no
+ * earlier rod lies closer than one diameter (FP32 segment distance, 27
+ * images when periodic, the reference's placement grid of cell size
+ * cell_size, or max(L/4, 1.25 d) when cell_size <= 0).  A rod that fails
+ * max_attempts times in a row ends the populate (the reference uses
+ * max(1000, populate.maxAttempts)).  Outputs, in placement order: centres and
+ * unit directions (n x 3 floats), quaternions (w,x,y,z) rotating +Y onto each
+ * direction (n x 4, may be NULL), *placed, *attempts (candidates drawn).
+ * `window` = candidates decided per parallel round (0 = automatic); the
+ * result does not depend on it. */
+int rdk_populate_nonoverlap(rdk_ctx* ctx, int n, float length, float diameter,
+                            const float* bmin, const float* bmax, int periodic,
+                            float cell_size, uint32_t seed, int max_attempts,
+                            int window, float* centres, float* dirs,
+                            float* quats, int* placed, int64_t* attempts);
 /* Thermal velocities: v ~ N(0, kT/m), w ~ N(0, kT/I_perp) on the two axes
  * perpendicular to the rod, zero spin about the axis. */
 int rdk_gen_thermal(int n, const double* q, const double* r, const double* h,

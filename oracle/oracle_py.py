@@ -89,6 +89,15 @@ def lib():
         L.orc_linking_number.argtypes = [_dp, _dp, _dp, _dp]
         L.orc_entanglement.restype = C.c_double
         L.orc_entanglement.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_longlong)]
+        _fp = C.POINTER(C.c_float)
+        L.orc_populate_nonoverlap.restype = C.c_int
+        L.orc_populate_nonoverlap.argtypes = [C.c_int, C.c_float, C.c_float, _fp, _fp, C.c_int, C.c_float,
+                                              C.c_uint, C.c_int, _fp, _fp, C.POINTER(C.c_longlong)]
+        L.orc_mt_uniform_float.argtypes = [C.c_uint, C.c_int, _fp]
+        L.orc_rsa_seg_d2.restype = C.c_float
+        L.orc_rsa_seg_d2.argtypes = [_fp] * 4 + [C.c_int]
+        L.orc_mt_raw.restype = C.c_uint
+        L.orc_mt_raw.argtypes = [C.c_uint, C.c_longlong]
         _lib = L
     return _lib
 
@@ -281,6 +290,37 @@ def linking_number(a0, a1, b0, b1):
     """Gauss linking number of the segments a0a1 and b0b1."""
     v = [np.ascontiguousarray(x, np.float64) for x in (a0, a1, b0, b1)]
     return lib().orc_linking_number(*[_d(x) for x in v])
+
+
+def populate_nonoverlap(n, length, diameter, bmin, bmax, pbc, cell_size, seed, max_attempts):
+    """The reference's serial non-overlapping populate in FP32: (placed,
+    centres (placed, 3), directions (placed, 3), candidates drawn)."""
+    fp = C.POINTER(C.c_float)
+    lo = np.ascontiguousarray(bmin, np.float32)
+    hi = np.ascontiguousarray(bmax, np.float32)
+    c = np.zeros((n, 3), np.float32)
+    u = np.zeros((n, 3), np.float32)
+    att = C.c_longlong()
+    k = lib().orc_populate_nonoverlap(int(n), float(length), float(diameter), lo.ctypes.data_as(fp),
+                                      hi.ctypes.data_as(fp), int(pbc), float(cell_size), int(seed) & 0xFFFFFFFF,
+                                      int(max_attempts), c.ctypes.data_as(fp), u.ctypes.data_as(fp), C.byref(att))
+    return k, c[:k], u[:k], att.value
+
+
+def rsa_seg_d2(p0, p1, q0, q1, pbc_rule):
+    fp = C.POINTER(C.c_float)
+    v = [np.ascontiguousarray(a, np.float32) for a in (p0, p1, q0, q1)]
+    return float(lib().orc_rsa_seg_d2(*[a.ctypes.data_as(fp) for a in v], int(pbc_rule)))
+
+
+def mt_uniform_float(seed, k):
+    out = np.zeros(k, np.float32)
+    lib().orc_mt_uniform_float(int(seed) & 0xFFFFFFFF, int(k), out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out
+
+
+def mt_raw(seed, k):
+    return lib().orc_mt_raw(int(seed) & 0xFFFFFFFF, int(k))
 
 
 def axis(q):

@@ -132,6 +132,15 @@ double orc_packing_fraction(orc_world* W);
 double orc_linking_number(const double* a0, const double* a1, const double* b0, const double* b1);
 double orc_entanglement(orc_world* W, double cutoff, long long* pairs);
 
+/* reference populate "nonoverlap" (main.cpp:1857-2140), FP32, serial:
+ * returns rods placed; c, u: n x 3 centres and unit directions */
+int orc_populate_nonoverlap(int n, float length, float diameter, const float* bmin, const float* bmax,
+                            int pbc, float cell_size, unsigned seed, int max_attempts, float* c,
+                            float* u, long long* attempts);
+float orc_rsa_seg_d2(const float* p0, const float* p1, const float* q0, const float* q1, int pbc_rule);
+void orc_mt_uniform_float(unsigned seed, int k, float* out);
+unsigned orc_mt_raw(unsigned seed, long long k);
+
 #define ORC_MF_FIELDS 42
 
 #ifdef __cplusplus

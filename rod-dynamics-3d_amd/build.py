@@ -18,7 +18,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("RDK_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["rdk_api.hip", "rdk_detect.hip", "rdk_nsc.hip", "rdk_forces.hip",
-           "rdk_integrate.hip", "rdk_sort.hip", "rdk_mlp.hip", "rdk_entangle.hip", "rdk_gen.cpp"]
+           "rdk_integrate.hip", "rdk_sort.hip", "rdk_mlp.hip", "rdk_entangle.hip", "rdk_populate.hip",
+           "rdk_gen.cpp"]
 
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          "-ffp-contract=off", "-fno-fast-math",

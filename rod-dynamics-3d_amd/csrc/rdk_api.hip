@@ -100,7 +100,8 @@ static const char* kKernelNames[K_NUM] = {
     "chunk_keys", "cell_ranges", "chunk_gather", "broad_narrow", "contact_record", "sort_scan",
     "manifold", "wall", "csr", "color", "permute", "warm_apply", "psor_sweep", "psor_small", "residual",
     "nsc_free", "nsc_pos", "vv_half1", "vv_half2", "force", "body_reduce", "history", "observe",
-    "mlp", "island", "island_tiny", "island_small", "island_medium", "island_large", "island_huge", "pos_project", "misc"};
+    "mlp", "island", "island_tiny", "island_small", "island_medium", "island_large", "island_huge", "pos_project",
+    "populate", "misc"};
 
 }  // namespace rdk
 
@@ -707,6 +708,23 @@ int rdk_entanglement(rdk_ctx* ctx, double cutoff, double* sums, int64_t* pairs) 
     return RDK_ERR_ARG;
   }
   RDK_TRY(entanglement_run(ctx, cutoff, sums, pairs));
+  return prof_flush(ctx);
+}
+
+int rdk_populate_nonoverlap(rdk_ctx* ctx, int n, float length, float diameter, const float* bmin,
+                            const float* bmax, int periodic, float cell_size, uint32_t seed, int max_attempts,
+                            int window, float* centres, float* dirs, float* quats, int* placed,
+                            int64_t* attempts) {
+  CTX_CHECK();
+  if (n < 0 || !bmin || !bmax || !placed || (n > 0 && (!centres || !dirs)) || max_attempts < 1 || window < 0 ||
+      !(length > 0.0f) || !(diameter > 0.0f) || !(cell_size == cell_size)) {
+    ctx->err = "rdk_populate_nonoverlap: invalid argument";
+    return RDK_ERR_ARG;
+  }
+  *placed = 0;
+  if (attempts) *attempts = 0;
+  RDK_TRY(populate_nonoverlap_run(ctx, n, length, diameter, bmin, bmax, periodic, cell_size, seed, max_attempts,
+                                  window, centres, dirs, quats, placed, attempts));
   return prof_flush(ctx);
 }
 

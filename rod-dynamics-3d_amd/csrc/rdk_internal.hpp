@@ -45,6 +45,7 @@ enum KernelId {
   K_ISL_LARGE,
   K_ISL_HUGE,
   K_POSPROJ,
+  K_POPULATE,
   K_MISC,
   K_NUM
 };
@@ -314,6 +315,9 @@ int integrate_run(rdk_ctx* ctx, int stage);
 int forces_run(rdk_ctx* ctx, int model, const void* params, double* pe);
 int observables_run(rdk_ctx* ctx, double* ke, double* phi);
 int entanglement_run(rdk_ctx* ctx, double cutoff, double* sums, int64_t* pairs);
+int populate_nonoverlap_run(rdk_ctx* ctx, int n, float length, float diameter, const float* bmin, const float* bmax,
+                            int periodic, float cell_size, uint32_t seed, int max_attempts, int window,
+                            float* centres, float* dirs, float* quats, int* placed, int64_t* attempts);
 int mlp_warm_start(rdk_ctx* ctx, bool all);
 int nsc_project_run(rdk_ctx* ctx, const rdk_detect_params* dp, const rdk_nsc_params* p, int* passes, int* npos);
 void mlp_free(rdk_ctx* ctx);

@@ -111,6 +111,10 @@ _SIGS = {
     "rdk_num_kernels": (C.c_int, []),
     "rdk_work_counters": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_int]),
     "rdk_entanglement": (C.c_int, [C.c_void_p, C.c_double, _dp, C.POINTER(C.c_int64)]),
+    "rdk_populate_nonoverlap": (C.c_int, [C.c_void_p, C.c_int, C.c_float, C.c_float, C.POINTER(C.c_float),
+                                          C.POINTER(C.c_float), C.c_int, C.c_float, C.c_uint32, C.c_int, C.c_int,
+                                          C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                          C.POINTER(C.c_int), C.POINTER(C.c_int64)]),
     "rdk_gen_rsa": (C.c_int, [C.c_int, _dp, _dp, C.POINTER(SystemDesc), C.c_double, C.c_uint64, C.c_int, _dp, _dp,
                               _ip]),
     "rdk_gen_thermal": (C.c_int, [C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_uint64, _dp, _dp]),
@@ -411,6 +415,26 @@ class Context:
         self._check(self._L.rdk_entanglement(self._h, float(cutoff), dptr(sums),
                                              pairs.ctypes.data_as(C.POINTER(C.c_int64))))
         return sums, pairs
+
+    def populate_nonoverlap(self, n, length, diameter, bmin, bmax, periodic, cell_size, seed, max_attempts,
+                            window=0):
+        """The reference's non-overlapping populate on the GPU: (centres, unit
+        directions, quaternions (w,x,y,z), candidates drawn), FP32 arrays of the
+        rods placed, in placement order."""
+        fp = C.POINTER(C.c_float)
+        lo = np.ascontiguousarray(bmin, np.float32)
+        hi = np.ascontiguousarray(bmax, np.float32)
+        c = np.zeros((max(n, 1), 3), np.float32)
+        u = np.zeros((max(n, 1), 3), np.float32)
+        q = np.zeros((max(n, 1), 4), np.float32)
+        placed = C.c_int()
+        att = C.c_int64()
+        self._check(self._L.rdk_populate_nonoverlap(
+            self._h, int(n), float(length), float(diameter), lo.ctypes.data_as(fp), hi.ctypes.data_as(fp),
+            int(bool(periodic)), float(cell_size), int(seed) & 0xFFFFFFFF, int(max_attempts), int(window),
+            c.ctypes.data_as(fp), u.ctypes.data_as(fp), q.ctypes.data_as(fp), C.byref(placed), C.byref(att)))
+        k = placed.value
+        return c[:k], u[:k], q[:k], att.value
 
     def sync(self):
         self._check(self._L.rdk_sync(self._h))

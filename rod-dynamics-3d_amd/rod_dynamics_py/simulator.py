@@ -16,16 +16,27 @@ physicsStep per step (main.cpp:4333-4858) run by the library's fused steps:
   ``physics.contact_model``): velocity Verlet with the force evaluated twice
   per step, as the reference's soft paths do.
 
+Beyond the pybind surface, the reference app's file I/O is kept:
+``init_state_path`` loads the last frame of an init-state CSV (the CLI's
+``--init-state``, App::loadInitialStateCSV, main.cpp:2935-3055), and
+``write_snapshot``, ``save_endpoints`` and ``write_per_rod`` write NDJSON
+snapshots, endpoint files and the per-rod log in the reference's formats
+(``sceneio``).
+
 The reference is y-up (default gravity (0, -10, 0), floors are box top faces
 in y); the library's plane walls are floors in z.  Scenes are therefore run in
 the frame rotated by +90 degrees about x, (x, y, z) -> (x, -z, y), and every
 quantity is rotated back in ``rods()``: a rotation leaves the dynamics
 unchanged.
 
-What differs from the reference, by design: random streams (populate,
-randomInit) come from numpy / the library's generator, not libstdc++'s
-mt19937 + distributions, so generated scenes are statistically but not
-bitwise the reference's; ``last_island_count`` reports the number of contact
+populate.mode "nonoverlap" places the reference's rods bit for bit (its FP32
+serial placement with its std::mt19937 stream, evaluated in parallel on the
+GPU by ``rdk_populate_nonoverlap``).
+
+What differs from the reference, by design: the other random streams
+(populate "grid"/"uniform", randomInit) come from numpy, not libstdc++'s
+mt19937 + distributions, so those scenes are statistically but not bitwise
+the reference's; ``last_island_count`` reports the number of contact
 islands of the last hard-contact solve (the reference never updates that
 field); the soft/Hertz-Mindlin paths use the library's capsule cell list
 instead of the reference's sphere grid.  Options the library does not model
@@ -39,6 +50,7 @@ import sys
 import numpy as np
 
 from . import _lib as L
+from . import sceneio
 
 # defaults of the reference's configuration (include/config/config.hpp,
 # src/config/config.cpp:defaultAppCfg)
@@ -217,7 +229,7 @@ def load_endpoints(path, cfg):
 class Simulator:
     """See the module docstring.  One GPU context per simulator."""
 
-    def __init__(self, scene_path, init_csv_path="", quiet=True):
+    def __init__(self, scene_path, init_csv_path="", quiet=True, init_state_path=""):
         self.cfg = load_scene(scene_path)
         self.quiet = quiet
         cfg = self.cfg
@@ -235,20 +247,30 @@ class Simulator:
             raise NotImplementedError("the scene selects no contact model this library implements "
                                       "(physics.nsc, physics.soft_contact or physics.hertz_mindlin)")
         self.dt = float(cfg["physics"]["dt"])
+        self.ctx = L.Context(0)
         csv = init_csv_path or cfg["initCsvPath"]
-        if csv:
+        frame0 = 0
+        loaded = False
+        if init_state_path:  # the CLI's --init-state: highest priority (main.cpp:1630-1638)
+            try:
+                frame0 = self._rods_from_state(init_state_path)
+                loaded = True
+            except ValueError as e:
+                print(f"{e}\n[init-state] Failed to load state from {init_state_path}", file=sys.stderr)
+        elif csv:
             self._rods_from_endpoints(csv)
-        elif cfg["populate"]["count"] > 0:
-            self._populate()
-        elif cfg["bodies"]:
-            self._rods_from_bodies()
-        else:
-            raise ValueError("the scene has no rods (no populate.count, bodies or initCsvPath)")
+            loaded = True
+        if not loaded:
+            if cfg["populate"]["count"] > 0:
+                self._populate()
+            elif cfg["bodies"]:
+                self._rods_from_bodies()
+            else:
+                raise ValueError("the scene has no rods (no populate.count, bodies or initCsvPath)")
         self._random_init()
-        self.frame_index = 0
+        self.frame_index = frame0
         self.last_hit_count = 0
         self.last_island_count = 0
-        self.ctx = L.Context(0)
         self._load()
         self.last_ke = self.ctx.observables()[0]
         if not quiet:
@@ -277,6 +299,27 @@ class Simulator:
         self.h = 0.5 * length
         self._materials(E["density"], E["restitution"], E["friction"])
 
+    def _rods_from_state(self, path):
+        """Rods of the last frame of an init-state CSV (App::loadInitialStateCSV):
+        length 0.5, radius 0.025, density 1000, restitution and friction 0.5,
+        with radius and density taken from scene.populate when its count or
+        density is set.  Returns the frame number."""
+        frame, p, v, w, q = sceneio.parse_state_csv(path)
+        pop = self.cfg["populate"]
+        rad, dens = 0.025, 1000.0
+        if pop["count"] > 0 or pop["density"] > 0:
+            rad, dens = pop["radius"], pop["density"]
+        if pop["shape"] != "capsule":
+            raise NotImplementedError("spheres are not implemented by this library")
+        n = len(p)
+        self.x = p
+        self.q = q / np.linalg.norm(q, axis=1, keepdims=True)
+        self.r = np.full(n, float(rad))
+        self.h = np.full(n, 0.25)
+        self._materials(dens, 0.5, 0.5)
+        self.v, self.w = v, w
+        return frame
+
     def _body0(self):
         b = dict(_BODY)
         if self.cfg["bodies"]:
@@ -300,15 +343,19 @@ class Simulator:
         rng = np.random.default_rng(pop["seed"] or None)
         rr, hh = np.full(n, r), np.full(n, h)
         if pop["mode"] == "nonoverlap":
-            # RSA in the library frame (27 images on periodic axes)
-            lo_l, hi_l = self._box_lib(lo, hi)
-            desc = L.system_desc(lo_l, hi_l, (1, 1, 1) if per["enabled"] else (0, 0, 0), 0, 0, 0.0)
-            xl, ql, placed = L.gen_rsa(rr, hh, desc, gap=0.0, seed=int(pop["seed"]) or int(rng.integers(1, 2**62)),
-                                       max_attempts=int(pop["maxAttempts"]))
-            if placed < n:
-                raise RuntimeError(f"populate: placed {placed} of {n} rods without overlap")
-            self.x = xl @ _R  # back to the reference frame (R^T x)
-            self.q = _qmul(np.array([_QR[0], -_QR[1], 0.0, 0.0]), ql)
+            # the reference's serial placement, reproduced bit for bit on the GPU
+            # (FP32, its mt19937 stream and grid; main.cpp:1857-2140)
+            seed = int(pop["seed"]) or int(rng.integers(1, 2**32))
+            att = max(1000, int(pop["maxAttempts"]))
+            c, _, q, _ = self.ctx.populate_nonoverlap(n, b["length"], b["diameter"], lo, hi, per["enabled"],
+                                                      float(per["cellSize"]), seed, att)
+            if len(c) < n:
+                print(f"[populate] nonoverlap: failed to place rod {len(c)}/{n} after attempts= {att}",
+                      file=sys.stderr)
+            self.x = c.astype(np.float64)
+            q = q.astype(np.float64)
+            self.q = q / np.linalg.norm(q, axis=1, keepdims=True)
+            rr, hh = rr[:len(c)], hh[:len(c)]
         elif pop["mode"] == "grid":
             sp = pop["spacingMul"] * b["diameter"]
             ext = hi - lo
@@ -460,11 +507,14 @@ class Simulator:
                 "last_hit_count": self.last_hit_count, "last_island_count": self.last_island_count,
                 "rod_count": len(self.r)}
 
-    def rods(self):
+    def _state_ref(self):
+        """Device state rotated back into the reference's frame."""
         s = self.ctx.download_state()
-        x = s["x"] @ _R
         q = _qmul(np.array([_QR[0], -_QR[1], 0.0, 0.0]), s["q"])
-        v, w, f, t = s["v"] @ _R, s["w"] @ _R, s["f"] @ _R, s["tau"] @ _R
+        return s["x"] @ _R, q, s["v"] @ _R, s["w"] @ _R, s["f"] @ _R, s["tau"] @ _R
+
+    def rods(self):
+        x, q, v, w, f, t = self._state_ref()
         u = _axis(q)
         out = []
         for i in range(len(self.r)):
@@ -478,6 +528,29 @@ class Simulator:
                         "restitution": float(self.rest[i]), "shape": "capsule", "radius": float(self.r[i]),
                         "half_height": float(self.h[i]), "endpoint_a": a.tolist(), "endpoint_b": b.tolist()})
         return out
+
+    # ---- reference file formats (App logging / I/O) --------------------------
+    def write_snapshot(self, stream):
+        """Append one NDJSON snapshot line of the current frame
+        (App::writeSnapshotLine)."""
+        x, q, v, w, _, _ = self._state_ref()
+        stream.write(sceneio.snapshot_line(self.frame_index, self.dt, x, q, v, w, self.r, self.h))
+
+    def save_endpoints(self, path):
+        """Write the rods as an endpoint file (App::saveInitialConfigCSV)."""
+        x, q, _, _, _, _ = self._state_ref()
+        per = self.cfg["periodic"]
+        box = float(per["max"][0]) - float(per["min"][0])
+        with open(path, "w") as f:
+            f.write(sceneio.endpoints_text(x, q, self.r, self.h, per["enabled"], box))
+
+    def write_per_rod(self, stream, header=False):
+        """Append the per-rod rows of the current frame (App::logPerRodFrame),
+        preceded by the log's header when ``header`` is set."""
+        x, q, v, w, _, _ = self._state_ref()
+        if header:
+            stream.write(sceneio.per_rod_header(self.r[0], self.h[0]))
+        stream.write(sceneio.per_rod_text(self.frame_index, x, v, w, q, self.mass, self.r, self.h))
 
     def close(self):
         if getattr(self, "ctx", None) is not None:

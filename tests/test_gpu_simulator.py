@@ -70,3 +70,45 @@ def test_reference_packing_from_its_endpoint_file(tmp_path):
     W.set_box(st["x"].min(0) - 10, st["x"].max(0) + 10, [0, 0, 0])
     assert sim.diagnostics()["last_hit_count"] == W.detect(0, 0.0)  # every overlapping pair is a manifold
     sim.close()
+
+
+def test_state_files_round_trip_through_the_simulator(tmp_path):
+    """A populated scene runs a few steps; its per-rod log restarts a second
+    simulator at the logged frame (init-state CSV) with the FP32-rounded
+    state; the NDJSON snapshot and the endpoint file carry the same rods."""
+    from rod_dynamics_py import sceneio
+
+    scene = {"physics": {"dt": 1e-3, "gravity": [0, -10, 0], "nsc": {"enabled": True}},
+             "scene": {"periodic": {"enabled": True, "min": [-2, -2, -2], "max": [2, 2, 2]},
+                       "populate": {"count": 60, "mode": "nonoverlap", "seed": 3, "radius": 0.025,
+                                    "density": 1000.0},
+                       "bodies": [{"length": 0.5, "diameter": 0.05, "density": 1000.0, "restitution": 0.5,
+                                   "friction": 0.5}]}}
+    (tmp_path / "s.json").write_text(json.dumps(scene))
+    sim = Simulator(str(tmp_path / "s.json"))
+    sim.step(5)
+    log = tmp_path / "per_rod.csv"
+    with open(log, "w") as f:
+        sim.write_per_rod(f, header=True)
+    with open(tmp_path / "snap.ndjson", "w") as f:
+        sim.write_snapshot(f)
+    sim.save_endpoints(str(tmp_path / "ends.txt"))
+    ref = sim.rods()
+    snap = json.loads((tmp_path / "snap.ndjson").read_text())
+    assert snap["frame"] == 5 and len(snap["bodies"]) == 60
+    sim2 = Simulator(str(tmp_path / "s.json"), init_state_path=str(log))
+    assert sim2.diagnostics()["frame_index"] == 5 and sim2.diagnostics()["rod_count"] == 60
+    r2 = sim2.rods()
+    for a, b, s in zip(ref, r2, snap["bodies"]):
+        for key, skey in (("position", "pos"), ("linear_velocity", "vel"), ("angular_velocity", "omega"),
+                          ("orientation_wxyz", "quat")):
+            assert np.allclose(a[key], b[key], rtol=1e-5, atol=1e-6), key
+            assert np.allclose(a[key], s[skey], rtol=1e-5, atol=1e-6), skey
+        assert b["radius"] == 0.025 and b["half_height"] == 0.25
+    E = np.loadtxt(tmp_path / "ends.txt", delimiter=",", comments="#", skiprows=5)
+    got = np.array([r["endpoint_a"] + r["endpoint_b"] for r in ref])
+    assert np.allclose(E, got, atol=2e-6)
+    sim2.step(2)
+    assert sim2.diagnostics()["frame_index"] == 7
+    sim.close()
+    sim2.close()
