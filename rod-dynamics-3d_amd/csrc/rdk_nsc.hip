@@ -469,39 +469,58 @@ struct StallTrack {
 };
 
 // One projected Gauss-Seidel update of manifold i (nsc_contact.cpp:385-552).
-// Body velocities are read once into registers and written back once; the
-// arithmetic is the serial reference update in the same order.
-template <class MP>
+// Every manifold field the update may need is loaded first, unconditionally
+// (the loads are independent, so they are in flight together, and the
+// branches below touch registers only); body velocities follow in a second
+// round.  The arithmetic is the serial reference update in the same order.
+// EARLY = false loads each field where it is first used instead (fewer live
+// registers, for kernels capped at 128 VGPRs).
+template <class MP, bool EARLY = true>
 __device__ __forceinline__ double psor_update(const BodyPtrs& B, MP& M, int i, const SolveConst& K, double mu) {
   const int a = M.ia(i), b = M.ib(i);
   const bool wall = b < 0;
+  const d3 ra = M.get3(F_RA, i), rb = M.get3(F_RB, i);
+  const d3 n = M.get3(F_N, i);
+  const double ima = M.at(F_IMA, i), imb = M.at(F_IMB, i);
+  const double phi = M.at(F_PHI, i), vnpre = M.at(F_VNPRE, i), rest = M.at(F_E, i);
+  const double gn = M.at(F_GN, i);
+  double ln = M.lr(0, i), lt1 = M.lr(1, i), lt2 = M.lr(2, i);
+  d3 IAn, IBn, t1, t2, IAt1, IBt1, IAt2, IBt2;
+  double g1, g2;
+  if (EARLY) {
+    IAn = M.get3(F_IAN, i);
+    IBn = M.get3(F_IBN, i);
+    t1 = M.get3(F_T1, i);
+    t2 = M.get3(F_T2, i);
+    IAt1 = M.get3(F_IAT1, i);
+    IBt1 = M.get3(F_IBT1, i);
+    IAt2 = M.get3(F_IAT2, i);
+    IBt2 = M.get3(F_IBT2, i);
+    g1 = K.proj == 0 ? M.at(F_GT1, i) : M.at(F_GTB, i);
+    g2 = K.proj == 0 ? M.at(F_GT2, i) : 0.0;
+  }
   d3 va = ld3(B.v, a), wa = ld3(B.w, a);
   d3 vb = mk3(0, 0, 0), wb = mk3(0, 0, 0);
   if (!wall) {
     vb = ld3(B.v, b);
     wb = ld3(B.w, b);
   }
-  const d3 ra = M.get3(F_RA, i), rb = M.get3(F_RB, i);
-  const d3 n = M.get3(F_N, i);
-  const double ima = M.at(F_IMA, i), imb = M.at(F_IMB, i);
   d3 v_rel = wall ? -(va + cross(wa, ra)) : (vb + cross(wb, rb)) - (va + cross(wa, ra));
-  const double phi = M.at(F_PHI, i), vnpre = M.at(F_VNPRE, i);
   const double b_n = (phi < 0.0) ? K.beta * phi / K.dt : 0.0;
-  const double b_rest = (vnpre < 0.0) ? M.at(F_E, i) * vnpre : 0.0;
-  double ln = M.lr(0, i), lt1 = M.lr(1, i), lt2 = M.lr(2, i);
+  const double b_rest = (vnpre < 0.0) ? rest * vnpre : 0.0;
   const double w_n = dot(n, v_rel) + b_n + b_rest + K.cfm * ln;
-  const double delta_n = -(K.omega / M.at(F_GN, i)) * w_n;
+  const double delta_n = -(K.omega / gn) * w_n;
   const double old_n = ln;
   ln = dmax(0.0, old_n + delta_n);
   const double dn = ln - old_n;
   if (dn != 0.0) {
-    const d3 IAn = M.get3(F_IAN, i);
+    if (!EARLY) IAn = M.get3(F_IAN, i);
     const d3 dvA = n * (dn * ima);
     va = va - dvA;
     wa = wa - IAn * dn;
     v_rel = v_rel + (dvA + cross(IAn * dn, ra));
     if (!wall) {
-      const d3 IBn = M.get3(F_IBN, i);
+      if (!EARLY) IBn = M.get3(F_IBN, i);
       const d3 dvB = n * (dn * imb);
       vb = vb + dvB;
       wb = wb + IBn * dn;
@@ -509,23 +528,28 @@ __device__ __forceinline__ double psor_update(const BodyPtrs& B, MP& M, int i, c
     }
   }
   const double max_fric = mu * ln;
-  const d3 t1 = M.get3(F_T1, i), t2 = M.get3(F_T2, i);
+  if (!EARLY) {
+    t1 = M.get3(F_T1, i);
+    t2 = M.get3(F_T2, i);
+    g1 = K.proj == 0 ? M.at(F_GT1, i) : M.at(F_GTB, i);
+    g2 = K.proj == 0 ? M.at(F_GT2, i) : 0.0;
+  }
   double dt1, dt2;
   if (K.proj == 0) {
     const double limit_t1 = sqrt(dmax(0.0, max_fric * max_fric - lt2 * lt2));
     const double w_t1 = dot(t1, v_rel) + K.cfm * lt1;
-    const double delta_t1 = -(K.omega / M.at(F_GT1, i)) * w_t1;
+    const double delta_t1 = -(K.omega / g1) * w_t1;
     const double old_t1 = lt1;
     lt1 = dclamp(old_t1 + delta_t1, -limit_t1, limit_t1);
     dt1 = lt1 - old_t1;
     if (dt1 != 0.0) {
-      const d3 IAt1 = M.get3(F_IAT1, i);
+      if (!EARLY) IAt1 = M.get3(F_IAT1, i);
       const d3 dvA = t1 * (dt1 * ima);
       va = va - dvA;
       wa = wa - IAt1 * dt1;
       v_rel = v_rel + (dvA + cross(IAt1 * dt1, ra));
       if (!wall) {
-        const d3 IBt1 = M.get3(F_IBT1, i);
+        if (!EARLY) IBt1 = M.get3(F_IBT1, i);
         const d3 dvB = t1 * (dt1 * imb);
         vb = vb + dvB;
         wb = wb + IBt1 * dt1;
@@ -534,7 +558,7 @@ __device__ __forceinline__ double psor_update(const BodyPtrs& B, MP& M, int i, c
     }
     const double limit_t2 = sqrt(dmax(0.0, max_fric * max_fric - lt1 * lt1));
     const double w_t2 = dot(t2, v_rel) + K.cfm * lt2;
-    const double delta_t2 = -(K.omega / M.at(F_GT2, i)) * w_t2;
+    const double delta_t2 = -(K.omega / g2) * w_t2;
     const double old_t2 = lt2;
     lt2 = dclamp(old_t2 + delta_t2, -limit_t2, limit_t2);
     dt2 = lt2 - old_t2;
@@ -542,7 +566,7 @@ __device__ __forceinline__ double psor_update(const BodyPtrs& B, MP& M, int i, c
     // Euclidean disc: one scalar step omega/g_tb for the tangential pair
     const double w_t1 = dot(t1, v_rel) + K.cfm * lt1;
     const double w_t2 = dot(t2, v_rel) + K.cfm * lt2;
-    const double st = K.omega / M.at(F_GTB, i);
+    const double st = K.omega / g1;
     double y1 = lt1 - st * w_t1;
     double y2 = lt2 - st * w_t2;
     const double ny = sqrt(y1 * y1 + y2 * y2);
@@ -556,20 +580,24 @@ __device__ __forceinline__ double psor_update(const BodyPtrs& B, MP& M, int i, c
     lt1 = y1;
     lt2 = y2;
     if (dt1 != 0.0) {
+      if (!EARLY) IAt1 = M.get3(F_IAT1, i);
       va = va - t1 * (dt1 * ima);
-      wa = wa - M.get3(F_IAT1, i) * dt1;
+      wa = wa - IAt1 * dt1;
       if (!wall) {
+        if (!EARLY) IBt1 = M.get3(F_IBT1, i);
         vb = vb + t1 * (dt1 * imb);
-        wb = wb + M.get3(F_IBT1, i) * dt1;
+        wb = wb + IBt1 * dt1;
       }
     }
   }
   if (dt2 != 0.0) {
+    if (!EARLY) IAt2 = M.get3(F_IAT2, i);
     va = va - t2 * (dt2 * ima);
-    wa = wa - M.get3(F_IAT2, i) * dt2;
+    wa = wa - IAt2 * dt2;
     if (!wall) {
+      if (!EARLY) IBt2 = M.get3(F_IBT2, i);
       vb = vb + t2 * (dt2 * imb);
-      wb = wb + M.get3(F_IBT2, i) * dt2;
+      wb = wb + IBt2 * dt2;
     }
   }
   M.lr(0, i) = ln;
@@ -835,7 +863,7 @@ __global__ void __launch_bounds__(SMALL_BLOCK) k_psor_small(BodyPtrs B, MfPtrs M
       dref = 0.0;
       for (int c = 0; c < ncolor; ++c) {
         for (int i = cs[c] + threadIdx.x; i < cs[c + 1]; i += blockDim.x)
-          dref = dmax(dref, psor_update(B, M, i, K, sys[B.sys[M.a[i]]].mu));
+          dref = dmax(dref, psor_update<MfPtrs, false>(B, M, i, K, sys[B.sys[M.a[i]]].mu));
         __syncthreads();
       }
     }
