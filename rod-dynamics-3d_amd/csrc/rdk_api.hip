@@ -118,6 +118,17 @@ extern "C" {
 
 int rdk_version(void) { return 1; }
 
+static void destroy_streams(rdk_ctx* ctx) {
+  for (hipEvent_t* e : {&ctx->ev_fork, &ctx->ev_join, &ctx->ev_join3, &ctx->ev_join4}) {
+    if (*e) (void)hipEventDestroy(*e);
+    *e = nullptr;
+  }
+  for (hipStream_t* s : {&ctx->stream4, &ctx->stream3, &ctx->stream2, &ctx->stream}) {
+    if (*s) (void)hipStreamDestroy(*s);
+    *s = nullptr;
+  }
+}
+
 int rdk_create(int device, rdk_ctx** out) {
   if (!out) return RDK_ERR_ARG;
   *out = nullptr;
@@ -130,8 +141,13 @@ int rdk_create(int device, rdk_ctx** out) {
   ctx->device = device;
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->stream3, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->stream4, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->ev_join3, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->ev_join4, hipEventDisableTiming) != hipSuccess) {
+    destroy_streams(ctx);
     delete ctx;
     return RDK_ERR_HIP;
   }
@@ -179,10 +195,7 @@ int rdk_destroy(rdk_ctx* ctx) {
     if (r.b) (void)hipEventDestroy(r.b);
   }
   for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
-  if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
-  if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
-  if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
-  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  destroy_streams(ctx);
   delete ctx;
   return RDK_OK;
 }

@@ -98,8 +98,9 @@ struct EventRec {
 struct rdk_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipStream_t stream2 = nullptr;  // side stream (default priority)
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipStream_t stream2 = nullptr;  // side streams (default priority): island classes run concurrently
+  hipStream_t stream3 = nullptr, stream4 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join3 = nullptr, ev_join4 = nullptr;
   std::string err;
 
   // ---- bodies (planar FP64) ----

@@ -1928,14 +1928,20 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
   RDK_HIP(hipMemsetAsync(S, 0, sizeof(IslandStats), ctx->stream));
   BodyPtrs B = body_ptrs(ctx);
   MfPtrs Mc = mf_ptrs(ctx, true);
-  // Tiny islands, and the host-driven solve of huge islands, run on the side
-  // stream, concurrently with the other classes (islands share no body).
-  const bool side = (cnt[0] > 0 || cnt[4] > 0) && (cnt[1] > 0 || cnt[2] > 0 || cnt[3] > 0 || (cnt[0] > 0 && cnt[4] > 0)) &&
-                    ctx->stream2;
-  hipStream_t s_side = side ? ctx->stream2 : ctx->stream;
-  if (side) {
+  // The classes run concurrently (islands share no body): tiny islands and
+  // the host-driven solve of huge islands on stream2, small on stream3, medium
+  // on stream4, large (LDS) on the main stream, joined before the impulses
+  // are unpermuted.
+  const bool side_used = cnt[0] > 0 || cnt[4] > 0;
+  const int used = (side_used ? 1 : 0) + (cnt[1] > 0) + (cnt[2] > 0) + (cnt[3] > 0);
+  const bool multi = used > 1 && ctx->stream2 && ctx->stream3 && ctx->stream4;
+  hipStream_t s_side = multi && side_used ? ctx->stream2 : ctx->stream;
+  hipStream_t s_small = multi && cnt[1] > 0 ? ctx->stream3 : ctx->stream;
+  hipStream_t s_med = multi && cnt[2] > 0 ? ctx->stream4 : ctx->stream;
+  if (multi) {
     RDK_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
-    RDK_HIP(hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
+    for (hipStream_t sx : {s_side, s_small, s_med})
+      if (sx != ctx->stream) RDK_HIP(hipStreamWaitEvent(sx, ctx->ev_fork, 0));
   }
   if (cnt[0] > 0) {
     ProfScope ps(ctx, K_ISL_TINY, s_side);
@@ -1944,13 +1950,13 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
                        ctx->sys_dev.p, ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p, cnt[0], P, S);
   }
   if (cnt[1] > 0) {
-    ProfScope ps(ctx, K_ISL_SMALL);
-    hipLaunchKernelGGL((k_island_reg<ISL_G1, ISL_WG1>), dim3(cnt[1]), dim3(ISL_WG1), 0, ctx->stream, B, Mc,
+    ProfScope ps(ctx, K_ISL_SMALL, s_small);
+    hipLaunchKernelGGL((k_island_reg<ISL_G1, ISL_WG1>), dim3(cnt[1]), dim3(ISL_WG1), 0, s_small, B, Mc,
                        ctx->sys_dev.p, ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p + nisl, cnt[1], P, S);
   }
   if (cnt[2] > 0) {
-    ProfScope ps(ctx, K_ISL_MEDIUM);
-    hipLaunchKernelGGL((k_island_reg<ISL_G2, ISL_WG2>), dim3(cnt[2]), dim3(ISL_WG2), 0, ctx->stream, B, Mc,
+    ProfScope ps(ctx, K_ISL_MEDIUM, s_med);
+    hipLaunchKernelGGL((k_island_reg<ISL_G2, ISL_WG2>), dim3(cnt[2]), dim3(ISL_WG2), 0, s_med, B, Mc,
                        ctx->sys_dev.p, ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p + 2 * nisl, cnt[2], P,
                        S);
   }
@@ -1962,9 +1968,14 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
   }
   HugeOut hu;
   if (cnt[4] > 0) RDK_TRY(huge_islands_solve(ctx, P, skey, ctx->ilists.p + 4 * nisl, cnt[4], s_side, hu));
-  if (side) {
-    RDK_HIP(hipEventRecord(ctx->ev_join, ctx->stream2));
-    RDK_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+  if (multi) {
+    const std::pair<hipStream_t, hipEvent_t> joins[3] = {
+        {s_side, ctx->ev_join}, {s_small, ctx->ev_join3}, {s_med, ctx->ev_join4}};
+    for (const auto& j : joins) {
+      if (j.first == ctx->stream) continue;
+      RDK_HIP(hipEventRecord(j.second, j.first));
+      RDK_HIP(hipStreamWaitEvent(ctx->stream, j.second, 0));
+    }
   }
   {
     ProfScope ps(ctx, K_PERMUTE);
