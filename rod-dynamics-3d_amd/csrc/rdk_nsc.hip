@@ -626,6 +626,7 @@ __global__ void __launch_bounds__(256) k_psor_sweep(BodyPtrs B, MfPtrs M, const 
   int i = i0 + blockIdx.x * blockDim.x + threadIdx.x;
   double d = 0.0;
   if (i < i1) d = psor_update(B, M, i, K, body_mu(B, sys, M.a[i]));
+  if (!ref_res) return;  // the caller does not need this sweep's largest change
   for (int off = 32; off > 0; off >>= 1) d = dmax(d, __shfl_down(d, off));
   if (__lane_id() == 0 && d > 0.0) atomic_max_pos(ref_res, d);
 }
@@ -1582,7 +1583,7 @@ static int run_sweeps(rdk_ctx* ctx, const rdk_nsc_params* p, int nsweeps) {
   BodyPtrs B = body_ptrs(ctx);
   MfPtrs M = mf_ptrs(ctx, true);
   SolveConst K = solve_const(ctx, p);
-  const int nb = 256;
+  const int nb = 64;  // one-wave workgroups spread a colour over more CUs
   double* ref = ctx->resid.p + 1;
   for (int s = 0; s < nsweeps; ++s) {
     RDK_HIP(hipMemsetAsync(ref, 0, sizeof(double), ctx->stream));
@@ -1707,7 +1708,9 @@ static int huge_islands_solve(rdk_ctx* ctx, const IslandSolve& P, const uint32_t
   RDK_HIP(hipStreamSynchronize(st));
   BodyPtrs B = body_ptrs(ctx);
   MfPtrs Mc = mf_ptrs(ctx, true);
-  const int nb = 256;
+  // one-wave workgroups: a colour's ~1000 manifolds spread over 16 CUs (and all
+  // XCDs) instead of 4; measured 4.9 us per colour launch against 7.1 us with 256
+  const int nb = 64;
   double* dres = ctx->hres.p;
   double* dref = ctx->hres.p + 1;
   const int kk = P.stop_rule == 1 ? P.check_every : 1;
@@ -1728,18 +1731,19 @@ static int huge_islands_solve(rdk_ctx* ctx, const IslandSolve& P, const uint32_t
           if (it == kk - 1) {
             hipLaunchKernelGGL(k_zero_doubles, dim3(1), dim3(64), 0, st, ctx->hres.p, 2);
           }
+          double* ref = it == kk - 1 ? dref : nullptr;  // only the last sweep's change is reported
           for (int c = 0; c < ncol; ++c) {
             if (cs[c + 1] <= cs[c]) continue;
             EventRec rec;
             if (time_sweeps && prof_ext_events(ctx, K_PSOR_SWEEP, rec)) {
               hipExtLaunchKernelGGL(k_psor_sweep, dim3(grid_for(cs[c + 1] - cs[c], nb)), dim3(nb), 0, st, rec.a,
-                                    rec.b, 0, B, Mc, ctx->sys_dev.p, cs[c], cs[c + 1], Kc, dref);
+                                    rec.b, 0, B, Mc, ctx->sys_dev.p, cs[c], cs[c + 1], Kc, ref);
               prof_ext_done(ctx, rec);
               ctx->work[W_PSOR_TIMED] += cs[c + 1] - cs[c];
             } else {
               ProfScope ps(ctx, K_ISL_HUGE, st);
               hipLaunchKernelGGL(k_psor_sweep, dim3(grid_for(cs[c + 1] - cs[c], nb)), dim3(nb), 0, st, B, Mc,
-                                 ctx->sys_dev.p, cs[c], cs[c + 1], Kc, dref);
+                                 ctx->sys_dev.p, cs[c], cs[c + 1], Kc, ref);
             }
           }
         }
