@@ -169,6 +169,7 @@ struct rdk_ctx {
   rdk::DevBuf<double> ent_a[3], ent_e[3], ent_out, ent_psum;  // entanglement observer
   rdk::DevBuf<unsigned long long> ent_pcnt;
   rdk::DevBuf<double> hres;   // residual and last-sweep change of the huge-island solve
+  rdk::DevBuf<double> hstate;  // device-side state of the huge-island solve
   rdk::DevBuf<int2> islot;  // local body slots of large-island manifolds
   // position projection
   rdk::DevBuf<int> ppa, ppb, pflag, ppos;

@@ -475,8 +475,11 @@ struct StallTrack {
 // round.  The arithmetic is the serial reference update in the same order.
 // EARLY = false loads each field where it is first used instead (fewer live
 // registers, for kernels capped at 128 VGPRs).
+// store = false computes the update and writes nothing (a sweep the
+// huge-island solve launched after its stop rule fired).
 template <class MP, bool EARLY = true>
-__device__ __forceinline__ double psor_update(const BodyPtrs& B, MP& M, int i, const SolveConst& K, double mu) {
+__device__ __forceinline__ double psor_update(const BodyPtrs& B, MP& M, int i, const SolveConst& K, double mu,
+                                              bool store = true) {
   const int a = M.ia(i), b = M.ib(i);
   const bool wall = b < 0;
   const d3 ra = M.get3(F_RA, i), rb = M.get3(F_RB, i);
@@ -600,6 +603,7 @@ __device__ __forceinline__ double psor_update(const BodyPtrs& B, MP& M, int i, c
       wb = wb + IBt2 * dt2;
     }
   }
+  if (!store) return fabs(dn) + fabs(dt1) + fabs(dt2);
   M.lr(0, i) = ln;
   M.lr(1, i) = lt1;
   M.lr(2, i) = lt2;
@@ -621,12 +625,16 @@ __device__ __forceinline__ void atomic_max_pos(double* addr, double v) {
   atomicMax((unsigned long long*)addr, (unsigned long long)__double_as_longlong(v));
 }
 
+// skip (may be null): when *skip is set the sweep writes nothing; it is read
+// alongside the manifold fields, so checking it adds no round trip.
 __global__ void __launch_bounds__(256) k_psor_sweep(BodyPtrs B, MfPtrs M, const SysDev* __restrict__ sys, int i0,
-                                                    int i1, SolveConst K, double* __restrict__ ref_res) {
+                                                    int i1, SolveConst K, double* __restrict__ ref_res,
+                                                    const int* __restrict__ skip) {
   int i = i0 + blockIdx.x * blockDim.x + threadIdx.x;
+  const bool sk = skip && *skip != 0;
   double d = 0.0;
-  if (i < i1) d = psor_update(B, M, i, K, body_mu(B, sys, M.a[i]));
-  if (!ref_res) return;  // the caller does not need this sweep's largest change
+  if (i < i1) d = psor_update(B, M, i, K, body_mu(B, sys, M.a[i]), !sk);
+  if (!ref_res || sk) return;  // the caller does not need this sweep's largest change
   for (int off = 32; off > 0; off >>= 1) d = dmax(d, __shfl_down(d, off));
   if (__lane_id() == 0 && d > 0.0) atomic_max_pos(ref_res, d);
 }
@@ -1593,12 +1601,12 @@ static int run_sweeps(rdk_ctx* ctx, const rdk_nsc_params* p, int nsweeps) {
       EventRec rec;
       if (prof_ext_events(ctx, K_PSOR_SWEEP, rec)) {
         hipExtLaunchKernelGGL(k_psor_sweep, dim3(grid_for(i1 - i0, nb)), dim3(nb), 0, ctx->stream, rec.a, rec.b, 0,
-                              B, M, ctx->sys_dev.p, i0, i1, K, ref);
+                              B, M, ctx->sys_dev.p, i0, i1, K, ref, (const int*)nullptr);
         prof_ext_done(ctx, rec);
         ctx->work[W_PSOR_TIMED] += i1 - i0;
       } else {
         hipLaunchKernelGGL(k_psor_sweep, dim3(grid_for(i1 - i0, nb)), dim3(nb), 0, ctx->stream, B, M,
-                           ctx->sys_dev.p, i0, i1, K, ref);
+                           ctx->sys_dev.p, i0, i1, K, ref, (const int*)nullptr);
       }
     }
   }
@@ -1690,6 +1698,39 @@ struct HugeOut {
   double res = 0.0, res_ref = 0.0;
 };
 
+// Device-side state of one huge-island solve.  The check after each block of
+// sweeps (sweep count, stop rules, stall rule) runs on the GPU, so several
+// blocks go out per host round trip; once `done` is set, the sweeps of blocks
+// already launched compute but write nothing.  done: 0 running, 1 finished,
+// 2 stalled with omega to lower to stall_omega (the host recaptures).
+struct HugeState {
+  int sweeps, done, pad0, pad1;
+  double res, res_ref;
+  StallTrack trk;
+};
+
+// The host loop's checks after one block, in its order: rule 0 (largest
+// change < 1e-6), the stall rule (only while res > tol), then the sweep budget
+// and the tolerance.
+__global__ void k_huge_check(HugeState* st, const double* __restrict__ r, int kk, int max_sweeps, int stop_rule,
+                             double tol, double omega, double stall_omega) {
+  if (threadIdx.x != 0 || st->done != 0) return;
+  const int sweeps = st->sweeps + kk;
+  const double res = r[0], res_ref = r[1];
+  st->sweeps = sweeps;
+  st->res = res;
+  st->res_ref = res_ref;
+  if (stop_rule == 0 && res_ref < 1e-6) {
+    st->done = 1;
+    return;
+  }
+  if (stop_rule == 1 && !(res <= tol) && st->trk.stalled(sweeps, res)) {
+    st->done = (stall_omega > 0.0 && omega > stall_omega && sweeps < max_sweeps) ? 2 : 1;
+    return;
+  }
+  if (sweeps >= max_sweeps || (stop_rule == 1 && res <= tol)) st->done = 1;
+}
+
 __global__ void k_zero_doubles(double* __restrict__ p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0.0;
@@ -1699,6 +1740,7 @@ static int huge_islands_solve(rdk_ctx* ctx, const IslandSolve& P, const uint32_t
                               hipStream_t st, HugeOut& out) {
   RDK_TRY(ensure(ctx, ctx->hcs, (size_t)nh * HUGE_CS));
   RDK_TRY(ensure(ctx, ctx->hres, 2));
+  RDK_TRY(ensure(ctx, ctx->hstate, (sizeof(HugeState) + sizeof(double) - 1) / sizeof(double)));
   {
     ProfScope ps(ctx, K_ISL_HUGE, st);
     hipLaunchKernelGGL(k_island_colours, dim3(nh), dim3(128), 0, st, ctx->istart.p, skey, list, nh, ctx->hcs.p);
@@ -1713,14 +1755,19 @@ static int huge_islands_solve(rdk_ctx* ctx, const IslandSolve& P, const uint32_t
   const int nb = 64;
   double* dres = ctx->hres.p;
   double* dref = ctx->hres.p + 1;
+  HugeState* dst = (HugeState*)ctx->hstate.p;
+  const int* skip = &dst->done;
   const int kk = P.stop_rule == 1 ? P.check_every : 1;
-  // Graph replay of one check block (kk sweeps, then the residual).  The
-  // counters are cleared by a kernel: no memset nodes in the graph.  While
-  // the colour launches are timed, one block in 16 runs eagerly with its
-  // launches timed (a sample over the whole run); timing the whole huge path
-  // turns the graphs off.
+  // A block is kk sweeps, the residual and the device-side check.  Blocks are
+  // replayed from captured graphs: one block per replay for the first two
+  // replays of a solve, then four (what a solve that ends mid-replay computes
+  // after its end is discarded).  The counters are cleared by a kernel: no
+  // memset nodes in the graphs.  While the colour launches are timed, one
+  // replay in 16 runs eagerly as a single block with its launches timed (a
+  // sample over the whole run); timing the whole huge path turns the graphs off.
   const bool graphs = graphs_enabled() && !timed(ctx, K_ISL_HUGE);
   const bool sample = timed(ctx, K_PSOR_SWEEP);
+  constexpr int LONG_REPLAY = 4;
   for (int h = 0; h < nh; ++h) {
     const int* cs = hcs.data() + (size_t)h * HUGE_CS;
     const int ncol = cs[129];
@@ -1737,13 +1784,13 @@ static int huge_islands_solve(rdk_ctx* ctx, const IslandSolve& P, const uint32_t
             EventRec rec;
             if (time_sweeps && prof_ext_events(ctx, K_PSOR_SWEEP, rec)) {
               hipExtLaunchKernelGGL(k_psor_sweep, dim3(grid_for(cs[c + 1] - cs[c], nb)), dim3(nb), 0, st, rec.a,
-                                    rec.b, 0, B, Mc, ctx->sys_dev.p, cs[c], cs[c + 1], Kc, ref);
+                                    rec.b, 0, B, Mc, ctx->sys_dev.p, cs[c], cs[c + 1], Kc, ref, skip);
               prof_ext_done(ctx, rec);
               ctx->work[W_PSOR_TIMED] += cs[c + 1] - cs[c];
             } else {
               ProfScope ps(ctx, K_ISL_HUGE, st);
               hipLaunchKernelGGL(k_psor_sweep, dim3(grid_for(cs[c + 1] - cs[c], nb)), dim3(nb), 0, st, B, Mc,
-                                 ctx->sys_dev.p, cs[c], cs[c + 1], Kc, ref);
+                                 ctx->sys_dev.p, cs[c], cs[c + 1], Kc, ref, skip);
             }
           }
         }
@@ -1753,15 +1800,9 @@ static int huge_islands_solve(rdk_ctx* ctx, const IslandSolve& P, const uint32_t
       ProfScope ps(ctx, K_ISL_HUGE, st);
       hipLaunchKernelGGL(k_residual, dim3(grid_for(e - s, nb)), dim3(nb), 0, st, B, Mc, ctx->sys_dev.p, s, e, Kc,
                          dres);
-    };
-    auto read_back = [&](double* res, double* ref) -> int {
-      double hv[2];
-      RDK_HIP(hipMemcpyAsync(hv, dres, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
-      RDK_HIP(hipStreamSynchronize(st));
-      *res = hv[0];
-      if (ref) *ref = hv[1];
-      ctx->work[W_RESID_CONTACTS] += e - s;
-      return RDK_OK;
+      if (sweep)
+        hipLaunchKernelGGL(k_huge_check, dim3(1), dim3(64), 0, st, dst, (const double*)ctx->hres.p, kk,
+                           P.max_sweeps, P.stop_rule, P.tol, Kc.omega, Kc.stall_omega);
     };
     if (P.warm) {
       for (int c = 0; c < ncol; ++c) {
@@ -1771,68 +1812,77 @@ static int huge_islands_solve(rdk_ctx* ctx, const IslandSolve& P, const uint32_t
                            cs[c + 1]);
       }
     }
-    int sweeps = 0;
-    double res = 0.0, res_ref = 0.0;
-    StallTrack trk;
-    trk.init(P.stall);
+    HugeState hs{};
+    hs.trk.init(P.stall);
     SolveConst K = P.K;  // omega may drop to stall_omega on a stall
     if (P.stop_rule == 1) {
       enqueue_block(K, false, false);
-      RDK_TRY(read_back(&res, nullptr));
+      double r0 = 0.0;
+      RDK_HIP(hipMemcpyAsync(&r0, dres, sizeof(double), hipMemcpyDeviceToHost, st));
+      RDK_HIP(hipStreamSynchronize(st));
+      hs.res = r0;
+      ctx->work[W_RESID_CONTACTS] += e - s;
     }
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    auto drop_graph = [&]() {
-      if (exec) (void)hipGraphExecDestroy(exec);
-      if (graph) (void)hipGraphDestroy(graph);
-      exec = nullptr;
-      graph = nullptr;
+    hs.done = (P.max_sweeps <= 0 || (P.stop_rule == 1 && hs.res <= P.tol)) ? 1 : 0;
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    auto drop_graphs = [&]() {
+      for (int g = 0; g < 2; ++g) {
+        if (exec[g]) (void)hipGraphExecDestroy(exec[g]);
+        if (graph[g]) (void)hipGraphDestroy(graph[g]);
+        exec[g] = nullptr;
+        graph[g] = nullptr;
+      }
     };
     int rc = RDK_OK;
-    while (rc == RDK_OK && sweeps < P.max_sweeps && !(P.stop_rule == 1 && res <= P.tol)) {
+    int replays = 0;
+    if (hs.done != 1) RDK_HIP(hipMemcpyAsync(dst, &hs, sizeof(HugeState), hipMemcpyHostToDevice, st));
+    while (rc == RDK_OK && hs.done != 1) {
       const bool timed_block = sample && (ctx->huge_blocks++ % 16 == 0);
       if (graphs && !timed_block) {
-        if (!exec) {
+        const int g = replays < 2 ? 0 : 1;
+        const int nblk = g == 0 ? 1 : LONG_REPLAY;
+        if (!exec[g]) {
           if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
             rc = RDK_ERR_HIP;
             break;
           }
-          enqueue_block(K, true, false);
-          if (hipStreamEndCapture(st, &graph) != hipSuccess ||
-              hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+          for (int b = 0; b < nblk; ++b) enqueue_block(K, true, false);
+          if (hipStreamEndCapture(st, &graph[g]) != hipSuccess ||
+              hipGraphInstantiate(&exec[g], graph[g], nullptr, nullptr, 0) != hipSuccess) {
             rc = RDK_ERR_HIP;
             break;
           }
         }
-        if (hipGraphLaunch(exec, st) != hipSuccess) {
+        if (hipGraphLaunch(exec[g], st) != hipSuccess) {
           rc = RDK_ERR_HIP;
           break;
         }
       } else {
         enqueue_block(K, true, timed_block);
       }
-      sweeps += kk;
-      if ((rc = read_back(&res, &res_ref)) != RDK_OK) break;
-      if (P.stop_rule == 0 && res_ref < 1e-6) break;
-      if (P.stop_rule == 1 && !(res <= P.tol) && trk.stalled(sweeps, res)) {
-        if (K.stall_omega > 0.0 && K.omega > K.stall_omega) {
-          K.omega = K.stall_omega;
-          drop_graph();  // the sweeps are recaptured with the new omega
-        } else {
-          break;
-        }
+      ++replays;
+      const int before = hs.sweeps;
+      RDK_HIP(hipMemcpyAsync(&hs, dst, sizeof(HugeState), hipMemcpyDeviceToHost, st));
+      RDK_HIP(hipStreamSynchronize(st));
+      ctx->work[W_RESID_CONTACTS] += (int64_t)((hs.sweeps - before) / kk) * (e - s);
+      if (hs.done == 2) {
+        K.omega = K.stall_omega;
+        drop_graphs();  // the sweeps are recaptured with the new omega
+        hs.done = 0;
+        RDK_HIP(hipMemcpyAsync(dst, &hs, sizeof(HugeState), hipMemcpyHostToDevice, st));
       }
     }
-    drop_graph();
+    drop_graphs();
     if (rc != RDK_OK) {
       ctx->err = "huge-island solve: graph capture or launch failed";
       return rc;
     }
-    out.max_sweeps = std::max(out.max_sweeps, sweeps);
-    out.manifold_sweeps += (int64_t)sweeps * (e - s);
-    out.res = std::max(out.res, res);
-    out.res_ref = std::max(out.res_ref, res_ref);
-    if (P.stop_rule == 1 && !(res <= P.tol)) out.nonconv++;
+    out.max_sweeps = std::max(out.max_sweeps, hs.sweeps);
+    out.manifold_sweeps += (int64_t)hs.sweeps * (e - s);
+    out.res = std::max(out.res, hs.res);
+    out.res_ref = std::max(out.res_ref, hs.res_ref);
+    if (P.stop_rule == 1 && !(hs.res <= P.tol)) out.nonconv++;
   }
   return RDK_OK;
 }

@@ -425,9 +425,10 @@ def test_huge_island_solve_reaches_residual_checked_by_oracle(gpu_ctx_factory):
 
 
 def test_huge_island_graph_replays_bitwise_equal_serial_sweeps(gpu_ctx_factory):
-    """Check blocks of four sweeps replayed from a captured graph (three
-    replays, a host read-back between them) equal twelve serial sweeps of the
-    oracle in the same order, bit for bit."""
+    """Check blocks of four sweeps replayed from captured graphs (one block,
+    one block, then a replay of four blocks whose last one starts after the
+    sweep budget is spent and must store nothing) equal twelve serial sweeps
+    of the oracle in the same order, bit for bit."""
     B = log_pile(ny=70, nz=40, seed=100)
     ctx = gpu_ctx_factory()
     gpu_load(ctx, B)
@@ -446,6 +447,39 @@ def test_huge_island_graph_replays_bitwise_equal_serial_sweeps(gpu_ctx_factory):
     W.set_state(v=s0["v"], w=s0["w"])
     W.set_manifolds(rec, ab, lam0)
     W.sweeps(O.nsc_params(mu=B["mu"], beta=0.1, cfm=1e-4, omega=1.3, friction_proj=1), 12, order)
+    _, _, lam_o = W.manifolds()
+    so = W.state()
+    assert np.array_equal(lam_g, lam_o)
+    assert np.array_equal(s1["v"], so["v"]) and np.array_equal(s1["w"], so["w"])
+
+
+def test_huge_island_stops_mid_replay_at_the_tolerance(gpu_ctx_factory):
+    """The device-side check ends the solve at the first block whose residual
+    meets the tolerance, inside a multi-block replay: the impulses and
+    velocities equal the oracle's serial sweeps for exactly the reported
+    sweep count, so the sweeps launched after the stop wrote nothing."""
+    B = log_pile(ny=70, nz=40, seed=101)
+    ctx = gpu_ctx_factory()
+    gpu_load(ctx, B)
+    p = L.nsc_params(beta=0.0, cfm=0.0, omega=1.3, tol=1e-4, max_sweeps=20000, check_every=4, friction_proj=1,
+                     warm_mode=0, stall_sweeps=0)
+    ctx.detect(L.detect_params())
+    nm = ctx.nsc_build(p)
+    rec, ab, lam0 = ctx.nsc_manifolds(nm)
+    m, nb = largest_island(ab, len(B["r"]))
+    assert nb > 2560 and m <= 8192, (m, nb)
+    s0 = ctx.download_state()
+    st = ctx.nsc_solve(p, allow_noconv=True)
+    k = st["sweeps"]
+    print(f"huge island converged after {k} sweeps, residual {st['residual']}", flush=True)
+    assert k % 4 == 0 and k > 8 and st["residual"] <= 1e-4, st
+    order = ctx.solve_order(nm)
+    _, _, lam_g = ctx.nsc_manifolds(nm)
+    s1 = ctx.download_state()
+    W = oracle_load(B)
+    W.set_state(v=s0["v"], w=s0["w"])
+    W.set_manifolds(rec, ab, lam0)
+    W.sweeps(O.nsc_params(mu=B["mu"], beta=0.0, cfm=0.0, omega=1.3, friction_proj=1), k, order)
     _, _, lam_o = W.manifolds()
     so = W.state()
     assert np.array_equal(lam_g, lam_o)
