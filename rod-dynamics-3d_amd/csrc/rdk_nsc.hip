@@ -1214,6 +1214,14 @@ struct MfSlots {
   __device__ __forceinline__ double& lr(int c, int i) const { return M.lam[c][i]; }
 };
 
+__device__ __forceinline__ void load_manifold(MfReg& R, const MfSlots& A, int i) {
+#pragma unroll
+  for (int f = 0; f < F_COUNT; ++f) R.f[f] = A.at(f, i);
+  for (int q = 0; q < 3; ++q) R.lam[q] = A.lr(q, i);
+  R.a = A.ia(i);
+  R.b = A.ib(i);
+}
+
 __global__ void __launch_bounds__(LDS_THREADS) k_island_lds(BodyPtrs B, MfPtrs M, const SysDev* __restrict__ sys,
                                                            const int* __restrict__ istart,
                                                            const uint32_t* __restrict__ skey,
@@ -1273,8 +1281,22 @@ __global__ void __launch_bounds__(LDS_THREADS) k_island_lds(BodyPtrs B, MfPtrs M
     double d = 0.0;
     for (int it = 0; it < kk; ++it) {
       d = 0.0;
+      // Each lane's first manifold of the next colour is loaded into registers
+      // before the barrier that ends the current colour (its fields and
+      // impulses are not written by other colours; the body velocities it
+      // needs are read from LDS after the barrier), so the global-memory
+      // latency overlaps the wait.
+      MfReg R;
+      if (cs[0] + tid < cs[1]) load_manifold(R, A, cs[0] + tid);
       for (int c = 0; c < ncol; ++c) {
-        for (int i = cs[c] + tid; i < cs[c + 1]; i += LDS_THREADS) d = dmax(d, psor_update(LB, A, i, K, mu));
+        int i = cs[c] + tid;
+        if (i < cs[c + 1]) {
+          d = dmax(d, psor_update(LB, R, 0, K, mu));
+          for (int q = 0; q < 3; ++q) A.lr(q, i) = R.lam[q];
+          i += LDS_THREADS;
+        }
+        for (; i < cs[c + 1]; i += LDS_THREADS) d = dmax(d, psor_update<MfSlots, false>(LB, A, i, K, mu));
+        if (c + 1 < ncol && cs[c + 1] + tid < cs[c + 2]) load_manifold(R, A, cs[c + 1] + tid);
         __syncthreads();
       }
     }
