@@ -1202,7 +1202,10 @@ constexpr int ISL_G2 = 256, ISL_WG2 = 256;  // medium: one island per 4-wave wor
 // manifold fields and impulses in global memory (L2-resident), each
 // manifold's local body slots looked up once per solve.
 constexpr int LDS_THREADS = 512;
-constexpr int LDS_BCAP = 2560;  // 6 doubles + 1 id per body: 133 KB of the 160 KB LDS
+// 6 doubles per body (the island's sorted body list stays in global memory):
+// 161 280 B of the 160 KiB LDS, enough for the percolating cluster of the
+// settling bed at 1e5 rods (about 3200 bodies)
+constexpr int LDS_BCAP = 3360;
 
 struct MfSlots {
   MfPtrs M;
@@ -1229,16 +1232,15 @@ __global__ void __launch_bounds__(LDS_THREADS) k_island_lds(BodyPtrs B, MfPtrs M
                                                            const int* __restrict__ list, IslandSolve P,
                                                            IslandStats* S, int2* __restrict__ slot) {
   __shared__ double v_l[3][LDS_BCAP], w_l[3][LDS_BCAP];
-  __shared__ int body_l[LDS_BCAP];
   __shared__ double red[LDS_THREADS / 64];
   __shared__ int cs[129];
   const int k = list[blockIdx.x];
   const int s = istart[k], e = istart[k + 1];
   const int bs = bseg[2 * k], nb = bseg[2 * k + 1] - bs;
+  const int* body_l = sbody + bs;  // the island's bodies, ascending: LDS slot q holds body body_l[q]
   const int tid = threadIdx.x;
   for (int q = tid; q < nb; q += LDS_THREADS) {
-    const int g = sbody[bs + q];
-    body_l[q] = g;
+    const int g = body_l[q];
     for (int c = 0; c < 3; ++c) {
       v_l[c][q] = B.v[c][g];
       w_l[c][q] = B.w[c][g];
@@ -1295,6 +1297,8 @@ __global__ void __launch_bounds__(LDS_THREADS) k_island_lds(BodyPtrs B, MfPtrs M
           for (int q = 0; q < 3; ++q) A.lr(q, i) = R.lam[q];
           i += LDS_THREADS;
         }
+        // further manifolds of this lane load their fields where first used
+        // (fewer live registers next to the prefetched one)
         for (; i < cs[c + 1]; i += LDS_THREADS) d = dmax(d, psor_update<MfSlots, false>(LB, A, i, K, mu));
         if (c + 1 < ncol && cs[c + 1] + tid < cs[c + 2]) load_manifold(R, A, cs[c + 1] + tid);
         __syncthreads();

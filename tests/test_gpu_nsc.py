@@ -211,7 +211,7 @@ def big_clusters(nclust=3, k=160, seed=91, mu=0.5):
                 side=30.0 * nclust, periodic=(1, 1, 0), walls=True, mu=mu)
 
 
-def log_pile(ny=100, nz=30, seed=97, mu=0.5):
+def log_pile(ny=120, nz=30, seed=97, mu=0.5):
     """Rods along x stacked on a (y, z) lattice with spacing 0.78 < 2r = 0.8
     and small random tilts: every rod overlaps its four lattice neighbours, so
     the pile is one island of ny * nz bodies (more than the LDS island class
@@ -238,7 +238,7 @@ def log_pile(ny=100, nz=30, seed=97, mu=0.5):
 def pile_and_clusters():
     """A huge island (log pile) next to clusters of every smaller class, so the
     huge-island solve runs beside the island kernels on the side stream."""
-    P = log_pile(ny=70, nz=40, seed=99)
+    P = log_pile(ny=90, nz=40, seed=99)
     C = clusters(nclust=30, seed=78)
     C["x"] = C["x"] + np.array([40.0, 0.0, 0.0])
     out = {k: np.concatenate([P[k], C[k]]) for k in ("x", "q", "v", "w", "r", "h", "mass", "rest")}
@@ -291,7 +291,7 @@ def test_island_sweeps_bitwise_equal_serial_sweeps_in_solve_order(gpu_ctx_factor
         assert counts[3] > 0, counts  # large islands (bodies in LDS)
     else:
         m, nb = largest_island(ab, len(B["r"]))
-        assert nb > 2560 and m <= 8192 and counts[3] >= 1, (m, nb, counts)  # a huge island (colour-by-colour path)
+        assert nb > 3360 and m <= 8192 and counts[3] >= 1, (m, nb, counts)  # a huge island (colour-by-colour path)
         if scene == "pile_and_clusters":
             assert counts[0] > 0 and counts[1] > 0, counts
     order = ctx.solve_order(nm)
@@ -396,7 +396,7 @@ def test_huge_island_solve_reaches_residual_checked_by_oracle(gpu_ctx_factory):
     """The host-driven colour-by-colour solve of a huge island (residual
     checks, stall rule) stops at the tolerance; the oracle's residual of the
     GPU impulses confirms it, and the impulses are admissible."""
-    B = log_pile(ny=70, nz=40, seed=98)
+    B = log_pile(ny=90, nz=40, seed=98)
     ctx = gpu_ctx_factory()
     gpu_load(ctx, B)
     p = L.nsc_params(beta=0.0, cfm=0.0, omega=1.3, tol=1e-6, max_sweeps=20000, check_every=4, friction_proj=1,
@@ -405,7 +405,7 @@ def test_huge_island_solve_reaches_residual_checked_by_oracle(gpu_ctx_factory):
     nm = ctx.nsc_build(p)
     rec, ab, lam0 = ctx.nsc_manifolds(nm)
     m, nb = largest_island(ab, len(B["r"]))
-    assert nb > 2560 and m <= 8192, (m, nb)
+    assert nb > 3360 and m <= 8192, (m, nb)
     s0 = ctx.download_state()
     st = ctx.nsc_solve(p)
     assert st["residual"] <= 1e-6 and st["sweeps"] > 4, st
@@ -429,7 +429,7 @@ def test_huge_island_graph_replays_bitwise_equal_serial_sweeps(gpu_ctx_factory):
     one block, then a replay of four blocks whose last one starts after the
     sweep budget is spent and must store nothing) equal twelve serial sweeps
     of the oracle in the same order, bit for bit."""
-    B = log_pile(ny=70, nz=40, seed=100)
+    B = log_pile(ny=90, nz=40, seed=100)
     ctx = gpu_ctx_factory()
     gpu_load(ctx, B)
     p = L.nsc_params(beta=0.1, cfm=1e-4, omega=1.3, tol=0.0, max_sweeps=12, check_every=4, friction_proj=1,
@@ -458,7 +458,7 @@ def test_huge_island_stops_mid_replay_at_the_tolerance(gpu_ctx_factory):
     meets the tolerance, inside a multi-block replay: the impulses and
     velocities equal the oracle's serial sweeps for exactly the reported
     sweep count, so the sweeps launched after the stop wrote nothing."""
-    B = log_pile(ny=70, nz=40, seed=101)
+    B = log_pile(ny=90, nz=40, seed=101)
     ctx = gpu_ctx_factory()
     gpu_load(ctx, B)
     p = L.nsc_params(beta=0.0, cfm=0.0, omega=1.3, tol=1e-4, max_sweeps=20000, check_every=4, friction_proj=1,
@@ -467,7 +467,7 @@ def test_huge_island_stops_mid_replay_at_the_tolerance(gpu_ctx_factory):
     nm = ctx.nsc_build(p)
     rec, ab, lam0 = ctx.nsc_manifolds(nm)
     m, nb = largest_island(ab, len(B["r"]))
-    assert nb > 2560 and m <= 8192, (m, nb)
+    assert nb > 3360 and m <= 8192, (m, nb)
     s0 = ctx.download_state()
     st = ctx.nsc_solve(p, allow_noconv=True)
     k = st["sweeps"]
