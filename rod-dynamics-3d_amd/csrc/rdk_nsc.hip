@@ -1297,9 +1297,13 @@ __global__ void __launch_bounds__(LDS_THREADS) k_island_lds(BodyPtrs B, MfPtrs M
           for (int q = 0; q < 3; ++q) A.lr(q, i) = R.lam[q];
           i += LDS_THREADS;
         }
-        // further manifolds of this lane load their fields where first used
-        // (fewer live registers next to the prefetched one)
-        for (; i < cs[c + 1]; i += LDS_THREADS) d = dmax(d, psor_update<MfSlots, false>(LB, A, i, K, mu));
+        // further manifolds of this lane reuse the same registers: all fields
+        // in one round of loads, then the update
+        for (; i < cs[c + 1]; i += LDS_THREADS) {
+          load_manifold(R, A, i);
+          d = dmax(d, psor_update(LB, R, 0, K, mu));
+          for (int q = 0; q < 3; ++q) A.lr(q, i) = R.lam[q];
+        }
         if (c + 1 < ncol && cs[c + 1] + tid < cs[c + 2]) load_manifold(R, A, cs[c + 1] + tid);
         __syncthreads();
       }
