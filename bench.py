@@ -44,6 +44,7 @@ HBM_PEAK_GBS = 8000.0
 BYTES = {
     "broad_narrow": ("bodies", 72),      # x, q, r, h of every body once per pass
     "psor_sweep": ("psor_timed_updates", 568),  # manifold 328 + lambda r/w 48 + 2x(v,w) gather 96 + scatter 96
+    "island_large": ("lds_manifold_sweeps", 376),  # manifold 328 + lambda r/w 48 (body velocities stay in LDS)
     "residual": ("resid_contacts", 472),
     "manifold": ("manifolds", 696),
     "nsc_free": ("bodies", 280),
@@ -147,9 +148,10 @@ def autotune(ctx, run_steps, cands, reps=3):
 # Dominant kernel of a workload whose warm-up is not representative, from the
 # rocprofv3 kernel statistics of the whole protocol (profiles/): on the
 # settling bed the 20 warm-up steps are the dilute phase (broad phase first),
-# while over the 5000 steps the colour launches of the huge-island PSOR solve
-# take the largest share of GPU time.
-DOMINANT = {"c2": "psor_sweep", "c5": "psor_sweep"}
+# while over the 5000 steps the LDS island kernel takes the largest share of GPU
+# time (the bed's percolating cluster is solved there); on the 2e5-rod bed the
+# colour launches of the huge-island solve.
+DOMINANT = {"c2": "island_large", "c5": "psor_sweep"}
 
 
 def rank_kernels(ctx, run_warmup, fixed=None):
@@ -175,6 +177,7 @@ def roofline(ctx, dom, n, workload):
     units = {"bodies": wc["body_passes"] * n if dom == "broad_narrow" else launches * n,
              "sweep_contacts": wc["sweep_contacts"], "resid_contacts": wc["resid_contacts"],
              "psor_timed_updates": wc["psor_timed_updates"],
+             "lds_manifold_sweeps": wc["lds_manifold_sweeps"],
              "manifolds": wc["manifolds"], "contacts": wc["contacts"], "force_contacts": wc["force_contacts"]}[unit]
     achieved = (bpu * units / 1e9) / (kms / 1e3) if kms > 0 else 0.0
     return {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

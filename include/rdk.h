@@ -247,7 +247,9 @@ int rdk_num_kernels(void);
 /* algorithmic-byte counters accumulated by the fused steps, in this order:
  * steps, detection passes, chunks binned, chunk pairs passing the sphere
  * test, contacts, manifolds, manifold updates in sweeps, manifold residual
- * evaluations, body integrations, force evaluations, FP64 narrow-phase calls */
+ * evaluations, body integrations, force evaluations, FP64 narrow-phase
+ * calls, manifold updates in timed colour launches, manifold updates in
+ * sweeps of the LDS island class */
 int rdk_work_counters(rdk_ctx* ctx, int64_t* counters, int n);
 
 /* ---- scene generation (resetScene populate/randomInit,

@@ -64,6 +64,7 @@ enum WorkId {
   W_FORCE_CONTACTS,   // soft/HM force evaluations
   W_NARROW,           // FP64 narrow-phase calls (pairs passing the separating-axis test)
   W_PSOR_TIMED,       // manifold updates in k_psor_sweep launches whose time was recorded
+  W_LDS_SWEEPS,       // manifold updates in sweeps of the LDS island class
   W_NUM
 };
 

@@ -1013,6 +1013,7 @@ struct IslandStats {
   int max_sweeps, nonconv, pad0, pad1;
   unsigned long long manifold_sweeps;
   double res, res_ref;
+  unsigned long long lds_manifold_sweeps;  // the LDS class's share of manifold_sweeps
 };
 
 __device__ __forceinline__ void island_report(IslandStats* S, int sweeps, int size, double res, double res_ref,
@@ -1326,7 +1327,10 @@ __global__ void __launch_bounds__(LDS_THREADS) k_island_lds(BodyPtrs B, MfPtrs M
       B.w[c][g] = w_l[c][q];
     }
   }
-  if (tid == 0) island_report(S, sweeps, e - s, res, res_ref, P.stop_rule == 1 && !(res <= P.tol));
+  if (tid == 0) {
+    island_report(S, sweeps, e - s, res, res_ref, P.stop_rule == 1 && !(res <= P.tol));
+    atomicAdd(&S->lds_manifold_sweeps, (unsigned long long)sweeps * (unsigned long long)(e - s));
+  }
 }
 
 // Colour starts of listed islands (the huge class), for the host-driven
@@ -2080,6 +2084,7 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
   for (int c = 0; c < 3; ++c) ctx->island_counts[c] = cnt[c];
   ctx->island_counts[3] = cnt[3] + cnt[4];
   ctx->work[W_SWEEP_CONTACTS] += (int64_t)hs.manifold_sweeps + hu.manifold_sweeps;
+  ctx->work[W_LDS_SWEEPS] += (int64_t)hs.lds_manifold_sweeps;
   *status = hs.nonconv + hu.nonconv > 0 ? RDK_ERR_NOCONV : RDK_OK;
   if (st) {
     st->sweeps = std::max(hs.max_sweeps, hu.max_sweeps);

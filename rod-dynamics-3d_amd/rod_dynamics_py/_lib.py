@@ -458,7 +458,8 @@ class Context:
         arr = (C.c_int64 * 16)()
         self._check(self._L.rdk_work_counters(self._h, arr, 16))
         names = ["steps", "body_passes", "chunks", "candidates", "contacts", "manifolds", "sweep_contacts",
-                 "resid_contacts", "integrate_bodies", "force_contacts", "narrow_calls", "psor_timed_updates"]
+                 "resid_contacts", "integrate_bodies", "force_contacts", "narrow_calls", "psor_timed_updates",
+                 "lds_manifold_sweeps"]
         return {nm: arr[i] for i, nm in enumerate(names)}
 
     def affine_scale(self, s):
