@@ -60,7 +60,8 @@ BYTES = {
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 # (FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0, from a calibrated 1 GiB copy)
 # for (workload, kernel); recorded with the command in profiles/
-TRAFFIC = {("c2", "broad_narrow"): 128.2e6 + 0.21e6, ("c4", "broad_narrow"): 479.4e6 + 6.42e6}
+TRAFFIC = {("c2", "broad_narrow"): 128.2e6 + 0.21e6, ("c4", "broad_narrow"): 479.4e6 + 6.42e6,
+           ("c2", "island_large"): 961.8 * 1024 * 2.0 + 759.3 * 1024}
 
 
 def parse():
