@@ -148,6 +148,36 @@ int rdk_get_contacts(rdk_ctx* ctx, int cap, int* a, int* b, double* pa,
                      double* s, double* t, int* type);
 /* test hook: force the pair-buffer capacity (0 = automatic) */
 int rdk_set_pair_capacity(rdk_ctx* ctx, int cap);
+
+/* The reference's GPU plug-in record (GpuContactRaw, cuda_broadphase.hpp:29-38),
+ * same fields, order and types: closest points (point_b on b's periodic
+ * image), normal a -> b, segment distance, r_a + r_b, Lumelsky s and t, the
+ * minimum-image shift applied to b. */
+typedef struct rdk_contact_raw {
+  int a, b;
+  float px_a, py_a, pz_a;
+  float px_b, py_b, pz_b;
+  float nx, ny, nz;
+  float dist;
+  float surface_limit;
+  float s, t;
+  float shift_bx, shift_by, shift_bz;
+} rdk_contact_raw;
+/* The current contact list (after rdk_detect) in that layout, FP32, sorted by
+ * (a, b).  RDK_ERR_CAPACITY when cap < the contact count. */
+int rdk_get_contacts_raw(rdk_ctx* ctx, int cap, rdk_contact_raw* out, int* count);
+/* One call in place of cudaDetectCapsulePairsTwoPass (cuda_broadphase.hpp:95-101):
+ * n capsules given in FP32 as the reference holds them (centres x[3n],
+ * quaternions q[4n] as (w,x,y,z), radii r[n], half lengths h[n]); contact when
+ * the segment distance is below r_a + r_b + margin; periodic images by the
+ * box extents sx, sy, sz when pbc is set (any box origin).  Detection runs in
+ * FP64 on the device; the records are rounded to FP32.  Replaces the bodies
+ * and systems of ctx.  *count = contacts found; RDK_ERR_CAPACITY when cap is
+ * smaller (out then holds the first cap). */
+int rdk_detect_capsule_pairs_raw(rdk_ctx* ctx, int n, const float* x, const float* q,
+                                 const float* r, const float* h, float margin, int pbc,
+                                 float sx, float sy, float sz, int cap,
+                                 rdk_contact_raw* out, int* count);
 int rdk_detect_info(rdk_ctx* ctx, int* n_chunks, int* n_cells,
                     int64_t* candidates, int* regrows);
 

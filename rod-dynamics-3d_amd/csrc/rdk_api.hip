@@ -186,7 +186,7 @@ int rdk_destroy(rdk_ctx* ctx) {
   release(ctx->iflag); release(ctx->ipos); release(ctx->istart); release(ctx->ilists); release(ctx->bseg);
   release(ctx->icnt); release(ctx->blab); release(ctx->blab_alt); release(ctx->istats); release(ctx->hcs);
   for (int k = 0; k < 3; ++k) { release(ctx->ent_a[k]); release(ctx->ent_e[k]); }
-  release(ctx->ent_out); release(ctx->ent_psum); release(ctx->ent_pcnt); release(ctx->hres); release(ctx->hstate); release(ctx->islot); release(ctx->prev_keys); release(ctx->prev_wkeys); release(ctx->resid);
+  release(ctx->ent_out); release(ctx->ent_psum); release(ctx->ent_pcnt); release(ctx->hres); release(ctx->hstate); release(ctx->raw_ptrs); release(ctx->raw_out); release(ctx->islot); release(ctx->prev_keys); release(ctx->prev_wkeys); release(ctx->resid);
   release(ctx->hkeys); release(ctx->hkeys_new); release(ctx->hlast); release(ctx->hlast_new);
   release(ctx->hflag); release(ctx->hscan); release(ctx->pe); release(ctx->temp); release(ctx->red);
   if (ctx->h_counters) (void)hipHostFree(ctx->h_counters);
@@ -465,6 +465,115 @@ int rdk_get_contacts(rdk_ctx* ctx, int cap, int* a, int* b, double* pa, double* 
   RDK_TRY(dl3(ctx, shift, ctx->csh, n));
   RDK_HIP(hipStreamSynchronize(ctx->stream));
   return RDK_OK;
+}
+
+// FP64 contact list -> the plug-in's FP32 records
+__global__ void k_pack_raw(int n, const int* __restrict__ a, const int* __restrict__ b, const double* __restrict__ r,
+                           const double* const* __restrict__ pa, const double* const* __restrict__ pb,
+                           const double* const* __restrict__ nrm, const double* const* __restrict__ sh,
+                           const double* __restrict__ dist, const double* __restrict__ s,
+                           const double* __restrict__ t, rdk_contact_raw* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  rdk_contact_raw c;
+  c.a = a[i];
+  c.b = b[i];
+  c.px_a = (float)pa[0][i];
+  c.py_a = (float)pa[1][i];
+  c.pz_a = (float)pa[2][i];
+  c.px_b = (float)pb[0][i];
+  c.py_b = (float)pb[1][i];
+  c.pz_b = (float)pb[2][i];
+  c.nx = (float)nrm[0][i];
+  c.ny = (float)nrm[1][i];
+  c.nz = (float)nrm[2][i];
+  c.dist = (float)dist[i];
+  c.surface_limit = (float)(r[c.a] + r[c.b]);
+  c.s = (float)s[i];
+  c.t = (float)t[i];
+  c.shift_bx = (float)sh[0][i];
+  c.shift_by = (float)sh[1][i];
+  c.shift_bz = (float)sh[2][i];
+  out[i] = c;
+}
+
+int rdk_get_contacts_raw(rdk_ctx* ctx, int cap, rdk_contact_raw* out, int* count) {
+  CTX_CHECK();
+  if (cap < 0 || (cap > 0 && !out)) return RDK_ERR_ARG;
+  const int nc = ctx->nc;
+  if (count) *count = nc;
+  const int n = std::min(cap, nc);
+  if (n > 0) {
+    const double* ptrs[12];
+    for (int k = 0; k < 3; ++k) {
+      ptrs[k] = ctx->cpa[k].p;
+      ptrs[3 + k] = ctx->cpb[k].p;
+      ptrs[6 + k] = ctx->cn[k].p;
+      ptrs[9 + k] = ctx->csh[k].p;
+    }
+    RDK_TRY(ensure(ctx, ctx->raw_ptrs, 12));
+    RDK_TRY(ensure(ctx, ctx->raw_out, (size_t)n * sizeof(rdk_contact_raw)));
+    RDK_HIP(hipMemcpyAsync(ctx->raw_ptrs.p, ptrs, sizeof(ptrs), hipMemcpyHostToDevice, ctx->stream));
+    const double* const* P = (const double* const*)ctx->raw_ptrs.p;
+    rdk_contact_raw* dout = (rdk_contact_raw*)ctx->raw_out.p;
+    hipLaunchKernelGGL(k_pack_raw, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, n, ctx->ca.p, ctx->cb.p,
+                       ctx->r.p, P, P + 3, P + 6, P + 9, ctx->cdist.p, ctx->cs.p, ctx->ct.p, dout);
+    RDK_HIP(hipMemcpyAsync(out, dout, n * sizeof(rdk_contact_raw), hipMemcpyDeviceToHost, ctx->stream));
+    RDK_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  if (cap < nc) {
+    ctx->err = "rdk_get_contacts_raw: capacity below the contact count";
+    return RDK_ERR_CAPACITY;
+  }
+  return RDK_OK;
+}
+
+int rdk_detect_capsule_pairs_raw(rdk_ctx* ctx, int n, const float* x, const float* q, const float* r,
+                                 const float* h, float margin, int pbc, float sx, float sy, float sz, int cap,
+                                 rdk_contact_raw* out, int* count) {
+  CTX_CHECK();
+  if (n <= 0 || !x || !q || !r || !h || !count || cap < 0 || (cap > 0 && !out) || !(margin >= 0.0f) ||
+      (pbc && !(sx > 0.0f && sy > 0.0f && sz > 0.0f))) {
+    ctx->err = "rdk_detect_capsule_pairs_raw: invalid argument";
+    return RDK_ERR_ARG;
+  }
+  std::vector<double> X(3 * (size_t)n), Q(4 * (size_t)n), R(n), H(n), M(n, 1.0), E(n, 0.0), Z(3 * (size_t)n, 0.0);
+  double reach = 0.0;
+  for (int i = 0; i < n; ++i) {
+    for (int k = 0; k < 3; ++k) X[3 * i + k] = x[3 * i + k];
+    for (int k = 0; k < 4; ++k) Q[4 * i + k] = q[4 * i + k];
+    R[i] = r[i];
+    H[i] = h[i];
+    reach = std::max(reach, H[i] + R[i]);
+  }
+  RDK_TRY(rdk_set_bodies(ctx, n, X.data(), Q.data(), Z.data(), Z.data(), R.data(), H.data(), M.data(), E.data()));
+  rdk_system_desc d{};
+  if (pbc) {
+    // only the extents matter for the minimum image; cells wrap any position
+    const double L[3] = {sx, sy, sz};
+    for (int k = 0; k < 3; ++k) {
+      d.lo[k] = 0.0;
+      d.hi[k] = L[k];
+      d.periodic[k] = 1;
+    }
+  } else {
+    // a free box around the rods (positions beyond it fall in its edge cells)
+    for (int k = 0; k < 3; ++k) {
+      double lo = X[k], hi = X[k];
+      for (int i = 1; i < n; ++i) {
+        lo = std::min(lo, X[3 * i + k]);
+        hi = std::max(hi, X[3 * i + k]);
+      }
+      d.lo[k] = lo - 2.0 * reach - margin - 1e-6;
+      d.hi[k] = hi + 2.0 * reach + margin + 1e-6;
+    }
+  }
+  const int off[2] = {0, n};
+  RDK_TRY(rdk_set_systems(ctx, 1, off, &d));
+  rdk_detect_params dp{(double)margin, 0.0, 1.0};
+  int nc = 0;
+  RDK_TRY(rdk_detect(ctx, &dp, &nc));
+  return rdk_get_contacts_raw(ctx, cap, out, count);
 }
 
 int rdk_forces_soft(rdk_ctx* ctx, const rdk_soft_params* p, double* pe) {

@@ -171,6 +171,8 @@ struct rdk_ctx {
   rdk::DevBuf<unsigned long long> ent_pcnt;
   rdk::DevBuf<double> hres;   // residual and last-sweep change of the huge-island solve
   rdk::DevBuf<double> hstate;  // device-side state of the huge-island solve
+  rdk::DevBuf<const double*> raw_ptrs;  // contact component arrays for the FP32 record export
+  rdk::DevBuf<char> raw_out;            // FP32 contact records
   rdk::DevBuf<int2> islot;  // local body slots of large-island manifolds
   // position projection
   rdk::DevBuf<int> ppa, ppb, pflag, ppos;

@@ -111,6 +111,10 @@ _SIGS = {
     "rdk_num_kernels": (C.c_int, []),
     "rdk_work_counters": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_int]),
     "rdk_entanglement": (C.c_int, [C.c_void_p, C.c_double, _dp, C.POINTER(C.c_int64)]),
+    "rdk_get_contacts_raw": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_int)]),
+    "rdk_detect_capsule_pairs_raw": (C.c_int, [C.c_void_p, C.c_int] + [C.POINTER(C.c_float)] * 4 +
+                                     [C.c_float, C.c_int, C.c_float, C.c_float, C.c_float, C.c_int, C.c_void_p,
+                                      C.POINTER(C.c_int)]),
     "rdk_populate_nonoverlap": (C.c_int, [C.c_void_p, C.c_int, C.c_float, C.c_float, C.POINTER(C.c_float),
                                           C.POINTER(C.c_float), C.c_int, C.c_float, C.c_uint32, C.c_int, C.c_int,
                                           C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
@@ -229,6 +233,14 @@ def detect_params(delta=0.0, chunk_len=0.0, cell_scale=1.0):
     p = DetectParams()
     p.delta, p.chunk_len, p.cell_scale = delta, chunk_len, cell_scale
     return p
+
+
+# the reference plug-in's GpuContactRaw record (rdk_contact_raw)
+CONTACT_RAW = np.dtype([("a", np.int32), ("b", np.int32), ("px_a", np.float32), ("py_a", np.float32),
+                        ("pz_a", np.float32), ("px_b", np.float32), ("py_b", np.float32), ("pz_b", np.float32),
+                        ("nx", np.float32), ("ny", np.float32), ("nz", np.float32), ("dist", np.float32),
+                        ("surface_limit", np.float32), ("s", np.float32), ("t", np.float32),
+                        ("shift_bx", np.float32), ("shift_by", np.float32), ("shift_bz", np.float32)])
 
 
 class Context:
@@ -435,6 +447,35 @@ class Context:
             c.ctypes.data_as(fp), u.ctypes.data_as(fp), q.ctypes.data_as(fp), C.byref(placed), C.byref(att)))
         k = placed.value
         return c[:k], u[:k], q[:k], att.value
+
+    def contacts_raw(self):
+        """The current contact list as the plug-in's FP32 records."""
+        cnt = C.c_int()
+        self._check(self._L.rdk_get_contacts_raw(self._h, 0, None, C.byref(cnt)), allow=(RDK_ERR_CAPACITY,))
+        out = np.zeros(max(cnt.value, 1), CONTACT_RAW)
+        self._check(self._L.rdk_get_contacts_raw(self._h, cnt.value, out.ctypes.data_as(C.c_void_p),
+                                                 C.byref(cnt)))
+        return out[:cnt.value]
+
+    def detect_capsule_pairs_raw(self, x, q, r, h, margin, pbc=False, box=(0.0, 0.0, 0.0), cap=None):
+        """One call in place of the reference's cudaDetectCapsulePairsTwoPass:
+        FP32 capsules in, FP32 GpuContactRaw-layout records out."""
+        f = [np.ascontiguousarray(v, np.float32) for v in (x, q, r, h)]
+        n = len(f[2])
+        cnt = C.c_int()
+        fp = C.POINTER(C.c_float)
+        args = [self._h, n] + [v.ctypes.data_as(fp) for v in f] + [float(margin), int(bool(pbc))] + \
+            [float(v) for v in box]
+        if cap is None:
+            st = self._L.rdk_detect_capsule_pairs_raw(*args, 0, None, C.byref(cnt))
+            self._check(st, allow=(RDK_ERR_CAPACITY,))
+            cap = cnt.value
+        out = np.zeros(max(cap, 1), CONTACT_RAW)
+        self._check(self._L.rdk_detect_capsule_pairs_raw(*args, int(cap), out.ctypes.data_as(C.c_void_p),
+                                                         C.byref(cnt)))
+        self.n = n
+        self._nsys = 1
+        return out[:cnt.value]
 
     def sync(self):
         self._check(self._L.rdk_sync(self._h))

@@ -206,3 +206,50 @@ def test_reference_packings(gpu_ctx_factory):
             nc = ctx.detect(L.detect_params(delta, chunk_len, 1.0))
             assert nc == ref, (pid, chunk_len, nc, ref)
             compare(ctx.contacts(nc), oc, nc)
+
+
+@pytest.mark.parametrize("pbc", [False, True])
+def test_plugin_raw_records(gpu_ctx_factory, pbc):
+    """rdk_detect_capsule_pairs_raw, the one-call replacement of the reference's
+    cudaDetectCapsulePairsTwoPass: FP32 capsules in, GpuContactRaw-layout
+    records out.  Pairs equal the oracle's FP64 detection of the same
+    (FP32-valued) capsules; every float field is the oracle's FP64 value rounded
+    to FP32; surface_limit = r_a + r_b; a too-small capacity is reported."""
+    n, side = 2000, 25.0
+    x, q, r, h = rods(n, side, 12)
+    if pbc:
+        x = x - side / 2  # the reference keeps positions in [min, max); any origin works
+    x32, q32, r32, h32 = (np.asarray(v, np.float32) for v in (x, q, r, h))
+    print(f"raw plug-in records pbc={pbc}", flush=True)
+    ctx = gpu_ctx_factory()
+    margin = 0.05
+    raw = ctx.detect_capsule_pairs_raw(x32, q32, r32, h32, margin, pbc=pbc, box=(side, side, side))
+    W = O.World(x32.astype(np.float64), q32.astype(np.float64), None, None, r32.astype(np.float64),
+                h32.astype(np.float64), np.ones(n))
+    if pbc:
+        W.set_box(np.zeros(3), np.full(3, side), (1, 1, 1))
+    else:
+        W.set_box(x32.min(0) - 10.0, x32.max(0) + 10.0, (0, 0, 0))
+    no = W.detect(0, float(np.float32(margin)))
+    oc = W.contacts(no)
+    assert len(raw) == no > 0
+    assert np.array_equal(raw["a"], oc["a"]) and np.array_equal(raw["b"], oc["b"])
+    f32 = lambda v: np.asarray(v, np.float64).astype(np.float32)
+    for k, name in enumerate("xyz"):
+        assert np.array_equal(raw[f"p{name}_a"], f32(oc["pa"][:, k]))
+        assert np.array_equal(raw[f"p{name}_b"], f32(oc["pb"][:, k]))
+        assert np.array_equal(raw[f"n{name}"], f32(oc["n"][:, k]))
+        assert np.array_equal(raw[f"shift_b{name}"], f32(oc["shift"][:, k]))
+    assert np.array_equal(raw["dist"], f32(oc["dist"]))
+    assert np.array_equal(raw["s"], f32(oc["s"])) and np.array_equal(raw["t"], f32(oc["t"]))
+    rr = r32.astype(np.float64)
+    assert np.array_equal(raw["surface_limit"], f32(rr[raw["a"]] + rr[raw["b"]]))
+    assert np.all(raw["dist"] < raw["surface_limit"] + np.float32(margin) + 1e-6)
+    if pbc:
+        assert np.any(raw["shift_bx"] != 0) or np.any(raw["shift_by"] != 0)
+    from rod_dynamics_py import RdkError
+
+    with pytest.raises(RdkError):
+        ctx.detect_capsule_pairs_raw(x32, q32, r32, h32, margin, pbc=pbc, box=(side, side, side), cap=no - 1)
+    again = ctx.contacts_raw()
+    assert np.array_equal(again, raw)
