@@ -532,7 +532,11 @@ int rdk_detect_capsule_pairs_raw(rdk_ctx* ctx, int n, const float* x, const floa
                                  const float* h, float margin, int pbc, float sx, float sy, float sz, int cap,
                                  rdk_contact_raw* out, int* count) {
   CTX_CHECK();
-  if (n <= 0 || !x || !q || !r || !h || !count || cap < 0 || (cap > 0 && !out) || !(margin >= 0.0f) ||
+  if (n == 0 && count && cap >= 0) {  // no capsules: no contacts
+    *count = 0;
+    return RDK_OK;
+  }
+  if (n < 0 || !x || !q || !r || !h || !count || cap < 0 || (cap > 0 && !out) || !(margin >= 0.0f) ||
       (pbc && !(sx > 0.0f && sy > 0.0f && sz > 0.0f))) {
     ctx->err = "rdk_detect_capsule_pairs_raw: invalid argument";
     return RDK_ERR_ARG;

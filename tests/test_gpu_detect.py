@@ -253,3 +253,19 @@ def test_plugin_raw_records(gpu_ctx_factory, pbc):
         ctx.detect_capsule_pairs_raw(x32, q32, r32, h32, margin, pbc=pbc, box=(side, side, side), cap=no - 1)
     again = ctx.contacts_raw()
     assert np.array_equal(again, raw)
+
+
+def test_plugin_raw_records_edge_cases(gpu_ctx_factory):
+    """No capsules, and capsules far apart: no records, no error."""
+    ctx = gpu_ctx_factory()
+    z = np.zeros((0, 3), np.float32)
+    assert len(ctx.detect_capsule_pairs_raw(z, np.zeros((0, 4)), np.zeros(0), np.zeros(0), 0.1)) == 0
+    x = np.array([[0, 0, 0], [50, 0, 0]], np.float32)
+    q = np.array([[1, 0, 0, 0], [1, 0, 0, 0]], np.float32)
+    out = ctx.detect_capsule_pairs_raw(x, q, [0.5, 0.5], [1.0, 1.0], 0.1)
+    assert len(out) == 0
+    x[1, 0] = 0.9  # parallel rods 0.9 apart, radii 0.5: one contact, normal along +x
+    out = ctx.detect_capsule_pairs_raw(x, q, [0.5, 0.5], [1.0, 1.0], 0.0)
+    assert len(out) == 1 and out["a"][0] == 0 and out["b"][0] == 1
+    assert out["dist"][0] == np.float32(0.9) and out["nx"][0] == np.float32(1.0)
+    assert out["surface_limit"][0] == np.float32(1.0)
