@@ -241,6 +241,15 @@ int rdk_integrate(rdk_ctx* ctx, int stage);
 /* ---- fused steps (App::physicsStep, main.cpp:4333-4858) ---------------- */
 int rdk_step_nsc(rdk_ctx* ctx, const rdk_detect_params* dp,
                  const rdk_nsc_params* np, int nsteps, rdk_step_stats* st);
+/* Keep the velocity-solve manifolds of each fused hard-contact step (the
+ * position projection reuses the manifold buffers): what the reference's
+ * network export reads from NscContactSolver::getManifolds after a step
+ * (main.cpp:3699-3720).  rdk_nsc_get_kept_manifolds returns those of the
+ * last step in the layout of rdk_nsc_get_manifolds; *count = their number,
+ * RDK_ERR_CAPACITY when cap is smaller. */
+int rdk_keep_manifolds(rdk_ctx* ctx, int on);
+int rdk_nsc_get_kept_manifolds(rdk_ctx* ctx, int cap, double* rec, int* ab,
+                               double* lambda, int* count);
 /* model 0 soft (params = rdk_soft_params*), 1 Hertz-Mindlin (rdk_hm_params*) */
 int rdk_step_vv(rdk_ctx* ctx, const rdk_detect_params* dp, int model,
                 const void* params, int nsteps, rdk_step_stats* st);

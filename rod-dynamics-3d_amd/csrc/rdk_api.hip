@@ -186,7 +186,7 @@ int rdk_destroy(rdk_ctx* ctx) {
   release(ctx->iflag); release(ctx->ipos); release(ctx->istart); release(ctx->ilists); release(ctx->bseg);
   release(ctx->icnt); release(ctx->blab); release(ctx->blab_alt); release(ctx->istats); release(ctx->hcs);
   for (int k = 0; k < 3; ++k) { release(ctx->ent_a[k]); release(ctx->ent_e[k]); }
-  release(ctx->ent_out); release(ctx->ent_psum); release(ctx->ent_pcnt); release(ctx->hres); release(ctx->hstate); release(ctx->raw_ptrs); release(ctx->raw_out); release(ctx->islot); release(ctx->prev_keys); release(ctx->prev_wkeys); release(ctx->resid);
+  release(ctx->ent_out); release(ctx->ent_psum); release(ctx->ent_pcnt); release(ctx->hres); release(ctx->hstate); release(ctx->raw_ptrs); release(ctx->raw_out); release(ctx->kept_mf); release(ctx->kept_ab); release(ctx->kept_lam); release(ctx->islot); release(ctx->prev_keys); release(ctx->prev_wkeys); release(ctx->resid);
   release(ctx->hkeys); release(ctx->hkeys_new); release(ctx->hlast); release(ctx->hlast_new);
   release(ctx->hflag); release(ctx->hscan); release(ctx->pe); release(ctx->temp); release(ctx->red);
   if (ctx->h_counters) (void)hipHostFree(ctx->h_counters);
@@ -741,6 +741,75 @@ int rdk_integrate(rdk_ctx* ctx, int stage) {
   return prof_flush(ctx);
 }
 
+// copy of the natural-order manifolds, their bodies and impulses after the
+// velocity solve of a fused step
+static int keep_manifolds(rdk_ctx* ctx) {
+  const int m = ctx->nm;
+  ctx->kept_m = m;
+  ctx->kept_stride = ctx->mf.cap / RDK_MF_FIELDS;
+  if (m <= 0) return RDK_OK;
+  RDK_TRY(ensure(ctx, ctx->kept_mf, ctx->mf.cap));
+  RDK_TRY(ensure(ctx, ctx->kept_ab, 2 * (size_t)m));
+  RDK_TRY(ensure(ctx, ctx->kept_lam, 3 * (size_t)m));
+  RDK_HIP(hipMemcpyAsync(ctx->kept_mf.p, ctx->mf.p, ctx->mf.cap * sizeof(double), hipMemcpyDeviceToDevice,
+                         ctx->stream));
+  RDK_HIP(hipMemcpyAsync(ctx->kept_ab.p, ctx->ma.p, m * sizeof(int), hipMemcpyDeviceToDevice, ctx->stream));
+  RDK_HIP(hipMemcpyAsync(ctx->kept_ab.p + m, ctx->mb.p, m * sizeof(int), hipMemcpyDeviceToDevice, ctx->stream));
+  for (int k = 0; k < 3; ++k)
+    RDK_HIP(hipMemcpyAsync(ctx->kept_lam.p + (size_t)k * m, ctx->lam[k].p, m * sizeof(double),
+                           hipMemcpyDeviceToDevice, ctx->stream));
+  return RDK_OK;
+}
+
+int rdk_keep_manifolds(rdk_ctx* ctx, int on) {
+  CTX_CHECK();
+  ctx->keep_manifolds = on != 0;
+  ctx->kept_m = 0;
+  return RDK_OK;
+}
+
+int rdk_nsc_get_kept_manifolds(rdk_ctx* ctx, int cap, double* rec, int* ab, double* lambda, int* count) {
+  CTX_CHECK();
+  if (cap < 0 || !count) return RDK_ERR_ARG;
+  const int m = ctx->kept_m;
+  *count = m;
+  const int n = std::min(cap, m);
+  if (n > 0) {
+    std::vector<double> tmp(n);
+    if (rec) {
+      for (int f = 0; f < RDK_MF_FIELDS; ++f) {
+        RDK_HIP(hipMemcpyAsync(tmp.data(), ctx->kept_mf.p + (size_t)f * ctx->kept_stride, n * sizeof(double),
+                               hipMemcpyDeviceToHost, ctx->stream));
+        RDK_HIP(hipStreamSynchronize(ctx->stream));
+        for (int i = 0; i < n; ++i) rec[(size_t)RDK_MF_FIELDS * i + f] = tmp[i];
+      }
+    }
+    if (ab) {
+      std::vector<int> a(n), b(n);
+      RDK_HIP(hipMemcpyAsync(a.data(), ctx->kept_ab.p, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+      RDK_HIP(hipMemcpyAsync(b.data(), ctx->kept_ab.p + m, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+      RDK_HIP(hipStreamSynchronize(ctx->stream));
+      for (int i = 0; i < n; ++i) {
+        ab[2 * i] = a[i];
+        ab[2 * i + 1] = b[i];
+      }
+    }
+    if (lambda) {
+      for (int k = 0; k < 3; ++k) {
+        RDK_HIP(hipMemcpyAsync(tmp.data(), ctx->kept_lam.p + (size_t)k * m, n * sizeof(double),
+                               hipMemcpyDeviceToHost, ctx->stream));
+        RDK_HIP(hipStreamSynchronize(ctx->stream));
+        for (int i = 0; i < n; ++i) lambda[3 * i + k] = tmp[i];
+      }
+    }
+  }
+  if (cap < m) {
+    ctx->err = "rdk_nsc_get_kept_manifolds: capacity below the manifold count";
+    return RDK_ERR_CAPACITY;
+  }
+  return RDK_OK;
+}
+
 int rdk_step_nsc(rdk_ctx* ctx, const rdk_detect_params* dp, const rdk_nsc_params* np, int nsteps,
                  rdk_step_stats* st) {
   CTX_CHECK();
@@ -760,6 +829,7 @@ int rdk_step_nsc(rdk_ctx* ctx, const rdk_detect_params* dp, const rdk_nsc_params
     int rc = nsc_solve_run(ctx, np, &ss);
     if (rc == RDK_ERR_NOCONV) S.nonconverged_steps++;
     else if (rc != RDK_OK) return rc;
+    if (ctx->keep_manifolds) RDK_TRY(keep_manifolds(ctx));
     RDK_TRY(integrate_run(ctx, RDK_NSC_POS));
     if (np->position_stabilization && np->position_iters > 0)
       RDK_TRY(nsc_project_run(ctx, &d, np, nullptr, nullptr));  // main.cpp:4497-4503

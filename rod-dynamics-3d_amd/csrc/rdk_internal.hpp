@@ -171,6 +171,13 @@ struct rdk_ctx {
   rdk::DevBuf<unsigned long long> ent_pcnt;
   rdk::DevBuf<double> hres;   // residual and last-sweep change of the huge-island solve
   rdk::DevBuf<double> hstate;  // device-side state of the huge-island solve
+  // velocity-solve manifolds of the last fused hard-contact step, kept when
+  // keep_manifolds is set (the position projection reuses the manifold buffers)
+  bool keep_manifolds = false;
+  rdk::DevBuf<double> kept_mf, kept_lam;
+  rdk::DevBuf<int> kept_ab;
+  int kept_m = 0;
+  size_t kept_stride = 0;
   rdk::DevBuf<const double*> raw_ptrs;  // contact component arrays for the FP32 record export
   rdk::DevBuf<char> raw_out;            // FP32 contact records
   rdk::DevBuf<int2> islot;  // local body slots of large-island manifolds

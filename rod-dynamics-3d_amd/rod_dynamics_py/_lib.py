@@ -112,6 +112,9 @@ _SIGS = {
     "rdk_work_counters": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_int]),
     "rdk_entanglement": (C.c_int, [C.c_void_p, C.c_double, _dp, C.POINTER(C.c_int64)]),
     "rdk_get_contacts_raw": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_int)]),
+    "rdk_keep_manifolds": (C.c_int, [C.c_void_p, C.c_int]),
+    "rdk_nsc_get_kept_manifolds": (C.c_int, [C.c_void_p, C.c_int, _dp, C.POINTER(C.c_int), _dp,
+                                             C.POINTER(C.c_int)]),
     "rdk_detect_capsule_pairs_raw": (C.c_int, [C.c_void_p, C.c_int] + [C.POINTER(C.c_float)] * 4 +
                                      [C.c_float, C.c_int, C.c_float, C.c_float, C.c_float, C.c_int, C.c_void_p,
                                       C.POINTER(C.c_int)]),
@@ -354,6 +357,22 @@ class Context:
         lam = np.zeros((nm, 3))
         self._check(self._L.rdk_nsc_get_manifolds(self._h, nm, dptr(rec), iptr(ab), dptr(lam)))
         return rec, ab, lam
+
+    def keep_manifolds(self, on=True):
+        self._check(self._L.rdk_keep_manifolds(self._h, int(bool(on))))
+
+    def kept_manifolds(self):
+        """Velocity-solve manifolds of the last fused hard-contact step:
+        (records, body pairs, impulses) as nsc_manifolds returns them."""
+        cnt = C.c_int()
+        self._check(self._L.rdk_nsc_get_kept_manifolds(self._h, 0, None, None, None, C.byref(cnt)),
+                    allow=(RDK_ERR_CAPACITY,))
+        m = cnt.value
+        rec = np.zeros((max(m, 1), MF_FIELDS))
+        ab = np.zeros((max(m, 1), 2), np.int32)
+        lam = np.zeros((max(m, 1), 3))
+        self._check(self._L.rdk_nsc_get_kept_manifolds(self._h, m, dptr(rec), iptr(ab), dptr(lam), C.byref(cnt)))
+        return rec[:m], ab[:m], lam[:m]
 
     def nsc_set_lambda(self, lam):
         lam = np.ascontiguousarray(lam, dtype=np.float64)

@@ -14,7 +14,9 @@
 * ``snapshot_line`` formats one NDJSON snapshot line like App::writeSnapshotLine
   (main.cpp:640-692, capsule branch), ``endpoints_text`` an endpoint file like
   App::saveInitialConfigCSV (main.cpp:2905-2933) and ``per_rod_text`` the rows
-  of the per-rod log like App::logPerRodFrame (main.cpp:3412-3449).  The
+  of the per-rod log like App::logPerRodFrame (main.cpp:3412-3449), and
+  ``network_text`` the hard-contact network log like App::logNetworkFrame
+  (main.cpp:3624-3730, its hard-contact branch).  The
   reference stores state in FP32 (glm::vec3 / glm::quat) and prints it with
   iostream defaults (``%g``, 6 significant digits) or ``std::fixed`` with 6
   decimals; the library's FP64 state is rounded to FP32 first and printed the
@@ -162,6 +164,24 @@ def endpoints_text(x, q, r, h, pbc, box_size=None):
     p1 = (x + d).astype(np.float32)
     for a, b in zip(p0, p1):
         out.append(",".join(cxx_fixed6(t) for t in (*a, *b)) + "\n")
+    return "".join(out)
+
+
+NETWORK_HEADER = ("frame,rod_i,rod_j,contact_x,contact_y,contact_z,normal_x,normal_y,normal_z,phi,lambda_n,"
+                  "lambda_t1,lambda_t2,vn_pre,vn_post\n")
+
+
+def network_text(frame, a, b, point, normal, phi, lam, vn_pre, vn_post):
+    """Rows of the hard-contact network log for one frame (App::logNetworkFrame,
+    main.cpp:3699-3720): rod-rod manifolds only, contact point x_a + r_a,
+    normal, gap, the three impulses and the normal relative velocity before and
+    after the solve."""
+    point, normal, lam = _f32(point), _f32(normal), _f32(lam)
+    phi, vn_pre, vn_post = _f32(phi), _f32(vn_pre), _f32(vn_post)
+    out = []
+    for i in range(len(a)):
+        vals = [*point[i], *normal[i], phi[i], *lam[i], vn_pre[i], vn_post[i]]
+        out.append("%d,%d,%d," % (frame, a[i], b[i]) + ",".join(cxx_g(t) for t in vals) + "\n")
     return "".join(out)
 
 

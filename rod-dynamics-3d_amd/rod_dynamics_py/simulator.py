@@ -19,9 +19,9 @@ physicsStep per step (main.cpp:4333-4858) run by the library's fused steps:
 Beyond the pybind surface, the reference app's file I/O is kept:
 ``init_state_path`` loads the last frame of an init-state CSV (the CLI's
 ``--init-state``, App::loadInitialStateCSV, main.cpp:2935-3055), and
-``write_snapshot``, ``save_endpoints`` and ``write_per_rod`` write NDJSON
-snapshots, endpoint files and the per-rod log in the reference's formats
-(``sceneio``).
+``write_snapshot``, ``save_endpoints``, ``write_per_rod`` and
+``write_network`` write NDJSON snapshots, endpoint files, the per-rod log and
+the hard-contact network log in the reference's formats (``sceneio``).
 
 The reference is y-up (default gravity (0, -10, 0), floors are box top faces
 in y); the library's plane walls are floors in z.  Scenes are therefore run in
@@ -272,6 +272,8 @@ class Simulator:
         self.last_hit_count = 0
         self.last_island_count = 0
         self._load()
+        if self.model == "nsc":
+            self.ctx.keep_manifolds(True)  # for write_network (one copy of the manifolds per step)
         self.last_ke = self.ctx.observables()[0]
         if not quiet:
             print(f"[simulator] {len(self.r)} rods, model {self.model}, dt {self.dt}", file=sys.stderr)
@@ -551,6 +553,29 @@ class Simulator:
         if header:
             stream.write(sceneio.per_rod_header(self.r[0], self.h[0]))
         stream.write(sceneio.per_rod_text(self.frame_index, x, v, w, q, self.mass, self.r, self.h))
+
+    def write_network(self, stream, header=False):
+        """Append the hard-contact network rows of the last step
+        (App::logNetworkFrame): one row per rod-rod manifold of the step's
+        velocity solve with its impulses, the normal velocity before the solve
+        and the normal velocity from the current body velocities."""
+        if self.model != "nsc":
+            raise NotImplementedError("the network log of the soft-contact models needs per-contact forces, "
+                                      "which this library does not export")
+        rec, ab, lam = self.ctx.kept_manifolds()
+        keep = ab[:, 1] >= 0
+        rec, ab, lam = rec[keep], ab[keep], lam[keep]
+        s = self.ctx.download_state()  # library frame
+        a, b = ab[:, 0], ab[:, 1]
+        n_l, ra_l, rb_l = rec[:, 0:3], rec[:, 9:12], rec[:, 12:15]
+        va = s["v"][a] + np.cross(s["w"][a], ra_l)
+        vb = s["v"][b] + np.cross(s["w"][b], rb_l)
+        vn_post = np.sum(n_l * (vb - va), axis=1)
+        point = (s["x"][a] + ra_l) @ _R  # back to the reference frame
+        if header:
+            stream.write(sceneio.NETWORK_HEADER)
+        stream.write(sceneio.network_text(self.frame_index, a, b, point, n_l @ _R, rec[:, 38], lam, rec[:, 39],
+                                          vn_post))
 
     def close(self):
         if getattr(self, "ctx", None) is not None:

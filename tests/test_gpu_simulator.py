@@ -112,3 +112,42 @@ def test_state_files_round_trip_through_the_simulator(tmp_path):
     assert sim2.diagnostics()["frame_index"] == 7
     sim.close()
     sim2.close()
+
+
+def test_network_log_of_a_hard_contact_step(tmp_path):
+    """Two crossed rods meet head-on with restitution 0 and position
+    stabilisation on (the projection reuses the manifold buffers): the network
+    log holds the step's rod-rod manifold with a positive normal impulse, the
+    approach velocity before the solve and zero normal velocity after it, and
+    matches the kept velocity-solve manifolds."""
+    import io
+
+    from rod_dynamics_py import sceneio
+
+    scene = {
+        "physics": {"dt": 1e-3, "gravity": [0, 0, 0], "lin_damp": 0.0, "ang_damp": 0.0,
+                    "nsc": {"enabled": True, "mu": 0.0, "beta": 0.0, "cfm": 0.0, "omega": 1.0, "velocity_iters": 200,
+                            "position_stabilization": True}},
+        "scene": {"bodies": [
+            {"length": 1.0, "diameter": 0.5, "density": 1000.0, "restitution": 0.0, "pos": [-0.249, 0.0, 0.0],
+             "v_lin": [1.0, 0.0, 0.0]},
+            {"length": 1.0, "diameter": 0.5, "density": 1000.0, "restitution": 0.0, "pos": [0.249, 0.0, 0.0],
+             "rot_quat": [0.7071067811865476, 0.7071067811865476, 0.0, 0.0], "v_lin": [-1.0, 0.0, 0.0]}]}}
+    (tmp_path / "s.json").write_text(json.dumps(scene))
+    sim = Simulator(str(tmp_path / "s.json"))
+    sim.step()
+    buf = io.StringIO()
+    sim.write_network(buf, header=True)
+    lines = buf.getvalue().splitlines()
+    assert lines[0] + "\n" == sceneio.NETWORK_HEADER
+    assert len(lines) == 2
+    row = lines[1].split(",")
+    assert row[:3] == ["1", "0", "1"]
+    vals = [float(t) for t in row[3:]]
+    lam_n, vn_pre, vn_post = vals[7], vals[10], vals[11]
+    assert lam_n > 0 and vn_pre == pytest.approx(-2.0, rel=1e-6) and abs(vn_post) < 1e-9
+    assert abs(vals[3]) == pytest.approx(1.0, rel=1e-6)  # normal along x in the reference frame
+    rec, ab, lam = sim.ctx.kept_manifolds()
+    rr = ab[:, 1] >= 0
+    assert lam[rr][0, 0] == pytest.approx(lam_n, rel=1e-5)  # printed with 6 significant digits
+    sim.close()
