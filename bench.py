@@ -317,7 +317,16 @@ def learned_ab(ctx, sc, a):
         if (k + 1) % 5 == 0:
             log(f"learned-start A/B step {k + 1}/{a.ab_steps}")
     res = WS.ab_summary(rows)
+    meta = os.path.splitext(a.weights)[0] + ".json"
+    select = json.load(open(meta)).get("select", "uncached") if os.path.exists(meta) else "uncached"
+    if select != "all":
+        # the start for every manifold needs weights trained on every manifold
+        for key in [k for k in res if "learned_all" in k]:
+            del res[key]
+        res["learned_all"] = {"status": "not reported: these weights were trained on manifolds without a "
+                                        "previous-step impulse only"}
     res["weights"] = os.path.relpath(a.weights, ROOT)
+    res["weights_trained_on"] = select
     res["bed"] = f"steps {a.warmup + a.steps + 1}..{a.warmup + a.steps + a.ab_steps} of the same run"
     return res
 
