@@ -5,6 +5,7 @@ statistics).  No GPU call is made."""
 import math
 import os
 import re
+import shutil
 
 import numpy as np
 import pytest
@@ -105,3 +106,27 @@ def test_no_gpu_means_loud_failure_not_fallback():
     assert st in (L.RDK_ERR_HIP, L.RDK_ERR_ARG)
     with pytest.raises(L.RdkError):
         L.Context(0)
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="gcc not available")
+def test_header_is_c99_and_plugin_record_matches_the_binding(tmp_path):
+    """include/rdk.h compiles as C99 (no C++ in the ABI), and the plug-in record
+    rdk_contact_raw has the size and field offsets of the Python binding's
+    dtype (the layout of the reference's GpuContactRaw)."""
+    import subprocess
+
+    from rod_dynamics_py._lib import CONTACT_RAW
+
+    fields = list(CONTACT_RAW.names)
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "rdk.h"\nint main(void) {\n'
+                   '  printf("%zu\\n", sizeof(rdk_contact_raw));\n'
+                   + "".join(f'  printf("%zu\\n", offsetof(rdk_contact_raw, {f}));\n' for f in fields)
+                   + "  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    inc = os.path.join(ROOT, "include")
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-pedantic", "-I", inc, str(src), "-o", str(exe)],
+                   check=True)
+    out = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert out[0] == CONTACT_RAW.itemsize == 72
+    assert out[1:] == [CONTACT_RAW.fields[f][1] for f in fields]
