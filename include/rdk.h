@@ -250,6 +250,16 @@ int rdk_step_nsc(rdk_ctx* ctx, const rdk_detect_params* dp,
 int rdk_keep_manifolds(rdk_ctx* ctx, int on);
 int rdk_nsc_get_kept_manifolds(rdk_ctx* ctx, int cap, double* rec, int* ab,
                                double* lambda, int* count);
+/* Keep the per-contact forces of each soft force evaluation: what the
+ * reference's network export reads from SoftContactSolver::getContacts
+ * (main.cpp:3674-3688; force_a/b and friction_a/b of soft_contact.cpp:115-147).
+ * rdk_get_soft_forces returns those of the last evaluation, 6 doubles per
+ * contact in the order of rdk_get_contacts_raw: the normal force on b
+ * (spring + damping; force_a is its negative) and the friction force on a
+ * (friction_b is its negative).  A Hertz-Mindlin evaluation keeps none.
+ * *count = their number, RDK_ERR_CAPACITY when cap is smaller. */
+int rdk_keep_soft_forces(rdk_ctx* ctx, int on);
+int rdk_get_soft_forces(rdk_ctx* ctx, int cap, double* out, int* count);
 /* model 0 soft (params = rdk_soft_params*), 1 Hertz-Mindlin (rdk_hm_params*) */
 int rdk_step_vv(rdk_ctx* ctx, const rdk_detect_params* dp, int model,
                 const void* params, int nsteps, rdk_step_stats* st);

@@ -186,7 +186,7 @@ int rdk_destroy(rdk_ctx* ctx) {
   release(ctx->iflag); release(ctx->ipos); release(ctx->istart); release(ctx->ilists); release(ctx->bseg);
   release(ctx->icnt); release(ctx->blab); release(ctx->blab_alt); release(ctx->istats); release(ctx->hcs);
   for (int k = 0; k < 3; ++k) { release(ctx->ent_a[k]); release(ctx->ent_e[k]); }
-  release(ctx->ent_out); release(ctx->ent_psum); release(ctx->ent_pcnt); release(ctx->hres); release(ctx->hstate); release(ctx->raw_ptrs); release(ctx->raw_out); release(ctx->kept_mf); release(ctx->kept_ab); release(ctx->kept_lam); release(ctx->islot); release(ctx->prev_keys); release(ctx->prev_wkeys); release(ctx->resid);
+  release(ctx->ent_out); release(ctx->ent_psum); release(ctx->ent_pcnt); release(ctx->hres); release(ctx->hstate); release(ctx->raw_ptrs); release(ctx->raw_out); release(ctx->kept_mf); release(ctx->kept_ab); release(ctx->kept_lam); release(ctx->soft_kept); release(ctx->islot); release(ctx->prev_keys); release(ctx->prev_wkeys); release(ctx->resid);
   release(ctx->hkeys); release(ctx->hkeys_new); release(ctx->hlast); release(ctx->hlast_new);
   release(ctx->hflag); release(ctx->hscan); release(ctx->pe); release(ctx->temp); release(ctx->red);
   if (ctx->h_counters) (void)hipHostFree(ctx->h_counters);
@@ -805,6 +805,31 @@ int rdk_nsc_get_kept_manifolds(rdk_ctx* ctx, int cap, double* rec, int* ab, doub
   }
   if (cap < m) {
     ctx->err = "rdk_nsc_get_kept_manifolds: capacity below the manifold count";
+    return RDK_ERR_CAPACITY;
+  }
+  return RDK_OK;
+}
+
+int rdk_keep_soft_forces(rdk_ctx* ctx, int on) {
+  CTX_CHECK();
+  ctx->keep_soft = on != 0;
+  ctx->soft_kept_n = 0;
+  return RDK_OK;
+}
+
+int rdk_get_soft_forces(rdk_ctx* ctx, int cap, double* out, int* count) {
+  CTX_CHECK();
+  if (cap < 0 || !count || (cap > 0 && !out)) return RDK_ERR_ARG;
+  const int m = ctx->soft_kept_n;
+  *count = m;
+  const int n = std::min(cap, m);
+  if (n > 0) {
+    RDK_HIP(hipMemcpyAsync(out, ctx->soft_kept.p, 6 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    RDK_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  if (cap < m) {
+    ctx->err = "rdk_get_soft_forces: capacity below the contact count";
     return RDK_ERR_CAPACITY;
   }
   return RDK_OK;

@@ -43,6 +43,7 @@ struct CIn {
 
 struct COut {
   double* c[12];  // F on B (0..2), tau A (3..5), tau B (6..8), history vector (9..11)
+  double* keep;   // null, or 6 doubles per contact: normal force on B, friction force on A
 };
 
 struct Hist {
@@ -177,6 +178,11 @@ __global__ void k_soft_force(BodyPtrs B, CIn C, int nc, SoftK P, Hist H, COut O,
     }
     const d3 Ta = fa + fra;
     const d3 Tb = fb + (-fra);
+    if (O.keep) {
+      double* k6 = O.keep + 6 * (size_t)i;
+      k6[0] = fb.x; k6[1] = fb.y; k6[2] = fb.z;
+      k6[3] = fra.x; k6[4] = fra.y; k6[5] = fra.z;
+    }
     put(O, 0, i, Tb);
     put(O, 3, i, cross(ra, Ta));
     put(O, 6, i, cross(rb, Tb));
@@ -432,6 +438,15 @@ int forces_run(rdk_ctx* ctx, int model, const void* params, double* pe) {
   for (int k = 0; k < 12; ++k) RDK_TRY(ensure(ctx, ctx->cf[k], std::max(nc, 1)));
   COut O;
   for (int k = 0; k < 12; ++k) O.c[k] = ctx->cf[k].p;
+  O.keep = nullptr;
+  if (ctx->keep_soft) {
+    ctx->soft_kept_n = 0;  // a Hertz-Mindlin evaluation keeps nothing
+    if (model == 0) {
+      RDK_TRY(ensure(ctx, ctx->soft_kept, 6 * (size_t)std::max(nc, 1)));
+      O.keep = ctx->soft_kept.p;
+      ctx->soft_kept_n = nc;
+    }
+  }
   CIn C;
   C.a = ctx->ca.p;
   C.b = ctx->cb.p;

@@ -178,6 +178,12 @@ struct rdk_ctx {
   rdk::DevBuf<int> kept_ab;
   int kept_m = 0;
   size_t kept_stride = 0;
+  // per-contact terms of the last soft force evaluation when keep_soft is set:
+  // 6 doubles per contact in contact-list order, the normal force on b
+  // (spring + damping) then the friction force on a
+  bool keep_soft = false;
+  rdk::DevBuf<double> soft_kept;
+  int soft_kept_n = 0;
   rdk::DevBuf<const double*> raw_ptrs;  // contact component arrays for the FP32 record export
   rdk::DevBuf<char> raw_out;            // FP32 contact records
   rdk::DevBuf<int2> islot;  // local body slots of large-island manifolds

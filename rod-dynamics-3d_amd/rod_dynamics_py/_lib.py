@@ -113,6 +113,8 @@ _SIGS = {
     "rdk_entanglement": (C.c_int, [C.c_void_p, C.c_double, _dp, C.POINTER(C.c_int64)]),
     "rdk_get_contacts_raw": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_int)]),
     "rdk_keep_manifolds": (C.c_int, [C.c_void_p, C.c_int]),
+    "rdk_keep_soft_forces": (C.c_int, [C.c_void_p, C.c_int]),
+    "rdk_get_soft_forces": (C.c_int, [C.c_void_p, C.c_int, _dp, C.POINTER(C.c_int)]),
     "rdk_nsc_get_kept_manifolds": (C.c_int, [C.c_void_p, C.c_int, _dp, C.POINTER(C.c_int), _dp,
                                              C.POINTER(C.c_int)]),
     "rdk_detect_capsule_pairs_raw": (C.c_int, [C.c_void_p, C.c_int] + [C.POINTER(C.c_float)] * 4 +
@@ -360,6 +362,19 @@ class Context:
 
     def keep_manifolds(self, on=True):
         self._check(self._L.rdk_keep_manifolds(self._h, int(bool(on))))
+
+    def keep_soft_forces(self, on=True):
+        self._check(self._L.rdk_keep_soft_forces(self._h, int(bool(on))))
+
+    def soft_forces(self):
+        """Per-contact terms of the last soft force evaluation, (n, 6) in
+        contact-list order: normal force on b, friction force on a."""
+        cnt = C.c_int()
+        self._check(self._L.rdk_get_soft_forces(self._h, 0, None, C.byref(cnt)), allow=(RDK_ERR_CAPACITY,))
+        m = cnt.value
+        out = np.zeros((max(m, 1), 6))
+        self._check(self._L.rdk_get_soft_forces(self._h, m, dptr(out), C.byref(cnt)))
+        return out[:m]
 
     def kept_manifolds(self):
         """Velocity-solve manifolds of the last fused hard-contact step:

@@ -16,7 +16,8 @@
   App::saveInitialConfigCSV (main.cpp:2905-2933) and ``per_rod_text`` the rows
   of the per-rod log like App::logPerRodFrame (main.cpp:3412-3449), and
   ``network_text`` the hard-contact network log like App::logNetworkFrame
-  (main.cpp:3624-3730, its hard-contact branch).  The
+  (main.cpp:3624-3730, its hard-contact branch; ``network_soft_text`` its
+  soft-contact branch).  The
   reference stores state in FP32 (glm::vec3 / glm::quat) and prints it with
   iostream defaults (``%g``, 6 significant digits) or ``std::fixed`` with 6
   decimals; the library's FP64 state is rounded to FP32 first and printed the
@@ -181,6 +182,27 @@ def network_text(frame, a, b, point, normal, phi, lam, vn_pre, vn_post):
     out = []
     for i in range(len(a)):
         vals = [*point[i], *normal[i], phi[i], *lam[i], vn_pre[i], vn_post[i]]
+        out.append("%d,%d,%d," % (frame, a[i], b[i]) + ",".join(cxx_g(t) for t in vals) + "\n")
+    return "".join(out)
+
+
+NETWORK_SOFT_HEADER = ("frame,rod_i,rod_j,contact_x,contact_y,contact_z,normal_x,normal_y,normal_z,distance,"
+                       "force_a_x,force_a_y,force_a_z,force_b_x,force_b_y,force_b_z,"
+                       "friction_a_x,friction_a_y,friction_a_z,friction_b_x,friction_b_y,friction_b_z\n")
+
+
+def network_soft_text(frame, a, b, point_a, point_b, normal, dist, force_b, friction_a):
+    """Rows of the soft-contact network log for one frame (App::logNetworkFrame,
+    main.cpp:3674-3688): the FP32 midpoint 0.5 (p_a + p_b), normal, distance,
+    and the contact's forces; force_a = -force_b and friction_b = -friction_a
+    (soft_contact.cpp:115-147 and its friction laws)."""
+    pa, pb = np.asarray(point_a, np.float32), np.asarray(point_b, np.float32)
+    mid = (np.float32(0.5) * (pa + pb)).astype(np.float64)
+    normal, dist = _f32(normal), _f32(dist)
+    fb, fra = _f32(force_b), _f32(friction_a)
+    out = []
+    for i in range(len(a)):
+        vals = [*mid[i], *normal[i], dist[i], *(-fb[i]), *fb[i], *fra[i], *(-fra[i])]
         out.append("%d,%d,%d," % (frame, a[i], b[i]) + ",".join(cxx_g(t) for t in vals) + "\n")
     return "".join(out)
 

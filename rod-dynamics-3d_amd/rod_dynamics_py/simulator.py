@@ -274,6 +274,8 @@ class Simulator:
         self._load()
         if self.model == "nsc":
             self.ctx.keep_manifolds(True)  # for write_network (one copy of the manifolds per step)
+        elif self.model == "soft":
+            self.ctx.keep_soft_forces(True)  # for write_network (6 doubles per contact per evaluation)
         self.last_ke = self.ctx.observables()[0]
         if not quiet:
             print(f"[simulator] {len(self.r)} rods, model {self.model}, dt {self.dt}", file=sys.stderr)
@@ -555,13 +557,27 @@ class Simulator:
         stream.write(sceneio.per_rod_text(self.frame_index, x, v, w, q, self.mass, self.r, self.h))
 
     def write_network(self, stream, header=False):
-        """Append the hard-contact network rows of the last step
-        (App::logNetworkFrame): one row per rod-rod manifold of the step's
-        velocity solve with its impulses, the normal velocity before the solve
-        and the normal velocity from the current body velocities."""
+        """Append the network rows of the last step (App::logNetworkFrame).
+        Hard contact: one row per rod-rod manifold of the step's velocity solve
+        with its impulses, the normal velocity before the solve and the normal
+        velocity from the current body velocities.  Soft contact: one row per
+        contact of the last force evaluation with its forces.  Hertz-Mindlin:
+        the soft header and no rows, as the reference writes."""
         if self.model != "nsc":
-            raise NotImplementedError("the network log of the soft-contact models needs per-contact forces, "
-                                      "which this library does not export")
+            if header:
+                stream.write(sceneio.NETWORK_SOFT_HEADER)
+            if self.model != "soft":
+                return
+            raw = self.ctx.contacts_raw()
+            f = self.ctx.soft_forces()
+            if len(f) != len(raw):
+                raise RuntimeError("soft forces and contact list out of step (%d against %d)" % (len(f), len(raw)))
+            pa = np.stack([raw["px_a"], raw["py_a"], raw["pz_a"]], 1) @ _R.astype(np.float32)
+            pb = np.stack([raw["px_b"], raw["py_b"], raw["pz_b"]], 1) @ _R.astype(np.float32)
+            nrm = np.stack([raw["nx"], raw["ny"], raw["nz"]], 1).astype(np.float64) @ _R
+            stream.write(sceneio.network_soft_text(self.frame_index, raw["a"], raw["b"], pa, pb, nrm, raw["dist"],
+                                                   f[:, 0:3] @ _R, f[:, 3:6] @ _R))
+            return
         rec, ab, lam = self.ctx.kept_manifolds()
         keep = ab[:, 1] >= 0
         rec, ab, lam = rec[keep], ab[keep], lam[keep]

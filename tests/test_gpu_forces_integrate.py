@@ -158,3 +158,42 @@ def test_verlet_steps_match_oracle_short_horizon(gpu_ctx_factory, model):
     assert np.abs(sg["x"] - so["x"]).max() < 1e-11
     assert np.abs(sg["v"] - so["v"]).max() < 1e-9
     assert np.abs(sg["w"] - so["w"]).max() < 1e-9
+
+
+@pytest.mark.parametrize("model", [0, 1, 2])
+def test_kept_soft_contact_forces(gpu_ctx_factory, model):
+    """The per-contact soft forces kept for the network log: scattered onto
+    the bodies (b gets force_b + friction_b, a their negatives) they give the
+    body forces, which match the oracle; force_b lies along the contact
+    normal and the friction force across it."""
+    B = packing(seed=5)
+    B["f"] = np.zeros_like(B["f"])
+    B["tau"] = np.zeros_like(B["tau"])
+    ctx = gpu_ctx_factory()
+    W = both(ctx, B, g=(0, 0, 0))
+    ctx.keep_soft_forces(True)
+    gp = L.soft_params(delta=0.005, k_scaler=1e3, damping=0.2, mu=0.5, mu_static=0.8, nu=1e-2, kt=500.0,
+                       vel_deadband=0.5, friction_model=model)
+    op = O.soft_params(delta=0.005, k_scaler=1e3, damping=0.2, mu=0.5, mu_static=0.8, nu=1e-2, kt=500.0,
+                       vel_deadband=0.5, friction_model=model)
+    for it in range(2):  # the second call reads the Cundall-Strack history of the first
+        nc = ctx.detect(L.detect_params(0.005, 0.0, 1.0))
+        assert W.detect(0, 0.005) == nc and nc > 200
+        f0 = ctx.download_state()["f"]  # forces accumulate over the calls
+        ctx.forces_soft(gp)
+        W.soft_forces(op)
+    raw, kf = ctx.contacts_raw(), ctx.soft_forces()
+    assert kf.shape == (len(raw), 6)
+    fb, fra = kf[:, 0:3], kf[:, 3:6]
+    tot = np.zeros((len(B["r"]), 3))
+    np.add.at(tot, raw["b"], fb - fra)
+    np.add.at(tot, raw["a"], fra - fb)
+    sg, so = ctx.download_state(), W.state()
+    assert rel(sg["f"], so["f"]) < 1e-12
+    assert rel(tot, sg["f"] - f0) < 1e-12
+    n = np.stack([raw["nx"], raw["ny"], raw["nz"]], 1).astype(np.float64)
+    assert np.abs(np.cross(fb, n)).max() <= 1e-6 * np.abs(fb).max()
+    assert np.abs(np.sum(fra * n, 1)).max() <= 1e-6 * max(np.abs(fra).max(), 1e-300)
+    assert np.abs(fra).max() > 0
+    ctx.keep_soft_forces(False)
+    assert len(ctx.soft_forces()) == 0

@@ -151,3 +151,42 @@ def test_network_log_of_a_hard_contact_step(tmp_path):
     rr = ab[:, 1] >= 0
     assert lam[rr][0, 0] == pytest.approx(lam_n, rel=1e-5)  # printed with 6 significant digits
     sim.close()
+
+
+def test_network_log_of_a_soft_contact_step(tmp_path):
+    """Two overlapping rods under soft contact: the network log has the soft
+    header and one row per contact of the last force evaluation, with
+    force_a = -force_b along the normal and friction_b = -friction_a; a
+    Hertz-Mindlin scene writes the header only."""
+    import io
+
+    from rod_dynamics_py import sceneio
+
+    bodies = [{"length": 1.0, "diameter": 0.5, "density": 1000.0, "pos": [-0.24, 0.0, 0.0], "v_lin": [0.0, 0.0, 0.3]},
+              {"length": 1.0, "diameter": 0.5, "density": 1000.0, "pos": [0.24, 0.0, 0.0],
+               "rot_quat": [0.7071067811865476, 0.7071067811865476, 0.0, 0.0], "v_lin": [0.0, 0.0, -0.3]}]
+    for model in ("soft_contact", "hertz_mindlin"):
+        scene = {"physics": {"dt": 1e-4, "gravity": [0, 0, 0], "lin_damp": 0.0, "ang_damp": 0.0,
+                             model: {"enabled": True}},
+                 "scene": {"bodies": bodies}}
+        (tmp_path / "s.json").write_text(json.dumps(scene))
+        sim = Simulator(str(tmp_path / "s.json"))
+        sim.step()
+        buf = io.StringIO()
+        sim.write_network(buf, header=True)
+        lines = buf.getvalue().splitlines()
+        assert lines[0] + "\n" == sceneio.NETWORK_SOFT_HEADER
+        if model == "hertz_mindlin":
+            assert len(lines) == 1
+            sim.close()
+            continue
+        raw = sim.ctx.contacts_raw()
+        assert len(raw) >= 1 and len(lines) == 1 + len(raw)
+        for ln in lines[1:]:
+            row = ln.split(",")
+            assert row[0] == "1" and {row[1], row[2]} == {"0", "1"}
+            v = np.array([float(t) for t in row[3:]])
+            nrm, fa, fb, fra, frb = v[3:6], v[7:10], v[10:13], v[13:16], v[16:19]
+            assert np.array_equal(fa, -fb) and np.array_equal(fra, -frb)
+            assert np.linalg.norm(fb) > 0 and abs(abs(np.dot(fb, nrm)) / np.linalg.norm(fb) - 1) < 1e-5
+        sim.close()
