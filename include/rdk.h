@@ -259,6 +259,16 @@ int rdk_nsc_get_kept_manifolds(rdk_ctx* ctx, int cap, double* rec, int* ab,
  * (friction_b is its negative).  A Hertz-Mindlin evaluation keeps none.
  * *count = their number, RDK_ERR_CAPACITY when cap is smaller. */
 int rdk_keep_soft_forces(rdk_ctx* ctx, int on);
+/* Record the kinetic energy after the stages of each fused hard-contact step,
+ * the profile log's KE_after_integrate, KE_after_solve and
+ * KE_after_posCorrect (main.cpp:4411,4461,4505; KE_after_pbcWrap, :4519,
+ * equals the last: the wrap moves no velocity).  rdk_get_stage_energies
+ * writes those of the last step into out[3]. */
+int rdk_record_stage_energies(rdk_ctx* ctx, int on);
+int rdk_get_stage_energies(rdk_ctx* ctx, double* out);
+/* Contact potential energy of the last soft or Hertz-Mindlin force evaluation
+ * (the profile log's soft_PE, main.cpp:3321); 0 before the first. */
+int rdk_last_potential_energy(rdk_ctx* ctx, double* pe);
 int rdk_get_soft_forces(rdk_ctx* ctx, int cap, double* out, int* count);
 /* model 0 soft (params = rdk_soft_params*), 1 Hertz-Mindlin (rdk_hm_params*) */
 int rdk_step_vv(rdk_ctx* ctx, const rdk_detect_params* dp, int model,

@@ -810,6 +810,31 @@ int rdk_nsc_get_kept_manifolds(rdk_ctx* ctx, int cap, double* rec, int* ab, doub
   return RDK_OK;
 }
 
+int rdk_record_stage_energies(rdk_ctx* ctx, int on) {
+  CTX_CHECK();
+  ctx->stage_ke = on != 0;
+  for (double& e : ctx->ke_stage) e = 0.0;
+  return RDK_OK;
+}
+
+int rdk_last_potential_energy(rdk_ctx* ctx, double* pe) {
+  CTX_CHECK();
+  if (!pe) return RDK_ERR_ARG;
+  *pe = 0.0;
+  if (ctx->pe.p) {
+    RDK_HIP(hipMemcpyAsync(pe, ctx->pe.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    RDK_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  return RDK_OK;
+}
+
+int rdk_get_stage_energies(rdk_ctx* ctx, double* out) {
+  CTX_CHECK();
+  if (!out) return RDK_ERR_ARG;
+  for (int k = 0; k < 3; ++k) out[k] = ctx->ke_stage[k];
+  return RDK_OK;
+}
+
 int rdk_keep_soft_forces(rdk_ctx* ctx, int on) {
   CTX_CHECK();
   ctx->keep_soft = on != 0;
@@ -847,6 +872,7 @@ int rdk_step_nsc(rdk_ctx* ctx, const rdk_detect_params* dp, const rdk_nsc_params
   std::memset(&S, 0, sizeof(S));
   for (int s = 0; s < nsteps; ++s) {
     RDK_TRY(integrate_run(ctx, RDK_NSC_FREE));
+    if (ctx->stage_ke) RDK_TRY(observables_run(ctx, &ctx->ke_stage[0], nullptr));  // main.cpp:4411
     RDK_TRY(detect_run(ctx, &d));
     RDK_TRY(nsc_build_run(ctx, np));
     rdk_solve_stats ss;
@@ -855,9 +881,11 @@ int rdk_step_nsc(rdk_ctx* ctx, const rdk_detect_params* dp, const rdk_nsc_params
     if (rc == RDK_ERR_NOCONV) S.nonconverged_steps++;
     else if (rc != RDK_OK) return rc;
     if (ctx->keep_manifolds) RDK_TRY(keep_manifolds(ctx));
+    if (ctx->stage_ke) RDK_TRY(observables_run(ctx, &ctx->ke_stage[1], nullptr));  // main.cpp:4461
     RDK_TRY(integrate_run(ctx, RDK_NSC_POS));
     if (np->position_stabilization && np->position_iters > 0)
       RDK_TRY(nsc_project_run(ctx, &d, np, nullptr, nullptr));  // main.cpp:4497-4503
+    if (ctx->stage_ke) RDK_TRY(observables_run(ctx, &ctx->ke_stage[2], nullptr));  // main.cpp:4505,4519
     S.steps++;
     S.contacts_total += ss.rod_contacts;
     S.manifolds_total += ss.manifolds;

@@ -182,6 +182,11 @@ struct rdk_ctx {
   // 6 doubles per contact in contact-list order, the normal force on b
   // (spring + damping) then the friction force on a
   bool keep_soft = false;
+  // kinetic energy after the free-motion update, the velocity solve and the
+  // position update (+ projection) of the last fused hard-contact step, when
+  // stage_ke is set
+  bool stage_ke = false;
+  double ke_stage[3] = {0.0, 0.0, 0.0};
   rdk::DevBuf<double> soft_kept;
   int soft_kept_n = 0;
   rdk::DevBuf<const double*> raw_ptrs;  // contact component arrays for the FP32 record export

@@ -114,6 +114,9 @@ _SIGS = {
     "rdk_get_contacts_raw": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_int)]),
     "rdk_keep_manifolds": (C.c_int, [C.c_void_p, C.c_int]),
     "rdk_keep_soft_forces": (C.c_int, [C.c_void_p, C.c_int]),
+    "rdk_record_stage_energies": (C.c_int, [C.c_void_p, C.c_int]),
+    "rdk_get_stage_energies": (C.c_int, [C.c_void_p, _dp]),
+    "rdk_last_potential_energy": (C.c_int, [C.c_void_p, _dp]),
     "rdk_get_soft_forces": (C.c_int, [C.c_void_p, C.c_int, _dp, C.POINTER(C.c_int)]),
     "rdk_nsc_get_kept_manifolds": (C.c_int, [C.c_void_p, C.c_int, _dp, C.POINTER(C.c_int), _dp,
                                              C.POINTER(C.c_int)]),
@@ -362,6 +365,21 @@ class Context:
 
     def keep_manifolds(self, on=True):
         self._check(self._L.rdk_keep_manifolds(self._h, int(bool(on))))
+
+    def record_stage_energies(self, on=True):
+        self._check(self._L.rdk_record_stage_energies(self._h, int(bool(on))))
+
+    def stage_energies(self):
+        """Kinetic energy after the free-motion update, the velocity solve and
+        the position update of the last fused hard-contact step."""
+        out = np.zeros(3)
+        self._check(self._L.rdk_get_stage_energies(self._h, dptr(out)))
+        return out
+
+    def last_potential_energy(self):
+        out = np.zeros(1)
+        self._check(self._L.rdk_last_potential_energy(self._h, dptr(out)))
+        return float(out[0])
 
     def keep_soft_forces(self, on=True):
         self._check(self._L.rdk_keep_soft_forces(self._h, int(bool(on))))

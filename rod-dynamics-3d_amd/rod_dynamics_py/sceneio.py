@@ -17,7 +17,9 @@
   of the per-rod log like App::logPerRodFrame (main.cpp:3412-3449), and
   ``network_text`` the hard-contact network log like App::logNetworkFrame
   (main.cpp:3624-3730, its hard-contact branch; ``network_soft_text`` its
-  soft-contact branch).  The
+  soft-contact branch), and ``profile_row`` a row of the profile log like
+  App::logCsvFrame (main.cpp:3276-3355) with ``com``, ``gyration_sq`` and
+  ``RelDisp`` for its packing columns (main.cpp:938-1022, 3500-3560).  The
   reference stores state in FP32 (glm::vec3 / glm::quat) and prints it with
   iostream defaults (``%g``, 6 significant digits) or ``std::fixed`` with 6
   decimals; the library's FP64 state is rounded to FP32 first and printed the
@@ -205,6 +207,84 @@ def network_soft_text(frame, a, b, point_a, point_b, normal, dist, force_b, fric
         vals = [*mid[i], *normal[i], dist[i], *(-fb[i]), *fb[i], *fra[i], *(-fra[i])]
         out.append("%d,%d,%d," % (frame, a[i], b[i]) + ",".join(cxx_g(t) for t in vals) + "\n")
     return "".join(out)
+
+
+PROFILE_HEADER_SOFT = ("frame,rods,integrate_ms,sleep_ms,broadphase_ms,bpCount_ms,bpPrefix_ms,bpFill_ms,bpPairs_ms,"
+                       "solve_ms,pbcWrap_ms,render_ms,contacts,collisions,KE,soft_PE,gyration_sq,reldisp_sq,"
+                       "ent_pairs,ent_sum,step_ms\n")
+PROFILE_HEADER_NSC = ("frame,rods,integrate_ms,sleep_ms,broadphase_ms,bpCount_ms,bpPrefix_ms,bpFill_ms,bpPairs_ms,"
+                      "bpLongLong_ms,warmstart_ms,buildIslands_ms,solve_ms,floorSolve_ms,posCorrect_ms,pbcWrap_ms,"
+                      "render_ms,contacts,collisions,islands,KE,KE_after_integrate,KE_after_warmstart,KE_after_solve,"
+                      "KE_after_posCorrect,KE_after_pbcWrap,soft_PE,gyration_sq,reldisp_sq,jn_sum,jt_sum,"
+                      "nsc_residual,ent_pairs,ent_sum,step_ms\n")
+
+
+def profile_row(values):
+    """One profile-log row: ints as integers, everything else with iostream
+    defaults."""
+    return ",".join(str(v) if isinstance(v, (int, np.integer)) else cxx_g(v) for v in values) + "\n"
+
+
+def _box(lo, hi, pbc):
+    return (np.asarray(hi, np.float32) - np.asarray(lo, np.float32)) if pbc else None
+
+
+def _min_image(d, box):
+    """d - box floor(d / box + 1/2) in FP32 on the axes with a positive box."""
+    d = np.asarray(d, np.float32).copy()
+    for k in range(3):
+        if box is not None and box[k] > np.float32(0):
+            d[..., k] -= box[k] * np.floor(d[..., k] / box[k] + np.float32(0.5))
+    return d
+
+
+def com(x, mass, lo, hi, pbc):
+    """App::computeCOM: mass-weighted mean, on each periodic axis the circular
+    mean of 2 pi (x - lo) / L mapped back into [lo, hi); FP32 result."""
+    x = np.asarray(x, np.float32).astype(np.float64)
+    m = np.asarray(mass, np.float32).astype(np.float64)
+    out = np.zeros(3)
+    if len(m) == 0:
+        return out.astype(np.float32)
+    box = _box(lo, hi, pbc)
+    for k in range(3):
+        if box is None or not box[k] > 0:
+            out[k] = np.sum(m * x[:, k]) / np.sum(m)
+        else:
+            th = 2.0 * math.pi * ((x[:, k] - float(np.float32(lo[k]))) / float(box[k]))
+            a = math.atan2(np.sum(m * np.sin(th)) / np.sum(m), np.sum(m * np.cos(th)) / np.sum(m))
+            if a < 0.0:
+                a += 2.0 * math.pi
+            out[k] = float(np.float32(lo[k])) + (a / (2.0 * math.pi)) * float(box[k])
+    return out.astype(np.float32)
+
+
+def gyration_sq(x, mass, lo, hi, pbc):
+    """App::computeGyrationSq: mean squared (minimum-image) distance to the COM."""
+    if len(x) == 0:
+        return 0.0
+    d = _min_image(np.asarray(x, np.float32) - com(x, mass, lo, hi, pbc), _box(lo, hi, pbc))
+    return float(np.sum(np.sum(d * d, axis=1, dtype=np.float32).astype(np.float64)) / len(d))
+
+
+class RelDisp:
+    """App::computeRelativeMotionSq: mean squared displacement since the last
+    call relative to the COM's (minimum image); 0 on the first call."""
+
+    def __init__(self):
+        self.last = None
+
+    def __call__(self, x, mass, lo, hi, pbc):
+        x = np.asarray(x, np.float32)
+        c = com(x, mass, lo, hi, pbc)
+        if self.last is None or len(self.last[0]) != len(x):
+            self.last = (x.copy(), c)
+            return 0.0
+        box = _box(lo, hi, pbc)
+        dc = _min_image(c - self.last[1], box)
+        u = _min_image(x - self.last[0], box) - dc
+        self.last = (x.copy(), c)
+        return float(np.sum(np.sum(u * u, axis=1, dtype=np.float32).astype(np.float64)) / len(x))
 
 
 def per_rod_header(r0, h0):

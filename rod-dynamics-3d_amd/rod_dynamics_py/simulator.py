@@ -45,6 +45,7 @@ MuJoCo-style model) raise ``NotImplementedError`` instead of being ignored.
 """
 import json
 import math
+import time
 import sys
 
 import numpy as np
@@ -271,9 +272,13 @@ class Simulator:
         self.frame_index = frame0
         self.last_hit_count = 0
         self.last_island_count = 0
+        self.last_residual = 0.0
+        self.last_step_ms = 0.0
+        self._reldisp = sceneio.RelDisp()
         self._load()
         if self.model == "nsc":
             self.ctx.keep_manifolds(True)  # for write_network (one copy of the manifolds per step)
+            self.ctx.record_stage_energies(True)  # for write_profile (three KE reductions per step)
         elif self.model == "soft":
             self.ctx.keep_soft_forces(True)  # for write_network (6 doubles per contact per evaluation)
         self.last_ke = self.ctx.observables()[0]
@@ -496,13 +501,16 @@ class Simulator:
         if steps < 0:
             raise ValueError("steps must be non-negative")
         for _ in range(int(steps)):
+            t0 = time.perf_counter()
             if self.model == "nsc":
                 st = self.ctx.step_nsc(self.det, self.params, 1)
+                self.last_residual = float(st["residual_max"])
                 self.last_hit_count = int(st["manifolds_total"])
                 self.last_island_count = int(self.ctx.island_info()[0]) if st["manifolds_total"] > 0 else 0
             else:
                 st = self.ctx.step_vv(self.det, 1 if self.model == "hm" else 0, self.params, 1)
                 self.last_hit_count = int(st["contacts_total"] // 2)
+            self.last_step_ms = 1e3 * (time.perf_counter() - t0)
             self.frame_index += 1
         self.last_ke = self.ctx.observables()[0]
 
@@ -592,6 +600,41 @@ class Simulator:
             stream.write(sceneio.NETWORK_HEADER)
         stream.write(sceneio.network_text(self.frame_index, a, b, point, n_l @ _R, rec[:, 38], lam, rec[:, 39],
                                           vn_post))
+
+    def write_profile(self, stream, header=False, ent_pairs=0, ent_sum=0.0):
+        """Append one row of the profile log for the last step
+        (App::logCsvFrame): the hard-contact header with the kinetic energy
+        after each stage, the impulse sums and the solve's residual, or the
+        slim soft/Hertz-Mindlin header.  Stage timings are the reference's own
+        CPU timers and are written as 0 (the library's per-kernel times come
+        from its profiling counters); step_ms is the wall time of the last
+        step; ent_pairs / ent_sum are the entanglement observer's values when
+        the caller runs it."""
+        per = self.cfg["periodic"]
+        pbc = bool(per["enabled"])
+        lo, hi = per["min"], per["max"]
+        x, m = self._state_ref()[0], self.mass
+        gyr = sceneio.gyration_sq(x, m, lo, hi, pbc)
+        rel = self._reldisp(x, m, lo, hi, pbc)
+        n = len(self.r)
+        if self.model == "nsc":
+            if header:
+                stream.write(sceneio.PROFILE_HEADER_NSC)
+            rec, ab, lam = self.ctx.kept_manifolds()
+            impacts = int(np.sum(rec[:, 39] < -1e-6)) if len(rec) else 0
+            ke = self.ctx.stage_energies()
+            row = [self.frame_index, n] + [0.0] * 15 + [self.last_hit_count, impacts, self.last_island_count,
+                                                        self.last_ke, ke[0], 0.0, ke[1], ke[2], ke[2], 0.0, gyr, rel,
+                                                        float(np.sum(np.abs(lam[:, 0]))),
+                                                        float(np.sum(np.abs(lam[:, 1]) + np.abs(lam[:, 2]))),
+                                                        self.last_residual, int(ent_pairs), ent_sum, self.last_step_ms]
+        else:
+            if header:
+                stream.write(sceneio.PROFILE_HEADER_SOFT)
+            row = [self.frame_index, n] + [0.0] * 10 + [self.last_hit_count, -1, self.last_ke,
+                                                        self.ctx.last_potential_energy(), gyr, rel, int(ent_pairs),
+                                                        ent_sum, self.last_step_ms]
+        stream.write(sceneio.profile_row(row))
 
     def close(self):
         if getattr(self, "ctx", None) is not None:

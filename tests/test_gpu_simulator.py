@@ -2,6 +2,7 @@
 bodies (the elastic exchange of smoke_test.py:165-195 through Simulator), and
 a frozen relaxed packing of the reference loaded from its endpoint file."""
 import json
+import math
 import os
 
 import numpy as np
@@ -190,3 +191,54 @@ def test_network_log_of_a_soft_contact_step(tmp_path):
             assert np.array_equal(fa, -fb) and np.array_equal(fra, -frb)
             assert np.linalg.norm(fb) > 0 and abs(abs(np.dot(fb, nrm)) / np.linalg.norm(fb) - 1) < 1e-5
         sim.close()
+
+
+def test_profile_log_of_a_hard_contact_step(tmp_path):
+    """The profile row of the head-on hard-contact step: one manifold, one
+    impact, one island, the kinetic energy 2 x m v^2 / 2 after the free update
+    (no gravity or damping), less after the inelastic solve, the normal impulse
+    sum equal to the kept manifold's, and the slim header for soft contact."""
+    import io
+
+    from rod_dynamics_py import sceneio
+
+    bodies = [{"length": 1.0, "diameter": 0.5, "density": 1000.0, "restitution": 0.0, "pos": [-0.249, 0.0, 0.0],
+               "v_lin": [1.0, 0.0, 0.0]},
+              {"length": 1.0, "diameter": 0.5, "density": 1000.0, "restitution": 0.0, "pos": [0.249, 0.0, 0.0],
+               "rot_quat": [0.7071067811865476, 0.7071067811865476, 0.0, 0.0], "v_lin": [-1.0, 0.0, 0.0]}]
+    scene = {"physics": {"dt": 1e-3, "gravity": [0, 0, 0], "lin_damp": 0.0, "ang_damp": 0.0,
+                         "nsc": {"enabled": True, "mu": 0.0, "beta": 0.0, "cfm": 0.0, "omega": 1.0,
+                                 "velocity_iters": 200}},
+             "scene": {"bodies": bodies}}
+    (tmp_path / "s.json").write_text(json.dumps(scene))
+    sim = Simulator(str(tmp_path / "s.json"))
+    sim.step()
+    buf = io.StringIO()
+    sim.write_profile(buf, header=True)
+    head, row = buf.getvalue().splitlines()
+    assert head + "\n" == sceneio.PROFILE_HEADER_NSC
+    cols = dict(zip(head.split(","), row.split(",")))
+    assert len(row.split(",")) == 35
+    assert cols["frame"] == "1" and cols["rods"] == "2"
+    assert cols["contacts"] == "1" and cols["collisions"] == "1" and cols["islands"] == "1"
+    m = 1000.0 * math.pi * 0.25 ** 2 * 1.0
+    ke_int, ke_sol = float(cols["KE_after_integrate"]), float(cols["KE_after_solve"])
+    assert ke_int == pytest.approx(m, rel=1e-5) and ke_sol < 0.5 * ke_int
+    assert float(cols["KE"]) == pytest.approx(float(cols["KE_after_posCorrect"]), rel=1e-5)
+    rec, ab, lam = sim.ctx.kept_manifolds()
+    assert float(cols["jn_sum"]) == pytest.approx(np.abs(lam[:, 0]).sum(), rel=1e-5)
+    assert float(cols["step_ms"]) > 0.0
+    sim.close()
+    scene["physics"] = {"dt": 1e-4, "gravity": [0, 0, 0], "lin_damp": 0.0, "ang_damp": 0.0,
+                        "soft_contact": {"enabled": True}}
+    bodies[0]["pos"], bodies[1]["pos"] = [-0.24, 0.0, 0.0], [0.24, 0.0, 0.0]
+    (tmp_path / "s.json").write_text(json.dumps(scene))
+    sim = Simulator(str(tmp_path / "s.json"))
+    sim.step()
+    buf = io.StringIO()
+    sim.write_profile(buf, header=True)
+    head, row = buf.getvalue().splitlines()
+    assert head + "\n" == sceneio.PROFILE_HEADER_SOFT
+    cols = dict(zip(head.split(","), row.split(",")))
+    assert len(row.split(",")) == 21 and cols["collisions"] == "-1" and float(cols["soft_PE"]) > 0.0
+    sim.close()

@@ -190,3 +190,41 @@ def test_snapshot_line_fields():
         assert b["radius"] == 0.05 and b["halfHeight"] == 0.5
     # float32 time as the reference computes it: double(frame) * double(float dt)
     assert d["time"] == float("%g" % (120 * float(np.float32(1.0 / 600.0))))
+
+
+def test_profile_log_packing_columns_closed_forms():
+    """COM, gyration and relative displacement of the profile log
+    (main.cpp:938-1022, 3500-3560) on configurations with known answers: the
+    mass-weighted mean off a periodic box, the circular mean across a periodic
+    face (the plain mean would sit mid-box), minimum images, and displacements
+    relative to the COM's."""
+    from rod_dynamics_py import sceneio as S
+
+    lo, hi = [0.0, 0.0, 0.0], [10.0, 10.0, 10.0]
+    x = np.array([[0.0, 1.0, 1.0], [4.0, 1.0, 1.0]])
+    assert np.allclose(S.com(x, [1.0, 3.0], lo, hi, False), [3.0, 1.0, 1.0])
+    assert S.gyration_sq(x, [1.0, 3.0], lo, hi, False) == pytest.approx(5.0, rel=1e-6)
+    xp = np.array([[0.1, 5.0, 5.0], [9.9, 5.0, 5.0]])
+    c = S.com(xp, [1.0, 1.0], lo, hi, True)
+    assert min(abs(c[0]), abs(c[0] - 10.0)) < 1e-5 and abs(c[1] - 5.0) < 1e-5
+    assert S.gyration_sq(xp, [1.0, 1.0], lo, hi, True) == pytest.approx(0.01, rel=1e-4)
+    R = S.RelDisp()
+    assert R(x, [1.0, 1.0], lo, hi, False) == 0.0
+    assert R(x + [1.0, 2.0, 3.0], [1.0, 1.0], lo, hi, False) == pytest.approx(0.0, abs=1e-10)
+    x2 = x + [1.0, 2.0, 3.0]
+    x2[0, 0] += 1.0  # one of two equal rods moves: the COM moves half as far
+    assert R(x2, [1.0, 1.0], lo, hi, False) == pytest.approx(0.25, rel=1e-6)
+    Rp = S.RelDisp()
+    a = np.array([[9.95, 5.0, 5.0], [3.0, 5.0, 5.0]])
+    Rp(a, [1.0, 1.0], lo, hi, True)
+    b = a.copy()
+    b[0, 0] = 0.05  # crosses the face: a step of 0.1, not -9.9
+    assert Rp(b, [1.0, 1.0], lo, hi, True) == pytest.approx(0.05 ** 2, rel=1e-3)
+
+
+def test_profile_log_headers_and_row_format():
+    from rod_dynamics_py import sceneio as S
+
+    assert len(S.PROFILE_HEADER_NSC.strip().split(",")) == 35
+    assert len(S.PROFILE_HEADER_SOFT.strip().split(",")) == 21
+    assert S.profile_row([3, 10, 0.0, -1, 1.5e-7, 123456789.0]) == "3,10,0,-1,1.5e-07,1.23457e+08\n"
