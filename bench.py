@@ -80,6 +80,8 @@ def parse():
     ap.add_argument("--stall-sweeps", type=int, default=None,
                     help="override the workload's stall-rule window (hard-contact workloads)")
     ap.add_argument("--block", type=int, default=250)
+    ap.add_argument("--protocol-steps", type=int, default=PROTOCOL_STEPS,
+                    help="settling workloads: length of the protocol the timed blocks are spread over")
     ap.add_argument("--ab-steps", type=int, default=20,
                     help="steps after the timed protocol solved from cold, previous-step and learned starts")
     ap.add_argument("--weights", default=None,
@@ -148,15 +150,6 @@ def autotune(ctx, run_steps, cands, reps=3):
     return best, {str(k): v for k, v in tune.items()}, len(cands) * (reps + 1)
 
 
-# Dominant kernel of a workload whose warm-up is not representative, from the
-# rocprofv3 kernel statistics of the whole protocol (profiles/): on the
-# settling bed the 20 warm-up steps are the dilute phase (broad phase first),
-# while over the 5000 steps the LDS island kernel takes the largest share of GPU
-# time (the bed's percolating cluster is solved there); on the 2e5-rod bed the
-# colour launches of the huge-island solve.
-DOMINANT = {"c2": "island_large", "c5": "psor_sweep"}
-
-
 def rank_kernels(ctx, run_warmup, fixed=None):
     """Warm-up with every kernel timed; returns the dominant kernel among those
     with a byte model (or `fixed`) and leaves only that kernel's timer on."""
@@ -172,20 +165,25 @@ def rank_kernels(ctx, run_warmup, fixed=None):
     return dom
 
 
-def roofline(ctx, dom, n, workload):
-    kt = ctx.kernel_times()
-    wc = ctx.work_counters()
-    launches, kms = kt[dom]
+def roofline_from(dom, kt, wc, n, workload):
+    """kt = (timed launches, summed event ms) of kernel `dom`; wc = work
+    counters over the same launches."""
+    launches, kms = kt
     unit, bpu = BYTES[dom]
-    units = {"bodies": wc["body_passes"] * n if dom == "broad_narrow" else launches * n,
-             "sweep_contacts": wc["sweep_contacts"], "resid_contacts": wc["resid_contacts"],
-             "psor_timed_updates": wc["psor_timed_updates"],
-             "lds_manifold_sweeps": wc["lds_manifold_sweeps"],
-             "manifolds": wc["manifolds"], "contacts": wc["contacts"], "force_contacts": wc["force_contacts"]}[unit]
+    units = {"bodies": wc.get("body_passes", 0) * n if dom == "broad_narrow" else launches * n,
+             "sweep_contacts": wc.get("sweep_contacts", 0), "resid_contacts": wc.get("resid_contacts", 0),
+             "psor_timed_updates": wc.get("psor_timed_updates", 0),
+             "lds_manifold_sweeps": wc.get("lds_manifold_sweeps", 0),
+             "manifolds": wc.get("manifolds", 0), "contacts": wc.get("contacts", 0),
+             "force_contacts": wc.get("force_contacts", 0)}[unit]
     achieved = (bpu * units / 1e9) / (kms / 1e3) if kms > 0 else 0.0
     return {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC.get((workload, dom)), "launches": launches,
             "avg_launch_ms": kms / max(launches, 1), "bytes_per_unit": bpu, "unit_name": unit, "units": units}
+
+
+def roofline(ctx, dom, n, workload):
+    return roofline_from(dom, ctx.kernel_times()[dom], ctx.work_counters(), n, workload)
 
 
 def base_line(metric, value, a, el, dtype, config):
@@ -195,11 +193,51 @@ def base_line(metric, value, a, el, dtype, config):
 
 
 # ---------------------------------------------------------------------------- C2
+PROTOCOL_STEPS = 5000  # the settling protocol: 5000 steps at dt = 1e-3
+
+
+def phase_blocks(total, warm, k, max_blocks=20):
+    """The timed steps of a protocol of `total` steps whose first `warm` are
+    warm-up.  When warm + k covers the protocol, one block of every step after
+    the warm-up.  Otherwise the k steps are split into B equal blocks (B the
+    largest divisor of k not above max_blocks), block b centred in the b-th of
+    B equal strata of steps warm..total, so that the mean step time of the
+    blocks estimates the mean over the protocol (stratified sample)."""
+    span = total - warm
+    if k >= span:
+        return [(warm, span)]
+    nb = max(d for d in range(1, min(k, max_blocks) + 1) if k % d == 0)
+    size = k // nb
+    out = []
+    for b in range(nb):
+        s = int(warm + (b + 0.5) * span / nb - size / 2)
+        s = max(s, out[-1][0] + out[-1][1] if out else warm)
+        out.append((s, size))
+    return out
+
+
+def advance(ctx, det, p, cur, target, t_ref, block=250, acc=None):
+    """Steps cur..target, one progress line per `block` steps; the step
+    statistics are added into `acc` when given."""
+    first = cur
+    while cur < target:
+        k = min(block, target - cur)
+        st = ctx.step_nsc(det, p, k)
+        if acc is not None:
+            for key, val in st.items():
+                acc[key] = max(acc.get(key, 0.0), val) if key == "residual_max" else acc.get(key, 0) + val
+        cur += k
+        if k == block or (cur == target and target - first >= 10):
+            log(f"  step {cur} ({time.perf_counter() - t_ref:.1f}s)")
+    return cur
+
+
 def cpu_baseline_c2(sc, snaps, total_steps, O):
     """The FP64 oracle (reference spatial hash + serial sorted-order PSOR to
-    1e-8, OpenMP elsewhere) on GPU-state snapshots spread over the protocol:
-    at each snapshot one unmeasured step (fills the warm cache), then one timed
-    step; the per-phase step times are extrapolated to the whole protocol."""
+    1e-8, OpenMP elsewhere) from GPU-state snapshots at the start of timed
+    blocks (one per stratum of the protocol): at each, one unmeasured step
+    (fills the warm cache), then one timed step; the mean step time over the
+    strata is the protocol's mean step time."""
     p = O.nsc_params(mu=sc.extra["mu"], beta=0.0, cfm=0.0, omega=1.3, tol=1e-8, max_sweeps=10000, check_every=1,
                      friction_proj=sc.params.friction_proj, stop_rule=1, warm_start=1,
                      warm_walls=sc.params.warm_walls)
@@ -215,19 +253,27 @@ def cpu_baseline_c2(sc, snaps, total_steps, O):
         t_all += dt
         per.append({"step": step + 2, "seconds": dt, "sweeps": sweeps})
         log(f"cpu baseline: step {step + 2}: {dt:.2f}s, {sweeps} sweeps")
-    est = sum(r["seconds"] for r in per) * total_steps / len(per)
-    return {"value": sc.n * total_steps / est, "unit": "rod-steps/s", "cores": cores(), "kind": "port",
+    mean = sum(r["seconds"] for r in per) / len(per)
+    return {"value": sc.n / mean, "unit": "rod-steps/s", "cores": cores(), "kind": "port",
             "sample": f"FP64 oracle (reference spatial hash + serial sorted-order PSOR to residual 1e-8, OpenMP "
-                      f"{cores()} threads) timed for one step at each of {len(per)} GPU-state snapshots spread over "
-                      f"the same run (one unmeasured step before each); per-step times extrapolated to the "
-                      f"{total_steps}-step protocol",
+                      f"{cores()} threads): one timed step (after one unmeasured step) from the GPU state at each of "
+                      f"{len(per)} points spread evenly over the {total_steps}-step protocol (the starts of timed "
+                      f"blocks); value = rods / mean step time",
             "seconds_measured": t_all, "per_snapshot": per}
 
 
 def run_c2(a, L, scenes):
     """The settling protocol; --workload c5 runs it at N = 2e5 from an evaluation
     seed (the learned start is trained on other seeds) and then compares the
-    starts on the settled bed."""
+    starts on the settled bed.
+
+    The protocol runs twice from the same start; the trajectory is
+    deterministic (bitwise, checked on the counts of every timed block).  Run 1
+    (untimed) times every kernel inside the timed blocks to rank them and takes
+    the state snapshots of the CPU baseline.  Run 2 times the --steps steps as
+    equal blocks at evenly spaced phases of the protocol (phase_blocks), with
+    only the dominant kernel's HIP events recorded, and continues to the end of
+    the protocol, where the learned-start comparison runs on the settled bed."""
     t0 = time.perf_counter()
     c5 = a.workload == "c5"
     if c5:
@@ -241,7 +287,11 @@ def run_c2(a, L, scenes):
     p.check_every = a.check_every
     if a.stall_sweeps is not None:
         p.stall_sweeps = a.stall_sweeps
+    total = a.protocol_steps
+
+    # ---- run 1: warm-up (chunk length autotune), ranking and CPU snapshots
     ctx = L.Context(0)
+    names = [ctx._L.rdk_kernel_name(k).decode() for k in range(ctx._L.rdk_num_kernels())]
     sc.load(ctx)
     if a.chunk_len > 0:
         best, tune, used = a.chunk_len, {}, 0
@@ -249,60 +299,136 @@ def run_c2(a, L, scenes):
         best, tune, used = autotune(ctx, lambda cl, k: ctx.step_nsc(L.detect_params(0.0, cl, 1.0), p, k),
                                     (2.0, 4.0, 8.0, 16.0), reps=min(3, max(1, a.warmup // 3 - 1)))
     det = L.detect_params(0.0, best, 1.0)
-    dom = rank_kernels(ctx, lambda: ctx.step_nsc(det, p, max(1, a.warmup - used)), DOMINANT.get(a.workload))
-    log(f"warmup {a.warmup} steps done; dominant kernel {dom}; timing {a.steps} steps")
-    total = a.warmup + a.steps
-    snap_at = [int(total * (k + 0.5) / 5) for k in range(5)] if not a.no_cpu else []
+    warm = max(a.warmup, used)
+    if warm > used:
+        ctx.step_nsc(det, p, warm - used)
+    blocks = phase_blocks(total, warm, a.steps)
+    cpu_at = set()
+    if not a.no_cpu:
+        idx = range(len(blocks)) if len(blocks) <= 10 else [int((j + 0.5) * len(blocks) / 10) for j in range(10)]
+        cpu_at = {blocks[j][0] for j in idx}
+        if len(blocks) == 1:  # whole protocol timed: five points inside it
+            cpu_at = {warm + int((j + 0.5) * (total - warm) / 5) for j in range(5)}
+    log(f"warm-up {warm} steps; {len(blocks)} timed block(s) of {blocks[0][1]} step(s); run 1 (ranking)")
+    starts = {s: k for s, k in blocks}
+    ends = {s + k for s, k in blocks}
+    marks = sorted(set(starts) | ends | cpu_at)
+    rank_ms = {}
+    run1 = []
+    acc = None
+    cur = warm
     snaps = []
-    t_snap = 0.0
-    st = None
-    done = 0
     t1 = time.perf_counter()
-    while done < a.steps:  # blocks of steps, one progress line each (stderr)
-        nxt = [s - a.warmup for s in snap_at if s - a.warmup > done]
-        k = min(a.block, a.steps - done, (nxt[0] - done) if nxt else a.steps)
-        s_ = ctx.step_nsc(det, p, k)
-        done += k
-        if st is None:
-            st = dict(s_)
-        else:
-            for key in st:
-                st[key] = max(st[key], s_[key]) if key == "residual_max" else st[key] + s_[key]
-        if nxt and done == nxt[0]:
-            ts = time.perf_counter()
-            snaps.append((a.warmup + done, ctx.download_state()))  # excluded from the timed region below
-            t_snap += time.perf_counter() - ts
-        log(f"step {a.warmup + done}: {time.perf_counter() - t1 - t_snap:.2f}s, manifolds/step "
-            f"{s_['manifolds_total'] / k:.0f}, sweeps/step {s_['sweeps_total'] / k:.1f}")
-    ctx.sync()
-    el = time.perf_counter() - t1 - t_snap
-    out = base_line("rod-steps/s at PSOR residual 1e-8", sc.n * a.steps / el, a, el, "f64", {
+    for m in marks:
+        cur = advance(ctx, det, p, cur, m, t1, acc=acc)
+        if m in ends:  # a timed block ends here
+            for nm_, (n_, ms_) in ctx.kernel_times().items():
+                rank_ms[nm_] = rank_ms.get(nm_, 0.0) + ms_
+            ctx.set_profiling(False)
+            run1.append(acc)
+            acc = None
+        if m in cpu_at:
+            snaps.append((m, ctx.download_state()))
+        if m in starts:  # a timed block starts here: every kernel timed (resets the counters)
+            ctx.set_profiling(True)
+            acc = {}
+    ctx.close()
+    ranked = sorted(((ms, nm_) for nm_, ms in rank_ms.items() if nm_ in BYTES), reverse=True)
+    dom = ranked[0][1] if ranked else "broad_narrow"
+    share = {nm_: round(ms, 3) for ms, nm_ in sorted(((ms, nm_) for nm_, ms in rank_ms.items()), reverse=True)[:8]}
+    log(f"run 1 done in {time.perf_counter() - t1:.1f}s; dominant kernel in the timed blocks: {dom}; "
+        f"kernel ms in the blocks: {share}")
+
+    # ---- run 2: the same protocol, timed blocks
+    ctx = L.Context(0)
+    sc.load(ctx)
+    ctx.set_profiling(True)
+    ctx.set_profile_mask(1 << names.index(dom))
+    ctx.step_nsc(det, p, warm)
+    cur = warm
+    wc_names = ("manifolds", "sweep_contacts", "resid_contacts", "psor_timed_updates", "lds_manifold_sweeps",
+                "contacts", "body_passes", "force_contacts")
+    acc_wc = {k: 0 for k in wc_names}
+    acc_kt = [0, 0.0]
+    el = 0.0
+    phases = []
+    identical = True
+    t2 = time.perf_counter()
+    log("run 2 (timed)")
+    for (s0, k), ref in zip(blocks, run1):
+        cur = advance(ctx, det, p, cur, s0, t2)
+        kt0 = ctx.kernel_times()[dom]
+        wc0 = ctx.work_counters()
+        ctx.sync()
+        tb = time.perf_counter()
+        st = ctx.step_nsc(det, p, k)
+        ctx.sync()
+        dt = time.perf_counter() - tb
+        el += dt
+        cur += k
+        kt1 = ctx.kernel_times()[dom]
+        wc1 = ctx.work_counters()
+        acc_kt[0] += kt1[0] - kt0[0]
+        acc_kt[1] += kt1[1] - kt0[1]
+        for key in wc_names:
+            acc_wc[key] += wc1[key] - wc0[key]
+        same = all(st[key] == ref[key] for key in ("manifolds_total", "sweeps_total", "contacts_total",
+                                                    "nonconverged_steps"))
+        identical &= same
+        phases.append({"step": s0 + 1, "steps": k, "ms_per_step": 1e3 * dt / k,
+                       "manifolds_per_step": st["manifolds_total"] / k, "sweeps_per_step": st["sweeps_total"] / k,
+                       "rod_contacts_per_step": st["contacts_total"] / k, "colors_per_step": st["colors_total"] / k,
+                       "nonconverged_steps": st["nonconverged_steps"], "residual_max": st["residual_max"]})
+        log(f"timed block at step {s0 + 1}: {1e3 * dt / k:.2f} ms/step, manifolds/step "
+            f"{st['manifolds_total'] / k:.0f}, sweeps/step {st['sweeps_total'] / k:.1f}, residual "
+            f"{st['residual_max']:.2e}{'' if same else ' (differs from run 1)'}")
+    cur = advance(ctx, det, p, cur, total, t2)  # the rest of the protocol (untimed)
+    kt_all = ctx.kernel_times()[dom]
+    steps = sum(k for _, k in blocks)
+    nonconv = sum(ph["nonconverged_steps"] for ph in phases)
+    mpf = sum(ph["manifolds_per_step"] * ph["steps"] for ph in phases) / steps
+    if mpf <= 0:
+        raise SystemExit("bench.py: the timed blocks held no contact manifolds; no PSOR was measured")
+    converged = steps - nonconv
+    out = base_line("rod-steps/s at PSOR residual 1e-8", sc.n * converged / el, a, el, "f64", {
         "workload": ("learned-start bed: hard-contact settling, evaluation seed (C5)" if c5 else
                      "hard-contact PSOR settling (C2)"), "rods": sc.n, "L_over_d": 20, "phi0": 0.05,
         "box": round(sc.extra["side"], 3), "periodic": "x,y", "walls": "floor+ceiling", "mu": 0.5,
         "restitution": 0.0, "omega": 1.3, "residual_tol": 1e-8, "dt": 1e-3,
         "friction_projection": "euclidean disc" if p.friction_proj == 1 else "reference sequential",
-        "steps_total": total + max(0, used + max(1, a.warmup - used) - a.warmup), "seed": sc.extra["seed"],
-        "check_every": a.check_every, "chunk_len": best, "chunk_len_autotune_ms": tune,
+        "protocol_steps": total, "warmup_steps": warm,
+        "timed_blocks": f"{len(blocks)} block(s) of {blocks[0][1]} step(s), centred in equal strata of steps "
+                        f"{warm + 1}..{total}" if len(blocks) > 1 else f"steps {warm + 1}..{total}",
+        "seed": sc.extra["seed"], "check_every": a.check_every, "chunk_len": best, "chunk_len_autotune_ms": tune,
         "stall_sweeps": p.stall_sweeps, "stall_omega": p.stall_omega,
         "hipgraph": "check blocks of the huge-island solve (sweeps x colours + residual) captured once per island "
                     "and step, replayed; the step itself reads counts back and is not captured"})
-    out["psor"] = {"sweeps_per_step": st["sweeps_total"] / max(st["steps"], 1),
-                   "manifolds_per_step": st["manifolds_total"] / max(st["steps"], 1),
-                   "rod_contacts_per_step": st["contacts_total"] / max(st["steps"], 1),
-                   "colors_per_step": st["colors_total"] / max(st["steps"], 1),
-                   "residual_max": st["residual_max"], "nonconverged_steps": st["nonconverged_steps"],
+    out["value_counts"] = (f"rods x timed steps that reached residual 1e-8 ({converged} of {steps}) / wall time of "
+                           f"all {steps} timed steps")
+    out["psor"] = {"sweeps_per_step": sum(ph["sweeps_per_step"] * ph["steps"] for ph in phases) / steps,
+                   "manifolds_per_step": mpf,
+                   "rod_contacts_per_step": sum(ph["rod_contacts_per_step"] * ph["steps"] for ph in phases) / steps,
+                   "residual_max_converged": max([ph["residual_max"] for ph in phases if ph["nonconverged_steps"] == 0]
+                                                 or [0.0]),
+                   "nonconverged_steps": nonconv,
+                   "nonconverged_residuals": [ph["residual_max"] for ph in phases if ph["nonconverged_steps"] > 0],
                    "warm_start": "previous step (rod-rod and plane-wall contacts)"}
-    out["roofline"] = roofline(ctx, dom, sc.n, a.workload)
+    out["phases"] = phases
+    out["replay_identical"] = identical
+    out["roofline"] = roofline_from(dom, acc_kt, acc_wc, sc.n, a.workload)
+    out["roofline"]["protocol_launches"] = kt_all[0]
+    out["roofline"]["protocol_avg_launch_ms"] = kt_all[1] / max(kt_all[0], 1)
+    out["kernel_ms_in_timed_blocks_run1"] = share
     if a.ab_steps > 0:
-        out["learned_warm_start"] = learned_ab(ctx, sc, a)
+        out["learned_warm_start"] = learned_ab(ctx, sc, a, total)
     ctx.close()
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline_c2(sc, snaps, total, oracle())
+        out["cpu_baseline"]["ratio_gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     return out
 
 
-def learned_ab(ctx, sc, a):
+def learned_ab(ctx, sc, a, total):
     """Sweeps/step from the cold, previous-step and learned starts on the same
     manifolds and pre-solve velocities, over the steps that follow the timed
     protocol (the settled bed)."""
@@ -329,7 +455,7 @@ def learned_ab(ctx, sc, a):
                                         "previous-step impulse only"}
     res["weights"] = os.path.relpath(a.weights, ROOT)
     res["weights_trained_on"] = select
-    res["bed"] = f"steps {a.warmup + a.steps + 1}..{a.warmup + a.steps + a.ab_steps} of the same run"
+    res["bed"] = f"steps {total + 1}..{total + a.ab_steps} of the same run (after the protocol)"
     return res
 
 
