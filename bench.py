@@ -77,6 +77,8 @@ def parse():
     ap.add_argument("--check-every", type=int, default=8,
                     help="sweeps between residual checks on the settling workloads (4: 14.4M, 8: 15.1M, 16: 15.4M "
                          "rod-steps/s at 1e5 rods, 0.7 / 1.4 %% more sweeps)")
+    ap.add_argument("--hot-set", type=int, default=None, choices=[0, 1],
+                    help="override the workload's hot-set phase of the LDS island solve (hard-contact workloads)")
     ap.add_argument("--stall-sweeps", type=int, default=None,
                     help="override the workload's stall-rule window (hard-contact workloads)")
     ap.add_argument("--block", type=int, default=250)
@@ -287,6 +289,8 @@ def run_c2(a, L, scenes):
     p.check_every = a.check_every
     if a.stall_sweeps is not None:
         p.stall_sweeps = a.stall_sweeps
+    if a.hot_set is not None:
+        p.hot_set = a.hot_set
     total = a.protocol_steps
 
     # ---- run 1: warm-up (chunk length autotune), ranking and CPU snapshots
@@ -400,7 +404,7 @@ def run_c2(a, L, scenes):
         "timed_blocks": f"{len(blocks)} block(s) of {blocks[0][1]} step(s), centred in equal strata of steps "
                         f"{warm + 1}..{total}" if len(blocks) > 1 else f"steps {warm + 1}..{total}",
         "seed": sc.extra["seed"], "check_every": a.check_every, "chunk_len": best, "chunk_len_autotune_ms": tune,
-        "stall_sweeps": p.stall_sweeps, "stall_omega": p.stall_omega,
+        "stall_sweeps": p.stall_sweeps, "stall_omega": p.stall_omega, "hot_set": p.hot_set,
         "hipgraph": "check blocks of the huge-island solve (sweeps x colours + residual) captured once per island "
                     "and step, replayed; the step itself reads counts back and is not captured"})
     out["value_counts"] = (f"rods x timed steps that reached residual 1e-8 ({converged} of {steps}) / wall time of "

@@ -85,6 +85,14 @@ typedef struct rdk_nsc_params {
   double slop;             /* allowed penetration before correction (default 1e-4) */
   double stall_omega;      /* omega after the first stall (1 = plain Gauss-Seidel);
                               0 = stop at the first stall */
+  int hot_set;             /* stop_rule 1, islands solved in one workgroup with their
+                              bodies in LDS: 1 = once at least 64 sweeps have run and at
+                              most max(16, m/50) of the island's m manifolds are above
+                              tol, sweep only the manifolds sharing a body with one
+                              above tol/2 (at most 512, in the island's colour order)
+                              until their residual is at most tol/2, then check the
+                              whole island again (Gauss-Seidel on a subset; the stop
+                              rule is unchanged); 0 = full sweeps only */
 } rdk_nsc_params;
 
 typedef struct rdk_soft_params { /* SoftContactCfg, config.hpp:6-27 */
@@ -308,7 +316,7 @@ int rdk_num_kernels(void);
  * test, contacts, manifolds, manifold updates in sweeps, manifold residual
  * evaluations, body integrations, force evaluations, FP64 narrow-phase
  * calls, manifold updates in timed colour launches, manifold updates in
- * sweeps of the LDS island class */
+ * sweeps of the LDS island class, hot-set sweeps (summed over islands) */
 int rdk_work_counters(rdk_ctx* ctx, int64_t* counters, int n);
 
 /* ---- scene generation (resetScene populate/randomInit,

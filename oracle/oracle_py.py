@@ -70,6 +70,9 @@ def lib():
         L.orc_nsc_apply_warm.argtypes = [C.c_void_p]
         L.orc_nsc_sweeps.argtypes = [C.c_void_p, C.POINTER(NscParams), _ip, C.c_int]
         L.orc_nsc_solve.argtypes = [C.c_void_p, C.POINTER(NscParams), _ip, _dp, _ip]
+        L.orc_nsc_solve_island.restype = C.c_int
+        L.orc_nsc_solve_island.argtypes = [C.c_void_p, C.POINTER(NscParams), _ip, C.c_int, C.c_int, C.c_double,
+                                           C.c_int, _dp, C.POINTER(C.c_longlong), _ip]
         L.orc_nsc_residual.restype = C.c_double
         L.orc_nsc_residual.argtypes = [C.c_void_p, C.POINTER(NscParams)]
         L.orc_nsc_store_warm.argtypes = [C.c_void_p]
@@ -231,6 +234,17 @@ class World:
         s = C.c_int()
         self._L.orc_nsc_solve(self._h, C.byref(p), _i(order), C.byref(r), C.byref(s))
         return r.value, s.value
+
+    def solve_island(self, p, ix, stall_sweeps=0, stall_omega=0.0, hot=0):
+        """One island (manifold indices ix in solve order) solved as the
+        library's island kernels do: (sweeps, residual, updates, hot sweeps)."""
+        ix = np.ascontiguousarray(ix, np.int32)
+        r = C.c_double()
+        u = C.c_longlong()
+        hs = C.c_int()
+        sw = self._L.orc_nsc_solve_island(self._h, C.byref(p), _i(ix), len(ix), int(stall_sweeps),
+                                          float(stall_omega), int(hot), C.byref(r), C.byref(u), C.byref(hs))
+        return sw, r.value, u.value, hs.value
 
     def residual(self, p):
         return self._L.orc_nsc_residual(self._h, C.byref(p))

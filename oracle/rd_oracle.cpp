@@ -13,6 +13,7 @@
 #include <cstdint>
 #include <cstring>
 #include <unordered_map>
+#include <unordered_set>
 #include <utility>
 #include <vector>
 #ifdef _OPENMP
@@ -1292,6 +1293,95 @@ int orc_nsc_set_manifolds(orc_world* W, int M, const double* rec, const int* ab,
     m.lt2 = lambda ? lambda[3 * i + 2] : 0.0;
   }
   return M;
+}
+
+// Island solve with the stall rule and the hot-set phase, in the order the
+// library's island kernels run them (rdk_nsc.hip: StallTrack, k_island_lds).
+// Hot set: once at least 64 sweeps have run and 0 < (manifolds above tol) <=
+// max(16, m / 50), sweep only the manifolds sharing a body with one whose
+// residual exceeds tol / 2 (in solve order; skipped when more than 512, the
+// next attempt then waiting until the sweep count has doubled), in blocks of
+// check_every sweeps until their residual is <= tol / 2, then check the
+// island again.
+int orc_nsc_solve_island(orc_world* W, const orc_nsc_params* P, const int* ix, int m, int stall_sweeps,
+                         double stall_omega, int hot, double* res_out, long long* updates_out,
+                         int* hot_sweeps_out) {
+  orc_nsc_params Q = *P;
+  const double hot_thr = 0.5 * P->tol;
+  const int kk = std::max(1, P->check_every);
+  std::unordered_set<int> flagged;
+  int above = 0;
+  auto check = [&]() {
+    double r = 0.0;
+    above = 0;
+    flagged.clear();
+    for (int k = 0; k < m; ++k) {
+      const Manifold& mf = W->mf[ix[k]];
+      const double ri = residual_one(*W, mf, Q);
+      r = std::max(r, ri);
+      if (hot) {
+        if (ri > P->tol) ++above;
+        if (ri > hot_thr) {
+          flagged.insert(mf.a);
+          if (mf.b >= 0) flagged.insert(mf.b);
+        }
+      }
+    }
+    return r;
+  };
+  int sweeps = 0, hot_next = 64, hot_sw = 0;
+  long long updates = 0;
+  int next = stall_sweeps;  // stall milestones stall, 2 stall, 4 stall, ...
+  double ref = -1.0;
+  double res = check();
+  std::vector<int> H;
+  while (sweeps < P->max_sweeps && !(res <= P->tol)) {
+    H.clear();
+    if (hot && sweeps >= hot_next && above > 0 && above <= std::max(16, m / 50)) {
+      for (int k = 0; k < m; ++k) {
+        const Manifold& mf = W->mf[ix[k]];
+        if (flagged.count(mf.a) || (mf.b >= 0 && flagged.count(mf.b))) H.push_back(ix[k]);
+      }
+      if (H.size() > 512 || H.empty()) {
+        H.clear();
+        hot_next = 2 * sweeps;
+      }
+    }
+    if (!H.empty()) {
+      const int nh = (int)H.size();
+      int ls = 0;
+      while (true) {
+        for (int it = 0; it < kk; ++it)
+          for (int k = 0; k < nh; ++k) psor_one(*W, W->mf[H[k]], Q);
+        ls += kk;
+        double rh = 0.0;
+        for (int k = 0; k < nh; ++k) rh = std::max(rh, residual_one(*W, W->mf[H[k]], Q));
+        if (rh <= hot_thr || sweeps + ls >= P->max_sweeps) break;
+      }
+      sweeps += ls;
+      updates += (long long)ls * nh;
+      hot_sw += ls;
+    } else {
+      for (int it = 0; it < kk; ++it)
+        for (int k = 0; k < m; ++k) psor_one(*W, W->mf[ix[k]], Q);
+      sweeps += kk;
+      updates += (long long)kk * m;
+    }
+    res = check();
+    if (!(res <= P->tol) && next > 0 && sweeps >= next) {
+      const bool st = ref >= 0.0 && !(res <= 0.9 * ref);
+      ref = res;
+      while (next <= sweeps) next *= 2;
+      if (st) {
+        if (stall_omega > 0.0 && Q.omega > stall_omega) Q.omega = stall_omega;
+        else break;
+      }
+    }
+  }
+  if (res_out) *res_out = res;
+  if (updates_out) *updates_out = updates;
+  if (hot_sweeps_out) *hot_sweeps_out = hot_sw;
+  return sweeps;
 }
 
 void orc_nsc_apply_warm(orc_world* W) { apply_warm(*W); }

@@ -186,7 +186,7 @@ int rdk_destroy(rdk_ctx* ctx) {
   release(ctx->iflag); release(ctx->ipos); release(ctx->istart); release(ctx->ilists); release(ctx->bseg);
   release(ctx->icnt); release(ctx->blab); release(ctx->blab_alt); release(ctx->istats); release(ctx->hcs);
   for (int k = 0; k < 3; ++k) { release(ctx->ent_a[k]); release(ctx->ent_e[k]); }
-  release(ctx->ent_out); release(ctx->ent_psum); release(ctx->ent_pcnt); release(ctx->hres); release(ctx->hstate); release(ctx->raw_ptrs); release(ctx->raw_out); release(ctx->kept_mf); release(ctx->kept_ab); release(ctx->kept_lam); release(ctx->soft_kept); release(ctx->islot); release(ctx->prev_keys); release(ctx->prev_wkeys); release(ctx->resid);
+  release(ctx->ent_out); release(ctx->ent_psum); release(ctx->ent_pcnt); release(ctx->hres); release(ctx->hstate); release(ctx->raw_ptrs); release(ctx->raw_out); release(ctx->kept_mf); release(ctx->kept_ab); release(ctx->kept_lam); release(ctx->soft_kept); release(ctx->islot); release(ctx->ihot); release(ctx->prev_keys); release(ctx->prev_wkeys); release(ctx->resid);
   release(ctx->hkeys); release(ctx->hkeys_new); release(ctx->hlast); release(ctx->hlast_new);
   release(ctx->hflag); release(ctx->hscan); release(ctx->pe); release(ctx->temp); release(ctx->red);
   if (ctx->h_counters) (void)hipHostFree(ctx->h_counters);
@@ -606,7 +606,7 @@ int rdk_history_size(rdk_ctx* ctx, int* n) {
 
 static int check_nsc(rdk_ctx* ctx, const rdk_nsc_params* p) {
   if (!p || !(p->omega > 0.0) || p->max_sweeps < 0 || p->friction_proj < 0 || p->friction_proj > 1 ||
-      p->warm_mode < 0 || p->warm_mode > 3) {
+      p->warm_mode < 0 || p->warm_mode > 3 || p->hot_set < 0 || p->hot_set > 1) {
     ctx->err = "invalid rdk_nsc_params";
     return RDK_ERR_ARG;
   }

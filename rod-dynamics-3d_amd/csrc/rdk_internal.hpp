@@ -65,6 +65,7 @@ enum WorkId {
   W_NARROW,           // FP64 narrow-phase calls (pairs passing the separating-axis test)
   W_PSOR_TIMED,       // manifold updates in k_psor_sweep launches whose time was recorded
   W_LDS_SWEEPS,       // manifold updates in sweeps of the LDS island class
+  W_HOT_SWEEPS,       // sweeps of hot sets in the LDS island class, summed over islands
   W_NUM
 };
 
@@ -192,6 +193,7 @@ struct rdk_ctx {
   rdk::DevBuf<const double*> raw_ptrs;  // contact component arrays for the FP32 record export
   rdk::DevBuf<char> raw_out;            // FP32 contact records
   rdk::DevBuf<int2> islot;  // local body slots of large-island manifolds
+  rdk::DevBuf<int> ihot;    // hot sets of large islands (in each island's manifold range)
   // position projection
   rdk::DevBuf<int> ppa, ppb, pflag, ppos;
   rdk::DevBuf<double> pphs, ptmp, pdx[3], pdth[3];

@@ -1014,12 +1014,13 @@ struct IslandStats {
   unsigned long long manifold_sweeps;
   double res, res_ref;
   unsigned long long lds_manifold_sweeps;  // the LDS class's share of manifold_sweeps
+  unsigned long long hot_sweeps;          // sweeps of hot sets (LDS class), summed over islands
 };
 
-__device__ __forceinline__ void island_report(IslandStats* S, int sweeps, int size, double res, double res_ref,
-                                              bool nonconv) {
+__device__ __forceinline__ void island_report(IslandStats* S, int sweeps, unsigned long long updates, double res,
+                                              double res_ref, bool nonconv) {
   atomicMax(&S->max_sweeps, sweeps);
-  atomicAdd(&S->manifold_sweeps, (unsigned long long)sweeps * (unsigned long long)size);
+  atomicAdd(&S->manifold_sweeps, updates);
   if (res > 0.0 || res != res) atomic_max_pos(&S->res, res != res ? __longlong_as_double(0x7ff8000000000000ll) : res);
   if (res_ref > 0.0) atomic_max_pos(&S->res_ref, res_ref);
   if (nonconv) atomicAdd(&S->nonconv, 1);
@@ -1029,6 +1030,7 @@ struct IslandSolve {
   SolveConst K;
   double tol;
   int max_sweeps, check_every, stop_rule, warm, stall;
+  int hot;  // hot-set phase of the LDS class (rdk_nsc_params::hot_set)
 };
 
 
@@ -1191,7 +1193,9 @@ __global__ void __launch_bounds__(WG) k_island_reg(BodyPtrs B, MfPtrs M, const S
       B.w[c][g] = w_l[c][slot * BCAP + q];
     }
   }
-  if (has && j == 0) island_report(S, sweeps, n, res, res_ref, P.stop_rule == 1 && !(res <= P.tol));
+  if (has && j == 0)
+    island_report(S, sweeps, (unsigned long long)sweeps * (unsigned long long)n, res, res_ref,
+                  P.stop_rule == 1 && !(res <= P.tol));
 }
 
 // island classes: manifolds and bodies per island for the register kernels
@@ -1207,6 +1211,25 @@ constexpr int LDS_THREADS = 512;
 // 161 280 B of the 160 KiB LDS, enough for the percolating cluster of the
 // settling bed at 1e5 rods (about 3200 bodies)
 constexpr int LDS_BCAP = 3360;
+
+// Hot-set phase of the LDS island solve (IslandSolve::hot, stop rule 1).
+// Once at least HOT_MIN_SWEEPS sweeps have run and at most
+// max(HOT_MIN_COUNT, m / HOT_FRAC) of the island's m manifolds are above the
+// tolerance, the solve sweeps only the hot set H: the manifolds that share a
+// body with one whose residual exceeds HOT_TARGET * tol, in the island's
+// solve order (colour by colour), at most LDS_THREADS of them, one per lane
+// with its fields in registers, until the residual over H is at most
+// HOT_TARGET * tol.  Then the whole island is checked again, and the phase
+// repeats while the condition holds.  This is Gauss-Seidel on a subset of the
+// manifolds (the others' impulses stay as they are) and the stop rule is
+// still the residual of the whole island.  On the settling bed the last
+// decade or two of the residual sits in a few wall contacts at the stick/slip
+// boundary whose local mode takes thousands of sweeps; a full sweep updates
+// every manifold of the island for each of them.  A hot set larger than
+// LDS_THREADS is not used; the next attempt waits until the sweep count has
+// doubled.
+constexpr int HOT_MIN_SWEEPS = 64, HOT_MIN_COUNT = 16, HOT_FRAC = 50;
+constexpr double HOT_TARGET = 0.5;
 
 struct MfSlots {
   MfPtrs M;
@@ -1231,10 +1254,16 @@ __global__ void __launch_bounds__(LDS_THREADS) k_island_lds(BodyPtrs B, MfPtrs M
                                                            const uint32_t* __restrict__ skey,
                                                            const int* __restrict__ bseg, const int* __restrict__ sbody,
                                                            const int* __restrict__ list, IslandSolve P,
-                                                           IslandStats* S, int2* __restrict__ slot) {
+                                                           IslandStats* S, int2* __restrict__ slot,
+                                                           int* __restrict__ hlist) {
+  constexpr int HOT_WORDS = (LDS_BCAP + 31) / 32;
+  constexpr int NWAVE = LDS_THREADS / 64;
   __shared__ double v_l[3][LDS_BCAP], w_l[3][LDS_BCAP];
-  __shared__ double red[LDS_THREADS / 64];
+  __shared__ double red[NWAVE];
   __shared__ int cs[129];
+  __shared__ int hcs[129];                // colour starts inside the hot set
+  __shared__ unsigned hot_bits[HOT_WORDS];  // bodies (LDS slots) of manifolds above HOT_TARGET * tol
+  __shared__ int s_above, s_wcnt[NWAVE];
   const int k = list[blockIdx.x];
   const int s = istart[k], e = istart[k + 1];
   const int bs = bseg[2 * k], nb = bseg[2 * k + 1] - bs;
@@ -1250,6 +1279,8 @@ __global__ void __launch_bounds__(LDS_THREADS) k_island_lds(BodyPtrs B, MfPtrs M
   const uint32_t base = skey[s] & ~127u;
   const int ncol = (int)(skey[e - 1] & 127u) + 1;
   for (int c = tid; c <= ncol; c += LDS_THREADS) cs[c] = lb_u32(skey, s, e, base | (uint32_t)c);
+  for (int w = tid; w < HOT_WORDS; w += LDS_THREADS) hot_bits[w] = 0u;
+  if (tid == 0) s_above = 0;
   __syncthreads();
   for (int i = s + tid; i < e; i += LDS_THREADS) {
     const int gb = M.b[i];
@@ -1263,62 +1294,165 @@ __global__ void __launch_bounds__(LDS_THREADS) k_island_lds(BodyPtrs B, MfPtrs M
   }
   MfSlots A{M, slot};
   const double mu = sys[B.sys[M.a[s]]].mu;
+  const int m = e - s;
   if (P.warm) {
     for (int c = 0; c < ncol; ++c) {
       for (int i = cs[c] + tid; i < cs[c + 1]; i += LDS_THREADS) warm_apply_one(LB, A, i);
       __syncthreads();
     }
   }
-  int sweeps = 0;
+  int sweeps = 0, hot_next = HOT_MIN_SWEEPS;
+  unsigned long long updates = 0ull, hot_sweeps = 0ull;
   double res = 0.0, res_ref = 0.0;
   StallTrack trk;
   trk.init(P.stall);
   SolveConst K = P.K;  // omega may drop to stall_omega on a stall
-  if (P.stop_rule == 1) {
+  const bool hot_on = P.hot != 0 && P.stop_rule == 1;
+  const double hot_thr = HOT_TARGET * P.tol;
+  // Residual of the whole island; with the hot-set phase on, also counts the
+  // manifolds above tol (s_above) and marks the bodies of those above
+  // HOT_TARGET * tol.  Ends with a barrier (block_max).
+  auto check = [&]() -> double {
     double r = 0.0;
-    for (int i = s + tid; i < e; i += LDS_THREADS) r = dmax(r, residual_one(LB, A, i, K, mu));
-    res = block_max(r, red);
-  }
-  const int kk = P.stop_rule == 1 ? P.check_every : 1;
-  while (sweeps < P.max_sweeps && !(P.stop_rule == 1 && res <= P.tol)) {
-    double d = 0.0;
-    for (int it = 0; it < kk; ++it) {
-      d = 0.0;
-      // Each lane's first manifold of the next colour is loaded into registers
-      // before the barrier that ends the current colour (its fields and
-      // impulses are not written by other colours; the body velocities it
-      // needs are read from LDS after the barrier), so the global-memory
-      // latency overlaps the wait.
-      MfReg R;
-      if (cs[0] + tid < cs[1]) load_manifold(R, A, cs[0] + tid);
-      for (int c = 0; c < ncol; ++c) {
-        int i = cs[c] + tid;
-        if (i < cs[c + 1]) {
-          d = dmax(d, psor_update(LB, R, 0, K, mu));
-          for (int q = 0; q < 3; ++q) A.lr(q, i) = R.lam[q];
-          i += LDS_THREADS;
+    int above = 0;
+    for (int i = s + tid; i < e; i += LDS_THREADS) {
+      const double ri = residual_one(LB, A, i, K, mu);
+      r = dmax(r, ri);
+      if (hot_on) {
+        above += ri > P.tol ? 1 : 0;
+        if (ri > hot_thr) {
+          const int2 sl = slot[i];
+          atomicOr(&hot_bits[sl.x >> 5], 1u << (sl.x & 31));
+          if (sl.y >= 0) atomicOr(&hot_bits[sl.y >> 5], 1u << (sl.y & 31));
         }
-        // further manifolds of this lane reuse the same registers: all fields
-        // in one round of loads, then the update
-        for (; i < cs[c + 1]; i += LDS_THREADS) {
-          load_manifold(R, A, i);
-          d = dmax(d, psor_update(LB, R, 0, K, mu));
-          for (int q = 0; q < 3; ++q) A.lr(q, i) = R.lam[q];
-        }
-        if (c + 1 < ncol && cs[c + 1] + tid < cs[c + 2]) load_manifold(R, A, cs[c + 1] + tid);
-        __syncthreads();
       }
     }
-    sweeps += kk;
-    res_ref = block_max(d, red);
-    double r = 0.0;
-    for (int i = s + tid; i < e; i += LDS_THREADS) r = dmax(r, residual_one(LB, A, i, K, mu));
-    res = block_max(r, red);
-    if (P.stop_rule == 0 && res_ref < 1e-6) break;
-    if (P.stop_rule == 1 && !(res <= P.tol) && trk.stalled(sweeps, res)) {
+    if (above) atomicAdd(&s_above, above);
+    return block_max(r, red);
+  };
+  if (P.stop_rule == 1) res = check();
+  const int kk = P.stop_rule == 1 ? P.check_every : 1;
+  while (sweeps < P.max_sweeps && !(P.stop_rule == 1 && res <= P.tol)) {
+    // ---- hot set (block-uniform decision from the last check)
+    int nh = 0;
+    if (hot_on) {
+      const int above = s_above;
+      const bool want = sweeps >= hot_next && above > 0 && above <= max(HOT_MIN_COUNT, m / HOT_FRAC);
+      __syncthreads();  // everyone has read s_above
+      if (want) {
+        // H = manifolds with a marked body, in solve (storage) order
+        const int lane = tid & 63, wid = tid >> 6;
+        for (int c0 = s; c0 < e; c0 += LDS_THREADS) {
+          const int i = c0 + tid;
+          bool in = false;
+          if (i < e) {
+            const int2 sl = slot[i];
+            in = ((hot_bits[sl.x >> 5] >> (sl.x & 31)) & 1u) != 0u ||
+                 (sl.y >= 0 && ((hot_bits[sl.y >> 5] >> (sl.y & 31)) & 1u) != 0u);
+          }
+          const unsigned long long bal = __ballot(in);
+          if (lane == 0) s_wcnt[wid] = __popcll(bal);
+          __syncthreads();
+          int woff = 0, tot = 0;
+          for (int w = 0; w < NWAVE; ++w) {
+            woff += w < wid ? s_wcnt[w] : 0;
+            tot += s_wcnt[w];
+          }
+          const int pos = nh + woff + __popcll(bal & ((1ull << lane) - 1ull));
+          if (in && pos < LDS_THREADS) hlist[s + pos] = i;
+          nh += tot;
+          __syncthreads();  // s_wcnt is reused
+        }
+        if (nh > LDS_THREADS || nh == 0) {
+          nh = 0;
+          hot_next = 2 * sweeps;
+        } else {
+          // colour bounds of H as positions 0..nh, empty colours dropped
+          for (int c = tid; c <= ncol; c += LDS_THREADS) hcs[c] = lb_int(hlist + s, nh, cs[c]);
+          __syncthreads();
+          if (tid == 0) {
+            int j = 0;
+            for (int c = 0; c < ncol; ++c)
+              if (hcs[c] < hcs[c + 1]) hcs[j++] = hcs[c];
+            hcs[j] = nh;
+            s_wcnt[0] = j;
+          }
+        }
+      }
+      for (int w = tid; w < HOT_WORDS; w += LDS_THREADS) hot_bits[w] = 0u;
+      if (tid == 0) s_above = 0;
+      __syncthreads();
+    }
+    // ---- sweeps: the whole island (positions = manifold indices s..e) or the
+    // hot set (positions 0..nh, manifold hlist[s + position])
+    const bool hot = nh > 0;
+    const int* bnd = hot ? hcs : cs;
+    const int nbnd = hot ? s_wcnt[0] : ncol;
+    const int* hl = hlist + s;
+    int ls = 0;
+    double d = 0.0;
+    // Full sweeps: lane tid takes positions bnd[c] + tid, bnd[c] + tid + 512, ...
+    // of each colour.  Hot set (at most 512 positions): lane tid keeps
+    // position tid, whose manifold stays in its registers for the round.
+    MfReg R;
+    if (hot && tid < nh) load_manifold(R, A, hl[tid]);
+    while (true) {
+      for (int it = 0; it < kk; ++it) {
+        d = 0.0;
+        // Each lane's first manifold of the next colour is loaded into registers
+        // before the barrier that ends the current colour (its fields and
+        // impulses are not written by other colours; the body velocities it
+        // needs are read from LDS after the barrier), so the global-memory
+        // latency overlaps the wait.
+        if (!hot && bnd[0] + tid < bnd[1]) load_manifold(R, A, bnd[0] + tid);
+        for (int c = 0; c < nbnd; ++c) {
+          int q = hot ? tid : bnd[c] + tid;
+          if (hot ? (tid >= bnd[c] && tid < bnd[c + 1]) : q < bnd[c + 1]) {
+            d = dmax(d, psor_update(LB, R, 0, K, mu));
+            const int i = hot ? hl[q] : q;
+            for (int t = 0; t < 3; ++t) A.lr(t, i) = R.lam[t];
+            q += LDS_THREADS;
+          }
+          // further manifolds of this lane reuse the same registers: all fields
+          // in one round of loads, then the update
+          if (!hot) {
+            for (; q < bnd[c + 1]; q += LDS_THREADS) {
+              load_manifold(R, A, q);
+              d = dmax(d, psor_update(LB, R, 0, K, mu));
+              for (int t = 0; t < 3; ++t) A.lr(t, q) = R.lam[t];
+            }
+            if (c + 1 < nbnd && bnd[c + 1] + tid < bnd[c + 2]) load_manifold(R, A, bnd[c + 1] + tid);
+          }
+          __syncthreads();
+        }
+      }
+      ls += kk;
+      if (!hot) break;
+      // the hot set's residual from the manifolds in registers (one per lane)
+      if (block_max(tid < nh ? residual_one(LB, R, 0, K, mu) : 0.0, red) <= hot_thr || sweeps + ls >= P.max_sweeps)
+        break;
+    }
+    sweeps += ls;
+    updates += (unsigned long long)ls * (unsigned long long)(hot ? nh : m);
+    if (hot) {
+      hot_sweeps += (unsigned long long)ls;
+    } else {
+      res_ref = block_max(d, red);
+      if (P.stop_rule == 0) {
+        if (res_ref < 1e-6) break;
+        continue;
+      }
+    }
+    res = check();
+    if (!(res <= P.tol) && trk.stalled(sweeps, res)) {
       if (K.stall_omega > 0.0 && K.omega > K.stall_omega) K.omega = K.stall_omega;
       else break;
     }
+  }
+  if (P.stop_rule == 0) {  // the reference rule reports the residual it ends with
+    double r = 0.0;
+    for (int i = s + tid; i < e; i += LDS_THREADS) r = dmax(r, residual_one(LB, A, i, K, mu));
+    res = block_max(r, red);
   }
   for (int q = tid; q < nb; q += LDS_THREADS) {
     const int g = body_l[q];
@@ -1328,8 +1462,9 @@ __global__ void __launch_bounds__(LDS_THREADS) k_island_lds(BodyPtrs B, MfPtrs M
     }
   }
   if (tid == 0) {
-    island_report(S, sweeps, e - s, res, res_ref, P.stop_rule == 1 && !(res <= P.tol));
-    atomicAdd(&S->lds_manifold_sweeps, (unsigned long long)sweeps * (unsigned long long)(e - s));
+    island_report(S, sweeps, updates, res, res_ref, P.stop_rule == 1 && !(res <= P.tol));
+    atomicAdd(&S->lds_manifold_sweeps, updates);
+    if (hot_sweeps) atomicAdd(&S->hot_sweeps, hot_sweeps);
   }
 }
 
@@ -2012,6 +2147,7 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
   P.stop_rule = p->stop_rule;
   P.warm = p->warm_mode != 0 ? 1 : 0;
   P.stall = p->stall_sweeps;
+  P.hot = p->hot_set;
   IslandStats* S = (IslandStats*)ctx->istats.p;
   RDK_HIP(hipMemsetAsync(S, 0, sizeof(IslandStats), ctx->stream));
   BodyPtrs B = body_ptrs(ctx);
@@ -2050,9 +2186,11 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
   }
   if (cnt[3] > 0) {
     RDK_TRY(ensure(ctx, ctx->islot, nm));
+    RDK_TRY(ensure(ctx, ctx->ihot, nm));
     ProfScope ps(ctx, K_ISL_LARGE);
     hipLaunchKernelGGL(k_island_lds, dim3(cnt[3]), dim3(LDS_THREADS), 0, ctx->stream, B, Mc, ctx->sys_dev.p,
-                       ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p + 3 * nisl, P, S, ctx->islot.p);
+                       ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p + 3 * nisl, P, S, ctx->islot.p,
+                       ctx->ihot.p);
   }
   HugeOut hu;
   if (cnt[4] > 0) RDK_TRY(huge_islands_solve(ctx, P, skey, ctx->ilists.p + 4 * nisl, cnt[4], s_side, hu));
@@ -2085,6 +2223,7 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
   ctx->island_counts[3] = cnt[3] + cnt[4];
   ctx->work[W_SWEEP_CONTACTS] += (int64_t)hs.manifold_sweeps + hu.manifold_sweeps;
   ctx->work[W_LDS_SWEEPS] += (int64_t)hs.lds_manifold_sweeps;
+  ctx->work[W_HOT_SWEEPS] += (int64_t)hs.hot_sweeps;
   *status = hs.nonconv + hu.nonconv > 0 ? RDK_ERR_NOCONV : RDK_OK;
   if (st) {
     st->sweeps = std::max(hs.max_sweeps, hu.max_sweeps);

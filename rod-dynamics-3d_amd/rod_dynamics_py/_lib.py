@@ -40,7 +40,8 @@ class NscParams(C.Structure):
                 ("max_sweeps", C.c_int), ("check_every", C.c_int), ("friction_proj", C.c_int),
                 ("position_stabilization", C.c_int), ("warm_mode", C.c_int), ("stop_rule", C.c_int),
                 ("warm_walls", C.c_int), ("stall_sweeps", C.c_int), ("position_iters", C.c_int),
-                ("position_psor_iters", C.c_int), ("slop", C.c_double), ("stall_omega", C.c_double)]
+                ("position_psor_iters", C.c_int), ("slop", C.c_double), ("stall_omega", C.c_double),
+                ("hot_set", C.c_int)]
 
 
 class SoftParams(C.Structure):
@@ -201,8 +202,9 @@ def system_desc(lo, hi, periodic, floor_wall=False, ceil_wall=False, mu=0.0):
 
 def nsc_params(beta=0.0, cfm=0.0, omega=1.0, tol=1e-8, max_sweeps=10000, check_every=1, friction_proj=0,
                position_stabilization=0, warm_mode=1, stop_rule=1, warm_walls=0, stall_sweeps=0, position_iters=5,
-               position_psor_iters=50, slop=1e-4, stall_omega=1.0):
+               position_psor_iters=50, slop=1e-4, stall_omega=1.0, hot_set=0):
     p = NscParams()
+    p.hot_set = hot_set
     p.position_iters, p.position_psor_iters, p.slop = position_iters, position_psor_iters, slop
     p.stall_omega = stall_omega
     p.warm_walls = warm_walls
@@ -552,7 +554,7 @@ class Context:
         self._check(self._L.rdk_work_counters(self._h, arr, 16))
         names = ["steps", "body_passes", "chunks", "candidates", "contacts", "manifolds", "sweep_contacts",
                  "resid_contacts", "integrate_bodies", "force_contacts", "narrow_calls", "psor_timed_updates",
-                 "lds_manifold_sweeps"]
+                 "lds_manifold_sweeps", "hot_sweeps"]
         return {nm: arr[i] for i, nm in enumerate(names)}
 
     def affine_scale(self, s):

@@ -1,0 +1,97 @@
+"""Writes settled_island.npz: the largest contact island of the 1e5-rod
+settling workload (hard contact, L/d 20, mu 0.5, omega 1.3) at one step of
+the protocol, reduced to that island's rods.  Runs on a GPU (the settling
+protocol up to the step, about 20 s on one MI355X).
+
+Contents (island rods renumbered 0..nb-1 in ascending original index):
+  x, q, v, w      state at the end of step `step` - 1 (before the free-flight
+                  update of step `step`)
+  v_free, w_free  velocities after that free-flight update (the solve's start)
+  r, h, mass      rod constants; box lo/hi, periodic x,y, floor and ceiling
+  ab, rec, lam0   the step's manifolds of the island as the library built
+                  them (natural order; b < 0 for walls), with the
+                  previous-step impulses they start from
+  gpu_sweeps, gpu_residual, gpu_nonconverged
+                  what the library's full-sweep solve (stall window 1000,
+                  stop at the first stall) did on the whole bed at this step
+
+Usage: python tests/golden/make_settled_island.py [--step 4626] [--out PATH]
+"""
+import argparse
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, os.path.join(ROOT, "rod-dynamics-3d_amd"))
+
+
+def islands(ab, n):
+    par = np.arange(n)
+
+    def find(i):
+        while par[i] != i:
+            par[i] = par[par[i]]
+            i = par[i]
+        return i
+
+    for a, b in ab:
+        if b >= 0:
+            ra, rb = find(a), find(b)
+            if ra != rb:
+                par[max(ra, rb)] = min(ra, rb)
+    return np.array([find(a) for a in ab[:, 0]])
+
+
+def main():
+    from rod_dynamics_py import _lib as L
+    from rod_dynamics_py import scenes
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--step", type=int, default=4626)
+    ap.add_argument("--out", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "settled_island.npz"))
+    a = ap.parse_args()
+    sc = scenes.c2_settling(n=100_000)
+    p = sc.params
+    p.check_every, p.hot_set, p.stall_sweeps, p.stall_omega = 8, 0, 1000, 0.0
+    det = L.detect_params(0.0, 16.0, 1.0)
+    ctx = L.Context(0)
+    sc.load(ctx)
+    done = 0
+    while done < a.step - 1:
+        k = min(500, a.step - 1 - done)
+        ctx.step_nsc(det, p, k)
+        done += k
+        print(f"step {done}", flush=True)
+    s_prev = ctx.download_state()
+    ctx.integrate(L.RDK_NSC_FREE)
+    s_free = ctx.download_state()
+    ctx.detect(det)
+    nm = ctx.nsc_build(p)
+    rec, ab, lam0 = ctx.nsc_manifolds(nm)
+    st = ctx.nsc_solve(p, allow_noconv=True)
+    ctx.close()
+    lab = islands(ab, sc.n)
+    u, cnt = np.unique(lab, return_counts=True)
+    sel = np.where(lab == u[np.argmax(cnt)])[0]
+    bodies = np.unique(np.concatenate([ab[sel, 0], ab[sel, 1][ab[sel, 1] >= 0]]))
+    remap = -np.ones(sc.n, np.int64)
+    remap[bodies] = np.arange(len(bodies))
+    ab_l = ab[sel].copy()
+    ab_l[:, 0] = remap[ab_l[:, 0]]
+    rod = ab_l[:, 1] >= 0
+    ab_l[rod, 1] = remap[ab_l[rod, 1]]
+    d = sc.descs[0]
+    np.savez_compressed(a.out, step=a.step, x=s_prev["x"][bodies], q=s_prev["q"][bodies], v=s_prev["v"][bodies],
+                        w=s_prev["w"][bodies], v_free=s_free["v"][bodies], w_free=s_free["w"][bodies], r=sc.r[bodies], h=sc.h[bodies], mass=sc.mass[bodies],
+                        lo=np.array(d.lo[:]), hi=np.array(d.hi[:]), ab=ab_l.astype(np.int32), rec=rec[sel],
+                        lam0=lam0[sel], gpu_sweeps=st["sweeps"], gpu_residual=st["residual"],
+                        gpu_nonconverged=int(not st["residual"] <= p.tol), mu=sc.extra["mu"],
+                        gravity=np.array(sc.phys.gravity[:]), dt=sc.phys.dt)
+    print(f"island of {len(sel)} manifolds, {len(bodies)} rods at step {a.step}; whole-bed solve: {st['sweeps']} "
+          f"sweeps, residual {st['residual']:.3e} -> {a.out}", flush=True)
+
+
+if __name__ == "__main__":
+    main()

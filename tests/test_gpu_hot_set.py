@@ -1,0 +1,62 @@
+"""LDS island solve to the tolerance, with and without its hot-set phase,
+equal bit for bit to the oracle's restatement (orc_nsc_solve_island: stall
+rule, hot sets), on the settled-bed island of tests/golden/settled_island.npz
+(2546 manifolds, 1838 rods: one island of the LDS class), solved from a
+cold start."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+from rod_dynamics_py import _lib as L
+
+pytestmark = pytest.mark.gpu
+
+FIX = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "settled_island.npz")
+
+
+@pytest.mark.parametrize("hot", [0, 1])
+def test_lds_island_solve_bitwise_equal_oracle(gpu_ctx_factory, hot):
+    print(f"settled island solve hot_set={hot}", flush=True)
+    d = np.load(FIX)
+    n = len(d["r"])
+    mu = float(d["mu"])
+    ctx = gpu_ctx_factory()
+    ctx.set_bodies(d["x"], d["q"], d["v_free"], d["w_free"], d["r"], d["h"], d["mass"], np.zeros(n))
+    ctx.set_systems([0, n], [L.system_desc(d["lo"], d["hi"], (1, 1, 0), True, True, mu)])
+    ctx.set_physics(L.physics(tuple(d["gravity"]), float(d["dt"]), 0.08, 0.12, 80.0))
+    p = L.nsc_params(beta=0.0, cfm=0.0, omega=1.3, tol=1e-8, max_sweeps=10000, check_every=8, friction_proj=1,
+                     warm_mode=0, stop_rule=1, stall_sweeps=1000, stall_omega=0.0, hot_set=hot)
+    ctx.detect(L.detect_params(0.0, 16.0, 1.0))
+    nm = ctx.nsc_build(p)
+    assert nm == len(d["ab"])
+    rec, ab, lam0 = ctx.nsc_manifolds(nm)
+    assert not lam0.any()
+    s0 = ctx.download_state()
+    hot0 = ctx.work_counters()["hot_sweeps"]
+    st = ctx.nsc_solve(p, allow_noconv=True)
+    nisl, counts = ctx.island_info()
+    assert nisl == 1 and counts == [0, 0, 0, 1], counts  # one island, LDS class
+    order = ctx.solve_order(nm)
+    _, _, lam_g = ctx.nsc_manifolds(nm)
+    s1 = ctx.download_state()
+    hot_g = ctx.work_counters()["hot_sweeps"] - hot0
+
+    W = O.World(d["x"], d["q"], s0["v"], s0["w"], d["r"], d["h"], d["mass"])
+    W.set_box(d["lo"], d["hi"], [1, 1, 0], 1, 1)
+    W.set_physics(tuple(d["gravity"]), float(d["dt"]))
+    W.set_manifolds(rec, ab, lam0)
+    op = O.nsc_params(mu=mu, beta=0.0, cfm=0.0, omega=1.3, tol=1e-8, max_sweeps=10000, check_every=8, friction_proj=1,
+                      stop_rule=1, warm_start=0)
+    sw, res, upd, hs = W.solve_island(op, order, stall_sweeps=1000, stall_omega=0.0, hot=hot)
+    print(f"  GPU {st['sweeps']} sweeps (hot {hot_g}), residual {st['residual']:.3e}; oracle {sw} (hot {hs}), "
+          f"{res:.3e}, {upd} updates", flush=True)
+    assert st["sweeps"] == sw and hot_g == hs
+    assert st["residual"] == res
+    _, _, lam_o = W.manifolds()
+    so = W.state()
+    assert np.array_equal(lam_g, lam_o)
+    assert np.array_equal(s1["v"], so["v"]) and np.array_equal(s1["w"], so["w"])
+    if hot:
+        assert hs > 0 and res <= 1e-8  # the phase ran and the island converged
