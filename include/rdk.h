@@ -298,6 +298,21 @@ int rdk_observables(rdk_ctx* ctx, double* kinetic_energy,
  * sums[s] / pairs[s] is the reference's normalised entanglement (ebar). */
 int rdk_entanglement(rdk_ctx* ctx, double cutoff, double* sums, int64_t* pairs);
 
+/* Allocation-free variant of rdk_entanglement.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops).  Same sums and counts as
+ * rdk_entanglement, from the same kernels, written asynchronously on the
+ * context's stream into caller-owned device memory: d_sums[s] and d_pairs[s]
+ * for each system s (nsys entries each); the caller synchronises (rdk_sync).
+ * The call allocates and frees nothing: its scratch is one static device
+ * array of RDK_NOALLOC_SCRATCH_BYTES per GPU, shared by every context on that
+ * GPU and held until the process ends, so calls that overlap on one GPU
+ * (contexts on different streams) race on it.  RDK_ERR_CAPACITY when the
+ * context holds more than RDK_NOALLOC_MAX_RODS rods. */
+#define RDK_NOALLOC_MAX_RODS 32768
+#define RDK_NOALLOC_SCRATCH_BYTES (4u << 20)
+int rdk_entanglement_noalloc(rdk_ctx* ctx, double cutoff, double* d_sums, uint64_t* d_pairs);
+
 /* ---- profiling: per-kernel HIP-event timing on the stream each kernel runs on */
 int rdk_set_profiling(rdk_ctx* ctx, int on);
 /* time only the kernels whose id bit is set (bit k = kernel id k).  The

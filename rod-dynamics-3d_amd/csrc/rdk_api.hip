@@ -151,6 +151,11 @@ int rdk_create(int device, rdk_ctx** out) {
     delete ctx;
     return RDK_ERR_HIP;
   }
+  if (noalloc_scratch_init(ctx) != RDK_OK) {  // scratch of rdk_entanglement_noalloc on this device
+    destroy_streams(ctx);
+    delete ctx;
+    return RDK_ERR_HIP;
+  }
   *out = ctx;
   return RDK_OK;
 }

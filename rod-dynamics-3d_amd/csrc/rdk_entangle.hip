@@ -24,6 +24,7 @@
 // a sqrt per pair in FP64); nothing here is on the time-stepping path.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "rdk_internal.hpp"
@@ -242,4 +243,92 @@ int entanglement_run(rdk_ctx* ctx, double cutoff, double* sums, int64_t* pairs) 
   return RDK_OK;
 }
 
+// ---- allocation-free variant ----------------------------------------------
+// Scratch of rdk_entanglement_noalloc: the segments (6 doubles per rod) and
+// the per-tile partial sums and counts of one system at a time.  One copy per
+// GPU, held until the process ends.  Host code never passes `scratch` itself
+// to a kernel: its device address comes from hipGetSymbolAddress, once per
+// device, at context creation (noalloc_scratch_init).
+constexpr int NOALLOC_TILES = (RDK_NOALLOC_MAX_RODS + ENT_TILE - 1) / ENT_TILE;
+constexpr size_t NOALLOC_PAIRS = (size_t)NOALLOC_TILES * (NOALLOC_TILES + 1) / 2;  // tile pairs of one system
+constexpr size_t NOALLOC_SEG_BYTES = 6 * sizeof(double) * (size_t)RDK_NOALLOC_MAX_RODS;
+constexpr size_t NOALLOC_NEED = NOALLOC_SEG_BYTES + NOALLOC_PAIRS * (sizeof(double) + sizeof(unsigned long long));
+static_assert(NOALLOC_NEED <= RDK_NOALLOC_SCRATCH_BYTES, "scratch too small for its own limits");
+
+static __device__ __align__(256) char scratch[RDK_NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static char* g_scratch[NOALLOC_MAX_DEVICES] = {};
+static std::mutex g_scratch_mu;
+
+// Called by rdk_create with ctx->device current.
+int noalloc_scratch_init(rdk_ctx* ctx) {
+  if (ctx->device < 0 || ctx->device >= NOALLOC_MAX_DEVICES) return RDK_OK;  // variant unavailable there
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[ctx->device]) return RDK_OK;
+  void* ptr = nullptr;
+  RDK_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  g_scratch[ctx->device] = (char*)ptr;
+  return RDK_OK;
+}
+
+__global__ void k_ent_zero(double* __restrict__ sums, unsigned long long* __restrict__ pairs, int s) {
+  if (threadIdx.x == 0) {
+    sums[s] = 0.0;
+    pairs[s] = 0ull;
+  }
+}
+
+static int entanglement_noalloc_run(rdk_ctx* ctx, double cutoff, double* d_sums, unsigned long long* d_pairs) {
+  char* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (ctx->device >= 0 && ctx->device < NOALLOC_MAX_DEVICES) base = g_scratch[ctx->device];
+  }
+  if (!base) {
+    ctx->err = "rdk_entanglement_noalloc: no scratch address for this device";
+    return RDK_ERR_STATE;
+  }
+  const int n = ctx->n;
+  if (n > RDK_NOALLOC_MAX_RODS) {
+    ctx->err = "rdk_entanglement_noalloc: more than RDK_NOALLOC_MAX_RODS rods";
+    return RDK_ERR_CAPACITY;
+  }
+  Segs S;
+  double* seg = (double*)base;
+  for (int k = 0; k < 3; ++k) {
+    S.a[k] = seg + (size_t)k * RDK_NOALLOC_MAX_RODS;
+    S.e[k] = seg + (size_t)(3 + k) * RDK_NOALLOC_MAX_RODS;
+  }
+  double* psum = (double*)(base + NOALLOC_SEG_BYTES);
+  unsigned long long* pcnt = (unsigned long long*)(psum + NOALLOC_PAIRS);
+  if (n > 0)
+    hipLaunchKernelGGL(k_ent_segments, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, body_ptrs(ctx),
+                       ctx->sys_dev.p, n, S);
+  for (int s = 0; s < ctx->nsys; ++s) {
+    const int i0 = ctx->sys_off[s], ns = ctx->sys_off[s + 1] - i0;
+    if (ns < 2) {
+      hipLaunchKernelGGL(k_ent_zero, dim3(1), dim3(64), 0, ctx->stream, d_sums, d_pairs, s);
+      continue;
+    }
+    const int T = (ns + ENT_TILE - 1) / ENT_TILE;
+    const long long nb = (long long)T * (T + 1) / 2;  // <= NOALLOC_PAIRS since ns <= n
+    hipLaunchKernelGGL(k_ent_pairs, dim3((unsigned)nb), dim3(ENT_THREADS), 0, ctx->stream, S, i0, ns, T, cutoff,
+                       psum, pcnt);
+    hipLaunchKernelGGL(k_ent_reduce, dim3(1), dim3(1024), 0, ctx->stream, psum, pcnt, nb, d_sums + s, d_pairs + s);
+  }
+  RDK_HIP(hipGetLastError());
+  return RDK_OK;
+}
+
 }  // namespace rdk
+
+extern "C" int rdk_entanglement_noalloc(rdk_ctx* ctx, double cutoff, double* d_sums, uint64_t* d_pairs) {
+  if (!ctx || !d_sums || !d_pairs) return RDK_ERR_ARG;
+  if (hipSetDevice(ctx->device) != hipSuccess) {
+    ctx->err = "hipSetDevice failed";
+    return RDK_ERR_HIP;
+  }
+  if (ctx->n <= 0 || ctx->nsys <= 0) return RDK_ERR_STATE;
+  return rdk::entanglement_noalloc_run(ctx, cutoff, d_sums, (unsigned long long*)d_pairs);
+}

@@ -340,6 +340,8 @@ int integrate_run(rdk_ctx* ctx, int stage);
 int forces_run(rdk_ctx* ctx, int model, const void* params, double* pe);
 int observables_run(rdk_ctx* ctx, double* ke, double* phi);
 int entanglement_run(rdk_ctx* ctx, double cutoff, double* sums, int64_t* pairs);
+// device address of the allocation-free scratch array for ctx->device (rdk_entangle.hip)
+int noalloc_scratch_init(rdk_ctx* ctx);
 int populate_nonoverlap_run(rdk_ctx* ctx, int n, float length, float diameter, const float* bmin, const float* bmax,
                             int periodic, float cell_size, uint32_t seed, int max_attempts, int window,
                             float* centres, float* dirs, float* quats, int* placed, int64_t* attempts);

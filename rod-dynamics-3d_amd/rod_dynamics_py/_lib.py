@@ -112,6 +112,7 @@ _SIGS = {
     "rdk_num_kernels": (C.c_int, []),
     "rdk_work_counters": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_int]),
     "rdk_entanglement": (C.c_int, [C.c_void_p, C.c_double, _dp, C.POINTER(C.c_int64)]),
+    "rdk_entanglement_noalloc": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]),
     "rdk_get_contacts_raw": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_int)]),
     "rdk_keep_manifolds": (C.c_int, [C.c_void_p, C.c_int]),
     "rdk_keep_soft_forces": (C.c_int, [C.c_void_p, C.c_int]),
@@ -481,6 +482,12 @@ class Context:
         self._check(self._L.rdk_entanglement(self._h, float(cutoff), dptr(sums),
                                              pairs.ctypes.data_as(C.POINTER(C.c_int64))))
         return sums, pairs
+
+    def entanglement_noalloc(self, d_sums, d_pairs, cutoff=-1.0):
+        """rdk_entanglement_noalloc: sums and pair counts written into device
+        memory at d_sums (doubles) and d_pairs (uint64), one per system,
+        asynchronously on the context's stream."""
+        self._check(self._L.rdk_entanglement_noalloc(self._h, float(cutoff), C.c_void_p(d_sums), C.c_void_p(d_pairs)))
 
     def populate_nonoverlap(self, n, length, diameter, bmin, bmax, periodic, cell_size, seed, max_attempts,
                             window=0):
