@@ -89,9 +89,12 @@ def c2_settling(n=100_000, phi=0.05, seed=SEED0 + 2, mu=0.5, aspect=20.0, gap=0.
     # No omega = 1 continuation here: on this bed it lets the stalled islands
     # creep on for thousands of sweeps (measured: 20 % lower throughput for
     # 60 instead of about 100 non-converged steps out of 4980).
+    # Hot sets: once a large island has few manifolds above tol, only the
+    # manifolds around them are swept (the slow stick/slip modes of a few wall
+    # contacts that hold the last decades of the residual are local).
     nscp = L.nsc_params(beta=0.0, cfm=0.0, omega=1.3, tol=1e-8, max_sweeps=10000, check_every=4,
                         friction_proj=1, position_stabilization=0, warm_mode=1, stop_rule=1, warm_walls=1,
-                        stall_sweeps=1000, stall_omega=0.0)
+                        stall_sweeps=1000, stall_omega=0.0, hot_set=1)
     return Scene(name=f"C2_settling_N{n}", x=x, q=q, v=np.zeros((n, 3)), w=np.zeros((n, 3)), r=r, h=h, mass=mass,
                  rest=np.zeros(n), offsets=np.array([0, n], np.int32), descs=[desc],
                  phys=L.physics((0.0, 0.0, -1.0), 1e-3, 0.08, 0.12, 80.0), model="nsc", params=nscp,
@@ -211,7 +214,7 @@ def c4_batched(nsys=256, seed=SEED0 + 4, kT=1.0, restitution=0.5, threads=8):
     n_tot = int(offsets[-1])
     nscp = L.nsc_params(beta=0.0, cfm=0.0, omega=1.3, tol=1e-8, max_sweeps=10000, check_every=4, friction_proj=1,
                         position_stabilization=0, warm_mode=1, stop_rule=1, warm_walls=0, stall_sweeps=250,
-                        stall_omega=1.0)
+                        stall_omega=1.0, hot_set=1)
     return Scene(name=f"C4_batched_{nsys}x", x=cat(0), q=cat(1), v=cat(2), w=cat(3), r=cat(4), h=cat(5),
                  mass=cat(6), rest=np.full(n_tot, restitution), offsets=offsets, descs=[p[7] for p in parts],
                  phys=L.physics((0.0, 0.0, 0.0), 1e-3, 0.0, 0.0, 1e30), model="nsc", params=nscp,
