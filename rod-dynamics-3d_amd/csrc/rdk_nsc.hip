@@ -1409,8 +1409,10 @@ __global__ void __launch_bounds__(LDS_THREADS) k_island_lds(BodyPtrs B, MfPtrs M
           int q = hot ? tid : bnd[c] + tid;
           if (hot ? (tid >= bnd[c] && tid < bnd[c + 1]) : q < bnd[c + 1]) {
             d = dmax(d, psor_update(LB, R, 0, K, mu));
-            const int i = hot ? hl[q] : q;
-            for (int t = 0; t < 3; ++t) A.lr(t, i) = R.lam[t];
+            // a hot-set lane keeps its impulses in registers until the round
+            // ends: no global store waits at the colour barrier
+            if (!hot)
+              for (int t = 0; t < 3; ++t) A.lr(t, q) = R.lam[t];
             q += LDS_THREADS;
           }
           // further manifolds of this lane reuse the same registers: all fields
@@ -1431,6 +1433,11 @@ __global__ void __launch_bounds__(LDS_THREADS) k_island_lds(BodyPtrs B, MfPtrs M
       // the hot set's residual from the manifolds in registers (one per lane)
       if (block_max(tid < nh ? residual_one(LB, R, 0, K, mu) : 0.0, red) <= hot_thr || sweeps + ls >= P.max_sweeps)
         break;
+    }
+    if (hot) {
+      if (tid < nh)
+        for (int t = 0; t < 3; ++t) A.lr(t, hl[tid]) = R.lam[t];
+      __syncthreads();  // stored before the island check reads them
     }
     sweeps += ls;
     updates += (unsigned long long)ls * (unsigned long long)(hot ? nh : m);

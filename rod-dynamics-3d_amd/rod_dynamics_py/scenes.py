@@ -165,7 +165,8 @@ def c3_dense(n=1_000_000, phi0=0.05, phi1=0.30, seed=SEED0 + 3, mu=0.3, kT=1.0):
 RSA_SATURATION = {4: 0.256, 8: 0.199, 16: 0.142, 32: 0.090, 64: 0.054}
 
 
-def c4_batched(nsys=256, seed=SEED0 + 4, kT=1.0, restitution=0.5, threads=8):
+def c4_batched(nsys=256, seed=SEED0 + 4, kT=1.0, restitution=0.5, threads=8, n_range=(500, 20000),
+               aspects=(4, 8, 16, 32, 64)):
     """Many independent periodic packings in one context.
 
     Per system: N = floor(exp(U[ln 500, ln 20000])), L/d from {4, 8, 16, 32,
@@ -176,15 +177,16 @@ def c4_batched(nsys=256, seed=SEED0 + 4, kT=1.0, restitution=0.5, threads=8):
     is shorter than twice the rod reach (L + d) is redrawn, so the minimum
     image stays unique; the number of redraws is recorded.  Gravity 0, thermal
     velocities (T = kT), restitution as given (the configuration does not fix
-    one; a thermal gas needs some to stay a gas).
+    one; a thermal gas needs some to stay a gas).  n_range and aspects narrow
+    the draws for reduced test versions.
     """
     from concurrent.futures import ThreadPoolExecutor
 
     rng = np.random.default_rng(seed)
     draws, redraws = [], 0
     while len(draws) < nsys:
-        n = int(math.floor(math.exp(rng.uniform(math.log(500), math.log(20000)))))
-        aspect = float(rng.choice([4, 8, 16, 32, 64]))
+        n = int(math.floor(math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1])))))
+        aspect = float(rng.choice(list(aspects)))
         mu = float(rng.uniform(0.0, 1.0))
         cap = min(0.25, 0.85 * RSA_SATURATION[int(aspect)])
         phi = float(rng.uniform(min(0.05, cap), cap))
