@@ -61,7 +61,7 @@ BYTES = {
 # (FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0, from a calibrated 1 GiB copy)
 # for (workload, kernel); recorded with the command in profiles/
 TRAFFIC = {("c2", "broad_narrow"): 128.2e6 + 0.21e6, ("c4", "broad_narrow"): 479.4e6 + 6.42e6,
-           ("c2", "island_large"): 961.8 * 1024 * 2.0 + 759.3 * 1024}
+           ("c2", "island_large"): 942.8 * 1024 * 2.0 + 381.3 * 1024}
 
 
 def parse():
@@ -77,6 +77,9 @@ def parse():
     ap.add_argument("--check-every", type=int, default=8,
                     help="sweeps between residual checks on the settling workloads (4: 14.4M, 8: 15.1M, 16: 15.4M "
                          "rod-steps/s at 1e5 rods, 0.7 / 1.4 %% more sweeps)")
+    ap.add_argument("--warm-mode", type=int, default=None, choices=[0, 1, 2, 3, 4],
+                    help="override the settling workloads' PSOR start (1 previous step, 4 learned for every "
+                         "manifold from the contact alone; learned modes load --weights)")
     ap.add_argument("--hot-set", type=int, default=None, choices=[0, 1],
                     help="override the workload's hot-set phase of the LDS island solve (hard-contact workloads)")
     ap.add_argument("--stall-sweeps", type=int, default=None,
@@ -291,12 +294,21 @@ def run_c2(a, L, scenes):
         p.stall_sweeps = a.stall_sweeps
     if a.hot_set is not None:
         p.hot_set = a.hot_set
+    if a.warm_mode is not None:
+        p.warm_mode = a.warm_mode
+    weights = None
+    if p.warm_mode >= 2:
+        import numpy as np
+
+        weights = np.load(a.weights)
     total = a.protocol_steps
 
     # ---- run 1: warm-up (chunk length autotune), ranking and CPU snapshots
     ctx = L.Context(0)
     names = [ctx._L.rdk_kernel_name(k).decode() for k in range(ctx._L.rdk_num_kernels())]
     sc.load(ctx)
+    if weights is not None:
+        ctx.mlp_set_params(weights)
     if a.chunk_len > 0:
         best, tune, used = a.chunk_len, {}, 0
     else:  # --chunk-len 0: autotune inside the warm-up
@@ -346,6 +358,8 @@ def run_c2(a, L, scenes):
     # ---- run 2: the same protocol, timed blocks
     ctx = L.Context(0)
     sc.load(ctx)
+    if weights is not None:
+        ctx.mlp_set_params(weights)
     ctx.set_profiling(True)
     ctx.set_profile_mask(1 << names.index(dom))
     ctx.step_nsc(det, p, warm)
@@ -405,6 +419,7 @@ def run_c2(a, L, scenes):
                         f"{warm + 1}..{total}" if len(blocks) > 1 else f"steps {warm + 1}..{total}",
         "seed": sc.extra["seed"], "check_every": a.check_every, "chunk_len": best, "chunk_len_autotune_ms": tune,
         "stall_sweeps": p.stall_sweeps, "stall_omega": p.stall_omega, "hot_set": p.hot_set,
+        "warm_mode": p.warm_mode,
         "hipgraph": "check blocks of the huge-island solve (sweeps x colours + residual) captured once per island "
                     "and step, replayed; the step itself reads counts back and is not captured"})
     out["value_counts"] = (f"rods x timed steps that reached residual 1e-8 ({converged} of {steps}) / wall time of "
@@ -416,7 +431,9 @@ def run_c2(a, L, scenes):
                                                  or [0.0]),
                    "nonconverged_steps": nonconv,
                    "nonconverged_residuals": [ph["residual_max"] for ph in phases if ph["nonconverged_steps"] > 0],
-                   "warm_start": "previous step (rod-rod and plane-wall contacts)"}
+                   "warm_start": {1: "previous step (rod-rod and plane-wall contacts)",
+                                  4: f"learned for every manifold from the contact alone "
+                                     f"({os.path.relpath(a.weights, ROOT)})"}.get(p.warm_mode, p.warm_mode)}
     out["phases"] = phases
     out["replay_identical"] = identical
     out["roofline"] = roofline_from(dom, acc_kt, acc_wc, sc.n, a.workload)

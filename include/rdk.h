@@ -67,7 +67,8 @@ typedef struct rdk_nsc_params {
   int position_stabilization; /* scales beta by 0.01 (nsc_contact.cpp:339-341) */
   int warm_mode;           /* 0 cold, 1 previous step (reference), 2 learned (MLP) for the
                               manifolds without a previous-step normal impulse, 3 learned for
-                              every manifold (the previous-step impulse is an input) */
+                              every manifold (the previous-step impulse is an input), 4 learned
+                              for every manifold from the contact alone (impulse input 0) */
   int stop_rule;           /* 0 reference (max sum|dlambda| < 1e-6), 1 residual <= tol */
   int warm_walls;          /* 1: plane-wall manifolds also start from the previous
                               step's impulses, keyed by (rod, plane, endpoint); the

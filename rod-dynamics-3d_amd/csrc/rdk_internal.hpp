@@ -345,7 +345,7 @@ int noalloc_scratch_init(rdk_ctx* ctx);
 int populate_nonoverlap_run(rdk_ctx* ctx, int n, float length, float diameter, const float* bmin, const float* bmax,
                             int periodic, float cell_size, uint32_t seed, int max_attempts, int window,
                             float* centres, float* dirs, float* quats, int* placed, int64_t* attempts);
-int mlp_warm_start(rdk_ctx* ctx, bool all);
+int mlp_warm_start(rdk_ctx* ctx, int mode);
 int nsc_project_run(rdk_ctx* ctx, const rdk_detect_params* dp, const rdk_nsc_params* p, int* passes, int* npos);
 void mlp_free(rdk_ctx* ctx);
 

@@ -437,8 +437,9 @@ static int mlp_infer_dev(rdk_ctx* ctx, Mlp* M, const float* X, int n, float* Y) 
   return RDK_OK;
 }
 
-// features of the manifolds just built (natural order), into M->feat / M->lscale
-static int mlp_features(rdk_ctx* ctx, Mlp* M) {
+// features of the manifolds just built (natural order), into M->feat / M->lscale;
+// use_prev = false leaves the previous-step impulse feature at 0
+static int mlp_features(rdk_ctx* ctx, Mlp* M, bool use_prev = true) {
   const int nm = ctx->nm;
   RDK_TRY(ensure(ctx, M->feat, (size_t)std::max(nm, 1) * MLP_IN));
   RDK_TRY(ensure(ctx, M->lscale, std::max(nm, 1)));
@@ -450,10 +451,10 @@ static int mlp_features(rdk_ctx* ctx, Mlp* M) {
   F.b = ctx->mb.p;
   F.prev_keys = ctx->prev_keys.p;
   F.prev_ln = ctx->prev_lam[0].p;
-  F.nprev = ctx->nprev;
+  F.nprev = use_prev ? ctx->nprev : 0;
   F.prev_wkeys = ctx->prev_wkeys.p;
   F.prev_wln = ctx->prev_wlam[0].p;
-  F.nprev_w = ctx->nprev_w;
+  F.nprev_w = use_prev ? ctx->nprev_w : 0;
   const double g = std::sqrt(ctx->phys.gravity[0] * ctx->phys.gravity[0] + ctx->phys.gravity[1] * ctx->phys.gravity[1] +
                              ctx->phys.gravity[2] * ctx->phys.gravity[2]);
   double vs = g * ctx->phys.dt;
@@ -464,7 +465,11 @@ static int mlp_features(rdk_ctx* ctx, Mlp* M) {
   return RDK_OK;
 }
 
-int mlp_warm_start(rdk_ctx* ctx, bool all) {
+// warm_mode 2: prediction for the manifolds without a cached normal impulse;
+// 3: for every manifold, the cached impulse an input; 4: for every manifold
+// from the contact alone (the impulse input 0, the cache not used)
+int mlp_warm_start(rdk_ctx* ctx, int mode) {
+  const bool all = mode >= 3;
   Mlp* M = get_mlp(ctx);
   if (!M->ready) {
     ctx->err = "learned warm start: no network loaded (rdk_mlp_init / rdk_mlp_set_params / rdk_mlp_train)";
@@ -472,7 +477,7 @@ int mlp_warm_start(rdk_ctx* ctx, bool all) {
   }
   const int nm = ctx->nm;
   if (nm == 0) return RDK_OK;
-  RDK_TRY(mlp_features(ctx, M));
+  RDK_TRY(mlp_features(ctx, M, mode != 4));
   RDK_TRY(ensure(ctx, M->out, (size_t)nm * MLP_OUT));
   RDK_TRY(mlp_infer_dev(ctx, M, M->feat.p, nm, M->out.p));
   ProfScope ps(ctx, K_MLP);

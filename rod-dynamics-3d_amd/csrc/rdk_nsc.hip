@@ -1603,7 +1603,7 @@ int nsc_build_run(rdk_ctx* ctx, const rdk_nsc_params* p) {
   ctx->have_manifolds = true;
   ctx->have_coloring = false;
   ctx->work[W_MANIFOLDS] += nm;
-  if ((p->warm_mode == 2 || p->warm_mode == 3) && nm > 0) RDK_TRY(mlp_warm_start(ctx, p->warm_mode == 3));
+  if (p->warm_mode >= 2 && nm > 0) RDK_TRY(mlp_warm_start(ctx, p->warm_mode));
   return RDK_OK;
 }
 

@@ -157,9 +157,10 @@ def train(ctx, X, T, epochs=20, batch=2048, lr=2e-3, weight_decay=1e-4, seed=0, 
 
 def ab_step(ctx, det, p):
     """One step of the protocol in which the same manifolds are solved from the
-    cold, learned (new contacts only: warm_mode 2; every manifold: warm_mode 3)
-    and previous-step starts (same pre-solve velocities); the trajectory
-    continues from the previous-step solve, as the plain protocol."""
+    cold, learned (new contacts only: warm_mode 2; every manifold: warm_mode 3;
+    every manifold from the contact alone: warm_mode 4) and previous-step
+    starts (same pre-solve velocities); the trajectory continues from the
+    previous-step solve, as the plain protocol."""
     ctx.integrate(L.RDK_NSC_FREE)
     ctx.detect(_det0(det))
     p1 = _wo_warm(p, 1)
@@ -171,10 +172,12 @@ def ab_step(ctx, det, p):
         _, _, lam_learn = ctx.nsc_manifolds(nm)
         ctx.nsc_build(_wo_warm(p, 3))
         _, _, lam_all = ctx.nsc_manifolds(nm)
+        ctx.nsc_build(_wo_warm(p, 4))
+        _, _, lam_fresh = ctx.nsc_manifolds(nm)
         s0 = ctx.download_state()
         res = {"manifolds": nm}
         for name, lam0 in (("cold", np.zeros_like(lam_prev)), ("learned", lam_learn), ("learned_all", lam_all),
-                           ("previous", lam_prev)):
+                           ("learned_fresh", lam_fresh), ("previous", lam_prev)):
             ctx.upload_state(v=s0["v"], w=s0["w"])
             ctx.nsc_set_lambda(lam0)
             w0 = ctx.work_counters()["sweep_contacts"]
@@ -192,7 +195,7 @@ def ab_step(ctx, det, p):
 def ab_summary(rows):
     rows = [r for r in rows if r is not None]
     out = {"steps": len(rows), "manifolds_per_step": float(np.mean([r["manifolds"] for r in rows])) if rows else 0.0}
-    for name in ("cold", "previous", "learned", "learned_all"):
+    for name in ("cold", "previous", "learned", "learned_all", "learned_fresh"):
         if not rows:
             continue
         out[name] = {k: float(np.mean([r[name][k] for r in rows])) for k in ("sweeps", "manifold_updates", "ms")}
@@ -204,6 +207,10 @@ def ab_summary(rows):
                                                         max(out["learned"]["manifold_updates"], 1e-9))
             out[f"sweeps_ratio_{base}_over_learned_all"] = (out[base]["sweeps"] /
                                                            max(out["learned_all"]["sweeps"], 1e-9))
+            out[f"sweeps_ratio_{base}_over_learned_fresh"] = (out[base]["sweeps"] /
+                                                             max(out["learned_fresh"]["sweeps"], 1e-9))
+            out[f"updates_ratio_{base}_over_learned_fresh"] = (out[base]["manifold_updates"] /
+                                                              max(out["learned_fresh"]["manifold_updates"], 1e-9))
     return out
 
 

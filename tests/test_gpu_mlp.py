@@ -220,3 +220,32 @@ def test_learned_start_for_every_manifold(gpu_ctx_factory):
     assert np.allclose(lam3, expect, rtol=1e-12, atol=1e-15 * np.abs(expect).max())
     assert (lam_prev[:, 0] > 0).sum() > 100
     assert np.allclose(lam3[:, 0], 2.0 * ls, rtol=1e-6)
+
+
+def test_learned_start_from_the_contact_alone(gpu_ctx_factory):
+    """warm_mode 4: the prediction on every manifold, with the previous-step
+    impulse input left at 0 (the cache is not read): the network's output for
+    the features of the cached build with feature 11 zeroed."""
+    B = overlapping_bed(n=2000, seed=65, mu=0.4)
+    ctx = gpu_ctx_factory()
+    prm = _rand_params(7, 0.1)
+    ctx.mlp_set_params(prm)
+    p1 = L.nsc_params(beta=0.0, cfm=0.0, omega=1.3, tol=1e-8, max_sweeps=20000, check_every=4, friction_proj=1,
+                      warm_mode=1, warm_walls=1)
+    _settled_pair_of_builds(ctx, B, p1)
+    nm = ctx.nsc_build(p1)
+    _, _, lam_prev = ctx.nsc_manifolds(nm)
+    X, ls = ctx.mlp_features(nm)
+    assert (X[:, 11] > 0).sum() > 100  # cached impulses are an input of the cached build
+    X0 = X.copy()
+    X0[:, 11] = 0.0
+    Y0 = ctx.mlp_infer(X0)
+    p4 = L.nsc_params(beta=0.0, cfm=0.0, omega=1.3, tol=1e-8, max_sweeps=20000, check_every=4, friction_proj=1,
+                      warm_mode=4, warm_walls=1)
+    assert ctx.nsc_build(p4) == nm
+    _, _, lam4 = ctx.nsc_manifolds(nm)
+    expect = M.apply_prediction(Y0, ls, 0.4, lam_prev, all_manifolds=True)
+    assert np.allclose(lam4, expect, rtol=1e-12, atol=1e-15 * np.abs(expect).max())
+    # the mode really ignores the cache: the cached-input prediction differs
+    Y = ctx.mlp_infer(X)
+    assert not np.allclose(M.apply_prediction(Y, ls, 0.4, lam_prev, all_manifolds=True), lam4)

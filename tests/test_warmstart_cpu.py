@@ -32,10 +32,12 @@ def test_ab_summary_ratios():
              "cold": {"sweeps": 100, "residual": 1e-9, "manifold_updates": 1000, "ms": 2.0},
              "previous": {"sweeps": 40, "residual": 1e-9, "manifold_updates": 400, "ms": 1.0},
              "learned": {"sweeps": 20, "residual": 5e-9, "manifold_updates": 100, "ms": 0.5},
-             "learned_all": {"sweeps": 10, "residual": 6e-9, "manifold_updates": 80, "ms": 0.4}}, None]
+             "learned_all": {"sweeps": 10, "residual": 6e-9, "manifold_updates": 80, "ms": 0.4},
+             "learned_fresh": {"sweeps": 50, "residual": 7e-9, "manifold_updates": 160, "ms": 0.6}}, None]
     s = WS.ab_summary(rows)
     assert s["steps"] == 1
     assert s["sweeps_ratio_previous_over_learned"] == 2.0 and s["sweeps_ratio_cold_over_learned"] == 5.0
     assert s["sweeps_ratio_previous_over_learned_all"] == 4.0 and s["learned_all"]["residual_max"] == 6e-9
     assert s["updates_ratio_previous_over_learned"] == 4.0
     assert s["learned"]["residual_max"] == 5e-9
+    assert s["sweeps_ratio_previous_over_learned_fresh"] == 0.8 and s["updates_ratio_cold_over_learned_fresh"] == 6.25
