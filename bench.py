@@ -82,6 +82,9 @@ def parse():
                          "manifold from the contact alone; learned modes load --weights)")
     ap.add_argument("--hot-set", type=int, default=None, choices=[0, 1],
                     help="override the workload's hot-set phase of the LDS island solve (hard-contact workloads)")
+    ap.add_argument("--friction-proj", type=int, default=None, choices=[0, 1, 2, 3],
+                    help="override the hard-contact workloads' tangential PSOR update (1 scalar step on the "
+                         "Euclidean disc, 2 block step projected in the tangential block's metric)")
     ap.add_argument("--stall-sweeps", type=int, default=None,
                     help="override the workload's stall-rule window (hard-contact workloads)")
     ap.add_argument("--block", type=int, default=250)
@@ -294,6 +297,8 @@ def run_c2(a, L, scenes):
         p.stall_sweeps = a.stall_sweeps
     if a.hot_set is not None:
         p.hot_set = a.hot_set
+    if a.friction_proj is not None:
+        p.friction_proj = a.friction_proj
     if a.warm_mode is not None:
         p.warm_mode = a.warm_mode
     weights = None
@@ -413,7 +418,10 @@ def run_c2(a, L, scenes):
                      "hard-contact PSOR settling (C2)"), "rods": sc.n, "L_over_d": 20, "phi0": 0.05,
         "box": round(sc.extra["side"], 3), "periodic": "x,y", "walls": "floor+ceiling", "mu": 0.5,
         "restitution": 0.0, "omega": 1.3, "residual_tol": 1e-8, "dt": 1e-3,
-        "friction_projection": "euclidean disc" if p.friction_proj == 1 else "reference sequential",
+        "friction_projection": {0: "reference sequential", 1: "euclidean disc, scalar step",
+                                2: "euclidean disc, block step in the tangential metric",
+                                3: "euclidean disc, block step on sticking contacts, scalar step on sliding ones"}[
+                                    p.friction_proj],
         "protocol_steps": total, "warmup_steps": warm,
         "timed_blocks": f"{len(blocks)} block(s) of {blocks[0][1]} step(s), centred in equal strata of steps "
                         f"{warm + 1}..{total}" if len(blocks) > 1 else f"steps {warm + 1}..{total}",
@@ -597,7 +605,7 @@ def cpu_baseline_c4(sc, state, nsample, O):
         W.set_box(list(d.lo), list(d.hi), [int(v) for v in d.periodic], 0, 0)
         W.set_physics(tuple(sc.phys.gravity[:]), sc.phys.dt)
         p = O.nsc_params(mu=d.mu, beta=0.0, cfm=0.0, omega=1.3, tol=1e-8, max_sweeps=10000, check_every=1,
-                         friction_proj=1, stop_rule=1, warm_start=1)
+                         friction_proj=sc.params.friction_proj, stop_rule=1, warm_start=1)
         W.nsc_step(p, 1, method=1)
         t0 = time.perf_counter()
         sw, _ = W.nsc_step(p, 1, method=1)
@@ -620,6 +628,10 @@ def run_c4(a, L, scenes):
     p = sc.params
     if a.stall_sweeps is not None:
         p.stall_sweeps = a.stall_sweeps
+    if a.hot_set is not None:
+        p.hot_set = a.hot_set
+    if a.friction_proj is not None:
+        p.friction_proj = a.friction_proj
     ctx = L.Context(0)
     sc.load(ctx)
     best, tune, used = autotune(ctx, lambda cl, k: ctx.step_nsc(L.detect_params(0.0, cl, 1.0), p, k),

@@ -63,7 +63,12 @@ typedef struct rdk_nsc_params {
   double tol;              /* fixed-point residual tolerance (velocity units) */
   int max_sweeps;
   int check_every;         /* residual evaluated every k sweeps */
-  int friction_proj;       /* 0 reference sequential disc clamp, 1 Euclidean disc */
+  int friction_proj;       /* 0 reference sequential disc clamp, 1 Euclidean disc (one scalar
+                              step omega/g_tb), 2 Euclidean disc with the block step
+                              omega G^-1 w_t (G the 2x2 tangential Delassus block) projected
+                              in G's metric, 3 the block step where it stays in the disc
+                              (sticking) and 1's step where it leaves it (sliding); 1, 2 and
+                              3 share their fixed points and residual */
   int position_stabilization; /* scales beta by 0.01 (nsc_contact.cpp:339-341) */
   int warm_mode;           /* 0 cold, 1 previous step (reference), 2 learned (MLP) for the
                               manifolds without a previous-step normal impulse, 3 learned for

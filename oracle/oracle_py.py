@@ -73,6 +73,7 @@ def lib():
         L.orc_nsc_solve_island.restype = C.c_int
         L.orc_nsc_solve_island.argtypes = [C.c_void_p, C.POINTER(NscParams), _ip, C.c_int, C.c_int, C.c_double,
                                            C.c_int, _dp, C.POINTER(C.c_longlong), _ip]
+        L.orc_disc_project_metric.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, _dp]
         L.orc_nsc_residual.restype = C.c_double
         L.orc_nsc_residual.argtypes = [C.c_void_p, C.POINTER(NscParams)]
         L.orc_nsc_store_warm.argtypes = [C.c_void_p]
@@ -129,6 +130,14 @@ def nsc_params(mu=0.5, beta=0.0, cfm=0.0, omega=1.0, tol=1e-8, max_sweeps=10000,
     p.max_sweeps, p.check_every, p.friction_proj, p.stop_rule = max_sweeps, check_every, friction_proj, stop_rule
     p.position_stabilization, p.warm_start = position_stabilization, warm_start
     return p
+
+
+def disc_project_metric(a, b, c, r, y):
+    """argmin over |x| <= r of (x - y)^T G (x - y), G = [[a, b], [b, c]]
+    (the oracle's friction_proj 2 projection; |y| > r > 0)."""
+    out = np.array(y, dtype=np.float64)
+    lib().orc_disc_project_metric(a, b, c, r, _d(out))
+    return out
 
 
 def soft_params(delta=0.005, k_scaler=1000.0, damping=0.0, mu=0.5, mu_static=0.0, nu=1e-5, kt=1000.0,

@@ -632,6 +632,49 @@ static inline V3 rel_vel(const orc_world& W, const Manifold& m) {
   return sub(add(Bb.v, cross(Bb.w, m.rb)), add(A.v, cross(A.w, m.ra)));
 }
 
+// Projection of y onto the disc |x| <= r in the metric of the SPD tangential
+// block G = [[a, b], [b, c]] (|y| > r > 0): x = (G + nu I)^-1 G y with the nu > 0
+// at which |x| = r.  In G's eigenbasis (e1 >= e2 > 0, z = y rotated) the
+// residual h(nu) = 1/r - 1/|x(nu)| is convex and decreasing, and the root lies
+// in [e2 (|z|/r - 1), e1 (|z|/r - 1)]: Newton from the left end increases nu
+// monotonically to the root, and stops once a step no longer increases it
+// (at most 16 steps); x is then scaled onto the circle.
+static void disc_project_metric(double a, double b, double c, double r, double& y1, double& y2) {
+  const double hm = 0.5 * (a + c), hd = 0.5 * (a - c);
+  const double s = std::sqrt(hd * hd + b * b);
+  const double e1 = hm + s, e2 = hm - s;
+  double cs = 1.0, sn = 0.0;
+  if (s > 0.0) {
+    const double c2 = hd / s;
+    cs = std::sqrt(0.5 * (1.0 + c2));
+    sn = std::sqrt(0.5 * (1.0 - c2));
+    if (b < 0.0) sn = -sn;
+  }
+  const double z1 = cs * y1 + sn * y2, z2 = cs * y2 - sn * y1;
+  double nu = e2 * (std::sqrt(z1 * z1 + z2 * z2) / r - 1.0);
+  if (!(nu > 0.0)) nu = 0.0;
+  double x1, x2, nx;
+  for (int it = 0; it < 16; ++it) {
+    const double i1 = 1.0 / (e1 + nu), i2 = 1.0 / (e2 + nu);
+    x1 = e1 * z1 * i1;
+    x2 = e2 * z2 * i2;
+    nx = std::sqrt(x1 * x1 + x2 * x2);
+    const double d = x1 * x1 * i1 + x2 * x2 * i2;
+    const double nn = nu + (nx - r) * (nx * nx) / (r * d);
+    if (!(nn > nu)) break;
+    nu = nn;
+  }
+  x1 = e1 * z1 / (e1 + nu);
+  x2 = e2 * z2 / (e2 + nu);
+  nx = std::sqrt(x1 * x1 + x2 * x2);
+  if (nx > r) {
+    x1 = x1 * (r / nx);
+    x2 = x2 * (r / nx);
+  }
+  y1 = cs * x1 - sn * x2;
+  y2 = sn * x1 + cs * x2;
+}
+
 // One PSOR update of one manifold (nsc_contact.cpp:385-552).  Returns
 // |dn|+|dt1|+|dt2|.
 static double psor_one(orc_world& W, Manifold& m, const orc_nsc_params& P) {
@@ -687,6 +730,51 @@ static double psor_one(orc_world& W, Manifold& m, const orc_nsc_params& P) {
     double old_t2 = m.lt2;
     m.lt2 = clampd(old_t2 + delta_t2, -limit_t2, limit_t2);
     dt2 = m.lt2 - old_t2;
+  } else if (P.friction_proj >= 2) {
+    // block step lambda_t - omega G^-1 w_t with the 2x2 tangential block G
+    // (G11 = g_t1, G22 = g_t2, G12 from the cached I^-1 (r x t2)): a sticking
+    // contact's slip is solved in both directions at once.  Outside the disc,
+    // 2 projects the step in G's metric and 3 takes the scalar step of 1 from
+    // the current impulse instead; both keep 1's fixed points (maximal
+    // dissipation), and a sliding solution's block step always leaves the disc.
+    double w_t1 = dot(m.t1, v_rel) + cfm * m.lt1;
+    double w_t2 = dot(m.t2, v_rel) + cfm * m.lt2;
+    double g12 = dot(cross(m.ra, m.t1), m.IAt2);
+    if (!wall) g12 = g12 + dot(cross(m.rb, m.t1), m.IBt2);
+    const double det = m.gt1 * m.gt2 - g12 * g12;
+    const double s1 = (m.gt2 * w_t1 - g12 * w_t2) / det, s2 = (m.gt1 * w_t2 - g12 * w_t1) / det;
+    double y1 = m.lt1 - omega * s1;
+    double y2 = m.lt2 - omega * s2;
+    if (y1 * y1 + y2 * y2 > max_fric * max_fric) {
+      if (P.friction_proj == 3) {
+        const double st = omega / m.gtb;
+        y1 = m.lt1 - st * w_t1;
+        y2 = m.lt2 - st * w_t2;
+        double ny = std::sqrt(y1 * y1 + y2 * y2);
+        if (ny > max_fric) {
+          double sc = ny > 0.0 ? max_fric / ny : 0.0;
+          y1 = y1 * sc;
+          y2 = y2 * sc;
+        }
+      } else if (max_fric > 0.0) {
+        disc_project_metric(m.gt1, g12, m.gt2, max_fric, y1, y2);
+      } else {
+        y1 = y2 = 0.0;
+      }
+    }
+    dt1 = y1 - m.lt1;
+    dt2 = y2 - m.lt2;
+    m.lt1 = y1;
+    m.lt2 = y2;
+    if (dt1 != 0.0) {
+      A.v = sub(A.v, mul(m.t1, dt1 * m.invMA));
+      A.w = sub(A.w, mul(m.IAt1, dt1));
+      if (!wall) {
+        Body& Bb = W.B[m.b];
+        Bb.v = add(Bb.v, mul(m.t1, dt1 * m.invMB));
+        Bb.w = add(Bb.w, mul(m.IBt1, dt1));
+      }
+    }
   } else {
     // Euclidean disc: projected step of the joint tangential impulse with one
     // scalar step omega/g_tb (g_tb bounds the 2x2 tangential block), whose
@@ -1398,6 +1486,10 @@ int orc_nsc_solve(orc_world* W, const orc_nsc_params* P, const int* order, doubl
 }
 
 double orc_nsc_residual(orc_world* W, const orc_nsc_params* P) { return residual(*W, *P); }
+
+void orc_disc_project_metric(double a, double b, double c, double r, double* y) {
+  disc_project_metric(a, b, c, r, y[0], y[1]);
+}
 
 void orc_nsc_store_warm(orc_world* W) { store_warm(*W); }
 

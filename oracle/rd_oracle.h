@@ -39,7 +39,9 @@ typedef struct orc_nsc_params {
   double tol;            /* build-defined fixed-point residual tolerance */
   int max_sweeps;
   int check_every;       /* evaluate the residual every k sweeps */
-  int friction_proj;     /* 0 = reference sequential disc clamp, 1 = Euclidean disc */
+  int friction_proj;     /* 0 = reference sequential disc clamp, 1 = Euclidean disc (scalar step),
+                            2 = Euclidean disc, block step G^-1 w_t projected in G's metric,
+                            3 = block step where it stays in the disc, else the scalar step of 1 */
   int stop_rule;         /* 0 = reference rule (max sum|dlambda| < 1e-6), 1 = residual <= tol */
   int position_stabilization; /* only scales beta (x0.01) as the reference does */
   int warm_start;        /* 0 = cold, 1 = previous-step key match (reference) */
@@ -109,6 +111,9 @@ int orc_nsc_solve(orc_world* W, const orc_nsc_params* P, const int* order,
 int orc_nsc_solve_island(orc_world* W, const orc_nsc_params* P, const int* ix, int m, int stall_sweeps,
                          double stall_omega, int hot, double* res, long long* updates, int* hot_sweeps);
 double orc_nsc_residual(orc_world* W, const orc_nsc_params* P);
+/* The metric disc projection of friction_proj 2: y <- argmin over |x| <= r of
+ * (x - y)^T G (x - y), G = [[a, b], [b, c]] SPD, for |y| > r > 0. */
+void orc_disc_project_metric(double a, double b, double c, double r, double* y);
 void orc_nsc_store_warm(orc_world* W);
 int orc_nsc_step(orc_world* W, const orc_nsc_params* P, int nsteps,
                  int detect_method, double cell_size, double* sweeps_total,

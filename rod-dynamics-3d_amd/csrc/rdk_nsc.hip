@@ -468,6 +468,49 @@ struct StallTrack {
   }
 };
 
+// Projection of y onto the disc |x| <= r in the metric of the SPD tangential
+// block G = [[a, b], [b, c]] (|y| > r > 0): x = (G + nu I)^-1 G y with the nu > 0
+// at which |x| = r.  In G's eigenbasis (e1 >= e2 > 0, z = y rotated) the
+// residual h(nu) = 1/r - 1/|x(nu)| is convex and decreasing, and the root lies
+// in [e2 (|z|/r - 1), e1 (|z|/r - 1)]: Newton from the left end increases nu
+// monotonically to the root, and stops once a step no longer increases it
+// (at most 16 steps); x is then scaled onto the circle.
+__device__ __forceinline__ void disc_project_metric(double a, double b, double c, double r, double& y1, double& y2) {
+  const double hm = 0.5 * (a + c), hd = 0.5 * (a - c);
+  const double s = sqrt(hd * hd + b * b);
+  const double e1 = hm + s, e2 = hm - s;
+  double cs = 1.0, sn = 0.0;
+  if (s > 0.0) {
+    const double c2 = hd / s;
+    cs = sqrt(0.5 * (1.0 + c2));
+    sn = sqrt(0.5 * (1.0 - c2));
+    if (b < 0.0) sn = -sn;
+  }
+  const double z1 = cs * y1 + sn * y2, z2 = cs * y2 - sn * y1;
+  double nu = e2 * (sqrt(z1 * z1 + z2 * z2) / r - 1.0);
+  if (!(nu > 0.0)) nu = 0.0;
+  double x1, x2, nx;
+  for (int it = 0; it < 16; ++it) {
+    const double i1 = 1.0 / (e1 + nu), i2 = 1.0 / (e2 + nu);
+    x1 = e1 * z1 * i1;
+    x2 = e2 * z2 * i2;
+    nx = sqrt(x1 * x1 + x2 * x2);
+    const double d = x1 * x1 * i1 + x2 * x2 * i2;
+    const double nn = nu + (nx - r) * (nx * nx) / (r * d);
+    if (!(nn > nu)) break;
+    nu = nn;
+  }
+  x1 = e1 * z1 / (e1 + nu);
+  x2 = e2 * z2 / (e2 + nu);
+  nx = sqrt(x1 * x1 + x2 * x2);
+  if (nx > r) {
+    x1 = x1 * (r / nx);
+    x2 = x2 * (r / nx);
+  }
+  y1 = cs * x1 - sn * x2;
+  y2 = sn * x1 + cs * x2;
+}
+
 // One projected Gauss-Seidel update of manifold i (nsc_contact.cpp:385-552).
 // Every manifold field the update may need is loaded first, unconditionally
 // (the loads are independent, so they are in flight together, and the
@@ -499,8 +542,8 @@ __device__ __forceinline__ double psor_update(const BodyPtrs& B, MP& M, int i, c
     IBt1 = M.get3(F_IBT1, i);
     IAt2 = M.get3(F_IAT2, i);
     IBt2 = M.get3(F_IBT2, i);
-    g1 = K.proj == 0 ? M.at(F_GT1, i) : M.at(F_GTB, i);
-    g2 = K.proj == 0 ? M.at(F_GT2, i) : 0.0;
+    g1 = K.proj == 1 ? M.at(F_GTB, i) : M.at(F_GT1, i);
+    g2 = K.proj == 1 ? 0.0 : M.at(F_GT2, i);
   }
   d3 va = ld3(B.v, a), wa = ld3(B.w, a);
   d3 vb = mk3(0, 0, 0), wb = mk3(0, 0, 0);
@@ -534,8 +577,8 @@ __device__ __forceinline__ double psor_update(const BodyPtrs& B, MP& M, int i, c
   if (!EARLY) {
     t1 = M.get3(F_T1, i);
     t2 = M.get3(F_T2, i);
-    g1 = K.proj == 0 ? M.at(F_GT1, i) : M.at(F_GTB, i);
-    g2 = K.proj == 0 ? M.at(F_GT2, i) : 0.0;
+    g1 = K.proj == 1 ? M.at(F_GTB, i) : M.at(F_GT1, i);
+    g2 = K.proj == 1 ? 0.0 : M.at(F_GT2, i);
   }
   double dt1, dt2;
   if (K.proj == 0) {
@@ -565,6 +608,56 @@ __device__ __forceinline__ double psor_update(const BodyPtrs& B, MP& M, int i, c
     const double old_t2 = lt2;
     lt2 = dclamp(old_t2 + delta_t2, -limit_t2, limit_t2);
     dt2 = lt2 - old_t2;
+  } else if (K.proj >= 2) {
+    // block step lambda_t - omega G^-1 w_t with the 2x2 tangential block G
+    // (g1 = G11, g2 = G22, G12 from the cached I^-1 (r x t2)): a sticking
+    // contact's slip is solved in both directions at once.  Outside the disc,
+    // 2 projects the step in G's metric and 3 takes the scalar step of 1 from
+    // the current impulse instead; both keep 1's fixed points (maximal
+    // dissipation), and a sliding solution's block step always leaves the disc.
+    const double w_t1 = dot(t1, v_rel) + K.cfm * lt1;
+    const double w_t2 = dot(t2, v_rel) + K.cfm * lt2;
+    if (!EARLY) {
+      IAt2 = M.get3(F_IAT2, i);
+      if (!wall) IBt2 = M.get3(F_IBT2, i);
+    }
+    double g12 = dot(cross(ra, t1), IAt2);
+    if (!wall) g12 = g12 + dot(cross(rb, t1), IBt2);
+    const double det = g1 * g2 - g12 * g12;
+    const double s1 = (g2 * w_t1 - g12 * w_t2) / det, s2 = (g1 * w_t2 - g12 * w_t1) / det;
+    double y1 = lt1 - K.omega * s1;
+    double y2 = lt2 - K.omega * s2;
+    if (y1 * y1 + y2 * y2 > max_fric * max_fric) {
+      if (K.proj == 3) {
+        const double st = K.omega / M.at(F_GTB, i);
+        y1 = lt1 - st * w_t1;
+        y2 = lt2 - st * w_t2;
+        const double ny = sqrt(y1 * y1 + y2 * y2);
+        if (ny > max_fric) {
+          const double sc = ny > 0.0 ? max_fric / ny : 0.0;
+          y1 = y1 * sc;
+          y2 = y2 * sc;
+        }
+      } else if (max_fric > 0.0) {
+        disc_project_metric(g1, g12, g2, max_fric, y1, y2);
+      } else {
+        y1 = y2 = 0.0;
+      }
+    }
+    dt1 = y1 - lt1;
+    dt2 = y2 - lt2;
+    lt1 = y1;
+    lt2 = y2;
+    if (dt1 != 0.0) {
+      if (!EARLY) IAt1 = M.get3(F_IAT1, i);
+      va = va - t1 * (dt1 * ima);
+      wa = wa - IAt1 * dt1;
+      if (!wall) {
+        if (!EARLY) IBt1 = M.get3(F_IBT1, i);
+        vb = vb + t1 * (dt1 * imb);
+        wb = wb + IBt1 * dt1;
+      }
+    }
   } else {
     // Euclidean disc: one scalar step omega/g_tb for the tangential pair
     const double w_t1 = dot(t1, v_rel) + K.cfm * lt1;

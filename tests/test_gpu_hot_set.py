@@ -2,7 +2,8 @@
 equal bit for bit to the oracle's restatement (orc_nsc_solve_island: stall
 rule, hot sets), on the settled-bed island of tests/golden/settled_island.npz
 (2546 manifolds, 1838 rods: one island of the LDS class), solved from a
-cold start."""
+cold start, with the scalar (friction_proj 1) and the block (friction_proj 2)
+tangential update."""
 import os
 
 import numpy as np
@@ -16,9 +17,10 @@ pytestmark = pytest.mark.gpu
 FIX = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "settled_island.npz")
 
 
+@pytest.mark.parametrize("proj", [1, 2, 3])
 @pytest.mark.parametrize("hot", [0, 1])
-def test_lds_island_solve_bitwise_equal_oracle(gpu_ctx_factory, hot):
-    print(f"settled island solve hot_set={hot}", flush=True)
+def test_lds_island_solve_bitwise_equal_oracle(gpu_ctx_factory, hot, proj):
+    print(f"settled island solve hot_set={hot} friction_proj={proj}", flush=True)
     d = np.load(FIX)
     n = len(d["r"])
     mu = float(d["mu"])
@@ -26,7 +28,7 @@ def test_lds_island_solve_bitwise_equal_oracle(gpu_ctx_factory, hot):
     ctx.set_bodies(d["x"], d["q"], d["v_free"], d["w_free"], d["r"], d["h"], d["mass"], np.zeros(n))
     ctx.set_systems([0, n], [L.system_desc(d["lo"], d["hi"], (1, 1, 0), True, True, mu)])
     ctx.set_physics(L.physics(tuple(d["gravity"]), float(d["dt"]), 0.08, 0.12, 80.0))
-    p = L.nsc_params(beta=0.0, cfm=0.0, omega=1.3, tol=1e-8, max_sweeps=10000, check_every=8, friction_proj=1,
+    p = L.nsc_params(beta=0.0, cfm=0.0, omega=1.3, tol=1e-8, max_sweeps=10000, check_every=8, friction_proj=proj,
                      warm_mode=0, stop_rule=1, stall_sweeps=1000, stall_omega=0.0, hot_set=hot)
     ctx.detect(L.detect_params(0.0, 16.0, 1.0))
     nm = ctx.nsc_build(p)
@@ -47,8 +49,8 @@ def test_lds_island_solve_bitwise_equal_oracle(gpu_ctx_factory, hot):
     W.set_box(d["lo"], d["hi"], [1, 1, 0], 1, 1)
     W.set_physics(tuple(d["gravity"]), float(d["dt"]))
     W.set_manifolds(rec, ab, lam0)
-    op = O.nsc_params(mu=mu, beta=0.0, cfm=0.0, omega=1.3, tol=1e-8, max_sweeps=10000, check_every=8, friction_proj=1,
-                      stop_rule=1, warm_start=0)
+    op = O.nsc_params(mu=mu, beta=0.0, cfm=0.0, omega=1.3, tol=1e-8, max_sweeps=10000, check_every=8,
+                      friction_proj=proj, stop_rule=1, warm_start=0)
     sw, res, upd, hs = W.solve_island(op, order, stall_sweeps=1000, stall_omega=0.0, hot=hot)
     print(f"  GPU {st['sweeps']} sweeps (hot {hot_g}), residual {st['residual']:.3e}; oracle {sw} (hot {hs}), "
           f"{res:.3e}, {upd} updates", flush=True)
