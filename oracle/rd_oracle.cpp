@@ -737,12 +737,14 @@ static double psor_one(orc_world& W, Manifold& m, const orc_nsc_params& P) {
     // 2 projects the step in G's metric and 3 takes the scalar step of 1 from
     // the current impulse instead; both keep 1's fixed points (maximal
     // dissipation), and a sliding solution's block step always leaves the disc.
-    double w_t1 = dot(m.t1, v_rel) + cfm * m.lt1;
-    double w_t2 = dot(m.t2, v_rel) + cfm * m.lt2;
     double g12 = dot(cross(m.ra, m.t1), m.IAt2);
     if (!wall) g12 = g12 + dot(cross(m.rb, m.t1), m.IBt2);
+    // G^-1 from the manifold's fields alone (off the chain of velocity updates)
     const double det = m.gt1 * m.gt2 - g12 * g12;
-    const double s1 = (m.gt2 * w_t1 - g12 * w_t2) / det, s2 = (m.gt1 * w_t2 - g12 * w_t1) / det;
+    const double h11 = m.gt2 / det, h12 = g12 / det, h22 = m.gt1 / det;
+    double w_t1 = dot(m.t1, v_rel) + cfm * m.lt1;
+    double w_t2 = dot(m.t2, v_rel) + cfm * m.lt2;
+    const double s1 = h11 * w_t1 - h12 * w_t2, s2 = h22 * w_t2 - h12 * w_t1;
     double y1 = m.lt1 - omega * s1;
     double y2 = m.lt2 - omega * s2;
     if (y1 * y1 + y2 * y2 > max_fric * max_fric) {

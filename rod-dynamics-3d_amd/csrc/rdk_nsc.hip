@@ -615,16 +615,18 @@ __device__ __forceinline__ double psor_update(const BodyPtrs& B, MP& M, int i, c
     // 2 projects the step in G's metric and 3 takes the scalar step of 1 from
     // the current impulse instead; both keep 1's fixed points (maximal
     // dissipation), and a sliding solution's block step always leaves the disc.
-    const double w_t1 = dot(t1, v_rel) + K.cfm * lt1;
-    const double w_t2 = dot(t2, v_rel) + K.cfm * lt2;
     if (!EARLY) {
       IAt2 = M.get3(F_IAT2, i);
       if (!wall) IBt2 = M.get3(F_IBT2, i);
     }
     double g12 = dot(cross(ra, t1), IAt2);
     if (!wall) g12 = g12 + dot(cross(rb, t1), IBt2);
+    // G^-1 from the manifold's fields alone (off the chain of velocity updates)
     const double det = g1 * g2 - g12 * g12;
-    const double s1 = (g2 * w_t1 - g12 * w_t2) / det, s2 = (g1 * w_t2 - g12 * w_t1) / det;
+    const double h11 = g2 / det, h12 = g12 / det, h22 = g1 / det;
+    const double w_t1 = dot(t1, v_rel) + K.cfm * lt1;
+    const double w_t2 = dot(t2, v_rel) + K.cfm * lt2;
+    const double s1 = h11 * w_t1 - h12 * w_t2, s2 = h22 * w_t2 - h12 * w_t1;
     double y1 = lt1 - K.omega * s1;
     double y2 = lt2 - K.omega * s2;
     if (y1 * y1 + y2 * y2 > max_fric * max_fric) {

@@ -80,7 +80,12 @@ def c2_settling(n=100_000, phi=0.05, seed=SEED0 + 2, mu=0.5, aspect=20.0, gap=0.
         raise RuntimeError(f"RSA placed {placed} of {n} rods")
     mass = 1.0 * math.pi * r * r * (2.0 * h)
     # Euclidean disc projection: the reference's sequential clamp does not
-    # converge to a 1e-8 fixed-point residual on sliding beds
+    # converge to a 1e-8 fixed-point residual on sliding beds.  Tangential
+    # update friction_proj 3: a sticking contact's slip is solved with the 2x2
+    # tangential block in one step (the scalar step omega/g_tb of
+    # friction_proj 1 is short along the block's soft direction), a sliding
+    # one takes the scalar step; same fixed points (measured on the GPU at 1e5
+    # rods: 3.8x fewer sweeps per step, the whole protocol 1.5x faster).
     # Plane-wall contacts also start from the previous step's impulses
     # (the reference has no plane walls and starts its cylinder-wall contacts cold).
     # Islands whose residual no longer drops by 10 % between 1000, 2000, 4000,
@@ -93,7 +98,7 @@ def c2_settling(n=100_000, phi=0.05, seed=SEED0 + 2, mu=0.5, aspect=20.0, gap=0.
     # manifolds around them are swept (the slow stick/slip modes of a few wall
     # contacts that hold the last decades of the residual are local).
     nscp = L.nsc_params(beta=0.0, cfm=0.0, omega=1.3, tol=1e-8, max_sweeps=10000, check_every=4,
-                        friction_proj=1, position_stabilization=0, warm_mode=1, stop_rule=1, warm_walls=1,
+                        friction_proj=3, position_stabilization=0, warm_mode=1, stop_rule=1, warm_walls=1,
                         stall_sweeps=1000, stall_omega=0.0, hot_set=1)
     return Scene(name=f"C2_settling_N{n}", x=x, q=q, v=np.zeros((n, 3)), w=np.zeros((n, 3)), r=r, h=h, mass=mass,
                  rest=np.zeros(n), offsets=np.array([0, n], np.int32), descs=[desc],

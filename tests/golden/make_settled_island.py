@@ -54,7 +54,8 @@ def main():
     a = ap.parse_args()
     sc = scenes.c2_settling(n=100_000)
     p = sc.params
-    p.check_every, p.hot_set, p.stall_sweeps, p.stall_omega = 8, 0, 1000, 0.0
+    # the solve the fixture was captured with: scalar tangential step, no hot sets
+    p.check_every, p.hot_set, p.stall_sweeps, p.stall_omega, p.friction_proj = 8, 0, 1000, 0.0, 1
     det = L.detect_params(0.0, 16.0, 1.0)
     ctx = L.Context(0)
     sc.load(ctx)

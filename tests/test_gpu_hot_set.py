@@ -2,8 +2,8 @@
 equal bit for bit to the oracle's restatement (orc_nsc_solve_island: stall
 rule, hot sets), on the settled-bed island of tests/golden/settled_island.npz
 (2546 manifolds, 1838 rods: one island of the LDS class), solved from a
-cold start, with the scalar (friction_proj 1) and the block (friction_proj 2)
-tangential update."""
+cold start, with the scalar (friction_proj 1), the block (friction_proj 2)
+and the hybrid (friction_proj 3) tangential update."""
 import os
 
 import numpy as np

@@ -95,7 +95,7 @@ def test_c4_batched_systems_match_per_system_oracle(gpu_ctx_factory):
         W.set_box(list(d.lo), list(d.hi), [int(v) for v in d.periodic], 0, 0)
         W.set_physics(tuple(sc.phys.gravity[:]), sc.phys.dt)
         op = O.nsc_params(mu=d.mu, beta=0.0, cfm=0.0, omega=1.0, tol=1e-12, max_sweeps=10000, check_every=1,
-                          friction_proj=1, stop_rule=1, warm_start=1)
+                          friction_proj=p.friction_proj, stop_rule=1, warm_start=1)
         W.nsc_step(op, 40, method=1)
         so = W.state()
         # colour-order vs sorted-order Gauss-Seidel, both to 1e-12: equal up to the tolerance
