@@ -80,7 +80,7 @@ def parse():
     ap.add_argument("--warm-mode", type=int, default=None, choices=[0, 1, 2, 3, 4],
                     help="override the settling workloads' PSOR start (1 previous step, 4 learned for every "
                          "manifold from the contact alone; learned modes load --weights)")
-    ap.add_argument("--hot-set", type=int, default=None, choices=[0, 1],
+    ap.add_argument("--hot-set", type=int, default=None, choices=[0, 1, 2],
                     help="override the workload's hot-set phase of the LDS island solve (hard-contact workloads)")
     ap.add_argument("--friction-proj", type=int, default=None, choices=[0, 1, 2, 3],
                     help="override the hard-contact workloads' tangential PSOR update (1 scalar step on the "

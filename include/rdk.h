@@ -98,7 +98,11 @@ typedef struct rdk_nsc_params {
                               above tol/2 (at most 512, in the island's colour order)
                               until their residual is at most tol/2, then check the
                               whole island again (Gauss-Seidel on a subset; the stop
-                              rule is unchanged); 0 = full sweeps only */
+                              rule is unchanged); 2 = as 1, and when no such hot set is
+                              taken and at least 32 sweeps have run, rounds of
+                              check_every sweeps over the same set of any size up to
+                              half the island, each followed by the island check;
+                              0 = full sweeps only */
 } rdk_nsc_params;
 
 typedef struct rdk_soft_params { /* SoftContactCfg, config.hpp:6-27 */

@@ -1392,7 +1392,9 @@ int orc_nsc_set_manifolds(orc_world* W, int M, const double* rec, const int* ab,
 // residual exceeds tol / 2 (in solve order; skipped when more than 512, the
 // next attempt then waiting until the sweep count has doubled), in blocks of
 // check_every sweeps until their residual is <= tol / 2, then check the
-// island again.
+// island again.  hot == 2 adds selective rounds: when no hot set is taken and
+// at least 32 sweeps have run, one block of check_every sweeps over that set
+// (any size up to half the island), then the island check.
 int orc_nsc_solve_island(orc_world* W, const orc_nsc_params* P, const int* ix, int m, int stall_sweeps,
                          double stall_omega, int hot, double* res_out, long long* updates_out,
                          int* hot_sweeps_out) {
@@ -1427,6 +1429,7 @@ int orc_nsc_solve_island(orc_world* W, const orc_nsc_params* P, const int* ix, i
   std::vector<int> H;
   while (sweeps < P->max_sweeps && !(res <= P->tol)) {
     H.clear();
+    bool sel = false;
     if (hot && sweeps >= hot_next && above > 0 && above <= std::max(16, m / 50)) {
       for (int k = 0; k < m; ++k) {
         const Manifold& mf = W->mf[ix[k]];
@@ -1437,7 +1440,23 @@ int orc_nsc_solve_island(orc_world* W, const orc_nsc_params* P, const int* ix, i
         hot_next = 2 * sweeps;
       }
     }
-    if (!H.empty()) {
+    if (hot == 2 && H.empty() && sweeps >= 32) {
+      // selective round: the same set, any size up to half the island, one
+      // block of check_every sweeps, then the whole island is checked again
+      for (int k = 0; k < m; ++k) {
+        const Manifold& mf = W->mf[ix[k]];
+        if (flagged.count(mf.a) || (mf.b >= 0 && flagged.count(mf.b))) H.push_back(ix[k]);
+      }
+      if (H.empty() || 2 * (int)H.size() > m) H.clear();
+      else sel = true;
+    }
+    if (sel) {
+      const int nh = (int)H.size();
+      for (int it = 0; it < kk; ++it)
+        for (int k = 0; k < nh; ++k) psor_one(*W, W->mf[H[k]], Q);
+      sweeps += kk;
+      updates += (long long)kk * nh;
+    } else if (!H.empty()) {
       const int nh = (int)H.size();
       int ls = 0;
       while (true) {

@@ -105,7 +105,8 @@ int orc_nsc_solve(orc_world* W, const orc_nsc_params* P, const int* order,
 /* One island solved as the library's island kernels solve it (stop rule 1):
  * the island's manifolds ix[0..m) in solve order, the residual over them
  * checked every check_every sweeps, the stall rule (stall_sweeps, stall_omega;
- * 0 = off) and, when hot != 0, the hot-set phase of the LDS island class.
+ * 0 = off) and, when hot != 0, the hot-set phase of the LDS island class
+ * (hot == 2: also its selective rounds).
  * Returns the sweeps run; *res = final residual over the island, *updates =
  * manifold updates, *hot_sweeps = sweeps of hot sets. */
 int orc_nsc_solve_island(orc_world* W, const orc_nsc_params* P, const int* ix, int m, int stall_sweeps,

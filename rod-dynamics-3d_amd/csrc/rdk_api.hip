@@ -611,7 +611,7 @@ int rdk_history_size(rdk_ctx* ctx, int* n) {
 
 static int check_nsc(rdk_ctx* ctx, const rdk_nsc_params* p) {
   if (!p || !(p->omega > 0.0) || p->max_sweeps < 0 || p->friction_proj < 0 || p->friction_proj > 3 ||
-      p->warm_mode < 0 || p->warm_mode > 4 || p->hot_set < 0 || p->hot_set > 1) {
+      p->warm_mode < 0 || p->warm_mode > 4 || p->hot_set < 0 || p->hot_set > 2) {
     ctx->err = "invalid rdk_nsc_params";
     return RDK_ERR_ARG;
   }

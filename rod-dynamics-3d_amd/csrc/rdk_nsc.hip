@@ -1325,6 +1325,14 @@ constexpr int LDS_BCAP = 3360;
 // doubled.
 constexpr int HOT_MIN_SWEEPS = 64, HOT_MIN_COUNT = 16, HOT_FRAC = 50;
 constexpr double HOT_TARGET = 0.5;
+// Selective sweeps (IslandSolve::hot == 2): when no hot set is taken and at
+// least SEL_MIN_SWEEPS sweeps have run, a round of check_every sweeps covers
+// only the same set H (every manifold sharing a body with one above
+// HOT_TARGET * tol, any number up to half the island) in solve order, its
+// fields loaded from global memory as in full sweeps; the whole island is
+// checked after every round.  Late in a solve most of an island is already
+// below the tolerance and H is a fraction of it.
+constexpr int SEL_MIN_SWEEPS = 32;
 
 struct MfSlots {
   MfPtrs M;
@@ -1428,13 +1436,15 @@ __global__ void __launch_bounds__(LDS_THREADS) k_island_lds(BodyPtrs B, MfPtrs M
   if (P.stop_rule == 1) res = check();
   const int kk = P.stop_rule == 1 ? P.check_every : 1;
   while (sweeps < P.max_sweeps && !(P.stop_rule == 1 && res <= P.tol)) {
-    // ---- hot set (block-uniform decision from the last check)
+    // ---- hot set or selective round (block-uniform decision from the last check)
     int nh = 0;
+    bool sel = false;
     if (hot_on) {
       const int above = s_above;
       const bool want = sweeps >= hot_next && above > 0 && above <= max(HOT_MIN_COUNT, m / HOT_FRAC);
+      const bool want_sel = P.hot == 2 && sweeps >= SEL_MIN_SWEEPS;
       __syncthreads();  // everyone has read s_above
-      if (want) {
+      if (want || want_sel) {
         // H = manifolds with a marked body, in solve (storage) order
         const int lane = tid & 63, wid = tid >> 6;
         for (int c0 = s; c0 < e; c0 += LDS_THREADS) {
@@ -1454,14 +1464,18 @@ __global__ void __launch_bounds__(LDS_THREADS) k_island_lds(BodyPtrs B, MfPtrs M
             tot += s_wcnt[w];
           }
           const int pos = nh + woff + __popcll(bal & ((1ull << lane) - 1ull));
-          if (in && pos < LDS_THREADS) hlist[s + pos] = i;
+          if (in) hlist[s + pos] = i;
           nh += tot;
           __syncthreads();  // s_wcnt is reused
         }
-        if (nh > LDS_THREADS || nh == 0) {
-          nh = 0;
-          hot_next = 2 * sweeps;
-        } else {
+        if (!(want && nh <= LDS_THREADS && nh > 0)) {
+          // no hot set: a selective round over the same H when it is at most
+          // half the island, else full sweeps
+          if (want) hot_next = 2 * sweeps;
+          sel = want_sel && nh > 0 && 2 * nh <= m;
+          if (!sel) nh = 0;
+        }
+        if (nh > 0) {
           // colour bounds of H as positions 0..nh, empty colours dropped
           for (int c = tid; c <= ncol; c += LDS_THREADS) hcs[c] = lb_int(hlist + s, nh, cs[c]);
           __syncthreads();
@@ -1478,17 +1492,19 @@ __global__ void __launch_bounds__(LDS_THREADS) k_island_lds(BodyPtrs B, MfPtrs M
       if (tid == 0) s_above = 0;
       __syncthreads();
     }
-    // ---- sweeps: the whole island (positions = manifold indices s..e) or the
-    // hot set (positions 0..nh, manifold hlist[s + position])
-    const bool hot = nh > 0;
-    const int* bnd = hot ? hcs : cs;
-    const int nbnd = hot ? s_wcnt[0] : ncol;
+    // ---- sweeps: the whole island (positions = manifold indices s..e), the
+    // hot set or a selective round (positions 0..nh, manifold hlist[s + position])
+    const bool hot = nh > 0 && !sel;
+    const int* bnd = nh > 0 ? hcs : cs;
+    const int nbnd = nh > 0 ? s_wcnt[0] : ncol;
     const int* hl = hlist + s;
+    auto mfi = [&](int q) { return sel ? hl[q] : q; };  // position -> manifold (full sweeps, selective)
     int ls = 0;
     double d = 0.0;
-    // Full sweeps: lane tid takes positions bnd[c] + tid, bnd[c] + tid + 512, ...
-    // of each colour.  Hot set (at most 512 positions): lane tid keeps
-    // position tid, whose manifold stays in its registers for the round.
+    // Full sweeps and selective rounds: lane tid takes positions bnd[c] + tid,
+    // bnd[c] + tid + 512, ... of each colour.  Hot set (at most 512 positions):
+    // lane tid keeps position tid, whose manifold stays in its registers for
+    // the round.
     MfReg R;
     if (hot && tid < nh) load_manifold(R, A, hl[tid]);
     while (true) {
@@ -1499,26 +1515,29 @@ __global__ void __launch_bounds__(LDS_THREADS) k_island_lds(BodyPtrs B, MfPtrs M
         // impulses are not written by other colours; the body velocities it
         // needs are read from LDS after the barrier), so the global-memory
         // latency overlaps the wait.
-        if (!hot && bnd[0] + tid < bnd[1]) load_manifold(R, A, bnd[0] + tid);
+        if (!hot && bnd[0] + tid < bnd[1]) load_manifold(R, A, mfi(bnd[0] + tid));
         for (int c = 0; c < nbnd; ++c) {
           int q = hot ? tid : bnd[c] + tid;
           if (hot ? (tid >= bnd[c] && tid < bnd[c + 1]) : q < bnd[c + 1]) {
             d = dmax(d, psor_update(LB, R, 0, K, mu));
             // a hot-set lane keeps its impulses in registers until the round
             // ends: no global store waits at the colour barrier
-            if (!hot)
-              for (int t = 0; t < 3; ++t) A.lr(t, q) = R.lam[t];
+            if (!hot) {
+              const int i = mfi(q);
+              for (int t = 0; t < 3; ++t) A.lr(t, i) = R.lam[t];
+            }
             q += LDS_THREADS;
           }
           // further manifolds of this lane reuse the same registers: all fields
           // in one round of loads, then the update
           if (!hot) {
             for (; q < bnd[c + 1]; q += LDS_THREADS) {
-              load_manifold(R, A, q);
+              const int i = mfi(q);
+              load_manifold(R, A, i);
               d = dmax(d, psor_update(LB, R, 0, K, mu));
-              for (int t = 0; t < 3; ++t) A.lr(t, q) = R.lam[t];
+              for (int t = 0; t < 3; ++t) A.lr(t, i) = R.lam[t];
             }
-            if (c + 1 < nbnd && bnd[c + 1] + tid < bnd[c + 2]) load_manifold(R, A, bnd[c + 1] + tid);
+            if (c + 1 < nbnd && bnd[c + 1] + tid < bnd[c + 2]) load_manifold(R, A, mfi(bnd[c + 1] + tid));
           }
           __syncthreads();
         }
@@ -1535,7 +1554,7 @@ __global__ void __launch_bounds__(LDS_THREADS) k_island_lds(BodyPtrs B, MfPtrs M
       __syncthreads();  // stored before the island check reads them
     }
     sweeps += ls;
-    updates += (unsigned long long)ls * (unsigned long long)(hot ? nh : m);
+    updates += (unsigned long long)ls * (unsigned long long)(nh > 0 ? nh : m);
     if (hot) {
       hot_sweeps += (unsigned long long)ls;
     } else {

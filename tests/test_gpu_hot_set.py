@@ -1,6 +1,7 @@
-"""LDS island solve to the tolerance, with and without its hot-set phase,
-equal bit for bit to the oracle's restatement (orc_nsc_solve_island: stall
-rule, hot sets), on the settled-bed island of tests/golden/settled_island.npz
+"""LDS island solve to the tolerance, with and without its hot-set phase and
+selective rounds, equal bit for bit to the oracle's restatement
+(orc_nsc_solve_island: stall rule, hot sets, selective rounds; sweeps, hot
+sweeps and manifold updates counted alike), on the settled-bed island of tests/golden/settled_island.npz
 (2546 manifolds, 1838 rods: one island of the LDS class), solved from a
 cold start, with the scalar (friction_proj 1), the block (friction_proj 2)
 and the hybrid (friction_proj 3) tangential update."""
@@ -18,7 +19,7 @@ FIX = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "settle
 
 
 @pytest.mark.parametrize("proj", [1, 2, 3])
-@pytest.mark.parametrize("hot", [0, 1])
+@pytest.mark.parametrize("hot", [0, 1, 2])
 def test_lds_island_solve_bitwise_equal_oracle(gpu_ctx_factory, hot, proj):
     print(f"settled island solve hot_set={hot} friction_proj={proj}", flush=True)
     d = np.load(FIX)
@@ -36,14 +37,16 @@ def test_lds_island_solve_bitwise_equal_oracle(gpu_ctx_factory, hot, proj):
     rec, ab, lam0 = ctx.nsc_manifolds(nm)
     assert not lam0.any()
     s0 = ctx.download_state()
-    hot0 = ctx.work_counters()["hot_sweeps"]
+    wc0 = ctx.work_counters()
     st = ctx.nsc_solve(p, allow_noconv=True)
     nisl, counts = ctx.island_info()
     assert nisl == 1 and counts == [0, 0, 0, 1], counts  # one island, LDS class
     order = ctx.solve_order(nm)
     _, _, lam_g = ctx.nsc_manifolds(nm)
     s1 = ctx.download_state()
-    hot_g = ctx.work_counters()["hot_sweeps"] - hot0
+    wc1 = ctx.work_counters()
+    hot_g = wc1["hot_sweeps"] - wc0["hot_sweeps"]
+    upd_g = wc1["lds_manifold_sweeps"] - wc0["lds_manifold_sweeps"]
 
     W = O.World(d["x"], d["q"], s0["v"], s0["w"], d["r"], d["h"], d["mass"])
     W.set_box(d["lo"], d["hi"], [1, 1, 0], 1, 1)
@@ -54,7 +57,7 @@ def test_lds_island_solve_bitwise_equal_oracle(gpu_ctx_factory, hot, proj):
     sw, res, upd, hs = W.solve_island(op, order, stall_sweeps=1000, stall_omega=0.0, hot=hot)
     print(f"  GPU {st['sweeps']} sweeps (hot {hot_g}), residual {st['residual']:.3e}; oracle {sw} (hot {hs}), "
           f"{res:.3e}, {upd} updates", flush=True)
-    assert st["sweeps"] == sw and hot_g == hs
+    assert st["sweeps"] == sw and hot_g == hs and upd_g == upd
     assert st["residual"] == res
     _, _, lam_o = W.manifolds()
     so = W.state()
@@ -62,6 +65,12 @@ def test_lds_island_solve_bitwise_equal_oracle(gpu_ctx_factory, hot, proj):
     assert np.array_equal(s1["v"], so["v"]) and np.array_equal(s1["w"], so["w"])
     if hot:
         assert hs > 0 and res <= 1e-8  # the phase ran and the island converged
+    if hot == 2:
+        # selective rounds ran: fewer updates than the same solve with hot sets only
+        W.set_state(v=s0["v"], w=s0["w"])
+        W.set_manifolds(rec, ab, lam0)
+        _, _, upd1, _ = W.solve_island(op, order, stall_sweeps=1000, stall_omega=0.0, hot=1)
+        assert upd < upd1, (upd, upd1)
 
 
 def _solve_from(d, mode, weights=None):
