@@ -87,6 +87,13 @@ def parse():
                          "Euclidean disc, 2 block step projected in the tangential block's metric)")
     ap.add_argument("--stall-sweeps", type=int, default=None,
                     help="override the workload's stall-rule window (hard-contact workloads)")
+    ap.add_argument("--step-graph", type=int, default=None, choices=[0, 1, 2],
+                    help="hard-contact step: 0 eager (counts read back to the host), 1 fixed-capacity launches, "
+                         "2 the fixed step captured once in a HIP graph and replayed (default 2 on the batched "
+                         "workload)")
+    ap.add_argument("--roof-steps", type=int, default=50,
+                    help="batched workload with a captured step: eager steps after the timed run in which the "
+                         "dominant kernel is timed with HIP events")
     ap.add_argument("--block", type=int, default=250)
     ap.add_argument("--protocol-steps", type=int, default=PROTOCOL_STEPS,
                     help="settling workloads: length of the protocol the timed blocks are spread over")
@@ -638,7 +645,15 @@ def run_c4(a, L, scenes):
                                 (2.0, 4.0, 8.0, 16.0), reps=3)
     det = L.detect_params(0.0, best, 1.0)
     dom = rank_kernels(ctx, lambda: ctx.step_nsc(det, p, a.warmup))
-    log(f"warmup {a.warmup} steps done; dominant kernel {dom}; timing {a.steps} steps")
+    mode = 2 if a.step_graph is None else a.step_graph
+    if mode != 0:
+        # captured steps cannot be split into kernels by events: the dominant
+        # kernel is timed in eager steps after the timed run
+        ctx.set_profiling(False)
+        ctx.set_step_graph(mode)
+        ctx.step_nsc(det, p, 1)  # capture (or first fixed step) outside the timed region
+        ctx.sync()
+    log(f"warmup {a.warmup} steps done; dominant kernel {dom}; timing {a.steps} steps (step mode {mode})")
     st = None
     done = 0
     t1 = time.perf_counter()
@@ -656,6 +671,20 @@ def run_c4(a, L, scenes):
     ctx.sync()
     el = time.perf_counter() - t1
     tuned_steps = used
+    ginfo = ctx.step_graph_info()
+    roof_note = "timed launches inside the timed run"
+    eager_ms = None
+    if mode != 0:
+        ctx.set_step_graph(0)
+        names = [ctx._L.rdk_kernel_name(k).decode() for k in range(ctx._L.rdk_num_kernels())]
+        ctx.set_profiling(True)
+        ctx.set_profile_mask(1 << names.index(dom))
+        t2 = time.perf_counter()
+        ctx.step_nsc(det, p, a.roof_steps)
+        ctx.sync()
+        eager_ms = 1e3 * (time.perf_counter() - t2) / max(a.roof_steps, 1)
+        roof_note = (f"timed launches of {a.roof_steps} eager steps after the timed run (same kernels; a captured "
+                     f"step cannot be split into kernels by events)")
     out = base_line("rod-steps/s at PSOR residual 1e-8", sc.n * a.steps / el, a, el, "f64", {
         "workload": "batched hard-contact packings (C4)", "systems": len(sc.descs), "rods": sc.n,
         "N": "floor(exp(U[ln500, ln20000]))", "L_over_d": "{4,8,16,32,64}", "mu": "U[0,1]",
@@ -663,10 +692,16 @@ def run_c4(a, L, scenes):
         "restitution": sc.extra["restitution"], "kT": sc.extra["kT"], "gravity": 0.0, "omega": 1.3,
         "residual_tol": 1e-8, "dt": 1e-3, "redraws": sc.extra["redraws"], "seed": sc.extra["seed"],
         "chunk_len_autotuned": best, "autotune_ms_per_step": tune,
-        "steps_total": tuned_steps + a.warmup + a.steps, "stall_sweeps": p.stall_sweeps,
+        "steps_total": tuned_steps + a.warmup + a.steps + (1 + a.roof_steps if mode != 0 else 0),
+        "stall_sweeps": p.stall_sweeps,
         "stall_omega": p.stall_omega,
-        "hipgraph": "the step reads contact and island counts back to size its launches and is not captured; "
-                    "huge islands (none here) replay captured check blocks"})
+        "step_mode": mode,
+        "hipgraph": ("each step replays one captured fixed-capacity step (every count on the device)"
+                     if mode == 2 else "fixed-capacity launches, not captured" if mode == 1 else
+                     "eager: the step reads contact and island counts back to size its launches"),
+        "fixed_steps": ginfo["fixed_steps"], "fallback_steps": ginfo["fallbacks"],
+        "graph_captures": ginfo["captures"], "eager_ms_per_step_after": eager_ms,
+        "roofline_sample": roof_note})
     out["psor"] = {"sweeps_per_step": st["sweeps_total"] / max(st["steps"], 1),
                    "manifolds_per_step": st["manifolds_total"] / max(st["steps"], 1),
                    "residual_max": st["residual_max"], "nonconverged_steps": st["nonconverged_steps"]}

@@ -323,6 +323,36 @@ int rdk_entanglement(rdk_ctx* ctx, double cutoff, double* sums, int64_t* pairs);
 #define RDK_NOALLOC_SCRATCH_BYTES (4u << 20)
 int rdk_entanglement_noalloc(rdk_ctx* ctx, double cutoff, double* d_sums, uint64_t* d_pairs);
 
+/* ---- fixed-capacity, graph-captured hard-contact step ------------------
+ * No counterpart in the reference, whose step reads counts back to the host
+ * between stages (main.cpp:4333-4858, cuda_broadphase.cu:704-722).  Mode 0
+ * (default): rdk_step_nsc sizes every stage from counts read back to the
+ * host.  Mode 1: the same step with every launch sized for reserved
+ * capacities and every count read on the device (no host synchronisation
+ * inside a step).  Mode 2: mode 1 captured once into a HIP graph and replayed
+ * for each step (recaptured when the parameters or buffers change).  Results
+ * are bit-identical to mode 0.  pair_capacity = rod-rod contacts reserved per
+ * step (0: twice the most that a mode-0 step of this context has found, at
+ * least 1024; the first step of a context runs in mode 0 to measure it); wall
+ * manifolds reserved: twice the most found, at least 256.  A step the fixed
+ * path cannot finish (more contacts or wall manifolds than reserved,
+ * colouring beyond its enqueued rounds, an island with more bodies than one
+ * workgroup's LDS holds or more than 8192 manifolds) is rolled back to its
+ * start and redone by the mode-0 step, which raises the capacities; the fixed
+ * path then resumes.  Modes 1 and 2 apply when
+ * position stabilisation, stage energies, kept manifolds and learned starts
+ * are off and the island solver is on; otherwise rdk_step_nsc runs mode 0. */
+int rdk_set_step_graph(rdk_ctx* ctx, int mode, int pair_capacity);
+/* Launch-shape tuning (no reference counterpart; results do not depend on
+ * it).  RDK_TUNE_SMALL_M: manifold counts up to `value` are coloured (and on
+ * the global path solved) by one workgroup, larger ones by rounds over the
+ * whole GPU; -1 restores the default (16384, or the RDK_SMALL_M environment
+ * variable). */
+#define RDK_TUNE_SMALL_M 0
+int rdk_set_tuning(rdk_ctx* ctx, int what, int value);
+/* steps run by the fixed path, steps redone by the eager step, graph captures */
+int rdk_step_graph_info(rdk_ctx* ctx, int64_t* fixed_steps, int64_t* fallbacks, int64_t* captures);
+
 /* ---- profiling: per-kernel HIP-event timing on the stream each kernel runs on */
 int rdk_set_profiling(rdk_ctx* ctx, int on);
 /* time only the kernels whose id bit is set (bit k = kernel id k).  The
@@ -343,6 +373,19 @@ int rdk_num_kernels(void);
  * calls, manifold updates in timed colour launches, manifold updates in
  * sweeps of the LDS island class, hot-set sweeps (summed over islands) */
 int rdk_work_counters(rdk_ctx* ctx, int64_t* counters, int n);
+
+/* ---- the library's device sort and scan on host arrays ---------------------
+ * No counterpart in the reference (it sorts with std::sort,
+ * contact_detector.cpp:780-793, nsc_contact.cpp:87-102): these run the stable
+ * radix sorts and the exclusive scan that every stage of this library uses,
+ * so they can be checked on their own.  When cap > n the launches are sized
+ * for cap and read n from device memory, as in a captured step. */
+/* stable sort of n (key, value) pairs by keys below 2^key_bits, in place */
+int rdk_sort_pairs_u32(rdk_ctx* ctx, int n, int cap, uint32_t* keys, int32_t* vals, int key_bits);
+/* sort of n pair keys (lo << 32) | hi with lo, hi < 2^index_bits, in place */
+int rdk_sort_pair_keys_u64(rdk_ctx* ctx, int n, int cap, uint64_t* keys, int index_bits);
+/* out[i] = in[0] + ... + in[i-1] */
+int rdk_exclusive_scan_i32(rdk_ctx* ctx, int n, int cap, const int32_t* in, int32_t* out);
 
 /* ---- scene generation (resetScene populate/randomInit,
  *      main.cpp:1857-2140,2243-2263) ------------------------------------- */

@@ -160,6 +160,8 @@ int rdk_create(int device, rdk_ctx** out) {
   return RDK_OK;
 }
 
+static void drop_step_graph(rdk_ctx* ctx);
+
 int rdk_destroy(rdk_ctx* ctx) {
   if (!ctx) return RDK_OK;
   (void)hipSetDevice(ctx->device);
@@ -193,7 +195,9 @@ int rdk_destroy(rdk_ctx* ctx) {
   for (int k = 0; k < 3; ++k) { release(ctx->ent_a[k]); release(ctx->ent_e[k]); }
   release(ctx->ent_out); release(ctx->ent_psum); release(ctx->ent_pcnt); release(ctx->hres); release(ctx->hstate); release(ctx->raw_ptrs); release(ctx->raw_out); release(ctx->kept_mf); release(ctx->kept_ab); release(ctx->kept_lam); release(ctx->soft_kept); release(ctx->islot); release(ctx->ihot); release(ctx->prev_keys); release(ctx->prev_wkeys); release(ctx->resid);
   release(ctx->hkeys); release(ctx->hkeys_new); release(ctx->hlast); release(ctx->hlast_new);
-  release(ctx->hflag); release(ctx->hscan); release(ctx->pe); release(ctx->temp); release(ctx->red);
+  release(ctx->hflag); release(ctx->hscan); release(ctx->pe); release(ctx->temp); release(ctx->scan_tmp); release(ctx->red);
+  drop_step_graph(ctx);
+  release(ctx->fx); release(ctx->facc);
   if (ctx->h_counters) (void)hipHostFree(ctx->h_counters);
   for (auto& r : ctx->ev_pending) {
     if (r.a) (void)hipEventDestroy(r.a);
@@ -865,6 +869,238 @@ int rdk_get_soft_forces(rdk_ctx* ctx, int cap, double* out, int* count) {
   return RDK_OK;
 }
 
+// One eager hard-contact step (App::physicsStep, main.cpp:4333-4858): every
+// stage sized from counts read back to the host.
+static int eager_nsc_step(rdk_ctx* ctx, const rdk_detect_params* d, const rdk_nsc_params* np, rdk_step_stats& S) {
+  ctx->eager_nsc_steps++;
+  RDK_TRY(integrate_run(ctx, RDK_NSC_FREE));
+  if (ctx->stage_ke) RDK_TRY(observables_run(ctx, &ctx->ke_stage[0], nullptr));  // main.cpp:4411
+  RDK_TRY(detect_run(ctx, d));
+  RDK_TRY(nsc_build_run(ctx, np));
+  rdk_solve_stats ss;
+  std::memset(&ss, 0, sizeof(ss));
+  int rc = nsc_solve_run(ctx, np, &ss);
+  if (rc == RDK_ERR_NOCONV) S.nonconverged_steps++;
+  else if (rc != RDK_OK) return rc;
+  if (ctx->keep_manifolds) RDK_TRY(keep_manifolds(ctx));
+  if (ctx->stage_ke) RDK_TRY(observables_run(ctx, &ctx->ke_stage[1], nullptr));  // main.cpp:4461
+  RDK_TRY(integrate_run(ctx, RDK_NSC_POS));
+  if (np->position_stabilization && np->position_iters > 0)
+    RDK_TRY(nsc_project_run(ctx, d, np, nullptr, nullptr));  // main.cpp:4497-4503
+  if (ctx->stage_ke) RDK_TRY(observables_run(ctx, &ctx->ke_stage[2], nullptr));  // main.cpp:4505,4519
+  S.steps++;
+  S.contacts_total += ss.rod_contacts;
+  S.manifolds_total += ss.manifolds;
+  S.sweeps_total += ss.sweeps;
+  S.colors_total += ss.colors;
+  S.residual_max = std::max(S.residual_max, ss.residual);
+  ctx->work[W_STEPS] += 1;
+  if (ctx->profiling) RDK_TRY(prof_flush(ctx));
+  return RDK_OK;
+}
+
+// ---- fixed-capacity step (no reference counterpart) -------------------------
+// The step of eager_nsc_step with every launch sized for reserved capacities
+// and every count read on the device (detect_fixed_enqueue,
+// nsc_fixed_enqueue), so it can be captured once into a graph and replayed.
+// A step that fails
+// (see FxErr) does so before its first state change and makes every later
+// state-changing kernel return; the host then runs that step eagerly (which
+// regrows what it must) and goes on with the fixed path.
+
+static bool fixed_supported(rdk_ctx* ctx, const rdk_nsc_params* np) {
+  return !(np->position_stabilization && np->position_iters > 0) && np->warm_mode <= 1 && ctx->solver_mode == 0 &&
+         !ctx->stage_ke && !ctx->keep_manifolds && ctx->n < (1 << 25);
+}
+
+static void drop_step_graph(rdk_ctx* ctx) {
+  if (ctx->step_exec) (void)hipGraphExecDestroy(ctx->step_exec);
+  if (ctx->step_graph) (void)hipGraphDestroy(ctx->step_graph);
+  ctx->step_exec = nullptr;
+  ctx->step_graph = nullptr;
+  ctx->step_graph_key.clear();
+}
+
+// One fixed step: detection (positions only), then nsc_fixed_enqueue (checks,
+// free flight, manifolds, solve, warm cache), then the position update.  The
+// free-flight update does not move the rods, so detecting first finds the
+// contacts of the eager step's order (free flight, then detection).
+static int fixed_enqueue_step(rdk_ctx* ctx, const rdk_detect_params* d, const rdk_nsc_params* np) {
+  RDK_TRY(fixed_clear_enqueue(ctx));
+  RDK_TRY(detect_fixed_enqueue(ctx, d));
+  RDK_TRY(nsc_fixed_enqueue(ctx, np));
+  RDK_TRY(integrate_run(ctx, RDK_NSC_POS, ctx->fx.p + FX_ERR));
+  return RDK_OK;
+}
+
+// the parameters a captured step bakes in
+static std::vector<char> step_key(rdk_ctx* ctx, const rdk_detect_params* d, const rdk_nsc_params* np) {
+  std::vector<char> k(sizeof(*d) + sizeof(*np) + sizeof(ctx->phys) + 12 * sizeof(int));
+  char* o = k.data();
+  std::memcpy(o, d, sizeof(*d));
+  o += sizeof(*d);
+  std::memcpy(o, np, sizeof(*np));
+  o += sizeof(*np);
+  std::memcpy(o, &ctx->phys, sizeof(ctx->phys));
+  o += sizeof(ctx->phys);
+  // the class grids, and which classes have held islands (the streams they run on)
+  int seen = 0;
+  for (int c = 0; c < 4; ++c) seen |= (ctx->cls_seen_max[c] > 0 ? 1 : 0) << c;
+  const int v[12] = {ctx->n,         ctx->fx_pair_cap, ctx->fx_rounds,  ctx->nsys,       ctx->n_cells,    ctx->n_chunks,
+                     ctx->fx_wall_cap, ctx->fx_grid[0],  ctx->fx_grid[1], ctx->fx_grid[2], ctx->fx_grid[3], seen};
+  std::memcpy(o, v, sizeof(v));
+  return k;
+}
+
+static int fixed_capture(rdk_ctx* ctx, const rdk_detect_params* d, const rdk_nsc_params* np) {
+  drop_step_graph(ctx);
+  int64_t work[W_NUM];
+  std::memcpy(work, ctx->work, sizeof(work));
+  ctx->capturing = true;
+  int rc = RDK_OK;
+  if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    ctx->capturing = false;
+    ctx->err = "step capture: hipStreamBeginCapture failed";
+    return RDK_ERR_HIP;
+  }
+  rc = fixed_enqueue_step(ctx, d, np);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+  ctx->capturing = false;
+  std::memcpy(ctx->work, work, sizeof(work));
+  if (rc != RDK_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess || !g) {
+    ctx->err = std::string("step capture: hipStreamEndCapture: ") + hipGetErrorString(e);
+    return RDK_ERR_HIP;
+  }
+  ctx->step_graph = g;
+  if (hipGraphInstantiate(&ctx->step_exec, g, nullptr, nullptr, 0) != hipSuccess) {
+    drop_step_graph(ctx);
+    ctx->err = "step capture: hipGraphInstantiate failed";
+    return RDK_ERR_HIP;
+  }
+  ctx->step_graph_key = step_key(ctx, d, np);
+  ctx->step_graph_gen = ctx->alloc_gen;
+  ctx->graph_captures++;
+  return RDK_OK;
+}
+
+// Host warm-cache sizes -> device words (before fixed steps follow eager ones).
+static int fixed_upload_warm_counts(rdk_ctx* ctx) {
+  const int v[2] = {ctx->nprev, ctx->nprev_w};
+  RDK_HIP(hipMemcpyAsync(ctx->fx.p + FX_NPREV, v, sizeof(v), hipMemcpyHostToDevice, ctx->stream));
+  RDK_HIP(hipStreamSynchronize(ctx->stream));
+  return RDK_OK;
+}
+
+static int step_nsc_fixed(rdk_ctx* ctx, const rdk_detect_params* d, const rdk_nsc_params* np, int nsteps,
+                          rdk_step_stats& S) {
+  const bool graph = ctx->step_graph_mode == 2;
+  const int chunk_max = graph ? 16 : 1;
+  int done = 0;
+  bool uploaded = false;
+  int backoff = 0, eager_left = 0;  // steps left to run eagerly after a failed fixed step
+  while (done < nsteps) {
+    if (eager_left > 0) {  // repeated failures (e.g. a percolating island): stay eager for a while
+      RDK_TRY(eager_nsc_step(ctx, d, np, S));
+      --eager_left;
+      ++done;
+      continue;
+    }
+    if (ctx->eager_nsc_steps == 0) {  // one eager step measures the counts the capacities follow
+      RDK_TRY(eager_nsc_step(ctx, d, np, S));
+      ++done;
+      continue;
+    }
+    // capacities: twice the most contacts (wall manifolds) an eager step of
+    // this context has built (at least 1024 / 256), or the requested pair
+    // capacity; a step that needs more is redone eagerly, which raises them
+    auto pow2 = [](int64_t v) {
+      int64_t p = 1;
+      while (p < v) p <<= 1;
+      return (int)std::min<int64_t>(p, 1 << 30);
+    };
+    const int cap = ctx->step_graph_pair_cap > 0 ? ctx->step_graph_pair_cap
+                                                 : pow2(std::max(1024, 2 * ctx->nc_seen_max));
+    const int wcap = pow2(std::max(256, 2 * ctx->nwall_seen_max));
+    int grids[4];
+    fixed_class_grids(ctx, grids);
+    if (cap != ctx->fx_pair_cap || wcap != ctx->fx_wall_cap || !ctx->fx.p || ctx->n_chunks <= 0 ||
+        ctx->grid_dirty || d->chunk_len != ctx->built_chunk_len || 2 * ctx->color_rounds_max + 4 > ctx->fx_rounds ||
+        std::memcmp(grids, ctx->fx_grid, sizeof(grids)) != 0) {
+      RDK_TRY(fixed_prepare(ctx, d, np, cap, wcap));
+    }
+    if (!uploaded) {
+      RDK_TRY(fixed_upload_warm_counts(ctx));
+      uploaded = true;
+    }
+    RDK_HIP(hipMemsetAsync(ctx->fx.p, 0, 2 * sizeof(int), ctx->stream));  // FX_ERR, FX_STEPS
+    RDK_HIP(hipMemsetAsync(ctx->facc.p, 0, FA_NUM * sizeof(double), ctx->stream));
+    const int chunk = std::min(chunk_max, nsteps - done);
+    if (graph) {
+      if (!ctx->step_exec || ctx->step_graph_gen != ctx->alloc_gen || ctx->step_graph_key != step_key(ctx, d, np))
+        RDK_TRY(fixed_capture(ctx, d, np));
+      for (int k = 0; k < chunk; ++k) RDK_HIP(hipGraphLaunch(ctx->step_exec, ctx->stream));
+    } else {
+      int64_t work[W_NUM];
+      std::memcpy(work, ctx->work, sizeof(work));
+      ctx->capturing = true;  // same rules as a capture: no allocation, no timing
+      const int rc = fixed_enqueue_step(ctx, d, np);
+      ctx->capturing = false;
+      std::memcpy(ctx->work, work, sizeof(work));
+      RDK_TRY(rc);
+    }
+    int fh[2];
+    double acc[FA_NUM];
+    RDK_HIP(hipMemcpyAsync(fh, ctx->fx.p, sizeof(fh), hipMemcpyDeviceToHost, ctx->stream));
+    RDK_HIP(hipMemcpyAsync(acc, ctx->facc.p, sizeof(acc), hipMemcpyDeviceToHost, ctx->stream));
+    RDK_HIP(hipStreamSynchronize(ctx->stream));
+    const int ok = fh[1];
+    S.steps += ok;
+    S.contacts_total += (int64_t)acc[FA_CONTACTS];
+    S.manifolds_total += (int64_t)acc[FA_MANIFOLDS];
+    S.sweeps_total += (int64_t)acc[FA_SWEEPS];
+    S.colors_total += (int64_t)acc[FA_COLORS];
+    S.nonconverged_steps += (int64_t)acc[FA_NONCONV];
+    if (ok > 0) S.residual_max = std::max(S.residual_max, acc[FA_RESMAX]);
+    ctx->work[W_STEPS] += ok;
+    ctx->work[W_BODY_PASSES] += ok;
+    ctx->work[W_CHUNKS] += (int64_t)ok * ctx->n_chunks;
+    ctx->work[W_CANDIDATES] += (int64_t)acc[FA_CANDIDATES];
+    ctx->work[W_NARROW] += (int64_t)acc[FA_NARROW];
+    ctx->work[W_CONTACTS] += (int64_t)acc[FA_CONTACTS];
+    ctx->work[W_MANIFOLDS] += (int64_t)acc[FA_MANIFOLDS];
+    ctx->work[W_SWEEP_CONTACTS] += (int64_t)acc[FA_MSWEEPS];
+    ctx->work[W_LDS_SWEEPS] += (int64_t)acc[FA_LDS_SWEEPS];
+    ctx->work[W_HOT_SWEEPS] += (int64_t)acc[FA_HOT_SWEEPS];
+    ctx->work[W_INTEGRATE_BODIES] += (int64_t)2 * ok * ctx->n;
+    for (int c = 0; c < 4; ++c) ctx->cls_seen_max[c] = std::max(ctx->cls_seen_max[c], (int)acc[FA_CLS0 + c]);
+    ctx->graph_steps += ok;
+    done += ok;
+    if (fh[0] != 0) {
+      // the failed step changed no state: redo it eagerly (regrowing what it must)
+
+      RDK_TRY(fixed_sync_host_counts(ctx));
+      RDK_TRY(eager_nsc_step(ctx, d, np, S));
+      ctx->graph_fallbacks++;
+      done += 1;
+      uploaded = false;
+      eager_left = backoff;
+      backoff = std::min(256, std::max(1, 2 * backoff));
+    } else if (ok != chunk) {
+      ctx->err = "fixed step: fewer steps completed than launched";
+      return RDK_ERR_STATE;
+    } else {
+      backoff = 0;
+    }
+  }
+  if (uploaded) RDK_TRY(fixed_sync_host_counts(ctx));
+  return RDK_OK;
+}
+
 int rdk_step_nsc(rdk_ctx* ctx, const rdk_detect_params* dp, const rdk_nsc_params* np, int nsteps,
                  rdk_step_stats* st) {
   CTX_CHECK();
@@ -875,33 +1111,42 @@ int rdk_step_nsc(rdk_ctx* ctx, const rdk_detect_params* dp, const rdk_nsc_params
   d.delta = 0.0;  // NSC admits contacts at strict overlap (nsc_contact.cpp:24)
   rdk_step_stats S;
   std::memset(&S, 0, sizeof(S));
-  for (int s = 0; s < nsteps; ++s) {
-    RDK_TRY(integrate_run(ctx, RDK_NSC_FREE));
-    if (ctx->stage_ke) RDK_TRY(observables_run(ctx, &ctx->ke_stage[0], nullptr));  // main.cpp:4411
-    RDK_TRY(detect_run(ctx, &d));
-    RDK_TRY(nsc_build_run(ctx, np));
-    rdk_solve_stats ss;
-    std::memset(&ss, 0, sizeof(ss));
-    int rc = nsc_solve_run(ctx, np, &ss);
-    if (rc == RDK_ERR_NOCONV) S.nonconverged_steps++;
-    else if (rc != RDK_OK) return rc;
-    if (ctx->keep_manifolds) RDK_TRY(keep_manifolds(ctx));
-    if (ctx->stage_ke) RDK_TRY(observables_run(ctx, &ctx->ke_stage[1], nullptr));  // main.cpp:4461
-    RDK_TRY(integrate_run(ctx, RDK_NSC_POS));
-    if (np->position_stabilization && np->position_iters > 0)
-      RDK_TRY(nsc_project_run(ctx, &d, np, nullptr, nullptr));  // main.cpp:4497-4503
-    if (ctx->stage_ke) RDK_TRY(observables_run(ctx, &ctx->ke_stage[2], nullptr));  // main.cpp:4505,4519
-    S.steps++;
-    S.contacts_total += ss.rod_contacts;
-    S.manifolds_total += ss.manifolds;
-    S.sweeps_total += ss.sweeps;
-    S.colors_total += ss.colors;
-    S.residual_max = std::max(S.residual_max, ss.residual);
-    ctx->work[W_STEPS] += 1;
-    if (ctx->profiling) RDK_TRY(prof_flush(ctx));
+  if (ctx->step_graph_mode != 0 && fixed_supported(ctx, np) && nsteps > 0) {
+    RDK_TRY(step_nsc_fixed(ctx, &d, np, nsteps, S));
+  } else {
+    for (int s = 0; s < nsteps; ++s) RDK_TRY(eager_nsc_step(ctx, &d, np, S));
   }
   RDK_TRY(prof_flush(ctx));
   if (st) *st = S;
+  return RDK_OK;
+}
+
+int rdk_set_tuning(rdk_ctx* ctx, int what, int value) {
+  CTX_CHECK();
+  if (what == RDK_TUNE_SMALL_M) {
+    if (value < -1) return RDK_ERR_ARG;
+    ctx->small_m = value;
+    drop_step_graph(ctx);
+    return RDK_OK;
+  }
+  return RDK_ERR_ARG;
+}
+
+int rdk_set_step_graph(rdk_ctx* ctx, int mode, int pair_capacity) {
+  CTX_CHECK();
+  if (mode < 0 || mode > 2 || pair_capacity < 0) return RDK_ERR_ARG;
+  RDK_HIP(hipStreamSynchronize(ctx->stream));
+  if (mode != 2) drop_step_graph(ctx);
+  ctx->step_graph_mode = mode;
+  ctx->step_graph_pair_cap = pair_capacity;
+  return RDK_OK;
+}
+
+int rdk_step_graph_info(rdk_ctx* ctx, int64_t* fixed_steps, int64_t* fallbacks, int64_t* captures) {
+  CTX_CHECK();
+  if (fixed_steps) *fixed_steps = ctx->graph_steps;
+  if (fallbacks) *fallbacks = ctx->graph_fallbacks;
+  if (captures) *captures = ctx->graph_captures;
   return RDK_OK;
 }
 
@@ -1015,6 +1260,84 @@ const char* rdk_kernel_name(int kernel) {
 }
 
 int rdk_num_kernels(void) { return K_NUM; }
+
+// Device buffers of the sort/scan utilities: allocated per call (a test and
+// tooling path, not a step path).
+struct TmpDev {
+  void* p = nullptr;
+  ~TmpDev() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+static int util_count(rdk_ctx* ctx, int n, int cap, TmpDev& dn, Count& c) {
+  if (n < 0 || (cap != 0 && cap < n)) return RDK_ERR_ARG;
+  if (cap > n) {
+    RDK_HIP(hipMalloc(&dn.p, sizeof(int)));
+    RDK_HIP(hipMemcpyAsync(dn.p, &n, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    c = Count::dev((const int*)dn.p, cap);
+  } else {
+    c = Count::host(n);
+  }
+  return RDK_OK;
+}
+
+int rdk_sort_pairs_u32(rdk_ctx* ctx, int n, int cap, uint32_t* keys, int32_t* vals, int key_bits) {
+  CTX_CHECK();
+  if (!keys || !vals || key_bits < 1 || key_bits > 32) return RDK_ERR_ARG;
+  TmpDev dn, buf;
+  Count c;
+  RDK_TRY(util_count(ctx, n, cap, dn, c));
+  const int m = std::max(1, std::max(n, cap));
+  RDK_HIP(hipMalloc(&buf.p, (size_t)m * 16));
+  uint32_t* k0 = (uint32_t*)buf.p;
+  uint32_t* k1 = k0 + m;
+  int* v0 = (int*)(k1 + m);
+  int* v1 = v0 + m;
+  RDK_HIP(hipMemcpyAsync(k0, keys, (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream));
+  RDK_HIP(hipMemcpyAsync(v0, vals, (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream));
+  bool alt = false;
+  RDK_TRY(sort_pairs_u32(ctx, k0, k1, v0, v1, c, key_bits, alt, nullptr));
+  RDK_HIP(hipMemcpyAsync(keys, alt ? k1 : k0, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
+  RDK_HIP(hipMemcpyAsync(vals, alt ? v1 : v0, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
+  RDK_HIP(hipStreamSynchronize(ctx->stream));
+  return RDK_OK;
+}
+
+int rdk_sort_pair_keys_u64(rdk_ctx* ctx, int n, int cap, uint64_t* keys, int index_bits) {
+  CTX_CHECK();
+  if (!keys || index_bits < 1 || index_bits > 32) return RDK_ERR_ARG;
+  TmpDev dn, buf;
+  Count c;
+  RDK_TRY(util_count(ctx, n, cap, dn, c));
+  const int m = std::max(1, std::max(n, cap));
+  RDK_HIP(hipMalloc(&buf.p, (size_t)m * 16));
+  uint64_t* k0 = (uint64_t*)buf.p;
+  uint64_t* k1 = k0 + m;
+  RDK_HIP(hipMemcpyAsync(k0, keys, (size_t)n * 8, hipMemcpyHostToDevice, ctx->stream));
+  bool alt = false;
+  RDK_TRY(sort_pair_keys_u64(ctx, k0, k1, c, index_bits, alt, nullptr));
+  RDK_HIP(hipMemcpyAsync(keys, alt ? k1 : k0, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream));
+  RDK_HIP(hipStreamSynchronize(ctx->stream));
+  return RDK_OK;
+}
+
+int rdk_exclusive_scan_i32(rdk_ctx* ctx, int n, int cap, const int32_t* in, int32_t* out) {
+  CTX_CHECK();
+  if (!in || !out) return RDK_ERR_ARG;
+  TmpDev dn, buf;
+  Count c;
+  RDK_TRY(util_count(ctx, n, cap, dn, c));
+  const int m = std::max(1, std::max(n, cap));
+  RDK_HIP(hipMalloc(&buf.p, (size_t)m * 8));
+  int* a = (int*)buf.p;
+  int* b = a + m;
+  RDK_HIP(hipMemcpyAsync(a, in, (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream));
+  RDK_TRY(exclusive_scan_int(ctx, a, b, c, nullptr));
+  RDK_HIP(hipMemcpyAsync(out, b, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
+  RDK_HIP(hipStreamSynchronize(ctx->stream));
+  return RDK_OK;
+}
 
 int rdk_work_counters(rdk_ctx* ctx, int64_t* counters, int n) {
   if (!ctx || !counters) return RDK_ERR_ARG;

@@ -458,8 +458,9 @@ struct ContactPtrs {
   double *pa[3], *pb[3], *n[3], *sh[3], *dist, *hs, *s, *t;
 };
 
-__global__ void k_contact_record(BodyPtrs B, const SysDev* __restrict__ sys, const uint64_t* __restrict__ keys, int nc,
-                                 double delta, ContactPtrs C) {
+__global__ void k_contact_record(BodyPtrs B, const SysDev* __restrict__ sys, const uint64_t* __restrict__ keys,
+                                 Count ncc, double delta, ContactPtrs C) {
+  const int nc = ncc.get();
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nc) return;
   const uint64_t key = keys[i];
@@ -614,6 +615,82 @@ static int ensure_contacts(rdk_ctx* ctx, int nc) {
   return RDK_OK;
 }
 
+static ContactPtrs contact_ptrs(rdk_ctx* ctx) {
+  ContactPtrs C;
+  C.a = ctx->ca.p;
+  C.b = ctx->cb.p;
+  C.type = ctx->ctype.p;
+  for (int k = 0; k < 3; ++k) {
+    C.pa[k] = ctx->cpa[k].p;
+    C.pb[k] = ctx->cpb[k].p;
+    C.n[k] = ctx->cn[k].p;
+    C.sh[k] = ctx->csh[k].p;
+  }
+  C.dist = ctx->cdist.p;
+  C.hs = ctx->chs.p;
+  C.s = ctx->cs.p;
+  C.t = ctx->ct.p;
+  return C;
+}
+
+// Reserve everything the fixed detection path touches (called before capture).
+int detect_fixed_reserve(rdk_ctx* ctx, const rdk_detect_params* p, int pair_cap) {
+  RDK_TRY(build_grid(ctx, p));
+  RDK_TRY(ensure(ctx, ctx->counters, 8));
+  if (!ctx->h_counters) RDK_HIP(hipHostMalloc((void**)&ctx->h_counters, 64, hipHostMallocDefault));
+  RDK_TRY(ensure(ctx, ctx->pkeys, pair_cap));
+  RDK_TRY(ensure(ctx, ctx->pkeys_alt, pair_cap));
+  RDK_TRY(ensure_contacts(ctx, std::max(pair_cap, 1)));
+  return RDK_OK;
+}
+
+// Detection of the fixed-capacity step: the launches of detect_run with the
+// contact count left on the device (counters[0], capped at the reserved
+// capacity).  The grid must be current (detect_fixed_reserve).
+int detect_fixed_enqueue(rdk_ctx* ctx, const rdk_detect_params* p) {
+  const int M = ctx->n_chunks, cap = ctx->fx_pair_cap;
+  const int nb = 256;
+  if (M <= 0 || cap <= 0) {
+    ctx->err = "fixed step: detection not reserved";
+    return RDK_ERR_STATE;
+  }
+  BodyPtrs B = body_ptrs(ctx);
+  hipLaunchKernelGGL(k_chunk_keys, dim3(grid_for(M, nb)), dim3(nb), 0, ctx->stream, B, ctx->sys_dev.p,
+                     ctx->chunk_rod.p, ctx->chunk_k.p, ctx->rod_nchunk.p, M, ctx->ckey.p, ctx->cid.p);
+  bool alt = false;
+  RDK_TRY(sort_pairs_u32(ctx, ctx->ckey.p, ctx->ckey_alt.p, ctx->cid.p, ctx->cid_alt.p, Count::host(M),
+                         bits_for((uint64_t)std::max(1, ctx->n_cells - 1)), alt, nullptr));
+  const uint32_t* skey = alt ? ctx->ckey_alt.p : ctx->ckey.p;
+  const int* sid = alt ? ctx->cid_alt.p : ctx->cid.p;
+  hipLaunchKernelGGL(k_cell_start, dim3(grid_for((int64_t)ctx->n_cells + 1, nb)), dim3(nb), 0, ctx->stream, skey, M,
+                     ctx->n_cells, ctx->cell_start.p);
+  SortedChunks SC;
+  SC.key = skey;
+  SC.rod = ctx->srod.p;
+  SC.k = ctx->sk.p;
+  SC.cf = ctx->scf.p;
+  SC.ca = ctx->sca.p;
+  hipLaunchKernelGGL(k_chunk_gather, dim3(grid_for(M, nb)), dim3(nb), 0, ctx->stream, B, sid, ctx->chunk_rod.p,
+                     ctx->chunk_k.p, ctx->rod_nchunk.p, M, SC);
+  // counters[] were cleared by fixed_clear_enqueue; more contacts than cap fail
+  // the step in k_fx_counts
+  hipLaunchKernelGGL(k_broad_narrow, dim3(grid_for(M, BN_THREADS)), dim3(BN_THREADS), 0, ctx->stream, B,
+                     ctx->sys_dev.p, SC, ctx->cell_start.p, ctx->rod_nchunk.p, M, p->delta, ctx->pkeys.p,
+                     (unsigned)cap, ctx->counters.p, (unsigned long long*)(ctx->counters.p + 2));
+  const Count nc = Count::dev((const int*)ctx->counters.p, cap);
+  bool kalt = false;
+  RDK_TRY(sort_pair_keys_u64(ctx, ctx->pkeys.p, ctx->pkeys_alt.p, nc, bits_for((uint64_t)ctx->n), kalt, nullptr));
+  const uint64_t* keys = kalt ? ctx->pkeys_alt.p : ctx->pkeys.p;
+  hipLaunchKernelGGL(k_contact_record, dim3(grid_for(cap, nb)), dim3(nb), 0, ctx->stream, B, ctx->sys_dev.p, keys, nc,
+                     p->delta, contact_ptrs(ctx));
+  if (kalt) {  // the sorted keys are the canonical ones (warm cache, history)
+    std::swap(ctx->pkeys.p, ctx->pkeys_alt.p);
+    std::swap(ctx->pkeys.cap, ctx->pkeys_alt.cap);
+  }
+  ctx->last_delta = p->delta;
+  return RDK_OK;
+}
+
 int detect_run(rdk_ctx* ctx, const rdk_detect_params* p) {
   if (ctx->n <= 0 || ctx->nsys <= 0) {
     ctx->err = "rdk_detect: bodies and systems must be set first";
@@ -692,7 +769,8 @@ int detect_run(rdk_ctx* ctx, const rdk_detect_params* p) {
   RDK_TRY(ensure_contacts(ctx, std::max(nc, 1)));
   if (nc == 0) return RDK_OK;
   bool kalt = false;
-  RDK_TRY(sort_keys_u64(ctx, ctx->pkeys.p, ctx->pkeys_alt.p, nc, 32 + bits_for((uint64_t)ctx->n), kalt));
+  RDK_TRY(sort_pair_keys_u64(ctx, ctx->pkeys.p, ctx->pkeys_alt.p, Count::host(nc), bits_for((uint64_t)ctx->n), kalt,
+                             nullptr));
   const uint64_t* keys = kalt ? ctx->pkeys_alt.p : ctx->pkeys.p;
   ContactPtrs C;
   C.a = ctx->ca.p;
@@ -711,7 +789,7 @@ int detect_run(rdk_ctx* ctx, const rdk_detect_params* p) {
   {
     ProfScope ps(ctx, K_CONTACT_RECORD);
     hipLaunchKernelGGL(k_contact_record, dim3(grid_for(nc, nb)), dim3(nb), 0, ctx->stream, B, ctx->sys_dev.p, keys,
-                       nc, p->delta, C);
+                       Count::host(nc), p->delta, C);
   }
   // keep the canonical keys for history matching
   if (kalt) {

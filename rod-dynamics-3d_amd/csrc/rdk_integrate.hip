@@ -46,7 +46,8 @@ __device__ __forceinline__ double wrap1(double p, double lo, double hi, int peri
 }
 
 __global__ void __launch_bounds__(256) k_integrate(BodyPtrs B, const SysDev* __restrict__ sys, int n, int stage,
-                                                   PhysK P) {
+                                                   PhysK P, const int* __restrict__ guard) {
+  if (guard && *guard) return;  // a failed captured step changes no state
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const d3 zero = mk3(0, 0, 0);
@@ -155,7 +156,7 @@ __global__ void k_kinetic(BodyPtrs B, int n, double* __restrict__ out) {
   if (__lane_id() == 0 && ke != 0.0) atomicAdd(out, ke);
 }
 
-int integrate_run(rdk_ctx* ctx, int stage) {
+int integrate_run(rdk_ctx* ctx, int stage, const int* guard) {
   if (!ctx->have_phys) {
     ctx->err = "rdk_integrate: physics parameters not set";
     return RDK_ERR_STATE;
@@ -179,7 +180,7 @@ int integrate_run(rdk_ctx* ctx, int stage) {
                                           : K_MISC;
   ProfScope ps(ctx, kid);
   hipLaunchKernelGGL(k_integrate, dim3(grid_for(ctx->n, 256)), dim3(256), 0, ctx->stream, body_ptrs(ctx),
-                     ctx->sys_dev.p, ctx->n, stage, P);
+                     ctx->sys_dev.p, ctx->n, stage, P, guard);
   if (stage != RDK_CLEAR) ctx->work[W_INTEGRATE_BODIES] += ctx->n;
   return RDK_OK;
 }

@@ -75,6 +75,34 @@ struct DevBuf {
   size_t cap = 0;
 };
 
+// Element count of a launch: a host value (p == nullptr: cap), or a device
+// value read when the kernel runs, mul * *p + add clamped to [0, cap], so a
+// launch sized for cap can be enqueued (or captured) before the count exists.
+struct Count {
+  const int* p = nullptr;
+  int mul = 1, add = 0, cap = 0;
+  int skip_below = 0;  // a device count below this reads as 0 (another launch handles it)
+  static Count host(int n) {
+    Count c;
+    c.cap = n;
+    return c;
+  }
+  static Count dev(const int* ptr, int cap, int mul = 1, int add = 0) {
+    Count c;
+    c.p = ptr;
+    c.cap = cap;
+    c.mul = mul;
+    c.add = add;
+    return c;
+  }
+  __device__ __forceinline__ int get() const {
+    if (!p) return cap;
+    const long long v = (long long)mul * (long long)(*p) + add;
+    const int c = v < 0 ? 0 : (v > cap ? cap : (int)v);
+    return c < skip_below ? 0 : c;
+  }
+};
+
 // Per-system constants on the device.
 struct SysDev {
   double lo[3], hi[3], span[3], cell[3], inv_cell[3];
@@ -162,6 +190,7 @@ struct rdk_ctx {
   rdk::DevBuf<int> d_cstart;
   // islands
   int solver_mode = 0;  // 0 islands when possible, 1 always global coloured
+  int small_m = -1;     // single-workgroup colouring limit (RDK_TUNE_SMALL_M; -1: default)
   int nislands = 0;
   int island_counts[4] = {0, 0, 0, 0};  // tiny, small, medium, large
   rdk::DevBuf<int> uf_parent, uf_label, sbody, sbody_alt, iflag, ipos, istart, ilists, bseg, icnt;
@@ -223,6 +252,7 @@ struct rdk_ctx {
 
   // ---- scratch ----
   rdk::DevBuf<char> temp;
+  rdk::DevBuf<int> scan_tmp;  // per-tile sums of the exclusive scan
   rdk::DevBuf<double> red;  // reduction scratch
 
   // ---- profiling ----
@@ -234,6 +264,27 @@ struct rdk_ctx {
   int64_t huge_blocks = 0;              // check blocks of the huge-island solve (timing sample)
   double k_ms[rdk::K_NUM] = {};
   int64_t work[rdk::W_NUM] = {};
+
+  // ---- fixed-capacity (capturable) hard-contact step ----
+  // Device words fx[FX_*] (error, step counts, per-step counts, colouring
+  // round counters), accumulated statistics facc[FA_*], and the graph of one
+  // step.
+  bool capturing = false;  // inside stream capture: no allocation, no timing
+  uint64_t alloc_gen = 0;  // bumped by every reallocation (captured pointers go stale)
+  int fx_pair_cap = 0, fx_wall_cap = 0, fx_nm_cap = 0, fx_rounds = 0, color_rounds_max = 0;
+  int nc_seen_max = 0, nwall_seen_max = 0;  // most contacts / wall manifolds an eager step has built
+  int cls_seen_max[4] = {0, 0, 0, 0};       // most islands per class (tiny, small, medium, LDS) in a step
+  int fx_grid[4] = {0, 0, 0, 0};            // workgroups of the fixed step's class launches
+  int64_t eager_nsc_steps = 0;
+  int step_graph_mode = 0;  // rdk_set_step_graph
+  int step_graph_pair_cap = 0;
+  rdk::DevBuf<int> fx;
+  rdk::DevBuf<double> facc;
+  hipGraph_t step_graph = nullptr;
+  hipGraphExec_t step_exec = nullptr;
+  uint64_t step_graph_gen = 0;
+  std::vector<char> step_graph_key;  // parameters the graph was captured with
+  int64_t graph_steps = 0, graph_fallbacks = 0, graph_captures = 0;
 };
 
 namespace rdk {
@@ -257,6 +308,11 @@ namespace rdk {
 template <class T>
 int ensure(rdk_ctx* ctx, DevBuf<T>& b, size_t n) {
   if (n <= b.cap && b.p) return RDK_OK;
+  if (ctx->capturing) {  // a captured step must find every buffer at its capacity
+    ctx->err = "step capture: a buffer would have to grow";
+    return RDK_ERR_STATE;
+  }
+  ctx->alloc_gen++;
   size_t nc = n < 16 ? 16 : n + n / 4;
   if (b.p) {
     hipError_t e = hipFree(b.p);
@@ -273,6 +329,27 @@ int ensure(rdk_ctx* ctx, DevBuf<T>& b, size_t n) {
     return RDK_ERR_HIP;
   }
   b.cap = nc;
+  return RDK_OK;
+}
+
+// ensure() that keeps the first `used` elements (state carried between steps,
+// such as the warm-start cache).
+template <class T>
+int grow_keep(rdk_ctx* ctx, DevBuf<T>& b, size_t n, size_t used) {
+  if (n <= b.cap && b.p) return RDK_OK;
+  DevBuf<T> nb;
+  RDK_TRY(ensure(ctx, nb, n));
+  if (used > 0 && b.p) {
+    hipError_t e = hipMemcpyAsync(nb.p, b.p, used * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+      ctx->err = std::string("grow_keep: ") + hipGetErrorString(e);
+      (void)hipFree(nb.p);
+      return RDK_ERR_HIP;
+    }
+  }
+  if (b.p) (void)hipFree(b.p);
+  b = nb;
   return RDK_OK;
 }
 
@@ -297,7 +374,7 @@ struct ProfScope {
   hipStream_t s;
   bool on;
   ProfScope(rdk_ctx* c, int k, hipStream_t st = nullptr)
-      : ctx(c), s(st ? st : c->stream), on(c->profiling && ((c->prof_mask >> k) & 1)) {
+      : ctx(c), s(st ? st : c->stream), on(c->profiling && !c->capturing && ((c->prof_mask >> k) & 1)) {
     if (on) {
       c->k_launches[k] += 1;
       prof_begin(c, k, rec, s);
@@ -320,11 +397,48 @@ void prof_ext_done(rdk_ctx* ctx, EventRec& rec);
 
 // True when kernel k's launches are being timed (timed launches are kept out
 // of graphs: a graph replay cannot be split into its kernels by events).
-static inline bool timed(const rdk_ctx* ctx, int k) { return ctx->profiling && ((ctx->prof_mask >> k) & 1); }
+static inline bool timed(const rdk_ctx* ctx, int k) {
+  return ctx->profiling && !ctx->capturing && ((ctx->prof_mask >> k) & 1);
+}
 
 // Graph replay of repeated launch sequences; RDK_GRAPHS=0 turns it off (for
 // tools that cannot follow captured graphs).
 bool graphs_enabled();
+
+// Fixed-capacity step: device word and accumulator layout.
+enum FxWord {
+  FX_ERR = 0,     // first failure of the current run of captured steps (0: none)
+  FX_STEPS,       // steps completed since the host last cleared it
+  FX_NWALL,       // plane-wall manifolds of the current step
+  FX_NM,          // manifolds of the current step
+  FX_NISL,        // islands of the current step
+  FX_NPREV,       // rod-rod warm-cache entries
+  FX_NPREV_W,     // wall warm-cache entries
+  FX_ROUND0 = 8,  // colouring round r: manifolds left uncoloured (unsigned)
+  FX_WORDS = 8 + 256
+};
+// errors of a captured step (the step is then redone eagerly)
+enum FxErr { FXE_PAIRS = 1, FXE_COLOURS = 2, FXE_ROUNDS = 3, FXE_ISLANDS = 4, FXE_WALLS = 5 };
+enum FxAcc {
+  FA_STEPS = 0, FA_CONTACTS, FA_MANIFOLDS, FA_SWEEPS, FA_COLORS, FA_NONCONV, FA_RESMAX, FA_CANDIDATES, FA_NARROW,
+  FA_MSWEEPS, FA_LDS_SWEEPS, FA_HOT_SWEEPS, FA_WALLS, FA_CLS0, FA_CLS1, FA_CLS2, FA_CLS3, FA_NUM
+};
+
+// One hard-contact step with no host synchronisation (capturable): detection
+// into the reserved pair capacity, manifolds, colouring, islands, the island
+// classes that fit one workgroup, warm cache, statistics.  Requires
+// fixed_prepare (capacities reserved, grid built).  Failures (capacity,
+// colouring, an island too large for one workgroup) set fx[FX_ERR] before the
+// step's first state change; every later kernel that changes state then
+// returns at once.
+int fixed_prepare(rdk_ctx* ctx, const rdk_detect_params* dp, const rdk_nsc_params* np, int pair_cap, int wall_cap);
+int detect_fixed_enqueue(rdk_ctx* ctx, const rdk_detect_params* p);
+int nsc_fixed_enqueue(rdk_ctx* ctx, const rdk_nsc_params* p);
+int detect_fixed_reserve(rdk_ctx* ctx, const rdk_detect_params* p, int pair_cap);
+int nsc_fixed_reserve(rdk_ctx* ctx, const rdk_nsc_params* p);
+int fixed_sync_host_counts(rdk_ctx* ctx);
+int fixed_clear_enqueue(rdk_ctx* ctx);
+void fixed_class_grids(const rdk_ctx* ctx, int* grids);
 
 // stage entry points implemented in the .hip files
 int detect_run(rdk_ctx* ctx, const rdk_detect_params* p);
@@ -336,7 +450,7 @@ int nsc_sweeps_run(rdk_ctx* ctx, const rdk_nsc_params* p, int nsweeps);
 int nsc_color_run(rdk_ctx* ctx, bool need_host);
 int color_to_host(rdk_ctx* ctx);
 int nsc_residual_run(rdk_ctx* ctx, const rdk_nsc_params* p, double* res, double* res_ref);
-int integrate_run(rdk_ctx* ctx, int stage);
+int integrate_run(rdk_ctx* ctx, int stage, const int* guard = nullptr);
 int forces_run(rdk_ctx* ctx, int model, const void* params, double* pe);
 int observables_run(rdk_ctx* ctx, double* ke, double* phi);
 int entanglement_run(rdk_ctx* ctx, double cutoff, double* sums, int64_t* pairs);
@@ -349,12 +463,20 @@ int mlp_warm_start(rdk_ctx* ctx, int mode);
 int nsc_project_run(rdk_ctx* ctx, const rdk_detect_params* dp, const rdk_nsc_params* p, int* passes, int* npos);
 void mlp_free(rdk_ctx* ctx);
 
-// rocPRIM-backed sorting / scanning helpers (rdk_sort.hip)
+// Stable radix sorts and the exclusive scan (rdk_sort.hip).  The result of a
+// sort is in the *_alt buffers when result_in_alt is set, else in place.  A
+// Count with a device pointer sizes the launches for its capacity and reads
+// the element count when they run; st = nullptr is the context stream.
 int sort_pairs_u32(rdk_ctx* ctx, uint32_t* keys, uint32_t* keys_alt, int* vals, int* vals_alt, int n,
                    int end_bit, bool& result_in_alt);
-int sort_keys_u64(rdk_ctx* ctx, uint64_t* keys, uint64_t* keys_alt, int n, int end_bit,
-                  bool& result_in_alt);
+int sort_pairs_u32(rdk_ctx* ctx, uint32_t* keys, uint32_t* keys_alt, int* vals, int* vals_alt, Count n,
+                   int end_bit, bool& result_in_alt, hipStream_t st);
+// 64-bit pair keys (lo << 32) | hi with lo, hi < 2^index_bits
+int sort_pair_keys_u64(rdk_ctx* ctx, uint64_t* keys, uint64_t* keys_alt, Count n, int index_bits,
+                       bool& result_in_alt, hipStream_t st);
 int exclusive_scan_int(rdk_ctx* ctx, const int* in, int* out, int n);
+int exclusive_scan_int(rdk_ctx* ctx, const int* in, int* out, Count n, hipStream_t st);
+int sort_reserve(rdk_ctx* ctx, int max_count);
 
 static inline int bits_for(uint64_t maxval) {
   int b = 1;

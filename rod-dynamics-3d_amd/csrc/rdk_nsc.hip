@@ -15,6 +15,8 @@
 
 #include <hip/hip_ext.h>
 
+#include <climits>
+
 #include "rdk_internal.hpp"
 
 namespace rdk {
@@ -93,10 +95,13 @@ struct ContactIn {
   const double *pa[3], *pb[3], *n[3], *sh[3], *dist, *hs;
 };
 
-__global__ void k_manifold(BodyPtrs B, ContactIn C, int nc, MfPtrs M, double cfm, int warm, const uint64_t* prev_keys,
-                           int nprev, const double* pl0, const double* pl1, const double* pl2) {
+__global__ void k_manifold(BodyPtrs B, ContactIn C, Count ncc, MfPtrs M, double cfm, int warm, const uint64_t* prev_keys,
+                           int nprev, const int* __restrict__ nprev_dev, const double* pl0, const double* pl1,
+                           const double* pl2) {
+  const int nc = ncc.get();
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nc) return;
+  if (nprev_dev) nprev = *nprev_dev;
   const int a = C.a[i], b = C.b[i];
   const d3 n = mk3(C.n[0][i], C.n[1][i], C.n[2][i]);
   const d3 pa = mk3(C.pa[0][i], C.pa[1][i], C.pa[2][i]);
@@ -193,11 +198,14 @@ __global__ void k_wall_flags(BodyPtrs B, const SysDev* __restrict__ sys, int n, 
 }
 
 __global__ void k_wall_manifold(BodyPtrs B, const SysDev* __restrict__ sys, int nbody, const int* __restrict__ flag,
-                                const int* __restrict__ off, int base, MfPtrs M, double cfm, int warm,
-                                const uint64_t* __restrict__ prev_keys, int nprev, const double* __restrict__ pl0,
+                                const int* __restrict__ off, int wall_cap, Count basec, MfPtrs M, double cfm,
+                                int warm, const uint64_t* __restrict__ prev_keys, int nprev,
+                                const int* __restrict__ nprev_dev, const double* __restrict__ pl0,
                                 const double* __restrict__ pl1, const double* __restrict__ pl2) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 4 * nbody || !flag[t]) return;
+  if (t >= 4 * nbody || !flag[t] || off[t] >= wall_cap) return;  // past a fixed step's reserve: the step fails
+  const int base = basec.get();
+  if (nprev_dev) nprev = *nprev_dev;
   const int i = t >> 2, slot = t & 3;
   const SysDev S = sys[B.sys[i]];
   d3 n, pt;
@@ -256,9 +264,13 @@ __global__ void k_wall_manifold(BodyPtrs B, const SysDev* __restrict__ sys, int 
   M.lam[2][m] = l2;
 }
 
-__global__ void k_copy_wall_keys_lambda(MfPtrs nat, int base, int nw, uint64_t* __restrict__ pk,
-                                        double* __restrict__ p0, double* __restrict__ p1, double* __restrict__ p2) {
+__global__ void k_copy_wall_keys_lambda(MfPtrs nat, Count basec, Count nwc, uint64_t* __restrict__ pk,
+                                        double* __restrict__ p0, double* __restrict__ p1, double* __restrict__ p2,
+                                        int* __restrict__ nprev_out, const int* __restrict__ guard) {
+  if (guard && *guard) return;
+  const int base = basec.get(), nw = nwc.get();
   int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0 && nprev_out) *nprev_out = nw;
   if (i >= nw) return;
   const int m = base + i;
   pk[i] = ((uint64_t)(uint32_t)nat.a[m] << 2) | (uint64_t)(-1 - nat.b[m]);
@@ -267,14 +279,16 @@ __global__ void k_copy_wall_keys_lambda(MfPtrs nat, int base, int nw, uint64_t* 
   p2[i] = nat.lam[2][m];
 }
 
-__global__ void k_fill_color(int* __restrict__ a, int n, int v) {
+__global__ void k_fill_color(int* __restrict__ a, Count nc, int v) {
+  const int n = nc.get();
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = v;
 }
 
 // ---- body -> manifold CSR -------------------------------------------------
-__global__ void k_csr_entries(const int* __restrict__ ma, const int* __restrict__ mb, int nm, int nbody,
+__global__ void k_csr_entries(const int* __restrict__ ma, const int* __restrict__ mb, Count nmc, int nbody,
                               uint32_t* __restrict__ key, int* __restrict__ val) {
+  const int nm = nmc.get();
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nm) return;
   key[2 * j] = (uint32_t)ma[j];
@@ -284,7 +298,8 @@ __global__ void k_csr_entries(const int* __restrict__ ma, const int* __restrict_
   val[2 * j + 1] = j;
 }
 
-__global__ void k_body_start(const uint32_t* __restrict__ key, int ne, int nbody, int* __restrict__ start) {
+__global__ void k_body_start(const uint32_t* __restrict__ key, Count nec, int nbody, int* __restrict__ start) {
+  const int ne = nec.get();
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i > nbody) return;
   int lo = 0, hi = ne;
@@ -297,9 +312,21 @@ __global__ void k_body_start(const uint32_t* __restrict__ key, int ne, int nbody
 }
 
 // ---- colouring -------------------------------------------------------------
-__global__ void k_color_assign(const int* __restrict__ ma, const int* __restrict__ mb, int nm,
+// Captured steps launch a fixed number of rounds: a round whose predecessor
+// left nothing uncoloured (prev_left[0] == 0), a manifold count the
+// single-workgroup colouring handles (nm <= small_limit), or a failed step
+// (guard) makes it return at once.  Eager rounds pass nullptr and -1.
+__device__ __forceinline__ bool color_round_skipped(int nm, int small_limit, const unsigned* prev_left,
+                                                    const int* guard) {
+  return nm <= small_limit || (prev_left && *prev_left == 0u) || (guard && *guard);
+}
+
+__global__ void k_color_assign(const int* __restrict__ ma, const int* __restrict__ mb, Count nmc,
                                const int* __restrict__ bstart, const int* __restrict__ csr,
-                               const int* __restrict__ color, int* __restrict__ tent, unsigned* __restrict__ overflow) {
+                               const int* __restrict__ color, int* __restrict__ tent, unsigned* __restrict__ overflow,
+                               int small_limit, const unsigned* __restrict__ prev_left, const int* __restrict__ guard) {
+  const int nm = nmc.get();
+  if (color_round_skipped(nm, small_limit, prev_left, guard)) return;
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nm) return;
   if (color[j] >= 0) {
@@ -331,10 +358,13 @@ __global__ void k_color_assign(const int* __restrict__ ma, const int* __restrict
   tent[j] = pick;
 }
 
-__global__ void k_color_resolve(const int* __restrict__ ma, const int* __restrict__ mb, int nm,
+__global__ void k_color_resolve(const int* __restrict__ ma, const int* __restrict__ mb, Count nmc,
                                 const int* __restrict__ bstart, const int* __restrict__ csr,
                                 const int* __restrict__ tent, int* __restrict__ color, unsigned* __restrict__ left,
-                                unsigned* __restrict__ maxcol) {
+                                unsigned* __restrict__ maxcol, int small_limit, const unsigned* __restrict__ prev_left,
+                                const int* __restrict__ guard) {
+  const int nm = nmc.get();
+  if (color_round_skipped(nm, small_limit, prev_left, guard)) return;
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nm) return;
   const int tj = tent[j];
@@ -369,7 +399,8 @@ __global__ void k_color_keys(const int* __restrict__ color, int nm, uint32_t* __
   atomicAdd(&hist[color[j]], 1);
 }
 
-__global__ void k_permute(MfPtrs src, MfPtrs dst, const int* __restrict__ perm, int nm) {
+__global__ void k_permute(MfPtrs src, MfPtrs dst, const int* __restrict__ perm, Count nmc) {
+  const int nm = nmc.get();
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nm) return;
   const int j = perm[i];
@@ -381,7 +412,8 @@ __global__ void k_permute(MfPtrs src, MfPtrs dst, const int* __restrict__ perm, 
   dst.lam[2][i] = src.lam[2][j];
 }
 
-__global__ void k_unpermute_lambda(MfPtrs col, MfPtrs nat, const int* __restrict__ perm, int nm) {
+__global__ void k_unpermute_lambda(MfPtrs col, MfPtrs nat, const int* __restrict__ perm, Count nmc) {
+  const int nm = nmc.get();
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nm) return;
   const int j = perm[i];
@@ -794,9 +826,13 @@ __global__ void __launch_bounds__(256) k_residual(BodyPtrs B, MfPtrs M, const Sy
   if (__lane_id() == 0 && (r > 0.0 || r != r)) atomic_max_pos(out, r);
 }
 
-__global__ void k_copy_keys_lambda(const uint64_t* __restrict__ keys, MfPtrs nat, int nc, uint64_t* __restrict__ pk,
-                                   double* __restrict__ p0, double* __restrict__ p1, double* __restrict__ p2) {
+__global__ void k_copy_keys_lambda(const uint64_t* __restrict__ keys, MfPtrs nat, Count ncc, uint64_t* __restrict__ pk,
+                                   double* __restrict__ p0, double* __restrict__ p1, double* __restrict__ p2,
+                                   int* __restrict__ nprev_out, const int* __restrict__ guard) {
+  if (guard && *guard) return;
+  const int nc = ncc.get();
   int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0 && nprev_out) *nprev_out = nc;
   if (i >= nc) return;
   pk[i] = keys[i];
   p0[i] = nat.lam[0][i];
@@ -829,10 +865,13 @@ __device__ __forceinline__ double block_max(double v, double* red) {
 }
 
 __global__ void __launch_bounds__(SMALL_BLOCK) k_color_small(const int* __restrict__ ma, const int* __restrict__ mb,
-                                                             int nm, const int* __restrict__ bstart,
+                                                             Count nmc, const int* __restrict__ bstart,
                                                              const int* __restrict__ csr, int* __restrict__ color,
-                                                             int* __restrict__ tent, unsigned* __restrict__ ctr) {
+                                                             int* __restrict__ tent, unsigned* __restrict__ ctr,
+                                                             int small_limit, const int* __restrict__ guard) {
   __shared__ unsigned s_left, s_over, s_max;
+  const int nm = nmc.get();
+  if (nm > small_limit || (guard && *guard)) return;  // the rounds of a captured step colour it
   for (int j = threadIdx.x; j < nm; j += blockDim.x) color[j] = -1;
   if (threadIdx.x == 0) {
     s_over = 0;
@@ -1020,7 +1059,8 @@ __global__ void k_uf_init(int* __restrict__ parent, int n) {
 
 // link the higher root under the lower one: every root ends as the smallest
 // body index of its island, so labels are deterministic
-__global__ void k_uf_link(const int* __restrict__ ma, const int* __restrict__ mb, int nm, int* parent) {
+__global__ void k_uf_link(const int* __restrict__ ma, const int* __restrict__ mb, Count nmc, int* parent) {
+  const int nm = nmc.get();
   int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= nm) return;
   const int a = ma[m], b = mb[m];
@@ -1038,8 +1078,9 @@ __global__ void k_uf_flatten(int* parent, int n, int* __restrict__ label) {
   if (i < n) label[i] = uf_find(parent, i);
 }
 
-__global__ void k_island_keys(const int* __restrict__ ma, const int* __restrict__ color, int nm,
+__global__ void k_island_keys(const int* __restrict__ ma, const int* __restrict__ color, Count nmc,
                               const int* __restrict__ label, uint32_t* __restrict__ key, int* __restrict__ val) {
+  const int nm = nmc.get();
   int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= nm) return;
   key[m] = ((uint32_t)label[ma[m]] << 7) | (uint32_t)color[m];
@@ -1054,18 +1095,24 @@ __global__ void k_body_label_keys(const int* __restrict__ label, int n, uint32_t
   val[i] = i;
 }
 
-__global__ void k_island_flags(const uint32_t* __restrict__ skey, int nm, int* __restrict__ flag) {
+__global__ void k_island_flags(const uint32_t* __restrict__ skey, Count nmc, int* __restrict__ flag) {
+  const int nm = nmc.get();
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nm) return;
   flag[i] = (i == 0 || (skey[i] >> 7) != (skey[i - 1] >> 7)) ? 1 : 0;
 }
 
-__global__ void k_island_starts(const int* __restrict__ flag, const int* __restrict__ pos, int nm,
-                                int* __restrict__ istart) {
+__global__ void k_island_starts(const int* __restrict__ flag, const int* __restrict__ pos, Count nmc,
+                                int* __restrict__ istart, int* __restrict__ nisl_out) {
+  const int nm = nmc.get();
   int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0 && nisl_out && nm == 0) *nisl_out = 0;
   if (i >= nm) return;
   if (flag[i]) istart[pos[i]] = i;
-  if (i == nm - 1) istart[pos[i] + flag[i]] = nm;
+  if (i == nm - 1) {
+    istart[pos[i] + flag[i]] = nm;
+    if (nisl_out) *nisl_out = pos[i] + flag[i];
+  }
 }
 
 __device__ __forceinline__ int lb_u32(const uint32_t* a, int lo, int hi, uint32_t key) {
@@ -1084,9 +1131,11 @@ struct IslandClass {
 
 // island list per class; counts in cnt[0..5] (tiny, small, medium, large with
 // bodies in LDS, large on global memory, giant)
-__global__ void k_island_classify(const int* __restrict__ istart, int nisl, const uint32_t* __restrict__ skey,
+__global__ void k_island_classify(const int* __restrict__ istart, Count nislc, const uint32_t* __restrict__ skey,
                                   const uint32_t* __restrict__ blab, int nbody, IslandClass C,
-                                  int* __restrict__ lists, int* __restrict__ cnt, int* __restrict__ bseg) {
+                                  int* __restrict__ lists, int list_stride, int* __restrict__ cnt,
+                                  int* __restrict__ bseg) {
+  const int nisl = nislc.get();
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nisl) return;
   const int s = istart[k], e = istart[k + 1];
@@ -1101,7 +1150,7 @@ __global__ void k_island_classify(const int* __restrict__ istart, int nisl, cons
   else if (size <= C.max_m[2] && nb <= C.max_b[2]) cls = 2;
   else if (size <= C.large_max) cls = nb <= C.lds_bmax ? 3 : 4;
   const int slot = atomicAdd(&cnt[cls], 1);
-  lists[cls * nisl + slot] = k;
+  lists[cls * list_stride + slot] = k;
 }
 
 struct IslandStats {
@@ -1178,11 +1227,11 @@ __device__ __forceinline__ int lb_int(const int* a, int n, int key) {
 // colour in parallel: they share no body, so every sweep equals the serial
 // sweep in colour order bit for bit (nsc_contact.cpp:385-552 per update).
 template <int GROUP, int WG>
-__global__ void __launch_bounds__(WG) k_island_reg(BodyPtrs B, MfPtrs M, const SysDev* __restrict__ sys,
-                                                   const int* __restrict__ istart, const uint32_t* __restrict__ skey,
-                                                   const int* __restrict__ bseg, const int* __restrict__ sbody,
-                                                   const int* __restrict__ list, int count, IslandSolve P,
-                                                   IslandStats* S) {
+__device__ __forceinline__ void island_reg_block(BodyPtrs B, MfPtrs M, const SysDev* __restrict__ sys,
+                                                 const int* __restrict__ istart, const uint32_t* __restrict__ skey,
+                                                 const int* __restrict__ bseg, const int* __restrict__ sbody,
+                                                 const int* __restrict__ list, int count, IslandSolve P,
+                                                 IslandStats* S, int blk) {
   constexpr int IPB = WG / GROUP;     // islands per workgroup
   constexpr int BCAP = GROUP + 8;     // bodies per island (a connected island of m manifolds has <= m + 1)
   __shared__ double v_l[3][IPB * BCAP], w_l[3][IPB * BCAP];
@@ -1190,7 +1239,7 @@ __global__ void __launch_bounds__(WG) k_island_reg(BodyPtrs B, MfPtrs M, const S
   __shared__ double red[WG / 64 + 1];
   __shared__ int s_ncol;
   const int tid = threadIdx.x, slot = tid / GROUP, j = tid % GROUP;
-  const int t = blockIdx.x * IPB + slot;
+  const int t = blk * IPB + slot;
   const bool has = t < count;
   int s = 0, n = 0, bs = 0, nb = 0;
   if (has) {
@@ -1293,6 +1342,29 @@ __global__ void __launch_bounds__(WG) k_island_reg(BodyPtrs B, MfPtrs M, const S
                   P.stop_rule == 1 && !(res <= P.tol));
 }
 
+// One group of IPB islands per workgroup.  LOOP: a captured step launches a
+// fixed grid that walks the class's device-side island count (guard: a
+// failed step, nothing is solved); the eager launch has one workgroup per
+// group.
+template <int GROUP, int WG, bool LOOP>
+__global__ void __launch_bounds__(WG) k_island_reg(BodyPtrs B, MfPtrs M, const SysDev* __restrict__ sys,
+                                                   const int* __restrict__ istart, const uint32_t* __restrict__ skey,
+                                                   const int* __restrict__ bseg, const int* __restrict__ sbody,
+                                                   const int* __restrict__ list, Count countc, IslandSolve P,
+                                                   IslandStats* S, const int* __restrict__ guard) {
+  constexpr int IPB = WG / GROUP;
+  if (!LOOP) {
+    island_reg_block<GROUP, WG>(B, M, sys, istart, skey, bseg, sbody, list, countc.get(), P, S, blockIdx.x);
+    return;
+  }
+  if (guard && *guard) return;
+  const int count = countc.get();
+  for (int blk = blockIdx.x; blk * IPB < count; blk += gridDim.x) {
+    island_reg_block<GROUP, WG>(B, M, sys, istart, skey, bseg, sbody, list, count, P, S, blk);
+    __syncthreads();  // the next group reuses the LDS
+  }
+}
+
 // island classes: manifolds and bodies per island for the register kernels
 constexpr int ISL_G0 = 8, ISL_WG0 = 64;     // tiny: 8 islands per wavefront
 constexpr int ISL_G1 = 64, ISL_WG1 = 64;    // small: one island per wavefront
@@ -1352,13 +1424,11 @@ __device__ __forceinline__ void load_manifold(MfReg& R, const MfSlots& A, int i)
   R.b = A.ib(i);
 }
 
-__global__ void __launch_bounds__(LDS_THREADS) k_island_lds(BodyPtrs B, MfPtrs M, const SysDev* __restrict__ sys,
-                                                           const int* __restrict__ istart,
-                                                           const uint32_t* __restrict__ skey,
-                                                           const int* __restrict__ bseg, const int* __restrict__ sbody,
-                                                           const int* __restrict__ list, IslandSolve P,
-                                                           IslandStats* S, int2* __restrict__ slot,
-                                                           int* __restrict__ hlist) {
+__device__ __forceinline__ void island_lds_block(BodyPtrs B, MfPtrs M, const SysDev* __restrict__ sys,
+                                                 const int* __restrict__ istart, const uint32_t* __restrict__ skey,
+                                                 const int* __restrict__ bseg, const int* __restrict__ sbody,
+                                                 const int* __restrict__ list, IslandSolve P, IslandStats* S,
+                                                 int2* __restrict__ slot, int* __restrict__ hlist, int idx) {
   constexpr int HOT_WORDS = (LDS_BCAP + 31) / 32;
   constexpr int NWAVE = LDS_THREADS / 64;
   __shared__ double v_l[3][LDS_BCAP], w_l[3][LDS_BCAP];
@@ -1367,7 +1437,7 @@ __global__ void __launch_bounds__(LDS_THREADS) k_island_lds(BodyPtrs B, MfPtrs M
   __shared__ int hcs[129];                // colour starts inside the hot set
   __shared__ unsigned hot_bits[HOT_WORDS];  // bodies (LDS slots) of manifolds above HOT_TARGET * tol
   __shared__ int s_above, s_wcnt[NWAVE];
-  const int k = list[blockIdx.x];
+  const int k = list[idx];
   const int s = istart[k], e = istart[k + 1];
   const int bs = bseg[2 * k], nb = bseg[2 * k + 1] - bs;
   const int* body_l = sbody + bs;  // the island's bodies, ascending: LDS slot q holds body body_l[q]
@@ -1589,6 +1659,28 @@ __global__ void __launch_bounds__(LDS_THREADS) k_island_lds(BodyPtrs B, MfPtrs M
   }
 }
 
+// One island per workgroup (LOOP: a captured step launches a fixed grid that
+// walks the class's device-side count).
+template <bool LOOP>
+__global__ void __launch_bounds__(LDS_THREADS) k_island_lds(BodyPtrs B, MfPtrs M, const SysDev* __restrict__ sys,
+                                                           const int* __restrict__ istart,
+                                                           const uint32_t* __restrict__ skey,
+                                                           const int* __restrict__ bseg, const int* __restrict__ sbody,
+                                                           const int* __restrict__ list, Count countc, IslandSolve P,
+                                                           IslandStats* S, int2* __restrict__ slot,
+                                                           int* __restrict__ hlist, const int* __restrict__ guard) {
+  if (!LOOP) {
+    island_lds_block(B, M, sys, istart, skey, bseg, sbody, list, P, S, slot, hlist, blockIdx.x);
+    return;
+  }
+  if (guard && *guard) return;
+  const int count = countc.get();
+  for (int idx = blockIdx.x; idx < count; idx += gridDim.x) {
+    island_lds_block(B, M, sys, istart, skey, bseg, sbody, list, P, S, slot, hlist, idx);
+    __syncthreads();  // the next island reuses the LDS
+  }
+}
+
 // Colour starts of listed islands (the huge class), for the host-driven
 // colour-by-colour solve: out[k * HUGE_CS + c] = first manifold of colour c,
 // c = 0 .. ncol <= 128 (the last is the island end), out[k * HUGE_CS + 129] = ncol.
@@ -1702,18 +1794,23 @@ int nsc_build_run(rdk_ctx* ctx, const rdk_nsc_params* p) {
     C.hs = ctx->chs.p;
     const int warm = p->warm_mode != 0 ? 1 : 0;  // the learned start corrects the previous-step one
     ProfScope ps(ctx, K_MANIFOLD);
-    hipLaunchKernelGGL(k_manifold, dim3(grid_for(nc, nb)), dim3(nb), 0, ctx->stream, B, C, nc, M, p->cfm, warm,
-                       ctx->prev_keys.p, ctx->nprev, ctx->prev_lam[0].p, ctx->prev_lam[1].p, ctx->prev_lam[2].p);
+    hipLaunchKernelGGL(k_manifold, dim3(grid_for(nc, nb)), dim3(nb), 0, ctx->stream, B, C, Count::host(nc), M, p->cfm,
+                       warm, ctx->prev_keys.p, ctx->nprev, (const int*)nullptr, ctx->prev_lam[0].p, ctx->prev_lam[1].p,
+                       ctx->prev_lam[2].p);
   }
   if (nwall > 0) {
     ProfScope ps(ctx, K_WALL);
     const int wwarm = (p->warm_mode != 0 && p->warm_walls) ? 1 : 0;
     hipLaunchKernelGGL(k_wall_manifold, dim3(grid_for(4 * n, nb)), dim3(nb), 0, ctx->stream, B, ctx->sys_dev.p, n,
-                       ctx->wall_flag.p, ctx->wall_off.p, nc, M, p->cfm, wwarm, ctx->prev_wkeys.p, ctx->nprev_w,
-                       ctx->prev_wlam[0].p, ctx->prev_wlam[1].p, ctx->prev_wlam[2].p);
+                       ctx->wall_flag.p, ctx->wall_off.p, INT_MAX, Count::host(nc), M, p->cfm, wwarm,
+                       ctx->prev_wkeys.p,
+                       ctx->nprev_w, (const int*)nullptr, ctx->prev_wlam[0].p, ctx->prev_wlam[1].p,
+                       ctx->prev_wlam[2].p);
   }
   ctx->nm = nm;
   ctx->nm_wall = nwall;
+  ctx->nc_seen_max = std::max(ctx->nc_seen_max, nc);
+  ctx->nwall_seen_max = std::max(ctx->nwall_seen_max, nwall);
   ctx->have_manifolds = true;
   ctx->have_coloring = false;
   ctx->work[W_MANIFOLDS] += nm;
@@ -1721,7 +1818,10 @@ int nsc_build_run(rdk_ctx* ctx, const rdk_nsc_params* p) {
   return RDK_OK;
 }
 
-static int small_m_limit() {
+// Manifold counts up to this are coloured (and, on the global path, solved)
+// by one workgroup: rdk_set_tuning(RDK_TUNE_SMALL_M), else RDK_SMALL_M, else 16384.
+static int small_m_limit(const rdk_ctx* ctx) {
+  if (ctx->small_m >= 0) return ctx->small_m;
   static int lim = -1;
   if (lim < 0) {
     const char* e = getenv("RDK_SMALL_M");
@@ -1743,8 +1843,8 @@ static int color_compute(rdk_ctx* ctx) {
   RDK_TRY(ensure(ctx, ctx->d_cstart, 132));
   {
     ProfScope ps(ctx, K_CSR);
-    hipLaunchKernelGGL(k_csr_entries, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, ctx->ma.p, ctx->mb.p, nm, n,
-                       ctx->csr_key.p, ctx->csr_val.p);
+    hipLaunchKernelGGL(k_csr_entries, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, ctx->ma.p, ctx->mb.p,
+                       Count::host(nm), n, ctx->csr_key.p, ctx->csr_val.p);
   }
   bool alt = false;
   RDK_TRY(sort_pairs_u32(ctx, ctx->csr_key.p, ctx->csr_key_alt.p, ctx->csr_val.p, ctx->csr_val_alt.p, 2 * nm,
@@ -1753,21 +1853,23 @@ static int color_compute(rdk_ctx* ctx) {
   const int* csr = alt ? ctx->csr_val_alt.p : ctx->csr_val.p;
   {
     ProfScope ps(ctx, K_CSR);
-    hipLaunchKernelGGL(k_body_start, dim3(grid_for(n + 1, nb)), dim3(nb), 0, ctx->stream, skey, 2 * nm, n,
-                       ctx->body_start.p);
+    hipLaunchKernelGGL(k_body_start, dim3(grid_for(n + 1, nb)), dim3(nb), 0, ctx->stream, skey, Count::host(2 * nm),
+                       n, ctx->body_start.p);
   }
   unsigned* dctr = (unsigned*)ctx->uncolored.p;  // [0] left, [1] maxcol, [2] overflow, [3] ncolor
   RDK_HIP(hipMemsetAsync(dctr, 0, 4 * sizeof(unsigned), ctx->stream));
-  const bool small = nm <= small_m_limit();
+  const bool small = nm <= small_m_limit(ctx);
   (void)alt;
   if (small) {
     ProfScope ps(ctx, K_COLOR);
-    hipLaunchKernelGGL(k_color_small, dim3(1), dim3(SMALL_BLOCK), 0, ctx->stream, ctx->ma.p, ctx->mb.p, nm,
-                       ctx->body_start.p, csr, ctx->color.p, ctx->color_tmp.p, dctr);
+    hipLaunchKernelGGL(k_color_small, dim3(1), dim3(SMALL_BLOCK), 0, ctx->stream, ctx->ma.p, ctx->mb.p,
+                       Count::host(nm), ctx->body_start.p, csr, ctx->color.p, ctx->color_tmp.p, dctr, INT_MAX,
+                       (const int*)nullptr);
   } else {
     {
       ProfScope ps(ctx, K_COLOR);
-      hipLaunchKernelGGL(k_fill_color, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, ctx->color.p, nm, -1);
+      hipLaunchKernelGGL(k_fill_color, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, ctx->color.p,
+                         Count::host(nm), -1);
     }
     int rounds = 0;
     unsigned hc[4];
@@ -1776,11 +1878,14 @@ static int color_compute(rdk_ctx* ctx) {
       {
         ProfScope ps(ctx, K_COLOR);
         hipLaunchKernelGGL(k_color_assign, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, ctx->ma.p, ctx->mb.p,
-                           nm, ctx->body_start.p, csr, ctx->color.p, ctx->color_tmp.p, dctr + 2);
+                           Count::host(nm), ctx->body_start.p, csr, ctx->color.p, ctx->color_tmp.p, dctr + 2, -1,
+                           (const unsigned*)nullptr, (const int*)nullptr);
         hipLaunchKernelGGL(k_color_resolve, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, ctx->ma.p, ctx->mb.p,
-                           nm, ctx->body_start.p, csr, ctx->color_tmp.p, ctx->color.p, dctr, dctr + 1);
+                           Count::host(nm), ctx->body_start.p, csr, ctx->color_tmp.p, ctx->color.p, dctr, dctr + 1,
+                           -1, (const unsigned*)nullptr, (const int*)nullptr);
       }
       ++rounds;
+      ctx->color_rounds_max = std::max(ctx->color_rounds_max, rounds);
       RDK_HIP(hipMemcpyAsync(hc, dctr, 4 * sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
       RDK_HIP(hipStreamSynchronize(ctx->stream));
       if (hc[2] != 0) {
@@ -1801,7 +1906,7 @@ static int color_compute(rdk_ctx* ctx) {
 static int color_group(rdk_ctx* ctx, bool need_host) {
   const int nm = ctx->nm;
   const int nb = 256;
-  const bool small = nm <= small_m_limit();
+  const bool small = nm <= small_m_limit(ctx);
   unsigned* dctr = (unsigned*)ctx->uncolored.p;
   bool alt = false;
   RDK_TRY(ensure(ctx, ctx->wall_off, std::max<size_t>(ctx->wall_off.cap, 128)));
@@ -1823,7 +1928,7 @@ static int color_group(rdk_ctx* ctx, bool need_host) {
   {
     ProfScope ps(ctx, K_PERMUTE);
     hipLaunchKernelGGL(k_permute, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, mf_ptrs(ctx, false),
-                       mf_ptrs(ctx, true), ctx->perm.p, nm);
+                       mf_ptrs(ctx, true), ctx->perm.p, Count::host(nm));
   }
   ctx->have_coloring = true;
   ctx->color_host_valid = false;
@@ -1937,7 +2042,7 @@ int nsc_sweeps_run(rdk_ctx* ctx, const rdk_nsc_params* p, int nsweeps) {
   if (ctx->nm > 0) {
     ProfScope ps(ctx, K_PERMUTE);
     hipLaunchKernelGGL(k_unpermute_lambda, dim3(grid_for(ctx->nm, nb)), dim3(nb), 0, ctx->stream,
-                       mf_ptrs(ctx, true), mf_ptrs(ctx, false), ctx->perm.p, ctx->nm);
+                       mf_ptrs(ctx, true), mf_ptrs(ctx, false), ctx->perm.p, Count::host(ctx->nm));
   }
   return RDK_OK;
 }
@@ -1952,8 +2057,8 @@ static int store_warm_cache(rdk_ctx* ctx, const rdk_nsc_params* p) {
     for (int k = 0; k < 3; ++k) RDK_TRY(ensure(ctx, ctx->prev_lam[k], nc));
     ProfScope ps(ctx, K_MISC);
     hipLaunchKernelGGL(k_copy_keys_lambda, dim3(grid_for(nc, nb)), dim3(nb), 0, ctx->stream, ctx->pkeys.p,
-                       mf_ptrs(ctx, false), nc, ctx->prev_keys.p, ctx->prev_lam[0].p, ctx->prev_lam[1].p,
-                       ctx->prev_lam[2].p);
+                       mf_ptrs(ctx, false), Count::host(nc), ctx->prev_keys.p, ctx->prev_lam[0].p, ctx->prev_lam[1].p,
+                       ctx->prev_lam[2].p, (int*)nullptr, (const int*)nullptr);
     ctx->nprev = nc;
   } else {
     ctx->nprev = 0;
@@ -1964,7 +2069,8 @@ static int store_warm_cache(rdk_ctx* ctx, const rdk_nsc_params* p) {
     for (int k = 0; k < 3; ++k) RDK_TRY(ensure(ctx, ctx->prev_wlam[k], nw));
     ProfScope ps(ctx, K_MISC);
     hipLaunchKernelGGL(k_copy_wall_keys_lambda, dim3(grid_for(nw, nb)), dim3(nb), 0, ctx->stream, mf_ptrs(ctx, false),
-                       nc, nw, ctx->prev_wkeys.p, ctx->prev_wlam[0].p, ctx->prev_wlam[1].p, ctx->prev_wlam[2].p);
+                       Count::host(nc), Count::host(nw), ctx->prev_wkeys.p, ctx->prev_wlam[0].p, ctx->prev_wlam[1].p,
+                       ctx->prev_wlam[2].p, (int*)nullptr, (const int*)nullptr);
     ctx->nprev_w = nw;
   } else {
     ctx->nprev_w = 0;
@@ -2198,8 +2304,8 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
   {
     ProfScope ps(ctx, K_ISLAND);
     hipLaunchKernelGGL(k_uf_init, dim3(grid_for(n, nb)), dim3(nb), 0, ctx->stream, ctx->uf_parent.p, n);
-    hipLaunchKernelGGL(k_uf_link, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, ctx->ma.p, ctx->mb.p, nm,
-                       ctx->uf_parent.p);
+    hipLaunchKernelGGL(k_uf_link, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, ctx->ma.p, ctx->mb.p,
+                       Count::host(nm), ctx->uf_parent.p);
     hipLaunchKernelGGL(k_uf_flatten, dim3(grid_for(n, nb)), dim3(nb), 0, ctx->stream, ctx->uf_parent.p, n,
                        ctx->uf_label.p);
     hipLaunchKernelGGL(k_body_label_keys, dim3(grid_for(n, nb)), dim3(nb), 0, ctx->stream, ctx->uf_label.p, n,
@@ -2212,8 +2318,8 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
   const int* sbody = balt ? ctx->sbody_alt.p : ctx->sbody.p;
   {
     ProfScope ps(ctx, K_ISLAND);
-    hipLaunchKernelGGL(k_island_keys, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, ctx->ma.p, ctx->color.p, nm,
-                       ctx->uf_label.p, ctx->color_key.p, ctx->perm_alt.p);
+    hipLaunchKernelGGL(k_island_keys, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, ctx->ma.p, ctx->color.p,
+                       Count::host(nm), ctx->uf_label.p, ctx->color_key.p, ctx->perm_alt.p);
   }
   RDK_TRY(sort_pairs_u32(ctx, ctx->color_key.p, ctx->color_key_alt.p, ctx->perm_alt.p, ctx->perm.p, nm,
                          7 + bits_for((uint64_t)n), malt));
@@ -2223,17 +2329,18 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
   {
     ProfScope ps(ctx, K_PERMUTE);
     hipLaunchKernelGGL(k_permute, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, mf_ptrs(ctx, false),
-                       mf_ptrs(ctx, true), ctx->perm.p, nm);
+                       mf_ptrs(ctx, true), ctx->perm.p, Count::host(nm));
   }
   {
     ProfScope ps(ctx, K_ISLAND);
-    hipLaunchKernelGGL(k_island_flags, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, skey, nm, ctx->iflag.p);
+    hipLaunchKernelGGL(k_island_flags, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, skey, Count::host(nm),
+                       ctx->iflag.p);
   }
   RDK_TRY(exclusive_scan_int(ctx, ctx->iflag.p, ctx->ipos.p, nm));
   {
     ProfScope ps(ctx, K_ISLAND);
     hipLaunchKernelGGL(k_island_starts, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, ctx->iflag.p, ctx->ipos.p,
-                       nm, ctx->istart.p);
+                       Count::host(nm), ctx->istart.p, (int*)nullptr);
   }
   int last[2];
   RDK_HIP(hipMemcpyAsync(last, ctx->ipos.p + nm - 1, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -2252,13 +2359,14 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
   C.lds_bmax = LDS_BCAP;
   {
     ProfScope ps(ctx, K_ISLAND);
-    hipLaunchKernelGGL(k_island_classify, dim3(grid_for(nisl, nb)), dim3(nb), 0, ctx->stream, ctx->istart.p, nisl,
-                       skey, blab, n, C, ctx->ilists.p, ctx->icnt.p, ctx->bseg.p);
+    hipLaunchKernelGGL(k_island_classify, dim3(grid_for(nisl, nb)), dim3(nb), 0, ctx->stream, ctx->istart.p,
+                       Count::host(nisl), skey, blab, n, C, ctx->ilists.p, nisl, ctx->icnt.p, ctx->bseg.p);
   }
   int cnt[6];
   RDK_HIP(hipMemcpyAsync(cnt, ctx->icnt.p, 6 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   RDK_HIP(hipStreamSynchronize(ctx->stream));
   ctx->nislands = nisl;
+  for (int c = 0; c < 4; ++c) ctx->cls_seen_max[c] = std::max(ctx->cls_seen_max[c], cnt[c]);
   if (cnt[5] > 0) return RDK_OK;  // a giant island: global coloured path
   IslandSolve P;
   P.K = solve_const(ctx, p);
@@ -2291,27 +2399,29 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
   if (cnt[0] > 0) {
     ProfScope ps(ctx, K_ISL_TINY, s_side);
     constexpr int ipb = ISL_WG0 / ISL_G0;
-    hipLaunchKernelGGL((k_island_reg<ISL_G0, ISL_WG0>), dim3(grid_for(cnt[0], ipb)), dim3(ISL_WG0), 0, s_side, B, Mc,
-                       ctx->sys_dev.p, ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p, cnt[0], P, S);
+    hipLaunchKernelGGL((k_island_reg<ISL_G0, ISL_WG0, false>), dim3(grid_for(cnt[0], ipb)), dim3(ISL_WG0), 0, s_side, B, Mc,
+                       ctx->sys_dev.p, ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p, Count::host(cnt[0]), P, S,
+                       (const int*)nullptr);
   }
   if (cnt[1] > 0) {
     ProfScope ps(ctx, K_ISL_SMALL, s_small);
-    hipLaunchKernelGGL((k_island_reg<ISL_G1, ISL_WG1>), dim3(cnt[1]), dim3(ISL_WG1), 0, s_small, B, Mc,
-                       ctx->sys_dev.p, ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p + nisl, cnt[1], P, S);
+    hipLaunchKernelGGL((k_island_reg<ISL_G1, ISL_WG1, false>), dim3(cnt[1]), dim3(ISL_WG1), 0, s_small, B, Mc,
+                       ctx->sys_dev.p, ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p + nisl,
+                       Count::host(cnt[1]), P, S, (const int*)nullptr);
   }
   if (cnt[2] > 0) {
     ProfScope ps(ctx, K_ISL_MEDIUM, s_med);
-    hipLaunchKernelGGL((k_island_reg<ISL_G2, ISL_WG2>), dim3(cnt[2]), dim3(ISL_WG2), 0, s_med, B, Mc,
-                       ctx->sys_dev.p, ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p + 2 * nisl, cnt[2], P,
-                       S);
+    hipLaunchKernelGGL((k_island_reg<ISL_G2, ISL_WG2, false>), dim3(cnt[2]), dim3(ISL_WG2), 0, s_med, B, Mc,
+                       ctx->sys_dev.p, ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p + 2 * nisl,
+                       Count::host(cnt[2]), P, S, (const int*)nullptr);
   }
   if (cnt[3] > 0) {
     RDK_TRY(ensure(ctx, ctx->islot, nm));
     RDK_TRY(ensure(ctx, ctx->ihot, nm));
     ProfScope ps(ctx, K_ISL_LARGE);
-    hipLaunchKernelGGL(k_island_lds, dim3(cnt[3]), dim3(LDS_THREADS), 0, ctx->stream, B, Mc, ctx->sys_dev.p,
-                       ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p + 3 * nisl, P, S, ctx->islot.p,
-                       ctx->ihot.p);
+    hipLaunchKernelGGL(k_island_lds<false>, dim3(cnt[3]), dim3(LDS_THREADS), 0, ctx->stream, B, Mc, ctx->sys_dev.p,
+                       ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p + 3 * nisl, Count::host(cnt[3]), P, S,
+                       ctx->islot.p, ctx->ihot.p, (const int*)nullptr);
   }
   HugeOut hu;
   if (cnt[4] > 0) RDK_TRY(huge_islands_solve(ctx, P, skey, ctx->ilists.p + 4 * nisl, cnt[4], s_side, hu));
@@ -2327,7 +2437,7 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
   {
     ProfScope ps(ctx, K_PERMUTE);
     hipLaunchKernelGGL(k_unpermute_lambda, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, Mc, mf_ptrs(ctx, false),
-                       ctx->perm.p, nm);
+                       ctx->perm.p, Count::host(nm));
   }
   IslandStats hs;
   unsigned hc[4];
@@ -2391,7 +2501,7 @@ int nsc_solve_run(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* st) {
 static int global_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* st, bool colored) {
   RDK_TRY(ensure(ctx, ctx->resid, 4));
   const int nm = ctx->nm, nb = 256;
-  const bool small = nm > 0 && nm <= small_m_limit();
+  const bool small = nm > 0 && nm <= small_m_limit(ctx);
   if (!colored) RDK_TRY(nsc_color_run(ctx, false));
   BodyPtrs B = body_ptrs(ctx);
   MfPtrs Mc = mf_ptrs(ctx, true);
@@ -2429,7 +2539,7 @@ static int global_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
     if (p->stop_rule == 1 && !(res <= p->tol)) status = RDK_ERR_NOCONV;
     ProfScope ps(ctx, K_PERMUTE);
     hipLaunchKernelGGL(k_unpermute_lambda, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, Mc, mf_ptrs(ctx, false),
-                       ctx->perm.p, nm);
+                       ctx->perm.p, Count::host(nm));
   } else if (nm > 0) {
     RDK_TRY(color_to_host(ctx));
   }
@@ -2468,7 +2578,7 @@ static int global_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
     }
     ProfScope ps(ctx, K_PERMUTE);
     hipLaunchKernelGGL(k_unpermute_lambda, dim3(grid_for(nm, nb)), dim3(nb), 0, ctx->stream, Mc, mf_ptrs(ctx, false),
-                       ctx->perm.p, nm);
+                       ctx->perm.p, Count::host(nm));
   }
   RDK_TRY(store_warm_cache(ctx, p));
   if (st) {
@@ -2695,7 +2805,7 @@ int nsc_project_run(rdk_ctx* ctx, const rdk_detect_params* dp, const rdk_nsc_par
       RDK_HIP(hipMemsetAsync(D.dth[k], 0, n * sizeof(double), ctx->stream));
     }
     MfPtrs Mc = mf_ptrs(ctx, true);
-    if (m <= small_m_limit()) {
+    if (m <= small_m_limit(ctx)) {
       RDK_TRY(ensure(ctx, ctx->small_out, sizeof(int)));
       ProfScope ps(ctx, K_POSPROJ);
       hipLaunchKernelGGL(k_pos_small, dim3(1), dim3(SMALL_BLOCK), 0, ctx->stream, D, Mc, ctx->d_cstart.p,
@@ -2729,6 +2839,405 @@ int nsc_project_run(rdk_ctx* ctx, const rdk_detect_params* dp, const rdk_nsc_par
   ctx->have_manifolds = false;
   ctx->have_coloring = false;
   if (passes_out) *passes_out = passes;
+  return RDK_OK;
+}
+
+// ---- fixed-capacity (capturable) step ----------------------------------------
+// The launches of the free-flight update, nsc_build_run, island_solve and
+// store_warm_cache with every count left on the device: grids are sized for
+// the reserved capacities and each kernel reads its count when it runs.  The
+// colouring, the islands and every check that can fail depend only on
+// positions and the manifolds' body pairs, so they run first and a failed step
+// has changed no state when the free-flight update is reached.  Colouring runs the
+// single-workgroup kernel (at most small_m_limit manifolds) or a fixed number
+// of rounds that return at once after the last needed one; the island classes
+// that fit one workgroup (tiny, small, medium, LDS) run as persistent launches
+// over their device-side island lists.  What the fixed path cannot do (more
+// contacts than reserved, more rounds or colours than enqueued, an island too
+// large for one workgroup) sets fx[FX_ERR]; the caller restores the state and
+// runs that step eagerly.
+
+constexpr int FX_PERSIST_TINY = 2048, FX_PERSIST_SMALL = 2048, FX_PERSIST_MED = 1024, FX_PERSIST_LDS = 256;
+
+__global__ void k_fx_zero(int* __restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// Per-step device counts after detection: wall manifolds and manifolds.
+__device__ __forceinline__ void k_fx_counts_body(const unsigned* __restrict__ counters, int pair_cap,
+                                                 const int* __restrict__ wall_off, const int* __restrict__ wall_flag,
+                                                 int n4, int wall_cap, int* __restrict__ fx) {
+  // more contacts than reserved: the contact list was cut
+  if (counters[0] > (unsigned)pair_cap) atomicCAS(&fx[FX_ERR], 0, (int)FXE_PAIRS);
+  const int nc = (int)min(counters[0], (unsigned)pair_cap);
+  int nw = n4 > 0 ? wall_off[n4 - 1] + wall_flag[n4 - 1] : 0;
+  if (nw > wall_cap) {
+    atomicCAS(&fx[FX_ERR], 0, (int)FXE_WALLS);
+    nw = wall_cap;
+  }
+  fx[FX_NWALL] = nw;
+  fx[FX_NM] = nc + nw;
+}
+
+// k_uf_init, and (thread 0) the colouring check of a fixed step.
+__global__ void k_fx_uf_init_check(int* __restrict__ parent, int n, const unsigned* __restrict__ dctr,
+                                   int* __restrict__ fx, int small_limit, int rounds) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0 && !fx[FX_ERR]) {
+    const int nm = fx[FX_NM];
+    if (dctr[2] != 0u) atomicCAS(&fx[FX_ERR], 0, (int)FXE_COLOURS);
+    else if (nm <= small_limit ? dctr[0] != 0u : (rounds <= 0 || (unsigned)fx[FX_ROUND0 + rounds - 1] != 0u))
+      atomicCAS(&fx[FX_ERR], 0, (int)FXE_ROUNDS);
+  }
+  if (i < n) parent[i] = i;
+}
+
+__global__ void k_fx_class_check(const int* __restrict__ cnt, int* __restrict__ fx) {
+  if (threadIdx.x == 0 && cnt[4] + cnt[5] > 0) atomicCAS(&fx[FX_ERR], 0, (int)FXE_ISLANDS);
+}
+
+__global__ void k_fx_copy_int(const int* __restrict__ src, int* __restrict__ dst, Count nc) {
+  const int n = nc.get();
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+// Statistics of one completed step (the eager step's rdk_step_stats and work
+// counters), added to the accumulators; FX_STEPS counts completed steps.
+__global__ void k_fx_accumulate(const int* __restrict__ fxc, int* __restrict__ fx, const unsigned* __restrict__ counters,
+                                int pair_cap, const IslandStats* __restrict__ S, const unsigned* __restrict__ dctr,
+                                const int* __restrict__ cnt, double* __restrict__ acc) {
+  if (threadIdx.x != 0 || fxc[FX_ERR]) return;
+  for (int c = 0; c < 4; ++c) acc[FA_CLS0 + c] = fmax(acc[FA_CLS0 + c], (double)cnt[c]);
+  const int nc = (int)min(counters[0], (unsigned)pair_cap);
+  const int nm = fx[FX_NM];
+  const unsigned long long cand = ((unsigned long long)counters[3] << 32) | counters[2];
+  const unsigned long long narrow = ((unsigned long long)counters[5] << 32) | counters[4];
+  fx[FX_STEPS] += 1;
+  acc[FA_STEPS] += 1.0;
+  acc[FA_CONTACTS] += (double)nc;
+  acc[FA_MANIFOLDS] += (double)nm;
+  acc[FA_WALLS] += (double)fx[FX_NWALL];
+  acc[FA_CANDIDATES] += (double)cand;
+  acc[FA_NARROW] += (double)narrow;
+  if (nm > 0) {
+    acc[FA_SWEEPS] += (double)S->max_sweeps;
+    acc[FA_COLORS] += (double)(dctr[1] + 1u);
+    if (S->nonconv > 0) acc[FA_NONCONV] += 1.0;
+    acc[FA_MSWEEPS] += (double)S->manifold_sweeps;
+    acc[FA_LDS_SWEEPS] += (double)S->lds_manifold_sweeps;
+    acc[FA_HOT_SWEEPS] += (double)S->hot_sweeps;
+    const double r = S->res;  // NaN residuals are not folded in (the eager step's std::max)
+    if (r > acc[FA_RESMAX]) acc[FA_RESMAX] = r;
+  }
+}
+
+// The body pairs of the manifolds alone (rod-rod from the contact list, walls
+// from the slot offsets): colouring and islands need nothing else, so a fixed
+// step decides whether it can finish before any state changes.
+// Threads [0, pair_cap): rod-rod manifolds from the contact list; threads
+// [pair_cap, pair_cap + 4 nbody): wall slots (walls = false: none).  Thread 0
+// also stores the step's counts (k_fx_counts).
+__global__ void k_fx_pairs_ab(const int* __restrict__ ca, const int* __restrict__ cb, Count ncc, int pair_cap,
+                              const int* __restrict__ flag, const int* __restrict__ off, int nbody, int wall_cap,
+                              int walls, int* __restrict__ ma, int* __restrict__ mb,
+                              const unsigned* __restrict__ counters, int* __restrict__ fx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nc = ncc.get();
+  if (i == 0) k_fx_counts_body(counters, pair_cap, off, flag, walls ? 4 * nbody : 0, wall_cap, fx);
+  if (i < pair_cap) {
+    if (i >= nc) return;
+    ma[i] = ca[i];
+    mb[i] = cb[i];
+    return;
+  }
+  const int t = i - pair_cap;
+  if (!walls || t >= 4 * nbody || !flag[t] || off[t] >= wall_cap) return;
+  const int m = nc + off[t];
+  ma[m] = t >> 2;
+  mb[m] = -1 - (t & 3);
+}
+
+// Clears the per-step device counters of a fixed step in one launch: the
+// broad-phase counters, the colouring counters, the round counters, the
+// island class counts and the island statistics.
+__global__ void k_fx_clear(unsigned* __restrict__ counters, unsigned* __restrict__ dctr, int* __restrict__ rounds,
+                           int nrounds, int* __restrict__ icnt, int* __restrict__ stats, int nstats) {
+  const int i = threadIdx.x;
+  if (i < 8) counters[i] = 0u;
+  if (i < 4) dctr[i] = 0u;
+  if (i < 8) icnt[i] = 0;
+  if (i < nstats) stats[i] = 0;
+  for (int k = i; k < nrounds; k += blockDim.x) rounds[k] = 0;
+}
+
+int fixed_clear_enqueue(rdk_ctx* ctx) {
+  hipLaunchKernelGGL(k_fx_clear, dim3(1), dim3(256), 0, ctx->stream, ctx->counters.p, (unsigned*)ctx->uncolored.p,
+                     ctx->fx.p + FX_ROUND0, FX_WORDS - FX_ROUND0, ctx->icnt.p, (int*)ctx->istats.p,
+                     (int)(sizeof(IslandStats) / sizeof(int)));
+  return RDK_OK;
+}
+
+static bool fixed_any_wall(rdk_ctx* ctx) {
+  for (auto& s : ctx->sys_host)
+    if (s.floor_wall || s.ceil_wall) return true;
+  return false;
+}
+
+int nsc_fixed_reserve(rdk_ctx* ctx, const rdk_nsc_params* p) {
+  const int n = ctx->n, cap = ctx->fx_pair_cap, mcap = ctx->fx_nm_cap;
+  RDK_TRY(ensure(ctx, ctx->wall_flag, 4 * n + 1));
+  RDK_TRY(ensure(ctx, ctx->wall_off, std::max(4 * n + 1, 128)));
+  RDK_TRY(ensure_manifolds(ctx, mcap));
+  RDK_TRY(ensure(ctx, ctx->body_start, n + 1));
+  RDK_TRY(ensure(ctx, ctx->uncolored, 256));
+  RDK_TRY(ensure(ctx, ctx->uf_parent, n));
+  RDK_TRY(ensure(ctx, ctx->uf_label, n));
+  RDK_TRY(ensure(ctx, ctx->blab, n));
+  RDK_TRY(ensure(ctx, ctx->blab_alt, n));
+  RDK_TRY(ensure(ctx, ctx->sbody, n));
+  RDK_TRY(ensure(ctx, ctx->sbody_alt, n));
+  RDK_TRY(ensure(ctx, ctx->iflag, mcap + 1));
+  RDK_TRY(ensure(ctx, ctx->ipos, mcap + 1));
+  RDK_TRY(ensure(ctx, ctx->istart, mcap + 2));
+  RDK_TRY(ensure(ctx, ctx->ilists, 6 * (size_t)mcap + 6));
+  RDK_TRY(ensure(ctx, ctx->bseg, 2 * (size_t)mcap + 2));
+  RDK_TRY(ensure(ctx, ctx->icnt, 8));
+  RDK_TRY(ensure(ctx, ctx->istats, sizeof(IslandStats)));
+  RDK_TRY(ensure(ctx, ctx->islot, mcap));
+  RDK_TRY(ensure(ctx, ctx->ihot, mcap));
+  // the warm cache of the last step (eager or fixed) is kept
+  RDK_TRY(grow_keep(ctx, ctx->prev_keys, cap, ctx->nprev));
+  for (int k = 0; k < 3; ++k) RDK_TRY(grow_keep(ctx, ctx->prev_lam[k], cap, ctx->nprev));
+  RDK_TRY(grow_keep(ctx, ctx->prev_wkeys, 4 * n + 1, ctx->nprev_w));
+  for (int k = 0; k < 3; ++k) RDK_TRY(grow_keep(ctx, ctx->prev_wlam[k], 4 * n + 1, ctx->nprev_w));
+  (void)p;
+  return RDK_OK;
+}
+
+// Workgroups of the fixed step's class launches: enough for twice the most
+// islands of that class seen in a step, rounded up to a power of two (so the
+// captured step changes rarely), at least 1, at most FX_PERSIST_*.
+void fixed_class_grids(const rdk_ctx* ctx, int* g) {
+  const int ipb[4] = {ISL_WG0 / ISL_G0, 1, 1, 1};
+  const int lim[4] = {FX_PERSIST_TINY, FX_PERSIST_SMALL, FX_PERSIST_MED, FX_PERSIST_LDS};
+  for (int c = 0; c < 4; ++c) {
+    int want = grid_for(2 * (int64_t)ctx->cls_seen_max[c], ipb[c]), p2 = 1;
+    while (p2 < want && p2 < lim[c]) p2 <<= 1;
+    g[c] = std::min(p2, lim[c]);
+  }
+}
+
+int fixed_prepare(rdk_ctx* ctx, const rdk_detect_params* dp, const rdk_nsc_params* np, int pair_cap, int wall_cap) {
+  const int n = ctx->n;
+  ctx->fx_pair_cap = pair_cap;
+  ctx->fx_wall_cap = fixed_any_wall(ctx) ? std::min(wall_cap, 4 * n) : 0;
+  ctx->fx_nm_cap = pair_cap + ctx->fx_wall_cap;
+  ctx->fx_rounds = std::min(FX_WORDS - FX_ROUND0, std::max(8, 2 * ctx->color_rounds_max + 4));
+  fixed_class_grids(ctx, ctx->fx_grid);
+  RDK_TRY(ensure(ctx, ctx->fx, FX_WORDS));
+  RDK_TRY(ensure(ctx, ctx->facc, FA_NUM));
+  RDK_TRY(ensure(ctx, ctx->resid, 4));
+  RDK_TRY(detect_fixed_reserve(ctx, dp, pair_cap));
+  RDK_TRY(nsc_fixed_reserve(ctx, np));
+  RDK_TRY(sort_reserve(ctx, std::max(std::max(ctx->n_chunks, 2 * ctx->fx_nm_cap), std::max(4 * n, pair_cap))));
+  return RDK_OK;
+}
+
+int nsc_fixed_enqueue(rdk_ctx* ctx, const rdk_nsc_params* p) {
+  const int n = ctx->n, cap = ctx->fx_pair_cap, mcap = ctx->fx_nm_cap;
+  const int nb = 256;
+  int* fx = ctx->fx.p;
+  const int* guard = fx + FX_ERR;
+  hipStream_t st = ctx->stream;
+  BodyPtrs B = body_ptrs(ctx);
+  const Count nc = Count::dev((const int*)ctx->counters.p, cap);
+  const Count nm = Count::dev(fx + FX_NM, mcap);
+  const bool any_wall = fixed_any_wall(ctx);
+  unsigned* dctr = (unsigned*)ctx->uncolored.p;  // [0] left, [1] maxcol, [2] overflow, [3] ncolor
+  IslandStats* S = (IslandStats*)ctx->istats.p;
+  const int lim = small_m_limit(ctx);
+  const int R = mcap > lim ? ctx->fx_rounds : 0;
+  // ---- pairs of every manifold (positions only)
+  if (any_wall) {
+    hipLaunchKernelGGL(k_wall_flags, dim3(grid_for(4 * n, nb)), dim3(nb), 0, st, B, ctx->sys_dev.p, n,
+                       ctx->wall_flag.p);
+    RDK_TRY(exclusive_scan_int(ctx, ctx->wall_flag.p, ctx->wall_off.p, Count::host(4 * n), nullptr));
+  }
+  hipLaunchKernelGGL(k_fx_pairs_ab, dim3(grid_for((int64_t)cap + (any_wall ? 4 * n : 0), nb)), dim3(nb), 0, st,
+                     ctx->ca.p, ctx->cb.p, nc, cap, ctx->wall_flag.p, ctx->wall_off.p, n, ctx->fx_wall_cap,
+                     any_wall ? 1 : 0, ctx->ma.p, ctx->mb.p, (const unsigned*)ctx->counters.p, fx);
+  // ---- colouring (color_compute)
+  hipLaunchKernelGGL(k_csr_entries, dim3(grid_for(mcap, nb)), dim3(nb), 0, st, ctx->ma.p, ctx->mb.p, nm, n,
+                     ctx->csr_key.p, ctx->csr_val.p);
+  bool alt = false;
+  RDK_TRY(sort_pairs_u32(ctx, ctx->csr_key.p, ctx->csr_key_alt.p, ctx->csr_val.p, ctx->csr_val_alt.p,
+                         Count::dev(fx + FX_NM, 2 * mcap, 2, 0), bits_for((uint64_t)n), alt, nullptr));
+  const uint32_t* ckey = alt ? ctx->csr_key_alt.p : ctx->csr_key.p;
+  const int* csr = alt ? ctx->csr_val_alt.p : ctx->csr_val.p;
+  hipLaunchKernelGGL(k_body_start, dim3(grid_for(n + 1, nb)), dim3(nb), 0, st, ckey,
+                     Count::dev(fx + FX_NM, 2 * mcap, 2, 0), n, ctx->body_start.p);
+  hipLaunchKernelGGL(k_color_small, dim3(1), dim3(SMALL_BLOCK), 0, st, ctx->ma.p, ctx->mb.p, nm, ctx->body_start.p,
+                     csr, ctx->color.p, ctx->color_tmp.p, dctr, lim, guard);
+  if (R > 0) {
+    Count nmr = nm;
+    nmr.skip_below = lim + 1;  // counts the single workgroup coloured read as 0
+    hipLaunchKernelGGL(k_fill_color, dim3(grid_for(mcap, nb)), dim3(nb), 0, st, ctx->color.p, nmr, -1);
+    for (int r = 0; r < R; ++r) {
+      const unsigned* prev = r > 0 ? (const unsigned*)(fx + FX_ROUND0 + r - 1) : nullptr;
+      hipLaunchKernelGGL(k_color_assign, dim3(grid_for(mcap, nb)), dim3(nb), 0, st, ctx->ma.p, ctx->mb.p, nm,
+                         ctx->body_start.p, csr, ctx->color.p, ctx->color_tmp.p, dctr + 2, lim, prev, guard);
+      hipLaunchKernelGGL(k_color_resolve, dim3(grid_for(mcap, nb)), dim3(nb), 0, st, ctx->ma.p, ctx->mb.p, nm,
+                         ctx->body_start.p, csr, ctx->color_tmp.p, ctx->color.p, (unsigned*)(fx + FX_ROUND0 + r),
+                         dctr + 1, lim, prev, guard);
+    }
+  }
+  // ---- islands (island_solve); the colouring is checked by the first launch
+  hipLaunchKernelGGL(k_fx_uf_init_check, dim3(grid_for(n, nb)), dim3(nb), 0, st, ctx->uf_parent.p, n,
+                     (const unsigned*)dctr, fx, lim, R);
+  hipLaunchKernelGGL(k_uf_link, dim3(grid_for(mcap, nb)), dim3(nb), 0, st, ctx->ma.p, ctx->mb.p, nm,
+                     ctx->uf_parent.p);
+  hipLaunchKernelGGL(k_uf_flatten, dim3(grid_for(n, nb)), dim3(nb), 0, st, ctx->uf_parent.p, n, ctx->uf_label.p);
+  hipLaunchKernelGGL(k_body_label_keys, dim3(grid_for(n, nb)), dim3(nb), 0, st, ctx->uf_label.p, n, ctx->blab.p,
+                     ctx->sbody.p);
+  bool balt = false, malt = false;
+  RDK_TRY(sort_pairs_u32(ctx, ctx->blab.p, ctx->blab_alt.p, ctx->sbody.p, ctx->sbody_alt.p, Count::host(n),
+                         bits_for((uint64_t)n), balt, nullptr));
+  const uint32_t* blab = balt ? ctx->blab_alt.p : ctx->blab.p;
+  const int* sbody = balt ? ctx->sbody_alt.p : ctx->sbody.p;
+  hipLaunchKernelGGL(k_island_keys, dim3(grid_for(mcap, nb)), dim3(nb), 0, st, ctx->ma.p, ctx->color.p, nm,
+                     ctx->uf_label.p, ctx->color_key.p, ctx->perm_alt.p);
+  RDK_TRY(sort_pairs_u32(ctx, ctx->color_key.p, ctx->color_key_alt.p, ctx->perm_alt.p, ctx->perm.p, nm,
+                         7 + bits_for((uint64_t)n), malt, nullptr));
+  if (!malt)
+    hipLaunchKernelGGL(k_fx_copy_int, dim3(grid_for(mcap, nb)), dim3(nb), 0, st, ctx->perm_alt.p, ctx->perm.p, nm);
+  const uint32_t* skey = malt ? ctx->color_key_alt.p : ctx->color_key.p;
+  hipLaunchKernelGGL(k_island_flags, dim3(grid_for(mcap, nb)), dim3(nb), 0, st, skey, nm, ctx->iflag.p);
+  RDK_TRY(exclusive_scan_int(ctx, ctx->iflag.p, ctx->ipos.p, nm, nullptr));
+  hipLaunchKernelGGL(k_island_starts, dim3(grid_for(mcap, nb)), dim3(nb), 0, st, ctx->iflag.p, ctx->ipos.p, nm,
+                     ctx->istart.p, fx + FX_NISL);
+  IslandClass C;
+  C.max_m[0] = ISL_G0;
+  C.max_m[1] = ISL_G1;
+  C.max_m[2] = ISL_G2;
+  C.max_b[0] = ISL_G0 + 8;
+  C.max_b[1] = ISL_G1 + 8;
+  C.max_b[2] = ISL_G2 + 8;
+  C.large_max = 8192;
+  C.lds_bmax = LDS_BCAP;
+  const Count nisl = Count::dev(fx + FX_NISL, mcap);
+  hipLaunchKernelGGL(k_island_classify, dim3(grid_for(mcap, nb)), dim3(nb), 0, st, ctx->istart.p, nisl, skey, blab, n,
+                     C, ctx->ilists.p, mcap, ctx->icnt.p, ctx->bseg.p);
+  hipLaunchKernelGGL(k_fx_class_check, dim3(1), dim3(64), 0, st, (const int*)ctx->icnt.p, fx);
+  // ---- every check is behind: free flight, then the manifolds (nsc_build_run)
+  RDK_TRY(integrate_run(ctx, RDK_NSC_FREE, guard));
+  MfPtrs M = mf_ptrs(ctx, false);
+  {
+    ContactIn Ci;
+    Ci.a = ctx->ca.p;
+    Ci.b = ctx->cb.p;
+    for (int k = 0; k < 3; ++k) {
+      Ci.pa[k] = ctx->cpa[k].p;
+      Ci.pb[k] = ctx->cpb[k].p;
+      Ci.n[k] = ctx->cn[k].p;
+      Ci.sh[k] = ctx->csh[k].p;
+    }
+    Ci.dist = ctx->cdist.p;
+    Ci.hs = ctx->chs.p;
+    const int warm = p->warm_mode != 0 ? 1 : 0;
+    hipLaunchKernelGGL(k_manifold, dim3(grid_for(cap, nb)), dim3(nb), 0, st, B, Ci, nc, M, p->cfm, warm,
+                       ctx->prev_keys.p, 0, (const int*)(fx + FX_NPREV), ctx->prev_lam[0].p, ctx->prev_lam[1].p,
+                       ctx->prev_lam[2].p);
+  }
+  if (any_wall) {
+    const int wwarm = (p->warm_mode != 0 && p->warm_walls) ? 1 : 0;
+    hipLaunchKernelGGL(k_wall_manifold, dim3(grid_for(4 * n, nb)), dim3(nb), 0, st, B, ctx->sys_dev.p, n,
+                       ctx->wall_flag.p, ctx->wall_off.p, ctx->fx_wall_cap, nc, M, p->cfm, wwarm,
+                       ctx->prev_wkeys.p, 0, (const int*)(fx + FX_NPREV_W), ctx->prev_wlam[0].p, ctx->prev_wlam[1].p,
+                       ctx->prev_wlam[2].p);
+  }
+  MfPtrs Mc = mf_ptrs(ctx, true);
+  hipLaunchKernelGGL(k_permute, dim3(grid_for(mcap, nb)), dim3(nb), 0, st, M, Mc, ctx->perm.p, nm);
+  IslandSolve P;
+  P.K = solve_const(ctx, p);
+  P.tol = p->tol;
+  P.max_sweeps = p->max_sweeps;
+  P.check_every = std::max(1, p->check_every);
+  P.stop_rule = p->stop_rule;
+  P.warm = p->warm_mode != 0 ? 1 : 0;
+  P.stall = p->stall_sweeps;
+  P.hot = p->hot_set;
+  // The four classes, concurrently on four streams as in the eager solve when
+  // at least two of them have held islands in a step so far; classes never
+  // seen run on the main stream with one workgroup (they loop over whatever
+  // count a step brings).
+  const int* cnt = ctx->icnt.p;
+  const int* g = ctx->fx_grid;  // sized from the island counts seen so far
+  const bool active[3] = {ctx->cls_seen_max[0] > 0, ctx->cls_seen_max[1] > 0, ctx->cls_seen_max[2] > 0};
+  const int nactive = active[0] + active[1] + active[2] + (ctx->cls_seen_max[3] > 0);
+  const hipStream_t side[3] = {ctx->stream2, ctx->stream3, ctx->stream4};
+  const hipEvent_t joins[3] = {ctx->ev_join, ctx->ev_join3, ctx->ev_join4};
+  hipStream_t cs[3];
+  for (int c = 0; c < 3; ++c) cs[c] = (nactive >= 2 && active[c]) ? side[c] : st;
+  if (nactive >= 2) {
+    RDK_HIP(hipEventRecord(ctx->ev_fork, st));
+    for (int c = 0; c < 3; ++c)
+      if (cs[c] != st) RDK_HIP(hipStreamWaitEvent(cs[c], ctx->ev_fork, 0));
+  }
+  hipLaunchKernelGGL((k_island_reg<ISL_G0, ISL_WG0, true>), dim3(g[0]), dim3(ISL_WG0), 0, cs[0], B, Mc, ctx->sys_dev.p,
+                     ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p, Count::dev(cnt + 0, mcap), P, S, guard);
+  hipLaunchKernelGGL((k_island_reg<ISL_G1, ISL_WG1, true>), dim3(g[1]), dim3(ISL_WG1), 0, cs[1], B, Mc, ctx->sys_dev.p,
+                     ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p + mcap, Count::dev(cnt + 1, mcap), P, S,
+                     guard);
+  hipLaunchKernelGGL((k_island_reg<ISL_G2, ISL_WG2, true>), dim3(g[2]), dim3(ISL_WG2), 0, cs[2], B, Mc, ctx->sys_dev.p,
+                     ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p + 2 * (size_t)mcap,
+                     Count::dev(cnt + 2, mcap), P, S, guard);
+  hipLaunchKernelGGL(k_island_lds<true>, dim3(g[3]), dim3(LDS_THREADS), 0, st, B, Mc, ctx->sys_dev.p, ctx->istart.p,
+                     skey, ctx->bseg.p, sbody, ctx->ilists.p + 3 * (size_t)mcap, Count::dev(cnt + 3, mcap), P, S,
+                     ctx->islot.p, ctx->ihot.p, guard);
+  for (int c = 0; c < 3; ++c) {
+    if (cs[c] == st) continue;
+    RDK_HIP(hipEventRecord(joins[c], cs[c]));
+    RDK_HIP(hipStreamWaitEvent(st, joins[c], 0));
+  }
+  hipLaunchKernelGGL(k_unpermute_lambda, dim3(grid_for(mcap, nb)), dim3(nb), 0, st, Mc, M, ctx->perm.p, nm);
+  // ---- warm cache (store_warm_cache)
+  if (p->warm_mode != 0)
+    hipLaunchKernelGGL(k_copy_keys_lambda, dim3(grid_for(cap, nb)), dim3(nb), 0, st, ctx->pkeys.p, M, nc,
+                       ctx->prev_keys.p, ctx->prev_lam[0].p, ctx->prev_lam[1].p, ctx->prev_lam[2].p, fx + FX_NPREV,
+                       guard);
+  if (p->warm_mode != 0 && p->warm_walls && any_wall)
+    hipLaunchKernelGGL(k_copy_wall_keys_lambda, dim3(grid_for(4 * n, nb)), dim3(nb), 0, st, M, nc,
+                       Count::dev(fx + FX_NWALL, ctx->fx_wall_cap), ctx->prev_wkeys.p, ctx->prev_wlam[0].p,
+                       ctx->prev_wlam[1].p, ctx->prev_wlam[2].p, fx + FX_NPREV_W, guard);
+  hipLaunchKernelGGL(k_fx_accumulate, dim3(1), dim3(64), 0, st, (const int*)fx, fx, (const unsigned*)ctx->counters.p,
+                     cap, (const IslandStats*)S, (const unsigned*)dctr, (const int*)ctx->icnt.p, ctx->facc.p);
+  return RDK_OK;
+}
+
+// Host copies of the last fixed step's counts (what the eager step leaves in
+// the context), and the warm-cache sizes the eager path reads from the host.
+int fixed_sync_host_counts(rdk_ctx* ctx) {
+  int fxh[FX_ROUND0];
+  unsigned ctr[2], dc[4];
+  int cnt[6];
+  RDK_HIP(hipMemcpyAsync(fxh, ctx->fx.p, sizeof(fxh), hipMemcpyDeviceToHost, ctx->stream));
+  RDK_HIP(hipMemcpyAsync(ctr, ctx->counters.p, sizeof(ctr), hipMemcpyDeviceToHost, ctx->stream));
+  RDK_HIP(hipMemcpyAsync(dc, ctx->uncolored.p, sizeof(dc), hipMemcpyDeviceToHost, ctx->stream));
+  RDK_HIP(hipMemcpyAsync(cnt, ctx->icnt.p, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+  RDK_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->nc = (int)std::min<unsigned>(ctr[0], (unsigned)ctx->fx_pair_cap);
+  ctx->nm = fxh[FX_NM];
+  ctx->nm_wall = fxh[FX_NWALL];
+  ctx->nprev = fxh[FX_NPREV];
+  ctx->nprev_w = fxh[FX_NPREV_W];
+  ctx->nislands = fxh[FX_NISL];
+  ctx->ncolor = ctx->nm > 0 ? (int)dc[1] + 1 : 0;
+  for (int c = 0; c < 3; ++c) ctx->island_counts[c] = cnt[c];
+  ctx->island_counts[3] = cnt[3] + cnt[4];
+  ctx->have_manifolds = true;
+  ctx->have_coloring = false;
   return RDK_OK;
 }
 

@@ -15,6 +15,7 @@ RDK_OK, RDK_ERR_HIP, RDK_ERR_ARG, RDK_ERR_CAPACITY, RDK_ERR_STATE, RDK_ERR_NOCON
 RDK_NSC_FREE, RDK_NSC_POS, RDK_VV_HALF1, RDK_VV_HALF2, RDK_CLEAR = range(5)
 MF_FIELDS = 42  # RDK_MF_FIELDS
 MLP_IN, MLP_OUT, MLP_H = 12, 3, 64  # RDK_MLP_IN, RDK_MLP_OUT, hidden width
+TUNE_SMALL_M = 0  # RDK_TUNE_SMALL_M
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -133,6 +134,13 @@ _SIGS = {
                               _ip]),
     "rdk_gen_thermal": (C.c_int, [C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_uint64, _dp, _dp]),
     "rdk_affine_scale": (C.c_int, [C.c_void_p, C.c_double]),
+    "rdk_sort_pairs_u32": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int]),
+    "rdk_sort_pair_keys_u64": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int]),
+    "rdk_exclusive_scan_i32": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    "rdk_set_step_graph": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "rdk_step_graph_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                      C.POINTER(C.c_int64)]),
+    "rdk_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "rdk_mlp_num_params": (C.c_int, []),
     "rdk_mlp_init": (C.c_int, [C.c_void_p, C.c_uint64]),
     "rdk_mlp_set_params": (C.c_int, [C.c_void_p, _fp, C.c_int]),
@@ -563,6 +571,39 @@ class Context:
                  "resid_contacts", "integrate_bodies", "force_contacts", "narrow_calls", "psor_timed_updates",
                  "lds_manifold_sweeps", "hot_sweeps"]
         return {nm: arr[i] for i, nm in enumerate(names)}
+
+    # ---- fixed-capacity / graph-captured hard-contact step ----
+    def set_step_graph(self, mode, pair_capacity=0):
+        """0: eager step; 1: fixed-capacity launches; 2: the fixed step captured in a graph."""
+        self._check(self._L.rdk_set_step_graph(self._h, int(mode), int(pair_capacity)))
+
+    def step_graph_info(self):
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        self._check(self._L.rdk_step_graph_info(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"fixed_steps": a.value, "fallbacks": b.value, "captures": c.value}
+
+    def set_tuning(self, what, value):
+        self._check(self._L.rdk_set_tuning(self._h, int(what), int(value)))
+
+    # ---- the library's device sort and scan on host arrays ----
+    def sort_pairs_u32(self, keys, vals, key_bits, cap=0):
+        k = np.ascontiguousarray(keys, dtype=np.uint32).copy()
+        v = np.ascontiguousarray(vals, dtype=np.int32).copy()
+        assert k.shape == v.shape and k.ndim == 1
+        self._check(self._L.rdk_sort_pairs_u32(self._h, k.size, int(cap), k.ctypes.data, v.ctypes.data,
+                                               int(key_bits)))
+        return k, v
+
+    def sort_pair_keys_u64(self, keys, index_bits, cap=0):
+        k = np.ascontiguousarray(keys, dtype=np.uint64).copy()
+        self._check(self._L.rdk_sort_pair_keys_u64(self._h, k.size, int(cap), k.ctypes.data, int(index_bits)))
+        return k
+
+    def exclusive_scan(self, x, cap=0):
+        a = np.ascontiguousarray(x, dtype=np.int32)
+        out = np.empty_like(a)
+        self._check(self._L.rdk_exclusive_scan_i32(self._h, a.size, int(cap), a.ctypes.data, out.ctypes.data))
+        return out
 
     def affine_scale(self, s):
         self._check(self._L.rdk_affine_scale(self._h, float(s)))
