@@ -89,8 +89,8 @@ def parse():
                     help="override the workload's stall-rule window (hard-contact workloads)")
     ap.add_argument("--step-graph", type=int, default=None, choices=[0, 1, 2],
                     help="hard-contact step: 0 eager (counts read back to the host), 1 fixed-capacity launches, "
-                         "2 the fixed step captured once in a HIP graph and replayed (default 2 on the batched "
-                         "workload)")
+                         "2 the fixed step captured once in a HIP graph and replayed (default 2; the settling "
+                         "workloads' run 1, which times the kernels, is always eager)")
     ap.add_argument("--roof-steps", type=int, default=50,
                     help="batched workload with a captured step: eager steps after the timed run in which the "
                          "dominant kernel is timed with HIP events")
@@ -342,16 +342,27 @@ def run_c2(a, L, scenes):
     ends = {s + k for s, k in blocks}
     marks = sorted(set(starts) | ends | cpu_at)
     rank_ms = {}
+    rank_kt = {}  # kernel -> [timed launches, ms] inside the timed blocks
     run1 = []
+    run1_wc = {}
     acc = None
     cur = warm
     snaps = []
+    wc_names = ("manifolds", "sweep_contacts", "resid_contacts", "psor_timed_updates", "lds_manifold_sweeps",
+                "contacts", "body_passes", "force_contacts")
+    wcb = None
     t1 = time.perf_counter()
     for m in marks:
         cur = advance(ctx, det, p, cur, m, t1, acc=acc)
         if m in ends:  # a timed block ends here
             for nm_, (n_, ms_) in ctx.kernel_times().items():
                 rank_ms[nm_] = rank_ms.get(nm_, 0.0) + ms_
+                kt_ = rank_kt.setdefault(nm_, [0, 0.0])
+                kt_[0] += n_
+                kt_[1] += ms_
+            wce = ctx.work_counters()
+            for key in wc_names:
+                run1_wc[key] = run1_wc.get(key, 0) + wce[key] - wcb[key]
             ctx.set_profiling(False)
             run1.append(acc)
             acc = None
@@ -359,6 +370,7 @@ def run_c2(a, L, scenes):
             snaps.append((m, ctx.download_state()))
         if m in starts:  # a timed block starts here: every kernel timed (resets the counters)
             ctx.set_profiling(True)
+            wcb = ctx.work_counters()
             acc = {}
     ctx.close()
     ranked = sorted(((ms, nm_) for nm_, ms in rank_ms.items() if nm_ in BYTES), reverse=True)
@@ -368,16 +380,18 @@ def run_c2(a, L, scenes):
         f"kernel ms in the blocks: {share}")
 
     # ---- run 2: the same protocol, timed blocks
+    mode = (0 if p.warm_mode >= 2 else 2) if a.step_graph is None else a.step_graph
     ctx = L.Context(0)
     sc.load(ctx)
     if weights is not None:
         ctx.mlp_set_params(weights)
-    ctx.set_profiling(True)
-    ctx.set_profile_mask(1 << names.index(dom))
+    if mode == 0:
+        ctx.set_profiling(True)
+        ctx.set_profile_mask(1 << names.index(dom))
     ctx.step_nsc(det, p, warm)
+    if mode != 0:  # the captured steps: the dominant kernel's launches were timed in run 1
+        ctx.set_step_graph(mode)
     cur = warm
-    wc_names = ("manifolds", "sweep_contacts", "resid_contacts", "psor_timed_updates", "lds_manifold_sweeps",
-                "contacts", "body_passes", "force_contacts")
     acc_wc = {k: 0 for k in wc_names}
     acc_kt = [0, 0.0]
     el = 0.0
@@ -413,6 +427,8 @@ def run_c2(a, L, scenes):
             f"{st['manifolds_total'] / k:.0f}, sweeps/step {st['sweeps_total'] / k:.1f}, residual "
             f"{st['residual_max']:.2e}{'' if same else ' (differs from run 1)'}")
     cur = advance(ctx, det, p, cur, total, t2)  # the rest of the protocol (untimed)
+    ginfo = ctx.step_graph_info()
+    ctx.set_step_graph(0)
     kt_all = ctx.kernel_times()[dom]
     steps = sum(k for _, k in blocks)
     nonconv = sum(ph["nonconverged_steps"] for ph in phases)
@@ -434,9 +450,14 @@ def run_c2(a, L, scenes):
                         f"{warm + 1}..{total}" if len(blocks) > 1 else f"steps {warm + 1}..{total}",
         "seed": sc.extra["seed"], "check_every": a.check_every, "chunk_len": best, "chunk_len_autotune_ms": tune,
         "stall_sweeps": p.stall_sweeps, "stall_omega": p.stall_omega, "hot_set": p.hot_set,
-        "warm_mode": p.warm_mode,
-        "hipgraph": "check blocks of the huge-island solve (sweeps x colours + residual) captured once per island "
-                    "and step, replayed; the step itself reads counts back and is not captured"})
+        "warm_mode": p.warm_mode, "step_mode": mode,
+        "hipgraph": ("run 2 replays one captured fixed-capacity step per step (every count on the device); steps "
+                     "it cannot finish (an island beyond the LDS class, more contacts than reserved) are rolled back "
+                     "and run eagerly, whose huge-island check blocks replay their own captured graphs"
+                     if mode == 2 else "eager step: counts read back to the host; check blocks of the huge-island "
+                     "solve captured per island and step, replayed"),
+        "fixed_steps": ginfo["fixed_steps"], "fallback_steps": ginfo["fallbacks"],
+        "graph_captures": ginfo["captures"]})
     out["value_counts"] = (f"rods x timed steps that reached residual 1e-8 ({converged} of {steps}) / wall time of "
                            f"all {steps} timed steps")
     out["psor"] = {"sweeps_per_step": sum(ph["sweeps_per_step"] * ph["steps"] for ph in phases) / steps,
@@ -451,9 +472,14 @@ def run_c2(a, L, scenes):
                                      f"({os.path.relpath(a.weights, ROOT)})"}.get(p.warm_mode, p.warm_mode)}
     out["phases"] = phases
     out["replay_identical"] = identical
-    out["roofline"] = roofline_from(dom, acc_kt, acc_wc, sc.n, a.workload)
-    out["roofline"]["protocol_launches"] = kt_all[0]
-    out["roofline"]["protocol_avg_launch_ms"] = kt_all[1] / max(kt_all[0], 1)
+    if mode == 0:
+        out["roofline"] = roofline_from(dom, acc_kt, acc_wc, sc.n, a.workload)
+        out["roofline"]["protocol_launches"] = kt_all[0]
+        out["roofline"]["protocol_avg_launch_ms"] = kt_all[1] / max(kt_all[0], 1)
+    else:  # run 1's eager launches of the same steps (counts checked equal above)
+        out["roofline"] = roofline_from(dom, tuple(rank_kt.get(dom, (0, 0.0))), run1_wc, sc.n, a.workload)
+        out["roofline"]["timed_in"] = ("run 1: HIP events around the dominant kernel's eager launches in the same "
+                                       "timed blocks (a captured step cannot be split into kernels by events)")
     out["kernel_ms_in_timed_blocks_run1"] = share
     if a.ab_steps > 0:
         out["learned_warm_start"] = learned_ab(ctx, sc, a, total)

@@ -332,14 +332,15 @@ int rdk_entanglement_noalloc(rdk_ctx* ctx, double cutoff, double* d_sums, uint64
  * inside a step).  Mode 2: mode 1 captured once into a HIP graph and replayed
  * for each step (recaptured when the parameters or buffers change).  Results
  * are bit-identical to mode 0.  pair_capacity = rod-rod contacts reserved per
- * step (0: twice the most that a mode-0 step of this context has found, at
- * least 1024; the first step of a context runs in mode 0 to measure it); wall
- * manifolds reserved: twice the most found, at least 256.  A step the fixed
- * path cannot finish (more contacts or wall manifolds than reserved,
- * colouring beyond its enqueued rounds, an island with more bodies than one
- * workgroup's LDS holds or more than 8192 manifolds) is rolled back to its
- * start and redone by the mode-0 step, which raises the capacities; the fixed
- * path then resumes.  Modes 1 and 2 apply when
+ * step (0: 1.25 times the most that a mode-0 step of this context has
+ * found, at least 1024, rounded up to a power of two; the first step of a
+ * context runs in mode 0 to measure it); wall manifolds are reserved the same
+ * way (at least 256), islands per class for four times the most found.  A step the fixed path cannot finish (more
+ * contacts, wall manifolds or islands than reserved, colouring beyond its
+ * enqueued rounds, an island with more bodies than one workgroup's LDS holds
+ * or more than 8192 manifolds) fails before it changes any state and is
+ * redone by the mode-0 step, which raises the capacities; the fixed path then
+ * resumes.  Modes 1 and 2 apply when
  * position stabilisation, stage energies, kept manifolds and learned starts
  * are off and the island solver is on; otherwise rdk_step_nsc runs mode 0. */
 int rdk_set_step_graph(rdk_ctx* ctx, int mode, int pair_capacity);

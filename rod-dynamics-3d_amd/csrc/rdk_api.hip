@@ -998,7 +998,7 @@ static int fixed_upload_warm_counts(rdk_ctx* ctx) {
 
 static int step_nsc_fixed(rdk_ctx* ctx, const rdk_detect_params* d, const rdk_nsc_params* np, int nsteps,
                           rdk_step_stats& S) {
-  const bool graph = ctx->step_graph_mode == 2;
+  const bool graph = ctx->step_graph_mode == 2 && graphs_enabled();  // RDK_GRAPHS=0: the same launches, uncaptured
   const int chunk_max = graph ? 16 : 1;
   int done = 0;
   bool uploaded = false;
@@ -1015,17 +1015,19 @@ static int step_nsc_fixed(rdk_ctx* ctx, const rdk_detect_params* d, const rdk_ns
       ++done;
       continue;
     }
-    // capacities: twice the most contacts (wall manifolds) an eager step of
-    // this context has built (at least 1024 / 256), or the requested pair
-    // capacity; a step that needs more is redone eagerly, which raises them
+    // capacities: 1.25 times the most contacts (wall manifolds) an eager step
+    // of this context has built, rounded up to a power of two (at least 1024 /
+    // 256), or the requested pair capacity; a step that needs more is redone
+    // eagerly, which raises them.  Tight capacities keep the sorts of a step on
+    // their single-workgroup path as long as the eager step's are.
     auto pow2 = [](int64_t v) {
       int64_t p = 1;
       while (p < v) p <<= 1;
       return (int)std::min<int64_t>(p, 1 << 30);
     };
     const int cap = ctx->step_graph_pair_cap > 0 ? ctx->step_graph_pair_cap
-                                                 : pow2(std::max(1024, 2 * ctx->nc_seen_max));
-    const int wcap = pow2(std::max(256, 2 * ctx->nwall_seen_max));
+                                                 : pow2(std::max<int64_t>(1024, ctx->nc_seen_max * 5 / 4 + 1));
+    const int wcap = pow2(std::max<int64_t>(256, ctx->nwall_seen_max * 5 / 4 + 1));
     int grids[4];
     fixed_class_grids(ctx, grids);
     if (cap != ctx->fx_pair_cap || wcap != ctx->fx_wall_cap || !ctx->fx.p || ctx->n_chunks <= 0 ||

@@ -401,8 +401,9 @@ static inline bool timed(const rdk_ctx* ctx, int k) {
   return ctx->profiling && !ctx->capturing && ((ctx->prof_mask >> k) & 1);
 }
 
-// Graph replay of repeated launch sequences; RDK_GRAPHS=0 turns it off (for
-// tools that cannot follow captured graphs).
+// Graph replay of repeated launch sequences (the huge-island check blocks, the
+// captured fixed-capacity step); RDK_GRAPHS=0 turns it off, launching the same
+// kernels one by one (for tools that cannot follow captured graphs).
 bool graphs_enabled();
 
 // Fixed-capacity step: device word and accumulator layout.

@@ -1342,27 +1342,21 @@ __device__ __forceinline__ void island_reg_block(BodyPtrs B, MfPtrs M, const Sys
                   P.stop_rule == 1 && !(res <= P.tol));
 }
 
-// One group of IPB islands per workgroup.  LOOP: a captured step launches a
-// fixed grid that walks the class's device-side island count (guard: a
-// failed step, nothing is solved); the eager launch has one workgroup per
-// group.
-template <int GROUP, int WG, bool LOOP>
+// One group of IPB islands per workgroup.  A fixed step sizes the grid from
+// the island counts seen so far and reads the count on the device (a count
+// beyond the grid fails the step before this launch); guard: a failed step,
+// nothing is solved.
+template <int GROUP, int WG>
 __global__ void __launch_bounds__(WG) k_island_reg(BodyPtrs B, MfPtrs M, const SysDev* __restrict__ sys,
                                                    const int* __restrict__ istart, const uint32_t* __restrict__ skey,
                                                    const int* __restrict__ bseg, const int* __restrict__ sbody,
                                                    const int* __restrict__ list, Count countc, IslandSolve P,
                                                    IslandStats* S, const int* __restrict__ guard) {
   constexpr int IPB = WG / GROUP;
-  if (!LOOP) {
-    island_reg_block<GROUP, WG>(B, M, sys, istart, skey, bseg, sbody, list, countc.get(), P, S, blockIdx.x);
-    return;
-  }
   if (guard && *guard) return;
   const int count = countc.get();
-  for (int blk = blockIdx.x; blk * IPB < count; blk += gridDim.x) {
-    island_reg_block<GROUP, WG>(B, M, sys, istart, skey, bseg, sbody, list, count, P, S, blk);
-    __syncthreads();  // the next group reuses the LDS
-  }
+  if ((int)blockIdx.x * IPB >= count) return;
+  island_reg_block<GROUP, WG>(B, M, sys, istart, skey, bseg, sbody, list, count, P, S, blockIdx.x);
 }
 
 // island classes: manifolds and bodies per island for the register kernels
@@ -1659,9 +1653,7 @@ __device__ __forceinline__ void island_lds_block(BodyPtrs B, MfPtrs M, const Sys
   }
 }
 
-// One island per workgroup (LOOP: a captured step launches a fixed grid that
-// walks the class's device-side count).
-template <bool LOOP>
+// One island per workgroup (a fixed step sizes the grid as for k_island_reg).
 __global__ void __launch_bounds__(LDS_THREADS) k_island_lds(BodyPtrs B, MfPtrs M, const SysDev* __restrict__ sys,
                                                            const int* __restrict__ istart,
                                                            const uint32_t* __restrict__ skey,
@@ -1669,16 +1661,9 @@ __global__ void __launch_bounds__(LDS_THREADS) k_island_lds(BodyPtrs B, MfPtrs M
                                                            const int* __restrict__ list, Count countc, IslandSolve P,
                                                            IslandStats* S, int2* __restrict__ slot,
                                                            int* __restrict__ hlist, const int* __restrict__ guard) {
-  if (!LOOP) {
-    island_lds_block(B, M, sys, istart, skey, bseg, sbody, list, P, S, slot, hlist, blockIdx.x);
-    return;
-  }
   if (guard && *guard) return;
-  const int count = countc.get();
-  for (int idx = blockIdx.x; idx < count; idx += gridDim.x) {
-    island_lds_block(B, M, sys, istart, skey, bseg, sbody, list, P, S, slot, hlist, idx);
-    __syncthreads();  // the next island reuses the LDS
-  }
+  if ((int)blockIdx.x >= countc.get()) return;
+  island_lds_block(B, M, sys, istart, skey, bseg, sbody, list, P, S, slot, hlist, blockIdx.x);
 }
 
 // Colour starts of listed islands (the huge class), for the host-driven
@@ -2243,7 +2228,9 @@ static int huge_islands_solve(rdk_ctx* ctx, const IslandSolve& P, const uint32_t
             rc = RDK_ERR_HIP;
             break;
           }
+          ctx->capturing = true;  // no allocation, no timed launch inside the capture
           for (int b = 0; b < nblk; ++b) enqueue_block(K, true, false);
+          ctx->capturing = false;
           if (hipStreamEndCapture(st, &graph[g]) != hipSuccess ||
               hipGraphInstantiate(&exec[g], graph[g], nullptr, nullptr, 0) != hipSuccess) {
             rc = RDK_ERR_HIP;
@@ -2399,19 +2386,19 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
   if (cnt[0] > 0) {
     ProfScope ps(ctx, K_ISL_TINY, s_side);
     constexpr int ipb = ISL_WG0 / ISL_G0;
-    hipLaunchKernelGGL((k_island_reg<ISL_G0, ISL_WG0, false>), dim3(grid_for(cnt[0], ipb)), dim3(ISL_WG0), 0, s_side, B, Mc,
+    hipLaunchKernelGGL((k_island_reg<ISL_G0, ISL_WG0>), dim3(grid_for(cnt[0], ipb)), dim3(ISL_WG0), 0, s_side, B, Mc,
                        ctx->sys_dev.p, ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p, Count::host(cnt[0]), P, S,
                        (const int*)nullptr);
   }
   if (cnt[1] > 0) {
     ProfScope ps(ctx, K_ISL_SMALL, s_small);
-    hipLaunchKernelGGL((k_island_reg<ISL_G1, ISL_WG1, false>), dim3(cnt[1]), dim3(ISL_WG1), 0, s_small, B, Mc,
+    hipLaunchKernelGGL((k_island_reg<ISL_G1, ISL_WG1>), dim3(cnt[1]), dim3(ISL_WG1), 0, s_small, B, Mc,
                        ctx->sys_dev.p, ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p + nisl,
                        Count::host(cnt[1]), P, S, (const int*)nullptr);
   }
   if (cnt[2] > 0) {
     ProfScope ps(ctx, K_ISL_MEDIUM, s_med);
-    hipLaunchKernelGGL((k_island_reg<ISL_G2, ISL_WG2, false>), dim3(cnt[2]), dim3(ISL_WG2), 0, s_med, B, Mc,
+    hipLaunchKernelGGL((k_island_reg<ISL_G2, ISL_WG2>), dim3(cnt[2]), dim3(ISL_WG2), 0, s_med, B, Mc,
                        ctx->sys_dev.p, ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p + 2 * nisl,
                        Count::host(cnt[2]), P, S, (const int*)nullptr);
   }
@@ -2419,7 +2406,7 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
     RDK_TRY(ensure(ctx, ctx->islot, nm));
     RDK_TRY(ensure(ctx, ctx->ihot, nm));
     ProfScope ps(ctx, K_ISL_LARGE);
-    hipLaunchKernelGGL(k_island_lds<false>, dim3(cnt[3]), dim3(LDS_THREADS), 0, ctx->stream, B, Mc, ctx->sys_dev.p,
+    hipLaunchKernelGGL(k_island_lds, dim3(cnt[3]), dim3(LDS_THREADS), 0, ctx->stream, B, Mc, ctx->sys_dev.p,
                        ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p + 3 * nisl, Count::host(cnt[3]), P, S,
                        ctx->islot.p, ctx->ihot.p, (const int*)nullptr);
   }
@@ -2857,8 +2844,6 @@ int nsc_project_run(rdk_ctx* ctx, const rdk_detect_params* dp, const rdk_nsc_par
 // large for one workgroup) sets fx[FX_ERR]; the caller restores the state and
 // runs that step eagerly.
 
-constexpr int FX_PERSIST_TINY = 2048, FX_PERSIST_SMALL = 2048, FX_PERSIST_MED = 1024, FX_PERSIST_LDS = 256;
-
 __global__ void k_fx_zero(int* __restrict__ p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
@@ -2893,8 +2878,12 @@ __global__ void k_fx_uf_init_check(int* __restrict__ parent, int n, const unsign
   if (i < n) parent[i] = i;
 }
 
-__global__ void k_fx_class_check(const int* __restrict__ cnt, int* __restrict__ fx) {
-  if (threadIdx.x == 0 && cnt[4] + cnt[5] > 0) atomicCAS(&fx[FX_ERR], 0, (int)FXE_ISLANDS);
+// An island the fixed step cannot solve (more bodies than the LDS class holds,
+// more than 8192 manifolds), or more islands in a class than its grid holds.
+__global__ void k_fx_class_check(const int* __restrict__ cnt, int c0, int c1, int c2, int c3, int* __restrict__ fx) {
+  if (threadIdx.x != 0) return;
+  if (cnt[4] + cnt[5] > 0 || cnt[0] > c0 || cnt[1] > c1 || cnt[2] > c2 || cnt[3] > c3)
+    atomicCAS(&fx[FX_ERR], 0, (int)FXE_ISLANDS);
 }
 
 __global__ void k_fx_copy_int(const int* __restrict__ src, int* __restrict__ dst, Count nc) {
@@ -3016,16 +3005,20 @@ int nsc_fixed_reserve(rdk_ctx* ctx, const rdk_nsc_params* p) {
   return RDK_OK;
 }
 
-// Workgroups of the fixed step's class launches: enough for twice the most
-// islands of that class seen in a step, rounded up to a power of two (so the
-// captured step changes rarely), at least 1, at most FX_PERSIST_*.
+// Workgroups of the fixed step's class launches: enough for four times the
+// most islands of that class seen in a step (at least 16, 64, 16 and 4
+// workgroups), rounded up to a power of two, so that a growing packing needs
+// few recaptures.  A step with more islands in a class fails its class check
+// and is redone eagerly, which raises the count.  Workgroups past the count
+// return at once.
 void fixed_class_grids(const rdk_ctx* ctx, int* g) {
   const int ipb[4] = {ISL_WG0 / ISL_G0, 1, 1, 1};
-  const int lim[4] = {FX_PERSIST_TINY, FX_PERSIST_SMALL, FX_PERSIST_MED, FX_PERSIST_LDS};
+  const int gmin[4] = {16, 64, 16, 4};
   for (int c = 0; c < 4; ++c) {
-    int want = grid_for(2 * (int64_t)ctx->cls_seen_max[c], ipb[c]), p2 = 1;
-    while (p2 < want && p2 < lim[c]) p2 <<= 1;
-    g[c] = std::min(p2, lim[c]);
+    const int64_t want = std::max<int64_t>(gmin[c], grid_for(4 * (int64_t)ctx->cls_seen_max[c], ipb[c]));
+    int p2 = 1;
+    while (p2 < want && p2 < (1 << 24)) p2 <<= 1;
+    g[c] = p2;
   }
 }
 
@@ -3129,7 +3122,9 @@ int nsc_fixed_enqueue(rdk_ctx* ctx, const rdk_nsc_params* p) {
   const Count nisl = Count::dev(fx + FX_NISL, mcap);
   hipLaunchKernelGGL(k_island_classify, dim3(grid_for(mcap, nb)), dim3(nb), 0, st, ctx->istart.p, nisl, skey, blab, n,
                      C, ctx->ilists.p, mcap, ctx->icnt.p, ctx->bseg.p);
-  hipLaunchKernelGGL(k_fx_class_check, dim3(1), dim3(64), 0, st, (const int*)ctx->icnt.p, fx);
+  const int* g = ctx->fx_grid;  // sized from the island counts seen so far
+  hipLaunchKernelGGL(k_fx_class_check, dim3(1), dim3(64), 0, st, (const int*)ctx->icnt.p, g[0] * (ISL_WG0 / ISL_G0),
+                     g[1], g[2], g[3], fx);
   // ---- every check is behind: free flight, then the manifolds (nsc_build_run)
   RDK_TRY(integrate_run(ctx, RDK_NSC_FREE, guard));
   MfPtrs M = mf_ptrs(ctx, false);
@@ -3173,7 +3168,6 @@ int nsc_fixed_enqueue(rdk_ctx* ctx, const rdk_nsc_params* p) {
   // seen run on the main stream with one workgroup (they loop over whatever
   // count a step brings).
   const int* cnt = ctx->icnt.p;
-  const int* g = ctx->fx_grid;  // sized from the island counts seen so far
   const bool active[3] = {ctx->cls_seen_max[0] > 0, ctx->cls_seen_max[1] > 0, ctx->cls_seen_max[2] > 0};
   const int nactive = active[0] + active[1] + active[2] + (ctx->cls_seen_max[3] > 0);
   const hipStream_t side[3] = {ctx->stream2, ctx->stream3, ctx->stream4};
@@ -3185,15 +3179,15 @@ int nsc_fixed_enqueue(rdk_ctx* ctx, const rdk_nsc_params* p) {
     for (int c = 0; c < 3; ++c)
       if (cs[c] != st) RDK_HIP(hipStreamWaitEvent(cs[c], ctx->ev_fork, 0));
   }
-  hipLaunchKernelGGL((k_island_reg<ISL_G0, ISL_WG0, true>), dim3(g[0]), dim3(ISL_WG0), 0, cs[0], B, Mc, ctx->sys_dev.p,
+  hipLaunchKernelGGL((k_island_reg<ISL_G0, ISL_WG0>), dim3(g[0]), dim3(ISL_WG0), 0, cs[0], B, Mc, ctx->sys_dev.p,
                      ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p, Count::dev(cnt + 0, mcap), P, S, guard);
-  hipLaunchKernelGGL((k_island_reg<ISL_G1, ISL_WG1, true>), dim3(g[1]), dim3(ISL_WG1), 0, cs[1], B, Mc, ctx->sys_dev.p,
+  hipLaunchKernelGGL((k_island_reg<ISL_G1, ISL_WG1>), dim3(g[1]), dim3(ISL_WG1), 0, cs[1], B, Mc, ctx->sys_dev.p,
                      ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p + mcap, Count::dev(cnt + 1, mcap), P, S,
                      guard);
-  hipLaunchKernelGGL((k_island_reg<ISL_G2, ISL_WG2, true>), dim3(g[2]), dim3(ISL_WG2), 0, cs[2], B, Mc, ctx->sys_dev.p,
+  hipLaunchKernelGGL((k_island_reg<ISL_G2, ISL_WG2>), dim3(g[2]), dim3(ISL_WG2), 0, cs[2], B, Mc, ctx->sys_dev.p,
                      ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p + 2 * (size_t)mcap,
                      Count::dev(cnt + 2, mcap), P, S, guard);
-  hipLaunchKernelGGL(k_island_lds<true>, dim3(g[3]), dim3(LDS_THREADS), 0, st, B, Mc, ctx->sys_dev.p, ctx->istart.p,
+  hipLaunchKernelGGL(k_island_lds, dim3(g[3]), dim3(LDS_THREADS), 0, st, B, Mc, ctx->sys_dev.p, ctx->istart.p,
                      skey, ctx->bseg.p, sbody, ctx->ilists.p + 3 * (size_t)mcap, Count::dev(cnt + 3, mcap), P, S,
                      ctx->islot.p, ctx->ihot.p, guard);
   for (int c = 0; c < 3; ++c) {
