@@ -147,22 +147,36 @@ def cores():
     return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
 
 
-def autotune(ctx, run_steps, cands, reps=3):
+def autotune(ctx, run_steps, cands, reps=3, blocks=(64, 128, 256, 512)):
     """Time `reps` steps (after one untimed step) for each chunk length in
-    `cands`; detection results do not depend on the chunking, so these are
-    ordinary steps of the run.  Returns (best chunk length, ms per step by
-    candidate, steps taken)."""
-    tune = {}
-    for cl in cands:
+    `cands`, then, at the best one, for each broad-phase workgroup size in
+    `blocks`; the contact set depends on neither, so these are ordinary steps
+    of the run.  Leaves the best workgroup size set.  Returns (best chunk
+    length, {"chunk_len_ms": ms per step by chunk length, "broad_block": best
+    workgroup size, "broad_block_ms": ms per step by workgroup size}, steps
+    taken)."""
+    from rod_dynamics_py import _lib as L
+
+    def timed(cl):
         run_steps(cl, 1)
         ctx.sync()
         t = time.perf_counter()
         run_steps(cl, reps)
         ctx.sync()
-        tune[cl] = 1e3 * (time.perf_counter() - t) / reps
+        return 1e3 * (time.perf_counter() - t) / reps
+
+    tune = {cl: timed(cl) for cl in cands}
     best = min(tune, key=tune.get)
-    log(f"autotune chunk_len {best} ({', '.join(f'{k}: {v:.2f} ms' for k, v in tune.items())})")
-    return best, {str(k): v for k, v in tune.items()}, len(cands) * (reps + 1)
+    btune = {}
+    for b in blocks:
+        ctx.set_tuning(L.TUNE_BROAD_BLOCK, b)
+        btune[b] = timed(best)
+    bbest = min(btune, key=btune.get)
+    ctx.set_tuning(L.TUNE_BROAD_BLOCK, bbest)
+    log(f"autotune chunk_len {best} ({', '.join(f'{k}: {v:.2f} ms' for k, v in tune.items())}); broad-phase "
+        f"workgroup {bbest} ({', '.join(f'{k}: {v:.2f} ms' for k, v in btune.items())})")
+    return best, {"chunk_len_ms": {str(k): v for k, v in tune.items()}, "broad_block": bbest,
+                  "broad_block_ms": {str(k): v for k, v in btune.items()}}, (len(cands) + len(blocks)) * (reps + 1)
 
 
 def rank_kernels(ctx, run_warmup, fixed=None):
@@ -383,6 +397,8 @@ def run_c2(a, L, scenes):
     mode = (0 if p.warm_mode >= 2 else 2) if a.step_graph is None else a.step_graph
     ctx = L.Context(0)
     sc.load(ctx)
+    if tune:
+        ctx.set_tuning(L.TUNE_BROAD_BLOCK, tune["broad_block"])
     if weights is not None:
         ctx.mlp_set_params(weights)
     if mode == 0:
@@ -448,7 +464,7 @@ def run_c2(a, L, scenes):
         "protocol_steps": total, "warmup_steps": warm,
         "timed_blocks": f"{len(blocks)} block(s) of {blocks[0][1]} step(s), centred in equal strata of steps "
                         f"{warm + 1}..{total}" if len(blocks) > 1 else f"steps {warm + 1}..{total}",
-        "seed": sc.extra["seed"], "check_every": a.check_every, "chunk_len": best, "chunk_len_autotune_ms": tune,
+        "seed": sc.extra["seed"], "check_every": a.check_every, "chunk_len": best, "autotune": tune,
         "stall_sweeps": p.stall_sweeps, "stall_omega": p.stall_omega, "hot_set": p.hot_set,
         "warm_mode": p.warm_mode, "step_mode": mode,
         "hipgraph": ("run 2 replays one captured fixed-capacity step per step (every count on the device); steps "
@@ -596,14 +612,14 @@ def run_vv(a, L, scenes, which):
                "box": round(sc.extra["side"], 3), "periodic": "x,y,z", "mu": 0.5, "E": 1e7, "nu": 0.25,
                "restitution": 0.5, "dt": 1e-4, "steps_total": max(a.warmup, used + 1) + a.steps,
                "seed": sc.extra["seed"], "force_evaluations_per_step": 2, "chunk_len": sc.detect.chunk_len,
-               "chunk_len_autotune_ms": tune}
+               "autotune": tune}
     else:
         cfg = {"workload": "dense polydisperse Hertz-Mindlin (C3)", "rods": sc.n,
                "L": "lognormal median 16 sigma_log 0.3, truncated [4, 64]", "mean_L": round(sc.extra["mean_length"], 3),
                "phi": 0.30, "box": round(sc.extra["side_now"], 3), "periodic": "x,y,z", "mu": 0.3, "E": 1e7,
                "nu": 0.25, "restitution": 0.5, "dt": 5e-5, "strain_rate": 1e-3, "steps_timed": a.steps,
                "seed": sc.extra["seed"], "force_evaluations_per_step": 2, "chunk_len": sc.detect.chunk_len,
-               "chunk_len_autotune_ms": tune}
+               "autotune": tune}
         cfg.update(prep)
     out = base_line("rod-steps/s", sc.n * a.steps / el, a, el, "f64", cfg)
     out["contacts_per_evaluation"] = st["contacts_total"] / (2 * max(st["steps"], 1))
@@ -717,7 +733,7 @@ def run_c4(a, L, scenes):
         "phi": "U[min(0.05,cap), cap], cap = min(0.25, 0.85 x RSA saturation)", "periodic": "x,y,z",
         "restitution": sc.extra["restitution"], "kT": sc.extra["kT"], "gravity": 0.0, "omega": 1.3,
         "residual_tol": 1e-8, "dt": 1e-3, "redraws": sc.extra["redraws"], "seed": sc.extra["seed"],
-        "chunk_len_autotuned": best, "autotune_ms_per_step": tune,
+        "chunk_len_autotuned": best, "autotune": tune,
         "steps_total": tuned_steps + a.warmup + a.steps + (1 + a.roof_steps if mode != 0 else 0),
         "stall_sweeps": p.stall_sweeps,
         "stall_omega": p.stall_omega,

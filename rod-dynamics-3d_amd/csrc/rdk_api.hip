@@ -1131,6 +1131,12 @@ int rdk_set_tuning(rdk_ctx* ctx, int what, int value) {
     drop_step_graph(ctx);
     return RDK_OK;
   }
+  if (what == RDK_TUNE_BROAD_BLOCK) {
+    if (value != 64 && value != 128 && value != 256 && value != 512) return RDK_ERR_ARG;
+    ctx->bn_threads = value;
+    drop_step_graph(ctx);
+    return RDK_OK;
+  }
   return RDK_ERR_ARG;
 }
 

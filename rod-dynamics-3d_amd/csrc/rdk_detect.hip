@@ -213,7 +213,6 @@ struct StencilWalk {
   }
 };
 
-constexpr int BN_THREADS = 256;
 constexpr int BN_QUEUE = 128;  // per wavefront: < 64 pending + one append round of <= 64
 
 // Separating-axis lower bound on the distance between the axis segments of
@@ -320,14 +319,19 @@ constexpr int BN_UNROLL = 2;  // candidates loaded per lane per step (independen
 // 64, all lanes run the FP64 narrow phase on one entry each.  The expensive
 // stages thus run without divergence, and nothing is appended to global
 // memory but the contacts themselves.
-__global__ void __launch_bounds__(BN_THREADS, 4) k_broad_narrow(BodyPtrs B, const SysDev* __restrict__ sys,
-                                                             SortedChunks SC, const int* __restrict__ cstart,
-                                                             const int* __restrict__ rod_nchunk, int M, double delta,
-                                                             uint64_t* __restrict__ out, unsigned cap,
-                                                             unsigned* __restrict__ counter,
-                                                             unsigned long long* __restrict__ cand_counter) {
-  __shared__ int2 queue[BN_THREADS / 64][BN_QUEUE];
-  __shared__ int2 queue2[BN_THREADS / 64][BN_QUEUE];
+// NT = workgroup size (64 ... 512; rdk_set_tuning(RDK_TUNE_BROAD_BLOCK)): the
+// queues are per wavefront, so NT only changes how many wavefronts share a
+// workgroup slot and the granularity of the XCD-contiguous block order; the
+// contact set does not depend on it.
+template <int NT>
+__global__ void __launch_bounds__(NT, 4) k_broad_narrow(BodyPtrs B, const SysDev* __restrict__ sys,
+                                                        SortedChunks SC, const int* __restrict__ cstart,
+                                                        const int* __restrict__ rod_nchunk, int M, double delta,
+                                                        uint64_t* __restrict__ out, unsigned cap,
+                                                        unsigned* __restrict__ counter,
+                                                        unsigned long long* __restrict__ cand_counter) {
+  __shared__ int2 queue[NT / 64][BN_QUEUE];
+  __shared__ int2 queue2[NT / 64][BN_QUEUE];
   int2* Q = queue[threadIdx.x >> 6];
   int2* Q2 = queue2[threadIdx.x >> 6];
   const int lane = __lane_id();
@@ -615,6 +619,27 @@ static int ensure_contacts(rdk_ctx* ctx, int nc) {
   return RDK_OK;
 }
 
+// The broad phase with the context's workgroup size.
+static int launch_broad_narrow(rdk_ctx* ctx, const BodyPtrs& B, const SortedChunks& SC, int M, double delta,
+                               unsigned cap) {
+  const int nt = ctx->bn_threads;
+  auto go = [&](auto kern, int t) {
+    hipLaunchKernelGGL(kern, dim3(grid_for(M, t)), dim3(t), 0, ctx->stream, B, ctx->sys_dev.p, SC, ctx->cell_start.p,
+                       ctx->rod_nchunk.p, M, delta, ctx->pkeys.p, cap, ctx->counters.p,
+                       (unsigned long long*)(ctx->counters.p + 2));
+  };
+  switch (nt) {
+    case 64: go(k_broad_narrow<64>, 64); break;
+    case 128: go(k_broad_narrow<128>, 128); break;
+    case 256: go(k_broad_narrow<256>, 256); break;
+    case 512: go(k_broad_narrow<512>, 512); break;
+    default:
+      ctx->err = "broad phase: workgroup size must be 64, 128, 256 or 512";
+      return RDK_ERR_ARG;
+  }
+  return RDK_OK;
+}
+
 static ContactPtrs contact_ptrs(rdk_ctx* ctx) {
   ContactPtrs C;
   C.a = ctx->ca.p;
@@ -674,9 +699,7 @@ int detect_fixed_enqueue(rdk_ctx* ctx, const rdk_detect_params* p) {
                      ctx->chunk_k.p, ctx->rod_nchunk.p, M, SC);
   // counters[] were cleared by fixed_clear_enqueue; more contacts than cap fail
   // the step in k_fx_counts
-  hipLaunchKernelGGL(k_broad_narrow, dim3(grid_for(M, BN_THREADS)), dim3(BN_THREADS), 0, ctx->stream, B,
-                     ctx->sys_dev.p, SC, ctx->cell_start.p, ctx->rod_nchunk.p, M, p->delta, ctx->pkeys.p,
-                     (unsigned)cap, ctx->counters.p, (unsigned long long*)(ctx->counters.p + 2));
+  RDK_TRY(launch_broad_narrow(ctx, B, SC, M, p->delta, (unsigned)cap));
   const Count nc = Count::dev((const int*)ctx->counters.p, cap);
   bool kalt = false;
   RDK_TRY(sort_pair_keys_u64(ctx, ctx->pkeys.p, ctx->pkeys_alt.p, nc, bits_for((uint64_t)ctx->n), kalt, nullptr));
@@ -738,9 +761,7 @@ int detect_run(rdk_ctx* ctx, const rdk_detect_params* p) {
     RDK_HIP(hipMemsetAsync(ctx->counters.p, 0, 8 * sizeof(unsigned), ctx->stream));
     {
       ProfScope ps(ctx, K_BROAD_NARROW);
-      hipLaunchKernelGGL(k_broad_narrow, dim3(grid_for(M, BN_THREADS)), dim3(BN_THREADS), 0, ctx->stream, B,
-                         ctx->sys_dev.p, SC, ctx->cell_start.p, ctx->rod_nchunk.p, M, p->delta, ctx->pkeys.p, cap,
-                         ctx->counters.p, (unsigned long long*)(ctx->counters.p + 2));
+      RDK_TRY(launch_broad_narrow(ctx, B, SC, M, p->delta, cap));
     }
     RDK_HIP(hipMemcpyAsync(ctx->h_counters, ctx->counters.p, 6 * sizeof(unsigned), hipMemcpyDeviceToHost,
                            ctx->stream));

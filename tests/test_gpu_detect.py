@@ -48,6 +48,25 @@ def test_detect_bit_identical_polydisperse(gpu_ctx_factory, periodic, chunk_len)
         compare(ctx.contacts(nc), W.contacts(no), nc)
 
 
+@pytest.mark.parametrize("block", [64, 128, 512])
+def test_detect_bit_identical_for_every_broad_phase_workgroup_size(gpu_ctx_factory, block):
+    """The autotuned broad-phase workgroup size changes the launch shape only:
+    the contact list equals the oracle's for each size."""
+    n, side = 3000, 30.0
+    x, q, r, h = rods(n, side, 6)
+    print(f"detect with broad-phase workgroup {block}", flush=True)
+    ctx = gpu_ctx_factory()
+    load(ctx, x, q, r, h, np.ones(n), [0] * 3, [side] * 3, (1, 1, 1))
+    ctx.set_tuning(L.TUNE_BROAD_BLOCK, block)
+    W = O.World(x, q, None, None, r, h, np.ones(n))
+    W.set_box([0] * 3, [side] * 3, (1, 1, 1))
+    for chunk_len in (0.0, 1.0):
+        nc = ctx.detect(L.detect_params(0.0, chunk_len, 1.0))
+        no = W.detect(0, 0.0)
+        assert nc == no and nc > 100
+        compare(ctx.contacts(nc), W.contacts(no), nc)
+
+
 def test_detect_positions_outside_box_and_static_pairs(gpu_ctx_factory):
     n, side = 1500, 20.0
     x, q, r, h = rods(n, side, 9)
