@@ -365,27 +365,46 @@ def run_c2(a, L, scenes):
     wc_names = ("manifolds", "sweep_contacts", "resid_contacts", "psor_timed_updates", "lds_manifold_sweeps",
                 "contacts", "body_passes", "force_contacts")
     wcb = None
+    # Outside the timed blocks the kernels with a byte model are timed too, so
+    # run 1 also gives each one's per-launch average over the whole protocol
+    # (the figure rocprofv3 statistics of the same command reproduce).
+    cand_mask = sum(1 << names.index(k) for k in BYTES if k in names)
+    all_mask = (1 << 64) - 1
+    proto_kt = {}
+
+    def absorb(into):
+        for nm_, (n_, ms_) in ctx.kernel_times().items():
+            kt_ = into.setdefault(nm_, [0, 0.0])
+            kt_[0] += n_
+            kt_[1] += ms_
+
+    ctx.set_profiling(True)
+    ctx.set_profile_mask(cand_mask)
     t1 = time.perf_counter()
     for m in marks:
         cur = advance(ctx, det, p, cur, m, t1, acc=acc)
         if m in ends:  # a timed block ends here
             for nm_, (n_, ms_) in ctx.kernel_times().items():
                 rank_ms[nm_] = rank_ms.get(nm_, 0.0) + ms_
-                kt_ = rank_kt.setdefault(nm_, [0, 0.0])
-                kt_[0] += n_
-                kt_[1] += ms_
+            absorb(rank_kt)
+            absorb(proto_kt)
             wce = ctx.work_counters()
             for key in wc_names:
                 run1_wc[key] = run1_wc.get(key, 0) + wce[key] - wcb[key]
-            ctx.set_profiling(False)
+            ctx.set_profiling(True)  # resets the counters
+            ctx.set_profile_mask(cand_mask)
             run1.append(acc)
             acc = None
         if m in cpu_at:
             snaps.append((m, ctx.download_state()))
         if m in starts:  # a timed block starts here: every kernel timed (resets the counters)
+            absorb(proto_kt)
             ctx.set_profiling(True)
+            ctx.set_profile_mask(all_mask)
             wcb = ctx.work_counters()
             acc = {}
+    absorb(proto_kt)
+    ctx.set_profiling(False)
     ctx.close()
     ranked = sorted(((ms, nm_) for nm_, ms in rank_ms.items() if nm_ in BYTES), reverse=True)
     dom = ranked[0][1] if ranked else "broad_narrow"
@@ -496,6 +515,9 @@ def run_c2(a, L, scenes):
         out["roofline"] = roofline_from(dom, tuple(rank_kt.get(dom, (0, 0.0))), run1_wc, sc.n, a.workload)
         out["roofline"]["timed_in"] = ("run 1: HIP events around the dominant kernel's eager launches in the same "
                                        "timed blocks (a captured step cannot be split into kernels by events)")
+        pk = proto_kt.get(dom, [0, 0.0])
+        out["roofline"]["protocol_launches"] = pk[0]
+        out["roofline"]["protocol_avg_launch_ms"] = pk[1] / max(pk[0], 1)
     out["kernel_ms_in_timed_blocks_run1"] = share
     if a.ab_steps > 0:
         out["learned_warm_start"] = learned_ab(ctx, sc, a, total)

@@ -3005,16 +3005,20 @@ int nsc_fixed_reserve(rdk_ctx* ctx, const rdk_nsc_params* p) {
   return RDK_OK;
 }
 
-// Workgroups of the fixed step's class launches: enough for four times the
-// most islands of that class seen in a step (at least 16, 64, 16 and 4
-// workgroups), rounded up to a power of two, so that a growing packing needs
-// few recaptures.  A step with more islands in a class fails its class check
-// and is redone eagerly, which raises the count.  Workgroups past the count
-// return at once.
+// Workgroups of the fixed step's class launches: none for a class that has
+// held no island yet, else enough for four times the most islands of that
+// class seen in a step (at least 16, 64, 16 and 4 workgroups), rounded up to a
+// power of two, so that a growing packing needs few recaptures.  A step with
+// more islands in a class fails its class check and is redone eagerly, which
+// raises the count.  Workgroups past the count return at once.
 void fixed_class_grids(const rdk_ctx* ctx, int* g) {
   const int ipb[4] = {ISL_WG0 / ISL_G0, 1, 1, 1};
   const int gmin[4] = {16, 64, 16, 4};
   for (int c = 0; c < 4; ++c) {
+    if (ctx->cls_seen_max[c] <= 0) {
+      g[c] = 0;
+      continue;
+    }
     const int64_t want = std::max<int64_t>(gmin[c], grid_for(4 * (int64_t)ctx->cls_seen_max[c], ipb[c]));
     int p2 = 1;
     while (p2 < want && p2 < (1 << 24)) p2 <<= 1;
@@ -3163,10 +3167,9 @@ int nsc_fixed_enqueue(rdk_ctx* ctx, const rdk_nsc_params* p) {
   P.warm = p->warm_mode != 0 ? 1 : 0;
   P.stall = p->stall_sweeps;
   P.hot = p->hot_set;
-  // The four classes, concurrently on four streams as in the eager solve when
-  // at least two of them have held islands in a step so far; classes never
-  // seen run on the main stream with one workgroup (they loop over whatever
-  // count a step brings).
+  // The classes that have held islands, concurrently on four streams as in
+  // the eager solve when there are at least two of them; a class never seen is
+  // not launched (an island of it fails the class check above).
   const int* cnt = ctx->icnt.p;
   const bool active[3] = {ctx->cls_seen_max[0] > 0, ctx->cls_seen_max[1] > 0, ctx->cls_seen_max[2] > 0};
   const int nactive = active[0] + active[1] + active[2] + (ctx->cls_seen_max[3] > 0);
@@ -3179,17 +3182,22 @@ int nsc_fixed_enqueue(rdk_ctx* ctx, const rdk_nsc_params* p) {
     for (int c = 0; c < 3; ++c)
       if (cs[c] != st) RDK_HIP(hipStreamWaitEvent(cs[c], ctx->ev_fork, 0));
   }
-  hipLaunchKernelGGL((k_island_reg<ISL_G0, ISL_WG0>), dim3(g[0]), dim3(ISL_WG0), 0, cs[0], B, Mc, ctx->sys_dev.p,
-                     ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p, Count::dev(cnt + 0, mcap), P, S, guard);
-  hipLaunchKernelGGL((k_island_reg<ISL_G1, ISL_WG1>), dim3(g[1]), dim3(ISL_WG1), 0, cs[1], B, Mc, ctx->sys_dev.p,
-                     ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p + mcap, Count::dev(cnt + 1, mcap), P, S,
-                     guard);
-  hipLaunchKernelGGL((k_island_reg<ISL_G2, ISL_WG2>), dim3(g[2]), dim3(ISL_WG2), 0, cs[2], B, Mc, ctx->sys_dev.p,
-                     ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p + 2 * (size_t)mcap,
-                     Count::dev(cnt + 2, mcap), P, S, guard);
-  hipLaunchKernelGGL(k_island_lds, dim3(g[3]), dim3(LDS_THREADS), 0, st, B, Mc, ctx->sys_dev.p, ctx->istart.p,
-                     skey, ctx->bseg.p, sbody, ctx->ilists.p + 3 * (size_t)mcap, Count::dev(cnt + 3, mcap), P, S,
-                     ctx->islot.p, ctx->ihot.p, guard);
+  if (g[0] > 0)
+    hipLaunchKernelGGL((k_island_reg<ISL_G0, ISL_WG0>), dim3(g[0]), dim3(ISL_WG0), 0, cs[0], B, Mc, ctx->sys_dev.p,
+                       ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p, Count::dev(cnt + 0, mcap), P, S,
+                       guard);
+  if (g[1] > 0)
+    hipLaunchKernelGGL((k_island_reg<ISL_G1, ISL_WG1>), dim3(g[1]), dim3(ISL_WG1), 0, cs[1], B, Mc, ctx->sys_dev.p,
+                       ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p + mcap, Count::dev(cnt + 1, mcap), P,
+                       S, guard);
+  if (g[2] > 0)
+    hipLaunchKernelGGL((k_island_reg<ISL_G2, ISL_WG2>), dim3(g[2]), dim3(ISL_WG2), 0, cs[2], B, Mc, ctx->sys_dev.p,
+                       ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p + 2 * (size_t)mcap,
+                       Count::dev(cnt + 2, mcap), P, S, guard);
+  if (g[3] > 0)
+    hipLaunchKernelGGL(k_island_lds, dim3(g[3]), dim3(LDS_THREADS), 0, st, B, Mc, ctx->sys_dev.p, ctx->istart.p,
+                       skey, ctx->bseg.p, sbody, ctx->ilists.p + 3 * (size_t)mcap, Count::dev(cnt + 3, mcap), P, S,
+                       ctx->islot.p, ctx->ihot.p, guard);
   for (int c = 0; c < 3; ++c) {
     if (cs[c] == st) continue;
     RDK_HIP(hipEventRecord(joins[c], cs[c]));
