@@ -418,6 +418,25 @@ int rdk_populate_nonoverlap(rdk_ctx* ctx, int n, float length, float diameter,
                             float cell_size, uint32_t seed, int max_attempts,
                             int window, float* centres, float* dirs,
                             float* quats, int* placed, int64_t* attempts);
+/* The reference's randomInit (App::resetScene, main.cpp:2213-2294) bit for
+ * bit: one std::mt19937(seed) (seed 0: std::random_device) over the rods in
+ * order, fresh std::normal_distribution<float> objects per rod, FP32
+ * arithmetic.  Rods are in the reference's y-up frame; q = FP32 quaternions
+ * (w, x, y, z) rotating +Y onto the rod axis, as the reference holds them
+ * (rdk_populate_nonoverlap returns them).  mode RDK_RANDOM_INIT_GAUSSIAN:
+ * v ~ N(0, v_sigma), w ~ N(0, w_sigma) per component; THERMAL: sigma_v =
+ * sqrt(kbt_trans / mass), sigma_w = sqrt(kbt_rot / i_perp), w in the plane
+ * perpendicular to the axis (the reference's `dw(gen) * e1 + dw(gen) * e2`
+ * drawn in g++'s order, the e2 term first); UNIFORM: v ~ U(-v_sigma,
+ * v_sigma), w = w_speed times a uniform direction.  project != 0 removes the
+ * spin about the axis (gaussian, uniform).  skip[i] != 0 (may be NULL): rod i
+ * draws nothing and keeps its v, w.  v, w: n x 3 floats. */
+#define RDK_RANDOM_INIT_GAUSSIAN 0
+#define RDK_RANDOM_INIT_THERMAL 1
+#define RDK_RANDOM_INIT_UNIFORM 2
+int rdk_random_init(int n, const float* q, const float* mass, const float* i_perp, int mode, float v_sigma,
+                    float w_sigma, float w_speed, float kbt_trans, float kbt_rot, uint32_t seed, int project,
+                    const unsigned char* skip, float* v, float* w);
 /* Thermal velocities: v ~ N(0, kT/m), w ~ N(0, kT/I_perp) on the two axes
  * perpendicular to the rod, zero spin about the axis. */
 int rdk_gen_thermal(int n, const double* q, const double* r, const double* h,

@@ -102,6 +102,10 @@ def lib():
         L.orc_rsa_seg_d2.argtypes = [_fp] * 4 + [C.c_int]
         L.orc_mt_raw.restype = C.c_uint
         L.orc_mt_raw.argtypes = [C.c_uint, C.c_longlong]
+        L.orc_random_init.restype = None
+        L.orc_random_init.argtypes = [C.c_int, _fp, _fp, _fp, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
+                                      C.c_float, C.c_uint, C.c_int, C.c_void_p, _fp, _fp]
+        L.orc_mt_normal_float.argtypes = [C.c_uint, C.c_int, _fp]
         _lib = L
     return _lib
 
@@ -369,3 +373,34 @@ def world_from_scene(sc):
     p = sc.phys
     W.set_physics(tuple(p.gravity[:]), p.dt, p.lin_damp, p.ang_damp, p.w_max)
     return W
+
+
+def random_init(q, mode, v_sigma=0.0, w_sigma=0.0, w_speed=0.0, kbt_trans=1.0, kbt_rot=1.0, seed=1, project=True,
+                mass=None, i_perp=None, skip=None):
+    """The reference's randomInit stream (main.cpp:2213-2294) for FP32
+    quaternions q (n x 4, w x y z); mode 0 gaussian, 1 thermal, 2 uniform.
+    Returns (v, w) as float32 n x 3 (rods with skip set keep zeros)."""
+    import numpy as np
+
+    q = np.ascontiguousarray(q, np.float32)
+    n = len(q)
+    m = np.ascontiguousarray(mass if mass is not None else np.ones(n), np.float32)
+    ip = np.ascontiguousarray(i_perp if i_perp is not None else np.ones(n), np.float32)
+    sk = None if skip is None else np.ascontiguousarray(skip, np.uint8)
+    v = np.zeros((n, 3), np.float32)
+    w = np.zeros((n, 3), np.float32)
+    fp = C.POINTER(C.c_float)
+    lib().orc_random_init(n, q.ctypes.data_as(fp), m.ctypes.data_as(fp), ip.ctypes.data_as(fp), int(mode),
+                          float(v_sigma), float(w_sigma), float(w_speed), float(kbt_trans), float(kbt_rot), int(seed),
+                          int(bool(project)), None if sk is None else sk.ctypes.data, v.ctypes.data_as(fp),
+                          w.ctypes.data_as(fp))
+    return v, w
+
+
+def mt_normal_float(seed, k):
+    """The first k outputs of std::normal_distribution<float>(0, 1) on std::mt19937(seed)."""
+    import numpy as np
+
+    out = np.zeros(k, np.float32)
+    lib().orc_mt_normal_float(int(seed), int(k), out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out

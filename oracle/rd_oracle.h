@@ -154,6 +154,11 @@ int orc_populate_nonoverlap(int n, float length, float diameter, const float* bm
 float orc_rsa_seg_d2(const float* p0, const float* p1, const float* q0, const float* q1, int pbc_rule);
 void orc_mt_uniform_float(unsigned seed, int k, float* out);
 unsigned orc_mt_raw(unsigned seed, long long k);
+/* reference randomInit (main.cpp:2213-2294): mode 0 gaussian, 1 thermal, 2 uniform */
+void orc_random_init(int n, const float* q, const float* mass, const float* i_perp, int mode, float v_sigma,
+                     float w_sigma, float w_speed, float kbt_trans, float kbt_rot, unsigned seed, int project,
+                     const unsigned char* skip, float* v, float* w);
+void orc_mt_normal_float(unsigned seed, int k, float* out);
 
 #define ORC_MF_FIELDS 42
 

@@ -194,3 +194,106 @@ unsigned orc_mt_raw(unsigned seed, long long k) {
 }
 
 }  // extern "C"
+
+// ---- randomInit (App::resetScene, main.cpp:2213-2294) --------------------------
+// Restated in the reference's own expression forms on a minimal FP32 vector
+// type whose operators are free functions as glm's are, so that g++ (the
+// reference's compiler on Linux) decides the evaluation order of
+// `dw(gen) * e1 + dw(gen) * e2` itself.  glm's quaternion rotation:
+// v + ((qv x v) q.w + qv x (qv x v)) 2; normalize(v) = v * (1 / sqrt(dot(v, v))).
+namespace orc_glm {
+struct vec3 {
+  float x, y, z;
+  vec3(float a, float b, float c) : x(a), y(b), z(c) {}
+};
+inline vec3 operator*(float s, const vec3& v) { return vec3(s * v.x, s * v.y, s * v.z); }
+inline vec3 operator*(const vec3& v, float s) { return vec3(v.x * s, v.y * s, v.z * s); }
+inline vec3 operator+(const vec3& a, const vec3& b) { return vec3(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline vec3& operator-=(vec3& a, const vec3& b) {
+  a = vec3(a.x - b.x, a.y - b.y, a.z - b.z);
+  return a;
+}
+inline float dot(const vec3& a, const vec3& b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+inline vec3 cross(const vec3& a, const vec3& b) {
+  return vec3(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y);
+}
+inline vec3 normalize(const vec3& v) { return v * (1.0f / std::sqrt(dot(v, v))); }
+struct quat {
+  float w, x, y, z;
+};
+inline vec3 operator*(const quat& q, const vec3& v) {
+  const vec3 qv(q.x, q.y, q.z);
+  const vec3 uv = cross(qv, v);
+  const vec3 uuv = cross(qv, uv);
+  return v + ((uv * q.w) + uuv) * 2.0f;
+}
+}  // namespace orc_glm
+
+extern "C" {
+
+// mode 0 gaussian, 1 thermal, 2 uniform; q: (w, x, y, z) per rod; skip may be NULL.
+void orc_random_init(int n, const float* q, const float* mass, const float* i_perp, int mode, float v_sigma,
+                     float w_sigma, float w_speed, float kbt_trans, float kbt_rot, unsigned seed, int project,
+                     const unsigned char* skip, float* v, float* w) {
+  using namespace orc_glm;
+  std::mt19937 gen(seed);
+  std::uniform_real_distribution<float> uni(0.0f, 1.0f);
+  auto uniform_dir_s2 = [&](std::mt19937& g) {
+    float u = 2.0f * uni(g) - 1.0f;
+    float phi = 2.0f * float(M_PI) * uni(g);
+    float s = std::sqrt(std::max(0.0f, 1.0f - u * u));
+    return vec3(s * std::cos(phi), u, s * std::sin(phi));
+  };
+  auto arbitrary_perp = [](const vec3& u) -> vec3 {
+    vec3 ref = (std::abs(u.x) < 0.9f) ? vec3(1, 0, 0) : vec3(0, 1, 0);
+    return normalize(cross(u, ref));
+  };
+  for (int i = 0; i < n; ++i) {
+    if (skip && skip[i]) continue;
+    const quat qq{q[4 * i], q[4 * i + 1], q[4 * i + 2], q[4 * i + 3]};
+    vec3 rv(0, 0, 0), rw(0, 0, 0);
+    if (mode == 1) {
+      const float sigma_v = (mass[i] > 0.0f) ? std::sqrt(kbt_trans / mass[i]) : 0.0f;
+      const float sigma_w = (i_perp[i] > 0.0f) ? std::sqrt(kbt_rot / i_perp[i]) : 0.0f;
+      std::normal_distribution<float> dv(0.0f, sigma_v);
+      std::normal_distribution<float> dw(0.0f, sigma_w);
+      rv = {dv(gen), dv(gen), dv(gen)};
+      vec3 u_ax = qq * vec3(0, 1, 0);
+      vec3 e1 = arbitrary_perp(u_ax);
+      vec3 e2 = cross(u_ax, e1);
+      rw = dw(gen) * e1 + dw(gen) * e2;
+    } else if (mode == 0) {
+      std::normal_distribution<float> dv(0.0f, v_sigma);
+      std::normal_distribution<float> dw(0.0f, w_sigma);
+      rv = {dv(gen), dv(gen), dv(gen)};
+      rw = {dw(gen), dw(gen), dw(gen)};
+      if (project) {
+        vec3 u_ax = qq * vec3(0, 1, 0);
+        rw -= dot(rw, u_ax) * u_ax;
+      }
+    } else {
+      std::uniform_real_distribution<float> uniform(-v_sigma, v_sigma);
+      rv = {uniform(gen), uniform(gen), uniform(gen)};
+      rw = w_speed * uniform_dir_s2(gen);
+      if (project) {
+        vec3 u_ax = qq * vec3(0, 1, 0);
+        rw -= dot(rw, u_ax) * u_ax;
+      }
+    }
+    v[3 * i] = rv.x;
+    v[3 * i + 1] = rv.y;
+    v[3 * i + 2] = rv.z;
+    w[3 * i] = rw.x;
+    w[3 * i + 1] = rw.y;
+    w[3 * i + 2] = rw.z;
+  }
+}
+
+// The first k outputs of std::normal_distribution<float>(0, 1) on std::mt19937(seed), one distribution object.
+void orc_mt_normal_float(unsigned seed, int k, float* out) {
+  std::mt19937 gen(seed);
+  std::normal_distribution<float> d(0.0f, 1.0f);
+  for (int i = 0; i < k; ++i) out[i] = d(gen);
+}
+
+}  // extern "C"

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <random>
 #include <vector>
 
 #include "../../include/rdk.h"
@@ -244,6 +245,95 @@ extern "C" int rdk_gen_thermal(int n, const double* q, const double* r, const do
     w[3 * i] = om.x;
     w[3 * i + 1] = om.y;
     w[3 * i + 2] = om.z;
+  }
+  return RDK_OK;
+}
+
+// ---- the reference's randomInit stream (App::resetScene, main.cpp:2213-2294) ----
+// One std::mt19937(seed) for all rods in order; a rod that is skipped draws
+// nothing.  Per rod, FP32 throughout as the reference holds it:
+//   gaussian: fresh normal_distribution<float>(0, vSigma) and (0, wSigma);
+//             v = three draws, w = three draws (braced lists: left to right);
+//   thermal:  sigma_v = sqrt(kBT_trans / m), sigma_w = sqrt(kBT_rot / I_perp)
+//             (0 when m or I_perp is not positive); v = three draws;
+//             w = b1 e1 + b2 e2 with e1 a unit vector perpendicular to the
+//             axis u and e2 = u x e1; the reference's expression
+//             `dw(gen) * e1 + dw(gen) * e2` is evaluated right operand first
+//             by g++, so b2 is drawn before b1;
+//   uniform:  uniform_real_distribution<float>(-vSigma, vSigma) for v, and
+//             w = wSpeed * (s cos(phi), c, s sin(phi)) with c = 2 U - 1 and
+//             phi = 2 pi U drawn in that order from U(0, 1);
+// then, when project is set (gaussian, uniform), w -= (w . u) u.  The axis u
+// is the rod's quaternion (w, x, y, z) applied to +Y with glm's quaternion
+// rotation, v + 2 (q.w (qv x v) + qv x (qv x v)).
+namespace {
+struct F3 {
+  float x, y, z;
+};
+F3 f3(float x, float y, float z) { return F3{x, y, z}; }
+F3 fcross(const F3& a, const F3& b) {
+  return f3(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y);
+}
+float fdot(const F3& a, const F3& b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+F3 fscale(const F3& a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+F3 fadd(const F3& a, const F3& b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+F3 fsub(const F3& a, const F3& b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+F3 fnormalize(const F3& a) { return fscale(a, 1.0f / std::sqrt(fdot(a, a))); }
+F3 frot_y(const float* q) {  // quaternion (w, x, y, z) applied to (0, 1, 0)
+  const F3 qv = f3(q[1], q[2], q[3]), v = f3(0.0f, 1.0f, 0.0f);
+  const F3 uv = fcross(qv, v), uuv = fcross(qv, uv);
+  return fadd(v, fscale(fadd(fscale(uv, q[0]), uuv), 2.0f));
+}
+}  // namespace
+
+extern "C" int rdk_random_init(int n, const float* q, const float* mass, const float* i_perp, int mode, float v_sigma,
+                               float w_sigma, float w_speed, float kbt_trans, float kbt_rot, uint32_t seed,
+                               int project, const unsigned char* skip, float* v, float* w) {
+  if (n < 0 || !q || !v || !w || mode < 0 || mode > 2) return RDK_ERR_ARG;
+  if (mode == RDK_RANDOM_INIT_THERMAL && (!mass || !i_perp)) return RDK_ERR_ARG;
+  std::mt19937 gen(seed ? seed : std::random_device{}());
+  std::uniform_real_distribution<float> uni(0.0f, 1.0f);
+  for (int i = 0; i < n; ++i) {
+    if (skip && skip[i]) continue;
+    const float* qi = q + 4 * i;
+    F3 vv, ww;
+    if (mode == RDK_RANDOM_INIT_THERMAL) {
+      const float sv = mass[i] > 0.0f ? std::sqrt(kbt_trans / mass[i]) : 0.0f;
+      const float sw = i_perp[i] > 0.0f ? std::sqrt(kbt_rot / i_perp[i]) : 0.0f;
+      std::normal_distribution<float> dv(0.0f, sv), dw(0.0f, sw);
+      const float v0 = dv(gen), v1 = dv(gen), v2 = dv(gen);
+      vv = f3(v0, v1, v2);
+      const F3 u = frot_y(qi);
+      const F3 e1 = fnormalize(fcross(u, std::fabs(u.x) < 0.9f ? f3(1, 0, 0) : f3(0, 1, 0)));
+      const F3 e2 = fcross(u, e1);
+      const float b2 = dw(gen);  // the right operand of the reference's sum, first under g++
+      const float b1 = dw(gen);
+      ww = fadd(fscale(e1, b1), fscale(e2, b2));
+    } else if (mode == RDK_RANDOM_INIT_GAUSSIAN) {
+      std::normal_distribution<float> dv(0.0f, v_sigma), dw(0.0f, w_sigma);
+      const float v0 = dv(gen), v1 = dv(gen), v2 = dv(gen);
+      const float w0 = dw(gen), w1 = dw(gen), w2 = dw(gen);
+      vv = f3(v0, v1, v2);
+      ww = f3(w0, w1, w2);
+    } else {
+      std::uniform_real_distribution<float> un(-v_sigma, v_sigma);
+      const float v0 = un(gen), v1 = un(gen), v2 = un(gen);
+      vv = f3(v0, v1, v2);
+      const float c = 2.0f * uni(gen) - 1.0f;
+      const float phi = 2.0f * float(M_PI) * uni(gen);
+      const float s = std::sqrt(std::max(0.0f, 1.0f - c * c));
+      ww = fscale(f3(s * std::cos(phi), c, s * std::sin(phi)), w_speed);
+    }
+    if (project && mode != RDK_RANDOM_INIT_THERMAL) {
+      const F3 u = frot_y(qi);
+      ww = fsub(ww, fscale(u, fdot(ww, u)));
+    }
+    v[3 * i] = vv.x;
+    v[3 * i + 1] = vv.y;
+    v[3 * i + 2] = vv.z;
+    w[3 * i] = ww.x;
+    w[3 * i + 1] = ww.y;
+    w[3 * i + 2] = ww.z;
   }
   return RDK_OK;
 }

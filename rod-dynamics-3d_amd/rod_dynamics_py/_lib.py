@@ -141,6 +141,9 @@ _SIGS = {
     "rdk_step_graph_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                       C.POINTER(C.c_int64)]),
     "rdk_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "rdk_random_init": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.c_float,
+                                  C.c_float, C.c_float, C.c_float, C.c_uint32, C.c_int, C.c_void_p, C.c_void_p,
+                                  C.c_void_p]),
     "rdk_mlp_num_params": (C.c_int, []),
     "rdk_mlp_init": (C.c_int, [C.c_void_p, C.c_uint64]),
     "rdk_mlp_set_params": (C.c_int, [C.c_void_p, _fp, C.c_int]),
@@ -676,3 +679,43 @@ def gen_thermal(q, r, h, mass, kT, seed):
     if st != RDK_OK:
         raise RdkError(st, "rdk_gen_thermal failed")
     return v, w
+
+
+RANDOM_INIT_MODES = {"gaussian": 0, "thermal": 1, "uniform": 2}  # RDK_RANDOM_INIT_*
+
+
+def random_init(q32, mode, v_sigma=0.0, w_sigma=0.0, w_speed=0.0, kbt_trans=1.0, kbt_rot=1.0, seed=0, project=True,
+                mass32=None, i_perp32=None, skip=None):
+    """The reference's randomInit stream (rdk_random_init): q32 = FP32
+    quaternions (w, x, y, z) as the reference's rods hold them; mode 0
+    gaussian, 1 thermal, 2 uniform.  Returns (v, w) float32 n x 3; rods with
+    skip set keep zeros."""
+    q32 = np.ascontiguousarray(q32, np.float32)
+    n = len(q32)
+    m = np.ascontiguousarray(mass32 if mass32 is not None else np.ones(n), np.float32)
+    ip = np.ascontiguousarray(i_perp32 if i_perp32 is not None else np.ones(n), np.float32)
+    sk = None if skip is None else np.ascontiguousarray(skip, np.uint8)
+    v = np.zeros((n, 3), np.float32)
+    w = np.zeros((n, 3), np.float32)
+    st = lib().rdk_random_init(n, q32.ctypes.data, m.ctypes.data, ip.ctypes.data, int(mode), float(v_sigma),
+                               float(w_sigma), float(w_speed), float(kbt_trans), float(kbt_rot), int(seed) & 0xFFFFFFFF,
+                               int(bool(project)), None if sk is None else sk.ctypes.data, v.ctypes.data,
+                               w.ctypes.data)
+    if st != RDK_OK:
+        raise RdkError(st, "rdk_random_init failed")
+    return v, w
+
+
+def glm_normalize_quat_f32(q):
+    """glm::normalize of FP32 quaternions (w, x, y, z), as RigidBody::makeCapsule
+    applies it (rigid_body.cpp:17): length = sqrt((ww + xx) + (yy + zz)), then
+    every component times 1 / length (identity for length <= 0)."""
+    q = np.ascontiguousarray(q, np.float32)
+    sq = q * q
+    ln = np.sqrt((sq[:, 0] + sq[:, 1]) + (sq[:, 2] + sq[:, 3]))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        inv = np.float32(1.0) / ln
+        out = q * inv[:, None]
+    bad = ~(ln > 0)
+    out[bad] = np.array([1, 0, 0, 0], np.float32)
+    return out.astype(np.float32)

@@ -33,10 +33,17 @@ populate.mode "nonoverlap" places the reference's rods bit for bit (its FP32
 serial placement with its std::mt19937 stream, evaluated in parallel on the
 GPU by ``rdk_populate_nonoverlap``).
 
-What differs from the reference, by design: the other random streams
-(populate "grid"/"uniform", randomInit) come from numpy, not libstdc++'s
-mt19937 + distributions, so those scenes are statistically but not bitwise
-the reference's; ``last_island_count`` reports the number of contact
+randomInit draws the reference's own stream (std::mt19937 with fresh
+std::normal_distribution<float> objects per rod, FP32, main.cpp:2213-2294,
+``rdk_random_init``) from the FP32 quaternions the reference's rods hold
+(glm::normalize of the populate quaternion), so a scene populated with
+"nonoverlap" starts from the reference's state bit for bit (given a reference
+build without FMA contraction).
+
+What differs from the reference, by design: populate "grid"/"uniform" come
+from numpy, not libstdc++'s mt19937 + distributions, so those scenes are
+statistically but not bitwise the reference's (and randomInit then sees their
+orientations rounded to FP32); ``last_island_count`` reports the number of contact
 islands of the last hard-contact solve (the reference never updates that
 field); the soft/Hertz-Mindlin paths use the library's capsule cell list
 instead of the reference's sphere grid.  Options the library does not model
@@ -362,6 +369,7 @@ class Simulator:
                 print(f"[populate] nonoverlap: failed to place rod {len(c)}/{n} after attempts= {att}",
                       file=sys.stderr)
             self.x = c.astype(np.float64)
+            self.q32 = L.glm_normalize_quat_f32(q)  # what RigidBody::makeCapsule stores
             q = q.astype(np.float64)
             self.q = q / np.linalg.norm(q, axis=1, keepdims=True)
             rr, hh = rr[:len(c)], hh[:len(c)]
@@ -403,7 +411,8 @@ class Simulator:
 
     def _materials(self, density, restitution, friction):
         n = len(self.r)
-        self.mass = np.broadcast_to(np.asarray(density, float), (n,)) * math.pi * self.r ** 2 * (2.0 * self.h)
+        self.density = np.broadcast_to(np.asarray(density, float), (n,)).copy()
+        self.mass = self.density * math.pi * self.r ** 2 * (2.0 * self.h)
         self.rest = np.broadcast_to(np.asarray(restitution, float), (n,)).copy()
         self.fric = np.broadcast_to(np.asarray(friction, float), (n,)).copy()
         self.v = np.zeros((n, 3))
@@ -411,27 +420,32 @@ class Simulator:
         self.static = np.zeros(n, bool)
 
     def _random_init(self):
+        """The reference's randomInit (App::resetScene, main.cpp:2213-2294):
+        one std::mt19937(seed) over the rods in order, FP32, from the rods'
+        FP32 quaternions and the FP32 mass and transverse inertia of
+        RigidBody::makeCapsule (rigid_body.cpp:11-34)."""
         ri = self.cfg["randomInit"]
         if not ri["enabled"]:
             return
         n = len(self.r)
-        rng = np.random.default_rng(ri["seed"] or None)
-        u = _axis(self.q)
-        if ri["mode"] == "gaussian":
-            self.v = rng.normal(0.0, ri["vSigma"], (n, 3))
-            self.w = rng.normal(0.0, ri["wSigma"], (n, 3))
-        elif ri["mode"] == "uniform":
-            self.v = rng.uniform(-ri["vSigma"], ri["vSigma"], (n, 3))
-            d = rng.normal(size=(n, 3))
-            self.w = ri["wSpeed"] * d / np.linalg.norm(d, axis=1, keepdims=True)
-        else:  # thermal equipartition
-            kt = ri["kBTTrans"] if ri["kBTTrans"] > 0 else ri["kBT"]
-            kr = ri["kBTRot"] if ri["kBTRot"] > 0 else ri["kBT"]
-            i_perp = self.mass * (3.0 * self.r ** 2 + (2.0 * self.h) ** 2) / 12.0
-            self.v = rng.normal(size=(n, 3)) * np.sqrt(kt / self.mass)[:, None]
-            self.w = rng.normal(size=(n, 3)) * np.sqrt(kr / i_perp)[:, None]
-        if ri["projectParallelSpin"]:
-            self.w -= np.sum(self.w * u, axis=1, keepdims=True) * u
+        q32 = getattr(self, "q32", None)
+        if q32 is None or len(q32) != n:
+            q32 = L.glm_normalize_quat_f32(self.q.astype(np.float32))
+        f = np.float32
+        r32, h32, d32 = self.r.astype(f), self.h.astype(f), self.density.astype(f)
+        H = f(2.0) * h32
+        vol = ((f(math.pi) * r32) * r32) * H
+        m32 = np.maximum(f(1e-6), d32 * vol)
+        ip32 = (m32 * ((f(3.0) * r32) * r32 + H * H)) / f(12.0)
+        mode = L.RANDOM_INIT_MODES.get(ri["mode"], L.RANDOM_INIT_MODES["uniform"])
+        kbt = f(ri["kBT"])
+        kt = f(ri["kBTTrans"]) if ri["kBTTrans"] >= 0.0 else kbt
+        kr = f(ri["kBTRot"]) if ri["kBTRot"] >= 0.0 else kbt
+        seed = int(ri["seed"]) & 0xFFFFFFFF
+        v, w = L.random_init(q32, mode, ri["vSigma"], ri["wSigma"], ri["wSpeed"], kt, kr, seed,
+                             ri["projectParallelSpin"], m32, ip32)
+        self.v = v.astype(np.float64)
+        self.w = w.astype(np.float64)
         self.v[self.static] = 0.0
         self.w[self.static] = 0.0
 

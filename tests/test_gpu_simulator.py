@@ -242,3 +242,56 @@ def test_profile_log_of_a_hard_contact_step(tmp_path):
     cols = dict(zip(head.split(","), row.split(",")))
     assert len(row.split(",")) == 21 and cols["collisions"] == "-1" and float(cols["soft_PE"]) > 0.0
     sim.close()
+
+
+def test_reference_paper_scene_runs_unchanged():
+    """The reference's Table S1 scene (assets/scenes/default_entangled_nsc.json,
+    committed as tests/golden/default_entangled_nsc.json) through the session
+    API unchanged: 2000 rods of populate "nonoverlap" (seed 12345) equal to
+    the oracle's serial FP32 placement, velocities of randomInit "gaussian"
+    (seed 42, parallel spin projected) equal to the oracle's g++ restatement
+    of main.cpp:2213-2294 bit for bit, then 30 hard-contact steps with the
+    scene's solver settings (mu 0, cfm 0.05, 200 PGS iterations, omega 1)
+    against the oracle world started from the same FP64 state."""
+    path = os.path.join(os.path.dirname(__file__), "golden", "default_entangled_nsc.json")
+    scene = json.load(open(path))
+    sim = Simulator(path)
+    n = scene["scene"]["populate"]["count"]
+    assert sim.diagnostics()["rod_count"] == n
+    pb = scene["scene"]["periodic"]
+    body = scene["scene"]["bodies"][0]
+    k, c, u, _ = O.populate_nonoverlap(n, body["length"], body["diameter"], pb["min"], pb["max"], 0, pb["cellSize"],
+                                       scene["scene"]["populate"]["seed"], scene["scene"]["populate"]["maxAttempts"])
+    assert k == n
+    r = sim.rods()
+    x = np.array([q["position"] for q in r])
+    assert np.array_equal(x, c.astype(np.float64))
+    ax = np.array([np.subtract(q["endpoint_b"], q["endpoint_a"]) for q in r])
+    ax /= np.linalg.norm(ax, axis=1, keepdims=True)
+    assert np.abs(ax - u.astype(np.float64)).max() < 1e-6
+    ri = scene["scene"]["randomInit"]
+    vo, wo = O.random_init(sim.q32, 0, v_sigma=ri["vSigma"], w_sigma=ri["wSigma"], seed=ri["seed"],
+                           project=ri["projectParallelSpin"])
+    v = np.array([q["linear_velocity"] for q in r])
+    w = np.array([q["angular_velocity"] for q in r])
+    assert np.array_equal(v, vo.astype(np.float64)) and np.array_equal(w, wo.astype(np.float64))
+    assert np.any(v != 0.0) and np.abs(np.sum(w * ax, axis=1)).max() < 1e-6
+    st0 = sim.ctx.download_state()  # library frame
+    p = sim.params
+    m = 4.0 * float(np.max(sim.h + sim.r))
+    W = O.World(st0["x"], st0["q"], st0["v"], st0["w"], sim.r, sim.h, sim.mass, sim.rest)
+    W.set_box(st0["x"].min(0) - m, st0["x"].max(0) + m, [0, 0, 0])
+    W.set_physics((0.0, 0.0, 0.0), sim.dt)
+    op = O.nsc_params(mu=0.0, beta=p.beta, cfm=p.cfm, omega=p.omega, tol=0.0, max_sweeps=p.max_sweeps, check_every=1,
+                      friction_proj=0, stop_rule=0, position_stabilization=p.position_stabilization,
+                      warm_start=p.warm_mode, position_iters=p.position_iters,
+                      position_psor_iters=p.position_psor_iters, slop=p.slop)
+    print(f"paper scene: {n} rods, cfm {p.cfm}, {p.max_sweeps} sweeps max", flush=True)
+    sim.step(30)
+    W.nsc_step(op, 30, method=1)
+    sg, so = sim.ctx.download_state(), W.state()
+    assert np.abs(sg["x"] - so["x"]).max() < 1e-9
+    assert np.abs(sg["v"] - so["v"]).max() < 1e-9
+    assert np.abs(sg["w"] - so["w"]).max() < 1e-9
+    assert sim.diagnostics()["frame_index"] == 30
+    sim.close()
