@@ -150,11 +150,12 @@ def cores():
 def autotune(ctx, run_steps, cands, reps=3, blocks=(64, 128, 256, 512)):
     """Time `reps` steps (after one untimed step) for each chunk length in
     `cands`, then, at the best one, for each broad-phase workgroup size in
-    `blocks`; the contact set depends on neither, so these are ordinary steps
-    of the run.  Leaves the best workgroup size set.  Returns (best chunk
+    `blocks`, then with the stencil rows staged in LDS at the best size; the
+    contact set depends on none of them, so these are ordinary steps of the
+    run.  Leaves the best workgroup size and staging set.  Returns (best chunk
     length, {"chunk_len_ms": ms per step by chunk length, "broad_block": best
-    workgroup size, "broad_block_ms": ms per step by workgroup size}, steps
-    taken)."""
+    workgroup size, "broad_block_ms": ms per step by workgroup size,
+    "broad_stage": 0/1, "broad_stage_ms": ms per step staged}, steps taken)."""
     from rod_dynamics_py import _lib as L
 
     def timed(cl):
@@ -173,10 +174,17 @@ def autotune(ctx, run_steps, cands, reps=3, blocks=(64, 128, 256, 512)):
         btune[b] = timed(best)
     bbest = min(btune, key=btune.get)
     ctx.set_tuning(L.TUNE_BROAD_BLOCK, bbest)
+    # stencil rows staged in LDS at the best workgroup size
+    ctx.set_tuning(L.TUNE_BROAD_STAGE, 1)
+    staged_ms = timed(best)
+    stage = 1 if staged_ms < btune[bbest] else 0
+    ctx.set_tuning(L.TUNE_BROAD_STAGE, stage)
     log(f"autotune chunk_len {best} ({', '.join(f'{k}: {v:.2f} ms' for k, v in tune.items())}); broad-phase "
-        f"workgroup {bbest} ({', '.join(f'{k}: {v:.2f} ms' for k, v in btune.items())})")
+        f"workgroup {bbest} ({', '.join(f'{k}: {v:.2f} ms' for k, v in btune.items())}); LDS-staged rows "
+        f"{staged_ms:.2f} ms -> stage {stage}")
     return best, {"chunk_len_ms": {str(k): v for k, v in tune.items()}, "broad_block": bbest,
-                  "broad_block_ms": {str(k): v for k, v in btune.items()}}, (len(cands) + len(blocks)) * (reps + 1)
+                  "broad_block_ms": {str(k): v for k, v in btune.items()}, "broad_stage": stage,
+                  "broad_stage_ms": staged_ms}, (len(cands) + len(blocks) + 1) * (reps + 1)
 
 
 def rank_kernels(ctx, run_warmup, fixed=None):
@@ -418,6 +426,7 @@ def run_c2(a, L, scenes):
     sc.load(ctx)
     if tune:
         ctx.set_tuning(L.TUNE_BROAD_BLOCK, tune["broad_block"])
+        ctx.set_tuning(L.TUNE_BROAD_STAGE, tune["broad_stage"])
     if weights is not None:
         ctx.mlp_set_params(weights)
     if mode == 0:

@@ -349,9 +349,12 @@ int rdk_set_step_graph(rdk_ctx* ctx, int mode, int pair_capacity);
  * the global path solved) by one workgroup, larger ones by rounds over the
  * whole GPU; -1 restores the default (16384, or the RDK_SMALL_M environment
  * variable).  RDK_TUNE_BROAD_BLOCK: workgroup size of the broad phase, 64,
- * 128, 256 (default) or 512. */
+ * 128, 256 (default) or 512.  RDK_TUNE_BROAD_STAGE: 1 stages each
+ * workgroup's stencil rows in LDS before its candidate walk (three passes, one
+ * per z offset), 0 (default) walks them in global memory. */
 #define RDK_TUNE_SMALL_M 0
 #define RDK_TUNE_BROAD_BLOCK 1
+#define RDK_TUNE_BROAD_STAGE 2
 int rdk_set_tuning(rdk_ctx* ctx, int what, int value);
 /* steps run by the fixed path, steps redone by the eager step, graph captures */
 int rdk_step_graph_info(rdk_ctx* ctx, int64_t* fixed_steps, int64_t* fallbacks, int64_t* captures);

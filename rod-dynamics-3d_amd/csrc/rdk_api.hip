@@ -1137,6 +1137,12 @@ int rdk_set_tuning(rdk_ctx* ctx, int what, int value) {
     drop_step_graph(ctx);
     return RDK_OK;
   }
+  if (what == RDK_TUNE_BROAD_STAGE) {
+    if (value != 0 && value != 1) return RDK_ERR_ARG;
+    ctx->bn_stage = value;
+    drop_step_graph(ctx);
+    return RDK_OK;
+  }
   return RDK_ERR_ARG;
 }
 

@@ -185,27 +185,47 @@ __device__ __forceinline__ void narrow_pair(const BodyPtrs& B, const SysDev* __r
 struct StencilWalk {
   int p, cy, cz, xa0, xa1, xb;  // x ranges: [xa0, xa1] and, after a periodic wrap, the single cell xb
   int d0, d1, d2, off;  // the system's grid (kept in registers: on the path to the cstart loads)
+  int kp;               // cell key of chunk p
   bool two, py, pz;     // periodic in y, z
   int r, q, q1;
-  __device__ __forceinline__ bool open_next(const int* __restrict__ cstart) {
-    while (r < 18) {
-      const int row = r >> 1, part = r & 1;
+  // Cell keys [c0, c1] of range r, false when the range does not exist (a
+  // wrap part without a wrap, a row outside a closed box, a duplicate row of
+  // a one-cell periodic axis) or holds only sorted positions below p: its
+  // last cell key is below p's own cell (cstart is monotone and p >=
+  // cstart[kp]), so it is empty after the q > p clip; about half of the
+  // stencil is skipped this way without loading its bounds.
+  __device__ __forceinline__ bool interval(int rr, int& c0, int& c1) const {
+    const int row = rr >> 1, part = rr & 1;
+    if (part == 1 && !two) return false;
+    const int dz = row / 3 - 1, dy = row % 3 - 1;
+    int z = cz + dz;
+    if (pz) {
+      if (d2 == 1 && dz != 0) return false;
+      z += z < 0 ? d2 : (z >= d2 ? -d2 : 0);  // |dz| <= 1, d2 == 1 or >= 3
+    } else if (z < 0 || z >= d2) {
+      return false;
+    }
+    int y = cy + dy;
+    if (py) {
+      if (d1 == 1 && dy != 0) return false;
+      y += y < 0 ? d1 : (y >= d1 ? -d1 : 0);
+    } else if (y < 0 || y >= d1) {
+      return false;
+    }
+    const int base = off + d0 * (y + d1 * z);
+    c0 = base + (part ? xb : xa0);
+    c1 = base + (part ? xb : xa1);
+    return c1 >= kp;
+  }
+  // opens the next non-empty range below r_end (18: all of them)
+  __device__ __forceinline__ bool open_next(const int* __restrict__ cstart, int r_end = 18) {
+    while (r < r_end) {
+      int c0, c1;
+      const bool ok = interval(r, c0, c1);
       ++r;
-      if (part == 1 && !two) continue;
-      const int dz = row / 3 - 1, dy = row % 3 - 1;
-      int z = cz + dz;
-      if (pz) {
-        if (d2 == 1 && dz != 0) continue;
-        z += z < 0 ? d2 : (z >= d2 ? -d2 : 0);  // |dz| <= 1, d2 == 1 or >= 3
-      } else if (z < 0 || z >= d2) continue;
-      int y = cy + dy;
-      if (py) {
-        if (d1 == 1 && dy != 0) continue;
-        y += y < 0 ? d1 : (y >= d1 ? -d1 : 0);
-      } else if (y < 0 || y >= d1) continue;
-      const int base = off + d0 * (y + d1 * z);
-      q = cstart[base + (part ? xb : xa0)];
-      q1 = cstart[base + (part ? xb : xa1) + 1];
+      if (!ok) continue;
+      q = cstart[c0];
+      q1 = cstart[c1 + 1];
       if (q <= p) q = p + 1;
       if (q < q1) return true;
     }
@@ -306,6 +326,29 @@ __device__ __forceinline__ void sat_stage(const int2* __restrict__ src, int n, i
 
 constexpr int BN_UNROLL = 2;  // candidates loaded per lane per step (independent loads in flight)
 
+// LDS staging of the stencil rows (STAGED kernels).  A workgroup's chunks are
+// a contiguous run of sorted positions, i.e. consecutive cells of at most a
+// few cell rows (row = one (system, y, z)); BN_GROUPS of those rows (groups)
+// are staged.  The walk runs in three phases, one per z offset of the
+// stencil (ranges 6 dz .. 6 dz + 5); before each, the workgroup copies the
+// chunk records (FP32 centre/radius and rod) of the three rows (y - 1, y,
+// y + 1) at that z offset, over the group's cells x0 - 1 .. x1 + 1, into LDS,
+// and every lane whose range lies inside a staged entry reads its candidates
+// from there.  A range outside every entry (a periodic wrap in x, a group
+// beyond BN_GROUPS, an entry that did not fit in BN_CAP(NT) records) is read
+// from global memory as in the plain kernel, so the candidate set and the
+// contacts are the same either way.
+constexpr int BN_GROUPS = 4;
+template <int NT>
+constexpr int bn_cap() {
+  return 5 * NT;
+}
+// Each lane lists its non-empty-looking stencil ranges once, before the walk:
+// their bounds are loaded together (one round trip instead of one per range
+// opened during the walk) and the walk takes them from LDS.  A lane with more
+// than BN_SLOTS ranges (periodic wraps) opens the rest as it goes.
+constexpr int BN_SLOTS = 8;
+
 // Broad + narrow phase, one thread per sorted chunk p.  Every unordered chunk
 // pair (p, q) is examined once, by the thread with the lower sorted position.
 // The stencil walk is latency-bound (a lane's candidates are loaded from L2
@@ -322,33 +365,47 @@ constexpr int BN_UNROLL = 2;  // candidates loaded per lane per step (independen
 // NT = workgroup size (64 ... 512; rdk_set_tuning(RDK_TUNE_BROAD_BLOCK)): the
 // queues are per wavefront, so NT only changes how many wavefronts share a
 // workgroup slot and the granularity of the XCD-contiguous block order; the
-// contact set does not depend on it.
-template <int NT>
+// contact set does not depend on it, nor on STAGED
+// (rdk_set_tuning(RDK_TUNE_BROAD_STAGE)).
+template <int NT, bool STAGED>
 __global__ void __launch_bounds__(NT, 4) k_broad_narrow(BodyPtrs B, const SysDev* __restrict__ sys,
                                                         SortedChunks SC, const int* __restrict__ cstart,
                                                         const int* __restrict__ rod_nchunk, int M, double delta,
                                                         uint64_t* __restrict__ out, unsigned cap,
                                                         unsigned* __restrict__ counter,
                                                         unsigned long long* __restrict__ cand_counter) {
+  constexpr int CAP = STAGED ? bn_cap<NT>() : 1;
+  constexpr int NE = 3 * BN_GROUPS;  // staged entries per phase: (group, dy)
   __shared__ int2 queue[NT / 64][BN_QUEUE];
   __shared__ int2 queue2[NT / 64][BN_QUEUE];
+  __shared__ float4 st_cf[CAP];
+  __shared__ int st_rod[CAP];
+  __shared__ int st_lo[NE], st_hi[NE], st_off[NE];
+  __shared__ int g_x0[BN_GROUPS], g_x1[BN_GROUPS], g_cy[BN_GROUPS], g_cz[BN_GROUPS], g_rod[BN_GROUPS];
+  __shared__ int s_rowkey[STAGED ? NT : 1];
+  __shared__ int s_wcnt[NT / 64];
+  __shared__ int s_ng, s_tot;
+  __shared__ int rl_q[BN_SLOTS][NT], rl_q1[BN_SLOTS][NT];
+  __shared__ unsigned char rl_r[BN_SLOTS][NT];
   int2* Q = queue[threadIdx.x >> 6];
   int2* Q2 = queue2[threadIdx.x >> 6];
   const int lane = __lane_id();
+  const int tid = threadIdx.x;
   const int p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   unsigned long long ncand = 0, nnarrow = 0;
   StencilWalk W;
   W.p = p;
   W.r = 0;
-  bool active = p < M;
-  int a = 0;
+  W.q = W.q1 = 0;
+  const bool valid = p < M;
+  int a = 0, cx = 0;
   float4 me = make_float4(0.f, 0.f, 0.f, 0.f);
   bool a_static = false;
   LaneSys own;
   own.per = 0;
   own.extra = 0.f;
   for (int k = 0; k < 3; ++k) own.span[k] = own.ispan[k] = 0.f;
-  if (active) {
+  if (valid) {
     a = SC.rod[p];
     me = SC.cf[p];
     const SysDev& S = sys[B.sys[a]];
@@ -367,7 +424,8 @@ __global__ void __launch_bounds__(NT, 4) k_broad_narrow(BodyPtrs B, const SysDev
     W.py = S.periodic[1] != 0;
     W.pz = S.periodic[2] != 0;
     const int cell = (int)SC.key[p] - S.cell_off;
-    const int cx = cell % d0;
+    W.kp = (int)SC.key[p];
+    cx = cell % d0;
     W.cy = (cell / d0) % d1;
     W.cz = cell / (d0 * d1);
     W.two = false;
@@ -391,52 +449,218 @@ __global__ void __launch_bounds__(NT, 4) k_broad_narrow(BodyPtrs B, const SysDev
       W.xa0 = cx - 1;
       W.xa1 = cx + 1;
     }
-    active = W.open_next(cstart);
+  }
+  // ---- the lane's range list: cell intervals, then their bounds
+  int nl = 0;
+  W.r = 18;  // where open_next resumes after the list (18: nothing left)
+  if (valid) {
+    for (int rr = 0; rr < 18; ++rr) {
+      int c0, c1;
+      if (!W.interval(rr, c0, c1)) continue;
+      if (nl == BN_SLOTS) {
+        W.r = rr;
+        break;
+      }
+      rl_q[nl][tid] = c0;
+      rl_q1[nl][tid] = c1;
+      rl_r[nl][tid] = (unsigned char)rr;
+      ++nl;
+    }
+    int bq[BN_SLOTS], bq1[BN_SLOTS];
+#pragma unroll
+    for (int k = 0; k < BN_SLOTS; ++k) {
+      if (k < nl) {
+        bq[k] = cstart[rl_q[k][tid]];
+        bq1[k] = cstart[rl_q1[k][tid] + 1];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < BN_SLOTS; ++k) {
+      if (k < nl) {
+        rl_q[k][tid] = bq[k] <= p ? p + 1 : bq[k];
+        rl_q1[k][tid] = bq1[k];
+      }
+    }
+  }
+  // ---- row groups of the workgroup (STAGED): group = run of lanes with one
+  // row key (key - x, unique per system row); lanes are in sorted order
+  int grp = BN_GROUPS;
+  if constexpr (STAGED) {
+    const int rowkey = valid ? (int)SC.key[p] - cx : -1;
+    s_rowkey[tid] = rowkey;
+    __syncthreads();
+    const bool start = valid && (tid == 0 || s_rowkey[tid - 1] != rowkey);
+    const bool last = valid && (tid == NT - 1 || s_rowkey[tid + 1] != rowkey);
+    const unsigned long long bal = __ballot(start);
+    if (lane == 0) s_wcnt[tid >> 6] = __popcll(bal);
+    __syncthreads();
+    int before = 0, total = 0;
+    for (int w = 0; w < NT / 64; ++w) {
+      before += w < (tid >> 6) ? s_wcnt[w] : 0;
+      total += s_wcnt[w];
+    }
+    const int g = before + __popcll(bal & ((2ull << lane) - 1ull)) - 1;  // starts at or before this lane, - 1
+    if (valid && g < BN_GROUPS) {
+      grp = g;
+      if (start) {
+        g_x0[g] = cx;
+        g_cy[g] = W.cy;
+        g_cz[g] = W.cz;
+        g_rod[g] = a;
+      }
+      if (last) g_x1[g] = cx;
+    }
+    if (tid == 0) s_ng = total < BN_GROUPS ? total : BN_GROUPS;
   }
   const int p0 = p - lane;  // the wavefront's lanes hold chunks p0 .. p0 + 63
   int pending = 0, pend2 = 0;  // wave-uniform queue lengths
-  while (__any(active)) {
-    float4 o[BN_UNROLL];
-    int rb[BN_UNROLL];
-    const int qs = W.q;
-    const int nh = active ? min(BN_UNROLL, W.q1 - W.q) : 0;
-#pragma unroll
-    for (int j = 0; j < BN_UNROLL; ++j) {
-      if (j < nh) {
-        o[j] = SC.cf[qs + j];
-        rb[j] = SC.rod[qs + j];
+  int kl = 0;                  // next entry of the lane's range list
+  constexpr int NPH = STAGED ? 3 : 1;
+  for (int ph = 0; ph < NPH; ++ph) {
+    const int r_end = STAGED ? 6 * (ph + 1) : 18;
+    if constexpr (STAGED) {
+      // ---- stage the rows of z offset dz = ph - 1 (entries (group, dy))
+      __syncthreads();  // group data written; the previous phase's LDS reads are done
+      const int ng = s_ng;
+      if (tid < NE) {
+        int lo = 0, hi = 0;
+        const int g = tid / 3, dy = tid % 3 - 1, dz = ph - 1;
+        if (g < ng) {
+          const SysDev& S = sys[B.sys[g_rod[g]]];
+          const int d0 = S.dims[0], d1 = S.dims[1], d2 = S.dims[2];
+          int z = g_cz[g] + dz, y = g_cy[g] + dy;
+          bool ok = true;
+          if (S.periodic[2]) {
+            if (d2 == 1 && dz != 0) ok = false;
+            z += z < 0 ? d2 : (z >= d2 ? -d2 : 0);
+          } else if (z < 0 || z >= d2) {
+            ok = false;
+          }
+          if (S.periodic[1]) {
+            if (d1 == 1 && dy != 0) ok = false;
+            y += y < 0 ? d1 : (y >= d1 ? -d1 : 0);
+          } else if (y < 0 || y >= d1) {
+            ok = false;
+          }
+          if (ok) {
+            const int xa = g_x0[g] > 0 ? g_x0[g] - 1 : 0;
+            const int xb = g_x1[g] < d0 - 1 ? g_x1[g] + 1 : d0 - 1;
+            const int base = S.cell_off + d0 * (y + d1 * z);
+            // no lane of the group reads a row wholly below its first cell
+            if (base + xb >= S.cell_off + g_x0[g] + d0 * (g_cy[g] + d1 * g_cz[g])) {
+              lo = cstart[base + xa];
+              hi = cstart[base + xb + 1];
+            }
+          }
+        }
+        st_lo[tid] = lo;
+        st_hi[tid] = hi;
       }
+      __syncthreads();
+      if (tid == 0) {
+        int tot = 0;
+        for (int t = 0; t < NE; ++t) {
+          int n = st_hi[t] - st_lo[t];
+          if (tot + n > CAP) {  // does not fit: read from global memory
+            st_hi[t] = st_lo[t];
+            n = 0;
+          }
+          st_off[t] = tot;
+          tot += n;
+        }
+        s_tot = tot;
+      }
+      __syncthreads();
+      const int tot = s_tot;
+      for (int e = tid; e < tot; e += NT) {
+        int t = 0;
+        while (e >= st_off[t] + (st_hi[t] - st_lo[t])) ++t;
+        const int q = st_lo[t] + (e - st_off[t]);
+        st_cf[e] = SC.cf[q];
+        st_rod[e] = SC.rod[q];
+      }
+      __syncthreads();
     }
-    if (active) {
-      W.q += nh;
-      if (W.q >= W.q1) active = W.open_next(cstart);
-    }
-#pragma unroll
-    for (int j = 0; j < BN_UNROLL; ++j) {
-      bool hit = false;
-      if (j < nh) {
-        float dx = o[j].x - me.x, dy = o[j].y - me.y, dz = o[j].z - me.z;
-        if (own.per & 1) dx -= own.span[0] * rintf(dx * own.ispan[0]);
-        if (own.per & 2) dy -= own.span[1] * rintf(dy * own.ispan[1]);
-        if (own.per & 4) dz -= own.span[2] * rintf(dz * own.ispan[2]);
-        const float thr = me.w + o[j].w + own.extra;
-        if (dx * dx + dy * dy + dz * dz < thr * thr) {
-          const int b = rb[j];
-          hit = b != a && !(a_static && !(B.inv_mass[b] > 0.0));
+    // the lane's current range [W.q, W.q1) and where its records are read:
+    // LDS at lbase + q when a staged entry holds the whole range, else global
+    int lbase = 0;
+    bool lds = false;
+    auto open = [&]() -> bool {
+      int rr = -1;
+      while (kl < nl) {
+        const int r1 = rl_r[kl][tid];
+        if (r1 >= r_end) break;  // a later phase's range
+        W.q = rl_q[kl][tid];
+        W.q1 = rl_q1[kl][tid];
+        ++kl;
+        if (W.q < W.q1) {
+          rr = r1;
+          break;
         }
       }
-      const unsigned long long m = __ballot(hit);
-      if (hit) {
-        ++ncand;
-        Q[pending + __popcll(m & ((1ull << lane) - 1ull))] = make_int2(p, qs + j);
+      if (rr < 0) {
+        if (kl < nl || !W.open_next(cstart, r_end)) return false;
+        rr = W.r - 1;
       }
-      pending += __popcll(m);
-      if (pending >= 64) {
-        wave_sync();
-        sat_stage(Q + pending - 64, 64, p0, own, Q2, pend2, B, sys, SC, rod_nchunk, delta, out, cap, counter,
-                  nnarrow);
-        pending -= 64;
-        wave_sync();
+      if constexpr (STAGED) {
+        lds = false;
+        if (grp < BN_GROUPS) {
+          const int t = grp * 3 + (rr >> 1) % 3;  // entry (group, dy) of the range's row
+          lds = W.q >= st_lo[t] && W.q1 <= st_hi[t];
+          lbase = st_off[t] - st_lo[t];
+        }
+      }
+      return true;
+    };
+    bool active = valid && open();
+    while (__any(active)) {
+      float4 o[BN_UNROLL];
+      int rb[BN_UNROLL];
+      const int qs = W.q;
+      const int nh = active ? min(BN_UNROLL, W.q1 - W.q) : 0;
+#pragma unroll
+      for (int j = 0; j < BN_UNROLL; ++j) {
+        if (j < nh) {
+          if (STAGED && lds) {
+            o[j] = st_cf[lbase + qs + j];
+            rb[j] = st_rod[lbase + qs + j];
+          } else {
+            o[j] = SC.cf[qs + j];
+            rb[j] = SC.rod[qs + j];
+          }
+        }
+      }
+      if (active) {
+        W.q += nh;
+        if (W.q >= W.q1) active = open();
+      }
+#pragma unroll
+      for (int j = 0; j < BN_UNROLL; ++j) {
+        bool hit = false;
+        if (j < nh) {
+          float dx = o[j].x - me.x, dy = o[j].y - me.y, dz = o[j].z - me.z;
+          if (own.per & 1) dx -= own.span[0] * rintf(dx * own.ispan[0]);
+          if (own.per & 2) dy -= own.span[1] * rintf(dy * own.ispan[1]);
+          if (own.per & 4) dz -= own.span[2] * rintf(dz * own.ispan[2]);
+          const float thr = me.w + o[j].w + own.extra;
+          if (dx * dx + dy * dy + dz * dz < thr * thr) {
+            const int b = rb[j];
+            hit = b != a && !(a_static && !(B.inv_mass[b] > 0.0));
+          }
+        }
+        const unsigned long long m = __ballot(hit);
+        if (hit) {
+          ++ncand;
+          Q[pending + __popcll(m & ((1ull << lane) - 1ull))] = make_int2(p, qs + j);
+        }
+        pending += __popcll(m);
+        if (pending >= 64) {
+          wave_sync();
+          sat_stage(Q + pending - 64, 64, p0, own, Q2, pend2, B, sys, SC, rod_nchunk, delta, out, cap, counter,
+                    nnarrow);
+          pending -= 64;
+          wave_sync();
+        }
       }
     }
   }
@@ -623,16 +847,17 @@ static int ensure_contacts(rdk_ctx* ctx, int nc) {
 static int launch_broad_narrow(rdk_ctx* ctx, const BodyPtrs& B, const SortedChunks& SC, int M, double delta,
                                unsigned cap) {
   const int nt = ctx->bn_threads;
+  const bool stg = ctx->bn_stage != 0;
   auto go = [&](auto kern, int t) {
     hipLaunchKernelGGL(kern, dim3(grid_for(M, t)), dim3(t), 0, ctx->stream, B, ctx->sys_dev.p, SC, ctx->cell_start.p,
                        ctx->rod_nchunk.p, M, delta, ctx->pkeys.p, cap, ctx->counters.p,
                        (unsigned long long*)(ctx->counters.p + 2));
   };
   switch (nt) {
-    case 64: go(k_broad_narrow<64>, 64); break;
-    case 128: go(k_broad_narrow<128>, 128); break;
-    case 256: go(k_broad_narrow<256>, 256); break;
-    case 512: go(k_broad_narrow<512>, 512); break;
+    case 64: stg ? go(k_broad_narrow<64, true>, 64) : go(k_broad_narrow<64, false>, 64); break;
+    case 128: stg ? go(k_broad_narrow<128, true>, 128) : go(k_broad_narrow<128, false>, 128); break;
+    case 256: stg ? go(k_broad_narrow<256, true>, 256) : go(k_broad_narrow<256, false>, 256); break;
+    case 512: stg ? go(k_broad_narrow<512, true>, 512) : go(k_broad_narrow<512, false>, 512); break;
     default:
       ctx->err = "broad phase: workgroup size must be 64, 128, 256 or 512";
       return RDK_ERR_ARG;

@@ -192,6 +192,7 @@ struct rdk_ctx {
   int solver_mode = 0;  // 0 islands when possible, 1 always global coloured
   int small_m = -1;     // single-workgroup colouring limit (RDK_TUNE_SMALL_M; -1: default)
   int bn_threads = 256; // broad-phase workgroup size (RDK_TUNE_BROAD_BLOCK)
+  int bn_stage = 0;     // broad phase with LDS-staged stencil rows (RDK_TUNE_BROAD_STAGE)
   int nislands = 0;
   int island_counts[4] = {0, 0, 0, 0};  // tiny, small, medium, large
   rdk::DevBuf<int> uf_parent, uf_label, sbody, sbody_alt, iflag, ipos, istart, ilists, bseg, icnt;

@@ -67,6 +67,59 @@ def test_detect_bit_identical_for_every_broad_phase_workgroup_size(gpu_ctx_facto
         compare(ctx.contacts(nc), W.contacts(no), nc)
 
 
+@pytest.mark.parametrize("block", [64, 256, 512])
+@pytest.mark.parametrize("scene", ["periodic", "open", "dense", "multi"])
+def test_lds_staged_broad_phase_bit_identical(gpu_ctx_factory, block, scene):
+    """Stencil rows staged in LDS (RDK_TUNE_BROAD_STAGE 1) change where the
+    candidate records are read, not which pairs are tested: contact list equal
+    to the oracle's and candidate counts equal to the unstaged walk's, for
+    dilute, dense (many chunks per cell, entries that overflow the stage) and
+    multi-system layouts."""
+    print(f"staged broad phase: {scene}, workgroup {block}", flush=True)
+    if scene == "multi":
+        rng = np.random.default_rng(21)
+        parts, descs, offs = [], [], [0]
+        for k, (n, side, per, hl) in enumerate([(700, 18.0, (1, 1, 1), 2.0), (1500, 25.0, (1, 1, 0), 4.0),
+                                                (300, 9.0, (1, 1, 1), 1.0), (900, 30.0, (0, 0, 0), 8.0)]):
+            x, q, r, h = rods(n, side, 200 + k, hlo=0.5, hhi=hl)
+            x += 40.0 * k
+            parts.append((x, q, r, h, side, per, 40.0 * k))
+            descs.append(L.system_desc([40.0 * k] * 3, [40.0 * k + side] * 3, per, 0, 0, 0.5))
+            offs.append(offs[-1] + n)
+        X, Q, R, H = (np.concatenate([p_[i] for p_ in parts]) for i in range(4))
+        n = len(R)
+    else:
+        n, side, per = {"periodic": (3000, 30.0, (1, 1, 1)), "open": (3000, 30.0, (0, 0, 0)),
+                        "dense": (6000, 16.0, (1, 1, 0))}[scene]
+        X, Q, R, H = rods(n, side, 31)
+    res = {}
+    for stage in (0, 1):
+        ctx = gpu_ctx_factory()
+        if scene == "multi":
+            ctx.set_bodies(X, Q, np.zeros((n, 3)), np.zeros((n, 3)), R, H, np.ones(n), np.zeros(n))
+            ctx.set_systems(offs, descs)
+            ctx.set_physics(L.physics())
+        else:
+            load(ctx, X, Q, R, H, np.ones(n), [0] * 3, [side] * 3, per)
+        ctx.set_tuning(L.TUNE_BROAD_BLOCK, block)
+        ctx.set_tuning(L.TUNE_BROAD_STAGE, stage)
+        for chunk_len in (0.0, 1.0):
+            nc = ctx.detect(L.detect_params(0.02, chunk_len, 1.0))
+            res[stage, chunk_len] = (ctx.contacts(nc), ctx.detect_info())
+    for chunk_len in (0.0, 1.0):
+        (c0, i0), (c1, i1) = res[0, chunk_len], res[1, chunk_len]
+        assert len(c0["a"]) > 100
+        compare(c1, c0, len(c0["a"]))
+        assert i1 == i0, (i0, i1)
+    if scene != "multi":
+        W = O.World(X, Q, None, None, R, H, np.ones(n))
+        W.set_box([0] * 3, [side] * 3, per)
+        no = W.detect(0, 0.02)
+        c1 = res[1, 1.0][0]
+        assert len(c1["a"]) == no
+        compare(c1, W.contacts(no), no)
+
+
 def test_detect_positions_outside_box_and_static_pairs(gpu_ctx_factory):
     n, side = 1500, 20.0
     x, q, r, h = rods(n, side, 9)
