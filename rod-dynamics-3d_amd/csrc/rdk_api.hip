@@ -197,8 +197,10 @@ int rdk_destroy(rdk_ctx* ctx) {
   release(ctx->hkeys); release(ctx->hkeys_new); release(ctx->hlast); release(ctx->hlast_new);
   release(ctx->hflag); release(ctx->hscan); release(ctx->pe); release(ctx->temp); release(ctx->scan_tmp); release(ctx->red);
   drop_step_graph(ctx);
-  release(ctx->fx); release(ctx->facc);
+  release(ctx->fx); release(ctx->facc); release(ctx->freport);
   if (ctx->h_counters) (void)hipHostFree(ctx->h_counters);
+  if (ctx->h_report) (void)hipHostFree(ctx->h_report);
+  if (ctx->h_fx_up) (void)hipHostFree(ctx->h_fx_up);
   for (auto& r : ctx->ev_pending) {
     if (r.a) (void)hipEventDestroy(r.a);
     if (r.b) (void)hipEventDestroy(r.b);
@@ -635,21 +637,29 @@ int rdk_nsc_build(rdk_ctx* ctx, const rdk_nsc_params* p, int* nm) {
   return prof_flush(ctx);
 }
 
+// Fields of manifolds 0 .. n-1 into rec (RDK_MF_FIELDS per manifold), from the
+// device layout of field pairs (mf_off).
+static int download_fields(rdk_ctx* ctx, const double* mf, size_t stride, int n, double* rec) {
+  std::vector<double> tmp(2 * (size_t)n);
+  for (int k = 0; k < RDK_MF_FIELDS / 2; ++k) {
+    RDK_HIP(hipMemcpyAsync(tmp.data(), mf + mf_off(2 * k, 0, stride), 2 * (size_t)n * sizeof(double),
+                           hipMemcpyDeviceToHost, ctx->stream));
+    RDK_HIP(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < n; ++i) {
+      rec[(size_t)RDK_MF_FIELDS * i + 2 * k] = tmp[2 * (size_t)i];
+      rec[(size_t)RDK_MF_FIELDS * i + 2 * k + 1] = tmp[2 * (size_t)i + 1];
+    }
+  }
+  return RDK_OK;
+}
+
 int rdk_nsc_get_manifolds(rdk_ctx* ctx, int cap, double* rec, int* ab, double* lambda) {
   CTX_CHECK();
   if (!ctx->have_manifolds) return RDK_ERR_STATE;
   const int n = std::min(cap, ctx->nm);
   if (n <= 0) return RDK_OK;
   const size_t stride = ctx->mf.cap / RDK_MF_FIELDS;
-  if (rec) {
-    std::vector<double> tmp(n);
-    for (int f = 0; f < RDK_MF_FIELDS; ++f) {
-      RDK_HIP(hipMemcpyAsync(tmp.data(), ctx->mf.p + (size_t)f * stride, n * sizeof(double), hipMemcpyDeviceToHost,
-                             ctx->stream));
-      RDK_HIP(hipStreamSynchronize(ctx->stream));
-      for (int i = 0; i < n; ++i) rec[(size_t)RDK_MF_FIELDS * i + f] = tmp[i];
-    }
-  }
+  if (rec) RDK_TRY(download_fields(ctx, ctx->mf.p, stride, n, rec));
   if (ab) {
     std::vector<int> a(n), b(n);
     RDK_TRY(dl(ctx, a.data(), ctx->ma.p, n));
@@ -785,14 +795,7 @@ int rdk_nsc_get_kept_manifolds(rdk_ctx* ctx, int cap, double* rec, int* ab, doub
   const int n = std::min(cap, m);
   if (n > 0) {
     std::vector<double> tmp(n);
-    if (rec) {
-      for (int f = 0; f < RDK_MF_FIELDS; ++f) {
-        RDK_HIP(hipMemcpyAsync(tmp.data(), ctx->kept_mf.p + (size_t)f * ctx->kept_stride, n * sizeof(double),
-                               hipMemcpyDeviceToHost, ctx->stream));
-        RDK_HIP(hipStreamSynchronize(ctx->stream));
-        for (int i = 0; i < n; ++i) rec[(size_t)RDK_MF_FIELDS * i + f] = tmp[i];
-      }
-    }
+    if (rec) RDK_TRY(download_fields(ctx, ctx->kept_mf.p, ctx->kept_stride, n, rec));
     if (ab) {
       std::vector<int> a(n), b(n);
       RDK_HIP(hipMemcpyAsync(a.data(), ctx->kept_ab.p, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -989,10 +992,17 @@ static int fixed_capture(rdk_ctx* ctx, const rdk_detect_params* d, const rdk_nsc
 }
 
 // Host warm-cache sizes -> device words (before fixed steps follow eager ones).
+// The warm-cache sizes the fixed step reads from fx: uploaded when the host's
+// differ from what fx holds (after an eager step or a new fx), from pinned
+// memory the host does not touch again before the next synchronisation.
 static int fixed_upload_warm_counts(rdk_ctx* ctx) {
-  const int v[2] = {ctx->nprev, ctx->nprev_w};
-  RDK_HIP(hipMemcpyAsync(ctx->fx.p + FX_NPREV, v, sizeof(v), hipMemcpyHostToDevice, ctx->stream));
-  RDK_HIP(hipStreamSynchronize(ctx->stream));
+  if (ctx->fx_prev_dev[0] == ctx->nprev && ctx->fx_prev_dev[1] == ctx->nprev_w) return RDK_OK;
+  if (!ctx->h_fx_up) RDK_HIP(hipHostMalloc((void**)&ctx->h_fx_up, 2 * sizeof(int), hipHostMallocDefault));
+  ctx->h_fx_up[0] = ctx->nprev;
+  ctx->h_fx_up[1] = ctx->nprev_w;
+  RDK_HIP(hipMemcpyAsync(ctx->fx.p + FX_NPREV, ctx->h_fx_up, 2 * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  ctx->fx_prev_dev[0] = ctx->nprev;
+  ctx->fx_prev_dev[1] = ctx->nprev_w;
   return RDK_OK;
 }
 
@@ -1055,12 +1065,13 @@ static int step_nsc_fixed(rdk_ctx* ctx, const rdk_detect_params* d, const rdk_ns
       std::memcpy(ctx->work, work, sizeof(work));
       RDK_TRY(rc);
     }
-    int fh[2];
-    double acc[FA_NUM];
-    RDK_HIP(hipMemcpyAsync(fh, ctx->fx.p, sizeof(fh), hipMemcpyDeviceToHost, ctx->stream));
-    RDK_HIP(hipMemcpyAsync(acc, ctx->facc.p, sizeof(acc), hipMemcpyDeviceToHost, ctx->stream));
+    RDK_TRY(fixed_report_enqueue(ctx));
     RDK_HIP(hipStreamSynchronize(ctx->stream));
-    const int ok = fh[1];
+    const FxReport& R = *ctx->h_report;
+    const int* fh = R.fx;
+    const double* acc = R.acc;
+    fixed_apply_counts(ctx, R);  // the failed step of a chunk changed no state: its counts are the last good ones
+    const int ok = fh[FX_STEPS];
     S.steps += ok;
     S.contacts_total += (int64_t)acc[FA_CONTACTS];
     S.manifolds_total += (int64_t)acc[FA_MANIFOLDS];
@@ -1082,10 +1093,8 @@ static int step_nsc_fixed(rdk_ctx* ctx, const rdk_detect_params* d, const rdk_ns
     for (int c = 0; c < 4; ++c) ctx->cls_seen_max[c] = std::max(ctx->cls_seen_max[c], (int)acc[FA_CLS0 + c]);
     ctx->graph_steps += ok;
     done += ok;
-    if (fh[0] != 0) {
+    if (fh[FX_ERR] != 0) {
       // the failed step changed no state: redo it eagerly (regrowing what it must)
-
-      RDK_TRY(fixed_sync_host_counts(ctx));
       RDK_TRY(eager_nsc_step(ctx, d, np, S));
       ctx->graph_fallbacks++;
       done += 1;
@@ -1099,7 +1108,6 @@ static int step_nsc_fixed(rdk_ctx* ctx, const rdk_detect_params* d, const rdk_ns
       backoff = 0;
     }
   }
-  if (uploaded) RDK_TRY(fixed_sync_host_counts(ctx));
   return RDK_OK;
 }
 

@@ -123,6 +123,15 @@ struct EventRec {
   hipEvent_t a, b;
 };
 
+struct FxReport;  // fixed-step report (below)
+
+// Manifold field storage: fields 2k and 2k + 1 of a manifold side by side,
+// pair planes of `stride` manifolds each, so one 16-byte load fetches two
+// fields of one manifold and consecutive manifolds are consecutive in memory.
+__host__ __device__ __forceinline__ size_t mf_off(int field, size_t i, size_t stride) {
+  return (((size_t)(field >> 1) * stride + i) << 1) + (size_t)(field & 1);
+}
+
 }  // namespace rdk
 
 struct rdk_ctx {
@@ -282,6 +291,10 @@ struct rdk_ctx {
   int step_graph_pair_cap = 0;
   rdk::DevBuf<int> fx;
   rdk::DevBuf<double> facc;
+  rdk::DevBuf<double> freport;         // FxReport, packed on the device
+  rdk::FxReport* h_report = nullptr;   // pinned host copy
+  int* h_fx_up = nullptr;              // pinned: warm-cache counts uploaded to fx
+  int fx_prev_dev[2] = {-1, -1};       // warm-cache counts fx holds (-1: unknown)
   hipGraph_t step_graph = nullptr;
   hipGraphExec_t step_exec = nullptr;
   uint64_t step_graph_gen = 0;
@@ -426,6 +439,17 @@ enum FxAcc {
   FA_STEPS = 0, FA_CONTACTS, FA_MANIFOLDS, FA_SWEEPS, FA_COLORS, FA_NONCONV, FA_RESMAX, FA_CANDIDATES, FA_NARROW,
   FA_MSWEEPS, FA_LDS_SWEEPS, FA_HOT_SWEEPS, FA_WALLS, FA_CLS0, FA_CLS1, FA_CLS2, FA_CLS3, FA_NUM
 };
+// Everything the host reads after a run of fixed steps, packed on the device
+// by one kernel and copied back in one transfer: the step words, the pair and
+// wall counters, the colouring state, the island class counts and the step
+// statistics.
+struct FxReport {
+  double acc[FA_NUM];
+  int fx[FX_ROUND0];
+  unsigned ctr[2];
+  unsigned dc[4];
+  int cnt[6];
+};
 
 // One hard-contact step with no host synchronisation (capturable): detection
 // into the reserved pair capacity, manifolds, colouring, islands, the island
@@ -440,6 +464,8 @@ int nsc_fixed_enqueue(rdk_ctx* ctx, const rdk_nsc_params* p);
 int detect_fixed_reserve(rdk_ctx* ctx, const rdk_detect_params* p, int pair_cap);
 int nsc_fixed_reserve(rdk_ctx* ctx, const rdk_nsc_params* p);
 int fixed_sync_host_counts(rdk_ctx* ctx);
+int fixed_report_enqueue(rdk_ctx* ctx);
+void fixed_apply_counts(rdk_ctx* ctx, const FxReport& R);
 int fixed_clear_enqueue(rdk_ctx* ctx);
 void fixed_class_grids(const rdk_ctx* ctx, int* grids);
 

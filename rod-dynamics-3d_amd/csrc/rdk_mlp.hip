@@ -102,7 +102,7 @@ __global__ void k_mlp_features(BodyPtrs B, const SysDev* __restrict__ sys, FeatI
                                float* __restrict__ X, double* __restrict__ lscale) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nm) return;
-  auto at = [&](int fld) { return F.f[(size_t)fld * F.stride + i]; };
+  auto at = [&](int fld) { return F.f[mf_off(fld, i, F.stride)]; };
   const int a = F.a[i], b = F.b[i];
   const bool wall = b < 0;
   const d3 n = mk3(at(0), at(1), at(2)), t1 = mk3(at(3), at(4), at(5)), t2 = mk3(at(6), at(7), at(8));

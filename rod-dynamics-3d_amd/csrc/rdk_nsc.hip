@@ -21,7 +21,7 @@
 
 namespace rdk {
 
-// manifold field offsets (planar, stride = capacity)
+// manifold field offsets (field pairs interleaved per manifold: mf_off)
 enum MF {
   F_N = 0, F_T1 = 3, F_T2 = 6, F_RA = 9, F_RB = 12,
   F_IAN = 15, F_IAT1 = 18, F_IAT2 = 21, F_IBN = 24, F_IBT1 = 27, F_IBT2 = 30,
@@ -29,12 +29,14 @@ enum MF {
   F_GTB = 41, F_COUNT = 42
 };
 
+static_assert(F_COUNT % 2 == 0, "manifold fields are stored in pairs");
+
 struct MfPtrs {
   double* f;
   size_t stride;
   int *a, *b;
   double* lam[3];
-  __device__ __forceinline__ double& at(int field, int i) const { return f[(size_t)field * stride + i]; }
+  __device__ __forceinline__ double& at(int field, int i) const { return f[mf_off(field, i, stride)]; }
   __device__ __forceinline__ d3 get3(int field, int i) const {
     return mk3(at(field, i), at(field + 1, i), at(field + 2, i));
   }
@@ -1232,6 +1234,18 @@ struct MfReg {
   __device__ __forceinline__ double& lr(int c, int) { return lam[c]; }
 };
 
+// Every field of manifold i into f[0 .. F_COUNT): one 16-byte load per field
+// pair.
+__device__ __forceinline__ void load_fields(double* f, const MfPtrs& M, int i) {
+  const double2* f2 = reinterpret_cast<const double2*>(M.f);
+#pragma unroll
+  for (int k = 0; k < F_COUNT / 2; ++k) {
+    const double2 v = f2[(size_t)k * M.stride + i];
+    f[2 * k] = v.x;
+    f[2 * k + 1] = v.y;
+  }
+}
+
 // A register manifold with its velocity-independent update terms (PrepOf).
 struct MfRegP : MfReg {
   double ogn, og1, og2, h11, h12, h22;
@@ -1336,8 +1350,7 @@ __device__ __forceinline__ void island_reg_block(BodyPtrs B, MfPtrs M, const Sys
   int col = -1;
   if (live) {
     const int i = s + j;
-#pragma unroll
-    for (int f = 0; f < F_COUNT; ++f) R.f[f] = M.at(f, i);
+    load_fields(R.f, M, i);
     for (int c = 0; c < 3; ++c) R.lam[c] = M.lam[c][i];
     R.a = slot * BCAP + lb_int(bl, nb, M.a[i]);
     const int gb = M.b[i];
@@ -1487,8 +1500,7 @@ struct MfSlots {
 };
 
 __device__ __forceinline__ void load_manifold(MfReg& R, const MfSlots& A, int i) {
-#pragma unroll
-  for (int f = 0; f < F_COUNT; ++f) R.f[f] = A.at(f, i);
+  load_fields(R.f, A.M, i);
   for (int q = 0; q < 3; ++q) R.lam[q] = A.lr(q, i);
   R.a = A.ia(i);
   R.b = A.ib(i);
@@ -3116,6 +3128,8 @@ int fixed_prepare(rdk_ctx* ctx, const rdk_detect_params* dp, const rdk_nsc_param
   fixed_class_grids(ctx, ctx->fx_grid);
   RDK_TRY(ensure(ctx, ctx->fx, FX_WORDS));
   RDK_TRY(ensure(ctx, ctx->facc, FA_NUM));
+  RDK_TRY(ensure(ctx, ctx->freport, (int)((sizeof(FxReport) + sizeof(double) - 1) / sizeof(double))));
+  ctx->fx_prev_dev[0] = ctx->fx_prev_dev[1] = -1;  // fx may be new: upload the warm-cache counts
   RDK_TRY(ensure(ctx, ctx->resid, 4));
   RDK_TRY(detect_fixed_reserve(ctx, dp, pair_cap));
   RDK_TRY(nsc_fixed_reserve(ctx, np));
@@ -3299,28 +3313,57 @@ int nsc_fixed_enqueue(rdk_ctx* ctx, const rdk_nsc_params* p) {
   return RDK_OK;
 }
 
+// Packs the fixed-step words, counters, colouring state, island counts and
+// statistics into one FxReport (a single transfer to the host afterwards).
+__global__ void k_fx_report(const int* __restrict__ fx, const unsigned* __restrict__ ctr,
+                            const unsigned* __restrict__ dc, const int* __restrict__ cnt,
+                            const double* __restrict__ acc, FxReport* __restrict__ out) {
+  const int t = threadIdx.x;
+  if (t < FA_NUM) out->acc[t] = acc[t];
+  if (t < FX_ROUND0) out->fx[t] = fx[t];
+  if (t < 2) out->ctr[t] = ctr[t];
+  if (t < 4) out->dc[t] = dc[t];
+  if (t < 6) out->cnt[t] = cnt[t];
+}
+
+// Enqueues the report of the fixed steps run so far and its copy into the
+// pinned host FxReport (valid after the next synchronisation of the stream).
+int fixed_report_enqueue(rdk_ctx* ctx) {
+  constexpr int words = (int)((sizeof(FxReport) + sizeof(double) - 1) / sizeof(double));
+  if (ctx->freport.cap < (size_t)words) {
+    ctx->err = "fixed step: report buffer not reserved";
+    return RDK_ERR_STATE;
+  }
+  if (!ctx->h_report) RDK_HIP(hipHostMalloc((void**)&ctx->h_report, sizeof(FxReport), hipHostMallocDefault));
+  hipLaunchKernelGGL(k_fx_report, dim3(1), dim3(64), 0, ctx->stream, (const int*)ctx->fx.p,
+                     (const unsigned*)ctx->counters.p, (const unsigned*)ctx->uncolored.p, (const int*)ctx->icnt.p,
+                     (const double*)ctx->facc.p, (FxReport*)ctx->freport.p);
+  RDK_HIP(hipMemcpyAsync(ctx->h_report, ctx->freport.p, sizeof(FxReport), hipMemcpyDeviceToHost, ctx->stream));
+  return RDK_OK;
+}
+
 // Host copies of the last fixed step's counts (what the eager step leaves in
 // the context), and the warm-cache sizes the eager path reads from the host.
-int fixed_sync_host_counts(rdk_ctx* ctx) {
-  int fxh[FX_ROUND0];
-  unsigned ctr[2], dc[4];
-  int cnt[6];
-  RDK_HIP(hipMemcpyAsync(fxh, ctx->fx.p, sizeof(fxh), hipMemcpyDeviceToHost, ctx->stream));
-  RDK_HIP(hipMemcpyAsync(ctr, ctx->counters.p, sizeof(ctr), hipMemcpyDeviceToHost, ctx->stream));
-  RDK_HIP(hipMemcpyAsync(dc, ctx->uncolored.p, sizeof(dc), hipMemcpyDeviceToHost, ctx->stream));
-  RDK_HIP(hipMemcpyAsync(cnt, ctx->icnt.p, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
-  RDK_HIP(hipStreamSynchronize(ctx->stream));
-  ctx->nc = (int)std::min<unsigned>(ctr[0], (unsigned)ctx->fx_pair_cap);
-  ctx->nm = fxh[FX_NM];
-  ctx->nm_wall = fxh[FX_NWALL];
-  ctx->nprev = fxh[FX_NPREV];
-  ctx->nprev_w = fxh[FX_NPREV_W];
-  ctx->nislands = fxh[FX_NISL];
-  ctx->ncolor = ctx->nm > 0 ? (int)dc[1] + 1 : 0;
-  for (int c = 0; c < 3; ++c) ctx->island_counts[c] = cnt[c];
-  ctx->island_counts[3] = cnt[3] + cnt[4];
+void fixed_apply_counts(rdk_ctx* ctx, const FxReport& R) {
+  ctx->nc = (int)std::min<unsigned>(R.ctr[0], (unsigned)ctx->fx_pair_cap);
+  ctx->nm = R.fx[FX_NM];
+  ctx->nm_wall = R.fx[FX_NWALL];
+  ctx->nprev = R.fx[FX_NPREV];
+  ctx->nprev_w = R.fx[FX_NPREV_W];
+  ctx->fx_prev_dev[0] = ctx->nprev;
+  ctx->fx_prev_dev[1] = ctx->nprev_w;
+  ctx->nislands = R.fx[FX_NISL];
+  ctx->ncolor = ctx->nm > 0 ? (int)R.dc[1] + 1 : 0;
+  for (int c = 0; c < 3; ++c) ctx->island_counts[c] = R.cnt[c];
+  ctx->island_counts[3] = R.cnt[3] + R.cnt[4];
   ctx->have_manifolds = true;
   ctx->have_coloring = false;
+}
+
+int fixed_sync_host_counts(rdk_ctx* ctx) {
+  RDK_TRY(fixed_report_enqueue(ctx));
+  RDK_HIP(hipStreamSynchronize(ctx->stream));
+  fixed_apply_counts(ctx, *ctx->h_report);
   return RDK_OK;
 }
 
