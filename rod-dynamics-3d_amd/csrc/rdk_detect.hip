@@ -233,7 +233,7 @@ struct StencilWalk {
   }
 };
 
-constexpr int BN_QUEUE = 128;  // per wavefront: < 64 pending + one append round of <= 64
+constexpr int BN_QUEUE = 192;  // per wavefront: < 64 pending + one append round of <= 64 x BN_UNROLL
 
 // Separating-axis lower bound on the distance between the axis segments of
 // chunks p and q, over the axes u_p, u_q and u_p x u_q (the last only when it
@@ -325,6 +325,7 @@ __device__ __forceinline__ void sat_stage(const int2* __restrict__ src, int n, i
 }
 
 constexpr int BN_UNROLL = 2;  // candidates loaded per lane per step (independent loads in flight)
+static_assert(BN_QUEUE >= 64 + 64 * BN_UNROLL, "sphere-survivor queue: < 64 pending plus one append round");
 
 // LDS staging of the stencil rows (STAGED kernels).  A workgroup's chunks are
 // a contiguous run of sorted positions, i.e. consecutive cells of at most a
@@ -410,9 +411,9 @@ __global__ void __launch_bounds__(NT, 4) k_broad_narrow(BodyPtrs B, const SysDev
     me = SC.cf[p];
     const SysDev& S = sys[B.sys[a]];
     a_static = !(B.inv_mass[a] > 0.0);
-    for (int k = 0; k < 3; ++k) {
-      own.span[k] = (float)S.span[k];
-      own.ispan[k] = (float)(1.0 / S.span[k]);
+    for (int k = 0; k < 3; ++k) {  // closed axes 0: the walk's image shift is then dx - 0 * rint(0) = dx
+      own.span[k] = S.periodic[k] ? (float)S.span[k] : 0.0f;
+      own.ispan[k] = S.periodic[k] ? (float)(1.0 / S.span[k]) : 0.0f;
       own.per |= S.periodic[k] ? (1 << k) : 0;
     }
     own.extra = (float)delta + S.fmargin;
@@ -618,49 +619,58 @@ __global__ void __launch_bounds__(NT, 4) k_broad_narrow(BodyPtrs B, const SysDev
       int rb[BN_UNROLL];
       const int qs = W.q;
       const int nh = active ? min(BN_UNROLL, W.q1 - W.q) : 0;
+      // every lane loads BN_UNROLL records, no branch around the loads: past
+      // the range's end it repeats its last candidate, an idle lane reads
+      // record 0; the j < nh test below drops those
 #pragma unroll
       for (int j = 0; j < BN_UNROLL; ++j) {
-        if (j < nh) {
-          if (STAGED && lds) {
-            o[j] = st_cf[lbase + qs + j];
-            rb[j] = st_rod[lbase + qs + j];
-          } else {
-            o[j] = SC.cf[qs + j];
-            rb[j] = SC.rod[qs + j];
-          }
+        const int q = nh > 0 ? qs + min(j, nh - 1) : 0;
+        if (STAGED && lds && nh > 0) {
+          o[j] = st_cf[lbase + q];
+          rb[j] = st_rod[lbase + q];
+        } else {
+          o[j] = SC.cf[q];
+          rb[j] = SC.rod[q];
         }
       }
       if (active) {
         W.q += nh;
         if (W.q >= W.q1) active = open();
       }
+      bool hit[BN_UNROLL];
 #pragma unroll
       for (int j = 0; j < BN_UNROLL; ++j) {
-        bool hit = false;
-        if (j < nh) {
-          float dx = o[j].x - me.x, dy = o[j].y - me.y, dz = o[j].z - me.z;
-          if (own.per & 1) dx -= own.span[0] * rintf(dx * own.ispan[0]);
-          if (own.per & 2) dy -= own.span[1] * rintf(dy * own.ispan[1]);
-          if (own.per & 4) dz -= own.span[2] * rintf(dz * own.ispan[2]);
-          const float thr = me.w + o[j].w + own.extra;
-          if (dx * dx + dy * dy + dz * dz < thr * thr) {
-            const int b = rb[j];
-            hit = b != a && !(a_static && !(B.inv_mass[b] > 0.0));
-          }
-        }
-        const unsigned long long m = __ballot(hit);
-        if (hit) {
+        float dx = o[j].x - me.x, dy = o[j].y - me.y, dz = o[j].z - me.z;
+        dx -= own.span[0] * rintf(dx * own.ispan[0]);
+        dy -= own.span[1] * rintf(dy * own.ispan[1]);
+        dz -= own.span[2] * rintf(dz * own.ispan[2]);
+        const float thr = me.w + o[j].w + own.extra;
+        hit[j] = j < nh && dx * dx + dy * dy + dz * dz < thr * thr && rb[j] != a;
+      }
+      if (a_static) {  // a fixed rod pairs only with movable ones (rare: the load only then)
+#pragma unroll
+        for (int j = 0; j < BN_UNROLL; ++j)
+          if (hit[j]) hit[j] = B.inv_mass[rb[j]] > 0.0;
+      }
+      // one append round for the step's candidates
+      const unsigned long long lt = (1ull << lane) - 1ull;
+      int off = pending;
+#pragma unroll
+      for (int j = 0; j < BN_UNROLL; ++j) {
+        const unsigned long long m = __ballot(hit[j]);
+        if (hit[j]) {
           ++ncand;
-          Q[pending + __popcll(m & ((1ull << lane) - 1ull))] = make_int2(p, qs + j);
+          Q[off + __popcll(m & lt)] = make_int2(p, qs + j);
         }
-        pending += __popcll(m);
-        if (pending >= 64) {
-          wave_sync();
-          sat_stage(Q + pending - 64, 64, p0, own, Q2, pend2, B, sys, SC, rod_nchunk, delta, out, cap, counter,
-                    nnarrow);
-          pending -= 64;
-          wave_sync();
-        }
+        off += __popcll(m);
+      }
+      pending = off;
+      while (pending >= 64) {
+        wave_sync();
+        sat_stage(Q + pending - 64, 64, p0, own, Q2, pend2, B, sys, SC, rod_nchunk, delta, out, cap, counter,
+                  nnarrow);
+        pending -= 64;
+        wave_sync();
       }
     }
   }
