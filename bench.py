@@ -447,11 +447,13 @@ def run_c2(a, L, scenes):
         cur = advance(ctx, det, p, cur, s0, t2)
         kt0 = ctx.kernel_times()[dom]
         wc0 = ctx.work_counters()
+        g0 = ctx.step_graph_info()
         ctx.sync()
         tb = time.perf_counter()
         st = ctx.step_nsc(det, p, k)
         ctx.sync()
         dt = time.perf_counter() - tb
+        g1 = ctx.step_graph_info()
         el += dt
         cur += k
         kt1 = ctx.kernel_times()[dom]
@@ -466,7 +468,8 @@ def run_c2(a, L, scenes):
         phases.append({"step": s0 + 1, "steps": k, "ms_per_step": 1e3 * dt / k,
                        "manifolds_per_step": st["manifolds_total"] / k, "sweeps_per_step": st["sweeps_total"] / k,
                        "rod_contacts_per_step": st["contacts_total"] / k, "colors_per_step": st["colors_total"] / k,
-                       "nonconverged_steps": st["nonconverged_steps"], "residual_max": st["residual_max"]})
+                       "nonconverged_steps": st["nonconverged_steps"], "residual_max": st["residual_max"],
+                       "eager_steps": g1["fallbacks"] - g0["fallbacks"], "captures": g1["captures"] - g0["captures"]})
         log(f"timed block at step {s0 + 1}: {1e3 * dt / k:.2f} ms/step, manifolds/step "
             f"{st['manifolds_total'] / k:.0f}, sweeps/step {st['sweeps_total'] / k:.1f}, residual "
             f"{st['residual_max']:.2e}{'' if same else ' (differs from run 1)'}")
