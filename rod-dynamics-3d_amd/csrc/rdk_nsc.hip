@@ -1322,7 +1322,8 @@ __device__ __forceinline__ void island_reg_block(BodyPtrs B, MfPtrs M, const Sys
   __shared__ double v_l[3][IPB * BCAP], w_l[3][IPB * BCAP];
   __shared__ int body_l[IPB * BCAP];
   __shared__ double red[WG / 64 + 1];
-  __shared__ int s_ncol;
+  __shared__ int s_ncol[IPB];     // colours per island
+  __shared__ int s_live_ncol[IPB];  // colours of the islands still sweeping (0 once done)
   const int tid = threadIdx.x, slot = tid / GROUP, j = tid % GROUP;
   const int t = blk * IPB + slot;
   const bool has = t < count;
@@ -1343,7 +1344,7 @@ __device__ __forceinline__ void island_reg_block(BodyPtrs B, MfPtrs M, const Sys
       w_l[c][slot * BCAP + q] = B.w[c][g];
     }
   }
-  if (tid == 0) s_ncol = 0;
+  if (tid < IPB) s_ncol[tid] = 0;
   __syncthreads();
   const bool live = has && j < n;
   MfRegP R;
@@ -1356,7 +1357,7 @@ __device__ __forceinline__ void island_reg_block(BodyPtrs B, MfPtrs M, const Sys
     const int gb = M.b[i];
     R.b = gb >= 0 ? slot * BCAP + lb_int(bl, nb, gb) : -1;
     col = (int)(skey[i] & 127u);
-    atomicMax(&s_ncol, col + 1);
+    atomicMax(&s_ncol[slot], col + 1);
   } else {
 #pragma unroll
     for (int f = 0; f < F_COUNT; ++f) R.f[f] = 0.0;
@@ -1364,7 +1365,8 @@ __device__ __forceinline__ void island_reg_block(BodyPtrs B, MfPtrs M, const Sys
     R.a = R.b = 0;
   }
   __syncthreads();
-  const int ncol = s_ncol;
+  int ncol = 0;  // colours of the workgroup's islands (the warm start)
+  for (int k = 0; k < IPB; ++k) ncol = max(ncol, s_ncol[k]);
   BodyPtrs LB = B;
   for (int c = 0; c < 3; ++c) {
     LB.v[c] = v_l[c];
@@ -1391,9 +1393,16 @@ __device__ __forceinline__ void island_reg_block(BodyPtrs B, MfPtrs M, const Sys
   const int kk = P.stop_rule == 1 ? P.check_every : 1;
   while (__syncthreads_or(!done)) {
     const bool act = live && !done;
+    // sweep only as many colours as the islands still sweeping have (several
+    // islands share a workgroup in the tiny class; a converged island's
+    // colours would otherwise cost a barrier round each)
+    if (j == 0) s_live_ncol[slot] = done ? 0 : s_ncol[slot];
+    __syncthreads();
+    int ncol_live = 0;
+    for (int k = 0; k < IPB; ++k) ncol_live = max(ncol_live, s_live_ncol[k]);
     double d = 0.0;
     for (int it = 0; it < kk; ++it) {
-      for (int c = 0; c < ncol; ++c) {
+      for (int c = 0; c < ncol_live; ++c) {
         if (act && col == c) d = psor_update(LB, R, 0, K, mu);
         __syncthreads();
       }
