@@ -53,7 +53,26 @@ def build(verbose=False, jobs=None):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)
+    build_facade_check()
     return LIB
+
+
+FACADE_SRC = os.path.join(HERE, "cpp", "facade_check.cpp")
+FACADE_LIB = os.path.join(HERE, "librdk_facade_check.so")
+
+
+def build_facade_check():
+    """The C++ classes of include/rdk_physics.hpp behind two C entry points
+    (librdk_facade_check.so, host code only, linked against librdk.so)."""
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    tmp = FACADE_LIB + ".tmp"
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-shared", "-fPIC", "-I", inc, FACADE_SRC,
+           "-L", HERE, "-lrdk", "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("facade check build failed:\n" + r.stdout + r.stderr)
+    os.replace(tmp, FACADE_LIB)
+    return FACADE_LIB
 
 
 if __name__ == "__main__":
