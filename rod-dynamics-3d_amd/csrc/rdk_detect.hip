@@ -237,7 +237,8 @@ constexpr int BN_QUEUE = 192;  // per wavefront: < 64 pending + one append round
 
 // Separating-axis lower bound on the distance between the axis segments of
 // chunks p and q, over the axes u_p, u_q and u_p x u_q (the last only when it
-// is not short, so its normalisation stays accurate): the distance between
+// is not short, so its normalisation stays accurate) and the closest-point
+// direction described below: the distance between
 // two point sets is at least the gap between their projections on any unit
 // axis.  If the owner chunk pair of a touching rod pair is (p, q), its
 // segments are closer than r_p + r_q + delta, so the bound is too; the pair is
@@ -268,7 +269,29 @@ __device__ __forceinline__ bool sat_keep(const SortedChunks& SC, int p, int q, c
   const float nx = ap.y * aq.z - ap.z * aq.y, ny = ap.z * aq.x - ap.x * aq.z, nz = ap.x * aq.y - ap.y * aq.x;
   const float nn = nx * nx + ny * ny + nz * nz;
   if (nn > 1.0f / 256.0f) lb = fmaxf(lb, fabsf(dx * nx + dy * ny + dz * nz) * rsqrtf(nn));
-  return lb <= lim;
+  if (lb > lim) return false;
+  // Fourth axis: the direction between approximate closest points of the two
+  // chunk axes (clamp s, solve t, clamp t, re-solve s).  Any unit axis gives
+  // a valid bound, so the FP32 closest points need no accuracy of their own;
+  // near the true closest points the bound is the segment distance itself,
+  // which rejects the many pairs of long rods whose chunk axes pass each
+  // other farther apart than the three fixed axes can show.  The margin
+  // above covers the axis normalisation (a few ulps of rsqrt).
+  const float b = ap.x * aq.x + ap.y * aq.y + ap.z * aq.z;
+  const float f = dx * ap.x + dy * ap.y + dz * ap.z, g = dx * aq.x + dy * aq.y + dz * aq.z;
+  const float den = 1.0f - b * b;
+  float s = den > 1e-6f ? (f - b * g) * __builtin_amdgcn_rcpf(den) : 0.0f;
+  s = fminf(fmaxf(s, -hp), hp);
+  const float t = fminf(fmaxf(s * b - g, -hq), hq);
+  s = fminf(fmaxf(f + t * b, -hp), hp);
+  const float vx = dx + t * aq.x - s * ap.x, vy = dy + t * aq.y - s * ap.y, vz = dz + t * aq.z - s * ap.z;
+  const float vv = vx * vx + vy * vy + vz * vz;
+  if (!(vv > lim * lim)) return true;  // the closest points may touch: no axis to add
+  const float iv = rsqrtf(vv);
+  const float mx = vx * iv, my = vy * iv, mz = vz * iv;
+  const float gap = fabsf(dx * mx + dy * my + dz * mz) - hp * fabsf(ap.x * mx + ap.y * my + ap.z * mz) -
+                    hq * fabsf(aq.x * mx + aq.y * my + aq.z * mz);
+  return gap <= lim;
 }
 
 // The system constants of one chunk lane, shared across the wavefront: every
