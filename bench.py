@@ -60,8 +60,8 @@ BYTES = {
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 # (FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0, from a calibrated 1 GiB copy)
 # for (workload, kernel); recorded with the command in profiles/
-TRAFFIC = {("c2", "broad_narrow"): 62086.7 * 1024 * 2.0 + 204.5 * 1024,
-           ("c4", "broad_narrow"): 215613.3 * 1024 * 2.0 + 5090.2 * 1024,
+TRAFFIC = {("c2", "broad_narrow"): 11760.8 * 1024 * 2.0 + 153.9 * 1024,
+           ("c4", "broad_narrow"): 127276.0 * 1024 * 2.0 + 3022.6 * 1024,
            ("c2", "island_large"): 848.6 * 1024 * 2.0 + 383.1 * 1024}
 
 
