@@ -180,7 +180,7 @@ int rdk_destroy(rdk_ctx* ctx) {
   release(ctx->sys_of); release(ctx->sys_dev);
   release(ctx->chunk_rod); release(ctx->chunk_k); release(ctx->rod_nchunk);
   release(ctx->ckey); release(ctx->ckey_alt); release(ctx->cid); release(ctx->cid_alt);
-  release(ctx->cell_start); release(ctx->scf); release(ctx->sca); release(ctx->srod); release(ctx->sk);
+  release(ctx->cell_start); release(ctx->scf); release(ctx->sca); release(ctx->ucf); release(ctx->uca); release(ctx->srod); release(ctx->sk);
   release(ctx->pkeys); release(ctx->pkeys_alt); release(ctx->counters);
   release(ctx->ca); release(ctx->cb); release(ctx->ctype); release(ctx->cdist); release(ctx->chs);
   release(ctx->cs); release(ctx->ct);

@@ -9,15 +9,16 @@
 //                      into a uniform per-system grid whose cell is at least
 //                      2*max(chunk radius) + delta, so touching chunks always sit
 //                      in the same or adjacent cells (27-cell stencil, periodic
-//                      wrap on periodic axes).
+//                      wrap on periodic axes); the FP32 chunk records below
+//                      are written here too, in chunk order.
 //   2. radix sort of (cell key, chunk id)
 //   3. cell_start      dense start array: start[c] = first sorted entry with
 //                      cell >= c, for every cell (and one past the last), so
 //                      the three x-adjacent cells of a stencil row are one
 //                      contiguous range [start[x-1], start[x+2])
-//   4. chunk_gather    FP32 (centre, radius), FP32 (axis, half length), rod id
-//                      and chunk index in sorted (cell) order so the stencil
-//                      walk reads contiguous memory
+//   4. chunk_gather    permutes the FP32 (centre, radius), FP32 (axis, half
+//                      length), rod id and chunk index into sorted (cell)
+//                      order so the stencil walk reads contiguous memory
 //   5. broad_narrow    one thread per sorted chunk walks the 9 stencil rows
 //                      (27 cells), several candidates in flight per lane;
 //                      each unordered chunk pair is examined once (by the
@@ -72,21 +73,30 @@ __device__ __forceinline__ void chunk_geom(const BodyPtrs& B, int i, int k, int 
   rad = h / (double)K + B.r[i];
 }
 
+// Cell key of each chunk, and its FP32 prefilter records in chunk order
+// (centre and radius; axis and half length of the chunk segment), which
+// k_chunk_gather then permutes into sorted order: the rod's fields are read
+// here, in chunk order (a rod's chunks are consecutive), instead of once per
+// sorted chunk from scattered positions.
 __global__ void k_chunk_keys(BodyPtrs B, const SysDev* __restrict__ sys, const int* __restrict__ chunk_rod,
                              const int* __restrict__ chunk_k, const int* __restrict__ rod_nchunk, int M,
-                             uint32_t* __restrict__ key, int* __restrict__ id) {
+                             uint32_t* __restrict__ key, int* __restrict__ id, float4* __restrict__ ucf,
+                             float4* __restrict__ uca) {
   int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= M) return;
   const int i = chunk_rod[m];
   const SysDev S = sys[B.sys[i]];
   d3 c, u;
   double rad;
-  chunk_geom(B, i, chunk_k[m], rod_nchunk[i], c, rad, u);
+  const int K = rod_nchunk[i];
+  chunk_geom(B, i, chunk_k[m], K, c, rad, u);
   int cx = cell_coord(c.x, S.lo[0], S.inv_cell[0], S.dims[0], S.periodic[0]);
   int cy = cell_coord(c.y, S.lo[1], S.inv_cell[1], S.dims[1], S.periodic[1]);
   int cz = cell_coord(c.z, S.lo[2], S.inv_cell[2], S.dims[2], S.periodic[2]);
   key[m] = (uint32_t)(S.cell_off + cx + S.dims[0] * (cy + S.dims[1] * cz));
   id[m] = m;
+  ucf[m] = make_float4((float)c.x, (float)c.y, (float)c.z, (float)rad);
+  uca[m] = make_float4((float)u.x, (float)u.y, (float)u.z, (float)(B.h[i] / (double)K));
 }
 
 __global__ void k_fill_int(int* __restrict__ a, int n, int v) {
@@ -114,22 +124,16 @@ struct SortedChunks {
   float4* ca;  // FP32 (axis, half length of the chunk segment)
 };
 
-__global__ void k_chunk_gather(BodyPtrs B, const int* __restrict__ sorted_id, const int* __restrict__ chunk_rod,
-                               const int* __restrict__ chunk_k, const int* __restrict__ rod_nchunk, int M,
-                               SortedChunks S) {
+__global__ void k_chunk_gather(const int* __restrict__ sorted_id, const int* __restrict__ chunk_rod,
+                               const int* __restrict__ chunk_k, const float4* __restrict__ ucf,
+                               const float4* __restrict__ uca, int M, SortedChunks S) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= M) return;
   const int m = sorted_id[p];
-  const int i = chunk_rod[m];
-  const int k = chunk_k[m];
-  d3 c, u;
-  double rad;
-  const int K = rod_nchunk[i];
-  chunk_geom(B, i, k, K, c, rad, u);
-  S.rod[p] = i;
-  S.k[p] = k;
-  S.cf[p] = make_float4((float)c.x, (float)c.y, (float)c.z, (float)rad);
-  S.ca[p] = make_float4((float)u.x, (float)u.y, (float)u.z, (float)(B.h[i] / (double)K));
+  S.rod[p] = chunk_rod[m];
+  S.k[p] = chunk_k[m];
+  S.cf[p] = ucf[m];
+  S.ca[p] = uca[m];
 }
 
 // Workgroups are dispatched round-robin over the XCDs (8 on MI355X, each with
@@ -848,6 +852,8 @@ static int build_grid(rdk_ctx* ctx, const rdk_detect_params* p) {
     RDK_TRY(ensure(ctx, ctx->cid_alt, M));
     RDK_TRY(ensure(ctx, ctx->scf, M));
     RDK_TRY(ensure(ctx, ctx->sca, M));
+    RDK_TRY(ensure(ctx, ctx->ucf, M));
+    RDK_TRY(ensure(ctx, ctx->uca, M));
     RDK_TRY(ensure(ctx, ctx->srod, M));
     RDK_TRY(ensure(ctx, ctx->sk, M));
   }
@@ -939,7 +945,8 @@ int detect_fixed_enqueue(rdk_ctx* ctx, const rdk_detect_params* p) {
   }
   BodyPtrs B = body_ptrs(ctx);
   hipLaunchKernelGGL(k_chunk_keys, dim3(grid_for(M, nb)), dim3(nb), 0, ctx->stream, B, ctx->sys_dev.p,
-                     ctx->chunk_rod.p, ctx->chunk_k.p, ctx->rod_nchunk.p, M, ctx->ckey.p, ctx->cid.p);
+                     ctx->chunk_rod.p, ctx->chunk_k.p, ctx->rod_nchunk.p, M, ctx->ckey.p, ctx->cid.p, ctx->ucf.p,
+                     ctx->uca.p);
   bool alt = false;
   RDK_TRY(sort_pairs_u32(ctx, ctx->ckey.p, ctx->ckey_alt.p, ctx->cid.p, ctx->cid_alt.p, Count::host(M),
                          bits_for((uint64_t)std::max(1, ctx->n_cells - 1)), alt, nullptr));
@@ -953,8 +960,8 @@ int detect_fixed_enqueue(rdk_ctx* ctx, const rdk_detect_params* p) {
   SC.k = ctx->sk.p;
   SC.cf = ctx->scf.p;
   SC.ca = ctx->sca.p;
-  hipLaunchKernelGGL(k_chunk_gather, dim3(grid_for(M, nb)), dim3(nb), 0, ctx->stream, B, sid, ctx->chunk_rod.p,
-                     ctx->chunk_k.p, ctx->rod_nchunk.p, M, SC);
+  hipLaunchKernelGGL(k_chunk_gather, dim3(grid_for(M, nb)), dim3(nb), 0, ctx->stream, sid, ctx->chunk_rod.p,
+                     ctx->chunk_k.p, ctx->ucf.p, ctx->uca.p, M, SC);
   // counters[] were cleared by fixed_clear_enqueue; more contacts than cap fail
   // the step in k_fx_counts
   RDK_TRY(launch_broad_narrow(ctx, B, SC, M, p->delta, (unsigned)cap));
@@ -984,7 +991,8 @@ int detect_run(rdk_ctx* ctx, const rdk_detect_params* p) {
   {
     ProfScope ps(ctx, K_CHUNK_KEYS);
     hipLaunchKernelGGL(k_chunk_keys, dim3(grid_for(M, nb)), dim3(nb), 0, ctx->stream, B, ctx->sys_dev.p,
-                       ctx->chunk_rod.p, ctx->chunk_k.p, ctx->rod_nchunk.p, M, ctx->ckey.p, ctx->cid.p);
+                       ctx->chunk_rod.p, ctx->chunk_k.p, ctx->rod_nchunk.p, M, ctx->ckey.p, ctx->cid.p, ctx->ucf.p,
+                       ctx->uca.p);
   }
   bool alt = false;
   RDK_TRY(sort_pairs_u32(ctx, ctx->ckey.p, ctx->ckey_alt.p, ctx->cid.p, ctx->cid_alt.p, M,
@@ -1004,8 +1012,8 @@ int detect_run(rdk_ctx* ctx, const rdk_detect_params* p) {
   SC.ca = ctx->sca.p;
   {
     ProfScope ps(ctx, K_CHUNK_GATHER);
-    hipLaunchKernelGGL(k_chunk_gather, dim3(grid_for(M, nb)), dim3(nb), 0, ctx->stream, B, sid, ctx->chunk_rod.p,
-                       ctx->chunk_k.p, ctx->rod_nchunk.p, M, SC);
+    hipLaunchKernelGGL(k_chunk_gather, dim3(grid_for(M, nb)), dim3(nb), 0, ctx->stream, sid, ctx->chunk_rod.p,
+                       ctx->chunk_k.p, ctx->ucf.p, ctx->uca.p, M, SC);
   }
   RDK_TRY(ensure(ctx, ctx->counters, 8));
   if (!ctx->h_counters) RDK_HIP(hipHostMalloc((void**)&ctx->h_counters, 64, hipHostMallocDefault));

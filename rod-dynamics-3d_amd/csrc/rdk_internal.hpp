@@ -169,6 +169,7 @@ struct rdk_ctx {
   rdk::DevBuf<int> cid, cid_alt;
   rdk::DevBuf<int> cell_start;  // dense: n_cells + 1 entries
   rdk::DevBuf<float4> scf, sca;
+  rdk::DevBuf<float4> ucf, uca;  // the same records in chunk order (k_chunk_keys)
   rdk::DevBuf<int> srod, sk;
   rdk::DevBuf<uint64_t> pkeys, pkeys_alt;
   rdk::DevBuf<unsigned int> counters;  // [0] pair count, [1..] misc
