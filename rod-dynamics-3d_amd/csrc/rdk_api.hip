@@ -178,7 +178,7 @@ int rdk_destroy(rdk_ctx* ctx) {
   release(ctx->r); release(ctx->h); release(ctx->mass); release(ctx->inv_mass); release(ctx->i_perp);
   release(ctx->i_axial); release(ctx->i_perp_inv); release(ctx->i_axial_inv); release(ctx->rest);
   release(ctx->sys_of); release(ctx->sys_dev);
-  release(ctx->chunk_rod); release(ctx->chunk_k); release(ctx->rod_nchunk);
+  release(ctx->chunk_rod); release(ctx->chunk_k); release(ctx->rod_nchunk); release(ctx->rodrec); release(ctx->rodinfo);
   release(ctx->ckey); release(ctx->ckey_alt); release(ctx->cid); release(ctx->cid_alt);
   release(ctx->cell_start); release(ctx->scf); release(ctx->sca); release(ctx->ucf); release(ctx->uca); release(ctx->srod); release(ctx->sk);
   release(ctx->pkeys); release(ctx->pkeys_alt); release(ctx->counters);

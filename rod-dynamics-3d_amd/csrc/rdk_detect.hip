@@ -73,6 +73,30 @@ __device__ __forceinline__ void chunk_geom(const BodyPtrs& B, int i, int k, int 
   rad = h / (double)K + B.r[i];
 }
 
+// Narrow-phase inputs of one rod, one 64-byte record (position, FP64 axis,
+// half length, radius as four double2) plus (system, chunk count): the FP64
+// narrow phase and the contact records read two lines per rod instead of one
+// per field array.  Written by k_chunk_keys from the rod's chunk 0 with the
+// same FP64 operations the narrow phase would apply to the body arrays.
+struct RodRecs {
+  double2* rec;  // 4 per rod
+  int2* info;    // (system, chunk count)
+};
+struct RodIn {
+  d3 x, u;
+  double h, r;
+};
+__device__ __forceinline__ RodIn load_rod(const RodRecs& R, int i) {
+  const double2 a = R.rec[4 * i], b = R.rec[4 * i + 1], c = R.rec[4 * i + 2], d = R.rec[4 * i + 3];
+  return RodIn{mk3(a.x, a.y, b.x), mk3(b.y, c.x, c.y), d.x, d.y};
+}
+static RodRecs rod_recs(rdk_ctx* ctx) {
+  RodRecs R;
+  R.rec = (double2*)ctx->rodrec.p;
+  R.info = (int2*)ctx->rodinfo.p;
+  return R;
+}
+
 // Cell key of each chunk, and its FP32 prefilter records in chunk order
 // (centre and radius; axis and half length of the chunk segment), which
 // k_chunk_gather then permutes into sorted order: the rod's fields are read
@@ -81,7 +105,7 @@ __device__ __forceinline__ void chunk_geom(const BodyPtrs& B, int i, int k, int 
 __global__ void k_chunk_keys(BodyPtrs B, const SysDev* __restrict__ sys, const int* __restrict__ chunk_rod,
                              const int* __restrict__ chunk_k, const int* __restrict__ rod_nchunk, int M,
                              uint32_t* __restrict__ key, int* __restrict__ id, float4* __restrict__ ucf,
-                             float4* __restrict__ uca) {
+                             float4* __restrict__ uca, RodRecs R) {
   int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= M) return;
   const int i = chunk_rod[m];
@@ -89,7 +113,8 @@ __global__ void k_chunk_keys(BodyPtrs B, const SysDev* __restrict__ sys, const i
   d3 c, u;
   double rad;
   const int K = rod_nchunk[i];
-  chunk_geom(B, i, chunk_k[m], K, c, rad, u);
+  const int k = chunk_k[m];
+  chunk_geom(B, i, k, K, c, rad, u);
   int cx = cell_coord(c.x, S.lo[0], S.inv_cell[0], S.dims[0], S.periodic[0]);
   int cy = cell_coord(c.y, S.lo[1], S.inv_cell[1], S.dims[1], S.periodic[1]);
   int cz = cell_coord(c.z, S.lo[2], S.inv_cell[2], S.dims[2], S.periodic[2]);
@@ -97,6 +122,14 @@ __global__ void k_chunk_keys(BodyPtrs B, const SysDev* __restrict__ sys, const i
   id[m] = m;
   ucf[m] = make_float4((float)c.x, (float)c.y, (float)c.z, (float)rad);
   uca[m] = make_float4((float)u.x, (float)u.y, (float)u.z, (float)(B.h[i] / (double)K));
+  if (k == 0) {
+    const d3 x = load3(B.x, i);
+    R.rec[4 * i] = make_double2(x.x, x.y);
+    R.rec[4 * i + 1] = make_double2(x.z, u.x);
+    R.rec[4 * i + 2] = make_double2(u.y, u.z);
+    R.rec[4 * i + 3] = make_double2(B.h[i], B.r[i]);
+    R.info[i] = make_int2(B.sys[i], K);
+  }
 }
 
 __global__ void k_fill_int(int* __restrict__ a, int n, int v) {
@@ -165,18 +198,18 @@ __device__ __forceinline__ int owner_chunk(double s, int K) {
 // FP64 narrow phase of one surviving chunk pair (p, q), both in sorted
 // order; emits the canonical (lower, higher) rod pair when this chunk pair
 // owns the closest points.
-__device__ __forceinline__ void narrow_pair(const BodyPtrs& B, const SysDev* __restrict__ sys,
-                                            const SortedChunks& SC, const int* __restrict__ rod_nchunk, int p, int q,
-                                            double delta, uint64_t* __restrict__ out, unsigned cap,
+__device__ __forceinline__ void narrow_pair(const RodRecs& R, const SysDev* __restrict__ sys, const SortedChunks& SC,
+                                            int p, int q, double delta, uint64_t* __restrict__ out, unsigned cap,
                                             unsigned* __restrict__ counter) {
   const int a = SC.rod[p], b = SC.rod[q];
-  const SysDev& S = sys[B.sys[a]];
   const int A = a < b ? a : b, Bi = a < b ? b : a;
   const int kA = a < b ? SC.k[p] : SC.k[q], kB = a < b ? SC.k[q] : SC.k[p];
+  const int2 ia = R.info[A], ib = R.info[Bi];
+  const RodIn ra = load_rod(R, A), rb = load_rod(R, Bi);
+  const SysDev& S = sys[ia.x];
   NarrowOut no;
-  if (capsule_narrow(load3(B.x, A), axis_of(load_q(B, A)), B.h[A], B.r[A], load3(B.x, Bi), axis_of(load_q(B, Bi)),
-                     B.h[Bi], B.r[Bi], S.span, S.periodic, delta, false, no)) {
-    if (owner_chunk(no.s, rod_nchunk[A]) == kA && owner_chunk(no.t, rod_nchunk[Bi]) == kB) {
+  if (capsule_narrow(ra.x, ra.u, ra.h, ra.r, rb.x, rb.u, rb.h, rb.r, S.span, S.periodic, delta, false, no)) {
+    if (owner_chunk(no.s, ia.y) == kA && owner_chunk(no.t, ib.y) == kB) {
       const unsigned pos = wave_append(counter);
       if (pos < cap) out[pos] = ((uint64_t)(uint32_t)A << 32) | (uint64_t)(uint32_t)Bi;
     }
@@ -319,9 +352,8 @@ __device__ __forceinline__ void wave_sync() {
 // whenever the narrow queue holds 64, all lanes run the FP64 narrow phase on
 // one entry each.  Called by the whole wavefront (n and pend2 are uniform).
 __device__ __forceinline__ void sat_stage(const int2* __restrict__ src, int n, int p0, const LaneSys& own,
-                                          int2* __restrict__ Q2, int& pend2, const BodyPtrs& B,
-                                          const SysDev* __restrict__ sys, const SortedChunks& SC,
-                                          const int* __restrict__ rod_nchunk, double delta,
+                                          int2* __restrict__ Q2, int& pend2, const RodRecs& R,
+                                          const SysDev* __restrict__ sys, const SortedChunks& SC, double delta,
                                           uint64_t* __restrict__ out, unsigned cap, unsigned* __restrict__ counter,
                                           unsigned long long& nnarrow) {
   const int lane = __lane_id();
@@ -345,7 +377,7 @@ __device__ __forceinline__ void sat_stage(const int2* __restrict__ src, int n, i
   if (pend2 >= 64) {
     wave_sync();
     const int2 f = Q2[pend2 - 64 + lane];
-    narrow_pair(B, sys, SC, rod_nchunk, f.x, f.y, delta, out, cap, counter);
+    narrow_pair(R, sys, SC, f.x, f.y, delta, out, cap, counter);
     pend2 -= 64;
     wave_sync();
   }
@@ -397,8 +429,8 @@ constexpr int BN_SLOTS = 8;
 // (rdk_set_tuning(RDK_TUNE_BROAD_STAGE)).
 template <int NT, bool STAGED>
 __global__ void __launch_bounds__(NT, 4) k_broad_narrow(BodyPtrs B, const SysDev* __restrict__ sys,
-                                                        SortedChunks SC, const int* __restrict__ cstart,
-                                                        const int* __restrict__ rod_nchunk, int M, double delta,
+                                                        SortedChunks SC, const int* __restrict__ cstart, RodRecs R,
+                                                        int M, double delta,
                                                         uint64_t* __restrict__ out, unsigned cap,
                                                         unsigned* __restrict__ counter,
                                                         unsigned long long* __restrict__ cand_counter) {
@@ -694,7 +726,7 @@ __global__ void __launch_bounds__(NT, 4) k_broad_narrow(BodyPtrs B, const SysDev
       pending = off;
       while (pending >= 64) {
         wave_sync();
-        sat_stage(Q + pending - 64, 64, p0, own, Q2, pend2, B, sys, SC, rod_nchunk, delta, out, cap, counter,
+        sat_stage(Q + pending - 64, 64, p0, own, Q2, pend2, R, sys, SC, delta, out, cap, counter,
                   nnarrow);
         pending -= 64;
         wave_sync();
@@ -703,11 +735,11 @@ __global__ void __launch_bounds__(NT, 4) k_broad_narrow(BodyPtrs B, const SysDev
   }
   wave_sync();
   if (pending > 0)
-    sat_stage(Q, pending, p0, own, Q2, pend2, B, sys, SC, rod_nchunk, delta, out, cap, counter, nnarrow);
+    sat_stage(Q, pending, p0, own, Q2, pend2, R, sys, SC, delta, out, cap, counter, nnarrow);
   wave_sync();
   if (lane < pend2) {
     const int2 e = Q2[lane];
-    narrow_pair(B, sys, SC, rod_nchunk, e.x, e.y, delta, out, cap, counter);
+    narrow_pair(R, sys, SC, e.x, e.y, delta, out, cap, counter);
   }
   // wave-level reduction of the sphere-survivor and narrow-phase counts
   for (int off = 32; off > 0; off >>= 1) {
@@ -723,20 +755,20 @@ struct ContactPtrs {
   double *pa[3], *pb[3], *n[3], *sh[3], *dist, *hs, *s, *t;
 };
 
-__global__ void k_contact_record(BodyPtrs B, const SysDev* __restrict__ sys, const uint64_t* __restrict__ keys,
+__global__ void k_contact_record(RodRecs R, const SysDev* __restrict__ sys, const uint64_t* __restrict__ keys,
                                  Count ncc, double delta, ContactPtrs C) {
   const int nc = ncc.get();
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nc) return;
   const uint64_t key = keys[i];
   const int a = (int)(key >> 32), b = (int)(key & 0xffffffffu);
-  const SysDev S = sys[B.sys[a]];
+  const RodIn ra = load_rod(R, a), rb = load_rod(R, b);
+  const SysDev S = sys[R.info[a].x];
   NarrowOut o;
   o.type = 0;
   o.dist = 0.0;
   o.pa = o.pb = o.n = mk3(0, 0, 0);
-  capsule_narrow(load3(B.x, a), axis_of(load_q(B, a)), B.h[a], B.r[a], load3(B.x, b), axis_of(load_q(B, b)), B.h[b],
-                 B.r[b], S.span, S.periodic, delta, true, o);
+  capsule_narrow(ra.x, ra.u, ra.h, ra.r, rb.x, rb.u, rb.h, rb.r, S.span, S.periodic, delta, true, o);
   C.a[i] = a;
   C.b[i] = b;
   C.type[i] = o.type;
@@ -753,7 +785,7 @@ __global__ void k_contact_record(BodyPtrs B, const SysDev* __restrict__ sys, con
   C.sh[1][i] = o.shift.y;
   C.sh[2][i] = o.shift.z;
   C.dist[i] = o.dist;
-  C.hs[i] = B.r[a] + B.r[b];
+  C.hs[i] = ra.r + rb.r;
   C.s[i] = o.s;
   C.t[i] = o.t;
 }
@@ -843,6 +875,8 @@ static int build_grid(rdk_ctx* ctx, const rdk_detect_params* p) {
     RDK_TRY(ensure(ctx, ctx->chunk_rod, M));
     RDK_TRY(ensure(ctx, ctx->chunk_k, M));
     RDK_TRY(ensure(ctx, ctx->rod_nchunk, n));
+    RDK_TRY(ensure(ctx, ctx->rodrec, 8 * (size_t)n));
+    RDK_TRY(ensure(ctx, ctx->rodinfo, 2 * (size_t)n));
     RDK_HIP(hipMemcpyAsync(ctx->chunk_rod.p, crod.data(), M * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     RDK_HIP(hipMemcpyAsync(ctx->chunk_k.p, ck.data(), M * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     RDK_HIP(hipMemcpyAsync(ctx->rod_nchunk.p, nchunk.data(), n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
@@ -889,7 +923,7 @@ static int launch_broad_narrow(rdk_ctx* ctx, const BodyPtrs& B, const SortedChun
   const bool stg = ctx->bn_stage != 0;
   auto go = [&](auto kern, int t) {
     hipLaunchKernelGGL(kern, dim3(grid_for(M, t)), dim3(t), 0, ctx->stream, B, ctx->sys_dev.p, SC, ctx->cell_start.p,
-                       ctx->rod_nchunk.p, M, delta, ctx->pkeys.p, cap, ctx->counters.p,
+                       rod_recs(ctx), M, delta, ctx->pkeys.p, cap, ctx->counters.p,
                        (unsigned long long*)(ctx->counters.p + 2));
   };
   switch (nt) {
@@ -946,7 +980,7 @@ int detect_fixed_enqueue(rdk_ctx* ctx, const rdk_detect_params* p) {
   BodyPtrs B = body_ptrs(ctx);
   hipLaunchKernelGGL(k_chunk_keys, dim3(grid_for(M, nb)), dim3(nb), 0, ctx->stream, B, ctx->sys_dev.p,
                      ctx->chunk_rod.p, ctx->chunk_k.p, ctx->rod_nchunk.p, M, ctx->ckey.p, ctx->cid.p, ctx->ucf.p,
-                     ctx->uca.p);
+                     ctx->uca.p, rod_recs(ctx));
   bool alt = false;
   RDK_TRY(sort_pairs_u32(ctx, ctx->ckey.p, ctx->ckey_alt.p, ctx->cid.p, ctx->cid_alt.p, Count::host(M),
                          bits_for((uint64_t)std::max(1, ctx->n_cells - 1)), alt, nullptr));
@@ -969,8 +1003,8 @@ int detect_fixed_enqueue(rdk_ctx* ctx, const rdk_detect_params* p) {
   bool kalt = false;
   RDK_TRY(sort_pair_keys_u64(ctx, ctx->pkeys.p, ctx->pkeys_alt.p, nc, bits_for((uint64_t)ctx->n), kalt, nullptr));
   const uint64_t* keys = kalt ? ctx->pkeys_alt.p : ctx->pkeys.p;
-  hipLaunchKernelGGL(k_contact_record, dim3(grid_for(cap, nb)), dim3(nb), 0, ctx->stream, B, ctx->sys_dev.p, keys, nc,
-                     p->delta, contact_ptrs(ctx));
+  hipLaunchKernelGGL(k_contact_record, dim3(grid_for(cap, nb)), dim3(nb), 0, ctx->stream, rod_recs(ctx),
+                     ctx->sys_dev.p, keys, nc, p->delta, contact_ptrs(ctx));
   if (kalt) {  // the sorted keys are the canonical ones (warm cache, history)
     std::swap(ctx->pkeys.p, ctx->pkeys_alt.p);
     std::swap(ctx->pkeys.cap, ctx->pkeys_alt.cap);
@@ -992,7 +1026,7 @@ int detect_run(rdk_ctx* ctx, const rdk_detect_params* p) {
     ProfScope ps(ctx, K_CHUNK_KEYS);
     hipLaunchKernelGGL(k_chunk_keys, dim3(grid_for(M, nb)), dim3(nb), 0, ctx->stream, B, ctx->sys_dev.p,
                        ctx->chunk_rod.p, ctx->chunk_k.p, ctx->rod_nchunk.p, M, ctx->ckey.p, ctx->cid.p, ctx->ucf.p,
-                       ctx->uca.p);
+                       ctx->uca.p, rod_recs(ctx));
   }
   bool alt = false;
   RDK_TRY(sort_pairs_u32(ctx, ctx->ckey.p, ctx->ckey_alt.p, ctx->cid.p, ctx->cid_alt.p, M,
@@ -1075,8 +1109,8 @@ int detect_run(rdk_ctx* ctx, const rdk_detect_params* p) {
   C.t = ctx->ct.p;
   {
     ProfScope ps(ctx, K_CONTACT_RECORD);
-    hipLaunchKernelGGL(k_contact_record, dim3(grid_for(nc, nb)), dim3(nb), 0, ctx->stream, B, ctx->sys_dev.p, keys,
-                       Count::host(nc), p->delta, C);
+    hipLaunchKernelGGL(k_contact_record, dim3(grid_for(nc, nb)), dim3(nb), 0, ctx->stream, rod_recs(ctx),
+                       ctx->sys_dev.p, keys, Count::host(nc), p->delta, C);
   }
   // keep the canonical keys for history matching
   if (kalt) {

@@ -165,6 +165,8 @@ struct rdk_ctx {
   std::vector<double> sys_radmax;  // largest chunk bounding radius per system
   bool grid_dirty = false;         // a box changed since the grid was sized
   rdk::DevBuf<int> chunk_rod, chunk_k, rod_nchunk;
+  rdk::DevBuf<double> rodrec;  // per rod: x, FP64 axis, half length, radius (k_chunk_keys)
+  rdk::DevBuf<int> rodinfo;     // per rod: system, chunk count
   rdk::DevBuf<uint32_t> ckey, ckey_alt;
   rdk::DevBuf<int> cid, cid_alt;
   rdk::DevBuf<int> cell_start;  // dense: n_cells + 1 entries
