@@ -270,7 +270,8 @@ struct StencilWalk {
   }
 };
 
-constexpr int BN_QUEUE = 192;  // per wavefront: < 64 pending + one append round of <= 64 x BN_UNROLL
+constexpr int BN_QUEUE = 448;  // per wavefront: < 64 pending + one append round of <= 64 x BN_UNROLL
+constexpr int BN_QUEUE2 = 128;  // separating-axis survivors: < 64 pending + one round of <= 64
 
 // Separating-axis lower bound on the distance between the axis segments of
 // chunks p and q, over the axes u_p, u_q and u_p x u_q (the last only when it
@@ -383,7 +384,7 @@ __device__ __forceinline__ void sat_stage(const int2* __restrict__ src, int n, i
   }
 }
 
-constexpr int BN_UNROLL = 2;  // candidates loaded per lane per step (independent loads in flight)
+constexpr int BN_UNROLL = 6;  // candidates loaded per lane per step (independent loads in flight)
 static_assert(BN_QUEUE >= 64 + 64 * BN_UNROLL, "sphere-survivor queue: < 64 pending plus one append round");
 
 // LDS staging of the stencil rows (STAGED kernels).  A workgroup's chunks are
@@ -437,7 +438,7 @@ __global__ void __launch_bounds__(NT, 4) k_broad_narrow(BodyPtrs B, const SysDev
   constexpr int CAP = STAGED ? bn_cap<NT>() : 1;
   constexpr int NE = 3 * BN_GROUPS;  // staged entries per phase: (group, dy)
   __shared__ int2 queue[NT / 64][BN_QUEUE];
-  __shared__ int2 queue2[NT / 64][BN_QUEUE];
+  __shared__ int2 queue2[NT / 64][BN_QUEUE2];
   __shared__ float4 st_cf[CAP];
   __shared__ int st_rod[CAP];
   __shared__ int st_lo[NE], st_hi[NE], st_off[NE];
