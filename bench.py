@@ -62,7 +62,7 @@ BYTES = {
 # for (workload, kernel); recorded with the command in profiles/
 TRAFFIC = {("c2", "broad_narrow"): 11760.8 * 1024 * 2.0 + 153.9 * 1024,
            ("c4", "broad_narrow"): 127276.0 * 1024 * 2.0 + 3022.6 * 1024,
-           ("c3", "broad_narrow"): 1057140.0 * 1024 * 2.0 + 48165.0 * 1024,
+           ("c3", "broad_narrow"): 697756.0 * 1024 * 2.0 + 48165.0 * 1024,
            ("c2", "island_large"): 848.6 * 1024 * 2.0 + 383.1 * 1024}
 
 

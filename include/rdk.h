@@ -351,10 +351,15 @@ int rdk_set_step_graph(rdk_ctx* ctx, int mode, int pair_capacity);
  * variable).  RDK_TUNE_BROAD_BLOCK: workgroup size of the broad phase, 64,
  * 128, 256 (default) or 512.  RDK_TUNE_BROAD_STAGE: 1 stages each
  * workgroup's stencil rows in LDS before its candidate walk (three passes, one
- * per z offset), 0 (default) walks them in global memory. */
+ * per z offset), 0 (default) walks them in global memory.
+ * RDK_TUNE_CELL_BAND: the cell order of each system's grid runs z over bands
+ * of y-rows so that a large dense grid's stencil planes stay in L2; `value`
+ * >= 1 fixes the band height in rows (capped at the grid's rows), 0 (default)
+ * sizes it from the grid and the chunk count. */
 #define RDK_TUNE_SMALL_M 0
 #define RDK_TUNE_BROAD_BLOCK 1
 #define RDK_TUNE_BROAD_STAGE 2
+#define RDK_TUNE_CELL_BAND 3
 int rdk_set_tuning(rdk_ctx* ctx, int what, int value);
 /* steps run by the fixed path, steps redone by the eager step, graph captures */
 int rdk_step_graph_info(rdk_ctx* ctx, int64_t* fixed_steps, int64_t* fallbacks, int64_t* captures);

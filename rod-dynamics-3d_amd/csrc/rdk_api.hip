@@ -349,6 +349,7 @@ int rdk_set_systems(rdk_ctx* ctx, int nsys, const int* off, const rdk_system_des
     S.floor_wall = desc[s].floor_wall ? 1 : 0;
     S.ceil_wall = desc[s].ceil_wall ? 1 : 0;
     S.cell_off = 0;
+    S.ytile = 1;
     S.mu = desc[s].mu;
   }
   RDK_HIP(hipMemcpyAsync(ctx->sys_of.p, ctx->h_sys_of.data(), ctx->n * sizeof(int), hipMemcpyHostToDevice,
@@ -1148,6 +1149,13 @@ int rdk_set_tuning(rdk_ctx* ctx, int what, int value) {
   if (what == RDK_TUNE_BROAD_STAGE) {
     if (value != 0 && value != 1) return RDK_ERR_ARG;
     ctx->bn_stage = value;
+    drop_step_graph(ctx);
+    return RDK_OK;
+  }
+  if (what == RDK_TUNE_CELL_BAND) {
+    if (value < 0) return RDK_ERR_ARG;
+    ctx->cell_band = value;
+    ctx->grid_dirty = true;
     drop_step_graph(ctx);
     return RDK_OK;
   }

@@ -57,6 +57,25 @@ __device__ __forceinline__ int cell_coord(double c, double lo, double inv_cell, 
   return i;
 }
 
+// Cell order of a system's grid: x fastest, then rows (y, z) in bands of
+// `T` y-rows; inside a band z runs over the whole grid before the next band.
+// Every row stays contiguous in x (the stencil walk needs that) and the
+// sorted chunks walk one band's z-planes in turn, so the three planes a
+// stencil reads stay small enough for one XCD's L2 on large dense grids.
+// T = d1 is the plain order row = y + d1 * z.
+__host__ __device__ __forceinline__ int cell_row(int d1, int d2, int T, int y, int z) {
+  const int y0 = (y / T) * T;
+  const int tl = d1 - y0 < T ? d1 - y0 : T;
+  return y0 * d2 + z * tl + (y - y0);
+}
+__host__ __device__ __forceinline__ void row_yz(int d1, int d2, int T, int row, int& y, int& z) {
+  const int y0 = (row / (T * d2)) * T;
+  const int tl = d1 - y0 < T ? d1 - y0 : T;
+  const int rem = row - y0 * d2;
+  z = rem / tl;
+  y = y0 + rem % tl;
+}
+
 __device__ __forceinline__ q4 load_q(const BodyPtrs& B, int i) {
   return q4{B.q[0][i], B.q[1][i], B.q[2][i], B.q[3][i]};
 }
@@ -118,7 +137,7 @@ __global__ void k_chunk_keys(BodyPtrs B, const SysDev* __restrict__ sys, const i
   int cx = cell_coord(c.x, S.lo[0], S.inv_cell[0], S.dims[0], S.periodic[0]);
   int cy = cell_coord(c.y, S.lo[1], S.inv_cell[1], S.dims[1], S.periodic[1]);
   int cz = cell_coord(c.z, S.lo[2], S.inv_cell[2], S.dims[2], S.periodic[2]);
-  key[m] = (uint32_t)(S.cell_off + cx + S.dims[0] * (cy + S.dims[1] * cz));
+  key[m] = (uint32_t)(S.cell_off + cx + S.dims[0] * cell_row(S.dims[1], S.dims[2], S.ytile, cy, cz));
   id[m] = m;
   ucf[m] = make_float4((float)c.x, (float)c.y, (float)c.z, (float)rad);
   uca[m] = make_float4((float)u.x, (float)u.y, (float)u.z, (float)(B.h[i] / (double)K));
@@ -221,7 +240,7 @@ __device__ __forceinline__ void narrow_pair(const RodRecs& R, const SysDev* __re
 // row + part.  Returns false when no range is left; q is clipped to q > p.
 struct StencilWalk {
   int p, cy, cz, xa0, xa1, xb;  // x ranges: [xa0, xa1] and, after a periodic wrap, the single cell xb
-  int d0, d1, d2, off;  // the system's grid (kept in registers: on the path to the cstart loads)
+  int d0, d1, d2, off, ytile;  // the system's grid (kept in registers: on the path to the cstart loads)
   int kp;               // cell key of chunk p
   bool two, py, pz;     // periodic in y, z
   int r, q, q1;
@@ -249,7 +268,7 @@ struct StencilWalk {
     } else if (y < 0 || y >= d1) {
       return false;
     }
-    const int base = off + d0 * (y + d1 * z);
+    const int base = off + d0 * cell_row(d1, d2, ytile, y, z);
     c0 = base + (part ? xb : xa0);
     c1 = base + (part ? xb : xa1);
     return c1 >= kp;
@@ -482,13 +501,13 @@ __global__ void __launch_bounds__(NT, 4) k_broad_narrow(BodyPtrs B, const SysDev
     W.d1 = d1;
     W.d2 = S.dims[2];
     W.off = S.cell_off;
+    W.ytile = S.ytile;
     W.py = S.periodic[1] != 0;
     W.pz = S.periodic[2] != 0;
     const int cell = (int)SC.key[p] - S.cell_off;
     W.kp = (int)SC.key[p];
     cx = cell % d0;
-    W.cy = (cell / d0) % d1;
-    W.cz = cell / (d0 * d1);
+    row_yz(d1, W.d2, S.ytile, cell / d0, W.cy, W.cz);
     W.two = false;
     W.xb = 0;
     if (d0 == 1) {
@@ -606,9 +625,9 @@ __global__ void __launch_bounds__(NT, 4) k_broad_narrow(BodyPtrs B, const SysDev
           if (ok) {
             const int xa = g_x0[g] > 0 ? g_x0[g] - 1 : 0;
             const int xb = g_x1[g] < d0 - 1 ? g_x1[g] + 1 : d0 - 1;
-            const int base = S.cell_off + d0 * (y + d1 * z);
+            const int base = S.cell_off + d0 * cell_row(d1, d2, S.ytile, y, z);
             // no lane of the group reads a row wholly below its first cell
-            if (base + xb >= S.cell_off + g_x0[g] + d0 * (g_cy[g] + d1 * g_cz[g])) {
+            if (base + xb >= S.cell_off + g_x0[g] + d0 * cell_row(d1, d2, S.ytile, g_cy[g], g_cz[g])) {
               lo = cstart[base + xa];
               hi = cstart[base + xb + 1];
             }
@@ -830,6 +849,14 @@ static int size_grids(rdk_ctx* ctx, double delta, double scale) {
     // FP32 prefilter margin: rounding of centres, radii, the image shift and
     // the squared distance, bounded by a few ulps of the largest coordinate
     S.fmargin = (float)(64.0 * amax * 1.1920928955078125e-07 + 1e-6 * (2.0 * radmax + delta) + 1e-6);
+    // rows per band of the cell order: three z-planes of a band (and the row
+    // either side) within about 2 MB of chunk records (56 B each: centre and
+    // radius, axis and half length, rod, chunk index), at least 4 rows
+    const double nck = s < (int)ctx->sys_nchunk.size() ? (double)ctx->sys_nchunk[s] : 0.0;
+    const double row_bytes = 56.0 * S.dims[0] * nck / (double)total;
+    const double rows = row_bytes > 0.0 ? 2.0 * 1024.0 * 1024.0 / (3.0 * row_bytes) - 2.0 : 1e9;
+    S.ytile = rows < (double)S.dims[1] ? std::min(S.dims[1], std::max(4, (int)rows)) : S.dims[1];
+    if (ctx->cell_band > 0) S.ytile = std::min(S.dims[1], ctx->cell_band);
     S.cell_off = cell_off;
     cell_off += (int)total;
   }
@@ -859,6 +886,7 @@ static int build_grid(rdk_ctx* ctx, const rdk_detect_params* p) {
     crod.reserve(n);
     ck.reserve(n);
     ctx->sys_radmax.assign(ctx->nsys, 0.0);
+    ctx->sys_nchunk.assign(ctx->nsys, 0);
     for (int i = 0; i < n; ++i) {
       int K = 1;
       if (cl > 0.0) K = std::max(1, (int)std::ceil(2.0 * ctx->h_h[i] / cl));
@@ -870,6 +898,7 @@ static int build_grid(rdk_ctx* ctx, const rdk_detect_params* p) {
       const double rad = ctx->h_h[i] / K + ctx->h_r[i];
       const int s = ctx->h_sys_of[i];
       ctx->sys_radmax[s] = std::max(ctx->sys_radmax[s], rad);
+      ctx->sys_nchunk[s] += K;
     }
     const int M = (int)crod.size();
     ctx->n_chunks = M;

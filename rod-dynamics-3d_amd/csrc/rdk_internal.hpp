@@ -108,6 +108,7 @@ struct SysDev {
   double lo[3], hi[3], span[3], cell[3], inv_cell[3];
   int periodic[3], dims[3];
   int cell_off, floor_wall, ceil_wall;
+  int ytile;      // grid rows (y) per band of the cell order; dims[1] when the order is not banded
   float fmargin;  // FP32 broad-phase prefilter margin
   double mu;
 };
@@ -163,6 +164,7 @@ struct rdk_ctx {
   double built_chunk_len = -2.0, built_cell_scale = -1.0, built_delta = -1.0;
   int n_chunks = 0, n_cells = 0;
   std::vector<double> sys_radmax;  // largest chunk bounding radius per system
+  std::vector<int64_t> sys_nchunk; // chunks per system
   bool grid_dirty = false;         // a box changed since the grid was sized
   rdk::DevBuf<int> chunk_rod, chunk_k, rod_nchunk;
   rdk::DevBuf<double> rodrec;  // per rod: x, FP64 axis, half length, radius (k_chunk_keys)
@@ -205,6 +207,7 @@ struct rdk_ctx {
   int small_m = -1;     // single-workgroup colouring limit (RDK_TUNE_SMALL_M; -1: default)
   int bn_threads = 256; // broad-phase workgroup size (RDK_TUNE_BROAD_BLOCK)
   int bn_stage = 0;     // broad phase with LDS-staged stencil rows (RDK_TUNE_BROAD_STAGE)
+  int cell_band = 0;    // rows per band of the cell order, 0: sized per grid (RDK_TUNE_CELL_BAND)
   int nislands = 0;
   int island_counts[4] = {0, 0, 0, 0};  // tiny, small, medium, large
   rdk::DevBuf<int> uf_parent, uf_label, sbody, sbody_alt, iflag, ipos, istart, ilists, bseg, icnt;

@@ -15,7 +15,7 @@ RDK_OK, RDK_ERR_HIP, RDK_ERR_ARG, RDK_ERR_CAPACITY, RDK_ERR_STATE, RDK_ERR_NOCON
 RDK_NSC_FREE, RDK_NSC_POS, RDK_VV_HALF1, RDK_VV_HALF2, RDK_CLEAR = range(5)
 MF_FIELDS = 42  # RDK_MF_FIELDS
 MLP_IN, MLP_OUT, MLP_H = 12, 3, 64  # RDK_MLP_IN, RDK_MLP_OUT, hidden width
-TUNE_SMALL_M, TUNE_BROAD_BLOCK, TUNE_BROAD_STAGE = 0, 1, 2  # RDK_TUNE_SMALL_M, _BROAD_BLOCK, _BROAD_STAGE
+TUNE_SMALL_M, TUNE_BROAD_BLOCK, TUNE_BROAD_STAGE, TUNE_CELL_BAND = 0, 1, 2, 3  # RDK_TUNE_SMALL_M, _BROAD_BLOCK, _BROAD_STAGE
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)

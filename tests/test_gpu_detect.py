@@ -67,6 +67,68 @@ def test_detect_bit_identical_for_every_broad_phase_workgroup_size(gpu_ctx_facto
         compare(ctx.contacts(nc), W.contacts(no), nc)
 
 
+def staged_scene(scene):
+    """Rods and systems of the broad-phase layout tests: (n, X, Q, R, H, loader)."""
+    if scene == "multi":
+        rng = np.random.default_rng(21)
+        parts, descs, offs = [], [], [0]
+        for k, (n, side, per, hl) in enumerate([(700, 18.0, (1, 1, 1), 2.0), (1500, 25.0, (1, 1, 0), 4.0),
+                                                (300, 9.0, (1, 1, 1), 1.0), (900, 30.0, (0, 0, 0), 8.0)]):
+            x, q, r, h = rods(n, side, 200 + k, hlo=0.5, hhi=hl)
+            x += 40.0 * k
+            parts.append((x, q, r, h, side, per, 40.0 * k))
+            descs.append(L.system_desc([40.0 * k] * 3, [40.0 * k + side] * 3, per, 0, 0, 0.5))
+            offs.append(offs[-1] + n)
+        X, Q, R, H = (np.concatenate([p_[i] for p_ in parts]) for i in range(4))
+        n = len(R)
+
+        def put(ctx):
+            ctx.set_bodies(X, Q, np.zeros((n, 3)), np.zeros((n, 3)), R, H, np.ones(n), np.zeros(n))
+            ctx.set_systems(offs, descs)
+            ctx.set_physics(L.physics())
+        return n, X, Q, R, H, None, None, put
+    n, side, per = {"periodic": (3000, 30.0, (1, 1, 1)), "open": (3000, 30.0, (0, 0, 0)),
+                    "dense": (6000, 16.0, (1, 1, 0))}[scene]
+    X, Q, R, H = rods(n, side, 31)
+    return n, X, Q, R, H, side, per, lambda ctx: load(ctx, X, Q, R, H, np.ones(n), [0] * 3, [side] * 3, per)
+
+
+@pytest.mark.parametrize("band", [1, 2, 5])
+@pytest.mark.parametrize("stage", [0, 1])
+@pytest.mark.parametrize("scene", ["periodic", "open", "dense", "multi"])
+def test_banded_cell_order_bit_identical(gpu_ctx_factory, band, stage, scene):
+    """The cell order in bands of y-rows (RDK_TUNE_CELL_BAND) changes which
+    lane owns a chunk pair and the order of the sorted chunks, not which pairs
+    are tested: contact list and candidate counts equal to the unbanded
+    order's, and the contact list equal to the oracle's, for the plain and
+    the LDS-staged walk (bands of 1, 2 and 5 rows, so the last band of a grid
+    is short and periodic rows wrap across bands)."""
+    print(f"banded cell order: {scene}, band {band}, staged {stage}", flush=True)
+    n, X, Q, R, H, side, per, put = staged_scene(scene)
+    res = {}
+    for b in (0, band):
+        ctx = gpu_ctx_factory()
+        put(ctx)
+        ctx.set_tuning(L.TUNE_BROAD_BLOCK, 256)
+        ctx.set_tuning(L.TUNE_BROAD_STAGE, stage)
+        ctx.set_tuning(L.TUNE_CELL_BAND, b)
+        for chunk_len in (0.0, 1.0):
+            nc = ctx.detect(L.detect_params(0.02, chunk_len, 1.0))
+            res[b, chunk_len] = (ctx.contacts(nc), ctx.detect_info())
+    for chunk_len in (0.0, 1.0):
+        (c0, i0), (c1, i1) = res[0, chunk_len], res[band, chunk_len]
+        assert len(c0["a"]) > 100
+        compare(c1, c0, len(c0["a"]))
+        assert i1 == i0, (i0, i1)
+    if scene != "multi":
+        W = O.World(X, Q, None, None, R, H, np.ones(n))
+        W.set_box([0] * 3, [side] * 3, per)
+        no = W.detect(0, 0.02)
+        c1 = res[band, 1.0][0]
+        assert len(c1["a"]) == no
+        compare(c1, W.contacts(no), no)
+
+
 @pytest.mark.parametrize("block", [64, 256, 512])
 @pytest.mark.parametrize("scene", ["periodic", "open", "dense", "multi"])
 def test_lds_staged_broad_phase_bit_identical(gpu_ctx_factory, block, scene):
