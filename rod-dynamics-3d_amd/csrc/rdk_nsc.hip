@@ -2485,6 +2485,18 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
     for (hipStream_t sx : {s_side, s_small, s_med})
       if (sx != ctx->stream) RDK_HIP(hipStreamWaitEvent(sx, ctx->ev_fork, 0));
   }
+  // The LDS class first: its workgroup needs all of one CU's LDS, so it is
+  // dispatched before the register classes spread over every CU (launched
+  // after them it could wait for a CU to drain).  The classes hold disjoint
+  // islands, so the order does not change any result.
+  if (cnt[3] > 0) {
+    RDK_TRY(ensure(ctx, ctx->islot, nm));
+    RDK_TRY(ensure(ctx, ctx->ihot, nm));
+    ProfScope ps(ctx, K_ISL_LARGE);
+    hipLaunchKernelGGL(k_island_lds, dim3(cnt[3]), dim3(LDS_THREADS), 0, ctx->stream, B, Mc, ctx->sys_dev.p,
+                       ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p + 3 * nisl, Count::host(cnt[3]), P, S,
+                       ctx->islot.p, ctx->ihot.p, (const int*)nullptr);
+  }
   if (cnt[0] > 0) {
     ProfScope ps(ctx, K_ISL_TINY, s_side);
     constexpr int ipb = ISL_WG0 / ISL_G0;
@@ -2503,14 +2515,6 @@ static int island_solve(rdk_ctx* ctx, const rdk_nsc_params* p, rdk_solve_stats* 
     hipLaunchKernelGGL((k_island_reg<ISL_G2, ISL_WG2>), dim3(cnt[2]), dim3(ISL_WG2), 0, s_med, B, Mc,
                        ctx->sys_dev.p, ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p + 2 * nisl,
                        Count::host(cnt[2]), P, S, (const int*)nullptr);
-  }
-  if (cnt[3] > 0) {
-    RDK_TRY(ensure(ctx, ctx->islot, nm));
-    RDK_TRY(ensure(ctx, ctx->ihot, nm));
-    ProfScope ps(ctx, K_ISL_LARGE);
-    hipLaunchKernelGGL(k_island_lds, dim3(cnt[3]), dim3(LDS_THREADS), 0, ctx->stream, B, Mc, ctx->sys_dev.p,
-                       ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p + 3 * nisl, Count::host(cnt[3]), P, S,
-                       ctx->islot.p, ctx->ihot.p, (const int*)nullptr);
   }
   HugeOut hu;
   if (cnt[4] > 0) RDK_TRY(huge_islands_solve(ctx, P, skey, ctx->ilists.p + 4 * nisl, cnt[4], s_side, hu));
@@ -3286,6 +3290,11 @@ int nsc_fixed_enqueue(rdk_ctx* ctx, const rdk_nsc_params* p) {
     for (int c = 0; c < 3; ++c)
       if (cs[c] != st) RDK_HIP(hipStreamWaitEvent(cs[c], ctx->ev_fork, 0));
   }
+  // the LDS class first, as in island_solve: it needs a whole CU's LDS
+  if (g[3] > 0)
+    hipLaunchKernelGGL(k_island_lds, dim3(g[3]), dim3(LDS_THREADS), 0, st, B, Mc, ctx->sys_dev.p, ctx->istart.p,
+                       skey, ctx->bseg.p, sbody, ctx->ilists.p + 3 * (size_t)mcap, Count::dev(cnt + 3, mcap), P, S,
+                       ctx->islot.p, ctx->ihot.p, guard);
   if (g[0] > 0)
     hipLaunchKernelGGL((k_island_reg<ISL_G0, ISL_WG0>), dim3(g[0]), dim3(ISL_WG0), 0, cs[0], B, Mc, ctx->sys_dev.p,
                        ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p, Count::dev(cnt + 0, mcap), P, S,
@@ -3298,10 +3307,6 @@ int nsc_fixed_enqueue(rdk_ctx* ctx, const rdk_nsc_params* p) {
     hipLaunchKernelGGL((k_island_reg<ISL_G2, ISL_WG2>), dim3(g[2]), dim3(ISL_WG2), 0, cs[2], B, Mc, ctx->sys_dev.p,
                        ctx->istart.p, skey, ctx->bseg.p, sbody, ctx->ilists.p + 2 * (size_t)mcap,
                        Count::dev(cnt + 2, mcap), P, S, guard);
-  if (g[3] > 0)
-    hipLaunchKernelGGL(k_island_lds, dim3(g[3]), dim3(LDS_THREADS), 0, st, B, Mc, ctx->sys_dev.p, ctx->istart.p,
-                       skey, ctx->bseg.p, sbody, ctx->ilists.p + 3 * (size_t)mcap, Count::dev(cnt + 3, mcap), P, S,
-                       ctx->islot.p, ctx->ihot.p, guard);
   for (int c = 0; c < 3; ++c) {
     if (cs[c] == st) continue;
     RDK_HIP(hipEventRecord(joins[c], cs[c]));
