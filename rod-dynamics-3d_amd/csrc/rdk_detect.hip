@@ -593,6 +593,7 @@ __global__ void __launch_bounds__(NT, 4) k_broad_narrow(BodyPtrs B, const SysDev
     if (tid == 0) s_ng = total < BN_GROUPS ? total : BN_GROUPS;
   }
   const int p0 = p - lane;  // the wavefront's lanes hold chunks p0 .. p0 + 63
+  const float me_ext = me.w + own.extra;  // the lane's sphere radius plus margin (sphere threshold less q's radius)
   int pending = 0, pend2 = 0;  // wave-uniform queue lengths
   int kl = 0;                  // next entry of the lane's range list
   constexpr int NPH = STAGED ? 3 : 1;
@@ -719,12 +720,14 @@ __global__ void __launch_bounds__(NT, 4) k_broad_narrow(BodyPtrs B, const SysDev
       bool hit[BN_UNROLL];
 #pragma unroll
       for (int j = 0; j < BN_UNROLL; ++j) {
+        // fused image shift and squared distance: one rounding each instead
+        // of two, so the test stays inside the margin it was sized for
         float dx = o[j].x - me.x, dy = o[j].y - me.y, dz = o[j].z - me.z;
-        dx -= own.span[0] * rintf(dx * own.ispan[0]);
-        dy -= own.span[1] * rintf(dy * own.ispan[1]);
-        dz -= own.span[2] * rintf(dz * own.ispan[2]);
-        const float thr = me.w + o[j].w + own.extra;
-        hit[j] = j < nh && dx * dx + dy * dy + dz * dz < thr * thr && rb[j] != a;
+        dx = fmaf(-own.span[0], rintf(dx * own.ispan[0]), dx);
+        dy = fmaf(-own.span[1], rintf(dy * own.ispan[1]), dy);
+        dz = fmaf(-own.span[2], rintf(dz * own.ispan[2]), dz);
+        const float thr = me_ext + o[j].w;
+        hit[j] = j < nh && fmaf(dz, dz, fmaf(dy, dy, dx * dx)) < thr * thr && rb[j] != a;
       }
       if (a_static) {  // a fixed rod pairs only with movable ones (rare: the load only then)
 #pragma unroll
